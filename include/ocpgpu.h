@@ -1,0 +1,205 @@
+/*
+ * ocpgpu.h — C-ABI of the MI355X (gfx950) EoS flash + FIM assembly library.
+ *
+ * The library replaces the per-cell compositional flash and the fully implicit
+ * residual/Jacobian assembly of OpenCAEPoro 0.5.0.  Every entry point below
+ * names the reference interface it stands in for (paths are relative to the
+ * OpenCAEPoro source tree).  Plain pointers and sizes only: no C++ or torch
+ * types cross this boundary.  Every call returns OCPG_OK (0) or a negative
+ * OCPG_E* code; no call aborts the process.
+ *
+ * Units and meanings follow the reference: P in psia, T in degrees F at the
+ * boundary (the flash adds 459.67 internally, src/MixtureComp.cpp:513), moles
+ * in lbmol, volumes in ft3.  Hydrocarbon components are 0..NC-1, water is
+ * component NC (always appended, src/MixtureComp.cpp:34-35); phases are
+ * oil=0, gas=1, water=2 (include/OCPConst.hpp:91-93).  nc = NC+1, np = 3,
+ * ncol = nc+1 (block dimension of the Jacobian).
+ */
+#ifndef OCPGPU_H
+#define OCPGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OCPG_MAXNC 16   /* largest hydrocarbon component count accepted      */
+#define OCPG_NP 3       /* oil, gas, water                                   */
+#define OCPG_MAXTAB 64  /* rows per saturation / PVTW table                  */
+
+/* Status codes. */
+#define OCPG_OK 0
+#define OCPG_EINVAL (-1)     /* bad argument (null pointer, size, shape)       */
+#define OCPG_EUNSUP (-2)     /* NC without a compiled kernel instantiation     */
+#define OCPG_EHIP (-3)       /* HIP runtime error (allocation, launch, copy)   */
+#define OCPG_ESTATE (-4)     /* call order: e.g. assemble before set_grid      */
+#define OCPG_ENODEV (-5)     /* no gfx950 device visible                       */
+#define OCPG_ETOOBIG (-6)    /* request exceeds a documented maximum           */
+
+/* Equation-of-state description, as MixtureComp holds it after its
+ * constructor (src/MixtureComp.cpp:28-165): Vc already derived from Zc,
+ * Vshift already scaled by R*OmegaB*Tc/Pc, LBCcoef already multiplied by 10,
+ * BIC already symmetrised.  ocpg_eos_from_deck() performs those steps from
+ * deck-level values.                                                         */
+typedef struct ocpg_eos {
+    int32_t nc;                         /* NC, hydrocarbon components           */
+    int32_t pvtw_rows;                  /* rows used in pvtw                    */
+    double Tc[OCPG_MAXNC];              /* degR                                 */
+    double Pc[OCPG_MAXNC];              /* psia                                 */
+    double Vc[OCPG_MAXNC];              /* ft3/lbmol                            */
+    double MW[OCPG_MAXNC];
+    double Acf[OCPG_MAXNC];
+    double OmegaA[OCPG_MAXNC];
+    double OmegaB[OCPG_MAXNC];
+    double Vshift[OCPG_MAXNC];          /* ft3/lbmol                            */
+    double Vcvis[OCPG_MAXNC];
+    double BIC[OCPG_MAXNC * OCPG_MAXNC];/* row-major NC x NC, symmetric         */
+    double LBCcoef[5];
+    /* EoS solver controls: SSMSTA / NRSTA / SSMSP / NRSP / RR keywords
+     * (include/MixtureComp.hpp:28-104, src/MixtureComp.cpp:140-163).          */
+    int32_t ssmsta_maxit; double ssmsta_tol; double ssmsta_eyt;
+    int32_t nrsta_maxit;  double nrsta_tol;
+    int32_t ssmsp_maxit;  double ssmsp_tol;
+    int32_t nrsp_maxit;   double nrsp_tol;
+    int32_t rr_maxit;     double rr_tol;
+    /* PVTW table rows: Pref, Bw, Cw, muw, Cvw (src/MixtureComp.cpp:521-546). */
+    double pvtw[OCPG_MAXTAB][5];
+    double std_rhow;                    /* lb/ft3 at surface                    */
+} ocpg_eos;
+
+/* Deck-level component data, as ParamReservoir's ComponentParam holds it
+ * (include/ParamReservoir.hpp:86-193, keywords TCRIT/PCRIT/VCRIT/ZCRIT/MW/
+ * ACF/OMEGAA/OMEGAB/SSHIFT/VCRITVIS/ZCRITVIS/BIC/LBCCOEF).  has_* flags say
+ * which optional keywords were given.                                        */
+typedef struct ocpg_eos_deck {
+    int32_t nc;
+    int32_t pvtw_rows;
+    double Tc[OCPG_MAXNC];
+    double Pc[OCPG_MAXNC];
+    double MW[OCPG_MAXNC];
+    double Acf[OCPG_MAXNC];
+    int32_t has_Vc;     double Vc[OCPG_MAXNC];
+    int32_t has_Zc;     double Zc[OCPG_MAXNC];
+    int32_t has_OmegaA; double OmegaA[OCPG_MAXNC];
+    int32_t has_OmegaB; double OmegaB[OCPG_MAXNC];
+    int32_t has_Vshift; double Vshift[OCPG_MAXNC]; /* dimensionless SSHIFT    */
+    int32_t has_Vcvis;  double Vcvis[OCPG_MAXNC];
+    int32_t has_Zcvis;  double Zcvis[OCPG_MAXNC];
+    double BIC_lower[OCPG_MAXNC * (OCPG_MAXNC - 1) / 2]; /* strict lower, row by row */
+    double LBCcoef[5];  /* deck values (ParamReservoir defaults 0.1023, 0.023364,
+                           0.058533, -0.040758, 0.0093324)                    */
+    int32_t ssmsta_maxit; double ssmsta_tol; double ssmsta_eyt;
+    int32_t nrsta_maxit;  double nrsta_tol;
+    int32_t ssmsp_maxit;  double ssmsp_tol;
+    int32_t nrsp_maxit;   double nrsp_tol;
+    int32_t rr_maxit;     double rr_tol;
+    double pvtw[OCPG_MAXTAB][5];
+    double std_rhow;
+} ocpg_eos_deck;
+
+/* FlowUnit_ODGW01 tables (src/FlowUnit.cpp:178-299): SWOF rows
+ * (Sw, krw, krow, Pcow) and SGOF rows (Sg, krg, krog, Pcgo); Stone II.        */
+typedef struct ocpg_satfunc {
+    int32_t swof_rows;
+    int32_t sgof_rows;
+    double swof[OCPG_MAXTAB][4];
+    double sgof[OCPG_MAXTAB][4];
+} ocpg_satfunc;
+
+/* Rock_Linear (src/Rock.cpp:19-27): poro = poroInit*(1+(cp1+cp2/2*dP)*dP). */
+typedef struct ocpg_rock {
+    double Pref;
+    double cp1;
+    double cp2;
+} ocpg_rock;
+
+/* Cell ("bulk") state in the reference's host layout (include/Bulk.hpp:98-102,
+ * src/OCPFluidMethod.cpp:980-1084): cell-major, e.g. Ni[n*nc+i],
+ * S[n*np+j], xij[(n*np+j)*nc+i], dSec_dPri[n*maxLendSdP + ...] with
+ * maxLendSdP = (nc+1)^2*np.  Any pointer may be NULL where a call does not
+ * read or write that field.                                                  */
+typedef struct ocpg_bulk {
+    int64_t nb;
+    double* P;            /* nb                                                 */
+    double* T;            /* nb, degF                                           */
+    double* Ni;           /* nb*nc                                              */
+    double* lNi;          /* nb*nc, moles at the last time step                 */
+    double* S;            /* nb*np                                              */
+    int32_t* phaseNum;    /* nb: phases present incl. water                     */
+    double* xij;          /* nb*np*nc                                           */
+    uint8_t* phaseExist;  /* nb*np                                              */
+    double* Nt;           /* nb                                                 */
+    double* vf;           /* nb                                                 */
+    double* vfP;          /* nb                                                 */
+    double* vfi;          /* nb*nc                                              */
+    double* rho;          /* nb*np                                              */
+    double* xi;           /* nb*np                                              */
+    double* mu;           /* nb*np                                              */
+    double* rhoP;         /* nb*np                                              */
+    double* xiP;          /* nb*np                                              */
+    double* muP;          /* nb*np                                              */
+    double* rhox;         /* nb*np*nc                                           */
+    double* xix;          /* nb*np*nc                                           */
+    double* mux;          /* nb*np*nc                                           */
+    uint8_t* pSderExist;  /* nb*np                                              */
+    int32_t* pVnumCom;    /* nb*np                                              */
+    int32_t* bRowSizedSdP;/* nb                                                 */
+    double* dSec_dPri;    /* nb*maxLendSdP                                      */
+    /* SkipStaAnaly state (include/AcceleratePVT.hpp:71-85)                    */
+    uint8_t* skipFlag;    /* nb                                                 */
+    double* skipMinEig;   /* nb (an FP32 eigenvalue held in FP64)               */
+    double* skipP;        /* nb                                                 */
+    double* skipT;        /* nb, degR                                           */
+    double* skipZi;       /* nb*NC                                              */
+    /* saturation functions (IsoT_FIM::CalKrPc)                                */
+    double* kr;           /* nb*np                                              */
+    double* Pc;           /* nb*np                                              */
+    double* Pj;           /* nb*np                                              */
+    double* dKr_dS;       /* nb*np*np                                           */
+    double* dPcj_dS;      /* nb*np*np                                           */
+    /* rock and geometry                                                       */
+    double* v;            /* nb, bulk volume                                    */
+    double* depth;        /* nb                                                 */
+    double* poroInit;     /* nb                                                 */
+    double* poro;         /* nb                                                 */
+    double* poroP;        /* nb                                                 */
+    double* rockVp;       /* nb                                                 */
+    /* diagnostics (no reference counterpart; NULL to skip)                    */
+    int32_t* ftype;       /* nb: 0 full test, 1 skipped, 2 straight to split    */
+    int32_t* iters;       /* nb*5: SSMSTA, NRSTA, SSMSP, NRSP, RR iterations    */
+} ocpg_bulk;
+
+/* Connection list in BulkConn order (src/BulkConn.cpp:31-117): bId < eId,
+ * sorted by bId then eId; area is the harmonic TPFA transmissibility
+ * 1/(1/T1+1/T2) before the CONV1*CONV2 factor.                              */
+typedef struct ocpg_conn {
+    int64_t nconn;
+    const int32_t* bId;
+    const int32_t* eId;
+    const double* area;
+} ocpg_conn;
+
+/* Residual norms of IsoT_FIM::CalRes (src/OCPFluidMethod.cpp:1362-1385). */
+typedef struct ocpg_resnorm {
+    double maxRelRes_V;
+    double maxRelRes_N;
+    int64_t maxId_V;
+    int64_t maxId_N;
+} ocpg_resnorm;
+
+/* Totals of MixtureComp's iteration counters (include/MixtureComp.hpp:133-146)
+ * over the cells of one flash call.                                          */
+typedef struct ocpg_flash_stats {
+    int64_t cells;
+    int64_t ftype_count[3];
+    int64_t two_phase;
+    int64_t iters[5];     /* SSMSTA, NRSTA, SSMSP, NRSP, RR                     */
+    int64_t nonconverged; /* cells whose ePEC would read 1e8                    */
+} ocpg_flash_stats;
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OCPGPU_H */

@@ -1,0 +1,182 @@
+"""ctypes mirror of include/ocpgpu.h.
+
+The structures here must match the C header field for field; the library
+exports ocpg_sizeof() so a test can check the layout on both sides.
+"""
+import ctypes as C
+
+import numpy as np
+
+MAXNC = 16
+NP = 3
+MAXTAB = 64
+
+OK = 0
+EINVAL = -1
+EUNSUP = -2
+EHIP = -3
+ESTATE = -4
+ENODEV = -5
+ETOOBIG = -6
+
+_dblN = C.c_double * MAXNC
+
+
+class Eos(C.Structure):
+    _fields_ = [
+        ("nc", C.c_int32),
+        ("pvtw_rows", C.c_int32),
+        ("Tc", _dblN),
+        ("Pc", _dblN),
+        ("Vc", _dblN),
+        ("MW", _dblN),
+        ("Acf", _dblN),
+        ("OmegaA", _dblN),
+        ("OmegaB", _dblN),
+        ("Vshift", _dblN),
+        ("Vcvis", _dblN),
+        ("BIC", C.c_double * (MAXNC * MAXNC)),
+        ("LBCcoef", C.c_double * 5),
+        ("ssmsta_maxit", C.c_int32), ("ssmsta_tol", C.c_double), ("ssmsta_eyt", C.c_double),
+        ("nrsta_maxit", C.c_int32), ("nrsta_tol", C.c_double),
+        ("ssmsp_maxit", C.c_int32), ("ssmsp_tol", C.c_double),
+        ("nrsp_maxit", C.c_int32), ("nrsp_tol", C.c_double),
+        ("rr_maxit", C.c_int32), ("rr_tol", C.c_double),
+        ("pvtw", (C.c_double * 5) * MAXTAB),
+        ("std_rhow", C.c_double),
+    ]
+
+
+class EosDeck(C.Structure):
+    _fields_ = [
+        ("nc", C.c_int32),
+        ("pvtw_rows", C.c_int32),
+        ("Tc", _dblN),
+        ("Pc", _dblN),
+        ("MW", _dblN),
+        ("Acf", _dblN),
+        ("has_Vc", C.c_int32), ("Vc", _dblN),
+        ("has_Zc", C.c_int32), ("Zc", _dblN),
+        ("has_OmegaA", C.c_int32), ("OmegaA", _dblN),
+        ("has_OmegaB", C.c_int32), ("OmegaB", _dblN),
+        ("has_Vshift", C.c_int32), ("Vshift", _dblN),
+        ("has_Vcvis", C.c_int32), ("Vcvis", _dblN),
+        ("has_Zcvis", C.c_int32), ("Zcvis", _dblN),
+        ("BIC_lower", C.c_double * (MAXNC * (MAXNC - 1) // 2)),
+        ("LBCcoef", C.c_double * 5),
+        ("ssmsta_maxit", C.c_int32), ("ssmsta_tol", C.c_double), ("ssmsta_eyt", C.c_double),
+        ("nrsta_maxit", C.c_int32), ("nrsta_tol", C.c_double),
+        ("ssmsp_maxit", C.c_int32), ("ssmsp_tol", C.c_double),
+        ("nrsp_maxit", C.c_int32), ("nrsp_tol", C.c_double),
+        ("rr_maxit", C.c_int32), ("rr_tol", C.c_double),
+        ("pvtw", (C.c_double * 5) * MAXTAB),
+        ("std_rhow", C.c_double),
+    ]
+
+
+class SatFunc(C.Structure):
+    _fields_ = [
+        ("swof_rows", C.c_int32),
+        ("sgof_rows", C.c_int32),
+        ("swof", (C.c_double * 4) * MAXTAB),
+        ("sgof", (C.c_double * 4) * MAXTAB),
+    ]
+
+
+class Rock(C.Structure):
+    _fields_ = [("Pref", C.c_double), ("cp1", C.c_double), ("cp2", C.c_double)]
+
+
+_P = C.c_void_p
+BULK_FIELDS = [
+    ("P", "f8", "nb"), ("T", "f8", "nb"), ("Ni", "f8", "nb*nc"), ("lNi", "f8", "nb*nc"),
+    ("S", "f8", "nb*np"), ("phaseNum", "i4", "nb"), ("xij", "f8", "nb*np*nc"),
+    ("phaseExist", "u1", "nb*np"), ("Nt", "f8", "nb"), ("vf", "f8", "nb"), ("vfP", "f8", "nb"),
+    ("vfi", "f8", "nb*nc"), ("rho", "f8", "nb*np"), ("xi", "f8", "nb*np"), ("mu", "f8", "nb*np"),
+    ("rhoP", "f8", "nb*np"), ("xiP", "f8", "nb*np"), ("muP", "f8", "nb*np"),
+    ("rhox", "f8", "nb*np*nc"), ("xix", "f8", "nb*np*nc"), ("mux", "f8", "nb*np*nc"),
+    ("pSderExist", "u1", "nb*np"), ("pVnumCom", "i4", "nb*np"), ("bRowSizedSdP", "i4", "nb"),
+    ("dSec_dPri", "f8", "nb*lend"),
+    ("skipFlag", "u1", "nb"), ("skipMinEig", "f8", "nb"), ("skipP", "f8", "nb"),
+    ("skipT", "f8", "nb"), ("skipZi", "f8", "nb*NC"),
+    ("kr", "f8", "nb*np"), ("Pc", "f8", "nb*np"), ("Pj", "f8", "nb*np"),
+    ("dKr_dS", "f8", "nb*np*np"), ("dPcj_dS", "f8", "nb*np*np"),
+    ("v", "f8", "nb"), ("depth", "f8", "nb"), ("poroInit", "f8", "nb"), ("poro", "f8", "nb"),
+    ("poroP", "f8", "nb"), ("rockVp", "f8", "nb"),
+    ("ftype", "i4", "nb"), ("iters", "i4", "nb*5"),
+]
+
+
+class BulkC(C.Structure):
+    _fields_ = [("nb", C.c_int64)] + [(f, _P) for f, _, _ in BULK_FIELDS]
+
+
+class Conn(C.Structure):
+    _fields_ = [("nconn", C.c_int64), ("bId", _P), ("eId", _P), ("area", _P)]
+
+
+class ResNorm(C.Structure):
+    _fields_ = [("maxRelRes_V", C.c_double), ("maxRelRes_N", C.c_double),
+                ("maxId_V", C.c_int64), ("maxId_N", C.c_int64)]
+
+
+class FlashStats(C.Structure):
+    _fields_ = [("cells", C.c_int64), ("ftype_count", C.c_int64 * 3), ("two_phase", C.c_int64),
+                ("iters", C.c_int64 * 5), ("nonconverged", C.c_int64)]
+
+
+def max_len_dsdp(nc):
+    """maxLendSdP of src/OCPFluidMethod.cpp:1056: (nc+1)^2 * np."""
+    return (nc + 1) * (nc + 1) * NP
+
+
+class Bulk:
+    """Host arrays in the reference's Bulk layout (cell-major), numpy-owned.
+
+    ``Bulk.cstruct()`` returns the ocpg_bulk view handed to C.  Fields absent
+    from ``fields`` stay NULL.
+    """
+
+    def __init__(self, nb, NC, fields=None):
+        self.nb = int(nb)
+        self.NC = int(NC)
+        self.nc = self.NC + 1
+        dims = {"nb": self.nb, "nc": self.nc, "np": NP, "NC": self.NC,
+                "lend": max_len_dsdp(self.nc)}
+        self.arrays = {}
+        for name, dt, shape in BULK_FIELDS:
+            if fields is not None and name not in fields:
+                continue
+            n = 1
+            for tok in shape.split("*"):
+                n *= dims[tok] if tok in dims else int(tok)
+            self.arrays[name] = np.zeros(n, dtype=dt)
+
+    def __getattr__(self, name):
+        arrays = self.__dict__.get("arrays")
+        if arrays is not None and name in arrays:
+            return arrays[name]
+        raise AttributeError(name)
+
+    def copy(self):
+        b = Bulk.__new__(Bulk)
+        b.nb, b.NC, b.nc = self.nb, self.NC, self.nc
+        b.arrays = {k: v.copy() for k, v in self.arrays.items()}
+        return b
+
+    def cstruct(self):
+        s = BulkC()
+        s.nb = self.nb
+        for name, _, _ in BULK_FIELDS:
+            a = self.arrays.get(name)
+            setattr(s, name, a.ctypes.data if a is not None else None)
+        return s
+
+
+def conn_struct(bId, eId, area):
+    s = Conn()
+    s.nconn = len(bId)
+    s.bId = bId.ctypes.data
+    s.eId = eId.ctypes.data
+    s.area = area.ctypes.data
+    return s

@@ -1,0 +1,105 @@
+"""Seeded synthetic inputs for the flash + assembly workloads.
+
+* ``cell_inputs``: per-cell P, z, Sw as the standalone-kernel configuration
+  draws them (P ~ U[1000, 5000] psia, z_i proportional to Exp(1) * bias_i,
+  Sw ~ U[0.1, 0.4], T = 160 F).
+* ``cartesian_grid``: an orthogonal nx x ny x nz grid with the reference's
+  7-point neighbour rule and TPFA transmissibility
+  (src/Grid.cpp:114-198, src/BulkConn.cpp:31-117), in BulkConn order.
+"""
+import numpy as np
+
+T_F = 160.0
+BIAS6 = (10.0, 1.0, 1.0, 1.0, 1.0, 1.0)
+BIAS10 = (10.0, 2.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0)
+
+
+def cell_inputs(nb, NC, seed, bias=None, p_range=(1000.0, 5000.0), sw_range=(0.1, 0.4)):
+    rng = np.random.default_rng(seed)
+    if bias is None:
+        bias = BIAS6 if NC == 6 else BIAS10
+    P = rng.uniform(p_range[0], p_range[1], nb)
+    z = rng.exponential(1.0, (nb, NC)) * np.asarray(bias, dtype=np.float64)[None, :]
+    z /= z.sum(axis=1, keepdims=True)
+    Sw = rng.uniform(sw_range[0], sw_range[1], nb)
+    return P, z, Sw
+
+
+def moles_from_z(z, seed, nh_range=(50.0, 500.0), nw_range=(10.0, 300.0)):
+    """Ni in the reference layout (nb, NC+1) for flash-only tests."""
+    rng = np.random.default_rng(seed)
+    nb, NC = z.shape
+    Nh = rng.uniform(nh_range[0], nh_range[1], nb)
+    Ni = np.empty((nb, NC + 1))
+    Ni[:, :NC] = z * Nh[:, None]
+    Ni[:, NC] = rng.uniform(nw_range[0], nw_range[1], nb)
+    return Ni
+
+
+class Grid:
+    """Cell geometry and connection list of an orthogonal grid."""
+
+    def __init__(self, nx, ny, nz, dx, dy, dz, tops, perm, ntg=None):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        nb = nx * ny * nz
+        self.nb = nb
+        dx = np.broadcast_to(np.asarray(dx, dtype=np.float64), (nb,)).copy()
+        dy = np.broadcast_to(np.asarray(dy, dtype=np.float64), (nb,)).copy()
+        dz = np.broadcast_to(np.asarray(dz, dtype=np.float64), (nb,)).copy()
+        kx, ky, kz = perm
+        ntg = np.ones(nb) if ntg is None else np.asarray(ntg, dtype=np.float64)
+        nxny = nx * ny
+        # Grid::CalDepthVOrthogonalGrid
+        depth = np.empty(nb)
+        tops = np.broadcast_to(np.asarray(tops, dtype=np.float64), (nxny,))
+        depth[:nxny] = tops + dz[:nxny] / 2
+        for k in range(1, nz):
+            s = slice(k * nxny, (k + 1) * nxny)
+            sp = slice((k - 1) * nxny, k * nxny)
+            depth[s] = depth[sp] + dz[sp] / 2 + dz[s] / 2
+        self.depth = depth
+        self.v = dx * dy * dz
+        idx = np.arange(nb, dtype=np.int64)
+        i = idx % nx
+        j = (idx // nx) % ny
+        k = idx // nxny
+        conns = []
+        # direction 1 (x), 2 (y), 3 (z): T = ntg*k*area for x/y, k*area for z
+        for mask, off, aB, aE, tB, tE in (
+            (i < nx - 1, 1, lambda b: 2 * dy[b] * dz[b] / dx[b], None, kx, None),
+            (j < ny - 1, nx, lambda b: 2 * dz[b] * dx[b] / dy[b], None, ky, None),
+            (k < nz - 1, nxny, lambda b: 2 * dx[b] * dy[b] / dz[b], None, kz, None),
+        ):
+            b = idx[mask]
+            e = b + off
+            areaB = aB(b)
+            areaE = aB(e)
+            if off == nxny:
+                T1 = tB[b] * areaB
+                T2 = tB[e] * areaE
+            else:
+                T1 = ntg[b] * tB[b] * areaB
+                T2 = ntg[e] * tB[e] * areaE
+            conns.append((b, e, 1.0 / (1.0 / T1 + 1.0 / T2)))
+        bId = np.concatenate([c[0] for c in conns])
+        eId = np.concatenate([c[1] for c in conns])
+        area = np.concatenate([c[2] for c in conns])
+        order = np.lexsort((eId, bId))
+        self.bId = bId[order].astype(np.int32)
+        self.eId = eId[order].astype(np.int32)
+        self.area = area[order].astype(np.float64)
+        self.nconn = len(self.bId)
+
+    def neighbour_counts(self):
+        cnt = np.ones(self.nb, dtype=np.int64)
+        np.add.at(cnt, self.bId, 1)
+        np.add.at(cnt, self.eId, 1)
+        return cnt
+
+
+def cartesian_grid(nx, ny, nz, seed, dx=100.0, dy=100.0, dz=10.0, tops=8325.0,
+                   lnk_mean=np.log(100.0), lnk_sigma=1.5):
+    rng = np.random.default_rng(seed)
+    nb = nx * ny * nz
+    k = np.exp(rng.normal(lnk_mean, lnk_sigma, nb))
+    return Grid(nx, ny, nz, dx, dy, dz, tops, (k, k, k))

@@ -1,0 +1,83 @@
+// C entry points of the oracle (loaded by the tests and the benchmark's
+// CPU-baseline leg through ctypes).  TEST INFRASTRUCTURE ONLY.
+#include <chrono>
+
+#include "oracle_fim.hpp"
+
+extern "C" {
+
+int orc_flash_fim(const ocpg_eos* eos, ocpg_bulk* bk, int nthreads)
+{
+    if (!eos || !bk || eos->nc < 1 || eos->nc > OCPG_MAXNC) return OCPG_EINVAL;
+    oracle::CalFlash(*eos, *bk, nthreads);
+    return OCPG_OK;
+}
+
+int orc_init_flash_fim(const ocpg_eos* eos, ocpg_bulk* bk, const double* zi, int nthreads)
+{
+    if (!eos || !bk || !zi || eos->nc < 1 || eos->nc > OCPG_MAXNC) return OCPG_EINVAL;
+    oracle::InitFlash(*eos, *bk, zi, nthreads);
+    return OCPG_OK;
+}
+
+int orc_calkrpc(const ocpg_satfunc* sat, ocpg_bulk* bk, int nthreads)
+{
+    if (!sat || !bk) return OCPG_EINVAL;
+    oracle::CalKrPc(*sat, *bk, nthreads);
+    return OCPG_OK;
+}
+
+int orc_calrock(const ocpg_rock* rock, ocpg_bulk* bk, int nthreads)
+{
+    if (!rock || !bk) return OCPG_EINVAL;
+    oracle::CalRock(*rock, *bk, nthreads);
+    return OCPG_OK;
+}
+
+int orc_calres(int nc, const ocpg_conn* conn, ocpg_bulk* bk, double dt, double* resAbs,
+               int32_t* upblock, double* upblock_Rho, double* upblock_Velocity, ocpg_resnorm* norms)
+{
+    if (!conn || !bk || !resAbs || !upblock || !upblock_Rho || !upblock_Velocity) return OCPG_EINVAL;
+    oracle::CalRes(nc, *conn, *bk, dt, resAbs, upblock, upblock_Rho, upblock_Velocity, norms);
+    return OCPG_OK;
+}
+
+int orc_bsr_pattern(int64_t nb, const ocpg_conn* conn, int32_t* IA, int32_t* JA)
+{
+    if (!conn || !IA || !JA) return OCPG_EINVAL;
+    oracle::BsrPattern(nb, *conn, IA, JA);
+    return OCPG_OK;
+}
+
+int orc_assemble_bulks(int nc, const ocpg_conn* conn, ocpg_bulk* bk, double dt,
+                       const int32_t* upblock, const double* upblock_Rho, const int32_t* IA,
+                       const int32_t* JA, double* val)
+{
+    if (!conn || !bk || !upblock || !upblock_Rho || !IA || !JA || !val) return OCPG_EINVAL;
+    oracle::AssembleBulks(nc, *conn, *bk, dt, upblock, upblock_Rho, IA, JA, val);
+    return OCPG_OK;
+}
+
+// Wall time of one CalFlash pass (seconds), for the CPU baseline.
+double orc_time_flash_fim(const ocpg_eos* eos, ocpg_bulk* bk, int nthreads)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    oracle::CalFlash(*eos, *bk, nthreads);
+    auto t1 = std::chrono::steady_clock::now();
+    return std::chrono::duration<double>(t1 - t0).count();
+}
+
+#ifdef OCP_COUNT_FLOPS
+// Flop tally of one serial CalFlash pass: out[0] = add/sub/mul/div/sqrt,
+// out[1] = transcendental calls.
+int orc_count_flash_fim(const ocpg_eos* eos, ocpg_bulk* bk, uint64_t* out)
+{
+    oracle::g_tally = oracle::FlopTally();
+    oracle::CalFlash(*eos, *bk, 1);
+    out[0] = oracle::g_tally.flops;
+    out[1] = oracle::g_tally.trans;
+    return OCPG_OK;
+}
+#endif
+
+}  // extern "C"

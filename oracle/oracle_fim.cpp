@@ -1,0 +1,539 @@
+// CPU restatement of the FIM property pass and assembly of IsoT_FIM.
+// TEST INFRASTRUCTURE ONLY (oracle + CPU baseline); never in the product path.
+//
+// Restates, on the reference's cell-major Bulk layout:
+//   IsoT_FIM::CalFlash + PassFlashValue   src/OCPFluidMethod.cpp:1108-1187
+//   IsoT_FIM::InitFlash                    src/OCPFluidMethod.cpp:1095-1106
+//   IsoT_FIM::CalKrPc -> FlowUnit_ODGW01   src/OCPFluidMethod.cpp:1189-1199,
+//                                          src/FlowUnit.cpp:235-328
+//   IsothermalMethod::CalRock              src/OCPFluidMethod.cpp:25-32, src/Rock.cpp:19-27
+//   IsoT_FIM::CalRes (bulk part)           src/OCPFluidMethod.cpp:1201-1389
+//   IsoT_FIM::AssembleMatBulksNew          src/OCPFluidMethod.cpp:1569-1839
+#include "oracle_fim.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace oracle {
+
+static inline int maxLendSdP(int nc) { return (nc + 1) * (nc + 1) * NPH; }
+
+// IsoT_FIM::PassFlashValue, src/OCPFluidMethod.cpp:1122-1187 (Newton
+// bookkeeping dSNR/maxNRdSSP is not part of this path).
+static void PassFlashValue(const Mixture& mix, const ocpg_bulk& bk, int64_t n)
+{
+    const int np = NPH;
+    const int nc = mix.numCom;
+    const int64_t bIdp = n * np;
+    int len = 0;
+    bk.phaseNum[n] = 0;
+    bk.Nt[n] = todbl(mix.Nt);
+    bk.vf[n] = todbl(mix.vf);
+    for (int j = 0; j < np; j++) {
+        bk.S[bIdp + j] = todbl(mix.S[j]);
+        bk.phaseExist[bIdp + j] = mix.phaseExist[j];
+        if (mix.phaseExist[j]) {
+            bk.phaseNum[n]++;
+            bk.rho[bIdp + j] = todbl(mix.rho[j]);
+            bk.xi[bIdp + j] = todbl(mix.xi[j]);
+            bk.mu[bIdp + j] = todbl(mix.mu[j]);
+            bk.rhoP[bIdp + j] = todbl(mix.rhoP[j]);
+            bk.xiP[bIdp + j] = todbl(mix.xiP[j]);
+            bk.muP[bIdp + j] = todbl(mix.muP[j]);
+            for (int i = 0; i < nc; i++) {
+                bk.xij[bIdp * nc + j * nc + i] = todbl(mix.xij[j * nc + i]);
+                bk.rhox[bIdp * nc + j * nc + i] = todbl(mix.rhox[j * nc + i]);
+                bk.xix[bIdp * nc + j * nc + i] = todbl(mix.xix[j * nc + i]);
+                bk.mux[bIdp * nc + j * nc + i] = todbl(mix.mux[j * nc + i]);
+            }
+        }
+        bk.pSderExist[bIdp + j] = mix.pSderExist[j];
+        bk.pVnumCom[bIdp + j] = mix.pVnumCom[j];
+        if (mix.pSderExist[j]) len++;
+        len += mix.pVnumCom[j];
+    }
+    bk.vfP[n] = todbl(mix.vfP);
+    for (int i = 0; i < nc; i++) bk.vfi[n * nc + i] = todbl(mix.vfi[i]);
+    bk.bRowSizedSdP[n] = len;
+    const int L = maxLendSdP(nc);
+    for (int k = 0; k < len * (nc + 1); k++) bk.dSec_dPri[n * L + k] = todbl(mix.dXsdXp[k]);
+    if (bk.ftype) bk.ftype[n] = mix.ftype;
+    if (bk.iters)
+        for (int q = 0; q < 5; q++) bk.iters[n * 5 + q] = (int32_t)mix.iters[q];
+}
+
+static void LoadSkip(const ocpg_bulk& bk, int NC, int64_t n, SkipState& s)
+{
+    s.flag = bk.skipFlag[n] != 0;
+    s.minEigen = bk.skipMinEig[n];
+    s.P = bk.skipP[n];
+    s.T = bk.skipT[n];
+    for (int i = 0; i < NC; i++) s.zi[i] = bk.skipZi[n * NC + i];
+}
+
+static void StoreSkip(const ocpg_bulk& bk, int NC, int64_t n, const SkipState& s)
+{
+    bk.skipFlag[n] = s.flag ? 1 : 0;
+    bk.skipMinEig[n] = s.minEigen;
+    bk.skipP[n] = s.P;
+    bk.skipT[n] = s.T;
+    for (int i = 0; i < NC; i++) bk.skipZi[n * NC + i] = s.zi[i];
+}
+
+void CalFlash(const ocpg_eos& eos, const ocpg_bulk& bk, int nthreads)
+{
+    const int NC = eos.nc;
+    const int nc = NC + 1;
+    const int np = NPH;
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : 1)
+    {
+        Mixture mix(eos);
+#pragma omp for schedule(dynamic, 256)
+        for (int64_t n = 0; n < bk.nb; n++) {
+            SkipState s;
+            LoadSkip(bk, NC, n, s);
+            mix.FlashFIM(bk.P[n], bk.T[n], &bk.Ni[n * nc], &bk.S[n * np], bk.phaseNum[n],
+                         &bk.xij[n * np * nc], s);
+            StoreSkip(bk, NC, n, s);
+            PassFlashValue(mix, bk, n);
+        }
+    }
+}
+
+// IsoT_FIM::InitFlash: Ni is an output (from z, Sw and the pore volume).
+void InitFlash(const ocpg_eos& eos, const ocpg_bulk& bk, const double* zi, int nthreads)
+{
+    const int NC = eos.nc;
+    const int nc = NC + 1;
+    const int np = NPH;
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : 1)
+    {
+        Mixture mix(eos);
+#pragma omp for schedule(dynamic, 256)
+        for (int64_t n = 0; n < bk.nb; n++) {
+            SkipState s;
+            LoadSkip(bk, NC, n, s);
+            mix.InitFlashFIM(bk.P[n], bk.T[n], &bk.S[n * np], bk.rockVp[n], &zi[n * NC], s);
+            StoreSkip(bk, NC, n, s);
+            for (int i = 0; i < nc; i++) bk.Ni[n * nc + i] = todbl(mix.Ni[i]);
+            PassFlashValue(mix, bk, n);
+        }
+    }
+}
+
+// OCPTable::Eval_All on column 0 of a 4-column table.
+static void TableEval4(const double (*tab)[4], int nRow, double val, double* out, double* slope)
+{
+    int seg = -1;
+    for (int i = 0; i + 1 < nRow; i++) {
+        if (val >= tab[i][0] && val < tab[i + 1][0]) {
+            seg = i;
+            break;
+        }
+    }
+    if (seg < 0) {
+        const int r = (val < tab[0][0]) ? 0 : nRow - 1;
+        for (int k = 0; k < 4; k++) {
+            slope[k] = 0;
+            out[k] = tab[r][k];
+        }
+        return;
+    }
+    for (int k = 0; k < 4; k++) {
+        slope[k] = (tab[seg + 1][k] - tab[seg][k]) / (tab[seg + 1][0] - tab[seg][0]);
+        out[k] = tab[seg][k] + slope[k] * (val - tab[seg][0]);
+    }
+}
+
+// FlowUnit_ODGW01::CalKrPcDeriv + CalKro_Stone2Der, src/FlowUnit.cpp:235-328.
+static void KrPcDeriv(const ocpg_satfunc& sat, const double* S_in, double* kr_out, double* pc_out,
+                      double* dkrdS, double* dPcjdS)
+{
+    double data[4], cdata[4];
+    const double Sg = S_in[1];
+    const double Sw = S_in[2];
+    TableEval4(sat.swof, sat.swof_rows, Sw, data, cdata);
+    const double krw = data[1];
+    const double dKrwdSw = cdata[1];
+    const double krow = data[2];
+    const double dKrowdSw = cdata[2];
+    const double Pcwo = -data[3];
+    const double dPcwdSw = -cdata[3];
+    TableEval4(sat.sgof, sat.sgof_rows, Sg, data, cdata);
+    const double krg = data[1];
+    const double dKrgdSg = cdata[1];
+    const double krog = data[2];
+    const double dKrogdSg = cdata[2];
+    const double Pcgo = data[3];
+    const double dPcgdSg = cdata[3];
+
+    const double kroMax = sat.swof[0][2];
+    double kro = kroMax * ((krow / kroMax + krw) * (krog / kroMax + krg) - (krw + krg));
+    double dKrodSw = kroMax * ((dKrowdSw / kroMax + dKrwdSw) * (krog / kroMax + krg) - (dKrwdSw));
+    double dKrodSg = kroMax * ((krow / kroMax + krw) * (dKrogdSg / kroMax + dKrgdSg) - (dKrgdSg));
+    if (kro < 0) {
+        kro = 0;
+        dKrodSg = 0;
+        dKrodSw = 0;
+    }
+    kr_out[0] = kro;
+    kr_out[1] = krg;
+    kr_out[2] = krw;
+    pc_out[0] = 0;
+    pc_out[1] = Pcgo;
+    pc_out[2] = Pcwo;
+    dkrdS[0] = 0;
+    dkrdS[1] = dKrodSg;
+    dkrdS[2] = dKrodSw;
+    dkrdS[3] = 0;
+    dkrdS[4] = dKrgdSg;
+    dkrdS[5] = 0;
+    dkrdS[6] = 0;
+    dkrdS[7] = 0;
+    dkrdS[8] = dKrwdSw;
+    dPcjdS[0] = 0;
+    dPcjdS[1] = 0;
+    dPcjdS[2] = 0;
+    dPcjdS[3] = 0;
+    dPcjdS[4] = dPcgdSg;
+    dPcjdS[5] = 0;
+    dPcjdS[6] = 0;
+    dPcjdS[7] = 0;
+    dPcjdS[8] = dPcwdSw;
+}
+
+void CalKrPc(const ocpg_satfunc& sat, const ocpg_bulk& bk, int nthreads)
+{
+    const int np = NPH;
+#pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1) schedule(static)
+    for (int64_t n = 0; n < bk.nb; n++) {
+        const int64_t bId = n * np;
+        KrPcDeriv(sat, &bk.S[bId], &bk.kr[bId], &bk.Pc[bId], &bk.dKr_dS[bId * np],
+                  &bk.dPcj_dS[bId * np]);
+        for (int j = 0; j < np; j++) bk.Pj[bId + j] = bk.P[n] + bk.Pc[bId + j];
+    }
+}
+
+void CalRock(const ocpg_rock& rock, const ocpg_bulk& bk, int nthreads)
+{
+#pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1) schedule(static)
+    for (int64_t n = 0; n < bk.nb; n++) {
+        const double dP = (bk.P[n] - rock.Pref);
+        bk.poro[n] = bk.poroInit[n] * (1 + (rock.cp1 + rock.cp2 / 2 * dP) * dP);
+        bk.poroP[n] = bk.poroInit[n] * (rock.cp1 + rock.cp2 * dP);
+        bk.rockVp[n] = bk.v[n] * bk.poro[n];
+    }
+}
+
+// IsoT_FIM::CalRes, bulk-to-bulk part and norms (no wells), serial in
+// connection order as the reference accumulates.
+void CalRes(int nc, const ocpg_conn& conn, const ocpg_bulk& bk, double dt, double* resAbs,
+            int32_t* upblock, double* upblock_Rho, double* upblock_Velocity, ocpg_resnorm* norms)
+{
+    const int np = NPH;
+    const int len = nc + 1;
+    const int64_t nb = bk.nb;
+    std::memset(resAbs, 0, sizeof(double) * nb * len);
+    for (int64_t n = 0; n < nb; n++) {
+        const int64_t bId = n * len;
+        const int64_t bIdb = n * nc;
+        resAbs[bId] = bk.rockVp[n] - bk.vf[n];
+        for (int i = 0; i < nc; i++) resAbs[bId + 1 + i] = bk.Ni[bIdb + i] - bk.lNi[bIdb + i];
+    }
+    for (int64_t c = 0; c < conn.nconn; c++) {
+        const int64_t bId = conn.bId[c];
+        const int64_t eId = conn.eId[c];
+        const double Akd = CONV1 * CONV2 * conn.area[c];
+        for (int j = 0; j < np; j++) {
+            const int64_t bId_np_j = bId * np + j;
+            const int64_t eId_np_j = eId * np + j;
+            const bool exbegin = bk.phaseExist[bId_np_j];
+            const bool exend = bk.phaseExist[eId_np_j];
+            double rho;
+            if (exbegin && exend) {
+                rho = (bk.rho[bId_np_j] + bk.rho[eId_np_j]) / 2;
+            } else if (exbegin && (!exend)) {
+                rho = bk.rho[bId_np_j];
+            } else if ((!exbegin) && exend) {
+                rho = bk.rho[eId_np_j];
+            } else {
+                upblock[c * np + j] = (int32_t)bId;
+                upblock_Rho[c * np + j] = 0;
+                continue;
+            }
+            int64_t uId = bId;
+            const double dP = (bk.Pj[bId_np_j] - GRAVITY_FACTOR * rho * bk.depth[bId]) -
+                              (bk.Pj[eId_np_j] - GRAVITY_FACTOR * rho * bk.depth[eId]);
+            if (dP < 0) uId = eId;
+            upblock_Rho[c * np + j] = rho;
+            upblock[c * np + j] = (int32_t)uId;
+            const int64_t uId_np_j = uId * np + j;
+            if (bk.phaseExist[uId_np_j]) {
+                upblock_Velocity[c * np + j] = Akd * bk.kr[uId_np_j] / bk.mu[uId_np_j] * dP;
+            } else {
+                upblock_Velocity[c * np + j] = 0;
+                continue;
+            }
+            const double tmp = dt * Akd * bk.xi[uId_np_j] * bk.kr[uId_np_j] / bk.mu[uId_np_j] * dP;
+            for (int i = 0; i < nc; i++) {
+                const double dNi = tmp * bk.xij[uId_np_j * nc + i];
+                resAbs[bId * len + 1 + i] += dNi;
+                resAbs[eId * len + 1 + i] -= dNi;
+            }
+        }
+    }
+    ocpg_resnorm nr = {0, 0, 0, 0};
+    for (int64_t n = 0; n < nb; n++) {
+        for (int i = 0; i < len; i++) {
+            const double tmp = std::fabs(resAbs[n * len + i] / bk.rockVp[n]);
+            if (nr.maxRelRes_V < tmp) {
+                nr.maxRelRes_V = tmp;
+                nr.maxId_V = n;
+            }
+        }
+        for (int i = 1; i < len; i++) {
+            const double tmp = std::fabs(resAbs[n * len + i] / bk.Nt[n]);
+            if (nr.maxRelRes_N < tmp) {
+                nr.maxRelRes_N = tmp;
+                nr.maxId_N = n;
+            }
+        }
+    }
+    for (int64_t k = 0; k < nb * len; k++) resAbs[k] = -resAbs[k];
+    if (norms) *norms = nr;
+}
+
+// BSR structure of LinearSystem after AssembleMatBulksNew: row n holds the
+// diagonal first, then neighbours in insertion order (ascending for a sorted
+// connection list).  IA has nb+1 entries.
+void BsrPattern(int64_t nb, const ocpg_conn& conn, int32_t* IA, int32_t* JA)
+{
+    std::vector<std::vector<int32_t>> cols(nb);
+    for (int64_t n = 0; n < nb; n++) cols[n].push_back((int32_t)n);
+    for (int64_t c = 0; c < conn.nconn; c++) {
+        cols[conn.eId[c]].push_back(conn.bId[c]);
+        cols[conn.bId[c]].push_back(conn.eId[c]);
+    }
+    IA[0] = 0;
+    int64_t k = 0;
+    for (int64_t n = 0; n < nb; n++) {
+        for (int32_t col : cols[n]) JA[k++] = col;
+        IA[n + 1] = (int32_t)k;
+    }
+}
+
+// DaABpbC, src/DenseMat.cpp:75-94: C(m x n) += A(m x k) B(k x n), row-major,
+// accumulated over k in order (reference BLAS dgemm loop order).
+static void DaABpbC(int m, int n, int k, const double* A, const double* B, double* C)
+{
+    for (int r = 0; r < m; r++)
+        for (int p = 0; p < k; p++) {
+            const double a = A[r * k + p];
+            for (int c = 0; c < n; c++) C[r * n + c] = C[r * n + c] + a * B[p * n + c];
+        }
+}
+
+// IsoT_FIM::AssembleMatBulksNew, src/OCPFluidMethod.cpp:1569-1839, into a BSR
+// whose pattern BsrPattern() produced.  val has IA[nb]*ncol*ncol entries.
+void AssembleBulks(int nc, const ocpg_conn& conn, const ocpg_bulk& bk, double dt,
+                   const int32_t* upblock, const double* upblock_Rho, const int32_t* IA,
+                   const int32_t* JA, double* val)
+{
+    const int np = NPH;
+    const int ncol = nc + 1;
+    const int ncol2 = np * nc + np;
+    const int bsize = ncol * ncol;
+    const int bsize2 = ncol * ncol2;
+    const int lendSdP = maxLendSdP(nc);
+    const int64_t nb = bk.nb;
+    std::vector<int32_t> fill(nb);
+    for (int64_t n = 0; n < nb; n++) fill[n] = IA[n] + 1;
+    std::memset(val, 0, sizeof(double) * (size_t)IA[nb] * bsize);
+
+    std::vector<double> bmat(bsize, 0);
+    for (int i = 1; i < ncol; i++) bmat[i * ncol + i] = 1;
+    for (int64_t n = 0; n < nb; n++) {
+        bmat[0] = bk.v[n] * bk.poroP[n] - bk.vfP[n];
+        for (int i = 0; i < nc; i++) bmat[i + 1] = -bk.vfi[n * nc + i];
+        std::memcpy(&val[(size_t)IA[n] * bsize], bmat.data(), sizeof(double) * bsize);
+    }
+
+    std::vector<double> dFdXpB(bsize), dFdXpE(bsize), dFdXsB(bsize2), dFdXsE(bsize2);
+    bool phaseExistB[NPH], phaseExistE[NPH], phasedS_B[NPH], phasedS_E[NPH];
+    int pVnumComB[NPH], pVnumComE[NPH];
+    for (int64_t c = 0; c < conn.nconn; c++) {
+        const int64_t bId = conn.bId[c];
+        const int64_t eId = conn.eId[c];
+        const double Akd = CONV1 * CONV2 * conn.area[c];
+        std::fill(dFdXpB.begin(), dFdXpB.end(), 0.0);
+        std::fill(dFdXpE.begin(), dFdXpE.end(), 0.0);
+        std::fill(dFdXsB.begin(), dFdXsB.end(), 0.0);
+        std::fill(dFdXsE.begin(), dFdXsE.end(), 0.0);
+        const double dGamma = GRAVITY_FACTOR * (bk.depth[bId] - bk.depth[eId]);
+        int jxB = 0, jxE = 0, ncolB = 0, ncolE = 0;
+        for (int j = 0; j < np; j++) {
+            phaseExistB[j] = bk.phaseExist[bId * np + j];
+            phaseExistE[j] = bk.phaseExist[eId * np + j];
+            phasedS_B[j] = bk.pSderExist[bId * np + j];
+            phasedS_E[j] = bk.pSderExist[eId * np + j];
+            if (phasedS_B[j]) jxB++;
+            if (phasedS_E[j]) jxE++;
+            pVnumComB[j] = bk.pVnumCom[bId * np + j];
+            pVnumComE[j] = bk.pVnumCom[eId * np + j];
+            ncolB += pVnumComB[j];
+            ncolE += pVnumComE[j];
+        }
+        ncolB += jxB;
+        ncolE += jxE;
+        for (int j = 0; j < np; j++) {
+            const int64_t uId = upblock[c * np + j];
+            const bool phaseExistU = (uId == bId ? phaseExistB[j] : phaseExistE[j]);
+            if (!phaseExistU) {
+                jxB += pVnumComB[j];
+                jxE += pVnumComE[j];
+                continue;
+            }
+            const int64_t bId_np_j = bId * np + j;
+            const int64_t eId_np_j = eId * np + j;
+            const int64_t uId_np_j = uId * np + j;
+            const double dP = bk.Pj[bId_np_j] - bk.Pj[eId_np_j] - upblock_Rho[c * np + j] * dGamma;
+            const double xi = bk.xi[uId_np_j];
+            const double kr = bk.kr[uId_np_j];
+            const double mu = bk.mu[uId_np_j];
+            const double muP = bk.muP[uId_np_j];
+            const double xiP = bk.xiP[uId_np_j];
+            const double rhoP = bk.rhoP[uId_np_j];
+            const double transJ = Akd * kr / mu;
+            for (int i = 0; i < nc; i++) {
+                const double xij = bk.xij[uId_np_j * nc + i];
+                const double transIJ = xij * xi * transJ;
+                dFdXpB[(i + 1) * ncol] += transIJ;
+                dFdXpE[(i + 1) * ncol] -= transIJ;
+                double tmp = xij * transJ * xiP * dP;
+                tmp += -transIJ * muP / mu * dP;
+                if (!phaseExistE[j]) {
+                    tmp += transIJ * (-rhoP * dGamma);
+                    dFdXpB[(i + 1) * ncol] += tmp;
+                } else if (!phaseExistB[j]) {
+                    tmp += transIJ * (-rhoP * dGamma);
+                    dFdXpE[(i + 1) * ncol] += tmp;
+                } else {
+                    dFdXpB[(i + 1) * ncol] += transIJ * (-bk.rhoP[bId_np_j] * dGamma) / 2;
+                    dFdXpE[(i + 1) * ncol] += transIJ * (-bk.rhoP[eId_np_j] * dGamma) / 2;
+                    if (bId == uId)
+                        dFdXpB[(i + 1) * ncol] += tmp;
+                    else
+                        dFdXpE[(i + 1) * ncol] += tmp;
+                }
+                int j1SB = 0, j1SE = 0;
+                if (bId == uId) {
+                    for (int j1 = 0; j1 < np; j1++) {
+                        if (phasedS_B[j1]) {
+                            dFdXsB[(i + 1) * ncolB + j1SB] += transIJ * bk.dPcj_dS[bId_np_j * np + j1];
+                            tmp = Akd * xij * xi / mu * bk.dKr_dS[uId_np_j * np + j1] * dP;
+                            dFdXsB[(i + 1) * ncolB + j1SB] += tmp;
+                            j1SB++;
+                        }
+                        if (phasedS_E[j1]) {
+                            dFdXsE[(i + 1) * ncolE + j1SE] -= transIJ * bk.dPcj_dS[eId_np_j * np + j1];
+                            j1SE++;
+                        }
+                    }
+                    if (!phaseExistE[j]) {
+                        for (int k = 0; k < pVnumComB[j]; k++) {
+                            const double rhox = bk.rhox[uId_np_j * nc + k];
+                            const double xix = bk.xix[uId_np_j * nc + k];
+                            const double mux = bk.mux[uId_np_j * nc + k];
+                            tmp = -transIJ * rhox * dGamma;
+                            tmp += xij * transJ * xix * dP;
+                            tmp += -transIJ * mux / mu * dP;
+                            dFdXsB[(i + 1) * ncolB + jxB + k] += tmp;
+                        }
+                        if (i < pVnumComB[j]) dFdXsB[(i + 1) * ncolB + jxB + i] += xi * transJ * dP;
+                    } else {
+                        for (int k = 0; k < pVnumComB[j]; k++) {
+                            const double rhox = bk.rhox[bId_np_j * nc + k] / 2;
+                            const double xix = bk.xix[uId_np_j * nc + k];
+                            const double mux = bk.mux[uId_np_j * nc + k];
+                            tmp = -transIJ * rhox * dGamma;
+                            tmp += xij * transJ * xix * dP;
+                            tmp += -transIJ * mux / mu * dP;
+                            dFdXsB[(i + 1) * ncolB + jxB + k] += tmp;
+                            dFdXsE[(i + 1) * ncolE + jxE + k] +=
+                                -transIJ * bk.rhox[eId_np_j * nc + k] / 2 * dGamma;
+                        }
+                        if (i < pVnumComB[j]) dFdXsB[(i + 1) * ncolB + jxB + i] += xi * transJ * dP;
+                    }
+                } else {
+                    for (int j1 = 0; j1 < np; j1++) {
+                        if (phasedS_B[j1]) {
+                            dFdXsB[(i + 1) * ncolB + j1SB] += transIJ * bk.dPcj_dS[bId_np_j * np + j1];
+                            j1SB++;
+                        }
+                        if (phasedS_E[j1]) {
+                            dFdXsE[(i + 1) * ncolE + j1SE] -= transIJ * bk.dPcj_dS[eId_np_j * np + j1];
+                            tmp = Akd * xij * xi / mu * bk.dKr_dS[uId_np_j * np + j1] * dP;
+                            dFdXsE[(i + 1) * ncolE + j1SE] += tmp;
+                            j1SE++;
+                        }
+                    }
+                    if (!phaseExistB[j]) {
+                        for (int k = 0; k < pVnumComE[j]; k++) {
+                            const double rhox = bk.rhox[uId_np_j * nc + k];
+                            const double xix = bk.xix[uId_np_j * nc + k];
+                            const double mux = bk.mux[uId_np_j * nc + k];
+                            tmp = -transIJ * rhox * dGamma;
+                            tmp += xij * transJ * xix * dP;
+                            tmp += -transIJ * mux / mu * dP;
+                            dFdXsE[(i + 1) * ncolE + jxE + k] += tmp;
+                        }
+                        if (i < pVnumComE[j]) dFdXsE[(i + 1) * ncolE + jxE + i] += xi * transJ * dP;
+                    } else {
+                        for (int k = 0; k < pVnumComE[j]; k++) {
+                            const double rhox = bk.rhox[eId_np_j * nc + k] / 2;
+                            const double xix = bk.xix[uId_np_j * nc + k];
+                            const double mux = bk.mux[uId_np_j * nc + k];
+                            tmp = -transIJ * rhox * dGamma;
+                            tmp += xij * transJ * xix * dP;
+                            tmp += -transIJ * mux / mu * dP;
+                            dFdXsE[(i + 1) * ncolE + jxE + k] += tmp;
+                            dFdXsB[(i + 1) * ncolB + jxB + k] +=
+                                -transIJ * bk.rhox[bId_np_j * nc + k] / 2 * dGamma;
+                        }
+                        if (i < pVnumComE[j]) dFdXsE[(i + 1) * ncolE + jxE + i] += xi * transJ * dP;
+                    }
+                }
+            }
+            jxB += pVnumComB[j];
+            jxE += pVnumComE[j];
+        }
+        // b side: AddDiag(bId, bmat) ; NewOffDiag(eId, bId, -bmat)
+        bmat = dFdXpB;
+        DaABpbC(ncol, ncol, ncolB, dFdXsB.data(), &bk.dSec_dPri[bId * lendSdP], bmat.data());
+        for (auto& v : bmat) v = dt * v;
+        {
+            double* d = &val[(size_t)IA[bId] * bsize];
+            for (int q = 0; q < bsize; q++) d[q] += bmat[q];
+            double* o = &val[(size_t)fill[eId]++ * bsize];
+            for (int q = 0; q < bsize; q++) o[q] = -1.0 * bmat[q];
+        }
+        // e side: NewOffDiag(bId, eId, bmat) ; AddDiag(eId, -bmat)
+        bmat = dFdXpE;
+        DaABpbC(ncol, ncol, ncolE, dFdXsE.data(), &bk.dSec_dPri[eId * lendSdP], bmat.data());
+        for (auto& v : bmat) v = dt * v;
+        {
+            double* o = &val[(size_t)fill[bId]++ * bsize];
+            for (int q = 0; q < bsize; q++) o[q] = bmat[q];
+            double* d = &val[(size_t)IA[eId] * bsize];
+            for (int q = 0; q < bsize; q++) d[q] += -1.0 * bmat[q];
+        }
+    }
+    (void)JA;
+}
+
+}  // namespace oracle

@@ -1,0 +1,179 @@
+// CPU restatement of OpenCAEPoro's compositional flash with FIM derivatives.
+//
+// TEST INFRASTRUCTURE ONLY.  This is the oracle the HIP kernels are checked
+// against (and the CPU baseline the benchmark times).  Nothing in the product
+// path may include, link or call it.
+//
+// It restates MixtureComp (src/MixtureComp.cpp, include/MixtureComp.hpp) and
+// SkipStaAnaly (src/AcceleratePVT.cpp) for the FIM method, keeping the
+// reference's arithmetic order, and the LAPACK routines the reference calls
+// (dsysv, dgesv, ssyevd) by their published algorithms.
+#ifndef ORACLE_FLASH_HPP
+#define ORACLE_FLASH_HPP
+
+#include "../include/ocpgpu.h"
+#include "real.hpp"
+
+namespace oracle {
+
+constexpr int MAXNC = OCPG_MAXNC;
+constexpr int NPMAX = 2;  // hydrocarbon phases
+constexpr int NPH = 3;    // with water
+
+// Constants of include/OCPConst.hpp:37-69 (values as the reference spells them).
+constexpr double GAS_CONSTANT = 10.73159;
+constexpr double PI_TRUNC = 3.141592653;
+constexpr double CONV1 = 5.61458;
+constexpr double CONV2 = 1.12712E-3;
+constexpr double CONV5 = 459.67;
+constexpr double GRAVITY_FACTOR = 0.00694444;
+constexpr double TINY = 1E-8;
+
+// Per-cell skip-stability memory (include/AcceleratePVT.hpp:71-85).
+struct SkipState {
+    bool flag;
+    double minEigen;
+    double P;
+    double T;
+    double zi[MAXNC];
+};
+
+class Mixture {
+public:
+    explicit Mixture(const ocpg_eos& eos);
+
+    // MixtureComp::FlashFIM, src/MixtureComp.cpp:494-573.
+    void FlashFIM(double Pin, double Tin, const double* Niin, const double* Sjin,
+                  int lastNP, const double* xijin, SkipState& skip);
+    // MixtureComp::InitFlashFIM, src/MixtureComp.cpp:261-353.
+    void InitFlashFIM(double Pin, double Tin, const double* Sjin, double Vpore,
+                      const double* Ziin, SkipState& skip);
+
+    int NC, numCom, numPhase;
+    // Outputs (Mixture base members, include/Mixture.hpp:241-292).
+    real P, T;
+    real vf, Nt, vfP;
+    real Ni[MAXNC + 1];
+    bool phaseExist[NPH];
+    real S[NPH], vj[NPH], nj[NPH];
+    real xij[NPH * (MAXNC + 1)];
+    real rho[NPH], xi[NPH], mu[NPH];
+    real vfi[MAXNC + 1];
+    real rhoP[NPH], xiP[NPH], muP[NPH];
+    real rhox[NPH * (MAXNC + 1)], xix[NPH * (MAXNC + 1)], mux[NPH * (MAXNC + 1)];
+    real dXsdXp[(MAXNC + 2) * (NPH + NPH * (MAXNC + 1))];
+    bool pSderExist[NPH];
+    int pVnumCom[NPH];
+    int ftype;
+    real ePEC;
+    long iters[5];  // SSMSTA, NRSTA, SSMSP, NRSP, RR for the last call
+
+private:
+    // constants
+    real Tc[MAXNC], Pc[MAXNC], Vc[MAXNC], MWC[MAXNC], Acf[MAXNC];
+    real OmegaA[MAXNC], OmegaB[MAXNC], Vshift[MAXNC], Vcvis[MAXNC];
+    real BIC[MAXNC * MAXNC];
+    real LBCcoef[5];
+    real muAux1I[MAXNC], sqrtMWi[MAXNC];
+    real delta1, delta2, delta1P2, delta1M2, delta1T2;
+    int pvtwRows;
+    double pvtw[OCPG_MAXTAB][5];
+    real std_RhoW;
+    // controls
+    int ssmStaMaxIt, nrStaMaxIt, ssmSpMaxIt, nrSpMaxIt, rrMaxIt;
+    real ssmStaTol, ssmStaTol2, ssmStaEYt, ssmStaKtol, ssmStaDYtol, nrStaTol;
+    real ssmSpTol, ssmSpTol2, nrSpTol, rrTol;
+    bool ssmStaConflag, nrStaConflag, ssmSpConflag, nrSpConflag;
+    real ssmStaRealTol, nrStaRealTol, ssmSpRealTol, nrSpRealTol;
+    int ssmStaCurIt, nrStaCurIt, ssmSpCurIt, nrSpCurIt, rrCurIt;
+
+    // working state
+    int lNP, NP;
+    real Nh;
+    real zi[MAXNC];
+    real Ai[MAXNC], Bi[MAXNC], Aj[NPMAX], Bj[NPMAX], Zj[NPMAX];
+    real Ztmp[3];
+    real vC[NPMAX], nu[NPMAX], xiC[NPMAX], rhoC[NPMAX], MW[NPMAX];
+    int phaseLabel[NPMAX];
+    real x[NPMAX][MAXNC], phi[NPMAX][MAXNC], fug[NPMAX][MAXNC], n[NPMAX][MAXNC],
+        ln[NPMAX][MAXNC];
+    real Kw[4][MAXNC], Ks[NPMAX - 1][MAXNC], lKs[MAXNC];
+    real Asta, Bsta, Zsta;
+    real phiSta[MAXNC], fugSta[MAXNC];
+    real Y[MAXNC], Yt, di[MAXNC];
+    real resSTA[MAXNC], JmatSTA[MAXNC * MAXNC];
+    real fugX[MAXNC * MAXNC];
+    real Ax[MAXNC], Bx[MAXNC], Zx[MAXNC];
+    real resSP[MAXNC], JmatSP[MAXNC * MAXNC];
+    real fugN[NPMAX][MAXNC * MAXNC];
+    real An[MAXNC], Bn[MAXNC], Zn[NPMAX][MAXNC];
+    real muC[NPMAX], muAux[NPMAX][5];
+    real fugP[NPMAX][MAXNC], Zp[NPMAX];
+    real JmatDer[MAXNC * MAXNC], rhsDer[MAXNC * (MAXNC + 1)];
+    real vjp[NPH], vji[NPH][MAXNC + 1];
+    real xixC[NPMAX * MAXNC], xiPC[NPMAX];
+    bool flagSkip;
+    real phiN[MAXNC * MAXNC];
+
+    void InitPTN(double Pin, double Tin, const double* Niin);
+    void InitPTZ(double Pin, double Tin, const double* Ziin);
+    void CalFtypeFIM(const double* Sjin, const SkipState& skip);
+    void CalFlash();
+    void PhaseEquilibrium();
+    void CalAiBi();
+    void CalAjBj(real& AjT, real& BjT, const real* xj) const;
+    void SolEoS(real& ZjT, const real& AjT, const real& BjT);
+    int CubicRoot(const real& a, const real& b, const real& c, bool NTflag);
+    void CalFugPhi(real* phiT, real* fugT, const real* xj);
+    void CalFugPhiFugOnly(real* fugT, const real* xj);
+    void CalFugPhiAll();
+    void CalMW();
+    void CalVfXiRho();
+    void CalSaturation();
+    int FindMWmax();
+    void x2n();
+    void CalKwilson();
+    bool PhaseStable();
+    bool StableSSM01(int Id);
+    bool StableSSM(int Id);
+    bool StableNR(int Id);
+    void CalFugXSTA();
+    void AssembleJmatSTA();
+    bool CheckSplit();
+    void PhaseSplit();
+    void SplitSSM2(bool flag);
+    void RachfordRice2();
+    void UpdateXRR();
+    void SplitNR();
+    void CalResSP();
+    void CalFugNAll(bool Znflag);
+    void AssembleJmatSP();
+    real CalStepNRsp();
+    void IdentifyPhase();
+    void CopyPhase();
+    void CalViscoLBC();
+    void CalFugPAll(bool Zpflag);
+    void CalVfiVfp_full02();
+    void CaldXsdXpAPI02p();
+    void CalXiPNX_partial();
+    void CalRhoPX_partial();
+    void CalMuPXLBC_partial();
+    void WaterProperties(bool init, const double* Sjin, double Vpore);
+    void SetDerivFlags();
+    void CalPhiNSkip();
+    void CalSkipForNextStep(SkipState& skip);
+    void PVTWEval(real Pv, real* data, real* cdata) const;
+};
+
+// LAPACK restatements (published algorithms of reference LAPACK).
+// dsytf2+dsytrs with UPLO='U' on a column-major n x n array, one rhs.
+int SymIndefSolve(int n, real* a, real* b);
+// dgetf2+dgetrs on a column-major n x n array with nrhs columns.
+int LUSolveColMajor(int n, int nrhs, real* a, real* b);
+// Smallest eigenvalue of a symmetric float matrix (what ssyevd's first
+// eigenvalue is), computed by cyclic Jacobi in double and rounded to float.
+float MinEigenSym(int n, const float* a);
+
+}  // namespace oracle
+
+#endif

@@ -1,0 +1,145 @@
+"""Writes the flash golden vectors from the REFERENCE's own MixtureComp.
+
+Runs only where oracle/_ref/libocpref.so was compiled from the reference
+sources (``make -C oracle ref``); the committed .npz files are what the tests
+and the GPU box use.  Inputs are seeded; each file records the seeds, the
+deck, the EoS description the reference constructor derived, and for each
+pass (cold: lastNP = 0 and empty skip memory; warm: the cold result with
+P + U[-5, 5] psia) every array PassFlashValue hands to the Bulk.
+
+    python tests/golden/gen_flash_golden.py
+"""
+import ctypes as C
+import json
+import os
+import platform
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.abspath(os.path.join(HERE, "..", "..")))
+
+from opencaeporo_amd import abi, fluids, workloads  # noqa: E402
+from oracle.oracle_py import Reference  # noqa: E402
+
+OUT_FIELDS = ["phaseNum", "phaseExist", "S", "xij", "rho", "xi", "mu", "rhoP", "xiP", "muP",
+              "rhox", "xix", "mux", "Nt", "vf", "vfP", "vfi", "pSderExist", "pVnumCom",
+              "bRowSizedSdP", "dSec_dPri", "skipFlag", "skipMinEig", "skipP", "skipT",
+              "skipZi", "ftype", "iters"]
+IN_FIELDS = ["P", "T", "Ni", "S", "phaseNum", "xij", "skipFlag", "skipMinEig", "skipP",
+             "skipT", "skipZi"]
+
+
+def deck_arrays(deck):
+    NC = deck.nc
+    return dict(
+        deck_nc=np.int32(NC), deck_Tc=np.array(deck.Tc[:NC]), deck_Pc=np.array(deck.Pc[:NC]),
+        deck_MW=np.array(deck.MW[:NC]), deck_Acf=np.array(deck.Acf[:NC]),
+        deck_Zc=np.array(deck.Zc[:NC]),
+        deck_BIC_lower=np.array(deck.BIC_lower[:NC * (NC - 1) // 2]),
+        deck_LBCcoef=np.array(deck.LBCcoef[:]),
+        deck_pvtw=np.array([list(deck.pvtw[r]) for r in range(deck.pvtw_rows)]),
+        deck_std_rhow=np.float64(deck.std_rhow),
+        deck_controls=np.array([deck.ssmsta_maxit, deck.ssmsta_tol, deck.ssmsta_eyt,
+                                deck.nrsta_maxit, deck.nrsta_tol, deck.ssmsp_maxit,
+                                deck.ssmsp_tol, deck.nrsp_maxit, deck.nrsp_tol, deck.rr_maxit,
+                                deck.rr_tol]),
+    )
+
+
+def eos_arrays(eos):
+    NC = eos.nc
+    bic = np.array(eos.BIC[:]).reshape(abi.MAXNC, abi.MAXNC)[:NC, :NC]
+    return dict(eos_Vc=np.array(eos.Vc[:NC]), eos_OmegaA=np.array(eos.OmegaA[:NC]),
+                eos_OmegaB=np.array(eos.OmegaB[:NC]), eos_Vshift=np.array(eos.Vshift[:NC]),
+                eos_Vcvis=np.array(eos.Vcvis[:NC]), eos_BIC=bic,
+                eos_LBCcoef=np.array(eos.LBCcoef[:]))
+
+
+def used_dsec(b):
+    L = abi.max_len_dsdp(b.nc)
+    used = int(b.bRowSizedSdP.max()) * (b.nc + 1)
+    return b.dSec_dPri.reshape(b.nb, L)[:, :used].copy()
+
+
+def snapshot(b, fields, prefix):
+    out = {}
+    for f in fields:
+        a = used_dsec(b) if f == "dSec_dPri" else getattr(b, f).copy()
+        out[prefix + f] = a
+    return out
+
+
+def gen_flash(name, deck, nb, seeds, dp_seed):
+    NC = deck.nc
+    ref = Reference(deck, nb)
+    P, z, Sw = workloads.cell_inputs(nb, NC, seed=seeds[0])
+    Ni = workloads.moles_from_z(z, seed=seeds[1])
+    b = abi.Bulk(nb, NC)
+    b.P[:] = P
+    b.T[:] = workloads.T_F
+    b.Ni[:] = Ni.ravel()
+    data = dict(seeds=np.array(list(seeds) + [dp_seed]))
+    data.update(deck_arrays(deck))
+    data.update(eos_arrays(ref.eos()))
+    data.update(snapshot(b, IN_FIELDS, "cold_in_"))
+    ref.flash_fim(b)
+    data.update(snapshot(b, OUT_FIELDS, "cold_out_"))
+    rng = np.random.default_rng(dp_seed)
+    b.P[:] = P + rng.uniform(-5.0, 5.0, nb)
+    data.update(snapshot(b, IN_FIELDS, "warm_in_"))
+    ref.flash_fim(b)
+    data.update(snapshot(b, OUT_FIELDS, "warm_out_"))
+    path = os.path.join(HERE, name)
+    np.savez_compressed(path, **data)
+    two = np.mean(b.phaseNum == 3)
+    print(f"{name}: {nb} cells NC={NC}, two-phase {two:.2f}, {os.path.getsize(path)} bytes")
+
+
+def gen_init(name, deck, nb, seeds):
+    NC = deck.nc
+    ref = Reference(deck, nb)
+    P, z, Sw = workloads.cell_inputs(nb, NC, seed=seeds[0])
+    rng = np.random.default_rng(seeds[1])
+    b = abi.Bulk(nb, NC)
+    b.P[:] = P
+    b.T[:] = workloads.T_F
+    b.S.reshape(nb, 3)[:, 2] = Sw
+    b.rockVp[:] = rng.uniform(1.0e4, 5.0e4, nb)
+    data = dict(seeds=np.array(seeds), init_z=z.copy(), init_rockVp=b.rockVp.copy())
+    data.update(deck_arrays(deck))
+    data.update(snapshot(b, ["P", "T", "S"], "init_in_"))
+    ref.init_flash_fim(b, z)
+    data.update(snapshot(b, OUT_FIELDS + ["Ni"], "init_out_"))
+    path = os.path.join(HERE, name)
+    np.savez_compressed(path, **data)
+    print(f"{name}: {nb} cells NC={NC}, {os.path.getsize(path)} bytes")
+
+
+def mkl_version():
+    try:
+        meta = "/opt/conda/conda-meta"
+        return sorted(f for f in os.listdir(meta) if f.startswith("mkl-2"))[0][:-5]
+    except OSError:
+        return "unknown"
+
+
+def main():
+    if not Reference.available():
+        sys.exit("oracle/_ref/libocpref.so is not built (make -C oracle ref)")
+    gen_flash("flash_spe5.npz", fluids.spe5_deck(), 384, (101, 102), 103)
+    gen_flash("flash_bench6.npz", fluids.bench6_deck(), 256, (201, 202), 203)
+    gen_flash("flash_ten.npz", fluids.ten_deck(), 192, (301, 302), 303)
+    gen_init("initflash_spe5.npz", fluids.spe5_deck(), 128, (401, 402))
+    meta = dict(reference="OpenCAEPoro 0.5.0 MixtureComp (src/MixtureComp.cpp) compiled by "
+                          "oracle/Makefile 'ref'",
+                lapack=mkl_version() + " (sequential layer)",
+                compiler="g++ -O3 -std=c++14", host=platform.processor() or platform.machine(),
+                ten_component_bic_seed=20241203)
+    with open(os.path.join(HERE, "flash_meta.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()
