@@ -198,6 +198,96 @@ typedef struct ocpg_flash_stats {
     int64_t nonconverged; /* cells whose ePEC would read 1e8                    */
 } ocpg_flash_stats;
 
+/* ------------------------------------------------------------------------ */
+/* Entry points.                                                              */
+/* ------------------------------------------------------------------------ */
+
+typedef struct ocpg_ctx ocpg_ctx;   /* one device, one stream, one host thread */
+
+const char* ocpg_version(void);
+/* sizeof() of a structure above by name ("ocpg_bulk", ...), -1 if unknown.  */
+int64_t ocpg_sizeof(const char* name);
+
+/* MixtureComp::MixtureComp (src/MixtureComp.cpp:28-165): deck values to the
+ * EoS description the flash uses.  Host only; needs no device.              */
+int ocpg_eos_from_deck(const ocpg_eos_deck* deck, ocpg_eos* eos);
+
+/* Context: stands in for the per-region MixtureComp + FlowUnit + Rock
+ * objects (Bulk::flashCal/flow/rock) on one GPU.  sat and rock may be NULL
+ * when only the flash is used.                                               */
+int ocpg_create(ocpg_ctx** ctx, const ocpg_eos* eos, const ocpg_satfunc* sat,
+                const ocpg_rock* rock, int device);
+int ocpg_destroy(ocpg_ctx* ctx);
+
+/* Device storage for nb cells (IsoT_FIM::AllocateReservoir,
+ * src/OCPFluidMethod.cpp:980-1084), zero-filled.                            */
+int ocpg_set_cells(ocpg_ctx* ctx, int64_t nb);
+/* Host <-> device copies of every non-NULL ocpg_bulk field (reference layout).
+ * lNi, when given, is the last-step copy that CalRes reads.                  */
+int ocpg_upload_bulk(ocpg_ctx* ctx, const ocpg_bulk* bulk);
+int ocpg_download_bulk(ocpg_ctx* ctx, ocpg_bulk* bulk);
+
+/* IsoT_FIM::CalFlash (src/OCPFluidMethod.cpp:1108-1120): MixtureComp::
+ * FlashFIM (src/MixtureComp.cpp:494-573) + PassFlashValue (:1122-1187) for
+ * every cell, with the skip-stability memory of SkipStaAnaly.  stats may be
+ * NULL (then the call is asynchronous).                                      */
+int ocpg_flash_fim(ocpg_ctx* ctx, ocpg_flash_stats* stats);
+/* IsoT_FIM::InitFlash (src/OCPFluidMethod.cpp:1095-1106): MixtureComp::
+ * InitFlashFIM (src/MixtureComp.cpp:261-353) from P, T, Sw (column 2 of S),
+ * rockVp and the feed zi (host array, nb x NC row-major); Ni is an output.  */
+int ocpg_init_flash_fim(ocpg_ctx* ctx, const double* zi);
+
+/* IsoT_FIM::UpdateLastTimeStep / ResetToLastTimeStep, Bulk and skip-state
+ * part (src/OCPFluidMethod.cpp:2859-2957, src/AcceleratePVT.cpp:122-138).   */
+int ocpg_update_last_step(ocpg_ctx* ctx);
+int ocpg_reset_to_last_step(ocpg_ctx* ctx);
+
+/* BulkConn::iteratorConn (src/BulkConn.cpp:31-117) and the BSR pattern of
+ * LinearSystem (include/LinearSystem.hpp:62-125) it implies.                */
+int ocpg_set_conn(ocpg_ctx* ctx, const ocpg_conn* conn);
+int64_t ocpg_bsr_nnzb(ocpg_ctx* ctx);
+
+/* IsoT_FIM::CalKrPc (src/OCPFluidMethod.cpp:1189-1199) and
+ * IsothermalMethod::CalRock (src/OCPFluidMethod.cpp:25-32).                 */
+int ocpg_calkrpc(ocpg_ctx* ctx);
+int ocpg_calrock(ocpg_ctx* ctx);
+int ocpg_calkrpc_rock(ocpg_ctx* ctx);
+
+/* IsoT_FIM::CalRes (src/OCPFluidMethod.cpp:1201-1389), bulk part; the
+ * negated residual stays on the device as the right-hand side.  norms may be
+ * NULL (then the call is asynchronous).                                      */
+int ocpg_calres(ocpg_ctx* ctx, double dt, ocpg_resnorm* norms);
+/* IsoT_FIM::AssembleMat (src/OCPFluidMethod.cpp:860-875): bulk blocks of
+ * AssembleMatBulksNew (:1569-1839) into the device BSR.                     */
+int ocpg_assemble(ocpg_ctx* ctx, double dt);
+/* The assembled system as VectorFaspSolver::AssembleMat copies it into a
+ * dBSRmat (src/FaspSolver.cpp:362-429): IA, JA, val (row-major blocks), rhs.
+ * Any pointer may be NULL.                                                    */
+int ocpg_get_bsr(ocpg_ctx* ctx, int32_t* IA, int32_t* JA, double* val, double* rhs);
+/* BulkConn::upblock / upblock_Rho / upblock_Velocity as CalRes set them.    */
+int ocpg_get_upwind(ocpg_ctx* ctx, int32_t* upblock, double* upblock_Rho,
+                    double* upblock_Velocity);
+
+/* Copy / restore of the fields the flash starts from (P, Ni, S, phaseNum,
+ * xij, skip-stability memory), to re-run one Newton iteration from the same
+ * state (benchmarks, retries).  No reference counterpart.                    */
+int ocpg_save_flash_inputs(ocpg_ctx* ctx);
+int ocpg_restore_flash_inputs(ocpg_ctx* ctx);
+
+/* Per-kernel HIP-event timers on the context's stream.  Names: "flash",
+ * "krpc_rock", "calres", "assemble", "init_flash", "restore".  No reference
+ * counterpart (the reference times whole phases with GetWallTime,
+ * include/UtilTiming.hpp:31-62).                                             */
+int ocpg_timing_enable(ocpg_ctx* ctx, int on);
+int ocpg_timing_get(ocpg_ctx* ctx, const char* name, double* ms, int64_t* count);
+
+int ocpg_synchronize(ocpg_ctx* ctx);
+/* The context's HIP stream (as void*), for callers that order their own work
+ * after the library's.                                                        */
+void* ocpg_stream(ocpg_ctx* ctx);
+/* Device bytes held by the context.                                          */
+int64_t ocpg_device_bytes(ocpg_ctx* ctx);
+
 #ifdef __cplusplus
 }
 #endif

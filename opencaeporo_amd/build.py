@@ -1,0 +1,103 @@
+"""Builds libocpgpu.so (HIP, gfx950) in-tree.
+
+    python -m opencaeporo_amd.build [--incremental] [--nc 6,10]
+
+Each flash instantiation (one per supported hydrocarbon component count) is
+its own translation unit so the instantiations compile in parallel.  The
+library is written next to this file; the tests, the smoke check and the
+benchmark load it from there.
+"""
+import argparse
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libocpgpu.so")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+
+DEFAULT_NC = (6, 10)
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+# No FP contraction: the kernels keep the reference's rounding order.
+# Packed FP32 instructions are disabled (wrong results have been seen from
+# some of their forms on this hardware).
+FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch={ARCH}",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", CSRC, "-I", INCLUDE,
+         "-Wno-pass-failed"]
+
+
+def _headers():
+    return [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hpp", ".h"))] + \
+        [os.path.join(INCLUDE, "ocpgpu.h")]
+
+
+def _jobs(ncs):
+    nc_list = " ".join(f"X({n})" for n in ncs)
+    jobs = []
+    for n in ncs:
+        jobs.append(("flash_nc.hip", f"flash_nc{n}.o", [f"-DOCPG_NC={n}"]))
+    jobs.append(("flash_dispatch.cpp", "flash_dispatch.o", [f"-DOCPG_NC_LIST(X)={nc_list}"]))
+    jobs.append(("fim_kernels.hip", "fim_kernels.o", []))
+    jobs.append(("ocpgpu_capi.cpp", "ocpgpu_capi.o", []))
+    jobs.append(("fim_capi.cpp", "fim_capi.o", []))
+    return jobs
+
+
+def _compile(src, obj, extra, incremental, stamp):
+    srcp = os.path.join(CSRC, src)
+    objp = os.path.join(BUILD, obj)
+    if incremental and os.path.exists(objp):
+        t = os.path.getmtime(objp)
+        newest = max([os.path.getmtime(srcp)] + [os.path.getmtime(h) for h in _headers()])
+        if t >= newest and os.path.exists(stamp) and open(stamp).read() == " ".join(extra):
+            return obj, 0.0, ""
+    cmd = [HIPCC] + FLAGS + extra + ["-c", srcp, "-o", objp]
+    t0 = time.time()
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    with open(stamp, "w") as f:
+        f.write(" ".join(extra))
+    return obj, time.time() - t0, r.stderr
+
+
+def build(ncs=DEFAULT_NC, incremental=False, verbose=True, workers=None):
+    os.makedirs(BUILD, exist_ok=True)
+    jobs = _jobs(ncs)
+    workers = workers or min(len(jobs), os.cpu_count() or 4, 16)
+    objs = []
+    with cf.ThreadPoolExecutor(max_workers=workers) as ex:
+        futs = [ex.submit(_compile, s, o, e, incremental, os.path.join(BUILD, o + ".flags"))
+                for s, o, e in jobs]
+        for f in cf.as_completed(futs):
+            obj, dt, _ = f.result()
+            objs.append(obj)
+            if verbose and dt > 0:
+                print(f"  built {obj} in {dt:.1f}s", flush=True)
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + \
+        [os.path.join(BUILD, o) for _, o, _ in jobs]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"  linked {LIB}", flush=True)
+    return LIB
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--incremental", action="store_true")
+    ap.add_argument("--nc", default=",".join(str(n) for n in DEFAULT_NC))
+    a = ap.parse_args()
+    build(tuple(int(x) for x in a.nc.split(",")), incremental=a.incremental)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,79 @@
+// Device-side data layout of the flash / assembly library.
+//
+// Cell arrays are structure-of-arrays in HBM: field X of cell n, entry k, is
+// X[k * ld + n] with ld = the padded cell count.  Consecutive lanes of a
+// wavefront handle consecutive cells, so every per-cell load and store of a
+// wavefront is one coalesced 512-byte (FP64) or 64-byte (byte flags) access.
+// The entry index k follows the reference's in-cell order (include/Bulk.hpp:
+// 98-102): Ni[i], S[j], xij[j*nc+i], dKr_dS[j*np+k], dSec_dPri[r*ncol+c].
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ocpg {
+
+constexpr int kNP = 3;
+constexpr int kMaxNC = 16;
+constexpr int kMaxTab = 64;
+
+// EoS constants as the kernels read them (uniform across a launch: every
+// access compiles to scalar loads).
+struct DevEos {
+    int nc;
+    int pvtwRows;
+    double Tc[kMaxNC], Pc[kMaxNC], Vc[kMaxNC], MWC[kMaxNC], Acf[kMaxNC];
+    double OmegaA[kMaxNC], OmegaB[kMaxNC], Vshift[kMaxNC], Vcvis[kMaxNC];
+    double oneMinusBIC[kMaxNC * kMaxNC];  // (1 - BIC[i*NC+k]), row-major NC x NC
+    double LBCcoef[5];
+    double muAux1I[kMaxNC];               // 5.4402 * Tc^(1/6) / Pc^(2/3)
+    double sqrtMWi[kMaxNC];
+    double pvtw[kMaxTab][5];
+    double stdRhoW;
+    // controls
+    int ssmStaMaxIt, nrStaMaxIt, ssmSpMaxIt, nrSpMaxIt, rrMaxIt;
+    double ssmStaTol2, ssmStaEYt, ssmStaKtol, ssmStaDYtol, nrStaTol;
+    double ssmSpTol2, nrSpTol, rrTol;
+};
+
+struct DevSat {
+    int swofRows, sgofRows;
+    double swof[kMaxTab][4];
+    double sgof[kMaxTab][4];
+    double kroMax;
+};
+
+struct DevRock {
+    double Pref, cp1, cp2;
+};
+
+// SoA cell state.  Every pointer addresses ld * (entries per cell) values.
+struct DevBulk {
+    int64_t nb;  // cells
+    int64_t ld;  // leading dimension (>= nb)
+    double *P, *T, *Ni, *lNi, *S, *xij;
+    int32_t* phaseNum;
+    uint8_t* phaseExist;
+    double *Nt, *vf, *vfP, *vfi;
+    double *rho, *xi, *mu, *rhoP, *xiP, *muP;
+    double *rhox, *xix, *mux;
+    uint8_t* pSderExist;
+    int32_t *pVnumCom, *bRowSizedSdP;
+    double* dSec;  // maxLendSdP entries per cell
+    uint8_t* skipFlag;
+    double *skipMinEig, *skipP, *skipT, *skipZi;
+    double *kr, *Pc, *Pj, *dKr_dS, *dPcj_dS;
+    double *v, *depth, *poroInit, *poro, *poroP, *rockVp;
+    int32_t *ftype, *iters;
+};
+
+// Totals gathered by the flash kernels (one atomic per wavefront).
+struct DevFlashCounters {
+    unsigned long long cells;
+    unsigned long long ftype[3];
+    unsigned long long twoPhase;
+    unsigned long long iters[5];
+    unsigned long long nonconverged;
+};
+
+}  // namespace ocpg

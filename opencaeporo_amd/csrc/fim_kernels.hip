@@ -1,0 +1,501 @@
+// FIM kernels over the SoA cell state:
+//   krPcRockKernel : IsoT_FIM::CalKrPc (FlowUnit_ODGW01::CalKrPcDeriv, Stone II)
+//                    and IsothermalMethod::CalRock (Rock_Linear)
+//   calResKernel   : IsoT_FIM::CalRes, bulk part (accumulation + upwinded
+//                    TPFA fluxes), relative-residual maxima, negation
+//   assembleKernel : IsoT_FIM::AssembleMatBulksNew into the reference's BSR
+#include "fim_kernels.hpp"
+
+namespace ocpg {
+
+constexpr double kCONV1f = 5.61458;            // include/OCPConst.hpp:59
+constexpr double kCONV2f = 1.12712E-3;         // include/OCPConst.hpp:60
+constexpr double kGravityFactor = 0.00694444;  // include/OCPConst.hpp:51
+
+// OCPTable::Eval_All on column 0 (src/OCPTable.cpp:47-86): the interval
+// [r, r+1) holding val, flat extrapolation with zero slope outside.
+__device__ __forceinline__ void tableEval4(const double (*tab)[4], int nRow, double val, double* out,
+                                           double* slope)
+{
+    int seg = -1;
+    for (int r = 0; r + 1 < nRow; ++r) {
+        if (val >= tab[r][0] && val < tab[r + 1][0]) {
+            seg = r;
+            break;
+        }
+    }
+    if (seg < 0) {
+        const int r = (val < tab[0][0]) ? 0 : nRow - 1;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            slope[q] = 0;
+            out[q] = tab[r][q];
+        }
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        slope[q] = (tab[seg + 1][q] - tab[seg][q]) / (tab[seg + 1][0] - tab[seg][0]);
+        out[q] = tab[seg][q] + slope[q] * (val - tab[seg][0]);
+    }
+}
+
+// FlowUnit_ODGW01::CalKrPcDeriv + CalKro_Stone2Der (src/FlowUnit.cpp:235-328),
+// Pj = P + Pc (src/OCPFluidMethod.cpp:1189-1199); Rock_Linear::CalPoro
+// (src/Rock.cpp:19-27) and rockVp = v * poro (src/OCPFluidMethod.cpp:25-32).
+__global__ void __launch_bounds__(256) krPcRockKernel(DevBulk B, const DevSat* __restrict__ sat, DevRock rock,
+                                                      int krpc, int rockOn)
+{
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= B.nb) return;
+    const int64_t ld = B.ld;
+    if (krpc) {
+        double data[4], cdata[4];
+        const double Sg = B.S[1 * ld + n];
+        const double Sw = B.S[2 * ld + n];
+        tableEval4(sat->swof, sat->swofRows, Sw, data, cdata);
+        const double krw = data[1], dKrwdSw = cdata[1];
+        const double krow = data[2], dKrowdSw = cdata[2];
+        const double Pcwo = -data[3], dPcwdSw = -cdata[3];
+        tableEval4(sat->sgof, sat->sgofRows, Sg, data, cdata);
+        const double krg = data[1], dKrgdSg = cdata[1];
+        const double krog = data[2], dKrogdSg = cdata[2];
+        const double Pcgo = data[3], dPcgdSg = cdata[3];
+        const double kroMax = sat->kroMax;
+        double kro = kroMax * ((krow / kroMax + krw) * (krog / kroMax + krg) - (krw + krg));
+        double dKrodSw = kroMax * ((dKrowdSw / kroMax + dKrwdSw) * (krog / kroMax + krg) - (dKrwdSw));
+        double dKrodSg = kroMax * ((krow / kroMax + krw) * (dKrogdSg / kroMax + dKrgdSg) - (dKrgdSg));
+        if (kro < 0) {
+            kro = 0;
+            dKrodSg = 0;
+            dKrodSw = 0;
+        }
+        B.kr[0 * ld + n] = kro;
+        B.kr[1 * ld + n] = krg;
+        B.kr[2 * ld + n] = krw;
+        B.Pc[0 * ld + n] = 0;
+        B.Pc[1 * ld + n] = Pcgo;
+        B.Pc[2 * ld + n] = Pcwo;
+        const double dkr[9] = {0, dKrodSg, dKrodSw, 0, dKrgdSg, 0, 0, 0, dKrwdSw};
+        const double dpc[9] = {0, 0, 0, 0, dPcgdSg, 0, 0, 0, dPcwdSw};
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            B.dKr_dS[q * ld + n] = dkr[q];
+            B.dPcj_dS[q * ld + n] = dpc[q];
+        }
+        const double P = B.P[n];
+        B.Pj[0 * ld + n] = P + 0.0;
+        B.Pj[1 * ld + n] = P + Pcgo;
+        B.Pj[2 * ld + n] = P + Pcwo;
+    }
+    if (rockOn) {
+        const double dP = (B.P[n] - rock.Pref);
+        const double pi = B.poroInit[n];
+        const double poro = pi * (1 + (rock.cp1 + rock.cp2 / 2 * dP) * dP);
+        B.poro[n] = poro;
+        B.poroP[n] = pi * (rock.cp1 + rock.cp2 * dP);
+        B.rockVp[n] = B.v[n] * poro;
+    }
+}
+
+hipError_t launchKrPcRock(const DevBulk& B, const DevSat* sat, const DevRock& rock, bool krpc, bool rockOn,
+                          hipStream_t s)
+{
+    if (B.nb == 0) return hipSuccess;
+    const unsigned grid = (unsigned)((B.nb + 255) / 256);
+    hipLaunchKernelGGL(krPcRockKernel, dim3(grid), dim3(256), 0, s, B, sat, rock, krpc ? 1 : 0, rockOn ? 1 : 0);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// CalRes (src/OCPFluidMethod.cpp:1201-1389), one lane per cell.  A cell meets
+// its connections in the order of its BSR row (lower neighbours ascending,
+// then upper ones), which is the order the reference's connection loop adds
+// them to that cell, so each cell's residual is summed in the reference's
+// order.  The connection's upwind data is written by the bId side only.
+// ---------------------------------------------------------------------------
+constexpr int kResBlock = 256;
+
+template <int NC>
+__global__ void __launch_bounds__(kResBlock) calResKernel(DevBulk B, FimDevice F, double dt)
+{
+    constexpr int nc = NC + 1;
+    constexpr int len = nc + 1;
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t ld = B.ld;
+    double vmax = -1.0, nmax = -1.0;
+    int64_t vidx = 0, nidx = 0;
+    if (n < B.nb) {
+        double r[len];
+        r[0] = B.rockVp[n] - B.vf[n];
+#pragma unroll
+        for (int i = 0; i < nc; ++i) r[1 + i] = B.Ni[i * ld + n] - B.lNi[i * ld + n];
+        const int k0 = F.IA[n] + 1, k1 = F.IA[n + 1];
+        for (int k = k0; k < k1; ++k) {
+            const int64_t m = F.JA[k];
+            const int64_t c = F.slotConn[k];
+            const bool isB = n < m;
+            const int64_t bId = isB ? n : m;
+            const int64_t eId = isB ? m : n;
+            const double Akd = kCONV1f * kCONV2f * F.area[c];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const bool exB = B.phaseExist[j * ld + bId];
+                const bool exE = B.phaseExist[j * ld + eId];
+                double rho;
+                if (exB && exE)
+                    rho = (B.rho[j * ld + bId] + B.rho[j * ld + eId]) / 2;
+                else if (exB)
+                    rho = B.rho[j * ld + bId];
+                else if (exE)
+                    rho = B.rho[j * ld + eId];
+                else {
+                    if (isB) {
+                        F.upblock[c * 3 + j] = (int32_t)bId;
+                        F.upRho[c * 3 + j] = 0;
+                    }
+                    continue;
+                }
+                const double dP = (B.Pj[j * ld + bId] - kGravityFactor * rho * B.depth[bId]) -
+                                  (B.Pj[j * ld + eId] - kGravityFactor * rho * B.depth[eId]);
+                const int64_t uId = (dP < 0) ? eId : bId;
+                const bool exU = B.phaseExist[j * ld + uId];
+                if (isB) {
+                    F.upRho[c * 3 + j] = rho;
+                    F.upblock[c * 3 + j] = (int32_t)uId;
+                    F.upVel[c * 3 + j] =
+                        exU ? Akd * B.kr[j * ld + uId] / B.mu[j * ld + uId] * dP : 0.0;
+                }
+                if (!exU) continue;
+                const double tmp = dt * Akd * B.xi[j * ld + uId] * B.kr[j * ld + uId] / B.mu[j * ld + uId] * dP;
+#pragma unroll
+                for (int i = 0; i < nc; ++i) {
+                    const double dNi = tmp * B.xij[(j * nc + i) * ld + uId];
+                    if (isB)
+                        r[1 + i] += dNi;
+                    else
+                        r[1 + i] -= dNi;
+                }
+            }
+        }
+        const double rvp = B.rockVp[n], Nt = B.Nt[n];
+#pragma unroll
+        for (int e = 0; e < len; ++e) {
+            const double t = fabs(r[e] / rvp);
+            if (vmax < t) {
+                vmax = t;
+                vidx = n;
+            }
+        }
+#pragma unroll
+        for (int e = 1; e < len; ++e) {
+            const double t = fabs(r[e] / Nt);
+            if (nmax < t) {
+                nmax = t;
+                nidx = n;
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < len; ++e) F.res[n * len + e] = -r[e];
+    }
+    // block maxima, ties to the lowest cell (the reference scans cells upward
+    // and keeps the first strict maximum)
+    __shared__ double sv[kResBlock], sn[kResBlock];
+    __shared__ int64_t iv[kResBlock], in_[kResBlock];
+    sv[threadIdx.x] = vmax;
+    sn[threadIdx.x] = nmax;
+    iv[threadIdx.x] = vidx;
+    in_[threadIdx.x] = nidx;
+    __syncthreads();
+    for (int s = kResBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            const int o = threadIdx.x + s;
+            if (sv[o] > sv[threadIdx.x] || (sv[o] == sv[threadIdx.x] && iv[o] < iv[threadIdx.x])) {
+                sv[threadIdx.x] = sv[o];
+                iv[threadIdx.x] = iv[o];
+            }
+            if (sn[o] > sn[threadIdx.x] || (sn[o] == sn[threadIdx.x] && in_[o] < in_[threadIdx.x])) {
+                sn[threadIdx.x] = sn[o];
+                in_[threadIdx.x] = in_[o];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        F.normPart[2 * blockIdx.x + 0] = sv[0];
+        F.normPart[2 * blockIdx.x + 1] = sn[0];
+        F.normIdx[2 * blockIdx.x + 0] = iv[0];
+        F.normIdx[2 * blockIdx.x + 1] = in_[0];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// AssembleMatBulksNew (src/OCPFluidMethod.cpp:1569-1839), one wavefront per
+// BSR row.  For each connection the reference forms two blocks, one per side
+// (B: dFdXpB + dFdXsB * dSec[bId], E: dFdXpE + dFdXsE * dSec[eId]), adds each
+// to its own cell's diagonal and writes it, signed, as the other cell's
+// off-diagonal.  Here the row of cell n forms only its own side's block for
+// each of its connections: it accumulates the diagonal in registers (in the
+// reference's connection order) and writes the off-diagonal into the
+// neighbour's row at the slot the reference would have filled.  Every block
+// is computed once, nothing is accumulated across waves, no atomics.
+// ---------------------------------------------------------------------------
+constexpr int kAsmBlock = 64;
+
+template <int NC>
+struct AsmDims {
+    static constexpr int nc = NC + 1;
+    static constexpr int ncol = nc + 1;
+    static constexpr int bsize = ncol * ncol;
+    static constexpr int maxRows = 3 + 2 * NC;          // compact secondary rows
+    static constexpr int perLane = (bsize + kAsmBlock - 1) / kAsmBlock;
+    static constexpr int dfEntries = nc * (maxRows + 1);  // P column + secondary columns
+};
+
+template <int NC>
+__global__ void __launch_bounds__(kAsmBlock) assembleKernel(DevBulk B, FimDevice F, double dt)
+{
+    using D = AsmDims<NC>;
+    constexpr int nc = D::nc;
+    constexpr int ncol = D::ncol;
+    constexpr int bsize = D::bsize;
+    const int64_t n = blockIdx.x;
+    if (n >= B.nb) return;
+    const int64_t ld = B.ld;
+    const int lane = threadIdx.x;
+
+    __shared__ double sSec[D::maxRows * ncol];  // own dSec_dPri (compact rows x ncol)
+    __shared__ double sDF[D::dfEntries];        // own side: [i][0] = dFdXp, [i][1+k] = dFdXs
+
+    // own compact derivative block
+    const int rowsOwn = min(max(B.bRowSizedSdP[n], 0), D::maxRows);
+    for (int q = lane; q < rowsOwn * ncol; q += kAsmBlock) sSec[q] = B.dSec[(int64_t)q * ld + n];
+
+    // diagonal starts as the accumulation block (src/OCPFluidMethod.cpp:1586-1597)
+    double acc[D::perLane];
+#pragma unroll
+    for (int t = 0; t < D::perLane; ++t) {
+        const int e = lane + t * kAsmBlock;
+        const int r = e / ncol, cc = e % ncol;
+        double v = 0;
+        if (e < bsize) {
+            if (r == 0 && cc == 0)
+                v = B.v[n] * B.poroP[n] - B.vfP[n];
+            else if (r == 0)
+                v = -B.vfi[(cc - 1) * ld + n];
+            else if (r == cc)
+                v = 1;
+        }
+        acc[t] = v;
+    }
+    __syncthreads();
+
+    const int k0 = F.IA[n] + 1, k1 = F.IA[n + 1];
+    for (int k = k0; k < k1; ++k) {
+        const int64_t m = F.JA[k];
+        const int64_t c = F.slotConn[k];
+        const bool ownB = n < m;  // this row is the connection's bId side
+        const int64_t bId = ownB ? n : m;
+        const int64_t eId = ownB ? m : n;
+        const double Akd = kCONV1f * kCONV2f * F.area[c];
+        const double dGamma = kGravityFactor * (B.depth[bId] - B.depth[eId]);
+        bool exB[3], exE[3], sdO[3];
+        int pvB[3], pvE[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            exB[j] = B.phaseExist[j * ld + bId];
+            exE[j] = B.phaseExist[j * ld + eId];
+            sdO[j] = B.pSderExist[j * ld + n];
+            pvB[j] = min(max(B.pVnumCom[j * ld + bId], 0), NC);
+            pvE[j] = min(max(B.pVnumCom[j * ld + eId], 0), NC);
+        }
+        int nsat = 0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) nsat += sdO[j] ? 1 : 0;
+        const int* pvO = ownB ? pvB : pvE;
+        const int ncolO = min(nsat + pvO[0] + pvO[1] + pvO[2], D::maxRows);
+
+        // ---- own side's dFdXp column and dFdXs, entry by entry ----
+        for (int q = lane; q < nc * (ncolO + 1); q += kAsmBlock) {
+            const int i = q / (ncolO + 1);
+            const int col = q % (ncolO + 1) - 1;  // -1: pressure column of dFdXp
+            // decode the secondary column: saturation of phase j1S, or x of phase jX, comp kX
+            int j1S = -1, jX = -1, kX = -1;
+            if (col >= 0 && col < nsat) {
+                int s = 0;
+#pragma unroll
+                for (int j1 = 0; j1 < 3; ++j1)
+                    if (sdO[j1]) {
+                        if (s == col) j1S = j1;
+                        s++;
+                    }
+            } else if (col >= nsat) {
+                int start = nsat;
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    if (col >= start && col < start + pvO[j]) {
+                        jX = j;
+                        kX = col - start;
+                    }
+                    start += pvO[j];
+                }
+            }
+            double v = 0;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int64_t uId = F.upblock[c * 3 + j];
+                const bool exU = (uId == bId) ? exB[j] : exE[j];
+                if (!exU) continue;
+                const bool upB = (uId == bId);
+                const double dP = B.Pj[j * ld + bId] - B.Pj[j * ld + eId] - F.upRho[c * 3 + j] * dGamma;
+                const double xi = B.xi[j * ld + uId];
+                const double kr = B.kr[j * ld + uId];
+                const double mu = B.mu[j * ld + uId];
+                const double transJ = Akd * kr / mu;
+                const double xij = B.xij[(j * nc + i) * ld + uId];
+                const double transIJ = xij * xi * transJ;
+                if (col < 0) {
+                    // pressure column
+                    const double muP = B.muP[j * ld + uId];
+                    const double xiP = B.xiP[j * ld + uId];
+                    const double rhoP = B.rhoP[j * ld + uId];
+                    double tmp = xij * transJ * xiP * dP;
+                    tmp += -transIJ * muP / mu * dP;
+                    if (ownB) {
+                        v += transIJ;
+                        if (!exE[j]) {
+                            tmp += transIJ * (-rhoP * dGamma);
+                            v += tmp;
+                        } else if (!exB[j]) {
+                        } else {
+                            v += transIJ * (-B.rhoP[j * ld + bId] * dGamma) / 2;
+                            if (upB) v += tmp;
+                        }
+                    } else {
+                        v -= transIJ;
+                        if (!exE[j]) {
+                        } else if (!exB[j]) {
+                            tmp += transIJ * (-rhoP * dGamma);
+                            v += tmp;
+                        } else {
+                            v += transIJ * (-B.rhoP[j * ld + eId] * dGamma) / 2;
+                            if (!upB) v += tmp;
+                        }
+                    }
+                } else if (j1S >= 0) {
+                    // saturation column of phase j1S
+                    const double dpc = B.dPcj_dS[(j * 3 + j1S) * ld + n];
+                    if (ownB) {
+                        v += transIJ * dpc;
+                        if (upB) v += Akd * xij * xi / mu * B.dKr_dS[(j * 3 + j1S) * ld + uId] * dP;
+                    } else {
+                        v -= transIJ * dpc;
+                        if (!upB) v += Akd * xij * xi / mu * B.dKr_dS[(j * 3 + j1S) * ld + uId] * dP;
+                    }
+                } else if (jX == j) {
+                    // x_{kX, j} column (only phase j's own block is touched in pass j)
+                    const int kk = kX;
+                    const bool ownUp = ownB ? upB : !upB;
+                    const bool otherEx = ownB ? exE[j] : exB[j];
+                    if (ownUp) {
+                        const double rhox = otherEx ? B.rhox[(j * nc + kk) * ld + n] / 2
+                                                    : B.rhox[(j * nc + kk) * ld + uId];
+                        const double xix = B.xix[(j * nc + kk) * ld + uId];
+                        const double mux = B.mux[(j * nc + kk) * ld + uId];
+                        double tmp = -transIJ * rhox * dGamma;
+                        tmp += xij * transJ * xix * dP;
+                        tmp += -transIJ * mux / mu * dP;
+                        v += tmp;
+                        if (kk == i) v += xi * transJ * dP;
+                    } else if (otherEx) {
+                        v += -transIJ * B.rhox[(j * nc + kk) * ld + n] / 2 * dGamma;
+                    }
+                }
+            }
+            sDF[i * (D::maxRows + 1) + col + 1] = v;
+        }
+        __syncthreads();
+
+        // ---- block = (dFdXp + dFdXs * dSec_own) * dt, then scatter ----
+        const int64_t mirror = F.slotMirror[k];
+        double* dst = F.val + mirror * bsize;
+#pragma unroll
+        for (int t = 0; t < D::perLane; ++t) {
+            const int e = lane + t * kAsmBlock;
+            if (e < bsize) {
+                const int r = e / ncol, cc = e % ncol;
+                double v = 0;
+                if (r > 0) {
+                    const double* row = &sDF[(r - 1) * (D::maxRows + 1)];
+                    v = (cc == 0) ? row[0] : 0.0;
+                    for (int kq = 0; kq < ncolO; ++kq) v = v + row[1 + kq] * sSec[kq * ncol + cc];
+                }
+                v = dt * v;
+                if (ownB) {
+                    acc[t] += v;
+                    dst[e] = -1.0 * v;
+                } else {
+                    acc[t] += -1.0 * v;
+                    dst[e] = v;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    double* diag = F.val + (int64_t)F.IA[n] * bsize;
+#pragma unroll
+    for (int t = 0; t < D::perLane; ++t) {
+        const int e = lane + t * kAsmBlock;
+        if (e < bsize) diag[e] = acc[t];
+    }
+}
+
+template <int NC>
+static hipError_t launchCalResT(const DevBulk& B, FimDevice& F, double dt, hipStream_t s)
+{
+    const unsigned grid = (unsigned)((B.nb + kResBlock - 1) / kResBlock);
+    hipLaunchKernelGGL(calResKernel<NC>, dim3(grid), dim3(kResBlock), 0, s, B, F, dt);
+    return hipGetLastError();
+}
+
+template <int NC>
+static hipError_t launchAssembleT(const DevBulk& B, FimDevice& F, double dt, hipStream_t s)
+{
+    hipLaunchKernelGGL(assembleKernel<NC>, dim3((unsigned)B.nb), dim3(kAsmBlock), 0, s, B, F, dt);
+    return hipGetLastError();
+}
+
+#define OCPG_FIM_NC(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
+
+hipError_t launchCalRes(int NC, const DevBulk& B, FimDevice& F, double dt, hipStream_t s)
+{
+    if (B.nb == 0) return hipSuccess;
+#define X(N) \
+    if (NC == N) return launchCalResT<N>(B, F, dt, s);
+    OCPG_FIM_NC(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launchAssemble(int NC, const DevBulk& B, FimDevice& F, double dt, hipStream_t s)
+{
+    if (B.nb == 0) return hipSuccess;
+#define X(N) \
+    if (NC == N) return launchAssembleT<N>(B, F, dt, s);
+    OCPG_FIM_NC(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+void fimFree(FimDevice& f)
+{
+    void* ptrs[] = {f.bId, f.eId, f.area, f.IA, f.JA, f.slotConn, f.slotMirror, f.upblock,
+                    f.upRho, f.upVel, f.res, f.val, f.normPart, f.normIdx};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    f = FimDevice{};
+}
+
+size_t fimBytes(const FimDevice& f) { return f.bytes; }
+
+}  // namespace ocpg

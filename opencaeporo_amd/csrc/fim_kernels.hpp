@@ -1,0 +1,46 @@
+// FIM property, residual and Jacobian kernels (fim_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device_state.hpp"
+
+namespace ocpg {
+
+// Static connection / BSR structure on the device.
+struct FimDevice {
+    int64_t nb = 0;        // bulk rows
+    int64_t nconn = 0;
+    int64_t nnzb = 0;      // blocks in the BSR (bulk part)
+    int ncol = 0;
+    // connection list (BulkConn order)
+    int32_t* bId = nullptr;
+    int32_t* eId = nullptr;
+    double* area = nullptr;
+    // BSR pattern of AssembleMatBulksNew: row n = [n, neighbours ascending]
+    int32_t* IA = nullptr;      // nb + 1
+    int32_t* JA = nullptr;      // nnzb
+    int32_t* slotConn = nullptr;    // connection of each off-diagonal slot (-1 on the diagonal)
+    int32_t* slotMirror = nullptr;  // index of slot (JA[k], row) for slot k = (row, JA[k])
+    // per-connection upwinding written by CalRes and read by the assembly
+    int32_t* upblock = nullptr;     // nconn * np
+    double* upRho = nullptr;        // nconn * np
+    double* upVel = nullptr;        // nconn * np
+    // outputs
+    double* res = nullptr;          // nb * ncol, cell-major (LinearSystem rhs layout), negated
+    double* val = nullptr;          // nnzb * ncol * ncol, row-major blocks
+    double* normPart = nullptr;     // per-block partial maxima of CalRes norms
+    int64_t* normIdx = nullptr;
+    int normBlocks = 0;
+    size_t bytes = 0;
+};
+
+void fimFree(FimDevice& f);
+size_t fimBytes(const FimDevice& f);
+
+hipError_t launchKrPcRock(const DevBulk& B, const DevSat* sat, const DevRock& rock, bool krpc, bool rockOn,
+                          hipStream_t s);
+hipError_t launchCalRes(int NC, const DevBulk& B, FimDevice& F, double dt, hipStream_t s);
+hipError_t launchAssemble(int NC, const DevBulk& B, FimDevice& F, double dt, hipStream_t s);
+
+}  // namespace ocpg

@@ -1,0 +1,631 @@
+// Flash kernel templates: IsoT_FIM::CalFlash (FlashFIM + PassFlashValue per cell,
+// src/OCPFluidMethod.cpp:1108-1187) and IsoT_FIM::InitFlash (InitFlashFIM,
+// :1095-1106), one lane per cell over the SoA state of device_state.hpp.
+#pragma once
+
+#include "flash_device.hpp"
+#include "flash_kernel.hpp"
+
+namespace ocpg {
+
+template <int NC>
+struct FlashKernel {
+    static constexpr int nc = NC + 1;
+    static constexpr int ncol = nc + 1;
+
+    // Everything after PhaseEquilibrium: MixtureComp::CalFlash tail
+    // (src/MixtureComp.cpp:697-708), CalVfiVfp_full02 (:3911-3989), the
+    // water block (:521-546 / :321-345), CalSaturation (:1178-1186),
+    // CaldXsdXpAPI02p (:4386-4493), CalXiPNX_partial / CalRhoPX_partial /
+    // CalMuPXLBC_partial (:3037-3213), pSderExist/pVnumCom (:559-570),
+    // CalSkipForNextStep (:5155-5254), then PassFlashValue's stores.
+    __device__ static void finish(FlashCell<NC>& F, const DevEos& E, const DevBulk& B, int64_t cell,
+                                  bool init, double Sw_in, double Vpore)
+    {
+        const int64_t ld = B.ld;
+        const int NP = F.NP;
+        const double P = F.P, T = F.T;
+        // ---- CalMW, CalVfXiRho, CalViscoLBC (per internal phase) ----
+        double MW[2], vC[2], xiC[2], rhoC[2], muC[2], muA[2][5];
+        double vf = 0;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            if (j >= NP) break;
+            MW[j] = 0;
+#pragma unroll
+            for (int i = 0; i < NC; ++i) MW[j] += F.x[j][i] * E.MWC[i];
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            if (j >= NP) break;
+            double tmp = F.Zj[j] * kGasConst * T / P;
+#pragma unroll
+            for (int i = 0; i < NC; ++i) tmp -= F.x[j][i] * E.Vshift[i];
+            vC[j] = tmp * F.nu[j];
+            vf += vC[j];
+            xiC[j] = 1 / tmp;
+            rhoC[j] = MW[j] * xiC[j];
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            if (j >= NP) break;
+            double m0 = 0, m1 = 0, xT = 0, xP = 0, xV = 0;
+            const double sMW = sqrt(MW[j]);
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                double tmp = 5.4402 * pow(E.Tc[i], 1.0 / 6) / sMW / pow(E.Pc[i], 2.0 / 3);
+                const double Tri = T / E.Tc[i];
+                if (Tri <= 1.5)
+                    tmp = 34 * 1E-5 * pow(Tri, 0.94) / tmp;
+                else
+                    tmp = 17.78 * 1E-5 * pow((4.58 * Tri - 1.67), 0.625) / tmp;
+                m0 += F.x[j][i] * E.sqrtMWi[i] * tmp;
+                m1 += F.x[j][i] * E.sqrtMWi[i];
+                xT += F.x[j][i] * E.Tc[i];
+                xP += F.x[j][i] * E.Pc[i];
+                xV += F.x[j][i] * E.Vcvis[i];
+            }
+            muA[j][0] = m0;
+            muA[j][1] = m1;
+            muA[j][2] = 5.4402 * pow(xT, 1.0 / 6) / sMW / pow(xP, 2.0 / 3);
+            muA[j][3] = xiC[j] * xV;
+            const double a3 = muA[j][3];
+            const double* L = E.LBCcoef;
+            muA[j][4] = a3 * (a3 * (a3 * (L[4] * a3 + L[3]) + L[2]) + L[1]) + L[0];
+            muC[j] = muA[j][0] / muA[j][1] + 1E-4 * (pow(muA[j][4], 4.0) - 1) / muA[j][2];
+        }
+        // ---- IdentifyPhase (src/MixtureComp.cpp:2352-2384) ----
+        int lab[2];
+        if (NP == 1) {
+            double A = 0, Bv = 0;
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                A += F.x[0][i] * E.Vc[i] * E.Tc[i];
+                Bv += F.x[0][i] * E.Vc[i];
+            }
+            lab[0] = (T > A / Bv) ? 1 : 0;
+            lab[1] = 1 - lab[0];
+        } else {
+            lab[0] = (MW[0] > MW[1]) ? 0 : 1;
+            lab[1] = 1 - lab[0];
+        }
+        // ---- InitFlashFIM: scale to moles from the pore volume ----
+        double Nh = F.Nh;
+        if (init) {
+            Nh = Vpore * (1 - Sw_in) / vf;
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                if (j < NP) F.nu[j] = Nh * F.nu[j];
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                if (j < NP) vC[j] = Nh * vC[j];
+            vf *= Nh;
+            F.Nh = Nh;
+#pragma unroll
+            for (int i = 0; i < NC; ++i) F.Ni[i] = F.zi[i] * Nh;
+        }
+        // ---- CalVfiVfp_full02 ----
+        const double CgTP = kGasConst * T / P;
+        double vfi[nc], vfP;
+        double vjp[2], vji[2][NC];  // internal phase order
+        double dn[NC * (NC + 1)];   // rhsDer after the solve (column-major)
+        double Zp[2];
+        if (NP == 1) {
+            const double aj = F.Aj[0], bj = F.Bj[0], zj = F.Zj[0];
+            const double den = dZden(aj, bj, zj);
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                const double tmp = F.aSum1(i, F.x[0]);
+                const double An = 2 / F.nu[0] * (tmp - aj);
+                const double Bn = 1 / F.nu[0] * (F.Bi[i] - bj);
+                const double Zn = dZnum(aj, bj, zj, An, Bn) / den;
+                vfi[i] = CgTP * (zj + F.nu[0] * Zn) - E.Vshift[i];
+            }
+            Zp[0] = dZnum(aj, bj, zj, aj, bj) / P / den;
+            vfP = CgTP * F.nu[0] * (Zp[0] - zj / P);
+        } else {
+            double Zn[2][NC];
+            double A[NC * NC];
+            {
+                double f1[FlashCell<NC>::NT];
+                F.calFugN(0, f1, Zn[0]);
+#pragma unroll
+                for (int i = 0; i < NC; ++i)
+#pragma unroll
+                    for (int m = 0; m <= i; ++m) A[i * NC + m] = f1[FlashCell<NC>::tri(i, m)];
+                F.calFugN(1, f1, Zn[1]);
+#pragma unroll
+                for (int i = 0; i < NC; ++i)
+#pragma unroll
+                    for (int m = 0; m <= i; ++m) {
+                        const double v1 = f1[FlashCell<NC>::tri(i, m)];
+                        const double s = A[i * NC + m] + v1;
+                        A[i * NC + m] = s;
+                        A[m * NC + i] = s;
+                        // rhs column k+1 = fugN[1] row k: entries i of row m and m of row i
+                        dn[(m + 1) * NC + i] = v1;
+                        dn[(i + 1) * NC + m] = v1;
+                    }
+            }
+            // CalFugPAll (src/MixtureComp.cpp:2519-2579)
+            double fugP[2][NC];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const double aj = F.Aj[j], bj = F.Bj[j], zj = F.Zj[j];
+                const double Ap = aj / P;
+                const double Bp = bj / P;
+                Zp[j] = dZnum(aj, bj, zj, Ap, Bp) / dZden(aj, bj, zj);
+                const double G = (zj + kDelta1 * bj) / (zj + kDelta2 * bj);
+                const double Gp = (kDelta1 - kDelta2) / ((zj + kDelta2 * bj) * (zj + kDelta2 * bj)) *
+                                  (Bp * zj - Zp[j] * bj);
+#pragma unroll
+                for (int i = 0; i < NC; ++i) {
+                    const double C = P / (zj - bj);
+                    const double tmp = F.aSum1(i, F.x[j]);
+                    const double Ei = -aj / ((kDelta1 - kDelta2) * bj) * (2 * tmp / aj - F.Bi[i] / bj);
+                    const double Cp = ((zj - bj) - P * (Zp[j] - Bp)) / ((zj - bj) * (zj - bj));
+                    const double Dp = F.Bi[i] / bj * Zp[j];
+                    fugP[j][i] = Cp / C + Dp + Ei / G * Gp;
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < NC; ++i) dn[i] = fugP[1][i] - fugP[0][i];
+            luSolve<NC, NC + 1>(A, dn);
+            vjp[0] = CgTP * F.nu[0] * (Zp[0] - F.Zj[0] / P);
+            vjp[1] = CgTP * F.nu[1] * (Zp[1] - F.Zj[1] / P);
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                vjp[0] += (CgTP * (F.Zj[0] + F.nu[0] * Zn[0][k]) - E.Vshift[k]) * dn[k];
+                vjp[1] -= (CgTP * (F.Zj[1] + F.nu[1] * Zn[1][k]) - E.Vshift[k]) * dn[k];
+            }
+            vfP = vjp[0] + vjp[1];
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                const double* d = dn + (i + 1) * NC;
+                double a0 = 0, a1 = 0;
+#pragma unroll
+                for (int k = 0; k < NC; ++k) {
+                    a0 += (CgTP * (F.Zj[0] + F.nu[0] * Zn[0][k]) - E.Vshift[k]) * d[k];
+                    a1 += (CgTP * (F.Zj[1] + F.nu[1] * Zn[1][k]) - E.Vshift[k]) * ((i == k ? 1.0 : 0.0) - d[k]);
+                }
+                vji[0][i] = a0;
+                vji[1][i] = a1;
+                vfi[i] = a0 + a1;
+            }
+        }
+        // ---- water (PVTW, OCPTable::Eval_All on column 0) ----
+        double pw[5], cw[5];
+        {
+            const int nRow = E.pvtwRows;
+            int seg = -1;
+            for (int r = 0; r + 1 < nRow; ++r)
+                if (P >= E.pvtw[r][0] && P < E.pvtw[r + 1][0]) {
+                    seg = r;
+                    break;
+                }
+            if (seg < 0) {
+                const int r = (P < E.pvtw[0][0]) ? 0 : nRow - 1;
+#pragma unroll
+                for (int q = 0; q < 5; ++q) {
+                    cw[q] = 0;
+                    pw[q] = E.pvtw[r][q];
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < 5; ++q) {
+                    cw[q] = (E.pvtw[seg + 1][q] - E.pvtw[seg][q]) / (E.pvtw[seg + 1][0] - E.pvtw[seg][0]);
+                    pw[q] = E.pvtw[seg][q] + cw[q] * (P - E.pvtw[seg][0]);
+                }
+            }
+        }
+        const double bw = pw[1] * (1 - pw[2] * (P - pw[0]));
+        const double bwp = -pw[2] * pw[1];
+        const double muW = pw[3];
+        const double xiW = 1 / (kCONV1 * bw);
+        const double rhoW = E.stdRhoW / bw;
+        const double muPW = cw[3];
+        const double xiPW = -bwp / (bw * bw * kCONV1);
+        const double rhoPW = kCONV1 * xiPW * E.stdRhoW;
+        if (init) F.Ni[NC] = Vpore * Sw_in * xiW;
+        const double NiW = F.Ni[NC];
+        const double Nt = Nh + NiW;
+        const double vw = kCONV1 * NiW * bw;
+        vfi[NC] = kCONV1 * bw;
+        const double vwp = kCONV1 * NiW * bwp;
+        vf += vw;
+        vfP += vwp;
+        // ---- CalSaturation ----
+        double S[3] = {0, 0, 0};
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            if (j < NP) S[lab[j]] = vC[j] / vf;
+        S[2] = vw / vf;
+
+        // ---- stores shared by every cell (PassFlashValue) ----
+        const int64_t n = cell;
+        B.phaseNum[n] = NP + 1;
+        B.Nt[n] = Nt;
+        B.vf[n] = vf;
+        B.vfP[n] = vfP;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) B.S[j * ld + n] = S[j];
+#pragma unroll
+        for (int i = 0; i < nc; ++i) B.vfi[i * ld + n] = vfi[i];
+        if (init) {
+#pragma unroll
+            for (int i = 0; i < nc; ++i) B.Ni[i * ld + n] = F.Ni[i];
+        }
+        const bool exist0 = (NP == 2) || lab[0] == 0;
+        const bool exist1 = (NP == 2) || lab[0] == 1;
+        B.phaseExist[0 * ld + n] = exist0;
+        B.phaseExist[1 * ld + n] = exist1;
+        B.phaseExist[2 * ld + n] = 1;
+        B.pSderExist[0 * ld + n] = exist0;
+        B.pSderExist[1 * ld + n] = exist1;
+        B.pSderExist[2 * ld + n] = 1;
+        B.pVnumCom[0 * ld + n] = exist0 ? NC : 0;
+        B.pVnumCom[1 * ld + n] = exist1 ? NC : 0;
+        B.pVnumCom[2 * ld + n] = 0;
+        B.bRowSizedSdP[n] = (NP == 2) ? 3 + 2 * NC : 2 + NC;
+        // water phase slot
+        B.rho[2 * ld + n] = rhoW;
+        B.xi[2 * ld + n] = xiW;
+        B.mu[2 * ld + n] = muW;
+        B.rhoP[2 * ld + n] = rhoPW;
+        B.xiP[2 * ld + n] = xiPW;
+        B.muP[2 * ld + n] = muPW;
+#pragma unroll
+        for (int i = 0; i < nc; ++i) {
+            B.xij[(2 * nc + i) * ld + n] = (i == NC) ? 1.0 : 0.0;
+            B.rhox[(2 * nc + i) * ld + n] = 0.0;
+            B.xix[(2 * nc + i) * ld + n] = 0.0;
+            B.mux[(2 * nc + i) * ld + n] = 0.0;
+        }
+
+        // ---- CalXiPNX_partial, CalRhoPX_partial, CalMuPXLBC_partial, stores ----
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            if (j >= NP) break;
+            const int s = lab[j];
+            const double aj = F.Aj[j], bj = F.Bj[j], zj = F.Zj[j];
+            const double den = dZden(aj, bj, zj);
+            const double mxi = -xiC[j] * xiC[j];
+            double xix[NC];
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                const double Ax = 2 * F.aSumX(i, F.x[j]);
+                const double Zx = dZnum(aj, bj, zj, Ax, F.Bi[i]) / den;
+                xix[i] = mxi * (CgTP * Zx - E.Vshift[i]);
+            }
+            const double xiP = mxi * CgTP * (Zp[j] - zj / P);
+            const double rhoP = xiP * MW[j];
+            // LBC derivatives
+            double xTj = 0, xPj = 0, xVj = 0;
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                xTj += F.x[j][i] * E.Tc[i];
+                xPj += F.x[j][i] * E.Pc[i];
+                xVj += F.x[j][i] * E.Vcvis[i];
+            }
+            const double* L = E.LBCcoef;
+            const double a3 = muA[j][3];
+            const double poly = (L[1] + a3 * (2 * L[2] + a3 * (3 * L[3] + a3 * 4 * L[4])));
+            const double der7P = xVj * xiP;
+            const double der8P = der7P * poly;
+            const double a4c = pow(muA[j][4], 3.0);
+            const double muP = (4 * 1E-4) * a4c * der8P / muA[j][2];
+            const double sMW = sqrt(MW[j]);
+            const double pMW15 = pow(MW[j], -1.5);
+            const double a4q = pow(muA[j][4], 4.0);
+            B.rho[s * ld + n] = rhoC[j];
+            B.xi[s * ld + n] = xiC[j];
+            B.mu[s * ld + n] = muC[j];
+            B.rhoP[s * ld + n] = rhoP;
+            B.xiP[s * ld + n] = xiP;
+            B.muP[s * ld + n] = muP;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                // der1IJ of the reference is -(1 / 2) * ... = 0 (integer division)
+                double der3J = 0;
+#pragma unroll
+                for (int i = 0; i < NC; ++i) {
+                    const double val1IJ = E.muAux1I[i] / sMW;
+                    const double der1IJ = 0 * E.muAux1I[i] * pMW15 * E.MWC[k];
+                    const double Tri = T / E.Tc[i];
+                    double tmp;
+                    if (Tri <= 1.5)
+                        tmp = 34 * 1E-5 * pow(Tri, 0.94);
+                    else
+                        tmp = 17.78 * 1E-5 * pow(4.58 * Tri - 1.67, 0.625);
+                    const double val2IJ = tmp / val1IJ;
+                    const double der2IJ = -tmp * der1IJ / (val1IJ * val1IJ);
+                    der3J += F.x[j][i] * E.sqrtMWi[i] * der2IJ + (i == k ? 1.0 : 0.0) * E.sqrtMWi[k] * val2IJ;
+                }
+                const double der4J = E.sqrtMWi[k];
+                const double der6J = 5.4402 *
+                                     (1.0 / 6 * pow(xTj, -5.0 / 6) * E.Tc[k] -
+                                      pow(xTj, 1.0 / 6) * (0.5 / MW[j] * E.MWC[k] + 2.0 / 3 / xPj * E.Pc[k])) /
+                                     (sMW * pow(xPj, 2.0 / 3));
+                const double der7J = xix[k] * xVj + xiC[j] * E.Vcvis[k];
+                const double der8J = der7J * poly;
+                const double mux = (der3J * muA[j][1] - muA[j][0] * der4J) / (muA[j][1] * muA[j][1]) +
+                                   1E-4 * (4 * a4c * der8J * muA[j][2] - (a4q - 1) * der6J) /
+                                       (muA[j][2] * muA[j][2]);
+                const double rhox = xix[k] * MW[j] + xiC[j] * E.MWC[k];
+                B.xij[(s * nc + k) * ld + n] = F.x[j][k];
+                B.rhox[(s * nc + k) * ld + n] = rhox;
+                B.xix[(s * nc + k) * ld + n] = xix[k];
+                B.mux[(s * nc + k) * ld + n] = mux;
+            }
+            B.xij[(s * nc + NC) * ld + n] = 0.0;
+            B.rhox[(s * nc + NC) * ld + n] = 0.0;
+            B.xix[(s * nc + NC) * ld + n] = 0.0;
+            B.mux[(s * nc + NC) * ld + n] = 0.0;
+        }
+
+        // ---- CaldXsdXpAPI02p: compact d(S, x)/d(P, N) rows ----
+        double* D = B.dSec;
+        const double vf2 = vf * vf;
+        if (NP == 1) {
+            const double vjh = vC[0];
+            double r0[ncol];
+            r0[0] = ((vfP - vwp) * vf - vfP * vjh) / vf2;
+#pragma unroll
+            for (int m = 0; m < NC; ++m) r0[1 + m] = vfi[m] * vw / vf2;
+            r0[1 + NC] = -vfi[NC] * vjh / vf2;
+#pragma unroll
+            for (int c = 0; c < ncol; ++c) {
+                D[(0 * ncol + c) * ld + n] = r0[c];
+                D[(1 * ncol + c) * ld + n] = -r0[c];
+            }
+            const double Nh2 = Nh * Nh;
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                D[((2 + i) * ncol + 0) * ld + n] = 0.0;
+#pragma unroll
+                for (int m = 0; m < NC; ++m)
+                    D[((2 + i) * ncol + 1 + m) * ld + n] = ((i == m ? 1.0 : 0.0) * Nh - F.Ni[i]) / Nh2;
+                D[((2 + i) * ncol + 1 + NC) * ld + n] = 0.0;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int row = lab[j];
+                D[(row * ncol + 0) * ld + n] = (vjp[j] * vf - vfP * vC[j]) / vf2;
+#pragma unroll
+                for (int m = 0; m < NC; ++m)
+                    D[(row * ncol + 1 + m) * ld + n] = (vji[j][m] * vf - vfi[m] * vC[j]) / vf2;
+                D[(row * ncol + 1 + NC) * ld + n] = (0.0 * vf - vfi[NC] * vC[j]) / vf2;
+            }
+            D[(2 * ncol + 0) * ld + n] = (vwp * vf - vfP * vw) / vf2;
+#pragma unroll
+            for (int m = 0; m < NC; ++m) D[(2 * ncol + 1 + m) * ld + n] = (0.0 * vf - vfi[m] * vw) / vf2;
+            D[(2 * ncol + 1 + NC) * ld + n] = (vfi[NC] * vf - vfi[NC] * vw) / vf2;
+            const int base0 = 3 + lab[0] * NC;
+            const int base1 = 3 + lab[1] * NC;
+            const double nu0s = F.nu[0] * F.nu[0];
+            const double nu1s = F.nu[1] * F.nu[1];
+#pragma unroll
+            for (int c = 0; c <= NC; ++c) {
+                const double* d = dn + c * NC;
+                double sum = 0;
+#pragma unroll
+                for (int i = 0; i < NC; ++i) sum += d[i];
+#pragma unroll
+                for (int i = 0; i < NC; ++i) {
+                    const double v0 = (d[i] * F.nu[0] - sum * Nh * F.n[0][i]) / nu0s;
+                    double v1;
+                    if (c == 0)
+                        v1 = (-d[i] * F.nu[1] + sum * Nh * F.n[1][i]) / nu1s;
+                    else
+                        v1 = (((i == c - 1) ? 1.0 : 0.0) - d[i]) * F.nu[1] - (1 - sum) * Nh * F.n[1][i];
+                    if (c != 0) v1 = v1 / nu1s;
+                    D[((base0 + i) * ncol + c) * ld + n] = v0;
+                    D[((base1 + i) * ncol + c) * ld + n] = v1;
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                D[((base0 + i) * ncol + 1 + NC) * ld + n] = 0.0;
+                D[((base1 + i) * ncol + 1 + NC) * ld + n] = 0.0;
+            }
+        }
+
+        // ---- CalSkipForNextStep ----
+        if (F.flagSkip && F.ftype == 0) {
+            // CalPhiNSkip (src/MixtureComp.cpp:5155-5217) + AssembleSkipMatSTA
+            const double aj = F.Aj[0], bj = F.Bj[0], zj = F.Zj[0], nu0 = F.nu[0];
+            const double* xj = F.x[0];
+            double An[NC], Bn[NC], Zn[NC], tI[NC];
+            const double den = dZden(aj, bj, zj);
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                tI[i] = F.aSum1(i, xj);
+                An[i] = 2 / nu0 * (tI[i] - aj);
+                Bn[i] = 1 / nu0 * (F.Bi[i] - bj);
+                Zn[i] = dZnum(aj, bj, zj, An[i], Bn[i]) / den;
+            }
+            const double G = (zj + kDelta1 * bj) / (zj + kDelta2 * bj);
+            const double lG = log(G);
+            float M[NC * NC];
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                const double C = 1 / (zj - bj);
+                const double tmp = tI[i];
+                const double Ei = -aj / ((kDelta1 - kDelta2) * bj) * (2 * tmp / aj - F.Bi[i] / bj);
+#pragma unroll
+                for (int k = 0; k <= i; ++k) {
+                    const double aik = F.omb(i, k) * F.sq(i, k);
+                    const double Cnk = (Bn[k] - Zn[k]) / ((zj - bj) * (zj - bj));
+                    const double Dnk = F.Bi[i] / bj * (Zn[k] - (F.Bi[k] - bj) * (zj - 1) / (nu0 * bj));
+                    const double Gnk = (kDelta1 - kDelta2) / ((zj + kDelta2 * bj) * (zj + kDelta2 * bj)) *
+                                       (Bn[k] * zj - Zn[k] * bj);
+                    double Enk = -1 / (kDelta1 - kDelta2) / (bj * bj) *
+                                 (2 * (bj * aik / nu0 + Bn[k] * (F.Bi[i] * aj / bj - tmp)) - An[k] * F.Bi[i] -
+                                  aj * F.Bi[i] / nu0);
+                    Enk -= Ei / nu0;
+                    const double phiN = 1 / C * Cnk + Dnk + Enk * lG + Ei / G * Gnk;
+                    const float v = static_cast<float>((i == k ? 1.0 : 0.0) + nu0 * sqrt(xj[i] * xj[k]) * phiN);
+                    M[i * NC + k] = v;
+                    M[k * NC + i] = v;
+                }
+            }
+            const float lam = minEigenSym<NC>(M);
+            B.skipMinEig[n] = static_cast<double>(lam);
+            B.skipP[n] = P;
+            B.skipT[n] = T;
+#pragma unroll
+            for (int i = 0; i < NC; ++i) B.skipZi[i * ld + n] = F.zi[i];
+        }
+        B.skipFlag[n] = F.flagSkip ? 1 : 0;
+    }
+};
+
+// SkipStaAnaly::IfSkip + CalFtypeFIM, src/AcceleratePVT.cpp:54-120.
+template <int NC>
+__device__ __forceinline__ int calFtypeFIM(const FlashCell<NC>& F, const DevBulk& B, int64_t n, const double* S)
+{
+    const int64_t ld = B.ld;
+    if (B.skipFlag[n]) {
+        const double lam = B.skipMinEig[n];
+        bool skip = true;
+        if (fabs(1 - B.skipP[n] / F.P) >= lam / 10) skip = false;
+        else if (fabs(B.skipT[n] - F.T) >= lam * 10) skip = false;
+        else {
+#pragma unroll
+            for (int i = 0; i < NC; ++i)
+                if (fabs(F.Ni[i] / F.Nh - B.skipZi[i * ld + n]) >= lam / 10) skip = false;
+        }
+        if (skip) return 1;
+    }
+    if (F.lNP >= 2) {
+        if (S[0] < 1E-4 || S[1] < 1E-4) return 0;
+        return 2;
+    }
+    return 0;
+}
+
+__device__ __forceinline__ void accumulateCounters(DevFlashCounters* ctr, bool active, int ftype, int NP,
+                                                   const int* it, bool nonconv)
+{
+    if (!ctr) return;
+    // one atomic per wavefront: reduce across lanes with shuffles
+    unsigned long long v[11];
+    v[0] = active ? 1 : 0;
+    v[1] = (active && ftype == 0) ? 1 : 0;
+    v[2] = (active && ftype == 1) ? 1 : 0;
+    v[3] = (active && ftype == 2) ? 1 : 0;
+    v[4] = (active && NP == 2) ? 1 : 0;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) v[5 + q] = active ? (unsigned long long)it[q] : 0ull;
+    v[10] = (active && nonconv) ? 1 : 0;
+#pragma unroll
+    for (int q = 0; q < 11; ++q) {
+        unsigned long long s = v[q];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        v[q] = s;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        unsigned long long* dst = &ctr->cells;
+        atomicAdd(&dst[0], v[0]);
+        atomicAdd(&dst[1], v[1]);
+        atomicAdd(&dst[2], v[2]);
+        atomicAdd(&dst[3], v[3]);
+        atomicAdd(&dst[4], v[4]);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) atomicAdd(&dst[5 + q], v[5 + q]);
+        atomicAdd(&dst[10], v[10]);
+    }
+}
+
+template <int NC>
+__global__ void __launch_bounds__(kFlashBlock) flashFimKernel(const DevEos* __restrict__ eos, DevBulk B,
+                                                              DevFlashCounters* ctr)
+{
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = n < B.nb;
+    FlashCell<NC> F(*eos);
+    bool nonconv = false;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) F.it[q] = 0;
+    F.ftype = 0;
+    F.NP = 1;
+    if (active) {
+        const int64_t ld = B.ld;
+        constexpr int nc = NC + 1;
+        const int lastNP = B.phaseNum[n];
+        F.lNP = lastNP > 0 ? lastNP - 1 : 0;
+        if (F.lNP == 2) {
+#pragma unroll
+            for (int i = 0; i < NC; ++i) F.lKs[i] = B.xij[(0 * nc + i) * ld + n] / B.xij[(1 * nc + i) * ld + n];
+        }
+        F.P = B.P[n];
+        F.T = B.T[n] + kCONV5;
+#pragma unroll
+        for (int i = 0; i < nc; ++i) F.Ni[i] = B.Ni[i * ld + n];
+        F.Nh = 0;
+#pragma unroll
+        for (int i = 0; i < NC; ++i) F.Nh += fabs(F.Ni[i]);
+#pragma unroll
+        for (int i = 0; i < NC; ++i) F.zi[i] = F.Ni[i] / F.Nh;
+        double S[2] = {B.S[0 * ld + n], B.S[1 * ld + n]};
+        F.ftype = calFtypeFIM<NC>(F, B, n, S);
+        nonconv = F.phaseEquilibrium();
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            if (j < F.NP) F.nu[j] = F.Nh * F.nu[j];
+        FlashKernel<NC>::finish(F, *eos, B, n, false, 0.0, 0.0);
+        if (B.ftype) B.ftype[n] = F.ftype;
+        if (B.iters) {
+#pragma unroll
+            for (int q = 0; q < 5; ++q) B.iters[q * ld + n] = F.it[q];
+        }
+    }
+    accumulateCounters(ctr, active, F.ftype, F.NP, F.it, nonconv);
+}
+
+template <int NC>
+__global__ void __launch_bounds__(kFlashBlock) initFlashKernel(const DevEos* __restrict__ eos, DevBulk B,
+                                                               const double* __restrict__ zin, int64_t zld)
+{
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= B.nb) return;
+    const int64_t ld = B.ld;
+    FlashCell<NC> F(*eos);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) F.it[q] = 0;
+    F.ftype = 0;
+    F.lNP = 0;
+    F.P = B.P[n];
+    F.T = B.T[n] + kCONV5;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) F.zi[i] = zin[i * zld + n];
+    F.phaseEquilibrium();
+    FlashKernel<NC>::finish(F, *eos, B, n, true, B.S[2 * ld + n], B.rockVp[n]);
+    if (B.ftype) B.ftype[n] = F.ftype;
+    if (B.iters) {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) B.iters[q * ld + n] = F.it[q];
+    }
+}
+
+template <int NC>
+hipError_t launchFlashT(const DevEos* eos, const DevBulk& B, DevFlashCounters* ctr, hipStream_t s)
+{
+    if (B.nb == 0) return hipSuccess;
+    const unsigned grid = (unsigned)((B.nb + kFlashBlock - 1) / kFlashBlock);
+    hipLaunchKernelGGL(flashFimKernel<NC>, dim3(grid), dim3(kFlashBlock), 0, s, eos, B, ctr);
+    return hipGetLastError();
+}
+
+template <int NC>
+hipError_t launchInitFlashT(const DevEos* eos, const DevBulk& B, const double* z, int64_t zld, hipStream_t s)
+{
+    if (B.nb == 0) return hipSuccess;
+    const unsigned grid = (unsigned)((B.nb + kFlashBlock - 1) / kFlashBlock);
+    hipLaunchKernelGGL(initFlashKernel<NC>, dim3(grid), dim3(kFlashBlock), 0, s, eos, B, z, zld);
+    return hipGetLastError();
+}
+
+}  // namespace ocpg

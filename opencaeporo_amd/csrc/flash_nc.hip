@@ -1,0 +1,13 @@
+// One instantiation of the flash kernels; compiled once per supported
+// component count with -DOCPG_NC=<NC> so the instantiations build in parallel.
+#include "flash_impl.hpp"
+
+#ifndef OCPG_NC
+#error "compile with -DOCPG_NC=<number of hydrocarbon components>"
+#endif
+
+namespace ocpg {
+template hipError_t launchFlashT<OCPG_NC>(const DevEos*, const DevBulk&, DevFlashCounters*, hipStream_t);
+template hipError_t launchInitFlashT<OCPG_NC>(const DevEos*, const DevBulk&, const double*, int64_t,
+                                               hipStream_t);
+}  // namespace ocpg

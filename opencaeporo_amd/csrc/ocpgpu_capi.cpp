@@ -1,0 +1,638 @@
+// Host side of the C-ABI declared in include/ocpgpu.h: context, device
+// memory layout, transfers in the reference's Bulk layout, and the calls
+// that stand in for IsoT_FIM's property pass and assembly.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <utility>
+#include <vector>
+
+#include "../../include/ocpgpu.h"
+#include "device_state.hpp"
+#include "fim_kernels.hpp"
+#include "flash_kernel.hpp"
+
+using namespace ocpg;
+
+namespace {
+
+// One SoA field: where its host array lives in ocpg_bulk, its element size
+// and how many entries a cell holds.
+enum Cnt { C1, CNC, CNCh, CNP, CNPNC, CNPNP, CLEN, C5 };
+
+struct FieldDesc {
+    size_t hostOff;   // offset of the pointer inside ocpg_bulk
+    size_t devOff;    // offset of the pointer inside DevBulk
+    int elem;         // bytes per entry
+    Cnt cnt;
+    bool lastStep;    // part of the last-time-step copy
+};
+
+#define HB(f) offsetof(ocpg_bulk, f)
+#define DB(f) offsetof(DevBulk, f)
+
+// The fields IsoT_FIM::UpdateLastTimeStep copies (src/OCPFluidMethod.cpp:
+// 2915-2957) come first so that they form one contiguous block; SkipStaAnaly
+// keeps its own last-step copy (src/AcceleratePVT.cpp:122-138), included.
+const FieldDesc kFields[] = {
+    {HB(poro), DB(poro), 8, C1, true},          {HB(poroP), DB(poroP), 8, C1, true},
+    {HB(rockVp), DB(rockVp), 8, C1, true},      {HB(phaseNum), DB(phaseNum), 4, C1, true},
+    {HB(Nt), DB(Nt), 8, C1, true},              {HB(Ni), DB(Ni), 8, CNC, true},
+    {HB(vf), DB(vf), 8, C1, true},              {HB(P), DB(P), 8, C1, true},
+    {HB(Pj), DB(Pj), 8, CNP, true},             {HB(Pc), DB(Pc), 8, CNP, true},
+    {HB(phaseExist), DB(phaseExist), 1, CNP, true}, {HB(S), DB(S), 8, CNP, true},
+    {HB(xij), DB(xij), 8, CNPNC, true},         {HB(rho), DB(rho), 8, CNP, true},
+    {HB(xi), DB(xi), 8, CNP, true},             {HB(mu), DB(mu), 8, CNP, true},
+    {HB(kr), DB(kr), 8, CNP, true},             {HB(vfP), DB(vfP), 8, C1, true},
+    {HB(vfi), DB(vfi), 8, CNC, true},           {HB(rhoP), DB(rhoP), 8, CNP, true},
+    {HB(rhox), DB(rhox), 8, CNPNC, true},       {HB(xiP), DB(xiP), 8, CNP, true},
+    {HB(xix), DB(xix), 8, CNPNC, true},         {HB(muP), DB(muP), 8, CNP, true},
+    {HB(mux), DB(mux), 8, CNPNC, true},         {HB(dPcj_dS), DB(dPcj_dS), 8, CNPNP, true},
+    {HB(dKr_dS), DB(dKr_dS), 8, CNPNP, true},   {HB(bRowSizedSdP), DB(bRowSizedSdP), 4, C1, true},
+    {HB(dSec_dPri), DB(dSec), 8, CLEN, true},   {HB(pSderExist), DB(pSderExist), 1, CNP, true},
+    {HB(pVnumCom), DB(pVnumCom), 4, CNP, true}, {HB(skipFlag), DB(skipFlag), 1, C1, true},
+    {HB(skipMinEig), DB(skipMinEig), 8, C1, true}, {HB(skipP), DB(skipP), 8, C1, true},
+    {HB(skipT), DB(skipT), 8, C1, true},        {HB(skipZi), DB(skipZi), 8, CNCh, true},
+    // not copied at the end of a time step
+    {HB(T), DB(T), 8, C1, false},               {HB(v), DB(v), 8, C1, false},
+    {HB(depth), DB(depth), 8, C1, false},       {HB(poroInit), DB(poroInit), 8, C1, false},
+    {HB(ftype), DB(ftype), 4, C1, false},       {HB(iters), DB(iters), 4, C5, false},
+};
+constexpr int kNumFields = sizeof(kFields) / sizeof(kFields[0]);
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// Timer slots of ocpg_timing_get.
+enum TimerSlot { T_FLASH = 0, T_KRPC, T_RES, T_ASM, T_INIT, T_RESTORE, kTimerSlots };
+const char* kTimerNames[kTimerSlots] = {"flash", "krpc_rock", "calres", "assemble", "init_flash", "restore"};
+
+// Fields the flash reads (and overwrites): what one Newton iteration starts from.
+bool isFlashInput(size_t hostOff)
+{
+    return hostOff == HB(P) || hostOff == HB(Ni) || hostOff == HB(S) || hostOff == HB(phaseNum) ||
+           hostOff == HB(xij) || hostOff == HB(skipFlag) || hostOff == HB(skipMinEig) ||
+           hostOff == HB(skipP) || hostOff == HB(skipT) || hostOff == HB(skipZi);
+}
+
+}  // namespace
+
+struct ocpg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int NC = 0;
+    ocpg_eos eosHost{};
+    DevEos* dEos = nullptr;
+    DevSat satHost{};
+    DevSat* dSat = nullptr;
+    DevRock rock{};
+    DevFlashCounters* dCtr = nullptr;
+    // cells
+    DevBulk B{};
+    char* cur = nullptr;
+    char* last = nullptr;
+    size_t lastBytes = 0;  // bytes of the last-step block at the front of `cur`
+    size_t curBytes = 0;
+    size_t fieldOff[kNumFields] = {};
+    bool lastValid = false;
+    // connections and BSR (fim_kernels)
+    FimDevice fim{};
+    // copy of the flash inputs (ocpg_save_flash_inputs)
+    char* snap = nullptr;
+    size_t snapBytes = 0;
+    // per-kernel timers (ocpg_timing_*)
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events[kTimerSlots];
+    hipEvent_t open[kTimerSlots] = {};
+};
+
+static int hipStatus(hipError_t e)
+{
+    if (e == hipSuccess) return OCPG_OK;
+    std::fprintf(stderr, "ocpgpu: HIP error %d (%s)\n", (int)e, hipGetErrorString(e));
+    return OCPG_EHIP;
+}
+#define HIPCHK(x)                          \
+    do {                                   \
+        hipError_t e_ = (x);               \
+        if (e_ != hipSuccess) return hipStatus(e_); \
+    } while (0)
+
+static int64_t cellEntries(const ocpg_ctx* c, Cnt k)
+{
+    const int nc = c->NC + 1;
+    switch (k) {
+        case C1: return 1;
+        case CNC: return nc;
+        case CNCh: return c->NC;
+        case CNP: return OCPG_NP;
+        case CNPNC: return OCPG_NP * nc;
+        case CNPNP: return OCPG_NP * OCPG_NP;
+        case CLEN: return (int64_t)(nc + 1) * (nc + 1) * OCPG_NP;
+        case C5: return 5;
+    }
+    return 0;
+}
+
+static void buildDevEos(const ocpg_eos& e, DevEos& d)
+{
+    std::memset(&d, 0, sizeof(d));
+    const int NC = e.nc;
+    d.nc = NC;
+    d.pvtwRows = e.pvtw_rows;
+    for (int i = 0; i < NC; i++) {
+        d.Tc[i] = e.Tc[i];
+        d.Pc[i] = e.Pc[i];
+        d.Vc[i] = e.Vc[i];
+        d.MWC[i] = e.MW[i];
+        d.Acf[i] = e.Acf[i];
+        d.OmegaA[i] = e.OmegaA[i];
+        d.OmegaB[i] = e.OmegaB[i];
+        d.Vshift[i] = e.Vshift[i];
+        d.Vcvis[i] = e.Vcvis[i];
+        for (int k = 0; k < NC; k++) d.oneMinusBIC[i * NC + k] = 1 - e.BIC[i * OCPG_MAXNC + k];
+        // MixtureComp::AllocateOthers, src/MixtureComp.cpp:2323-2332
+        d.sqrtMWi[i] = std::sqrt(e.MW[i]);
+        d.muAux1I[i] = 5.4402 * std::pow(e.Tc[i], 1.0 / 6) / std::pow(e.Pc[i], 2.0 / 3);
+    }
+    for (int q = 0; q < 5; q++) d.LBCcoef[q] = e.LBCcoef[q];
+    for (int r = 0; r < e.pvtw_rows; r++)
+        for (int q = 0; q < 5; q++) d.pvtw[r][q] = e.pvtw[r][q];
+    d.stdRhoW = e.std_rhow;
+    d.ssmStaMaxIt = e.ssmsta_maxit;
+    d.ssmStaTol2 = e.ssmsta_tol * e.ssmsta_tol;
+    d.ssmStaEYt = e.ssmsta_eyt;
+    d.ssmStaKtol = 1E-4;   // SSMparamSTA::Ktol default, include/MixtureComp.hpp:33
+    d.ssmStaDYtol = 1E-6;  // SSMparamSTA::dYtol default, include/MixtureComp.hpp:34
+    d.nrStaMaxIt = e.nrsta_maxit;
+    d.nrStaTol = e.nrsta_tol;
+    d.ssmSpMaxIt = e.ssmsp_maxit;
+    d.ssmSpTol2 = e.ssmsp_tol * e.ssmsp_tol;
+    d.nrSpMaxIt = e.nrsp_maxit;
+    d.nrSpTol = e.nrsp_tol;
+    d.rrMaxIt = e.rr_maxit;
+    d.rrTol = e.rr_tol;
+}
+
+static void timerBegin(ocpg_ctx* c, int slot)
+{
+    if (!c->timing) return;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    (void)hipEventRecord(e, c->stream);
+    c->open[slot] = e;
+}
+
+static void timerEnd(ocpg_ctx* c, int slot)
+{
+    if (!c->timing || !c->open[slot]) return;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    (void)hipEventRecord(e, c->stream);
+    c->events[slot].push_back({c->open[slot], e});
+    c->open[slot] = nullptr;
+}
+
+static void timerClear(ocpg_ctx* c)
+{
+    for (int s = 0; s < kTimerSlots; s++) {
+        for (auto& p : c->events[s]) {
+            (void)hipEventDestroy(p.first);
+            (void)hipEventDestroy(p.second);
+        }
+        c->events[s].clear();
+        if (c->open[s]) (void)hipEventDestroy(c->open[s]);
+        c->open[s] = nullptr;
+    }
+}
+
+static bool tablesValid(int rows, int cols, const double* tab)
+{
+    if (rows < 1 || rows > OCPG_MAXTAB) return false;
+    for (int r = 1; r < rows; r++)
+        if (!(tab[r * cols] >= tab[(r - 1) * cols])) return false;
+    return true;
+}
+
+extern "C" {
+
+const char* ocpg_version(void) { return "ocpgpu 0.1 (gfx950)"; }
+
+// Size of the structures of include/ocpgpu.h, for binding checks.
+int64_t ocpg_sizeof(const char* name)
+{
+    if (!name) return -1;
+    if (!std::strcmp(name, "ocpg_eos")) return sizeof(ocpg_eos);
+    if (!std::strcmp(name, "ocpg_eos_deck")) return sizeof(ocpg_eos_deck);
+    if (!std::strcmp(name, "ocpg_satfunc")) return sizeof(ocpg_satfunc);
+    if (!std::strcmp(name, "ocpg_rock")) return sizeof(ocpg_rock);
+    if (!std::strcmp(name, "ocpg_bulk")) return sizeof(ocpg_bulk);
+    if (!std::strcmp(name, "ocpg_conn")) return sizeof(ocpg_conn);
+    if (!std::strcmp(name, "ocpg_resnorm")) return sizeof(ocpg_resnorm);
+    if (!std::strcmp(name, "ocpg_flash_stats")) return sizeof(ocpg_flash_stats);
+    return -1;
+}
+
+// MixtureComp::MixtureComp(const ComponentParam&, ...), src/MixtureComp.cpp:
+// 28-165: derive Vc (from ZCRIT), default OmegaA/OmegaB, scale SSHIFT, Vcvis,
+// LBCcoef x 10, symmetrise BIC, copy controls.
+int ocpg_eos_from_deck(const ocpg_eos_deck* d, ocpg_eos* e)
+{
+    if (!d || !e || d->nc < 1 || d->nc > OCPG_MAXNC) return OCPG_EINVAL;
+    if (!d->has_Vc && !d->has_Zc) return OCPG_EINVAL;
+    std::memset(e, 0, sizeof(*e));
+    const int NC = d->nc;
+    const double R = 10.73159;  // GAS_CONSTANT
+    e->nc = NC;
+    for (int i = 0; i < NC; i++) {
+        e->Tc[i] = d->Tc[i];
+        e->Pc[i] = d->Pc[i];
+        e->MW[i] = d->MW[i];
+        e->Acf[i] = d->Acf[i];
+        e->Vc[i] = d->has_Vc ? d->Vc[i] : 10.73159 * d->Zc[i] * d->Tc[i] / d->Pc[i];
+        e->OmegaA[i] = d->has_OmegaA ? d->OmegaA[i] : 0.457235529;
+        e->OmegaB[i] = d->has_OmegaB ? d->OmegaB[i] : 0.077796074;
+    }
+    for (int i = 0; i < NC; i++) {
+        e->Vshift[i] = d->has_Vshift ? d->Vshift[i] * (R * e->OmegaB[i] * e->Tc[i] / e->Pc[i]) : 0.0;
+        if (d->has_Vcvis)
+            e->Vcvis[i] = d->Vcvis[i];
+        else if (d->has_Zcvis)
+            e->Vcvis[i] = R * d->Zcvis[i] * e->Tc[i] / e->Pc[i];
+        else
+            e->Vcvis[i] = e->Vc[i];
+    }
+    for (int q = 0; q < 5; q++) e->LBCcoef[q] = d->LBCcoef[q] * 10;
+    int it = 0;
+    for (int i = 1; i < NC; i++)
+        for (int j = 0; j < i; j++) {
+            e->BIC[i * OCPG_MAXNC + j] = d->BIC_lower[it];
+            e->BIC[j * OCPG_MAXNC + i] = d->BIC_lower[it];
+            it++;
+        }
+    e->ssmsta_maxit = d->ssmsta_maxit;
+    e->ssmsta_tol = d->ssmsta_tol;
+    e->ssmsta_eyt = d->ssmsta_eyt;
+    e->nrsta_maxit = d->nrsta_maxit;
+    e->nrsta_tol = d->nrsta_tol;
+    e->ssmsp_maxit = d->ssmsp_maxit;
+    e->ssmsp_tol = d->ssmsp_tol;
+    e->nrsp_maxit = d->nrsp_maxit;
+    e->nrsp_tol = d->nrsp_tol;
+    e->rr_maxit = d->rr_maxit;
+    e->rr_tol = d->rr_tol;
+    e->pvtw_rows = d->pvtw_rows;
+    std::memcpy(e->pvtw, d->pvtw, sizeof(e->pvtw));
+    e->std_rhow = d->std_rhow;
+    return OCPG_OK;
+}
+
+int ocpg_create(ocpg_ctx** out, const ocpg_eos* eos, const ocpg_satfunc* sat, const ocpg_rock* rock,
+                int device)
+{
+    if (!out || !eos) return OCPG_EINVAL;
+    *out = nullptr;
+    if (eos->nc < 1 || eos->nc > OCPG_MAXNC) return OCPG_EINVAL;
+    if (!flashSupportsNC(eos->nc)) return OCPG_EUNSUP;
+    if (!tablesValid(eos->pvtw_rows, 5, &eos->pvtw[0][0])) return OCPG_EINVAL;
+    if (sat && (!tablesValid(sat->swof_rows, 4, &sat->swof[0][0]) ||
+                !tablesValid(sat->sgof_rows, 4, &sat->sgof[0][0])))
+        return OCPG_EINVAL;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return OCPG_ENODEV;
+    if (device < 0 || device >= ndev) return OCPG_EINVAL;
+    ocpg_ctx* c = new (std::nothrow) ocpg_ctx;
+    if (!c) return OCPG_EINVAL;
+    c->device = device;
+    c->NC = eos->nc;
+    c->eosHost = *eos;
+    int rc = OCPG_OK;
+    do {
+        if ((rc = hipStatus(hipSetDevice(device)))) break;
+        if ((rc = hipStatus(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)))) break;
+        DevEos de;
+        buildDevEos(*eos, de);
+        if ((rc = hipStatus(hipMalloc(&c->dEos, sizeof(DevEos))))) break;
+        if ((rc = hipStatus(hipMemcpy(c->dEos, &de, sizeof(DevEos), hipMemcpyHostToDevice)))) break;
+        if ((rc = hipStatus(hipMalloc(&c->dCtr, sizeof(DevFlashCounters))))) break;
+        if (sat) {
+            std::memset(&c->satHost, 0, sizeof(DevSat));
+            c->satHost.swofRows = sat->swof_rows;
+            c->satHost.sgofRows = sat->sgof_rows;
+            std::memcpy(c->satHost.swof, sat->swof, sizeof(sat->swof));
+            std::memcpy(c->satHost.sgof, sat->sgof, sizeof(sat->sgof));
+            c->satHost.kroMax = sat->swof[0][2];  // FlowUnit_ODGW01: SWOF col 2, row 0
+            if ((rc = hipStatus(hipMalloc(&c->dSat, sizeof(DevSat))))) break;
+            if ((rc = hipStatus(hipMemcpy(c->dSat, &c->satHost, sizeof(DevSat), hipMemcpyHostToDevice)))) break;
+        }
+        if (rock) {
+            c->rock.Pref = rock->Pref;
+            c->rock.cp1 = rock->cp1;
+            c->rock.cp2 = rock->cp2;
+        }
+    } while (0);
+    if (rc != OCPG_OK) {
+        if (c->dEos) (void)hipFree(c->dEos);
+        if (c->dCtr) (void)hipFree(c->dCtr);
+        if (c->dSat) (void)hipFree(c->dSat);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return OCPG_OK;
+}
+
+int ocpg_destroy(ocpg_ctx* c)
+{
+    if (!c) return OCPG_EINVAL;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    timerClear(c);
+    fimFree(c->fim);
+    if (c->snap) (void)hipFree(c->snap);
+    if (c->cur) (void)hipFree(c->cur);
+    if (c->last) (void)hipFree(c->last);
+    if (c->dEos) (void)hipFree(c->dEos);
+    if (c->dCtr) (void)hipFree(c->dCtr);
+    if (c->dSat) (void)hipFree(c->dSat);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return OCPG_OK;
+}
+
+// Device memory for nb cells (SoA, leading dimension rounded up to 64 cells);
+// zero-filled.  Replaces any earlier allocation.
+int ocpg_set_cells(ocpg_ctx* c, int64_t nb)
+{
+    if (!c || nb < 0 || nb > (int64_t(1) << 31)) return OCPG_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->cur) HIPCHK(hipFree(c->cur));
+    if (c->last) HIPCHK(hipFree(c->last));
+    c->cur = c->last = nullptr;
+    c->lastValid = false;
+    const int64_t ld = (nb + 63) / 64 * 64;
+    size_t off = 0;
+    for (int f = 0; f < kNumFields; f++) {
+        c->fieldOff[f] = off;
+        off += align256((size_t)kFields[f].elem * cellEntries(c, kFields[f].cnt) * (size_t)(ld > 0 ? ld : 64));
+    }
+    c->lastBytes = 0;
+    for (int f = 0; f < kNumFields; f++)
+        if (!kFields[f].lastStep) {
+            c->lastBytes = c->fieldOff[f];
+            break;
+        }
+    c->curBytes = off;
+    HIPCHK(hipMalloc(&c->cur, c->curBytes));
+    HIPCHK(hipMalloc(&c->last, c->lastBytes));
+    HIPCHK(hipMemsetAsync(c->cur, 0, c->curBytes, c->stream));
+    HIPCHK(hipMemsetAsync(c->last, 0, c->lastBytes, c->stream));
+    std::memset(&c->B, 0, sizeof(DevBulk));
+    c->B.nb = nb;
+    c->B.ld = ld;
+    for (int f = 0; f < kNumFields; f++) {
+        void* p = c->cur + c->fieldOff[f];
+        std::memcpy(reinterpret_cast<char*>(&c->B) + kFields[f].devOff, &p, sizeof(void*));
+    }
+    // IsoT_FIM::CalRes reads the last-step moles as lNi.
+    for (int f = 0; f < kNumFields; f++)
+        if (kFields[f].hostOff == HB(Ni)) c->B.lNi = reinterpret_cast<double*>(c->last + c->fieldOff[f]);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return OCPG_OK;
+}
+
+// Copy every non-NULL field of `h` (reference layout, h->nb cells) into the
+// device state.  lNi, when given, goes to the last-step copy.
+int ocpg_upload_bulk(ocpg_ctx* c, const ocpg_bulk* h)
+{
+    if (!c || !h || !c->cur || h->nb != c->B.nb) return OCPG_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    const int64_t nb = h->nb, ld = c->B.ld;
+    std::vector<char> tmp;
+    for (int f = 0; f < kNumFields; f++) {
+        const FieldDesc& d = kFields[f];
+        const char* src;
+        std::memcpy(&src, reinterpret_cast<const char*>(h) + d.hostOff, sizeof(void*));
+        if (!src) continue;
+        const int64_t cnt = cellEntries(c, d.cnt);
+        tmp.assign((size_t)d.elem * cnt * ld, 0);
+        for (int64_t n = 0; n < nb; n++)
+            for (int64_t k = 0; k < cnt; k++)
+                std::memcpy(&tmp[(size_t)(k * ld + n) * d.elem], src + (size_t)(n * cnt + k) * d.elem, d.elem);
+        HIPCHK(hipMemcpy(c->cur + c->fieldOff[f], tmp.data(), tmp.size(), hipMemcpyHostToDevice));
+    }
+    if (h->lNi) {
+        const int64_t cnt = c->NC + 1;
+        std::vector<double> t((size_t)cnt * ld, 0.0);
+        for (int64_t n = 0; n < nb; n++)
+            for (int64_t k = 0; k < cnt; k++) t[k * ld + n] = h->lNi[n * cnt + k];
+        HIPCHK(hipMemcpy(c->B.lNi, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    return OCPG_OK;
+}
+
+int ocpg_download_bulk(ocpg_ctx* c, ocpg_bulk* h)
+{
+    if (!c || !h || !c->cur || h->nb != c->B.nb) return OCPG_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int64_t nb = h->nb, ld = c->B.ld;
+    std::vector<char> tmp;
+    for (int f = 0; f < kNumFields; f++) {
+        const FieldDesc& d = kFields[f];
+        char* dst;
+        std::memcpy(&dst, reinterpret_cast<char*>(h) + d.hostOff, sizeof(void*));
+        if (!dst) continue;
+        const int64_t cnt = cellEntries(c, d.cnt);
+        tmp.resize((size_t)d.elem * cnt * ld);
+        HIPCHK(hipMemcpy(tmp.data(), c->cur + c->fieldOff[f], tmp.size(), hipMemcpyDeviceToHost));
+        for (int64_t n = 0; n < nb; n++)
+            for (int64_t k = 0; k < cnt; k++)
+                std::memcpy(dst + (size_t)(n * cnt + k) * d.elem, &tmp[(size_t)(k * ld + n) * d.elem], d.elem);
+    }
+    if (h->lNi) {
+        const int64_t cnt = c->NC + 1;
+        std::vector<double> t((size_t)cnt * ld);
+        HIPCHK(hipMemcpy(t.data(), c->B.lNi, t.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int64_t n = 0; n < nb; n++)
+            for (int64_t k = 0; k < cnt; k++) h->lNi[n * cnt + k] = t[k * ld + n];
+    }
+    return OCPG_OK;
+}
+
+// IsoT_FIM::CalFlash (src/OCPFluidMethod.cpp:1108-1120): FlashFIM +
+// PassFlashValue for every cell.  Asynchronous unless stats is non-NULL.
+int ocpg_flash_fim(ocpg_ctx* c, ocpg_flash_stats* stats)
+{
+    if (!c || !c->cur) return OCPG_ESTATE;
+    HIPCHK(hipSetDevice(c->device));
+    if (stats) HIPCHK(hipMemsetAsync(c->dCtr, 0, sizeof(DevFlashCounters), c->stream));
+    timerBegin(c, T_FLASH);
+    HIPCHK(launchFlashFim(c->NC, c->dEos, c->B, stats ? c->dCtr : nullptr, c->stream));
+    timerEnd(c, T_FLASH);
+    if (stats) {
+        DevFlashCounters h;
+        HIPCHK(hipMemcpyAsync(&h, c->dCtr, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        stats->cells = (int64_t)h.cells;
+        for (int q = 0; q < 3; q++) stats->ftype_count[q] = (int64_t)h.ftype[q];
+        stats->two_phase = (int64_t)h.twoPhase;
+        for (int q = 0; q < 5; q++) stats->iters[q] = (int64_t)h.iters[q];
+        stats->nonconverged = (int64_t)h.nonconverged;
+    }
+    return OCPG_OK;
+}
+
+// IsoT_FIM::InitFlash (src/OCPFluidMethod.cpp:1095-1106): InitFlashFIM from
+// P, T, Sw (S column 2), rockVp and the feed z (host, nb x NC row-major).
+// Ni becomes an output.
+int ocpg_init_flash_fim(ocpg_ctx* c, const double* zi)
+{
+    if (!c || !c->cur) return OCPG_ESTATE;
+    if (!zi) return OCPG_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    const int64_t nb = c->B.nb, ld = c->B.ld, NC = c->NC;
+    std::vector<double> t((size_t)NC * ld, 0.0);
+    for (int64_t n = 0; n < nb; n++)
+        for (int64_t i = 0; i < NC; i++) t[i * ld + n] = zi[n * NC + i];
+    double* dz = nullptr;
+    HIPCHK(hipMalloc(&dz, t.size() * sizeof(double)));
+    hipError_t e = hipMemcpy(dz, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice);
+    timerBegin(c, T_INIT);
+    if (e == hipSuccess) e = launchInitFlash(c->NC, c->dEos, c->B, dz, ld, c->stream);
+    timerEnd(c, T_INIT);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(dz);
+    return hipStatus(e);
+}
+
+// IsoT_FIM::UpdateLastTimeStep (src/OCPFluidMethod.cpp:2915-2957) plus the
+// skip-stability memory: device-to-device copy into the last-step block.
+int ocpg_update_last_step(ocpg_ctx* c)
+{
+    if (!c || !c->cur) return OCPG_ESTATE;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemcpyAsync(c->last, c->cur, c->lastBytes, hipMemcpyDeviceToDevice, c->stream));
+    c->lastValid = true;
+    return OCPG_OK;
+}
+
+// IsoT_FIM::ResetToLastTimeStep's Bulk part (src/OCPFluidMethod.cpp:2859-2895).
+int ocpg_reset_to_last_step(ocpg_ctx* c)
+{
+    if (!c || !c->cur || !c->lastValid) return OCPG_ESTATE;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemcpyAsync(c->cur, c->last, c->lastBytes, hipMemcpyDeviceToDevice, c->stream));
+    return OCPG_OK;
+}
+
+// Copy of the fields the flash starts from (P, Ni, S, phaseNum, xij and the
+// skip-stability memory), so that one Newton iteration can be re-run from
+// the same state.  No reference counterpart.
+int ocpg_save_flash_inputs(ocpg_ctx* c)
+{
+    if (!c || !c->cur) return OCPG_ESTATE;
+    HIPCHK(hipSetDevice(c->device));
+    size_t need = 0;
+    for (int f = 0; f < kNumFields; f++)
+        if (isFlashInput(kFields[f].hostOff))
+            need += align256((size_t)kFields[f].elem * cellEntries(c, kFields[f].cnt) * c->B.ld);
+    if (need != c->snapBytes) {
+        if (c->snap) HIPCHK(hipFree(c->snap));
+        c->snap = nullptr;
+        HIPCHK(hipMalloc(&c->snap, need));
+        c->snapBytes = need;
+    }
+    size_t off = 0;
+    for (int f = 0; f < kNumFields; f++) {
+        if (!isFlashInput(kFields[f].hostOff)) continue;
+        const size_t b = (size_t)kFields[f].elem * cellEntries(c, kFields[f].cnt) * c->B.ld;
+        HIPCHK(hipMemcpyAsync(c->snap + off, c->cur + c->fieldOff[f], b, hipMemcpyDeviceToDevice, c->stream));
+        off += align256(b);
+    }
+    return OCPG_OK;
+}
+
+int ocpg_restore_flash_inputs(ocpg_ctx* c)
+{
+    if (!c || !c->cur || !c->snap) return OCPG_ESTATE;
+    HIPCHK(hipSetDevice(c->device));
+    timerBegin(c, T_RESTORE);
+    size_t off = 0;
+    for (int f = 0; f < kNumFields; f++) {
+        if (!isFlashInput(kFields[f].hostOff)) continue;
+        const size_t b = (size_t)kFields[f].elem * cellEntries(c, kFields[f].cnt) * c->B.ld;
+        HIPCHK(hipMemcpyAsync(c->cur + c->fieldOff[f], c->snap + off, b, hipMemcpyDeviceToDevice, c->stream));
+        off += align256(b);
+    }
+    timerEnd(c, T_RESTORE);
+    return OCPG_OK;
+}
+
+// Per-kernel timing with HIP events on the context's stream.
+int ocpg_timing_enable(ocpg_ctx* c, int on)
+{
+    if (!c) return OCPG_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    timerClear(c);
+    c->timing = on != 0;
+    return OCPG_OK;
+}
+
+// Total milliseconds and launch count recorded for a timer ("flash",
+// "krpc_rock", "calres", "assemble", "init_flash", "restore").
+int ocpg_timing_get(ocpg_ctx* c, const char* name, double* ms, int64_t* count)
+{
+    if (!c || !name) return OCPG_EINVAL;
+    int slot = -1;
+    for (int s = 0; s < kTimerSlots; s++)
+        if (!std::strcmp(name, kTimerNames[s])) slot = s;
+    if (slot < 0) return OCPG_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double tot = 0;
+    for (auto& p : c->events[slot]) {
+        float t = 0;
+        HIPCHK(hipEventElapsedTime(&t, p.first, p.second));
+        tot += t;
+    }
+    if (ms) *ms = tot;
+    if (count) *count = (int64_t)c->events[slot].size();
+    return OCPG_OK;
+}
+
+int ocpg_synchronize(ocpg_ctx* c)
+{
+    if (!c) return OCPG_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return OCPG_OK;
+}
+
+// Internal access for the C++ shim and the benchmark: the context's stream
+// and device cell state.
+void* ocpg_stream(ocpg_ctx* c) { return c ? (void*)c->stream : nullptr; }
+int64_t ocpg_device_bytes(ocpg_ctx* c)
+{
+    return c ? (int64_t)(c->curBytes + c->lastBytes + fimBytes(c->fim)) : -1;
+}
+
+}  // extern "C"
+
+// Accessors used by the FIM half of the C-ABI (fim_capi.cpp).
+namespace ocpg {
+DevBulk& ctxBulk(ocpg_ctx* c) { return c->B; }
+FimDevice& ctxFim(ocpg_ctx* c) { return c->fim; }
+hipStream_t ctxStream(ocpg_ctx* c) { return c->stream; }
+int ctxNC(ocpg_ctx* c) { return c->NC; }
+int ctxDevice(ocpg_ctx* c) { return c->device; }
+const DevSat* ctxSat(ocpg_ctx* c) { return c->dSat; }
+const DevRock& ctxRock(ocpg_ctx* c) { return c->rock; }
+void ctxTimerBegin(ocpg_ctx* c, int slot) { timerBegin(c, slot); }
+void ctxTimerEnd(ocpg_ctx* c, int slot) { timerEnd(c, slot); }
+}  // namespace ocpg

@@ -1,0 +1,199 @@
+"""Loader and thin Python mirror of the C-ABI (include/ocpgpu.h).
+
+Loading fails loudly when libocpgpu.so is not built: there is no CPU
+fallback in the product path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libocpgpu.so")
+
+_lib = None
+
+
+class OcpgError(RuntimeError):
+    pass
+
+
+def load():
+    """The HIP library, with argument types set."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OcpgError(f"{LIB_PATH} is not built; run `python -m opencaeporo_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    sig = {
+        "ocpg_version": ([], C.c_char_p),
+        "ocpg_sizeof": ([C.c_char_p], C.c_int64),
+        "ocpg_eos_from_deck": ([P, P], C.c_int),
+        "ocpg_create": ([P, P, P, P, C.c_int], C.c_int),
+        "ocpg_destroy": ([P], C.c_int),
+        "ocpg_set_cells": ([P, C.c_int64], C.c_int),
+        "ocpg_upload_bulk": ([P, P], C.c_int),
+        "ocpg_download_bulk": ([P, P], C.c_int),
+        "ocpg_flash_fim": ([P, P], C.c_int),
+        "ocpg_init_flash_fim": ([P, P], C.c_int),
+        "ocpg_update_last_step": ([P], C.c_int),
+        "ocpg_reset_to_last_step": ([P], C.c_int),
+        "ocpg_set_conn": ([P, P], C.c_int),
+        "ocpg_bsr_nnzb": ([P], C.c_int64),
+        "ocpg_calkrpc": ([P], C.c_int),
+        "ocpg_calrock": ([P], C.c_int),
+        "ocpg_calkrpc_rock": ([P], C.c_int),
+        "ocpg_calres": ([P, C.c_double, P], C.c_int),
+        "ocpg_assemble": ([P, C.c_double], C.c_int),
+        "ocpg_get_bsr": ([P, P, P, P, P], C.c_int),
+        "ocpg_get_upwind": ([P, P, P, P], C.c_int),
+        "ocpg_save_flash_inputs": ([P], C.c_int),
+        "ocpg_restore_flash_inputs": ([P], C.c_int),
+        "ocpg_timing_enable": ([P, C.c_int], C.c_int),
+        "ocpg_timing_get": ([P, C.c_char_p, P, P], C.c_int),
+        "ocpg_synchronize": ([P], C.c_int),
+        "ocpg_stream": ([P], C.c_void_p),
+        "ocpg_device_bytes": ([P], C.c_int64),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def check(rc, what):
+    if rc != abi.OK:
+        raise OcpgError(f"{what} failed with code {rc}")
+
+
+def eos_from_deck(deck):
+    e = abi.Eos()
+    check(load().ocpg_eos_from_deck(C.byref(deck), C.byref(e)), "ocpg_eos_from_deck")
+    return e
+
+
+class Context:
+    """One ocpg_ctx: EoS + saturation tables + rock on one device."""
+
+    def __init__(self, eos, sat=None, rock=None, device=0):
+        self.lib = load()
+        self.eos = eos
+        self.NC = eos.nc
+        self.nc = self.NC + 1
+        self.ncol = self.nc + 1
+        h = C.c_void_p()
+        check(self.lib.ocpg_create(C.byref(h), C.byref(eos), C.byref(sat) if sat else None,
+                                   C.byref(rock) if rock else None, device), "ocpg_create")
+        self.h = h
+        self.nb = 0
+        self.grid = None
+
+    def close(self):
+        if self.h:
+            self.lib.ocpg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_cells(self, nb):
+        check(self.lib.ocpg_set_cells(self.h, nb), "ocpg_set_cells")
+        self.nb = nb
+
+    def upload(self, bulk):
+        s = bulk.cstruct()
+        check(self.lib.ocpg_upload_bulk(self.h, C.byref(s)), "ocpg_upload_bulk")
+
+    def download(self, bulk):
+        s = bulk.cstruct()
+        check(self.lib.ocpg_download_bulk(self.h, C.byref(s)), "ocpg_download_bulk")
+
+    def flash_fim(self, stats=True):
+        st = abi.FlashStats() if stats else None
+        check(self.lib.ocpg_flash_fim(self.h, C.byref(st) if st else None), "ocpg_flash_fim")
+        return st
+
+    def init_flash_fim(self, zi):
+        zi = np.ascontiguousarray(zi, dtype=np.float64)
+        check(self.lib.ocpg_init_flash_fim(self.h, zi.ctypes.data), "ocpg_init_flash_fim")
+
+    def update_last_step(self):
+        check(self.lib.ocpg_update_last_step(self.h), "ocpg_update_last_step")
+
+    def reset_to_last_step(self):
+        check(self.lib.ocpg_reset_to_last_step(self.h), "ocpg_reset_to_last_step")
+
+    def set_conn(self, grid):
+        conn = abi.conn_struct(grid.bId, grid.eId, grid.area)
+        check(self.lib.ocpg_set_conn(self.h, C.byref(conn)), "ocpg_set_conn")
+        self.grid = grid
+
+    def nnzb(self):
+        return int(self.lib.ocpg_bsr_nnzb(self.h))
+
+    def calkrpc(self):
+        check(self.lib.ocpg_calkrpc(self.h), "ocpg_calkrpc")
+
+    def calrock(self):
+        check(self.lib.ocpg_calrock(self.h), "ocpg_calrock")
+
+    def calkrpc_rock(self):
+        check(self.lib.ocpg_calkrpc_rock(self.h), "ocpg_calkrpc_rock")
+
+    def calres(self, dt, norms=True):
+        nr = abi.ResNorm() if norms else None
+        check(self.lib.ocpg_calres(self.h, dt, C.byref(nr) if nr else None), "ocpg_calres")
+        return nr
+
+    def assemble(self, dt):
+        check(self.lib.ocpg_assemble(self.h, dt), "ocpg_assemble")
+
+    def get_bsr(self):
+        nnzb = self.nnzb()
+        IA = np.zeros(self.nb + 1, dtype=np.int32)
+        JA = np.zeros(nnzb, dtype=np.int32)
+        val = np.zeros(nnzb * self.ncol * self.ncol)
+        rhs = np.zeros(self.nb * self.ncol)
+        check(self.lib.ocpg_get_bsr(self.h, IA.ctypes.data, JA.ctypes.data, val.ctypes.data,
+                                    rhs.ctypes.data), "ocpg_get_bsr")
+        return IA, JA, val, rhs
+
+    def get_upwind(self):
+        n = self.grid.nconn * abi.NP
+        up = np.zeros(n, dtype=np.int32)
+        urho = np.zeros(n)
+        uvel = np.zeros(n)
+        check(self.lib.ocpg_get_upwind(self.h, up.ctypes.data, urho.ctypes.data, uvel.ctypes.data),
+              "ocpg_get_upwind")
+        return up, urho, uvel
+
+    def save_flash_inputs(self):
+        check(self.lib.ocpg_save_flash_inputs(self.h), "ocpg_save_flash_inputs")
+
+    def restore_flash_inputs(self):
+        check(self.lib.ocpg_restore_flash_inputs(self.h), "ocpg_restore_flash_inputs")
+
+    def timing(self, on=True):
+        check(self.lib.ocpg_timing_enable(self.h, 1 if on else 0), "ocpg_timing_enable")
+
+    def timer(self, name):
+        ms = C.c_double()
+        cnt = C.c_int64()
+        check(self.lib.ocpg_timing_get(self.h, name.encode(), C.byref(ms), C.byref(cnt)),
+              "ocpg_timing_get")
+        return ms.value, cnt.value
+
+    def synchronize(self):
+        check(self.lib.ocpg_synchronize(self.h), "ocpg_synchronize")
+
+    def device_bytes(self):
+        return int(self.lib.ocpg_device_bytes(self.h))

@@ -896,11 +896,15 @@ void Mixture::PhaseSplit()
     rrCurIt = 0;
     SplitSSM2(false);
     SplitNR();
+    // Bounded at 100 restarts (the reference has no bound; the HIP kernel has
+    // the same one so that no cell can spin forever).
+    int restarts = 0;
     while (!nrSpConflag) {
         SplitSSM2(true);
         SplitNR();
         if (!CheckSplit()) break;
         if (ssmSpConflag) break;
+        if (++restarts >= 100) break;
     }
     CheckSplit();
     iters[2] += ssmSpCurIt;
@@ -1494,7 +1498,12 @@ void Mixture::CalMuPXLBC_partial()
             der3J = 0;
             for (int i = 0; i < NC; i++) {
                 val1IJ = muAux1I[i] / sqrt(MW[j]);
+#ifdef OCP_ORACLE_EXACT_LBC
+                // test-only variant: the derivative the reference meant to take
+                der1IJ = -(1.0 / 2) * muAux1I[i] * pow(MW[j], -1.5) * derMWj;
+#else
                 der1IJ = -(1 / 2) * muAux1I[i] * pow(MW[j], -1.5) * derMWj;
+#endif
                 Tri = T / Tc[i];
                 if (Tri <= 1.5) {
                     tmp = 34 * 1E-5 * pow(Tri, 0.94);

@@ -14,6 +14,7 @@ from opencaeporo_amd import abi
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "build", "liboracle.so")
 COUNT_SO = os.path.join(HERE, "build", "liboracle_count.so")
+EXACT_SO = os.path.join(HERE, "build", "liboracle_exact.so")
 REF_SO = os.path.join(HERE, "_ref", "libocpref.so")
 
 

@@ -1,0 +1,65 @@
+"""The C-ABI library loads without a GPU, exports what include/ocpgpu.h
+declares, agrees with the Python mirror on struct sizes, and derives the EoS
+description from deck values as the reference constructor does."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from opencaeporo_amd import abi, fluids, lib
+from tests import parity
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include",
+                      "ocpgpu.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const char\*|int64_t|int|void\*)\s+(ocpg_\w+)\s*\(",
+                                 text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = lib.load()
+    names = declared_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+@pytest.mark.parametrize("name,cls", [("ocpg_eos", abi.Eos), ("ocpg_eos_deck", abi.EosDeck),
+                                      ("ocpg_satfunc", abi.SatFunc), ("ocpg_rock", abi.Rock),
+                                      ("ocpg_bulk", abi.BulkC), ("ocpg_conn", abi.Conn),
+                                      ("ocpg_resnorm", abi.ResNorm),
+                                      ("ocpg_flash_stats", abi.FlashStats)])
+def test_struct_sizes_match(name, cls):
+    assert lib.load().ocpg_sizeof(name.encode()) == C.sizeof(cls)
+
+
+@pytest.mark.parametrize("fixture,deck", [("flash_spe5.npz", fluids.spe5_deck),
+                                          ("flash_bench6.npz", fluids.bench6_deck),
+                                          ("flash_ten.npz", fluids.ten_deck)])
+def test_eos_from_deck_matches_reference_constructor(fixture, deck):
+    d = parity.load(fixture)
+    ref = parity.eos_from_fixture(d)
+    got = lib.eos_from_deck(deck())
+    NC = ref.nc
+    assert got.nc == NC
+    for f in ("Tc", "Pc", "Vc", "MW", "Acf", "OmegaA", "OmegaB", "Vshift", "Vcvis"):
+        np.testing.assert_array_equal(np.array(getattr(got, f)[:NC]), np.array(getattr(ref, f)[:NC]), f)
+    np.testing.assert_array_equal(np.array(got.BIC[:]), np.array(ref.BIC[:]))
+    np.testing.assert_array_equal(np.array(got.LBCcoef[:]), np.array(ref.LBCcoef[:]))
+    for f in ("ssmsta_maxit", "ssmsta_tol", "ssmsta_eyt", "nrsta_maxit", "nrsta_tol", "ssmsp_maxit",
+              "ssmsp_tol", "nrsp_maxit", "nrsp_tol", "rr_maxit", "rr_tol", "pvtw_rows", "std_rhow"):
+        assert getattr(got, f) == getattr(ref, f), f
+
+
+def test_eos_from_deck_rejects_bad_input():
+    d = fluids.spe5_deck()
+    d.nc = 0
+    assert lib.load().ocpg_eos_from_deck(C.byref(d), C.byref(abi.Eos())) == abi.EINVAL
+    d = fluids.spe5_deck()
+    d.has_Zc = 0
+    assert lib.load().ocpg_eos_from_deck(C.byref(d), C.byref(abi.Eos())) == abi.EINVAL

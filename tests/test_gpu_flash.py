@@ -1,0 +1,74 @@
+"""HIP flash (ocpg_flash_fim / ocpg_init_flash_fim) against the oracle and
+the reference's golden vectors, through the C-ABI."""
+import numpy as np
+import pytest
+
+from opencaeporo_amd import abi, lib
+from oracle.oracle_py import Oracle
+from tests import gpu_util, parity
+
+pytestmark = pytest.mark.gpu
+
+CASES = ["flash_spe5.npz", "flash_bench6.npz", "flash_ten.npz"]
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return Oracle()
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("phase", ["cold", "warm"])
+def test_flash_matches_oracle_and_reference(orc, name, phase):
+    gpu_util.say(f"[flash parity] {name} {phase}")
+    d = parity.load(name)
+    eos = parity.eos_from_fixture(d)
+    b = parity.bulk_from_fixture(d, phase)
+    ctx = lib.Context(eos)
+    g, stats = gpu_util.gpu_flash(ctx, b)
+    o = b.copy()
+    orc.flash_fim(eos, o)
+    ref = gpu_util.snapshot_dict(o, "orc_")
+    rep = parity.compare_state(g, ref, "orc_", gpu_util.GPU_TOL, frac=gpu_util.GPU_FRAC,
+                               loose=gpu_util.GPU_LOOSE)
+    gpu_util.say("  vs oracle:", {k: v for k, v in rep.items() if k != "cells_beyond_tol"})
+    rep2 = parity.compare_state(g, d, f"{phase}_out_", 1e-8, frac=0.97, loose=1e-4)
+    gpu_util.say("  vs reference: within 1e-8", rep2["frac_within_tol"])
+    assert stats.cells == b.nb
+    assert stats.two_phase == int(np.sum(g.phaseNum == 3))
+    ctx.close()
+
+
+def test_initflash_matches_oracle_and_reference(orc):
+    gpu_util.say("[initflash parity]")
+    d = parity.load("initflash_spe5.npz")
+    eos = parity.eos_from_fixture(parity.load("flash_spe5.npz"))
+    nb = len(d["init_in_P"])
+    b = abi.Bulk(nb, int(d["deck_nc"]))
+    b.P[:] = d["init_in_P"]
+    b.T[:] = d["init_in_T"]
+    b.S[:] = d["init_in_S"]
+    b.rockVp[:] = d["init_rockVp"]
+    ctx = lib.Context(eos)
+    g, _ = gpu_util.gpu_flash(ctx, b, init_z=d["init_z"])
+    o = b.copy()
+    orc.init_flash_fim(eos, o, d["init_z"])
+    ref = gpu_util.snapshot_dict(o, "orc_")
+    parity.compare_state(g, ref, "orc_", gpu_util.GPU_TOL, frac=gpu_util.GPU_FRAC,
+                         loose=gpu_util.GPU_LOOSE)
+    err = parity.scaled_err(g.Ni.reshape(nb, -1), o.Ni.reshape(nb, -1))
+    assert err.max() <= gpu_util.GPU_LOOSE and np.mean(err <= gpu_util.GPU_TOL) >= gpu_util.GPU_FRAC
+    ctx.close()
+
+
+def test_flash_is_deterministic():
+    """Two runs on the same inputs agree bit for bit."""
+    d = parity.load("flash_spe5.npz")
+    eos = parity.eos_from_fixture(d)
+    b = parity.bulk_from_fixture(d, "cold")
+    ctx = lib.Context(eos)
+    g1, _ = gpu_util.gpu_flash(ctx, b)
+    g2, _ = gpu_util.gpu_flash(ctx, b)
+    for f in parity.STATE_FIELDS:
+        assert np.array_equal(getattr(g1, f), getattr(g2, f)), f
+    ctx.close()
