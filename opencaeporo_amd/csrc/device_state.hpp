@@ -67,6 +67,22 @@ struct DevBulk {
     int32_t *ftype, *iters;
 };
 
+// Scratch of the staged flash (cells in SoA with the bulk's ld; lists of
+// cell indices with their lengths in count[4]).
+struct FlashWork {
+    double* zi;      // NC per cell: feed mole fractions
+    double* Nh;      // hydrocarbon moles
+    double* Yt;      // sum of the stability trial phase (PhaseSplit input)
+    double* x;       // 2*NC: split compositions
+    double* n;       // 2*NC: split moles (fractions of Nh)
+    double* nu;      // 2: phase fractions
+    int8_t* ftype;
+    int8_t* lNP;
+    uint8_t* flags;
+    int32_t* list[4];
+    int32_t* count;  // 4 counters
+};
+
 // Totals gathered by the flash kernels (one atomic per wavefront).
 struct DevFlashCounters {
     unsigned long long cells;

@@ -16,20 +16,11 @@ bool flashSupportsNC(int nc)
     return false;
 }
 
-hipError_t launchFlashFim(int NC, const DevEos* eos, const DevBulk& B, DevFlashCounters* ctr, hipStream_t s)
+hipError_t launchFlash(int NC, const DevEos* eos, const DevBulk& B, const FlashWork& W, DevFlashCounters* ctr,
+                       const double* zin, hipStream_t s)
 {
 #define X(N) \
-    if (NC == N) return launchFlashT<N>(eos, B, ctr, s);
-    OCPG_NC_LIST(X)
-#undef X
-    return hipErrorInvalidValue;
-}
-
-hipError_t launchInitFlash(int NC, const DevEos* eos, const DevBulk& B, const double* z, int64_t zld,
-                           hipStream_t s)
-{
-#define X(N) \
-    if (NC == N) return launchInitFlashT<N>(eos, B, z, zld, s);
+    if (NC == N) return launchFlashT<N>(eos, B, W, ctr, zin, s);
     OCPG_NC_LIST(X)
 #undef X
     return hipErrorInvalidValue;

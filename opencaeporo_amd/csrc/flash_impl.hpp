@@ -19,11 +19,12 @@ struct FlashKernel {
     // CaldXsdXpAPI02p (:4386-4493), CalXiPNX_partial / CalRhoPX_partial /
     // CalMuPXLBC_partial (:3037-3213), pSderExist/pVnumCom (:559-570),
     // CalSkipForNextStep (:5155-5254), then PassFlashValue's stores.
+    template <int NPT>
     __device__ static void finish(FlashCell<NC>& F, const DevEos& E, const DevBulk& B, int64_t cell,
                                   bool init, double Sw_in, double Vpore)
     {
         const int64_t ld = B.ld;
-        const int NP = F.NP;
+        constexpr int NP = NPT;
         const double P = F.P, T = F.T;
         // ---- CalMW, CalVfXiRho, CalViscoLBC (per internal phase) ----
         double MW[2], vC[2], xiC[2], rhoC[2], muC[2], muA[2][5];
@@ -505,126 +506,307 @@ __device__ __forceinline__ int calFtypeFIM(const FlashCell<NC>& F, const DevBulk
     return 0;
 }
 
-__device__ __forceinline__ void accumulateCounters(DevFlashCounters* ctr, bool active, int ftype, int NP,
-                                                   const int* it, bool nonconv)
+// ---------------------------------------------------------------------------
+// Staged flash.  One call of IsoT_FIM::CalFlash runs five kernels over the
+// cells:
+//   classify  : ftype of every cell (SkipStaAnaly::CalFtypeFIM), lists
+//   stability : PhaseStable on the ftype-0 list
+//   split     : PhaseSplit on the cells that need it (ftype 2, unstable)
+//   finish1/2 : CalFlash tail + derivatives + stores for the cells that end
+//               with one / two hydrocarbon phases
+// Each cell's result is what one FlashFIM call computes; the stages only
+// regroup cells so that a wavefront runs one kind of work, and each kernel
+// carries only its own working set.  State between stages lives in the
+// FlashWork buffers (SoA, leading dimension ld).
+// ---------------------------------------------------------------------------
+
+enum { FL_SKIP = 1, FL_SSMSTA = 2, FL_NRSTA = 4, FL_SSMSP = 8, FL_NRSP = 16 };
+enum { LIST_STAB = 0, LIST_SPLIT = 1, LIST_NP1 = 2, LIST_NP2 = 3 };
+
+// Append the active lanes' cells to list k: one atomic per wavefront.
+__device__ __forceinline__ void appendList(const FlashWork& W, int k, bool pred, int32_t cell)
 {
-    if (!ctr) return;
-    // one atomic per wavefront: reduce across lanes with shuffles
-    unsigned long long v[11];
-    v[0] = active ? 1 : 0;
-    v[1] = (active && ftype == 0) ? 1 : 0;
-    v[2] = (active && ftype == 1) ? 1 : 0;
-    v[3] = (active && ftype == 2) ? 1 : 0;
-    v[4] = (active && NP == 2) ? 1 : 0;
-#pragma unroll
-    for (int q = 0; q < 5; ++q) v[5 + q] = active ? (unsigned long long)it[q] : 0ull;
-    v[10] = (active && nonconv) ? 1 : 0;
-#pragma unroll
-    for (int q = 0; q < 11; ++q) {
-        unsigned long long s = v[q];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-        v[q] = s;
+    const unsigned long long mask = __ballot(pred);
+    if (mask == 0) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&W.count[k], __popcll(mask));
+    base = __shfl(base, leader, 64);
+    if (pred) {
+        const unsigned long long below = (lane == 0) ? 0ull : (mask & ((1ull << lane) - 1ull));
+        W.list[k][base + __popcll(below)] = cell;
     }
-    if ((threadIdx.x & 63) == 0) {
-        unsigned long long* dst = &ctr->cells;
-        atomicAdd(&dst[0], v[0]);
-        atomicAdd(&dst[1], v[1]);
-        atomicAdd(&dst[2], v[2]);
-        atomicAdd(&dst[3], v[3]);
-        atomicAdd(&dst[4], v[4]);
+}
+
+__device__ __forceinline__ unsigned long long waveSum(unsigned long long v)
+{
 #pragma unroll
-        for (int q = 0; q < 5; ++q) atomicAdd(&dst[5 + q], v[5 + q]);
-        atomicAdd(&dst[10], v[10]);
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ void addCounter(unsigned long long* dst, unsigned long long v)
+{
+    v = waveSum(v);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, v);
+}
+
+__global__ void flashResetKernel(FlashWork W, DevFlashCounters* ctr)
+{
+    if (threadIdx.x < 4) W.count[threadIdx.x] = 0;
+    if (ctr && threadIdx.x < sizeof(DevFlashCounters) / sizeof(unsigned long long))
+        reinterpret_cast<unsigned long long*>(ctr)[threadIdx.x] = 0;
+}
+
+// Stage 1: zi, Nh, lNP, ftype; lists.  zin non-NULL: InitFlashFIM (the feed
+// comes in, ftype 0, lNP 0).
+template <int NC>
+__global__ void __launch_bounds__(kFlashBlock) flashClassifyKernel(const DevEos* __restrict__ eos, DevBulk B,
+                                                                   FlashWork W, DevFlashCounters* ctr,
+                                                                   const double* __restrict__ zin)
+{
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = n < B.nb;
+    const int64_t ld = B.ld;
+    constexpr int nc = NC + 1;
+    int ftype = 0;
+    if (active) {
+        FlashCell<NC> F(*eos);
+        int lNP = 0;
+        if (zin) {
+#pragma unroll
+            for (int i = 0; i < NC; ++i) W.zi[i * ld + n] = zin[i * ld + n];
+        } else {
+            const int lastNP = B.phaseNum[n];
+            lNP = lastNP > 0 ? lastNP - 1 : 0;
+            F.lNP = lNP;
+            F.P = B.P[n];
+            F.T = B.T[n] + kCONV5;
+#pragma unroll
+            for (int i = 0; i < nc; ++i) F.Ni[i] = B.Ni[i * ld + n];
+            double Nh = 0;
+#pragma unroll
+            for (int i = 0; i < NC; ++i) Nh += fabs(F.Ni[i]);
+            F.Nh = Nh;
+#pragma unroll
+            for (int i = 0; i < NC; ++i) W.zi[i * ld + n] = F.Ni[i] / Nh;
+            W.Nh[n] = Nh;
+            double S[2] = {B.S[0 * ld + n], B.S[1 * ld + n]};
+            ftype = calFtypeFIM<NC>(F, B, n, S);
+        }
+        W.ftype[n] = (int8_t)ftype;
+        W.lNP[n] = (int8_t)lNP;
+        W.flags[n] = (ftype == 2) ? 0 : FL_SKIP;
+        if (ftype == 2) W.Yt[n] = 1.01;
+        if (B.ftype) B.ftype[n] = ftype;
+        if (B.iters) {
+#pragma unroll
+            for (int q = 0; q < 5; ++q) B.iters[q * ld + n] = 0;
+        }
+    }
+    appendList(W, LIST_STAB, active && ftype == 0, (int32_t)n);
+    appendList(W, LIST_SPLIT, active && ftype == 2, (int32_t)n);
+    appendList(W, LIST_NP1, active && ftype == 1, (int32_t)n);
+    if (ctr) {
+        addCounter(&ctr->cells, active ? 1 : 0);
+        addCounter(&ctr->ftype[0], (active && ftype == 0) ? 1 : 0);
+        addCounter(&ctr->ftype[1], (active && ftype == 1) ? 1 : 0);
+        addCounter(&ctr->ftype[2], (active && ftype == 2) ? 1 : 0);
     }
 }
 
 template <int NC>
-__global__ void __launch_bounds__(kFlashBlock) flashFimKernel(const DevEos* __restrict__ eos, DevBulk B,
-                                                              DevFlashCounters* ctr)
+__device__ __forceinline__ void loadFeed(FlashCell<NC>& F, const DevBulk& B, const FlashWork& W, int64_t n)
 {
-    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = n < B.nb;
-    FlashCell<NC> F(*eos);
-    bool nonconv = false;
+    const int64_t ld = B.ld;
+    F.P = B.P[n];
+    F.T = B.T[n] + kCONV5;
 #pragma unroll
-    for (int q = 0; q < 5; ++q) F.it[q] = 0;
-    F.ftype = 0;
-    F.NP = 1;
+    for (int i = 0; i < NC; ++i) F.zi[i] = W.zi[i * ld + n];
+    F.lNP = W.lNP[n];
+    F.ftype = W.ftype[n];
+}
+
+// Stage 2: PhaseEquilibrium, ftype 0 part up to the stability verdict.
+template <int NC>
+__global__ void __launch_bounds__(kFlashBlock) flashStabilityKernel(const DevEos* __restrict__ eos, DevBulk B,
+                                                                    FlashWork W, DevFlashCounters* ctr)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = idx < W.count[LIST_STAB];
+    bool stable = false;
+    int it0 = 0, it1 = 0;
+    bool nonconv = false;
+    int32_t n = 0;
     if (active) {
+        n = W.list[LIST_STAB][idx];
+        const int64_t ld = B.ld;
+        FlashCell<NC> F(*eos);
+        loadFeed<NC>(F, B, W, n);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) F.it[q] = 0;
+        F.flagSkip = true;
+        F.singlePhaseFeed();
+        F.calKwilson();
+        stable = F.phaseStable();
+        it0 = F.it[IT_SSMSTA];
+        it1 = F.it[IT_NRSTA];
+        int fl = F.flagSkip ? FL_SKIP : 0;
+        if (F.ssmStaConflag) fl |= FL_SSMSTA;
+        if (F.nrStaConflag) fl |= FL_NRSTA;
+        W.flags[n] = (uint8_t)fl;
+        if (!stable) W.Yt[n] = F.Yt;
+        if (stable) nonconv = !(F.nrStaConflag || F.ssmStaConflag);
+        if (B.iters) {
+            B.iters[IT_SSMSTA * ld + n] = it0;
+            B.iters[IT_NRSTA * ld + n] = it1;
+        }
+    }
+    appendList(W, LIST_NP1, active && stable, n);
+    appendList(W, LIST_SPLIT, active && !stable, n);
+    if (ctr) {
+        addCounter(&ctr->iters[IT_SSMSTA], (unsigned long long)it0);
+        addCounter(&ctr->iters[IT_NRSTA], (unsigned long long)it1);
+        addCounter(&ctr->nonconverged, nonconv ? 1 : 0);
+    }
+}
+
+// Stage 3: PhaseSplit (two hydrocarbon phases), src/MixtureComp.cpp:1743-1782.
+template <int NC>
+__global__ void __launch_bounds__(kFlashBlock) flashSplitKernel(const DevEos* __restrict__ eos, DevBulk B,
+                                                                FlashWork W, DevFlashCounters* ctr)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = idx < W.count[LIST_SPLIT];
+    int NP = 1;
+    int its[3] = {0, 0, 0};
+    bool nonconv = false;
+    int32_t n = 0;
+    if (active) {
+        n = W.list[LIST_SPLIT][idx];
         const int64_t ld = B.ld;
         constexpr int nc = NC + 1;
-        const int lastNP = B.phaseNum[n];
-        F.lNP = lastNP > 0 ? lastNP - 1 : 0;
+        FlashCell<NC> F(*eos);
+        loadFeed<NC>(F, B, W, n);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) F.it[q] = 0;
         if (F.lNP == 2) {
 #pragma unroll
             for (int i = 0; i < NC; ++i) F.lKs[i] = B.xij[(0 * nc + i) * ld + n] / B.xij[(1 * nc + i) * ld + n];
         }
-        F.P = B.P[n];
-        F.T = B.T[n] + kCONV5;
+        int fl = W.flags[n];
+        F.ssmStaConflag = fl & FL_SSMSTA;
+        F.nrStaConflag = fl & FL_NRSTA;
+        F.Yt = W.Yt[n];
+        F.NP = 2;
+        F.calAiBi();
+        F.calKwilson();
+        F.phaseSplit();
+        NP = F.NP;
+        if (F.ftype == 0) {
+            if (NP > 1) fl &= ~FL_SKIP;
+            nonconv = (NP == 1) ? !(F.nrStaConflag || F.ssmStaConflag) : !(F.nrSpConflag || F.ssmSpConflag);
+        } else {
+            nonconv = !(F.nrSpConflag || F.ssmSpConflag);
+        }
+        W.flags[n] = (uint8_t)fl;
+        if (NP == 2) {
 #pragma unroll
-        for (int i = 0; i < nc; ++i) F.Ni[i] = B.Ni[i * ld + n];
-        F.Nh = 0;
+            for (int j = 0; j < 2; ++j) {
+                W.nu[j * ld + n] = F.nu[j];
 #pragma unroll
-        for (int i = 0; i < NC; ++i) F.Nh += fabs(F.Ni[i]);
-#pragma unroll
-        for (int i = 0; i < NC; ++i) F.zi[i] = F.Ni[i] / F.Nh;
-        double S[2] = {B.S[0 * ld + n], B.S[1 * ld + n]};
-        F.ftype = calFtypeFIM<NC>(F, B, n, S);
-        nonconv = F.phaseEquilibrium();
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-            if (j < F.NP) F.nu[j] = F.Nh * F.nu[j];
-        FlashKernel<NC>::finish(F, *eos, B, n, false, 0.0, 0.0);
-        if (B.ftype) B.ftype[n] = F.ftype;
+                for (int i = 0; i < NC; ++i) {
+                    W.x[(j * NC + i) * ld + n] = F.x[j][i];
+                    W.n[(j * NC + i) * ld + n] = F.n[j][i];
+                }
+            }
+        }
+        its[0] = F.it[IT_SSMSP];
+        its[1] = F.it[IT_NRSP];
+        its[2] = F.it[IT_RR];
         if (B.iters) {
-#pragma unroll
-            for (int q = 0; q < 5; ++q) B.iters[q * ld + n] = F.it[q];
+            B.iters[IT_SSMSP * ld + n] = its[0];
+            B.iters[IT_NRSP * ld + n] = its[1];
+            B.iters[IT_RR * ld + n] = its[2];
         }
     }
-    accumulateCounters(ctr, active, F.ftype, F.NP, F.it, nonconv);
-}
-
-template <int NC>
-__global__ void __launch_bounds__(kFlashBlock) initFlashKernel(const DevEos* __restrict__ eos, DevBulk B,
-                                                               const double* __restrict__ zin, int64_t zld)
-{
-    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= B.nb) return;
-    const int64_t ld = B.ld;
-    FlashCell<NC> F(*eos);
-#pragma unroll
-    for (int q = 0; q < 5; ++q) F.it[q] = 0;
-    F.ftype = 0;
-    F.lNP = 0;
-    F.P = B.P[n];
-    F.T = B.T[n] + kCONV5;
-#pragma unroll
-    for (int i = 0; i < NC; ++i) F.zi[i] = zin[i * zld + n];
-    F.phaseEquilibrium();
-    FlashKernel<NC>::finish(F, *eos, B, n, true, B.S[2 * ld + n], B.rockVp[n]);
-    if (B.ftype) B.ftype[n] = F.ftype;
-    if (B.iters) {
-#pragma unroll
-        for (int q = 0; q < 5; ++q) B.iters[q * ld + n] = F.it[q];
+    appendList(W, LIST_NP2, active && NP == 2, n);
+    appendList(W, LIST_NP1, active && NP == 1, n);
+    if (ctr) {
+        addCounter(&ctr->iters[IT_SSMSP], (unsigned long long)its[0]);
+        addCounter(&ctr->iters[IT_NRSP], (unsigned long long)its[1]);
+        addCounter(&ctr->iters[IT_RR], (unsigned long long)its[2]);
+        addCounter(&ctr->nonconverged, nonconv ? 1 : 0);
+        addCounter(&ctr->twoPhase, (active && NP == 2) ? 1 : 0);
     }
 }
 
-template <int NC>
-hipError_t launchFlashT(const DevEos* eos, const DevBulk& B, DevFlashCounters* ctr, hipStream_t s)
+// Stage 4: the rest of FlashFIM / InitFlashFIM for cells ending with NPT
+// hydrocarbon phases.  The EoS state the split ended with (Aj, Bj, Zj of the
+// final compositions) is re-derived from the stored compositions.
+template <int NC, int NPT>
+__global__ void __launch_bounds__(kFlashBlock) flashFinishKernel(const DevEos* __restrict__ eos, DevBulk B,
+                                                                 FlashWork W, int init)
 {
-    if (B.nb == 0) return hipSuccess;
-    const unsigned grid = (unsigned)((B.nb + kFlashBlock - 1) / kFlashBlock);
-    hipLaunchKernelGGL(flashFimKernel<NC>, dim3(grid), dim3(kFlashBlock), 0, s, eos, B, ctr);
-    return hipGetLastError();
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int list = (NPT == 1) ? LIST_NP1 : LIST_NP2;
+    if (idx >= W.count[list]) return;
+    const int32_t n = W.list[list][idx];
+    const int64_t ld = B.ld;
+    constexpr int nc = NC + 1;
+    FlashCell<NC> F(*eos);
+    loadFeed<NC>(F, B, W, n);
+    F.flagSkip = (W.flags[n] & FL_SKIP) != 0;
+    F.NP = NPT;
+    if (!init) {
+#pragma unroll
+        for (int i = 0; i < nc; ++i) F.Ni[i] = B.Ni[i * ld + n];
+        F.Nh = W.Nh[n];
+    }
+    F.calAiBi();
+    if (NPT == 1) {
+        F.nu[0] = 1;
+#pragma unroll
+        for (int i = 0; i < NC; ++i) F.x[0][i] = F.zi[i];
+    } else {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            F.nu[j] = W.nu[j * ld + n];
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                F.x[j][i] = W.x[(j * NC + i) * ld + n];
+                F.n[j][i] = W.n[(j * NC + i) * ld + n];
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NPT; ++j) {
+        F.calAjBj(F.Aj[j], F.Bj[j], F.x[j]);
+        F.Zj[j] = solEoS(F.Aj[j], F.Bj[j]);
+    }
+    if (!init) {
+#pragma unroll
+        for (int j = 0; j < NPT; ++j) F.nu[j] = F.Nh * F.nu[j];
+    }
+    const double Sw = init ? B.S[2 * ld + n] : 0.0;
+    const double Vp = init ? B.rockVp[n] : 0.0;
+    FlashKernel<NC>::template finish<NPT>(F, *eos, B, n, init != 0, Sw, Vp);
 }
 
 template <int NC>
-hipError_t launchInitFlashT(const DevEos* eos, const DevBulk& B, const double* z, int64_t zld, hipStream_t s)
+hipError_t launchFlashT(const DevEos* eos, const DevBulk& B, const FlashWork& W, DevFlashCounters* ctr,
+                        const double* zin, hipStream_t s)
 {
     if (B.nb == 0) return hipSuccess;
     const unsigned grid = (unsigned)((B.nb + kFlashBlock - 1) / kFlashBlock);
-    hipLaunchKernelGGL(initFlashKernel<NC>, dim3(grid), dim3(kFlashBlock), 0, s, eos, B, z, zld);
+    hipLaunchKernelGGL(flashResetKernel, dim3(1), dim3(64), 0, s, W, ctr);
+    hipLaunchKernelGGL(flashClassifyKernel<NC>, dim3(grid), dim3(kFlashBlock), 0, s, eos, B, W, ctr, zin);
+    hipLaunchKernelGGL(flashStabilityKernel<NC>, dim3(grid), dim3(kFlashBlock), 0, s, eos, B, W, ctr);
+    hipLaunchKernelGGL(flashSplitKernel<NC>, dim3(grid), dim3(kFlashBlock), 0, s, eos, B, W, ctr);
+    const int init = zin ? 1 : 0;
+    hipLaunchKernelGGL((flashFinishKernel<NC, 1>), dim3(grid), dim3(kFlashBlock), 0, s, eos, B, W, init);
+    hipLaunchKernelGGL((flashFinishKernel<NC, 2>), dim3(grid), dim3(kFlashBlock), 0, s, eos, B, W, init);
     return hipGetLastError();
 }
 

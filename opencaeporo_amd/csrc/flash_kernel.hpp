@@ -10,15 +10,15 @@ namespace ocpg {
 constexpr int kFlashBlock = 256;
 
 bool flashSupportsNC(int nc);
-hipError_t launchFlashFim(int NC, const DevEos* eos, const DevBulk& B, DevFlashCounters* ctr, hipStream_t s);
-hipError_t launchInitFlash(int NC, const DevEos* eos, const DevBulk& B, const double* z, int64_t zld,
-                           hipStream_t s);
+// FlashFIM over every cell (zin == NULL) or InitFlashFIM from the feed zin
+// (SoA, leading dimension B.ld).
+hipError_t launchFlash(int NC, const DevEos* eos, const DevBulk& B, const FlashWork& W, DevFlashCounters* ctr,
+                       const double* zin, hipStream_t s);
 
 }  // namespace ocpg
 
 namespace ocpg {
 template <int NC>
-hipError_t launchFlashT(const DevEos* eos, const DevBulk& B, DevFlashCounters* ctr, hipStream_t s);
-template <int NC>
-hipError_t launchInitFlashT(const DevEos* eos, const DevBulk& B, const double* z, int64_t zld, hipStream_t s);
+hipError_t launchFlashT(const DevEos* eos, const DevBulk& B, const FlashWork& W, DevFlashCounters* ctr,
+                        const double* zin, hipStream_t s);
 }  // namespace ocpg

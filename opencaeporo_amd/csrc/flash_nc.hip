@@ -7,7 +7,6 @@
 #endif
 
 namespace ocpg {
-template hipError_t launchFlashT<OCPG_NC>(const DevEos*, const DevBulk&, DevFlashCounters*, hipStream_t);
-template hipError_t launchInitFlashT<OCPG_NC>(const DevEos*, const DevBulk&, const double*, int64_t,
-                                               hipStream_t);
+template hipError_t launchFlashT<OCPG_NC>(const DevEos*, const DevBulk&, const FlashWork&, DevFlashCounters*,
+                                           const double*, hipStream_t);
 }  // namespace ocpg

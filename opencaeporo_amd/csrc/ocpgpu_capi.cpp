@@ -97,6 +97,10 @@ struct ocpg_ctx {
     size_t curBytes = 0;
     size_t fieldOff[kNumFields] = {};
     bool lastValid = false;
+    // staged-flash scratch
+    FlashWork work{};
+    char* workMem = nullptr;
+    size_t workBytes = 0;
     // connections and BSR (fim_kernels)
     FimDevice fim{};
     // copy of the flash inputs (ocpg_save_flash_inputs)
@@ -353,6 +357,7 @@ int ocpg_destroy(ocpg_ctx* c)
     timerClear(c);
     fimFree(c->fim);
     if (c->snap) (void)hipFree(c->snap);
+    if (c->workMem) (void)hipFree(c->workMem);
     if (c->cur) (void)hipFree(c->cur);
     if (c->last) (void)hipFree(c->last);
     if (c->dEos) (void)hipFree(c->dEos);
@@ -389,6 +394,36 @@ int ocpg_set_cells(ocpg_ctx* c, int64_t nb)
     c->curBytes = off;
     HIPCHK(hipMalloc(&c->cur, c->curBytes));
     HIPCHK(hipMalloc(&c->last, c->lastBytes));
+    {
+        // FlashWork: doubles zi(NC), Nh, Yt, x(2NC), n(2NC), nu(2); bytes
+        // ftype, lNP, flags; four int32 lists; four counters.
+        const size_t L = (size_t)(ld > 0 ? ld : 64);
+        const int NC = c->NC;
+        const size_t dbl[6] = {(size_t)NC, 1, 1, (size_t)2 * NC, (size_t)2 * NC, 2};
+        size_t w = 0;
+        size_t offD[6], offB[3], offL[4], offC;
+        for (int q = 0; q < 6; q++) { offD[q] = w; w += align256(dbl[q] * L * sizeof(double)); }
+        for (int q = 0; q < 3; q++) { offB[q] = w; w += align256(L); }
+        for (int q = 0; q < 4; q++) { offL[q] = w; w += align256(L * sizeof(int32_t)); }
+        offC = w;
+        w += 256;
+        if (c->workMem) HIPCHK(hipFree(c->workMem));
+        HIPCHK(hipMalloc(&c->workMem, w));
+        c->workBytes = w;
+        char* b = c->workMem;
+        c->work.zi = reinterpret_cast<double*>(b + offD[0]);
+        c->work.Nh = reinterpret_cast<double*>(b + offD[1]);
+        c->work.Yt = reinterpret_cast<double*>(b + offD[2]);
+        c->work.x = reinterpret_cast<double*>(b + offD[3]);
+        c->work.n = reinterpret_cast<double*>(b + offD[4]);
+        c->work.nu = reinterpret_cast<double*>(b + offD[5]);
+        c->work.ftype = reinterpret_cast<int8_t*>(b + offB[0]);
+        c->work.lNP = reinterpret_cast<int8_t*>(b + offB[1]);
+        c->work.flags = reinterpret_cast<uint8_t*>(b + offB[2]);
+        for (int q = 0; q < 4; q++) c->work.list[q] = reinterpret_cast<int32_t*>(b + offL[q]);
+        c->work.count = reinterpret_cast<int32_t*>(b + offC);
+        HIPCHK(hipMemsetAsync(c->workMem, 0, w, c->stream));
+    }
     HIPCHK(hipMemsetAsync(c->cur, 0, c->curBytes, c->stream));
     HIPCHK(hipMemsetAsync(c->last, 0, c->lastBytes, c->stream));
     std::memset(&c->B, 0, sizeof(DevBulk));
@@ -470,9 +505,8 @@ int ocpg_flash_fim(ocpg_ctx* c, ocpg_flash_stats* stats)
 {
     if (!c || !c->cur) return OCPG_ESTATE;
     HIPCHK(hipSetDevice(c->device));
-    if (stats) HIPCHK(hipMemsetAsync(c->dCtr, 0, sizeof(DevFlashCounters), c->stream));
     timerBegin(c, T_FLASH);
-    HIPCHK(launchFlashFim(c->NC, c->dEos, c->B, stats ? c->dCtr : nullptr, c->stream));
+    HIPCHK(launchFlash(c->NC, c->dEos, c->B, c->work, stats ? c->dCtr : nullptr, nullptr, c->stream));
     timerEnd(c, T_FLASH);
     if (stats) {
         DevFlashCounters h;
@@ -503,7 +537,7 @@ int ocpg_init_flash_fim(ocpg_ctx* c, const double* zi)
     HIPCHK(hipMalloc(&dz, t.size() * sizeof(double)));
     hipError_t e = hipMemcpy(dz, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice);
     timerBegin(c, T_INIT);
-    if (e == hipSuccess) e = launchInitFlash(c->NC, c->dEos, c->B, dz, ld, c->stream);
+    if (e == hipSuccess) e = launchFlash(c->NC, c->dEos, c->B, c->work, nullptr, dz, c->stream);
     timerEnd(c, T_INIT);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(dz);
@@ -619,7 +653,7 @@ int ocpg_synchronize(ocpg_ctx* c)
 void* ocpg_stream(ocpg_ctx* c) { return c ? (void*)c->stream : nullptr; }
 int64_t ocpg_device_bytes(ocpg_ctx* c)
 {
-    return c ? (int64_t)(c->curBytes + c->lastBytes + fimBytes(c->fim)) : -1;
+    return c ? (int64_t)(c->curBytes + c->lastBytes + c->workBytes + c->snapBytes + fimBytes(c->fim)) : -1;
 }
 
 }  // extern "C"
