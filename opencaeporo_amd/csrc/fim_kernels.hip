@@ -239,8 +239,18 @@ __global__ void __launch_bounds__(kResBlock) calResKernel(DevBulk B, FimDevice F
 // reference's connection order) and writes the off-diagonal into the
 // neighbour's row at the slot the reference would have filled.  Every block
 // is computed once, nothing is accumulated across waves, no atomics.
+//
+// Memory plan: the cell values a connection side needs (its upwind phase
+// properties, pressures, depth, phase flags) form a record of AsmRec::LEN
+// doubles.  The wavefront gathers a record with one load per lane (each lane
+// owns a fixed record entry), keeps the own cell's record and the current
+// neighbour's in LDS, and prefetches the next neighbour's record into
+// registers while it works on the current connection.  Rows are handed to
+// workgroups so that each XCD walks a contiguous range of cells, which keeps
+// the neighbour records (cells n +- 1, n +- nx, ...) in that XCD's L2.
 // ---------------------------------------------------------------------------
 constexpr int kAsmBlock = 64;
+constexpr int kXcds = 8;
 
 template <int NC>
 struct AsmDims {
@@ -249,27 +259,104 @@ struct AsmDims {
     static constexpr int bsize = ncol * ncol;
     static constexpr int maxRows = 3 + 2 * NC;          // compact secondary rows
     static constexpr int perLane = (bsize + kAsmBlock - 1) / kAsmBlock;
-    static constexpr int dfEntries = nc * (maxRows + 1);  // P column + secondary columns
+    static constexpr int dfStride = maxRows + 1;        // [i][0] = dFdXp, [i][1+k] = dFdXs
 };
+
+// Entry offsets of the per-cell record (phase j, component i).
+template <int NC>
+struct AsmRec {
+    static constexpr int nc = NC + 1;
+    static constexpr int PJ = 0, XI = 3, KR = 6, MU = 9, MUP = 12, XIP = 15, RHOP = 18;
+    static constexpr int DKR = 21;  // dKr_dS[j*3+k]
+    static constexpr int XIJ = 30;  // xij[j*nc+i]
+    static constexpr int RHOX = XIJ + 3 * nc, XIX = RHOX + 3 * nc, MUX = XIX + 3 * nc;
+    static constexpr int DEPTH = MUX + 3 * nc, EX = DEPTH + 1, LEN = EX + 3;
+    static constexpr int perLane = (LEN + kAsmBlock - 1) / kAsmBlock;
+};
+
+// Where record entry q lives: a double field advanced to its entry, or (for
+// the phase flags) a byte field.
+struct RecSrc {
+    const double* d;
+    const uint8_t* b;
+};
+
+template <int NC>
+__device__ __forceinline__ RecSrc recSource(const DevBulk& B, int q)
+{
+    using R = AsmRec<NC>;
+    const int64_t ld = B.ld;
+    RecSrc s{nullptr, nullptr};
+    if (q < R::XI) s.d = B.Pj + (int64_t)(q - R::PJ) * ld;
+    else if (q < R::KR) s.d = B.xi + (int64_t)(q - R::XI) * ld;
+    else if (q < R::MU) s.d = B.kr + (int64_t)(q - R::KR) * ld;
+    else if (q < R::MUP) s.d = B.mu + (int64_t)(q - R::MU) * ld;
+    else if (q < R::XIP) s.d = B.muP + (int64_t)(q - R::MUP) * ld;
+    else if (q < R::RHOP) s.d = B.xiP + (int64_t)(q - R::XIP) * ld;
+    else if (q < R::DKR) s.d = B.rhoP + (int64_t)(q - R::RHOP) * ld;
+    else if (q < R::XIJ) s.d = B.dKr_dS + (int64_t)(q - R::DKR) * ld;
+    else if (q < R::RHOX) s.d = B.xij + (int64_t)(q - R::XIJ) * ld;
+    else if (q < R::XIX) s.d = B.rhox + (int64_t)(q - R::RHOX) * ld;
+    else if (q < R::MUX) s.d = B.xix + (int64_t)(q - R::XIX) * ld;
+    else if (q < R::DEPTH) s.d = B.mux + (int64_t)(q - R::MUX) * ld;
+    else if (q < R::EX) s.d = B.depth;
+    else if (q < R::LEN) s.b = B.phaseExist + (int64_t)(q - R::EX) * ld;
+    return s;
+}
+
+__device__ __forceinline__ double loadRec(const RecSrc& s, int64_t cell)
+{
+    if (s.d) return s.d[cell];
+    if (s.b) return s.b[cell] ? 1.0 : 0.0;
+    return 0.0;
+}
 
 template <int NC>
 __global__ void __launch_bounds__(kAsmBlock) assembleKernel(DevBulk B, FimDevice F, double dt)
 {
     using D = AsmDims<NC>;
+    using R = AsmRec<NC>;
     constexpr int nc = D::nc;
     constexpr int ncol = D::ncol;
     constexpr int bsize = D::bsize;
-    const int64_t n = blockIdx.x;
+    constexpr int W = D::dfStride;
+    // workgroup b runs on XCD b % kXcds; XCD x takes rows [x*chunk, (x+1)*chunk)
+    const int64_t chunk = gridDim.x / kXcds;
+    const int64_t n = (int64_t)(blockIdx.x % kXcds) * chunk + blockIdx.x / kXcds;
     if (n >= B.nb) return;
     const int64_t ld = B.ld;
     const int lane = threadIdx.x;
 
     __shared__ double sSec[D::maxRows * ncol];  // own dSec_dPri (compact rows x ncol)
-    __shared__ double sDF[D::dfEntries];        // own side: [i][0] = dFdXp, [i][1+k] = dFdXs
+    __shared__ double sDF[nc * W];              // own side's dFdXp | dFdXs
+    __shared__ double sRec[2][R::LEN];          // [0] own cell, [1] current neighbour
+    __shared__ double sDpc[9];                  // own dPcj_dS
 
-    // own compact derivative block
+    RecSrc src[R::perLane];
+#pragma unroll
+    for (int t = 0; t < R::perLane; ++t) src[t] = recSource<NC>(B, lane + t * kAsmBlock);
+
+#pragma unroll
+    for (int t = 0; t < R::perLane; ++t) {
+        const int q = lane + t * kAsmBlock;
+        if (q < R::LEN) sRec[0][q] = loadRec(src[t], n);
+    }
     const int rowsOwn = min(max(B.bRowSizedSdP[n], 0), D::maxRows);
     for (int q = lane; q < rowsOwn * ncol; q += kAsmBlock) sSec[q] = B.dSec[(int64_t)q * ld + n];
+    if (lane < 9) sDpc[lane] = B.dPcj_dS[(int64_t)lane * ld + n];
+
+    // secondary columns of this row's cell: existing saturations, then the
+    // x_ij of each phase with derivatives
+    bool sdO[3];
+    int pvO[3];
+    int nsat = 0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        sdO[j] = B.pSderExist[j * ld + n];
+        pvO[j] = min(max(B.pVnumCom[j * ld + n], 0), NC);
+        nsat += sdO[j] ? 1 : 0;
+    }
+    const int ncolO = min(nsat + pvO[0] + pvO[1] + pvO[2], D::maxRows);
 
     // diagonal starts as the accumulation block (src/OCPFluidMethod.cpp:1586-1597)
     double acc[D::perLane];
@@ -288,32 +375,43 @@ __global__ void __launch_bounds__(kAsmBlock) assembleKernel(DevBulk B, FimDevice
         }
         acc[t] = v;
     }
-    __syncthreads();
 
     const int k0 = F.IA[n] + 1, k1 = F.IA[n + 1];
+    double pre[R::perLane];
+    if (k0 < k1) {
+        const int64_t m = F.JA[k0];
+#pragma unroll
+        for (int t = 0; t < R::perLane; ++t) pre[t] = loadRec(src[t], m);
+    }
     for (int k = k0; k < k1; ++k) {
         const int64_t m = F.JA[k];
         const int64_t c = F.slotConn[k];
+#pragma unroll
+        for (int t = 0; t < R::perLane; ++t) {
+            const int q = lane + t * kAsmBlock;
+            if (q < R::LEN) sRec[1][q] = pre[t];
+        }
+        __syncthreads();
+        if (k + 1 < k1) {
+            const int64_t m2 = F.JA[k + 1];
+#pragma unroll
+            for (int t = 0; t < R::perLane; ++t) pre[t] = loadRec(src[t], m2);
+        }
         const bool ownB = n < m;  // this row is the connection's bId side
-        const int64_t bId = ownB ? n : m;
-        const int64_t eId = ownB ? m : n;
+        const double* rB = ownB ? sRec[0] : sRec[1];
+        const double* rE = ownB ? sRec[1] : sRec[0];
+        const double* rO = sRec[0];
         const double Akd = kCONV1f * kCONV2f * F.area[c];
-        const double dGamma = kGravityFactor * (B.depth[bId] - B.depth[eId]);
-        bool exB[3], exE[3], sdO[3];
-        int pvB[3], pvE[3];
+        const double dGamma = kGravityFactor * (rB[R::DEPTH] - rE[R::DEPTH]);
+        bool exB[3], exE[3], upOwn[3];
+        double upRho[3];
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            exB[j] = B.phaseExist[j * ld + bId];
-            exE[j] = B.phaseExist[j * ld + eId];
-            sdO[j] = B.pSderExist[j * ld + n];
-            pvB[j] = min(max(B.pVnumCom[j * ld + bId], 0), NC);
-            pvE[j] = min(max(B.pVnumCom[j * ld + eId], 0), NC);
+            exB[j] = rB[R::EX + j] != 0.0;
+            exE[j] = rE[R::EX + j] != 0.0;
+            upOwn[j] = F.upblock[c * 3 + j] == (int32_t)n;
+            upRho[j] = F.upRho[c * 3 + j];
         }
-        int nsat = 0;
-#pragma unroll
-        for (int j = 0; j < 3; ++j) nsat += sdO[j] ? 1 : 0;
-        const int* pvO = ownB ? pvB : pvE;
-        const int ncolO = min(nsat + pvO[0] + pvO[1] + pvO[2], D::maxRows);
 
         // ---- own side's dFdXp column and dFdXs, entry by entry ----
         for (int q = lane; q < nc * (ncolO + 1); q += kAsmBlock) {
@@ -343,22 +441,23 @@ __global__ void __launch_bounds__(kAsmBlock) assembleKernel(DevBulk B, FimDevice
             double v = 0;
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
-                const int64_t uId = F.upblock[c * 3 + j];
-                const bool exU = (uId == bId) ? exB[j] : exE[j];
+                // the upwind cell is bId or eId; upB: it is bId
+                const bool upB = ownB ? upOwn[j] : !upOwn[j];
+                const bool exU = upB ? exB[j] : exE[j];
                 if (!exU) continue;
-                const bool upB = (uId == bId);
-                const double dP = B.Pj[j * ld + bId] - B.Pj[j * ld + eId] - F.upRho[c * 3 + j] * dGamma;
-                const double xi = B.xi[j * ld + uId];
-                const double kr = B.kr[j * ld + uId];
-                const double mu = B.mu[j * ld + uId];
+                const double* rU = upOwn[j] ? sRec[0] : sRec[1];
+                const double dP = rB[R::PJ + j] - rE[R::PJ + j] - upRho[j] * dGamma;
+                const double xi = rU[R::XI + j];
+                const double kr = rU[R::KR + j];
+                const double mu = rU[R::MU + j];
                 const double transJ = Akd * kr / mu;
-                const double xij = B.xij[(j * nc + i) * ld + uId];
+                const double xij = rU[R::XIJ + j * nc + i];
                 const double transIJ = xij * xi * transJ;
                 if (col < 0) {
                     // pressure column
-                    const double muP = B.muP[j * ld + uId];
-                    const double xiP = B.xiP[j * ld + uId];
-                    const double rhoP = B.rhoP[j * ld + uId];
+                    const double muP = rU[R::MUP + j];
+                    const double xiP = rU[R::XIP + j];
+                    const double rhoP = rU[R::RHOP + j];
                     double tmp = xij * transJ * xiP * dP;
                     tmp += -transIJ * muP / mu * dP;
                     if (ownB) {
@@ -368,7 +467,7 @@ __global__ void __launch_bounds__(kAsmBlock) assembleKernel(DevBulk B, FimDevice
                             v += tmp;
                         } else if (!exB[j]) {
                         } else {
-                            v += transIJ * (-B.rhoP[j * ld + bId] * dGamma) / 2;
+                            v += transIJ * (-rO[R::RHOP + j] * dGamma) / 2;
                             if (upB) v += tmp;
                         }
                     } else {
@@ -378,19 +477,19 @@ __global__ void __launch_bounds__(kAsmBlock) assembleKernel(DevBulk B, FimDevice
                             tmp += transIJ * (-rhoP * dGamma);
                             v += tmp;
                         } else {
-                            v += transIJ * (-B.rhoP[j * ld + eId] * dGamma) / 2;
+                            v += transIJ * (-rO[R::RHOP + j] * dGamma) / 2;
                             if (!upB) v += tmp;
                         }
                     }
                 } else if (j1S >= 0) {
                     // saturation column of phase j1S
-                    const double dpc = B.dPcj_dS[(j * 3 + j1S) * ld + n];
+                    const double dpc = sDpc[j * 3 + j1S];
                     if (ownB) {
                         v += transIJ * dpc;
-                        if (upB) v += Akd * xij * xi / mu * B.dKr_dS[(j * 3 + j1S) * ld + uId] * dP;
+                        if (upB) v += Akd * xij * xi / mu * rU[R::DKR + j * 3 + j1S] * dP;
                     } else {
                         v -= transIJ * dpc;
-                        if (!upB) v += Akd * xij * xi / mu * B.dKr_dS[(j * 3 + j1S) * ld + uId] * dP;
+                        if (!upB) v += Akd * xij * xi / mu * rU[R::DKR + j * 3 + j1S] * dP;
                     }
                 } else if (jX == j) {
                     // x_{kX, j} column (only phase j's own block is touched in pass j)
@@ -398,21 +497,20 @@ __global__ void __launch_bounds__(kAsmBlock) assembleKernel(DevBulk B, FimDevice
                     const bool ownUp = ownB ? upB : !upB;
                     const bool otherEx = ownB ? exE[j] : exB[j];
                     if (ownUp) {
-                        const double rhox = otherEx ? B.rhox[(j * nc + kk) * ld + n] / 2
-                                                    : B.rhox[(j * nc + kk) * ld + uId];
-                        const double xix = B.xix[(j * nc + kk) * ld + uId];
-                        const double mux = B.mux[(j * nc + kk) * ld + uId];
+                        const double rhox = otherEx ? rO[R::RHOX + j * nc + kk] / 2 : rU[R::RHOX + j * nc + kk];
+                        const double xix = rU[R::XIX + j * nc + kk];
+                        const double mux = rU[R::MUX + j * nc + kk];
                         double tmp = -transIJ * rhox * dGamma;
                         tmp += xij * transJ * xix * dP;
                         tmp += -transIJ * mux / mu * dP;
                         v += tmp;
                         if (kk == i) v += xi * transJ * dP;
                     } else if (otherEx) {
-                        v += -transIJ * B.rhox[(j * nc + kk) * ld + n] / 2 * dGamma;
+                        v += -transIJ * rO[R::RHOX + j * nc + kk] / 2 * dGamma;
                     }
                 }
             }
-            sDF[i * (D::maxRows + 1) + col + 1] = v;
+            sDF[i * W + col + 1] = v;
         }
         __syncthreads();
 
@@ -426,7 +524,7 @@ __global__ void __launch_bounds__(kAsmBlock) assembleKernel(DevBulk B, FimDevice
                 const int r = e / ncol, cc = e % ncol;
                 double v = 0;
                 if (r > 0) {
-                    const double* row = &sDF[(r - 1) * (D::maxRows + 1)];
+                    const double* row = &sDF[(r - 1) * W];
                     v = (cc == 0) ? row[0] : 0.0;
                     for (int kq = 0; kq < ncolO; ++kq) v = v + row[1 + kq] * sSec[kq * ncol + cc];
                 }
@@ -440,7 +538,8 @@ __global__ void __launch_bounds__(kAsmBlock) assembleKernel(DevBulk B, FimDevice
                 }
             }
         }
-        __syncthreads();
+        // the next iteration overwrites sRec[1] (read only before the barrier
+        // above) and sDF (after its own barrier)
     }
     double* diag = F.val + (int64_t)F.IA[n] * bsize;
 #pragma unroll
@@ -461,7 +560,8 @@ static hipError_t launchCalResT(const DevBulk& B, FimDevice& F, double dt, hipSt
 template <int NC>
 static hipError_t launchAssembleT(const DevBulk& B, FimDevice& F, double dt, hipStream_t s)
 {
-    hipLaunchKernelGGL(assembleKernel<NC>, dim3((unsigned)B.nb), dim3(kAsmBlock), 0, s, B, F, dt);
+    const unsigned grid = (unsigned)(((B.nb + kXcds - 1) / kXcds) * kXcds);
+    hipLaunchKernelGGL(assembleKernel<NC>, dim3(grid), dim3(kAsmBlock), 0, s, B, F, dt);
     return hipGetLastError();
 }
 

@@ -19,6 +19,7 @@
 
 #include "dense_solve.hpp"
 #include "device_state.hpp"
+#include "flash_kernel.hpp"
 
 namespace ocpg {
 
@@ -132,7 +133,9 @@ struct FlashCell {
     bool flagSkip;
     // EoS
     double Ai[NC], Bi[NC];
-    double sqA[NT];  // sqrt(Ai[i]*Ai[k]), k <= i
+    // sqrt(Ai[i]*Ai[k]) for k <= i: NT values in LDS, this lane's entry t at
+    // sqA[t * kFlashBlock] (the block's lanes interleaved, no bank conflicts)
+    double* sqA;
     double Kw0[NC];  // Wilson K (Kw[0] of the reference)
     double Aj[2], Bj[2], Zj[2];
     double x[2][NC], phi[2][NC], fug[2][NC], n[2][NC];
@@ -145,10 +148,11 @@ struct FlashCell {
     bool ssmStaConflag, nrStaConflag, ssmSpConflag, nrSpConflag;
     int it[5];
 
-    __device__ __forceinline__ explicit FlashCell(const DevEos& e) : E(e) {}
+    __device__ __forceinline__ explicit FlashCell(const DevEos& e, double* sqLds = nullptr) : E(e), sqA(sqLds) {}
 
     __device__ __forceinline__ static constexpr int tri(int i, int k) { return i * (i + 1) / 2 + k; }
-    __device__ __forceinline__ double sq(int i, int k) const { return i >= k ? sqA[tri(i, k)] : sqA[tri(k, i)]; }
+    __device__ __forceinline__ double& sqE(int t) const { return sqA[t * kFlashBlock]; }
+    __device__ __forceinline__ double sq(int i, int k) const { return i >= k ? sqE(tri(i, k)) : sqE(tri(k, i)); }
     __device__ __forceinline__ double omb(int i, int k) const { return E.oneMinusBIC[i * NC + k]; }
 
     // MixtureComp::CalAiBi, src/MixtureComp.cpp:916-937 (+ the sqrt(Ai*Ak)
@@ -173,7 +177,7 @@ struct FlashCell {
 #pragma unroll
         for (int i = 0; i < NC; ++i)
 #pragma unroll
-            for (int k = 0; k <= i; ++k) sqA[tri(i, k)] = sqrt(Ai[i] * Ai[k]);
+            for (int k = 0; k <= i; ++k) sqE(tri(i, k)) = sqrt(Ai[i] * Ai[k]);
     }
 
     // MixtureComp::CalAjBj, src/MixtureComp.cpp:955-969.
@@ -186,7 +190,7 @@ struct FlashCell {
             BjT += Bi[i1] * xj[i1];
             AjT += xj[i1] * xj[i1] * Ai[i1] * omb(i1, i1);
 #pragma unroll
-            for (int i2 = 0; i2 < i1; ++i2) AjT += 2 * xj[i1] * xj[i2] * sqA[tri(i1, i2)] * omb(i1, i2);
+            for (int i2 = 0; i2 < i1; ++i2) AjT += 2 * xj[i1] * xj[i2] * sqE(tri(i1, i2)) * omb(i1, i2);
         }
     }
 

@@ -552,6 +552,8 @@ __device__ __forceinline__ void addCounter(unsigned long long* dst, unsigned lon
     if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, v);
 }
 
+// (a template so each per-NC translation unit has its own symbol)
+template <int NC>
 __global__ void flashResetKernel(FlashWork W, DevFlashCounters* ctr)
 {
     if (threadIdx.x < 4) W.count[threadIdx.x] = 0;
@@ -633,6 +635,7 @@ template <int NC>
 __global__ void __launch_bounds__(kFlashBlock) flashStabilityKernel(const DevEos* __restrict__ eos, DevBulk B,
                                                                     FlashWork W, DevFlashCounters* ctr)
 {
+    __shared__ double sSq[FlashCell<NC>::NT * kFlashBlock];
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = idx < W.count[LIST_STAB];
     bool stable = false;
@@ -642,7 +645,7 @@ __global__ void __launch_bounds__(kFlashBlock) flashStabilityKernel(const DevEos
     if (active) {
         n = W.list[LIST_STAB][idx];
         const int64_t ld = B.ld;
-        FlashCell<NC> F(*eos);
+        FlashCell<NC> F(*eos, &sSq[threadIdx.x]);
         loadFeed<NC>(F, B, W, n);
 #pragma unroll
         for (int q = 0; q < 5; ++q) F.it[q] = 0;
@@ -677,6 +680,7 @@ template <int NC>
 __global__ void __launch_bounds__(kFlashBlock) flashSplitKernel(const DevEos* __restrict__ eos, DevBulk B,
                                                                 FlashWork W, DevFlashCounters* ctr)
 {
+    __shared__ double sSq[FlashCell<NC>::NT * kFlashBlock];
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = idx < W.count[LIST_SPLIT];
     int NP = 1;
@@ -687,7 +691,7 @@ __global__ void __launch_bounds__(kFlashBlock) flashSplitKernel(const DevEos* __
         n = W.list[LIST_SPLIT][idx];
         const int64_t ld = B.ld;
         constexpr int nc = NC + 1;
-        FlashCell<NC> F(*eos);
+        FlashCell<NC> F(*eos, &sSq[threadIdx.x]);
         loadFeed<NC>(F, B, W, n);
 #pragma unroll
         for (int q = 0; q < 5; ++q) F.it[q] = 0;
@@ -749,13 +753,14 @@ template <int NC, int NPT>
 __global__ void __launch_bounds__(kFlashBlock) flashFinishKernel(const DevEos* __restrict__ eos, DevBulk B,
                                                                  FlashWork W, int init)
 {
+    __shared__ double sSq[FlashCell<NC>::NT * kFlashBlock];
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     const int list = (NPT == 1) ? LIST_NP1 : LIST_NP2;
     if (idx >= W.count[list]) return;
     const int32_t n = W.list[list][idx];
     const int64_t ld = B.ld;
     constexpr int nc = NC + 1;
-    FlashCell<NC> F(*eos);
+    FlashCell<NC> F(*eos, &sSq[threadIdx.x]);
     loadFeed<NC>(F, B, W, n);
     F.flagSkip = (W.flags[n] & FL_SKIP) != 0;
     F.NP = NPT;
@@ -800,7 +805,7 @@ hipError_t launchFlashT(const DevEos* eos, const DevBulk& B, const FlashWork& W,
 {
     if (B.nb == 0) return hipSuccess;
     const unsigned grid = (unsigned)((B.nb + kFlashBlock - 1) / kFlashBlock);
-    hipLaunchKernelGGL(flashResetKernel, dim3(1), dim3(64), 0, s, W, ctr);
+    hipLaunchKernelGGL(flashResetKernel<NC>, dim3(1), dim3(64), 0, s, W, ctr);
     hipLaunchKernelGGL(flashClassifyKernel<NC>, dim3(grid), dim3(kFlashBlock), 0, s, eos, B, W, ctr, zin);
     hipLaunchKernelGGL(flashStabilityKernel<NC>, dim3(grid), dim3(kFlashBlock), 0, s, eos, B, W, ctr);
     hipLaunchKernelGGL(flashSplitKernel<NC>, dim3(grid), dim3(kFlashBlock), 0, s, eos, B, W, ctr);

@@ -7,7 +7,7 @@
 
 namespace ocpg {
 
-constexpr int kFlashBlock = 256;
+constexpr int kFlashBlock = 128;
 
 bool flashSupportsNC(int nc);
 // FlashFIM over every cell (zin == NULL) or InitFlashFIM from the feed zin

@@ -13,12 +13,20 @@ def say(*a):
 
 
 # Floating-point parity of the HIP path against the FP64 oracle: per-cell
-# scaled error 1e-10 in at least 99% of cells; the remainder are near-critical
-# cells whose split Newton system is ill-conditioned (the same cells where the
-# oracle and the reference's MKL-backed flash differ), bounded loosely.
+# scaled error 1e-10, and in every cell at most max(1e-10, SENS_FACTOR x the
+# cell's rounding sensitivity) (parity.rounding_sensitivity: the oracle's own
+# response to a 1-ulp change of P).  Near-critical cells amplify last-bit
+# differences (GPU libm vs glibc) far beyond 1e-10; the oracle and the
+# reference's MKL-backed flash differ there by the same amount.  At least
+# GPU_FRAC of the cells must meet the flat 1e-10, and all of them GPU_LOOSE.
 GPU_TOL = 1e-10
-GPU_FRAC = 0.99
+GPU_FRAC = 0.9
 GPU_LOOSE = 1e-4
+SENS_FACTOR = 10.0
+
+
+def cell_bounds(sens):
+    return np.maximum(GPU_TOL, SENS_FACTOR * sens)
 
 
 def gpu_flash(ctx, bulk, init_z=None):

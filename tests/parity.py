@@ -92,11 +92,59 @@ def scaled_err(a, ref):
     return (np.abs(a - ref) / scale).max(axis=1)
 
 
-def compare_state(b, d, prefix, tol, int_exact=True, skip_tol=1e-4, frac=1.0, loose=None):
+def worst_scaled_err(b, d, prefix):
+    """Per cell: the largest scaled error over the floating state fields."""
+    worst = np.zeros(b.nb)
+    for f in FLOAT_FIELDS:
+        ref = d[prefix + f]
+        got = field(b, f, used=ref.shape[1] if f == "dSec_dPri" else None)
+        worst = np.maximum(worst, scaled_err(got, ref.reshape(b.nb, -1)))
+    return worst
+
+
+# (input, relative perturbation) pairs of rounding_sensitivity: a few units in
+# the last place of P, T, the first component's moles and all moles.
+SENS_PERTURBATIONS = [("P", 2e-16), ("P", -2e-16), ("P", 1e-15), ("P", -1e-15), ("T", 1e-15), ("T", -1e-15),
+                      ("N0", 2e-16), ("N0", -2e-16), ("N", 2e-16), ("N", -2e-16)]
+
+
+def rounding_sensitivity(flash, bulk, perturbations=SENS_PERTURBATIONS):
+    """Per cell: how far the FP64 flash's outputs move when its inputs (P, T,
+    moles) move by a few units in the last place.  ``flash(b)`` runs the
+    flash in place on a copy of ``bulk``.  A cell whose phase count changes
+    under a perturbation gets +inf.  This is the cell's conditioning: two
+    correct FP64 implementations that round differently (other libm, other
+    LAPACK) differ by about this much, whatever their quality."""
+    from tests import gpu_util
+    base = bulk.copy()
+    flash(base)
+    ref = gpu_util.snapshot_dict(base, "s_")
+    sens = np.zeros(bulk.nb)
+    for what, e in perturbations:
+        p = bulk.copy()
+        if what == "P":
+            p.P[:] = bulk.P * (1 + e)
+        elif what == "T":
+            p.T[:] = bulk.T * (1 + e)
+        else:
+            Ni = p.Ni.reshape(bulk.nb, bulk.nc)
+            if what == "N0":
+                Ni[:, 0] *= 1 + e
+            else:
+                Ni[:, :] *= 1 + e
+        flash(p)
+        w = worst_scaled_err(p, ref, "s_")
+        w[(field(p, "phaseNum") != field(base, "phaseNum")).any(axis=1)] = np.inf
+        sens = np.maximum(sens, w)
+    return sens
+
+
+def compare_state(b, d, prefix, tol, int_exact=True, skip_tol=1e-4, frac=1.0, loose=None, cell_tol=None):
     """Integer fields must match in every cell.  A floating field must be
     within ``tol`` (per-cell scaled error) in at least ``frac`` of the cells
-    and within ``loose`` everywhere (``loose`` defaults to ``tol``).
-    Returns a report dict; raises AssertionError on a breach."""
+    and within ``loose`` everywhere (``loose`` defaults to ``tol``).  With
+    ``cell_tol`` (one bound per cell), every cell must also be within its
+    own bound.  Returns a report dict; raises AssertionError on a breach."""
     loose = tol if loose is None else loose
     report = {}
     worst = np.zeros(b.nb)
@@ -118,6 +166,11 @@ def compare_state(b, d, prefix, tol, int_exact=True, skip_tol=1e-4, frac=1.0, lo
     report["frac_within_tol"] = within
     report["cells_beyond_tol"] = [int(i) for i in np.flatnonzero(worst > tol)]
     assert within >= frac, f"only {within:.4f} of cells within {tol:.0e} (need {frac})"
+    if cell_tol is not None:
+        over = np.flatnonzero(worst > cell_tol)
+        report["cells_over_own_bound"] = [int(i) for i in over]
+        assert len(over) == 0, (f"{len(over)} cells beyond their conditioning bound, first {over[:5]}: "
+                                f"err {worst[over[:5]]} bound {cell_tol[over[:5]]}")
     ref = d[prefix + "skipMinEig"]
     got = b.skipMinEig
     err = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-30)

@@ -77,3 +77,22 @@ def test_report_levels(orc):
             print(name, phase, f"worst scaled err {worst:.2e}",
                   f"within 1e-8: {rep['frac_within_tol']:.4f}", "iter-mismatch cells",
                   rep["iters_mismatch_cells"], "skipMinEig", f"{rep['skipMinEig']:.1e}")
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("phase", ["cold", "warm"])
+def test_reference_within_conditioning_bound(orc, name, phase):
+    """The per-cell bound the GPU parity tests use, max(1e-10, 10 x rounding
+    sensitivity), holds between the oracle and the reference's own flash (a
+    different LAPACK, the same libm) in every cell, and the sensitivity
+    itself is tiny in most cells (the bound is not a loose catch-all)."""
+    from tests import gpu_util
+    d = parity.load(name)
+    eos = parity.eos_from_fixture(d)
+    b = parity.bulk_from_fixture(d, phase)
+    sens = parity.rounding_sensitivity(lambda x: orc.flash_fim(eos, x), b)
+    assert np.quantile(sens, 0.9) < 1e-10 and np.median(sens) < 1e-12
+    o = b.copy()
+    orc.flash_fim(eos, o)
+    parity.compare_state(o, d, f"{phase}_out_", gpu_util.GPU_TOL, frac=gpu_util.GPU_FRAC,
+                         loose=gpu_util.GPU_LOOSE, cell_tol=gpu_util.cell_bounds(sens))
