@@ -186,7 +186,38 @@ typedef struct ocpg_resnorm {
     double maxRelRes_N;
     int64_t maxId_V;
     int64_t maxId_N;
+    double maxWellRelRes_mol;   /* over open rate-controlled wells (:1318-1352) */
 } ocpg_resnorm;
+
+/* Wells as IsoT_FIM reads them (include/Well.hpp, include/WellOpt.hpp,
+ * include/WellPerf.hpp).  Per-step inputs the reference computes once per
+ * time step in AllWells::PrepareWell (src/AllWells.cpp:201-214) come in:
+ * the perforation pressure offsets dG (Well::CaldG), the injected fluid's
+ * molar density at each perforation (MixtureComp::XiPhase in Well::CalFlux
+ * with ReCalXi) and the control mode after Well::CheckOptMode.  Arrays are
+ * well-major; perforations of well w are perfStart[w] .. perfStart[w+1]-1
+ * in the reference's perforation order.                                      */
+enum { OCPG_WELL_INJ = 0, OCPG_WELL_PROD = 1 };
+enum { OCPG_WELL_BHP = 0, OCPG_WELL_RATE = 1 };  /* RATE: RATE/ORATE/GRATE/WRATE/LRATE */
+typedef struct ocpg_wells {
+    int32_t nw;                      /* wells                                  */
+    const int32_t* type;             /* nw: OCPG_WELL_INJ / OCPG_WELL_PROD     */
+    const int32_t* open;             /* nw: nonzero when open (Well::IsOpen)   */
+    const int32_t* mode;             /* nw: OCPG_WELL_BHP / OCPG_WELL_RATE     */
+    const int32_t* unweight;         /* nw: Well::ifUseUnweight (injectors)    */
+    const double* bhp;               /* nw: Well::bhp, the well unknown        */
+    const double* ctrlBHP;           /* nw: maxBHP (injector) / minBHP (producer) */
+    const double* ctrlRate;          /* nw: WellOpt::maxRate                   */
+    const double* injZi;             /* nw*nc: WellOpt::injZi (injectors)      */
+    const double* prodPhaseWeight;   /* nw*3: WellOpt::prodPhaseWeight         */
+    const int32_t* perfStart;        /* nw+1                                   */
+    const int32_t* perfCell;         /* perforated cell (WellPerf::location)   */
+    const double* perfWI;            /* WellPerf::WI                           */
+    const double* perfMult;          /* WellPerf::multiplier                   */
+    const double* perfDG;            /* Well::dG[p]                            */
+    const double* perfXi;            /* WellPerf::xi (injectors)               */
+    double Psurf, Tsurf;             /* Well::Psurf, Well::Tsurf (F)           */
+} ocpg_wells;
 
 /* Totals of MixtureComp's iteration counters (include/MixtureComp.hpp:133-146)
  * over the cells of one flash call.                                          */
@@ -253,17 +284,38 @@ int ocpg_calkrpc(ocpg_ctx* ctx);
 int ocpg_calrock(ocpg_ctx* ctx);
 int ocpg_calkrpc_rock(ocpg_ctx* ctx);
 
-/* IsoT_FIM::CalRes (src/OCPFluidMethod.cpp:1201-1389), bulk part; the
+/* IsoT_FIM::CalRes (src/OCPFluidMethod.cpp:1201-1389), preceded by
+ * AllWells::CalTrans and AllWells::CalFlux (src/AllWells.cpp:216-236) when
+ * wells are set: bulk fluxes, well sources and the well equations; the
  * negated residual stays on the device as the right-hand side.  norms may be
  * NULL (then the call is asynchronous).                                      */
 int ocpg_calres(ocpg_ctx* ctx, double dt, ocpg_resnorm* norms);
-/* IsoT_FIM::AssembleMat (src/OCPFluidMethod.cpp:860-875): bulk blocks of
- * AssembleMatBulksNew (:1569-1839) into the device BSR.                     */
+/* IsoT_FIM::AssembleMat (src/OCPFluidMethod.cpp:860-875): AssembleMatBulksNew
+ * (:1569-1839) and AssembleMatWellsNew (:2417-2733) into the device BSR.
+ * Reads the upwinding and well rates of the preceding ocpg_calres.          */
 int ocpg_assemble(ocpg_ctx* ctx, double dt);
 /* The assembled system as VectorFaspSolver::AssembleMat copies it into a
  * dBSRmat (src/FaspSolver.cpp:362-429): IA, JA, val (row-major blocks), rhs.
  * Any pointer may be NULL.                                                    */
 int ocpg_get_bsr(ocpg_ctx* ctx, int32_t* IA, int32_t* JA, double* val, double* rhs);
+/* Wells: AllWells as IsoT_FIM::UpdateProperty, CalRes and
+ * AssembleMatWellsNew read it (src/OCPFluidMethod.cpp:914-935, 1284-1352,
+ * 2417-2733).  ocpg_set_wells fixes the wells and perforations and rebuilds
+ * the BSR pattern: after each bulk row's neighbours come its well columns
+ * (open wells in order, perforations in order), and one row per open well
+ * follows the bulk rows ([well diagonal, perforated cells in perforation
+ * order]), as LinearSystem::AddDim/NewDiag/NewOffDiag build them.  Needs
+ * ocpg_set_conn first.  nw == 0 removes the wells.                          */
+int ocpg_set_wells(ocpg_ctx* ctx, const ocpg_wells* wells);
+/* New per-step / per-iteration well values (bhp, mode, ctrlBHP, ctrlRate,
+ * perfDG, perfXi) for the wells ocpg_set_wells fixed; other fields ignored. */
+int ocpg_set_well_state(ocpg_ctx* ctx, const ocpg_wells* wells);
+/* Well::CalFlux and Well::CalProdWeight results of the last ocpg_calres:
+ * perforation molar rates perfQi (nperf*nc), well totals wellQi (nw*nc) and
+ * producer weights prodWeight (nw).  Any pointer may be NULL.               */
+int ocpg_get_well_flux(ocpg_ctx* ctx, double* perfQi, double* wellQi, double* prodWeight);
+/* Rows of the assembled system: bulk cells plus open wells.                  */
+int64_t ocpg_bsr_rows(ocpg_ctx* ctx);
 /* BulkConn::upblock / upblock_Rho / upblock_Velocity as CalRes set them.    */
 int ocpg_get_upwind(ocpg_ctx* ctx, int32_t* upblock, double* upblock_Rho,
                     double* upblock_Velocity);

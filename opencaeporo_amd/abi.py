@@ -117,7 +117,76 @@ class Conn(C.Structure):
 
 class ResNorm(C.Structure):
     _fields_ = [("maxRelRes_V", C.c_double), ("maxRelRes_N", C.c_double),
-                ("maxId_V", C.c_int64), ("maxId_N", C.c_int64)]
+                ("maxId_V", C.c_int64), ("maxId_N", C.c_int64), ("maxWellRelRes_mol", C.c_double)]
+
+
+WELL_INJ, WELL_PROD = 0, 1
+WELL_BHP, WELL_RATE = 0, 1
+
+
+class WellsC(C.Structure):
+    _fields_ = [("nw", C.c_int32), ("type", _P), ("open", _P), ("mode", _P), ("unweight", _P),
+                ("bhp", _P), ("ctrlBHP", _P), ("ctrlRate", _P), ("injZi", _P), ("prodPhaseWeight", _P),
+                ("perfStart", _P), ("perfCell", _P), ("perfWI", _P), ("perfMult", _P), ("perfDG", _P),
+                ("perfXi", _P), ("Psurf", C.c_double), ("Tsurf", C.c_double)]
+
+
+class Wells:
+    """Numpy-backed ocpg_wells (include/ocpgpu.h): per-well arrays of length
+    nw (injZi nw*nc, prodPhaseWeight nw*3) and per-perforation arrays in
+    well-major order."""
+
+    INT = ("type", "open", "mode", "unweight", "perfStart", "perfCell")
+    DBL = ("bhp", "ctrlBHP", "ctrlRate", "injZi", "prodPhaseWeight", "perfWI", "perfMult", "perfDG",
+           "perfXi")
+
+    def __init__(self, nc, wells, Psurf=14.7, Tsurf=60.0):
+        """wells: list of dicts with keys type, mode, bhp, ctrlBHP, ctrlRate,
+        perf (cells), WI, dG, and optionally open, unweight, mult, injZi,
+        prodPhaseWeight, perfXi."""
+        self.nc = nc
+        self.nw = len(wells)
+        self.Psurf, self.Tsurf = Psurf, Tsurf
+        i32 = lambda v: np.ascontiguousarray(v, dtype=np.int32)
+        f64 = lambda v: np.ascontiguousarray(v, dtype=np.float64)
+        self.type = i32([w["type"] for w in wells])
+        self.open = i32([w.get("open", 1) for w in wells])
+        self.mode = i32([w["mode"] for w in wells])
+        self.unweight = i32([w.get("unweight", 0) for w in wells])
+        self.bhp = f64([w["bhp"] for w in wells])
+        self.ctrlBHP = f64([w["ctrlBHP"] for w in wells])
+        self.ctrlRate = f64([w["ctrlRate"] for w in wells])
+        self.injZi = f64(np.concatenate([np.asarray(w.get("injZi", np.zeros(nc)), float) for w in wells])
+                         if wells else np.zeros(0))
+        self.prodPhaseWeight = f64(np.concatenate([np.asarray(w.get("prodPhaseWeight", [1, 0, 0]), float)
+                                                   for w in wells]) if wells else np.zeros(0))
+        counts = [len(w["perf"]) for w in wells]
+        self.perfStart = i32(np.concatenate([[0], np.cumsum(counts)]))
+        self.nperf = int(self.perfStart[-1])
+        cat = lambda key, default: f64(np.concatenate(
+            [np.asarray(w.get(key, np.full(len(w["perf"]), default)), float) for w in wells])
+            if wells else np.zeros(0))
+        self.perfCell = i32(np.concatenate([w["perf"] for w in wells]) if wells else np.zeros(0))
+        self.perfWI = cat("WI", 1.0)
+        self.perfMult = cat("mult", 1.0)
+        self.perfDG = cat("dG", 0.0)
+        self.perfXi = cat("perfXi", 0.0)
+
+    def cstruct(self):
+        s = WellsC()
+        s.nw = self.nw
+        for f in self.INT + self.DBL:
+            a = getattr(self, f)
+            setattr(s, f, a.ctypes.data if a.size else None)
+        s.Psurf, s.Tsurf = self.Psurf, self.Tsurf
+        return s
+
+    def open_count(self):
+        return int(np.count_nonzero(self.open))
+
+    def row_of(self, w):
+        """Index of open well w among the open wells (its row after the bulk rows)."""
+        return int(np.count_nonzero(self.open[:w]))
 
 
 class FlashStats(C.Structure):

@@ -10,11 +10,16 @@
 
 #include "../../include/ocpgpu.h"
 #include "device_state.hpp"
+#include "fim_host.hpp"
 #include "fim_kernels.hpp"
+#include "flash_kernel.hpp"
 
 namespace ocpg {
 DevBulk& ctxBulk(ocpg_ctx* c);
 FimDevice& ctxFim(ocpg_ctx* c);
+WellDevice& ctxWells(ocpg_ctx* c);
+FimHost& ctxFimHost(ocpg_ctx* c);
+const DevEos* ctxEos(ocpg_ctx* c);
 hipStream_t ctxStream(ocpg_ctx* c);
 int ctxNC(ocpg_ctx* c);
 int ctxDevice(ocpg_ctx* c);
@@ -49,13 +54,223 @@ static hipError_t devAlloc(T*& p, size_t count, size_t& total)
     return hipMalloc(&p, b);
 }
 
+static size_t align256w(size_t b) { return (b + 255) & ~size_t(255); }
+
+// The per-step / per-iteration well fields (bhp, control, dG, perforation
+// molar densities of injectors).
+static void setWellState(WellsHost& WH, const ocpg_wells& w)
+{
+    const int nw = WH.nw;
+    const int nperf = WH.perfStart[nw];
+    WH.mode.assign(w.mode, w.mode + nw);
+    WH.bhp.assign(w.bhp, w.bhp + nw);
+    WH.ctrlBHP.assign(w.ctrlBHP, w.ctrlBHP + nw);
+    WH.ctrlRate.assign(w.ctrlRate, w.ctrlRate + nw);
+    WH.perfDG.assign(w.perfDG, w.perfDG + nperf);
+    WH.perfXi.assign(nperf, 0.0);
+    if (w.perfXi) WH.perfXi.assign(w.perfXi, w.perfXi + nperf);
+}
+
+static int uploadWellState(ocpg_ctx* c)
+{
+    WellDevice& W = ctxWells(c);
+    const WellsHost& WH = ctxFimHost(c).wells;
+    if (W.nw == 0) return OCPG_OK;
+    const size_t NW = W.nw, NP = W.nperf;
+    HIPCHK2(hipMemcpy(W.mode, WH.mode.data(), NW * 4, hipMemcpyHostToDevice));
+    HIPCHK2(hipMemcpy(W.bhp, WH.bhp.data(), NW * 8, hipMemcpyHostToDevice));
+    HIPCHK2(hipMemcpy(W.ctrlBHP, WH.ctrlBHP.data(), NW * 8, hipMemcpyHostToDevice));
+    HIPCHK2(hipMemcpy(W.ctrlRate, WH.ctrlRate.data(), NW * 8, hipMemcpyHostToDevice));
+    if (NP) {
+        HIPCHK2(hipMemcpy(W.perfDG, WH.perfDG.data(), NP * 8, hipMemcpyHostToDevice));
+        HIPCHK2(hipMemcpy(W.perfXi, WH.perfXi.data(), NP * 8, hipMemcpyHostToDevice));
+    }
+    return OCPG_OK;
+}
+
+// The BSR pattern LinearSystem ends up with after AssembleMatBulksNew and
+// AssembleMatWellsNew (include/LinearSystem.hpp:62-125):
+//   bulk row n  = [n, its neighbours in the order connections reach them,
+//                  then the wells perforating n (open wells in order,
+//                  perforations in order)]
+//   well row w  = [nb + w, the cells of its perforations in order]
+// plus, for the kernels, each slot's connection and mirror slot, the end of
+// each bulk row's bulk columns and the slots of every perforation's blocks.
+static int buildPattern(ocpg_ctx* c)
+{
+    FimDevice& F = ctxFim(c);
+    WellDevice& W = ctxWells(c);
+    const FimHost& H = ctxFimHost(c);
+    const WellsHost& WH = H.wells;
+    const int64_t nb = F.nb, nconn = F.nconn;
+    const int nw = WH.nw;
+    int nwOpen = 0;
+    std::vector<int32_t> wrow(nw, -1);
+    for (int w = 0; w < nw; w++)
+        if (WH.open[w]) wrow[w] = nwOpen++;
+    const int nperf = nw > 0 ? WH.perfStart[nw] : 0;
+    const int64_t nrow = nb + nwOpen;
+    std::vector<int64_t> cnt(nrow, 1);
+    for (int64_t k = 0; k < nconn; k++) {
+        cnt[H.bId[k]]++;
+        cnt[H.eId[k]]++;
+    }
+    std::vector<int64_t> bulkCnt(cnt.begin(), cnt.begin() + nb);
+    for (int w = 0; w < nw; w++) {
+        if (!WH.open[w]) continue;
+        for (int p = WH.perfStart[w]; p < WH.perfStart[w + 1]; p++) {
+            cnt[WH.perfCell[p]]++;
+            cnt[nb + wrow[w]]++;
+        }
+    }
+    std::vector<int64_t> IA64(nrow + 1, 0);
+    for (int64_t n = 0; n < nrow; n++) IA64[n + 1] = IA64[n] + cnt[n];
+    if (IA64[nrow] > (int64_t)INT32_MAX) return OCPG_ETOOBIG;
+    const int64_t nnzb = IA64[nrow];
+    std::vector<int32_t> IA(nrow + 1), JA(nnzb), slotConn(nnzb), slotMirror(nnzb), bulkEnd(nb);
+    std::vector<int64_t> fill(nrow);
+    for (int64_t n = 0; n <= nrow; n++) IA[n] = (int32_t)IA64[n];
+    for (int64_t n = 0; n < nrow; n++) {
+        JA[IA[n]] = (int32_t)n;
+        slotConn[IA[n]] = -1;
+        slotMirror[IA[n]] = IA[n];
+        fill[n] = IA[n] + 1;
+    }
+    for (int64_t n = 0; n < nb; n++) bulkEnd[n] = (int32_t)(IA[n] + bulkCnt[n]);
+    for (int64_t k = 0; k < nconn; k++) {
+        const int32_t b = H.bId[k], e = H.eId[k];
+        const int32_t se = (int32_t)fill[e]++;  // NewOffDiag(eId, bId)
+        const int32_t sb = (int32_t)fill[b]++;  // NewOffDiag(bId, eId)
+        JA[se] = b;
+        JA[sb] = e;
+        slotConn[se] = slotConn[sb] = (int32_t)k;
+        slotMirror[se] = sb;
+        slotMirror[sb] = se;
+    }
+    std::vector<int32_t> slotBW(std::max(nperf, 1), -1), slotWB(std::max(nperf, 1), -1);
+    std::vector<int32_t> cellPerfPtr(nb + 1, 0), cellPerfIdx, pcell;
+    for (int w = 0; w < nw; w++) {
+        if (!WH.open[w]) continue;
+        const int64_t r = nb + wrow[w];
+        for (int p = WH.perfStart[w]; p < WH.perfStart[w + 1]; p++) {
+            const int32_t n = WH.perfCell[p];
+            const int32_t sBW = (int32_t)fill[n]++;  // NewOffDiag(n, wId)
+            const int32_t sWB = (int32_t)fill[r]++;  // NewOffDiag(wId, n)
+            JA[sBW] = (int32_t)r;
+            JA[sWB] = n;
+            slotConn[sBW] = slotConn[sWB] = -2;
+            slotMirror[sBW] = sWB;
+            slotMirror[sWB] = sBW;
+            slotBW[p] = sBW;
+            slotWB[p] = sWB;
+            cellPerfPtr[n + 1]++;
+        }
+    }
+    for (int64_t n = 0; n < nb; n++) {
+        if (cellPerfPtr[n + 1] > 0) pcell.push_back((int32_t)n);
+        cellPerfPtr[n + 1] += cellPerfPtr[n];
+    }
+    cellPerfIdx.assign(std::max<int64_t>(cellPerfPtr[nb], 1), 0);
+    {
+        std::vector<int32_t> pos(cellPerfPtr.begin(), cellPerfPtr.end() - 1);
+        for (int w = 0; w < nw; w++) {
+            if (!WH.open[w]) continue;
+            for (int p = WH.perfStart[w]; p < WH.perfStart[w + 1]; p++) cellPerfIdx[pos[WH.perfCell[p]]++] = p;
+        }
+    }
+    // device arrays of the pattern (the connection arrays stay)
+    const int ncol = F.ncol;
+    void* old[] = {F.IA, F.JA, F.slotConn, F.slotMirror, F.bulkEnd, F.res, F.val};
+    for (void* q : old)
+        if (q) HIPCHK2(hipFree(q));
+    F.IA = F.JA = F.slotConn = F.slotMirror = F.bulkEnd = nullptr;
+    F.res = F.val = nullptr;
+    size_t tot = F.bytes - F.patternBytes;
+    const size_t before = tot;
+    HIPCHK2(devAlloc(F.IA, nrow + 1, tot));
+    HIPCHK2(devAlloc(F.JA, nnzb, tot));
+    HIPCHK2(devAlloc(F.slotConn, nnzb, tot));
+    HIPCHK2(devAlloc(F.slotMirror, nnzb, tot));
+    HIPCHK2(devAlloc(F.bulkEnd, nb, tot));
+    HIPCHK2(devAlloc(F.res, nrow * ncol, tot));
+    HIPCHK2(devAlloc(F.val, (size_t)nnzb * ncol * ncol, tot));
+    F.patternBytes = tot - before;
+    F.bytes = tot;
+    F.nnzb = nnzb;
+    F.nrow = nrow;
+    HIPCHK2(hipMemcpy(F.IA, IA.data(), IA.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK2(hipMemcpy(F.JA, JA.data(), JA.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK2(hipMemcpy(F.slotConn, slotConn.data(), nnzb * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK2(hipMemcpy(F.slotMirror, slotMirror.data(), nnzb * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK2(hipMemcpy(F.bulkEnd, bulkEnd.data(), nb * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK2(hipMemset(F.res, 0, nrow * ncol * sizeof(double)));
+
+    // wells: one allocation for every per-well / per-perforation array
+    if (W.mem) HIPCHK2(hipFree(W.mem));
+    W = WellDevice{};
+    if (nw == 0) return OCPG_OK;
+    const int nc = ctxNC(c) + 1;
+    const size_t bsize = (size_t)ncol * ncol;
+    struct Part {
+        void** ptr;
+        size_t bytes;
+    };
+    const size_t NW = nw, NP = std::max(nperf, 1);
+    Part parts[] = {
+        {(void**)&W.type, NW * 4}, {(void**)&W.open, NW * 4}, {(void**)&W.mode, NW * 4},
+        {(void**)&W.unweight, NW * 4}, {(void**)&W.perfStart, (NW + 1) * 4}, {(void**)&W.perfCell, NP * 4},
+        {(void**)&W.row, NW * 4}, {(void**)&W.bhp, NW * 8}, {(void**)&W.ctrlBHP, NW * 8},
+        {(void**)&W.ctrlRate, NW * 8}, {(void**)&W.injZi, NW * nc * 8}, {(void**)&W.prodPW, NW * 3 * 8},
+        {(void**)&W.perfWI, NP * 8}, {(void**)&W.perfMult, NP * 8}, {(void**)&W.perfDG, NP * 8},
+        {(void**)&W.perfXi, NP * 8}, {(void**)&W.perfTransj, NP * 3 * 8}, {(void**)&W.perfQi, NP * nc * 8},
+        {(void**)&W.wellQi, NW * nc * 8}, {(void**)&W.prodWeight, NW * 8}, {(void**)&W.wellRelRes, NW * 8},
+        {(void**)&W.cellPerfPtr, (size_t)(nb + 1) * 4}, {(void**)&W.cellPerfIdx, cellPerfIdx.size() * 4},
+        {(void**)&W.pcell, std::max<size_t>(pcell.size(), 1) * 4}, {(void**)&W.perfBlock, NP * bsize * 8},
+        {(void**)&W.perfDQW, NP * nc * 8}, {(void**)&W.slotBW, NP * 4}, {(void**)&W.slotWB, NP * 4},
+    };
+    size_t total = 0;
+    for (const Part& q : parts) total += align256w(q.bytes);
+    char* mem = nullptr;
+    HIPCHK2(hipMalloc(&mem, total));
+    HIPCHK2(hipMemset(mem, 0, total));
+    W.mem = mem;
+    W.bytes = total;
+    size_t off = 0;
+    for (const Part& q : parts) {
+        *q.ptr = mem + off;
+        off += align256w(q.bytes);
+    }
+    W.nw = nw;
+    W.nwOpen = nwOpen;
+    W.nperf = nperf;
+    W.npcell = (int)pcell.size();
+    W.Psurf = WH.Psurf;
+    W.Tsurf = WH.Tsurf;
+    auto up = [](void* d, const void* h, size_t b) { return b ? hipMemcpy(d, h, b, hipMemcpyHostToDevice) : hipSuccess; };
+    HIPCHK2(up(W.type, WH.type.data(), NW * 4));
+    HIPCHK2(up(W.open, WH.open.data(), NW * 4));
+    HIPCHK2(up(W.unweight, WH.unweight.data(), NW * 4));
+    HIPCHK2(up(W.perfStart, WH.perfStart.data(), (NW + 1) * 4));
+    HIPCHK2(up(W.perfCell, WH.perfCell.data(), (size_t)nperf * 4));
+    HIPCHK2(up(W.row, wrow.data(), NW * 4));
+    HIPCHK2(up(W.injZi, WH.injZi.data(), NW * nc * 8));
+    HIPCHK2(up(W.prodPW, WH.prodPW.data(), NW * 3 * 8));
+    HIPCHK2(up(W.perfWI, WH.perfWI.data(), (size_t)nperf * 8));
+    HIPCHK2(up(W.perfMult, WH.perfMult.data(), (size_t)nperf * 8));
+    HIPCHK2(up(W.cellPerfPtr, cellPerfPtr.data(), (size_t)(nb + 1) * 4));
+    HIPCHK2(up(W.cellPerfIdx, cellPerfIdx.data(), cellPerfIdx.size() * 4));
+    HIPCHK2(up(W.pcell, pcell.data(), pcell.size() * 4));
+    HIPCHK2(up(W.slotBW, slotBW.data(), (size_t)nperf * 4));
+    HIPCHK2(up(W.slotWB, slotWB.data(), (size_t)nperf * 4));
+    return uploadWellState(c);
+}
+
 extern "C" {
 
 // BulkConn::Setup/CalAkd output (src/BulkConn.cpp:31-117) and the BSR
-// pattern that LinearSystem::AllocateColMem + AssembleMatBulksNew produce:
-// row n = [n, then its neighbours in the order connections reach them].
+// pattern that LinearSystem::AllocateColMem + AssembleMatBulksNew produce.
 // Connections must satisfy bId < eId and be sorted by (bId, eId), as
-// BulkConn::iteratorConn is.
+// BulkConn::iteratorConn is.  Removes any wells.
 int ocpg_set_conn(ocpg_ctx* c, const ocpg_conn* conn)
 {
     if (!c || !conn || (conn->nconn > 0 && (!conn->bId || !conn->eId || !conn->area))) return OCPG_EINVAL;
@@ -71,70 +286,128 @@ int ocpg_set_conn(ocpg_ctx* c, const ocpg_conn* conn)
     if (nb + 2 * nconn > (int64_t)INT32_MAX) return OCPG_ETOOBIG;
     HIPCHK2(hipSetDevice(ctxDevice(c)));
     HIPCHK2(hipStreamSynchronize(ctxStream(c)));
-    // pattern on the host
-    std::vector<int32_t> cnt(nb, 1);
-    for (int64_t k = 0; k < nconn; k++) {
-        cnt[conn->bId[k]]++;
-        cnt[conn->eId[k]]++;
-    }
-    std::vector<int32_t> IA(nb + 1, 0);
-    for (int64_t n = 0; n < nb; n++) IA[n + 1] = IA[n] + cnt[n];
-    const int64_t nnzb = IA[nb];
-    std::vector<int32_t> JA(nnzb), slotConn(nnzb), slotMirror(nnzb), fill(nb);
-    for (int64_t n = 0; n < nb; n++) {
-        JA[IA[n]] = (int32_t)n;
-        slotConn[IA[n]] = -1;
-        slotMirror[IA[n]] = IA[n];
-        fill[n] = IA[n] + 1;
-    }
-    for (int64_t k = 0; k < nconn; k++) {
-        const int32_t b = conn->bId[k], e = conn->eId[k];
-        const int32_t se = fill[e]++;  // NewOffDiag(eId, bId)
-        const int32_t sb = fill[b]++;  // NewOffDiag(bId, eId)
-        JA[se] = b;
-        JA[sb] = e;
-        slotConn[se] = slotConn[sb] = (int32_t)k;
-        slotMirror[se] = sb;
-        slotMirror[sb] = se;
-    }
+    FimHost& H = ctxFimHost(c);
+    H.bId.assign(conn->bId, conn->bId + nconn);
+    H.eId.assign(conn->eId, conn->eId + nconn);
+    H.wells = WellsHost{};
     FimDevice& F = ctxFim(c);
     fimFree(F);
-    const int ncol = ctxNC(c) + 2;
     F.nb = nb;
     F.nconn = nconn;
-    F.nnzb = nnzb;
-    F.ncol = ncol;
+    F.ncol = ctxNC(c) + 2;
     F.normBlocks = (int)((nb + 255) / 256);
     size_t tot = 0;
     HIPCHK2(devAlloc(F.bId, nconn, tot));
     HIPCHK2(devAlloc(F.eId, nconn, tot));
     HIPCHK2(devAlloc(F.area, nconn, tot));
-    HIPCHK2(devAlloc(F.IA, nb + 1, tot));
-    HIPCHK2(devAlloc(F.JA, nnzb, tot));
-    HIPCHK2(devAlloc(F.slotConn, nnzb, tot));
-    HIPCHK2(devAlloc(F.slotMirror, nnzb, tot));
     HIPCHK2(devAlloc(F.upblock, nconn * 3, tot));
     HIPCHK2(devAlloc(F.upRho, nconn * 3, tot));
     HIPCHK2(devAlloc(F.upVel, nconn * 3, tot));
-    HIPCHK2(devAlloc(F.res, nb * ncol, tot));
-    HIPCHK2(devAlloc(F.val, (size_t)nnzb * ncol * ncol, tot));
     HIPCHK2(devAlloc(F.normPart, (size_t)F.normBlocks * 2, tot));
     HIPCHK2(devAlloc(F.normIdx, (size_t)F.normBlocks * 2, tot));
     F.bytes = tot;
+    F.patternBytes = 0;
     HIPCHK2(hipMemcpy(F.bId, conn->bId, nconn * sizeof(int32_t), hipMemcpyHostToDevice));
     HIPCHK2(hipMemcpy(F.eId, conn->eId, nconn * sizeof(int32_t), hipMemcpyHostToDevice));
     HIPCHK2(hipMemcpy(F.area, conn->area, nconn * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK2(hipMemcpy(F.IA, IA.data(), IA.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIPCHK2(hipMemcpy(F.JA, JA.data(), JA.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIPCHK2(hipMemcpy(F.slotConn, slotConn.data(), nnzb * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIPCHK2(hipMemcpy(F.slotMirror, slotMirror.data(), nnzb * sizeof(int32_t), hipMemcpyHostToDevice));
     HIPCHK2(hipMemset(F.upblock, 0, nconn * 3 * sizeof(int32_t)));
     HIPCHK2(hipMemset(F.upRho, 0, nconn * 3 * sizeof(double)));
     HIPCHK2(hipMemset(F.upVel, 0, nconn * 3 * sizeof(double)));
-    return OCPG_OK;
+    return buildPattern(c);
 }
 
 int64_t ocpg_bsr_nnzb(ocpg_ctx* c) { return c ? ctxFim(c).nnzb : -1; }
+int64_t ocpg_bsr_rows(ocpg_ctx* c) { return c ? ctxFim(c).nrow : -1; }
+
+// AllWells (include/AllWells.hpp) as IsoT_FIM reads it; rebuilds the pattern.
+int ocpg_set_wells(ocpg_ctx* c, const ocpg_wells* w)
+{
+    if (!c || !w || w->nw < 0) return OCPG_EINVAL;
+    FimDevice& F = ctxFim(c);
+    if (F.nb <= 0 || !F.bId) return OCPG_ESTATE;
+    if (!flashSupportsNC(ctxNC(c))) return OCPG_EUNSUP;
+    const int nw = w->nw;
+    const int nc = ctxNC(c) + 1;
+    WellsHost WH;
+    WH.nw = nw;
+    if (nw > 0) {
+        if (!w->type || !w->open || !w->mode || !w->bhp || !w->ctrlBHP || !w->ctrlRate || !w->perfStart ||
+            !w->perfCell || !w->perfWI || !w->perfDG)
+            return OCPG_EINVAL;
+        if (w->perfStart[0] != 0) return OCPG_EINVAL;
+        for (int i = 0; i < nw; i++) {
+            if (w->perfStart[i + 1] < w->perfStart[i]) return OCPG_EINVAL;
+            if (w->type[i] != OCPG_WELL_INJ && w->type[i] != OCPG_WELL_PROD) return OCPG_EINVAL;
+            if (w->mode[i] != OCPG_WELL_BHP && w->mode[i] != OCPG_WELL_RATE) return OCPG_EINVAL;
+            if (w->type[i] == OCPG_WELL_INJ && (!w->injZi || !w->perfXi)) return OCPG_EINVAL;
+            if (w->type[i] == OCPG_WELL_PROD && w->mode[i] == OCPG_WELL_RATE && !w->prodPhaseWeight)
+                return OCPG_EINVAL;
+        }
+        const int nperf = w->perfStart[nw];
+        for (int p = 0; p < nperf; p++)
+            if (w->perfCell[p] < 0 || w->perfCell[p] >= F.nb) return OCPG_EINVAL;
+        WH.type.assign(w->type, w->type + nw);
+        WH.open.assign(w->open, w->open + nw);
+        WH.mode.assign(w->mode, w->mode + nw);
+        WH.unweight.assign(nw, 0);
+        if (w->unweight) WH.unweight.assign(w->unweight, w->unweight + nw);
+        WH.perfStart.assign(w->perfStart, w->perfStart + nw + 1);
+        WH.perfCell.assign(w->perfCell, w->perfCell + nperf);
+        WH.perfWI.assign(w->perfWI, w->perfWI + nperf);
+        WH.perfMult.assign(nperf, 1.0);
+        if (w->perfMult) WH.perfMult.assign(w->perfMult, w->perfMult + nperf);
+        WH.injZi.assign((size_t)nw * nc, 0.0);
+        WH.prodPW.assign((size_t)nw * 3, 0.0);
+        for (int i = 0; i < nw; i++) {
+            if (w->type[i] == OCPG_WELL_INJ)
+                std::copy(w->injZi + (size_t)i * nc, w->injZi + (size_t)(i + 1) * nc, WH.injZi.begin() + (size_t)i * nc);
+            else if (w->prodPhaseWeight)
+                std::copy(w->prodPhaseWeight + (size_t)i * 3, w->prodPhaseWeight + (size_t)(i + 1) * 3,
+                          WH.prodPW.begin() + (size_t)i * 3);
+        }
+        WH.Psurf = w->Psurf;
+        WH.Tsurf = w->Tsurf;
+        setWellState(WH, *w);
+    }
+    HIPCHK2(hipSetDevice(ctxDevice(c)));
+    HIPCHK2(hipStreamSynchronize(ctxStream(c)));
+    ctxFimHost(c).wells = WH;
+    return buildPattern(c);
+}
+
+// New values of the per-step / per-iteration well fields.
+int ocpg_set_well_state(ocpg_ctx* c, const ocpg_wells* w)
+{
+    if (!c || !w) return OCPG_EINVAL;
+    WellsHost& WH = ctxFimHost(c).wells;
+    if (w->nw != WH.nw) return OCPG_EINVAL;
+    if (WH.nw == 0) return OCPG_OK;
+    if (!w->mode || !w->bhp || !w->ctrlBHP || !w->ctrlRate || !w->perfDG) return OCPG_EINVAL;
+    for (int i = 0; i < WH.nw; i++) {
+        if (w->mode[i] != OCPG_WELL_BHP && w->mode[i] != OCPG_WELL_RATE) return OCPG_EINVAL;
+        if (WH.type[i] == OCPG_WELL_INJ && !w->perfXi) return OCPG_EINVAL;
+        if (WH.type[i] == OCPG_WELL_PROD && w->mode[i] == OCPG_WELL_RATE && WH.prodPW.empty()) return OCPG_EINVAL;
+    }
+    setWellState(WH, *w);
+    HIPCHK2(hipSetDevice(ctxDevice(c)));
+    HIPCHK2(hipStreamSynchronize(ctxStream(c)));
+    return uploadWellState(c);
+}
+
+// Well::CalFlux / CalProdWeight results of the last ocpg_calres.
+int ocpg_get_well_flux(ocpg_ctx* c, double* perfQi, double* wellQi, double* prodWeight)
+{
+    if (!c) return OCPG_EINVAL;
+    WellDevice& W = ctxWells(c);
+    HIPCHK2(hipSetDevice(ctxDevice(c)));
+    HIPCHK2(hipStreamSynchronize(ctxStream(c)));
+    if (W.nw == 0) return OCPG_OK;
+    const size_t nc = ctxNC(c) + 1;
+    if (perfQi && W.nperf) HIPCHK2(hipMemcpy(perfQi, W.perfQi, W.nperf * nc * 8, hipMemcpyDeviceToHost));
+    if (wellQi) HIPCHK2(hipMemcpy(wellQi, W.wellQi, W.nw * nc * 8, hipMemcpyDeviceToHost));
+    if (prodWeight) HIPCHK2(hipMemcpy(prodWeight, W.prodWeight, W.nw * 8, hipMemcpyDeviceToHost));
+    return OCPG_OK;
+}
 
 // IsoT_FIM::CalKrPc (src/OCPFluidMethod.cpp:1189-1199).
 int ocpg_calkrpc(ocpg_ctx* c)
@@ -175,8 +448,10 @@ int ocpg_calres(ocpg_ctx* c, double dt, ocpg_resnorm* norms)
     FimDevice& F = ctxFim(c);
     if (F.nb <= 0 || F.nb != ctxBulk(c).nb) return OCPG_ESTATE;
     HIPCHK2(hipSetDevice(ctxDevice(c)));
+    WellDevice& W = ctxWells(c);
     ctxTimerBegin(c, T_RES);
-    HIPCHK2(launchCalRes(ctxNC(c), ctxBulk(c), F, dt, ctxStream(c)));
+    if (W.nw > 0) HIPCHK2(launchWellFlux(ctxNC(c), ctxEos(c), ctxBulk(c), F, W, ctxStream(c)));
+    HIPCHK2(launchCalRes(ctxNC(c), ctxBulk(c), F, W, dt, ctxStream(c)));
     ctxTimerEnd(c, T_RES);
     if (norms) {
         std::vector<double> v((size_t)F.normBlocks * 2);
@@ -186,7 +461,14 @@ int ocpg_calres(ocpg_ctx* c, double dt, ocpg_resnorm* norms)
         HIPCHK2(hipMemcpyAsync(id.data(), F.normIdx, id.size() * sizeof(int64_t), hipMemcpyDeviceToHost,
                                ctxStream(c)));
         HIPCHK2(hipStreamSynchronize(ctxStream(c)));
-        ocpg_resnorm r = {0, 0, 0, 0};
+        ocpg_resnorm r = {0, 0, 0, 0, 0};
+        if (W.nw > 0) {
+            std::vector<double> rel(W.nw);
+            HIPCHK2(hipMemcpyAsync(rel.data(), W.wellRelRes, rel.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                   ctxStream(c)));
+            HIPCHK2(hipStreamSynchronize(ctxStream(c)));
+            for (double v : rel) r.maxWellRelRes_mol = std::max(r.maxWellRelRes_mol, v);
+        }
         for (int b = 0; b < F.normBlocks; b++) {
             if (r.maxRelRes_V < v[2 * b]) {
                 r.maxRelRes_V = v[2 * b];
@@ -210,8 +492,10 @@ int ocpg_assemble(ocpg_ctx* c, double dt)
     FimDevice& F = ctxFim(c);
     if (F.nb <= 0 || F.nb != ctxBulk(c).nb) return OCPG_ESTATE;
     HIPCHK2(hipSetDevice(ctxDevice(c)));
+    WellDevice& W = ctxWells(c);
     ctxTimerBegin(c, T_ASM);
     HIPCHK2(launchAssemble(ctxNC(c), ctxBulk(c), F, dt, ctxStream(c)));
+    if (W.nw > 0) HIPCHK2(launchWellAssemble(ctxNC(c), ctxBulk(c), F, W, dt, ctxStream(c)));
     ctxTimerEnd(c, T_ASM);
     return OCPG_OK;
 }
@@ -227,10 +511,10 @@ int ocpg_get_bsr(ocpg_ctx* c, int32_t* IA, int32_t* JA, double* val, double* rhs
     HIPCHK2(hipSetDevice(ctxDevice(c)));
     HIPCHK2(hipStreamSynchronize(ctxStream(c)));
     const size_t bs = (size_t)F.ncol * F.ncol;
-    if (IA) HIPCHK2(hipMemcpy(IA, F.IA, (F.nb + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (IA) HIPCHK2(hipMemcpy(IA, F.IA, (F.nrow + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
     if (JA) HIPCHK2(hipMemcpy(JA, F.JA, F.nnzb * sizeof(int32_t), hipMemcpyDeviceToHost));
     if (val) HIPCHK2(hipMemcpy(val, F.val, F.nnzb * bs * sizeof(double), hipMemcpyDeviceToHost));
-    if (rhs) HIPCHK2(hipMemcpy(rhs, F.res, F.nb * F.ncol * sizeof(double), hipMemcpyDeviceToHost));
+    if (rhs) HIPCHK2(hipMemcpy(rhs, F.res, F.nrow * F.ncol * sizeof(double), hipMemcpyDeviceToHost));
     return OCPG_OK;
 }
 

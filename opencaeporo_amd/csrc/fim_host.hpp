@@ -1,0 +1,23 @@
+// Host-side copies the library keeps to rebuild the BSR pattern: the
+// connection list and the wells as the caller last described them.
+#pragma once
+
+#include <stdint.h>
+
+#include <vector>
+
+namespace ocpg {
+
+struct WellsHost {
+    int nw = 0;
+    std::vector<int32_t> type, open, mode, unweight, perfStart, perfCell;
+    std::vector<double> bhp, ctrlBHP, ctrlRate, injZi, prodPW, perfWI, perfMult, perfDG, perfXi;
+    double Psurf = 0, Tsurf = 0;
+};
+
+struct FimHost {
+    std::vector<int32_t> bId, eId;
+    WellsHost wells;
+};
+
+}  // namespace ocpg

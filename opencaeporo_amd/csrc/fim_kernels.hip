@@ -117,7 +117,7 @@ hipError_t launchKrPcRock(const DevBulk& B, const DevSat* sat, const DevRock& ro
 constexpr int kResBlock = 256;
 
 template <int NC>
-__global__ void __launch_bounds__(kResBlock) calResKernel(DevBulk B, FimDevice F, double dt)
+__global__ void __launch_bounds__(kResBlock) calResKernel(DevBulk B, FimDevice F, WellDevice W, double dt)
 {
     constexpr int nc = NC + 1;
     constexpr int len = nc + 1;
@@ -130,7 +130,7 @@ __global__ void __launch_bounds__(kResBlock) calResKernel(DevBulk B, FimDevice F
         r[0] = B.rockVp[n] - B.vf[n];
 #pragma unroll
         for (int i = 0; i < nc; ++i) r[1 + i] = B.Ni[i * ld + n] - B.lNi[i * ld + n];
-        const int k0 = F.IA[n] + 1, k1 = F.IA[n + 1];
+        const int k0 = F.IA[n] + 1, k1 = F.bulkEnd[n];
         for (int k = k0; k < k1; ++k) {
             const int64_t m = F.JA[k];
             const int64_t c = F.slotConn[k];
@@ -176,6 +176,14 @@ __global__ void __launch_bounds__(kResBlock) calResKernel(DevBulk B, FimDevice F
                     else
                         r[1 + i] -= dNi;
                 }
+            }
+        }
+        // well sources, wells then perforations in order (:1284-1293)
+        if (W.nw > 0) {
+            for (int q = W.cellPerfPtr[n]; q < W.cellPerfPtr[n + 1]; ++q) {
+                const double* qi = W.perfQi + (int64_t)W.cellPerfIdx[q] * nc;
+#pragma unroll
+                for (int i = 0; i < nc; ++i) r[1 + i] += qi[i] * dt;
             }
         }
         const double rvp = B.rockVp[n], Nt = B.Nt[n];
@@ -376,7 +384,7 @@ __global__ void __launch_bounds__(kAsmBlock) assembleKernel(DevBulk B, FimDevice
         acc[t] = v;
     }
 
-    const int k0 = F.IA[n] + 1, k1 = F.IA[n + 1];
+    const int k0 = F.IA[n] + 1, k1 = F.bulkEnd[n];
     double pre[R::perLane];
     if (k0 < k1) {
         const int64_t m = F.JA[k0];
@@ -550,10 +558,10 @@ __global__ void __launch_bounds__(kAsmBlock) assembleKernel(DevBulk B, FimDevice
 }
 
 template <int NC>
-static hipError_t launchCalResT(const DevBulk& B, FimDevice& F, double dt, hipStream_t s)
+static hipError_t launchCalResT(const DevBulk& B, FimDevice& F, const WellDevice& W, double dt, hipStream_t s)
 {
     const unsigned grid = (unsigned)((B.nb + kResBlock - 1) / kResBlock);
-    hipLaunchKernelGGL(calResKernel<NC>, dim3(grid), dim3(kResBlock), 0, s, B, F, dt);
+    hipLaunchKernelGGL(calResKernel<NC>, dim3(grid), dim3(kResBlock), 0, s, B, F, W, dt);
     return hipGetLastError();
 }
 
@@ -567,11 +575,11 @@ static hipError_t launchAssembleT(const DevBulk& B, FimDevice& F, double dt, hip
 
 #define OCPG_FIM_NC(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
 
-hipError_t launchCalRes(int NC, const DevBulk& B, FimDevice& F, double dt, hipStream_t s)
+hipError_t launchCalRes(int NC, const DevBulk& B, FimDevice& F, const WellDevice& W, double dt, hipStream_t s)
 {
     if (B.nb == 0) return hipSuccess;
 #define X(N) \
-    if (NC == N) return launchCalResT<N>(B, F, dt, s);
+    if (NC == N) return launchCalResT<N>(B, F, W, dt, s);
     OCPG_FIM_NC(X)
 #undef X
     return hipErrorInvalidValue;
@@ -589,7 +597,7 @@ hipError_t launchAssemble(int NC, const DevBulk& B, FimDevice& F, double dt, hip
 
 void fimFree(FimDevice& f)
 {
-    void* ptrs[] = {f.bId, f.eId, f.area, f.IA, f.JA, f.slotConn, f.slotMirror, f.upblock,
+    void* ptrs[] = {f.bId, f.eId, f.area, f.IA, f.JA, f.slotConn, f.slotMirror, f.bulkEnd, f.upblock,
                     f.upRho, f.upVel, f.res, f.val, f.normPart, f.normIdx};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "device_state.hpp"
+#include "well_state.hpp"
 
 namespace ocpg {
 
@@ -22,17 +23,20 @@ struct FimDevice {
     int32_t* JA = nullptr;      // nnzb
     int32_t* slotConn = nullptr;    // connection of each off-diagonal slot (-1 on the diagonal)
     int32_t* slotMirror = nullptr;  // index of slot (JA[k], row) for slot k = (row, JA[k])
+    int32_t* bulkEnd = nullptr;     // per bulk row: end of its bulk columns (well columns follow)
     // per-connection upwinding written by CalRes and read by the assembly
     int32_t* upblock = nullptr;     // nconn * np
     double* upRho = nullptr;        // nconn * np
     double* upVel = nullptr;        // nconn * np
     // outputs
-    double* res = nullptr;          // nb * ncol, cell-major (LinearSystem rhs layout), negated
+    int64_t nrow = 0;               // rows: bulk cells, then open wells
+    double* res = nullptr;          // nrow * ncol, cell-major (LinearSystem rhs layout), negated
     double* val = nullptr;          // nnzb * ncol * ncol, row-major blocks
     double* normPart = nullptr;     // per-block partial maxima of CalRes norms
     int64_t* normIdx = nullptr;
     int normBlocks = 0;
     size_t bytes = 0;
+    size_t patternBytes = 0;        // part of bytes held by the pattern arrays, res and val
 };
 
 void fimFree(FimDevice& f);
@@ -40,7 +44,7 @@ size_t fimBytes(const FimDevice& f);
 
 hipError_t launchKrPcRock(const DevBulk& B, const DevSat* sat, const DevRock& rock, bool krpc, bool rockOn,
                           hipStream_t s);
-hipError_t launchCalRes(int NC, const DevBulk& B, FimDevice& F, double dt, hipStream_t s);
+hipError_t launchCalRes(int NC, const DevBulk& B, FimDevice& F, const WellDevice& W, double dt, hipStream_t s);
 hipError_t launchAssemble(int NC, const DevBulk& B, FimDevice& F, double dt, hipStream_t s);
 
 }  // namespace ocpg

@@ -26,4 +26,22 @@ hipError_t launchFlash(int NC, const DevEos* eos, const DevBulk& B, const FlashW
     return hipErrorInvalidValue;
 }
 
+hipError_t launchWellFlux(int NC, const DevEos* eos, const DevBulk& B, FimDevice& F, WellDevice& W, hipStream_t s)
+{
+#define X(N) \
+    if (NC == N) return launchWellFluxT<N>(eos, B, F, W, s);
+    OCPG_NC_LIST(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launchWellAssemble(int NC, const DevBulk& B, FimDevice& F, WellDevice& W, double dt, hipStream_t s)
+{
+#define X(N) \
+    if (NC == N) return launchWellAssembleT<N>(B, F, W, dt, s);
+    OCPG_NC_LIST(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
 }  // namespace ocpg

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include "device_state.hpp"
+#include "fim_kernels.hpp"
+#include "well_state.hpp"
 
 namespace ocpg {
 
@@ -14,6 +16,10 @@ bool flashSupportsNC(int nc);
 // (SoA, leading dimension B.ld).
 hipError_t launchFlash(int NC, const DevEos* eos, const DevBulk& B, const FlashWork& W, DevFlashCounters* ctr,
                        const double* zin, hipStream_t s);
+// Well::CalTrans + CalFlux + CalProdWeight and the well rows of CalRes.
+hipError_t launchWellFlux(int NC, const DevEos* eos, const DevBulk& B, FimDevice& F, WellDevice& W, hipStream_t s);
+// The well blocks of AssembleMatWellsNew (after the bulk assembly).
+hipError_t launchWellAssemble(int NC, const DevBulk& B, FimDevice& F, WellDevice& W, double dt, hipStream_t s);
 
 }  // namespace ocpg
 
@@ -21,4 +27,8 @@ namespace ocpg {
 template <int NC>
 hipError_t launchFlashT(const DevEos* eos, const DevBulk& B, const FlashWork& W, DevFlashCounters* ctr,
                         const double* zin, hipStream_t s);
+template <int NC>
+hipError_t launchWellFluxT(const DevEos* eos, const DevBulk& B, FimDevice& F, WellDevice& W, hipStream_t s);
+template <int NC>
+hipError_t launchWellAssembleT(const DevBulk& B, FimDevice& F, WellDevice& W, double dt, hipStream_t s);
 }  // namespace ocpg

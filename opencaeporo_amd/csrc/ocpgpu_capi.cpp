@@ -12,6 +12,7 @@
 
 #include "../../include/ocpgpu.h"
 #include "device_state.hpp"
+#include "fim_host.hpp"
 #include "fim_kernels.hpp"
 #include "flash_kernel.hpp"
 
@@ -101,8 +102,10 @@ struct ocpg_ctx {
     FlashWork work{};
     char* workMem = nullptr;
     size_t workBytes = 0;
-    // connections and BSR (fim_kernels)
+    // connections and BSR (fim_kernels), wells
     FimDevice fim{};
+    WellDevice wells{};
+    FimHost fimHost;
     // copy of the flash inputs (ocpg_save_flash_inputs)
     char* snap = nullptr;
     size_t snapBytes = 0;
@@ -235,6 +238,7 @@ int64_t ocpg_sizeof(const char* name)
     if (!std::strcmp(name, "ocpg_bulk")) return sizeof(ocpg_bulk);
     if (!std::strcmp(name, "ocpg_conn")) return sizeof(ocpg_conn);
     if (!std::strcmp(name, "ocpg_resnorm")) return sizeof(ocpg_resnorm);
+    if (!std::strcmp(name, "ocpg_wells")) return sizeof(ocpg_wells);
     if (!std::strcmp(name, "ocpg_flash_stats")) return sizeof(ocpg_flash_stats);
     return -1;
 }
@@ -356,6 +360,7 @@ int ocpg_destroy(ocpg_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     timerClear(c);
     fimFree(c->fim);
+    if (c->wells.mem) (void)hipFree(c->wells.mem);
     if (c->snap) (void)hipFree(c->snap);
     if (c->workMem) (void)hipFree(c->workMem);
     if (c->cur) (void)hipFree(c->cur);
@@ -653,7 +658,9 @@ int ocpg_synchronize(ocpg_ctx* c)
 void* ocpg_stream(ocpg_ctx* c) { return c ? (void*)c->stream : nullptr; }
 int64_t ocpg_device_bytes(ocpg_ctx* c)
 {
-    return c ? (int64_t)(c->curBytes + c->lastBytes + c->workBytes + c->snapBytes + fimBytes(c->fim)) : -1;
+    return c ? (int64_t)(c->curBytes + c->lastBytes + c->workBytes + c->snapBytes + fimBytes(c->fim) +
+                         c->wells.bytes)
+             : -1;
 }
 
 }  // extern "C"
@@ -662,6 +669,9 @@ int64_t ocpg_device_bytes(ocpg_ctx* c)
 namespace ocpg {
 DevBulk& ctxBulk(ocpg_ctx* c) { return c->B; }
 FimDevice& ctxFim(ocpg_ctx* c) { return c->fim; }
+WellDevice& ctxWells(ocpg_ctx* c) { return c->wells; }
+FimHost& ctxFimHost(ocpg_ctx* c) { return c->fimHost; }
+const DevEos* ctxEos(ocpg_ctx* c) { return c->dEos; }
 hipStream_t ctxStream(ocpg_ctx* c) { return c->stream; }
 int ctxNC(ocpg_ctx* c) { return c->NC; }
 int ctxDevice(ocpg_ctx* c) { return c->device; }

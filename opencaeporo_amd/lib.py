@@ -44,6 +44,10 @@ def load():
         "ocpg_reset_to_last_step": ([P], C.c_int),
         "ocpg_set_conn": ([P, P], C.c_int),
         "ocpg_bsr_nnzb": ([P], C.c_int64),
+        "ocpg_bsr_rows": ([P], C.c_int64),
+        "ocpg_set_wells": ([P, P], C.c_int),
+        "ocpg_set_well_state": ([P, P], C.c_int),
+        "ocpg_get_well_flux": ([P, P, P, P], C.c_int),
         "ocpg_calkrpc": ([P], C.c_int),
         "ocpg_calrock": ([P], C.c_int),
         "ocpg_calkrpc_rock": ([P], C.c_int),
@@ -157,12 +161,33 @@ class Context:
     def assemble(self, dt):
         check(self.lib.ocpg_assemble(self.h, dt), "ocpg_assemble")
 
+    def rows(self):
+        return int(self.lib.ocpg_bsr_rows(self.h))
+
+    def set_wells(self, wells):
+        """abi.Wells -> ocpg_set_wells (rebuilds the BSR pattern)."""
+        ws = wells.cstruct()
+        check(self.lib.ocpg_set_wells(self.h, C.byref(ws)), "ocpg_set_wells")
+        self.wells = wells
+
+    def set_well_state(self, wells):
+        ws = wells.cstruct()
+        check(self.lib.ocpg_set_well_state(self.h, C.byref(ws)), "ocpg_set_well_state")
+
+    def get_well_flux(self):
+        w = self.wells
+        nc = w.nc
+        out = (np.zeros(w.nperf * nc), np.zeros(w.nw * nc), np.zeros(w.nw))
+        check(self.lib.ocpg_get_well_flux(self.h, *[a.ctypes.data for a in out]), "ocpg_get_well_flux")
+        return out
+
     def get_bsr(self):
         nnzb = self.nnzb()
-        IA = np.zeros(self.nb + 1, dtype=np.int32)
+        nrow = self.rows()
+        IA = np.zeros(nrow + 1, dtype=np.int32)
         JA = np.zeros(nnzb, dtype=np.int32)
         val = np.zeros(nnzb * self.ncol * self.ncol)
-        rhs = np.zeros(self.nb * self.ncol)
+        rhs = np.zeros(nrow * self.ncol)
         check(self.lib.ocpg_get_bsr(self.h, IA.ctypes.data, JA.ctypes.data, val.ctypes.data,
                                     rhs.ctypes.data), "ocpg_get_bsr")
         return IA, JA, val, rhs

@@ -1,6 +1,7 @@
 // C entry points of the oracle (loaded by the tests and the benchmark's
 // CPU-baseline leg through ctypes).  TEST INFRASTRUCTURE ONLY.
 #include <chrono>
+#include <cstring>
 
 #include "oracle_fim.hpp"
 
@@ -55,6 +56,68 @@ int orc_assemble_bulks(int nc, const ocpg_conn* conn, ocpg_bulk* bk, double dt,
 {
     if (!conn || !bk || !upblock || !upblock_Rho || !IA || !JA || !val) return OCPG_EINVAL;
     oracle::AssembleBulks(nc, *conn, *bk, dt, upblock, upblock_Rho, IA, JA, val);
+    return OCPG_OK;
+}
+
+// Well::CalTrans + Well::CalFlux + Well::CalProdWeight of every open well.
+int orc_well_flux(const ocpg_eos* eos, const ocpg_wells* wells, ocpg_bulk* bk, double* perfTransj,
+                  double* perfQi, double* wellQi, double* prodWeight)
+{
+    if (!eos || !wells || !bk) return OCPG_EINVAL;
+    oracle::WellFlux wf;
+    oracle::CalWellFlux(*eos, *wells, *bk, wf);
+    const int nc = eos->nc + 1;
+    const int nperf = wells->nw > 0 ? wells->perfStart[wells->nw] : 0;
+    if (perfTransj) std::memcpy(perfTransj, wf.perfTransj.data(), sizeof(double) * nperf * 3);
+    if (perfQi) std::memcpy(perfQi, wf.perfQi.data(), sizeof(double) * nperf * nc);
+    if (wellQi) std::memcpy(wellQi, wf.wellQi.data(), sizeof(double) * wells->nw * nc);
+    if (prodWeight) std::memcpy(prodWeight, wf.prodWeight.data(), sizeof(double) * wells->nw);
+    return OCPG_OK;
+}
+
+// MixtureComp::CalProdWeight on the oracle's restatement.
+int orc_prod_weight(const ocpg_eos* eos, double Psurf, double Tsurf, const double* Ni, const double* prodPhase,
+                    double* weight)
+{
+    if (!eos || !Ni || !prodPhase || !weight) return OCPG_EINVAL;
+    oracle::Mixture mix(*eos);
+    *weight = mix.ProdWeight(Psurf, Tsurf, Ni, prodPhase);
+    return OCPG_OK;
+}
+
+// IsoT_FIM::UpdateProperty's well part + CalRes with wells: resAbs holds
+// (nb + open wells) * (nc + 1) entries.
+int orc_calres_wells(const ocpg_eos* eos, const ocpg_wells* wells, const ocpg_conn* conn, ocpg_bulk* bk,
+                     double dt, double* resAbs, int32_t* upblock, double* upblock_Rho,
+                     double* upblock_Velocity, ocpg_resnorm* norms)
+{
+    if (!eos || !wells || !conn || !bk || !resAbs || !upblock || !upblock_Rho || !upblock_Velocity)
+        return OCPG_EINVAL;
+    oracle::WellFlux wf;
+    oracle::CalWellFlux(*eos, *wells, *bk, wf);
+    oracle::CalRes(eos->nc + 1, *conn, *bk, dt, resAbs, upblock, upblock_Rho, upblock_Velocity, norms, wells,
+                   &wf);
+    return OCPG_OK;
+}
+
+int orc_bsr_pattern_wells(int64_t nb, const ocpg_conn* conn, const ocpg_wells* wells, int32_t* IA, int32_t* JA)
+{
+    if (!conn || !wells || !IA || !JA) return OCPG_EINVAL;
+    oracle::BsrPattern(nb, *conn, IA, JA, wells);
+    return OCPG_OK;
+}
+
+// IsoT_FIM::AssembleMat: AssembleMatBulksNew then AssembleMatWellsNew.
+int orc_assemble_wells(const ocpg_eos* eos, const ocpg_wells* wells, const ocpg_conn* conn, ocpg_bulk* bk,
+                       double dt, const int32_t* upblock, const double* upblock_Rho, const int32_t* IA,
+                       const int32_t* JA, double* val)
+{
+    if (!eos || !wells || !conn || !bk || !upblock || !upblock_Rho || !IA || !JA || !val) return OCPG_EINVAL;
+    const int nc = eos->nc + 1;
+    oracle::AssembleBulks(nc, *conn, *bk, dt, upblock, upblock_Rho, IA, JA, val);
+    oracle::WellFlux wf;
+    oracle::CalWellFlux(*eos, *wells, *bk, wf);
+    oracle::AssembleWells(nc, *conn, *wells, wf, *bk, dt, IA, val);
     return OCPG_OK;
 }
 

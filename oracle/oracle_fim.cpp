@@ -12,6 +12,7 @@
 #include "oracle_fim.hpp"
 
 #include <cmath>
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -21,7 +22,7 @@
 
 namespace oracle {
 
-static inline int maxLendSdP(int nc) { return (nc + 1) * (nc + 1) * NPH; }
+
 
 // IsoT_FIM::PassFlashValue, src/OCPFluidMethod.cpp:1122-1187 (Newton
 // bookkeeping dSNR/maxNRdSSP is not part of this path).
@@ -230,15 +231,18 @@ void CalRock(const ocpg_rock& rock, const ocpg_bulk& bk, int nthreads)
     }
 }
 
-// IsoT_FIM::CalRes, bulk-to-bulk part and norms (no wells), serial in
-// connection order as the reference accumulates.
+// IsoT_FIM::CalRes (src/OCPFluidMethod.cpp:1201-1389): bulk-to-bulk part
+// serial in connection order as the reference accumulates, then the well
+// sources and well equations (:1284-1352), norms and negation.
 void CalRes(int nc, const ocpg_conn& conn, const ocpg_bulk& bk, double dt, double* resAbs,
-            int32_t* upblock, double* upblock_Rho, double* upblock_Velocity, ocpg_resnorm* norms)
+            int32_t* upblock, double* upblock_Rho, double* upblock_Velocity, ocpg_resnorm* norms,
+            const ocpg_wells* wells, const WellFlux* wf)
 {
     const int np = NPH;
     const int len = nc + 1;
     const int64_t nb = bk.nb;
-    std::memset(resAbs, 0, sizeof(double) * nb * len);
+    const int64_t nrow = nb + OpenWells(wells);
+    std::memset(resAbs, 0, sizeof(double) * nrow * len);
     for (int64_t n = 0; n < nb; n++) {
         const int64_t bId = n * len;
         const int64_t bIdb = n * nc;
@@ -287,7 +291,36 @@ void CalRes(int nc, const ocpg_conn& conn, const ocpg_bulk& bk, double dt, doubl
             }
         }
     }
-    ocpg_resnorm nr = {0, 0, 0, 0};
+    ocpg_resnorm nr = {0, 0, 0, 0, 0};
+    if (wells && wf) {
+        int64_t wId = nb * len;
+        for (int w = 0; w < wells->nw; w++) {
+            if (!wells->open[w]) continue;
+            for (int p = wells->perfStart[w]; p < wells->perfStart[w + 1]; p++) {
+                const int64_t k = wells->perfCell[p];
+                for (int i = 0; i < nc; i++) resAbs[k * len + 1 + i] += wf->perfQi[(size_t)p * nc + i] * dt;
+            }
+            if (wells->type[w] == OCPG_WELL_INJ) {
+                if (wells->mode[w] == OCPG_WELL_BHP) {
+                    resAbs[wId] = wells->bhp[w] - wells->ctrlBHP[w];
+                } else {
+                    resAbs[wId] = wells->ctrlRate[w];
+                    for (int i = 0; i < nc; i++) resAbs[wId] += wf->wellQi[(size_t)w * nc + i];
+                    nr.maxWellRelRes_mol = std::max(nr.maxWellRelRes_mol, std::fabs(resAbs[wId] / wells->ctrlRate[w]));
+                }
+            } else {
+                if (wells->mode[w] == OCPG_WELL_BHP) {
+                    resAbs[wId] = wells->bhp[w] - wells->ctrlBHP[w];
+                } else {
+                    resAbs[wId] = -wells->ctrlRate[w];
+                    for (int i = 0; i < nc; i++)
+                        resAbs[wId] += wf->wellQi[(size_t)w * nc + i] * wf->prodWeight[w];
+                    nr.maxWellRelRes_mol = std::max(nr.maxWellRelRes_mol, std::fabs(resAbs[wId] / wells->ctrlRate[w]));
+                }
+            }
+            wId += len;
+        }
+    }
     for (int64_t n = 0; n < nb; n++) {
         for (int i = 0; i < len; i++) {
             const double tmp = std::fabs(resAbs[n * len + i] / bk.rockVp[n]);
@@ -304,24 +337,41 @@ void CalRes(int nc, const ocpg_conn& conn, const ocpg_bulk& bk, double dt, doubl
             }
         }
     }
-    for (int64_t k = 0; k < nb * len; k++) resAbs[k] = -resAbs[k];
+    for (int64_t k = 0; k < nrow * len; k++) resAbs[k] = -resAbs[k];
     if (norms) *norms = nr;
 }
 
 // BSR structure of LinearSystem after AssembleMatBulksNew: row n holds the
 // diagonal first, then neighbours in insertion order (ascending for a sorted
 // connection list).  IA has nb+1 entries.
-void BsrPattern(int64_t nb, const ocpg_conn& conn, int32_t* IA, int32_t* JA)
+// With wells, AssembleMatWellsNew then appends to each perforated cell's row
+// the column of the well (NewOffDiag(n, wId)) and adds one row per open well:
+// its diagonal, then the perforated cells in perforation order.
+void BsrPattern(int64_t nb, const ocpg_conn& conn, int32_t* IA, int32_t* JA, const ocpg_wells* wells)
 {
-    std::vector<std::vector<int32_t>> cols(nb);
+    const int nw = OpenWells(wells);
+    std::vector<std::vector<int32_t>> cols(nb + nw);
     for (int64_t n = 0; n < nb; n++) cols[n].push_back((int32_t)n);
     for (int64_t c = 0; c < conn.nconn; c++) {
         cols[conn.eId[c]].push_back(conn.bId[c]);
         cols[conn.bId[c]].push_back(conn.eId[c]);
     }
+    if (wells) {
+        int64_t wId = nb;
+        for (int w = 0; w < wells->nw; w++) {
+            if (!wells->open[w]) continue;
+            cols[wId].push_back((int32_t)wId);
+            for (int p = wells->perfStart[w]; p < wells->perfStart[w + 1]; p++) {
+                const int32_t n = wells->perfCell[p];
+                cols[n].push_back((int32_t)wId);
+                cols[wId].push_back(n);
+            }
+            wId++;
+        }
+    }
     IA[0] = 0;
     int64_t k = 0;
-    for (int64_t n = 0; n < nb; n++) {
+    for (int64_t n = 0; n < nb + nw; n++) {
         for (int32_t col : cols[n]) JA[k++] = col;
         IA[n + 1] = (int32_t)k;
     }

@@ -3,17 +3,36 @@
 #ifndef ORACLE_FIM_HPP
 #define ORACLE_FIM_HPP
 
+#include <vector>
+
 #include "oracle_flash.hpp"
 
 namespace oracle {
+
+// Bulk::maxLendSdP, src/OCPFluidMethod.cpp:1056.
+inline int maxLendSdP(int nc) { return (nc + 1) * (nc + 1) * NPH; }
 
 void CalFlash(const ocpg_eos& eos, const ocpg_bulk& bk, int nthreads);
 void InitFlash(const ocpg_eos& eos, const ocpg_bulk& bk, const double* zi, int nthreads);
 void CalKrPc(const ocpg_satfunc& sat, const ocpg_bulk& bk, int nthreads);
 void CalRock(const ocpg_rock& rock, const ocpg_bulk& bk, int nthreads);
+// Well::CalTrans / Well::CalFlux / Well::CalProdWeight results, well-major.
+struct WellFlux {
+    std::vector<double> perfTransj;  // nperf * 3
+    std::vector<double> perfQi;      // nperf * nc
+    std::vector<double> wellQi;      // nw * nc
+    std::vector<double> prodWeight;  // nw (rate-controlled producers)
+};
+int OpenWells(const ocpg_wells* w);
+void CalWellFlux(const ocpg_eos& eos, const ocpg_wells& w, const ocpg_bulk& bk, WellFlux& out);
+// With wells, resAbs holds (nb + open wells) * (nc + 1) entries.
 void CalRes(int nc, const ocpg_conn& conn, const ocpg_bulk& bk, double dt, double* resAbs,
-            int32_t* upblock, double* upblock_Rho, double* upblock_Velocity, ocpg_resnorm* norms);
-void BsrPattern(int64_t nb, const ocpg_conn& conn, int32_t* IA, int32_t* JA);
+            int32_t* upblock, double* upblock_Rho, double* upblock_Velocity, ocpg_resnorm* norms,
+            const ocpg_wells* wells = nullptr, const WellFlux* wf = nullptr);
+void BsrPattern(int64_t nb, const ocpg_conn& conn, int32_t* IA, int32_t* JA,
+                const ocpg_wells* wells = nullptr);
+void AssembleWells(int nc, const ocpg_conn& conn, const ocpg_wells& w, const WellFlux& wf,
+                   const ocpg_bulk& bk, double dt, const int32_t* IA, double* val);
 void AssembleBulks(int nc, const ocpg_conn& conn, const ocpg_bulk& bk, double dt,
                    const int32_t* upblock, const double* upblock_Rho, const int32_t* IA,
                    const int32_t* JA, double* val);

@@ -330,6 +330,38 @@ void Mixture::CalFlash()
     CopyPhase();
 }
 
+// MixtureComp::CalProdWeight (src/MixtureComp.cpp:830-853) after
+// MixtureComp::Flash (:167-193): a cold flash of the well stream at surface
+// conditions, water from PVTW, then the stream's volume per mole in the
+// producer's rate units.
+double Mixture::ProdWeight(double Pin, double Tin, const double* Niin, const double* prodPhase)
+{
+    ftype = 0;
+    lNP = 0;
+    InitPTN(Pin, Tin + CONV5, Niin);
+    for (int j = 0; j < NPH; j++) vj[j] = 0;
+    CalFlash();
+    const int Wpid = numPhase - 1;
+    const int Wcid = numCom - 1;
+    Nt = Nh + Ni[Wcid];
+    real data[5], cdata[5];
+    PVTWEval(P, data, cdata);
+    const real Pw0 = data[0];
+    const real bw0 = data[1];
+    const real cbw = data[2];
+    const real bw = bw0 * (1 - cbw * (P - Pw0));
+    xi[Wpid] = 1 / (CONV1 * bw);
+    vj[Wpid] = CONV1 * Ni[Wcid] * bw;
+    const real qt = Nt;
+    real factor[NPH];
+    factor[0] = vj[0] / qt / CONV1;
+    factor[1] = vj[1] / qt / 1000;
+    factor[2] = xi[2] * vj[2] / qt;
+    real tmp = 0;
+    for (int i = 0; i < 3; i++) tmp += factor[i] * prodPhase[i];
+    return static_cast<double>(tmp);
+}
+
 // MixtureComp::SolEoS, src/MixtureComp.cpp:889-914.
 void Mixture::SolEoS(real& ZjT, const real& AjT, const real& BjT)
 {

@@ -49,6 +49,12 @@ public:
     void InitFlashFIM(double Pin, double Tin, const double* Sjin, double Vpore,
                       const double* Ziin, SkipState& skip);
 
+    // MixtureComp::CalProdWeight, src/MixtureComp.cpp:830-853, on the
+    // surface flash MixtureComp::Flash (:167-193).  A hydrocarbon phase the
+    // flash does not produce contributes no volume (the reference reads what
+    // its shared flash object last left in vj for it).
+    double ProdWeight(double Pin, double Tin, const double* Niin, const double* prodPhase);
+
     int NC, numCom, numPhase;
     // Outputs (Mixture base members, include/Mixture.hpp:241-292).
     real P, T;

@@ -36,6 +36,11 @@ class Oracle:
         L.orc_calres.argtypes = [C.c_int, P, P, C.c_double, P, P, P, P, P]
         L.orc_bsr_pattern.argtypes = [C.c_int64, P, P, P]
         L.orc_assemble_bulks.argtypes = [C.c_int, P, P, C.c_double, P, P, P, P, P]
+        L.orc_well_flux.argtypes = [P, P, P, P, P, P, P]
+        L.orc_prod_weight.argtypes = [P, C.c_double, C.c_double, P, P, P]
+        L.orc_calres_wells.argtypes = [P, P, P, P, C.c_double, P, P, P, P, P]
+        L.orc_bsr_pattern_wells.argtypes = [C.c_int64, P, P, P, P]
+        L.orc_assemble_wells.argtypes = [P, P, P, P, C.c_double, P, P, P, P, P]
         L.orc_time_flash_fim.argtypes = [P, P, C.c_int]
         L.orc_time_flash_fim.restype = C.c_double
         if hasattr(L, "orc_count_flash_fim"):
@@ -97,6 +102,63 @@ class Oracle:
         return val
 
 
+    # ---- wells ----
+    def prod_weight(self, eos, Psurf, Tsurf, Ni, prod_phase):
+        Ni = np.ascontiguousarray(Ni, dtype=np.float64)
+        pp = np.ascontiguousarray(prod_phase, dtype=np.float64)
+        out = np.zeros(1)
+        assert self.lib.orc_prod_weight(C.byref(eos), Psurf, Tsurf, Ni.ctypes.data, pp.ctypes.data,
+                                        out.ctypes.data) == 0
+        return float(out[0])
+
+    def well_flux(self, eos, wells, bulk):
+        nc = wells.nc
+        out = (np.zeros(wells.nperf * abi.NP), np.zeros(wells.nperf * nc), np.zeros(wells.nw * nc),
+               np.zeros(wells.nw))
+        ws, s = wells.cstruct(), bulk.cstruct()
+        assert self.lib.orc_well_flux(C.byref(eos), C.byref(ws), C.byref(s),
+                                      *[a.ctypes.data for a in out]) == 0
+        return out
+
+    def calres_wells(self, eos, wells, grid, bulk, dt):
+        nc = wells.nc
+        conn = abi.conn_struct(grid.bId, grid.eId, grid.area)
+        res = np.zeros((bulk.nb + wells.open_count()) * (nc + 1))
+        up = np.zeros(grid.nconn * abi.NP, dtype=np.int32)
+        urho = np.zeros(grid.nconn * abi.NP)
+        uvel = np.zeros(grid.nconn * abi.NP)
+        norms = abi.ResNorm()
+        ws, s = wells.cstruct(), bulk.cstruct()
+        rc = self.lib.orc_calres_wells(C.byref(eos), C.byref(ws), C.byref(conn), C.byref(s), dt,
+                                       res.ctypes.data, up.ctypes.data, urho.ctypes.data,
+                                       uvel.ctypes.data, C.byref(norms))
+        assert rc == 0
+        return res, up, urho, uvel, norms
+
+    def bsr_pattern_wells(self, grid, wells):
+        conn = abi.conn_struct(grid.bId, grid.eId, grid.area)
+        nrow = grid.nb + wells.open_count()
+        nperf_open = sum(int(wells.perfStart[w + 1] - wells.perfStart[w])
+                         for w in range(wells.nw) if wells.open[w])
+        IA = np.zeros(nrow + 1, dtype=np.int32)
+        JA = np.zeros(grid.nb + 2 * grid.nconn + wells.open_count() + 2 * nperf_open, dtype=np.int32)
+        ws = wells.cstruct()
+        assert self.lib.orc_bsr_pattern_wells(grid.nb, C.byref(conn), C.byref(ws), IA.ctypes.data,
+                                              JA.ctypes.data) == 0
+        return IA, JA
+
+    def assemble_wells(self, eos, wells, grid, bulk, dt, up, urho, IA, JA):
+        conn = abi.conn_struct(grid.bId, grid.eId, grid.area)
+        ncol = wells.nc + 1
+        val = np.zeros(int(IA[-1]) * ncol * ncol)
+        ws, s = wells.cstruct(), bulk.cstruct()
+        rc = self.lib.orc_assemble_wells(C.byref(eos), C.byref(ws), C.byref(conn), C.byref(s), dt,
+                                         up.ctypes.data, urho.ctypes.data, IA.ctypes.data,
+                                         JA.ctypes.data, val.ctypes.data)
+        assert rc == 0
+        return val
+
+
 class FlopCounter(Oracle):
     def __init__(self):
         super().__init__(COUNT_SO)
@@ -128,6 +190,7 @@ class Reference:
         L.ref_get_eos.argtypes = [P, P]
         L.ref_flash_fim.argtypes = [P, P]
         L.ref_init_flash_fim.argtypes = [P, P, P]
+        L.ref_prod_weight.argtypes = [P, C.c_double, C.c_double, P, P, P]
         self.deck = deck
         self.h = L.ref_create(C.byref(deck), nb)
 
@@ -144,6 +207,14 @@ class Reference:
         zi = np.ascontiguousarray(zi, dtype=np.float64)
         s = bulk.cstruct()
         assert self.lib.ref_init_flash_fim(self.h, C.byref(s), zi.ctypes.data) == 0
+
+    def prod_weight(self, Psurf, Tsurf, Ni, prod_phase):
+        Ni = np.ascontiguousarray(Ni, dtype=np.float64)
+        pp = np.ascontiguousarray(prod_phase, dtype=np.float64)
+        out = np.zeros(1)
+        assert self.lib.ref_prod_weight(self.h, Psurf, Tsurf, Ni.ctypes.data, pp.ctypes.data,
+                                        out.ctypes.data) == 0
+        return float(out[0])
 
     def __del__(self):
         try:

@@ -260,4 +260,21 @@ int ref_init_flash_fim(void* p, ocpg_bulk* bk, const double* zi)
     return 0;
 }
 
+// MixtureComp::CalProdWeight (src/MixtureComp.cpp:830-853) on the
+// reference's own object.  The phase volumes vj start from zero, as in a
+// freshly allocated object, so a phase absent at surface conditions
+// contributes nothing.
+int ref_prod_weight(void* p, double Psurf, double Tsurf, const double* Ni, const double* prodPhase,
+                    double* weight)
+{
+    RefHandle* h = static_cast<RefHandle*>(p);
+    if (!h || !Ni || !prodPhase || !weight) return OCPG_EINVAL;
+    Quiet q;
+    std::fill(h->mix->vj.begin(), h->mix->vj.end(), 0.0);
+    std::vector<OCP_DBL> pp(prodPhase, prodPhase + 3), w(h->NC + 1, 0.0);
+    h->mix->CalProdWeight(Psurf, Tsurf, Ni, pp, w);
+    *weight = w[0];
+    return OCPG_OK;
+}
+
 }  // extern "C"

@@ -37,3 +37,45 @@ def build_case(nx=8, ny=6, nz=5, seed=7, deck=None, dP=20.0, dN=0.01):
     orc.calkrpc(sat, b)
     orc.calrock(rock, b)
     return dict(eos=eos, sat=sat, rock=rock, grid=grid, bulk=b, NC=NC, orc=orc)
+
+
+def add_wells(case, spec=None, seed=11):
+    """Wells on a case's grid (abi.Wells).  The default set exercises every
+    branch: a rate injector, a BHP injector, a rate producer and a BHP
+    producer, one cell perforated by two wells.  Injector perforation molar
+    densities and the dG offsets are per-step inputs; they get plausible
+    values here."""
+    b, NC = case["bulk"], case["NC"]
+    nc = NC + 1
+    nb = b.nb
+    rng = np.random.default_rng(seed)
+    P = b.P
+    inj_z = np.zeros(nc)
+    inj_z[0] = 0.9
+    inj_z[1] = 0.1
+    if spec is None:
+        c = [0, nb // 2, nb - 1, nb // 3, (2 * nb) // 3]
+        spec = [
+            dict(type=abi.WELL_INJ, mode=abi.WELL_RATE, perf=[c[0], c[1]], ctrlRate=500.0),
+            dict(type=abi.WELL_PROD, mode=abi.WELL_BHP, perf=[c[2], c[1]]),
+            dict(type=abi.WELL_PROD, mode=abi.WELL_RATE, perf=[c[3]], ctrlRate=20.0,
+                 prodPhaseWeight=[1.0, 0.0, 0.0]),
+            dict(type=abi.WELL_INJ, mode=abi.WELL_BHP, perf=[c[4]]),
+        ]
+    wells = []
+    for s in spec:
+        perf = np.asarray(s["perf"], dtype=np.int32)
+        Pp = P[perf]
+        inj = s["type"] == abi.WELL_INJ
+        bhp = s.get("bhp", (Pp.max() + 300.0) if inj else (Pp.min() - 300.0))
+        w = dict(type=s["type"], mode=s["mode"], perf=perf, bhp=bhp,
+                 ctrlBHP=s.get("ctrlBHP", bhp + (25.0 if inj else -25.0)),
+                 ctrlRate=s.get("ctrlRate", 100.0), open=s.get("open", 1),
+                 WI=rng.uniform(5.0, 50.0, len(perf)), mult=np.ones(len(perf)),
+                 dG=rng.uniform(-5.0, 5.0, len(perf)),
+                 prodPhaseWeight=s.get("prodPhaseWeight", [1.0, 0.0, 0.0]))
+        if inj:
+            w["injZi"] = inj_z
+            w["perfXi"] = rng.uniform(0.4, 0.6, len(perf))
+        wells.append(w)
+    return abi.Wells(nc, wells)

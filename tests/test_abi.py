@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
 @pytest.mark.parametrize("name,cls", [("ocpg_eos", abi.Eos), ("ocpg_eos_deck", abi.EosDeck),
                                       ("ocpg_satfunc", abi.SatFunc), ("ocpg_rock", abi.Rock),
                                       ("ocpg_bulk", abi.BulkC), ("ocpg_conn", abi.Conn),
-                                      ("ocpg_resnorm", abi.ResNorm),
+                                      ("ocpg_resnorm", abi.ResNorm), ("ocpg_wells", abi.WellsC),
                                       ("ocpg_flash_stats", abi.FlashStats)])
 def test_struct_sizes_match(name, cls):
     assert lib.load().ocpg_sizeof(name.encode()) == C.sizeof(cls)

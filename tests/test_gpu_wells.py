@@ -1,0 +1,154 @@
+"""HIP well terms (Well::CalTrans/CalFlux/CalProdWeight, the well rows of
+CalRes, AssembleMatWellsNew) against the oracle through the C-ABI: the
+pattern and every block, on the same cell state."""
+import numpy as np
+import pytest
+
+from opencaeporo_amd import abi, lib
+from tests import fim_case, gpu_util
+from tests.test_gpu_fim import block_err
+
+pytestmark = pytest.mark.gpu
+
+DT = 5.0
+
+
+def oracle_system(case, wells, b):
+    orc, grid = case["orc"], case["grid"]
+    res, up, urho, uvel, norms = orc.calres_wells(case["eos"], wells, grid, b, DT)
+    IA, JA = orc.bsr_pattern_wells(grid, wells)
+    val = orc.assemble_wells(case["eos"], wells, grid, b, DT, up, urho, IA, JA)
+    tj, qp, qw, pw = orc.well_flux(case["eos"], wells, b)
+    return dict(res=res, norms=norms, IA=IA, JA=JA, val=val, qp=qp, qw=qw, pw=pw)
+
+
+def gpu_system(ctx, case, wells):
+    ctx.set_conn(case["grid"])
+    ctx.set_wells(wells)
+    norms = ctx.calres(DT)
+    ctx.assemble(DT)
+    IA, JA, val, rhs = ctx.get_bsr()
+    qp, qw, pw = ctx.get_well_flux()
+    return dict(res=rhs, norms=norms, IA=IA, JA=JA, val=val, qp=qp, qw=qw, pw=pw)
+
+
+def rate_producer_rows(wells, nb):
+    return [nb + wells.row_of(w) for w in range(wells.nw)
+            if wells.open[w] and wells.type[w] == abi.WELL_PROD and wells.mode[w] == abi.WELL_RATE]
+
+
+def compare(case, wells, got, ref):
+    b = case["bulk"]
+    nb = b.nb
+    ncol = case["NC"] + 2
+    assert np.array_equal(got["IA"], ref["IA"]) and np.array_equal(got["JA"], ref["JA"])
+    # perforation and well rates: the same arithmetic in the same order
+    np.testing.assert_array_equal(got["qp"], ref["qp"])
+    np.testing.assert_array_equal(got["qw"], ref["qw"])
+    # producer surface weights come from a flash at surface conditions
+    np.testing.assert_allclose(got["pw"], ref["pw"], rtol=1e-12, atol=0)
+    prow = set(rate_producer_rows(wells, nb))
+    R, Rr = got["res"].reshape(-1, ncol), ref["res"].reshape(-1, ncol)
+    for r in range(R.shape[0]):
+        if r in prow:
+            np.testing.assert_allclose(R[r], Rr[r], rtol=1e-12, atol=1e-12 * np.abs(Rr[r]).max())
+        else:
+            np.testing.assert_array_equal(R[r], Rr[r])
+    assert got["norms"].maxRelRes_V == ref["norms"].maxRelRes_V
+    assert got["norms"].maxRelRes_N == ref["norms"].maxRelRes_N
+    np.testing.assert_allclose(got["norms"].maxWellRelRes_mol, ref["norms"].maxWellRelRes_mol, rtol=1e-12)
+    err = block_err(got["val"], ref["val"], ncol)
+    IA = ref["IA"]
+    loose = np.zeros(len(err), bool)
+    for r in prow:
+        loose[IA[r]:IA[r + 1]] = True
+    gpu_util.say("  blocks", len(err), "exact", np.mean(err == 0), "max err (rate-producer rows)",
+                 err[loose].max() if loose.any() else 0.0)
+    assert np.all(err[~loose] == 0), np.flatnonzero((err != 0) & ~loose)[:10]
+    assert np.all(err[loose] <= 1e-12)
+
+
+@pytest.mark.parametrize("shape,seed", [((8, 6, 5), 7), ((7, 7, 3), 3)])
+def test_wells_match_oracle(shape, seed):
+    gpu_util.say(f"[well parity] grid {shape} seed {seed}")
+    case = fim_case.build_case(*shape, seed=seed)
+    wells = fim_case.add_wells(case, seed=seed)
+    b = case["bulk"]
+    ref = oracle_system(case, wells, b)
+    ctx = lib.Context(case["eos"], case["sat"], case["rock"])
+    ctx.set_cells(b.nb)
+    ctx.upload(b)
+    got = gpu_system(ctx, case, wells)
+    compare(case, wells, got, ref)
+    ctx.close()
+
+
+def test_two_well_config_and_state_update():
+    """The 147-cell two-well case (gas injector and producer at opposite
+    corners); then a new BHP and control through ocpg_set_well_state."""
+    gpu_util.say("[well parity] 7x7x3, injector + producer, state update")
+    case = fim_case.build_case(7, 7, 3, seed=1)
+    nb = case["bulk"].nb
+    spec = [dict(type=abi.WELL_INJ, mode=abi.WELL_RATE, perf=[0, 49, 98], ctrlRate=1000.0),
+            dict(type=abi.WELL_PROD, mode=abi.WELL_BHP, perf=[48, 97, 146])]
+    wells = fim_case.add_wells(case, spec=spec, seed=2)
+    b = case["bulk"]
+    ctx = lib.Context(case["eos"], case["sat"], case["rock"])
+    ctx.set_cells(nb)
+    ctx.upload(b)
+    compare(case, wells, gpu_system(ctx, case, wells), oracle_system(case, wells, b))
+    wells.bhp[:] = wells.bhp + np.array([50.0, -40.0])
+    wells.mode[1] = abi.WELL_RATE
+    wells.ctrlRate[1] = 300.0
+    ctx.set_well_state(wells)
+    norms = ctx.calres(DT)
+    ctx.assemble(DT)
+    IA, JA, val, rhs = ctx.get_bsr()
+    qp, qw, pw = ctx.get_well_flux()
+    compare(case, wells, dict(res=rhs, norms=norms, IA=IA, JA=JA, val=val, qp=qp, qw=qw, pw=pw),
+            oracle_system(case, wells, b))
+    ctx.close()
+
+
+def test_shut_well_and_removal():
+    """A shut well contributes nothing and owns no row; nw = 0 restores the
+    bulk-only system."""
+    case = fim_case.build_case(5, 4, 3, seed=4)
+    b = case["bulk"]
+    wells = fim_case.add_wells(case, seed=4)
+    wells.open[1] = 0
+    ctx = lib.Context(case["eos"], case["sat"], case["rock"])
+    ctx.set_cells(b.nb)
+    ctx.upload(b)
+    compare(case, wells, gpu_system(ctx, case, wells), oracle_system(case, wells, b))
+    assert ctx.rows() == b.nb + wells.nw - 1
+    none = abi.Wells(case["NC"] + 1, [])
+    ctx.set_wells(none)
+    assert ctx.rows() == b.nb
+    ctx.calres(DT)
+    ctx.assemble(DT)
+    IA, JA, val, rhs = ctx.get_bsr()
+    res, up, urho, uvel, norms = case["orc"].calres(case["NC"] + 1, case["grid"], b, DT)
+    np.testing.assert_array_equal(rhs, res)
+    ctx.close()
+
+
+def test_bad_wells_are_rejected():
+    case = fim_case.build_case(3, 3, 2, seed=4)
+    b = case["bulk"]
+    ctx = lib.Context(case["eos"], case["sat"], case["rock"])
+    ctx.set_cells(b.nb)
+    ctx.upload(b)
+    wells = fim_case.add_wells(case, seed=4)
+    # before the connections
+    with pytest.raises(lib.OcpgError):
+        ctx.set_wells(wells)
+    ctx.set_conn(case["grid"])
+    wells.perfCell[0] = b.nb  # outside the grid
+    with pytest.raises(lib.OcpgError):
+        ctx.set_wells(wells)
+    wells = fim_case.add_wells(case, seed=4)
+    wells.mode[0] = 7
+    with pytest.raises(lib.OcpgError):
+        ctx.set_wells(wells)
+    ctx.close()
