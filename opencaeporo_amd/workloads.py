@@ -103,3 +103,57 @@ def cartesian_grid(nx, ny, nz, seed, dx=100.0, dy=100.0, dz=10.0, tops=8325.0,
     nb = nx * ny * nz
     k = np.exp(rng.normal(lnk_mean, lnk_sigma, nb))
     return Grid(nx, ny, nz, dx, dy, dz, tops, (k, k, k))
+
+
+def faulted_grid(nx, ny, nz, seed, frac=0.05, dx=100.0, dy=100.0, dz=10.0, tops=8325.0,
+                 lnk_mean=np.log(100.0), lnk_sigma=1.5):
+    """The orthogonal grid with a fraction ``frac`` of its (i, j) columns
+    thrown down by half a layer: every lateral face of a faulted column
+    connects layer k to layers k and k+1 of the neighbouring column (two
+    non-neighbour connections of half the face transmissibility each).  A
+    faulted column's cells have up to 10 connections, cells between two
+    faulted columns up to 12.  Connections stay in BulkConn order (sorted by
+    bId, then eId, with bId < eId)."""
+    g = cartesian_grid(nx, ny, nz, seed, dx, dy, dz, tops, lnk_mean, lnk_sigma)
+    rng = np.random.default_rng(seed + 101)
+    ncol = nx * ny
+    faulted = np.zeros(ncol, dtype=bool)
+    faulted[rng.choice(ncol, size=max(1, int(round(frac * ncol))), replace=False)] = True
+    nxny = nx * ny
+    col_of = lambda c: c % nxny
+    b, e, a = g.bId.astype(np.int64), g.eId.astype(np.int64), g.area.copy()
+    lateral = (e - b) < nxny
+    hit = lateral & (faulted[col_of(b)] | faulted[col_of(e)])
+    keep_b, keep_e, keep_a = [b[~hit]], [e[~hit]], [a[~hit]]
+    hb, he, ha = b[hit], e[hit], a[hit]
+    # the face keeps half its transmissibility to the same layer ...
+    keep_b.append(hb)
+    keep_e.append(he)
+    keep_a.append(ha / 2)
+    # ... and gains a connection of the other half to the next layer down of
+    # the neighbouring column (the thrown side sits half a layer lower)
+    down = he + nxny < nx * ny * nz
+    k_of = lambda c: c // nxny
+    down &= k_of(hb) < nz - 1
+    nb_ = hb[down]
+    ne_ = he[down] + nxny
+    lo, hi = np.minimum(nb_, ne_), np.maximum(nb_, ne_)
+    keep_b.append(lo)
+    keep_e.append(hi)
+    keep_a.append(ha[down] / 2)
+    bId = np.concatenate(keep_b)
+    eId = np.concatenate(keep_e)
+    area = np.concatenate(keep_a)
+    order = np.lexsort((eId, bId))
+    bId, eId, area = bId[order], eId[order], area[order]
+    # merge duplicates (a pair reached twice adds its transmissibilities)
+    key = bId * (nx * ny * nz) + eId
+    uniq, inv = np.unique(key, return_inverse=True)
+    area_u = np.zeros(len(uniq))
+    np.add.at(area_u, inv, area)
+    g.bId = (uniq // (nx * ny * nz)).astype(np.int32)
+    g.eId = (uniq % (nx * ny * nz)).astype(np.int32)
+    g.area = area_u
+    g.nconn = len(g.bId)
+    g.faulted_columns = np.flatnonzero(faulted)
+    return g

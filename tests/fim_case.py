@@ -7,7 +7,9 @@ from opencaeporo_amd import abi, fluids, workloads
 from oracle.oracle_py import Oracle
 
 
-def build_case(nx=8, ny=6, nz=5, seed=7, deck=None, dP=20.0, dN=0.01):
+def build_case(nx=8, ny=6, nz=5, seed=7, deck=None, dP=20.0, dN=0.01, grid=None):
+    """grid: None (orthogonal 7-point) or "faulted" (workloads.faulted_grid,
+    non-neighbour connections across thrown columns)."""
     orc = Oracle()
     from opencaeporo_amd import lib
     deck = deck or fluids.spe5_deck()
@@ -15,7 +17,10 @@ def build_case(nx=8, ny=6, nz=5, seed=7, deck=None, dP=20.0, dN=0.01):
     sat = fluids.make_satfunc()
     rock = fluids.make_rock()
     NC = eos.nc
-    grid = workloads.cartesian_grid(nx, ny, nz, seed=seed)
+    if grid == "faulted":
+        grid = workloads.faulted_grid(nx, ny, nz, seed=seed, frac=0.2)
+    else:
+        grid = workloads.cartesian_grid(nx, ny, nz, seed=seed)
     nb = grid.nb
     P, z, Sw = workloads.cell_inputs(nb, NC, seed=seed + 1)
     b = abi.Bulk(nb, NC)

@@ -152,3 +152,32 @@ def test_bad_wells_are_rejected():
     with pytest.raises(lib.OcpgError):
         ctx.set_wells(wells)
     ctx.close()
+
+
+@pytest.mark.parametrize("deck", ["spe5", "ten"])
+def test_irregular_stencil_with_wells(deck):
+    """The irregular-stencil configuration in small: thrown columns give
+    non-neighbour connections (up to 12 per cell), wells with several
+    perforations, and the 10-component fluid (12 x 12 blocks)."""
+    from opencaeporo_amd import fluids
+    gpu_util.say(f"[irregular parity] faulted 10x8x6, {deck}")
+    d = fluids.ten_deck() if deck == "ten" else fluids.spe5_deck()
+    case = fim_case.build_case(10, 8, 6, seed=12, deck=d, grid="faulted")
+    nb = case["bulk"].nb
+    rng = np.random.default_rng(5)
+    spec = []
+    for w in range(6):
+        perf = rng.choice(nb, size=int(rng.integers(2, 6)), replace=False)
+        inj = w % 3 == 0
+        spec.append(dict(type=abi.WELL_INJ if inj else abi.WELL_PROD,
+                         mode=abi.WELL_RATE if w % 2 == 0 else abi.WELL_BHP,
+                         perf=perf, ctrlRate=200.0 if inj else 30.0,
+                         prodPhaseWeight=[1.0, 0.0, 0.0] if w % 4 else [1.0, 0.0, 1.0]))
+    wells = fim_case.add_wells(case, spec=spec, seed=6)
+    b = case["bulk"]
+    ref = oracle_system(case, wells, b)
+    ctx = lib.Context(case["eos"], case["sat"], case["rock"])
+    ctx.set_cells(nb)
+    ctx.upload(b)
+    compare(case, wells, gpu_system(ctx, case, wells), ref)
+    ctx.close()
