@@ -32,6 +32,15 @@ from opencaeporo_amd import abi, fluids, lib, workloads  # noqa: E402
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 DT = 5.0
+# Algorithmic FP64 work of the warm flash pass per cell (add, sub, mul, div,
+# sqrt; transcendental calls counted apart), measured with the oracle's
+# counting build (oracle/liboracle_count.so, FlopCounter.count_flash_stages)
+# on this exact workload, all 1,122,000 cells: 21,584,690,202 flops and
+# 542,748,484 transcendentals in total, of which PhaseSplit 13,922,832,587
+# flops and 292,374,158 transcendentals.
+WARM_FLOPS_PER_CELL = 21584690202 / 1122000
+WARM_SPLIT_FLOPS_PER_CELL = 13922832587 / 1122000
+WARM_SPLIT_TRANS_PER_CELL = 292374158 / 1122000
 
 
 def log(*a):
@@ -172,7 +181,8 @@ def main():
     ctx.synchronize()
     t1 = time.perf_counter()
     ms_step = (t1 - t0) * 1e3 / a.steps
-    kern = {k: ctx.timer(k) for k in ("restore", "flash", "krpc_rock", "calres", "assemble")}
+    kern = {k: ctx.timer(k) for k in ("restore", "flash", "flash_stability", "flash_split", "flash_finish",
+                                      "krpc_rock", "calres", "assemble")}
     ctx.timing(False)
     per = {k: (v[0] / max(v[1], 1)) for k, v in kern.items()}
     log("[bench] per-launch ms: " + ", ".join(f"{k} {v:.3f}" for k, v in per.items()))
@@ -184,10 +194,22 @@ def main():
     asm_bytes = nnzb * ncol * ncol * 8 + nnzb * 4 * 3 + (nb + 1) * 4 + nb * 8 * (
         1 + 3 * 3 + 2 * 3 * (NC + 1) + 2 * 9 + 2 + 3 + (3 + 2 * NC) * ncol) * 1.0 + grid.nconn * 3 * 12
     flash_ms = per["flash"]
-    dominant = max(per, key=per.get)
-    roof = dict(kernel="assemble", bound="hbm", achieved=asm_bytes / (per["assemble"] / 1e3) / 1e9,
-                peak=HBM_PEAK_GBS, unit="GB/s", traffic=None)
-    roof["frac"] = roof["achieved"] / roof["peak"]
+    launches = {k: v for k, v in per.items() if k not in ("flash",)}
+    dominant = max(launches, key=launches.get)
+    # roofline of the dominant kernel: the split (FP64-bound; vector and
+    # matrix FP64 peaks coincide on MI355X) or the assembly (HBM-bound)
+    split_flops = WARM_SPLIT_FLOPS_PER_CELL * nb
+    roof_split = dict(kernel="flashSplitKernel", bound="mfma", note="FP64 VALU (no MFMA): vector and matrix "
+                      "FP64 peaks coincide on MI355X; flops exclude the transcendental calls",
+                      achieved=split_flops / (per["flash_split"] / 1e3) / 1e12, peak=FP64_PEAK_TFLOPS,
+                      unit="TFLOP/s", traffic=None, flops_per_launch=split_flops,
+                      transcendentals_per_launch=WARM_SPLIT_TRANS_PER_CELL * nb)
+    roof_asm = dict(kernel="assembleKernel", bound="hbm", achieved=asm_bytes / (per["assemble"] / 1e3) / 1e9,
+                    peak=HBM_PEAK_GBS, unit="GB/s", traffic=None, bytes_per_launch=asm_bytes)
+    for r in (roof_split, roof_asm):
+        r["frac"] = r["achieved"] / r["peak"]
+    roof = roof_split if per["flash_split"] >= per["assemble"] else roof_asm
+    other = roof_asm if roof is roof_split else roof_split
     cpu = None
     if rank == 0 and not a.no_cpu:
         try:
@@ -211,7 +233,8 @@ def main():
                                f"{grid.nconn} connections, 6-comp PR, warm Newton pass, 8x8 BSR",
                    "cells": nb, "connections": grid.nconn, "block": ncol,
                    "kernel_ms": per, "dominant_kernel": dominant, "flash_ms": flash_ms,
-                   "asm_bytes_per_launch": asm_bytes, "c_bar": c_bar},
+                   "flash_flops_per_cell": WARM_FLOPS_PER_CELL, "asm_bytes_per_launch": asm_bytes,
+                   "c_bar": c_bar, "roofline_other": other},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -69,6 +69,8 @@ struct DevBulk {
 
 // Scratch of the staged flash (cells in SoA with the bulk's ld; lists of
 // cell indices with their lengths in count[4]).
+constexpr int kFlashLists = 7;
+
 struct FlashWork {
     double* zi;      // NC per cell: feed mole fractions
     double* Nh;      // hydrocarbon moles
@@ -76,11 +78,15 @@ struct FlashWork {
     double* x;       // 2*NC: split compositions
     double* n;       // 2*NC: split moles (fractions of Nh)
     double* nu;      // 2: phase fractions
+    double* Ks;      // NC: K-values of the split's successive substitution
+    double* Se;      // its last residual
+    int32_t* ssmIt;  // its iterations so far
+    int32_t* rrIt;   // Rachford-Rice iterations so far
     int8_t* ftype;
     int8_t* lNP;
     uint8_t* flags;
-    int32_t* list[4];
-    int32_t* count;  // 4 counters
+    int32_t* list[kFlashLists];
+    int32_t* count;  // list lengths [0, 8), work-queue cursors [8, 16)
 };
 
 // Totals gathered by the flash kernels (one atomic per wavefront).

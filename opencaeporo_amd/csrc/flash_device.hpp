@@ -609,6 +609,59 @@ struct FlashCell {
         it[IT_SSMSP] += iter;
     }
 
+    // SplitSSM2(false) run in slices: at most `budget` iterations per call
+    // (budget < 0: no limit), resuming from (Ks, Se, iter).  The iterations
+    // are those of splitSSM2, in the same order; returns true once the loop
+    // has ended (converged, or past the iteration cap with ssmSpConflag
+    // cleared).  The caller sets ssmSpConflag = true, Se = 1, iter = 0 and
+    // the initial Ks before the first slice and adds iter to it[IT_SSMSP]
+    // after the last.
+    __device__ __forceinline__ bool splitSSM2Slice(double& Se, int& iter, int budget)
+    {
+        double Stol = E.ssmSpTol2;
+        int maxIt = E.ssmSpMaxIt;
+        if (Yt < 1.1) {
+            Stol *= 1E-1;
+            maxIt *= 2;
+        }
+        int steps = 0;
+        while (Se > Stol) {
+            if (steps == budget) return false;
+            rachfordRice2();
+            calFugPhiAll();
+            Se = 0;
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                const double r = fug[1][i] / fug[0][i] - 1;
+                Se += r * r;
+                Ks[i] = phi[1][i] / phi[0][i];
+            }
+            iter++;
+            steps++;
+            if (iter > maxIt) {
+                ssmSpConflag = false;
+                break;
+            }
+        }
+        return true;
+    }
+
+    // The rest of PhaseSplit after its first SplitSSM2: Newton, the restart
+    // loop and CheckSplit.
+    __device__ __forceinline__ void phaseSplitAfterSSM()
+    {
+        splitNR();
+        int restarts = 0;
+        while (!nrSpConflag) {
+            splitSSM2(true);
+            splitNR();
+            if (!checkSplit()) break;
+            if (ssmSpConflag) break;
+            if (++restarts >= kMaxSplitRestarts) break;
+        }
+        checkSplit();
+    }
+
     // d ln f_ij / d n_kj of phase j (CalFugNAll, src/MixtureComp.cpp:2168-2250),
     // packed lower triangle (the matrix is symmetric), plus Zn.
     __device__ __forceinline__ void calFugN(int j, double* fugn, double* Znj) const

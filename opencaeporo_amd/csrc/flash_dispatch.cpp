@@ -17,10 +17,10 @@ bool flashSupportsNC(int nc)
 }
 
 hipError_t launchFlash(int NC, const DevEos* eos, const DevBulk& B, const FlashWork& W, DevFlashCounters* ctr,
-                       const double* zin, hipStream_t s)
+                       const double* zin, hipStream_t s, hipEvent_t* ev)
 {
 #define X(N) \
-    if (NC == N) return launchFlashT<N>(eos, B, W, ctr, zin, s);
+    if (NC == N) return launchFlashT<N>(eos, B, W, ctr, zin, s, ev);
     OCPG_NC_LIST(X)
 #undef X
     return hipErrorInvalidValue;

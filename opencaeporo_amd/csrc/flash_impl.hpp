@@ -521,7 +521,15 @@ __device__ __forceinline__ int calFtypeFIM(const FlashCell<NC>& F, const DevBulk
 // ---------------------------------------------------------------------------
 
 enum { FL_SKIP = 1, FL_SSMSTA = 2, FL_NRSTA = 4, FL_SSMSP = 8, FL_NRSP = 16 };
-enum { LIST_STAB = 0, LIST_SPLIT = 1, LIST_NP1 = 2, LIST_NP2 = 3 };
+enum { LIST_STAB = 0, LIST_SPLIT = 1, LIST_NP1 = 2, LIST_NP2 = 3, LIST_C1 = 4, LIST_C2 = 5, LIST_NR = 6 };
+// Blocks of a work-queue stage: 256 CUs x 4 SIMDs x 2 waves, in blocks of
+// kFlashBlock / 64 waves.
+constexpr unsigned kQueueBlocks = 256 * 4 * 2 / (kFlashBlock / 64);
+// SplitSSM2 iterations of the first two successive-substitution slices (the
+// warm pass needs 8 or fewer in half of its two-phase cells, 15 at the 90th
+// percentile); the third slice runs to the end.
+constexpr int kSplitBudget0 = 8;
+constexpr int kSplitBudget1 = 24;
 
 // Append the active lanes' cells to list k: one atomic per wavefront.
 __device__ __forceinline__ void appendList(const FlashWork& W, int k, bool pred, int32_t cell)
@@ -537,6 +545,15 @@ __device__ __forceinline__ void appendList(const FlashWork& W, int k, bool pred,
         const unsigned long long below = (lane == 0) ? 0ull : (mask & ((1ull << lane) - 1ull));
         W.list[k][base + __popcll(below)] = cell;
     }
+}
+
+// Work queue over list k: each wavefront takes the next 64 entries (its
+// base index, or a value >= the list length when the list is exhausted).
+__device__ __forceinline__ int nextChunk(const FlashWork& W, int k)
+{
+    int base = 0;
+    if ((threadIdx.x & 63) == 0) base = atomicAdd(&W.count[8 + k], 64);
+    return __shfl(base, 0, 64);
 }
 
 __device__ __forceinline__ unsigned long long waveSum(unsigned long long v)
@@ -556,7 +573,7 @@ __device__ __forceinline__ void addCounter(unsigned long long* dst, unsigned lon
 template <int NC>
 __global__ void flashResetKernel(FlashWork W, DevFlashCounters* ctr)
 {
-    if (threadIdx.x < 4) W.count[threadIdx.x] = 0;
+    if (threadIdx.x < 16) W.count[threadIdx.x] = 0;
     if (ctr && threadIdx.x < sizeof(DevFlashCounters) / sizeof(unsigned long long))
         reinterpret_cast<unsigned long long*>(ctr)[threadIdx.x] = 0;
 }
@@ -636,8 +653,10 @@ __global__ void __launch_bounds__(kFlashBlock) flashStabilityKernel(const DevEos
                                                                     FlashWork W, DevFlashCounters* ctr)
 {
     __shared__ double sSq[FlashCell<NC>::NT * kFlashBlock];
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = idx < W.count[LIST_STAB];
+    const int total = W.count[LIST_STAB];
+    for (int base = nextChunk(W, LIST_STAB); base < total; base = nextChunk(W, LIST_STAB)) {
+    const int idx = base + (threadIdx.x & 63);
+    const bool active = idx < total;
     bool stable = false;
     int it0 = 0, it1 = 0;
     bool nonconv = false;
@@ -673,40 +692,128 @@ __global__ void __launch_bounds__(kFlashBlock) flashStabilityKernel(const DevEos
         addCounter(&ctr->iters[IT_NRSTA], (unsigned long long)it1);
         addCounter(&ctr->nonconverged, nonconv ? 1 : 0);
     }
+    }
 }
 
-// Stage 3: PhaseSplit (two hydrocarbon phases), src/MixtureComp.cpp:1743-1782.
+// Stage 3: PhaseSplit (two hydrocarbon phases), src/MixtureComp.cpp:1743-1782,
+// as successive-substitution slices and a Newton stage.  A slice runs at most
+// `budget` SplitSSM2 iterations per cell (budget < 0: to the end); cells
+// still iterating go to list `cont` with their state (Ks, Se, iteration
+// counts), finished ones to LIST_NR.  Bounding the iterations per slice keeps
+// a wavefront from idling on its slowest lane: the cells that need many
+// iterations continue together in the next slice.
 template <int NC>
-__global__ void __launch_bounds__(kFlashBlock) flashSplitKernel(const DevEos* __restrict__ eos, DevBulk B,
-                                                                FlashWork W, DevFlashCounters* ctr)
+__global__ void __launch_bounds__(kFlashBlock) flashSplitSSMKernel(const DevEos* __restrict__ eos, DevBulk B,
+                                                                   FlashWork W, int in, int cont, int budget,
+                                                                   int first)
 {
     __shared__ double sSq[FlashCell<NC>::NT * kFlashBlock];
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = idx < W.count[LIST_SPLIT];
+    constexpr int nc = NC + 1;
+    const int64_t ld = B.ld;
+    const int total = W.count[in];
+    for (int base = nextChunk(W, in); base < total; base = nextChunk(W, in)) {
+    const int idx = base + (threadIdx.x & 63);
+    const bool active = idx < total;
+    bool done = false;
+    int32_t n = 0;
+    if (active) {
+        n = W.list[in][idx];
+        FlashCell<NC> F(*eos, &sSq[threadIdx.x]);
+        loadFeed<NC>(F, B, W, n);
+        F.Yt = W.Yt[n];
+        F.NP = 2;
+        F.calAiBi();
+        double Se;
+        int iter;
+        int fl = W.flags[n];
+        if (first) {
+            if (F.lNP == 2) {
+#pragma unroll
+                for (int i = 0; i < NC; ++i)
+                    F.Ks[i] = B.xij[(0 * nc + i) * ld + n] / B.xij[(1 * nc + i) * ld + n];
+            } else {
+                F.calKwilson();
+#pragma unroll
+                for (int i = 0; i < NC; ++i) F.Ks[i] = F.Kw0[i];
+            }
+            Se = 1;
+            iter = 0;
+            F.it[IT_RR] = 0;
+            F.ssmSpConflag = true;
+        } else {
+#pragma unroll
+            for (int i = 0; i < NC; ++i) F.Ks[i] = W.Ks[i * ld + n];
+            Se = W.Se[n];
+            iter = W.ssmIt[n];
+            F.it[IT_RR] = W.rrIt[n];
+            F.ssmSpConflag = (fl & FL_SSMSP) != 0;
+        }
+        done = F.splitSSM2Slice(Se, iter, budget);
+#pragma unroll
+        for (int i = 0; i < NC; ++i) W.Ks[i * ld + n] = F.Ks[i];
+        W.Se[n] = Se;
+        W.ssmIt[n] = iter;
+        W.rrIt[n] = F.it[IT_RR];
+        W.flags[n] = (uint8_t)(F.ssmSpConflag ? (fl | FL_SSMSP) : (fl & ~FL_SSMSP));
+        if (done) {
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                W.nu[j * ld + n] = F.nu[j];
+#pragma unroll
+                for (int i = 0; i < NC; ++i) W.x[(j * NC + i) * ld + n] = F.x[j][i];
+            }
+        }
+    }
+    appendList(W, LIST_NR, active && done, n);
+    if (cont >= 0) appendList(W, cont, active && !done, n);
+    }
+}
+
+// The split's Newton stage and its restarts (PhaseSplit after the first
+// SplitSSM2), from the state the last slice left: the compositions and
+// fractions of its last Rachford-Rice solve and the fugacities of those
+// compositions (recomputed: the same arithmetic on the same inputs).
+template <int NC>
+__global__ void __launch_bounds__(kFlashBlock) flashSplitNRKernel(const DevEos* __restrict__ eos, DevBulk B,
+                                                                  FlashWork W, DevFlashCounters* ctr)
+{
+    __shared__ double sSq[FlashCell<NC>::NT * kFlashBlock];
+    const int64_t ld = B.ld;
+    const int total = W.count[LIST_NR];
+    for (int base = nextChunk(W, LIST_NR); base < total; base = nextChunk(W, LIST_NR)) {
+    const int idx = base + (threadIdx.x & 63);
+    const bool active = idx < total;
     int NP = 1;
     int its[3] = {0, 0, 0};
     bool nonconv = false;
     int32_t n = 0;
     if (active) {
-        n = W.list[LIST_SPLIT][idx];
-        const int64_t ld = B.ld;
-        constexpr int nc = NC + 1;
+        n = W.list[LIST_NR][idx];
         FlashCell<NC> F(*eos, &sSq[threadIdx.x]);
         loadFeed<NC>(F, B, W, n);
-#pragma unroll
-        for (int q = 0; q < 5; ++q) F.it[q] = 0;
-        if (F.lNP == 2) {
-#pragma unroll
-            for (int i = 0; i < NC; ++i) F.lKs[i] = B.xij[(0 * nc + i) * ld + n] / B.xij[(1 * nc + i) * ld + n];
-        }
         int fl = W.flags[n];
         F.ssmStaConflag = fl & FL_SSMSTA;
         F.nrStaConflag = fl & FL_NRSTA;
+        F.ssmSpConflag = (fl & FL_SSMSP) != 0;
+        F.nrSpConflag = false;
         F.Yt = W.Yt[n];
         F.NP = 2;
         F.calAiBi();
-        F.calKwilson();
-        F.phaseSplit();
+#pragma unroll
+        for (int i = 0; i < NC; ++i) F.Ks[i] = W.Ks[i * ld + n];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            F.nu[j] = W.nu[j * ld + n];
+#pragma unroll
+            for (int i = 0; i < NC; ++i) F.x[j][i] = W.x[(j * NC + i) * ld + n];
+        }
+        F.it[IT_SSMSTA] = 0;
+        F.it[IT_NRSTA] = 0;
+        F.it[IT_SSMSP] = W.ssmIt[n];
+        F.it[IT_NRSP] = 0;
+        F.it[IT_RR] = W.rrIt[n];
+        F.calFugPhiAll();
+        F.phaseSplitAfterSSM();
         NP = F.NP;
         if (F.ftype == 0) {
             if (NP > 1) fl &= ~FL_SKIP;
@@ -744,6 +851,7 @@ __global__ void __launch_bounds__(kFlashBlock) flashSplitKernel(const DevEos* __
         addCounter(&ctr->nonconverged, nonconv ? 1 : 0);
         addCounter(&ctr->twoPhase, (active && NP == 2) ? 1 : 0);
     }
+    }
 }
 
 // Stage 4: the rest of FlashFIM / InitFlashFIM for cells ending with NPT
@@ -754,9 +862,11 @@ __global__ void __launch_bounds__(kFlashBlock) flashFinishKernel(const DevEos* _
                                                                  FlashWork W, int init)
 {
     __shared__ double sSq[FlashCell<NC>::NT * kFlashBlock];
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     const int list = (NPT == 1) ? LIST_NP1 : LIST_NP2;
-    if (idx >= W.count[list]) return;
+    const int total = W.count[list];
+    for (int base = nextChunk(W, list); base < total; base = nextChunk(W, list)) {
+    const int idx = base + (threadIdx.x & 63);
+    if (idx >= total) continue;
     const int32_t n = W.list[list][idx];
     const int64_t ld = B.ld;
     constexpr int nc = NC + 1;
@@ -797,21 +907,40 @@ __global__ void __launch_bounds__(kFlashBlock) flashFinishKernel(const DevEos* _
     const double Sw = init ? B.S[2 * ld + n] : 0.0;
     const double Vp = init ? B.rockVp[n] : 0.0;
     FlashKernel<NC>::template finish<NPT>(F, *eos, B, n, init != 0, Sw, Vp);
+    }
 }
 
 template <int NC>
 hipError_t launchFlashT(const DevEos* eos, const DevBulk& B, const FlashWork& W, DevFlashCounters* ctr,
-                        const double* zin, hipStream_t s)
+                        const double* zin, hipStream_t s, hipEvent_t* ev)
 {
     if (B.nb == 0) return hipSuccess;
     const unsigned grid = (unsigned)((B.nb + kFlashBlock - 1) / kFlashBlock);
+    // the list stages run as work queues over a grid that fills the chip
+    // (idle waves leave as soon as the queue is empty)
+    const unsigned qgrid = std::min<unsigned>(grid, kQueueBlocks);
+    auto mark = [&](int k) {
+        if (ev) (void)hipEventRecord(ev[k], s);
+    };
     hipLaunchKernelGGL(flashResetKernel<NC>, dim3(1), dim3(64), 0, s, W, ctr);
     hipLaunchKernelGGL(flashClassifyKernel<NC>, dim3(grid), dim3(kFlashBlock), 0, s, eos, B, W, ctr, zin);
-    hipLaunchKernelGGL(flashStabilityKernel<NC>, dim3(grid), dim3(kFlashBlock), 0, s, eos, B, W, ctr);
-    hipLaunchKernelGGL(flashSplitKernel<NC>, dim3(grid), dim3(kFlashBlock), 0, s, eos, B, W, ctr);
+    mark(0);
+    hipLaunchKernelGGL(flashStabilityKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, ctr);
+    mark(1);
+    mark(2);
+    hipLaunchKernelGGL(flashSplitSSMKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, (int)LIST_SPLIT,
+                       (int)LIST_C1, kSplitBudget0, 1);
+    hipLaunchKernelGGL(flashSplitSSMKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, (int)LIST_C1,
+                       (int)LIST_C2, kSplitBudget1, 0);
+    hipLaunchKernelGGL(flashSplitSSMKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, (int)LIST_C2, -1,
+                       -1, 0);
+    hipLaunchKernelGGL(flashSplitNRKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, ctr);
+    mark(3);
+    mark(4);
     const int init = zin ? 1 : 0;
-    hipLaunchKernelGGL((flashFinishKernel<NC, 1>), dim3(grid), dim3(kFlashBlock), 0, s, eos, B, W, init);
-    hipLaunchKernelGGL((flashFinishKernel<NC, 2>), dim3(grid), dim3(kFlashBlock), 0, s, eos, B, W, init);
+    hipLaunchKernelGGL((flashFinishKernel<NC, 1>), dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, init);
+    hipLaunchKernelGGL((flashFinishKernel<NC, 2>), dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, init);
+    mark(5);
     return hipGetLastError();
 }
 

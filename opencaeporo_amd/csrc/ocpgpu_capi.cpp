@@ -67,8 +67,9 @@ constexpr int kNumFields = sizeof(kFields) / sizeof(kFields[0]);
 size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 // Timer slots of ocpg_timing_get.
-enum TimerSlot { T_FLASH = 0, T_KRPC, T_RES, T_ASM, T_INIT, T_RESTORE, kTimerSlots };
-const char* kTimerNames[kTimerSlots] = {"flash", "krpc_rock", "calres", "assemble", "init_flash", "restore"};
+enum TimerSlot { T_FLASH = 0, T_KRPC, T_RES, T_ASM, T_INIT, T_RESTORE, T_STAB, T_SPLIT, T_FINISH, kTimerSlots };
+const char* kTimerNames[kTimerSlots] = {"flash",   "krpc_rock",       "calres",      "assemble",    "init_flash",
+                                        "restore", "flash_stability", "flash_split", "flash_finish"};
 
 // Fields the flash reads (and overwrites): what one Newton iteration starts from.
 bool isFlashInput(size_t hostOff)
@@ -400,16 +401,18 @@ int ocpg_set_cells(ocpg_ctx* c, int64_t nb)
     HIPCHK(hipMalloc(&c->cur, c->curBytes));
     HIPCHK(hipMalloc(&c->last, c->lastBytes));
     {
-        // FlashWork: doubles zi(NC), Nh, Yt, x(2NC), n(2NC), nu(2); bytes
-        // ftype, lNP, flags; four int32 lists; four counters.
+        // FlashWork: doubles zi(NC), Nh, Yt, x(2NC), n(2NC), nu(2), Ks(NC),
+        // Se; int32 ssmIt, rrIt; bytes ftype, lNP, flags; the int32 lists;
+        // the counters.
         const size_t L = (size_t)(ld > 0 ? ld : 64);
         const int NC = c->NC;
-        const size_t dbl[6] = {(size_t)NC, 1, 1, (size_t)2 * NC, (size_t)2 * NC, 2};
+        const size_t dbl[8] = {(size_t)NC, 1, 1, (size_t)2 * NC, (size_t)2 * NC, 2, (size_t)NC, 1};
         size_t w = 0;
-        size_t offD[6], offB[3], offL[4], offC;
-        for (int q = 0; q < 6; q++) { offD[q] = w; w += align256(dbl[q] * L * sizeof(double)); }
+        size_t offD[8], offI[2], offB[3], offL[kFlashLists], offC;
+        for (int q = 0; q < 8; q++) { offD[q] = w; w += align256(dbl[q] * L * sizeof(double)); }
+        for (int q = 0; q < 2; q++) { offI[q] = w; w += align256(L * sizeof(int32_t)); }
         for (int q = 0; q < 3; q++) { offB[q] = w; w += align256(L); }
-        for (int q = 0; q < 4; q++) { offL[q] = w; w += align256(L * sizeof(int32_t)); }
+        for (int q = 0; q < kFlashLists; q++) { offL[q] = w; w += align256(L * sizeof(int32_t)); }
         offC = w;
         w += 256;
         if (c->workMem) HIPCHK(hipFree(c->workMem));
@@ -422,10 +425,14 @@ int ocpg_set_cells(ocpg_ctx* c, int64_t nb)
         c->work.x = reinterpret_cast<double*>(b + offD[3]);
         c->work.n = reinterpret_cast<double*>(b + offD[4]);
         c->work.nu = reinterpret_cast<double*>(b + offD[5]);
+        c->work.Ks = reinterpret_cast<double*>(b + offD[6]);
+        c->work.Se = reinterpret_cast<double*>(b + offD[7]);
+        c->work.ssmIt = reinterpret_cast<int32_t*>(b + offI[0]);
+        c->work.rrIt = reinterpret_cast<int32_t*>(b + offI[1]);
         c->work.ftype = reinterpret_cast<int8_t*>(b + offB[0]);
         c->work.lNP = reinterpret_cast<int8_t*>(b + offB[1]);
         c->work.flags = reinterpret_cast<uint8_t*>(b + offB[2]);
-        for (int q = 0; q < 4; q++) c->work.list[q] = reinterpret_cast<int32_t*>(b + offL[q]);
+        for (int q = 0; q < kFlashLists; q++) c->work.list[q] = reinterpret_cast<int32_t*>(b + offL[q]);
         c->work.count = reinterpret_cast<int32_t*>(b + offC);
         HIPCHK(hipMemsetAsync(c->workMem, 0, w, c->stream));
     }
@@ -510,9 +517,21 @@ int ocpg_flash_fim(ocpg_ctx* c, ocpg_flash_stats* stats)
 {
     if (!c || !c->cur) return OCPG_ESTATE;
     HIPCHK(hipSetDevice(c->device));
+    hipEvent_t ev[6] = {};
+    bool stageEv = c->timing;
+    for (int k = 0; k < 6 && stageEv; k++) stageEv = hipEventCreate(&ev[k]) == hipSuccess;
     timerBegin(c, T_FLASH);
-    HIPCHK(launchFlash(c->NC, c->dEos, c->B, c->work, stats ? c->dCtr : nullptr, nullptr, c->stream));
+    HIPCHK(launchFlash(c->NC, c->dEos, c->B, c->work, stats ? c->dCtr : nullptr, nullptr, c->stream,
+                       stageEv ? ev : nullptr));
     timerEnd(c, T_FLASH);
+    if (stageEv) {
+        c->events[T_STAB].push_back({ev[0], ev[1]});
+        c->events[T_SPLIT].push_back({ev[2], ev[3]});
+        c->events[T_FINISH].push_back({ev[4], ev[5]});
+    } else {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
     if (stats) {
         DevFlashCounters h;
         HIPCHK(hipMemcpyAsync(&h, c->dCtr, sizeof(h), hipMemcpyDeviceToHost, c->stream));

@@ -132,13 +132,18 @@ double orc_time_flash_fim(const ocpg_eos* eos, ocpg_bulk* bk, int nthreads)
 
 #ifdef OCP_COUNT_FLOPS
 // Flop tally of one serial CalFlash pass: out[0] = add/sub/mul/div/sqrt,
-// out[1] = transcendental calls.
+// out[1] = transcendental calls, out[2..3] the same within PhaseStable,
+// out[4..5] within PhaseSplit.
 int orc_count_flash_fim(const ocpg_eos* eos, ocpg_bulk* bk, uint64_t* out)
 {
     oracle::g_tally = oracle::FlopTally();
     oracle::CalFlash(*eos, *bk, 1);
     out[0] = oracle::g_tally.flops;
     out[1] = oracle::g_tally.trans;
+    out[2] = oracle::g_tally.staFlops;
+    out[3] = oracle::g_tally.staTrans;
+    out[4] = oracle::g_tally.splitFlops;
+    out[5] = oracle::g_tally.splitTrans;
     return OCPG_OK;
 }
 #endif

@@ -593,6 +593,33 @@ void Mixture::CalKwilson()
 }
 
 // MixtureComp::PhaseEquilibrium, src/MixtureComp.cpp:1279-1353.
+// PhaseStable / PhaseSplit, with their share of the flop tally in the
+// counting build.
+bool Mixture::CountedStable()
+{
+#ifdef OCP_COUNT_FLOPS
+    const uint64_t f0 = g_tally.flops, t0 = g_tally.trans;
+    const bool r = PhaseStable();
+    g_tally.staFlops += g_tally.flops - f0;
+    g_tally.staTrans += g_tally.trans - t0;
+    return r;
+#else
+    return PhaseStable();
+#endif
+}
+
+void Mixture::CountedSplit()
+{
+#ifdef OCP_COUNT_FLOPS
+    const uint64_t f0 = g_tally.flops, t0 = g_tally.trans;
+    PhaseSplit();
+    g_tally.splitFlops += g_tally.flops - f0;
+    g_tally.splitTrans += g_tally.trans - t0;
+#else
+    PhaseSplit();
+#endif
+}
+
 void Mixture::PhaseEquilibrium()
 {
     switch (ftype) {
@@ -605,9 +632,9 @@ void Mixture::PhaseEquilibrium()
             CalAjBj(Aj[0], Bj[0], x[0]);
             SolEoS(Zj[0], Aj[0], Bj[0]);
             CalKwilson();
-            while (!PhaseStable()) {
+            while (!CountedStable()) {
                 NP++;
-                PhaseSplit();
+                CountedSplit();
                 if (NP == NPMAX || NP == 1) break;
             }
             if (NP > 1) flagSkip = false;
@@ -643,7 +670,7 @@ void Mixture::PhaseEquilibrium()
             Yt = 1.01;
             CalAiBi();
             CalKwilson();
-            PhaseSplit();
+            CountedSplit();
             if (nrSpConflag)
                 ePEC = nrSpRealTol;
             else if (ssmSpConflag)

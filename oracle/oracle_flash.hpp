@@ -140,6 +140,8 @@ private:
     void x2n();
     void CalKwilson();
     bool PhaseStable();
+    bool CountedStable();
+    void CountedSplit();
     bool StableSSM01(int Id);
     bool StableSSM(int Id);
     bool StableNR(int Id);

@@ -164,10 +164,18 @@ class FlopCounter(Oracle):
         super().__init__(COUNT_SO)
 
     def count_flash_fim(self, eos, bulk):
-        out = np.zeros(2, dtype=np.uint64)
+        """(flops, transcendentals) of one flash pass over ``bulk``."""
+        t = self.count_flash_stages(eos, bulk)
+        return t["flops"], t["trans"]
+
+    def count_flash_stages(self, eos, bulk):
+        """Flop / transcendental tallies of one flash pass, in total and
+        within PhaseStable and PhaseSplit."""
+        out = np.zeros(6, dtype=np.uint64)
         s = bulk.cstruct()
         assert self.lib.orc_count_flash_fim(C.byref(eos), C.byref(s), out.ctypes.data) == 0
-        return int(out[0]), int(out[1])
+        keys = ("flops", "trans", "sta_flops", "sta_trans", "split_flops", "split_trans")
+        return {k: int(v) for k, v in zip(keys, out)}
 
 
 class Reference:

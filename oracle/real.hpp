@@ -18,6 +18,8 @@ namespace oracle {
 struct FlopTally {
     uint64_t flops = 0;  // add, sub, mul, div, sqrt, neg excluded
     uint64_t trans = 0;  // exp, log, pow, acos, cos
+    // the parts spent in PhaseStable and in PhaseSplit
+    uint64_t staFlops = 0, staTrans = 0, splitFlops = 0, splitTrans = 0;
 };
 extern thread_local FlopTally g_tally;
 

@@ -1,0 +1,45 @@
+// Test driver for the IsoT_FIM-shaped C++ shim (include/ocpgpu_fim.hpp): one
+// FIM Newton iteration the way IsothermalSolver sequences IsoT_FIM
+// (src/IsothermalSolver.cpp:79-155), from host arrays in the reference's
+// layout.  Built by opencaeporo_amd/build.py into tests/cpp/libshim_driver.so
+// and called from tests/test_gpu_shim.py.
+#include <cstring>
+#include <exception>
+#include <vector>
+
+#include "ocpgpu_fim.hpp"
+
+extern "C" int shim_newton_iteration(const ocpg_eos_deck* deck, const ocpg_satfunc* sat, const ocpg_rock* rock,
+                                     int64_t nb, const ocpg_conn* conn, ocpg_bulk* bk, const double* zi,
+                                     const ocpg_wells* wells, const double* newP, const double* newNi, double dt,
+                                     int32_t* IA, int32_t* JA, double* val, double* rhs, ocpg_resnorm* norms)
+{
+    try {
+        ocpg::IsoT_FIM_GPU fim(*deck, *sat, *rock, 0);
+        fim.Setup(nb, *conn);
+        fim.InitReservoir(*bk, zi);          // InitRock, InitFlash, CalKrPc, UpdateLastTimeStep
+        if (wells) fim.Prepare(*wells);      // PrepareWell's per-step values
+        // the Newton update (GetSolution's result) arrives as new P and N
+        std::memcpy(bk->P, newP, sizeof(double) * nb);
+        std::memcpy(bk->Ni, newNi, sizeof(double) * nb * (fim.Eos().nc + 1));
+        ocpg_bulk up{};
+        up.nb = nb;
+        up.P = bk->P;
+        up.Ni = bk->Ni;
+        fim.Upload(up);
+        *norms = fim.UpdateProperty(dt);     // CalFlash, CalKrPc, CalRock, wells, CalRes
+        fim.AssembleMat(dt);
+        ocpg::BsrSystem s;
+        fim.GetSystem(s);
+        std::memcpy(IA, s.IA.data(), s.IA.size() * sizeof(int32_t));
+        std::memcpy(JA, s.JA.data(), s.JA.size() * sizeof(int32_t));
+        std::memcpy(val, s.val.data(), s.val.size() * sizeof(double));
+        std::memcpy(rhs, s.rhs.data(), s.rhs.size() * sizeof(double));
+        fim.Download(*bk);
+        return OCPG_OK;
+    } catch (const ocpg::Error& e) {
+        return e.code;
+    } catch (const std::exception&) {
+        return OCPG_EINVAL;
+    }
+}
