@@ -316,6 +316,18 @@ int ocpg_set_well_state(ocpg_ctx* ctx, const ocpg_wells* wells);
 int ocpg_get_well_flux(ocpg_ctx* ctx, double* perfQi, double* wellQi, double* prodWeight);
 /* Rows of the assembled system: bulk cells plus open wells.                  */
 int64_t ocpg_bsr_rows(ocpg_ctx* ctx);
+
+/* One Newton iteration of IsoT_FIM on the resident state (the inputs saved by
+ * ocpg_save_flash_inputs restored, then CalFlash, CalKrPc, CalRock, the
+ * wells' CalTrans/CalFlux, CalRes and AssembleMat: src/OCPFluidMethod.cpp:
+ * 860-935) captured into a HIP graph, for callers that replay iterations
+ * many times (the serving loop).  No reference counterpart: IsoT_FIM issues
+ * the same calls one by one.  Timing must be off while capturing.            */
+int ocpg_capture_iteration(ocpg_ctx* ctx, double dt);
+/* Replays it on the context's stream (asynchronous).                        */
+int ocpg_launch_iteration(ocpg_ctx* ctx);
+/* 1 when the last replay has finished, 0 while it runs, < 0 on error.       */
+int ocpg_iteration_done(ocpg_ctx* ctx);
 /* BulkConn::upblock / upblock_Rho / upblock_Velocity as CalRes set them.    */
 int ocpg_get_upwind(ocpg_ctx* ctx, int32_t* upblock, double* upblock_Rho,
                     double* upblock_Velocity);

@@ -110,6 +110,9 @@ struct ocpg_ctx {
     // copy of the flash inputs (ocpg_save_flash_inputs)
     char* snap = nullptr;
     size_t snapBytes = 0;
+    // one Newton iteration captured as a graph (ocpg_capture_iteration)
+    hipGraphExec_t iterGraph = nullptr;
+    hipEvent_t iterDone = nullptr;
     // per-kernel timers (ocpg_timing_*)
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events[kTimerSlots];
@@ -360,6 +363,8 @@ int ocpg_destroy(ocpg_ctx* c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     timerClear(c);
+    if (c->iterGraph) (void)hipGraphExecDestroy(c->iterGraph);
+    if (c->iterDone) (void)hipEventDestroy(c->iterDone);
     fimFree(c->fim);
     if (c->wells.mem) (void)hipFree(c->wells.mem);
     if (c->snap) (void)hipFree(c->snap);
@@ -670,6 +675,61 @@ int ocpg_synchronize(ocpg_ctx* c)
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipStreamSynchronize(c->stream));
     return OCPG_OK;
+}
+
+// One Newton iteration on the resident state -- restore the saved inputs,
+// FlashFIM + PassFlashValue, CalKrPc + CalRock, the wells' rates and CalRes,
+// AssembleMat -- captured once into a HIP graph on the context's stream, so
+// a serving loop replays it with one launch.  Needs the connections, the
+// saved flash inputs and timing off.
+int ocpg_capture_iteration(ocpg_ctx* c, double dt)
+{
+    if (!c || !c->cur || !c->snap || c->timing) return OCPG_ESTATE;
+    if (c->fim.nb != c->B.nb || c->fim.nb <= 0) return OCPG_ESTATE;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->iterGraph) {
+        HIPCHK(hipGraphExecDestroy(c->iterGraph));
+        c->iterGraph = nullptr;
+    }
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    int rc = ocpg_restore_flash_inputs(c);
+    if (rc == OCPG_OK) rc = ocpg_flash_fim(c, nullptr);
+    if (rc == OCPG_OK) rc = ocpg_calkrpc_rock(c);
+    if (rc == OCPG_OK) rc = ocpg_calres(c, dt, nullptr);
+    if (rc == OCPG_OK) rc = ocpg_assemble(c, dt);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc != OCPG_OK) {
+        if (e == hipSuccess && g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    HIPCHK(e);
+    const hipError_t ei = hipGraphInstantiate(&c->iterGraph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHK(ei);
+    if (!c->iterDone) HIPCHK(hipEventCreateWithFlags(&c->iterDone, hipEventDisableTiming));
+    return OCPG_OK;
+}
+
+// Replays the captured iteration (asynchronous); ocpg_iteration_done polls it.
+int ocpg_launch_iteration(ocpg_ctx* c)
+{
+    if (!c || !c->iterGraph) return OCPG_ESTATE;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipGraphLaunch(c->iterGraph, c->stream));
+    HIPCHK(hipEventRecord(c->iterDone, c->stream));
+    return OCPG_OK;
+}
+
+// 1 when the last launched iteration has finished, 0 while it runs.
+int ocpg_iteration_done(ocpg_ctx* c)
+{
+    if (!c || !c->iterDone) return OCPG_ESTATE;
+    const hipError_t e = hipEventQuery(c->iterDone);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    return hipStatus(e);
 }
 
 // Internal access for the C++ shim and the benchmark: the context's stream

@@ -45,6 +45,9 @@ def load():
         "ocpg_set_conn": ([P, P], C.c_int),
         "ocpg_bsr_nnzb": ([P], C.c_int64),
         "ocpg_bsr_rows": ([P], C.c_int64),
+        "ocpg_capture_iteration": ([P, C.c_double], C.c_int),
+        "ocpg_launch_iteration": ([P], C.c_int),
+        "ocpg_iteration_done": ([P], C.c_int),
         "ocpg_set_wells": ([P, P], C.c_int),
         "ocpg_set_well_state": ([P, P], C.c_int),
         "ocpg_get_well_flux": ([P, P, P, P], C.c_int),
@@ -163,6 +166,20 @@ class Context:
 
     def rows(self):
         return int(self.lib.ocpg_bsr_rows(self.h))
+
+    def capture_iteration(self, dt):
+        """Capture one Newton iteration (restore, flash, kr/Pc + rock, CalRes,
+        AssembleMat) into a HIP graph."""
+        check(self.lib.ocpg_capture_iteration(self.h, dt), "ocpg_capture_iteration")
+
+    def launch_iteration(self):
+        check(self.lib.ocpg_launch_iteration(self.h), "ocpg_launch_iteration")
+
+    def iteration_done(self):
+        r = self.lib.ocpg_iteration_done(self.h)
+        if r < 0:
+            raise OcpgError(f"ocpg_iteration_done failed with code {r}")
+        return r == 1
 
     def set_wells(self, wells):
         """abi.Wells -> ocpg_set_wells (rebuilds the BSR pattern)."""
