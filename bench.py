@@ -47,14 +47,40 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def setup(nx, ny, nz, seed=2024):
+def irregular_wells(grid, P, NC, seed, n_wells=40):
+    """40 wells with 5-20 perforations each at random cells: injectors (rate
+    control, C1-rich gas) and producers (BHP or oil-rate control)."""
+    rng = np.random.default_rng(seed)
+    nc = NC + 1
+    inj_z = np.zeros(nc)
+    inj_z[0], inj_z[1] = 0.9, 0.1
+    wells = []
+    for w in range(n_wells):
+        perf = rng.choice(grid.nb, size=int(rng.integers(5, 21)), replace=False).astype(np.int32)
+        inj = w % 4 == 0
+        Pp = P[perf]
+        d = dict(type=abi.WELL_INJ if inj else abi.WELL_PROD,
+                 mode=abi.WELL_RATE if (inj or w % 4 == 1) else abi.WELL_BHP, perf=perf,
+                 bhp=(Pp.max() + 300.0) if inj else (Pp.min() - 300.0), ctrlRate=2000.0 if inj else 300.0,
+                 WI=rng.uniform(5.0, 50.0, len(perf)), dG=rng.uniform(-5.0, 5.0, len(perf)),
+                 prodPhaseWeight=[1.0, 0.0, 0.0])
+        d["ctrlBHP"] = d["bhp"] + (25.0 if inj else -25.0)
+        if inj:
+            d["injZi"] = inj_z
+            d["perfXi"] = rng.uniform(0.4, 0.6, len(perf))
+        wells.append(d)
+    return abi.Wells(nc, wells)
+
+
+def setup(nx, ny, nz, seed=2024, irregular=False):
     t0 = time.time()
-    deck = fluids.bench6_deck()
+    deck = fluids.ten_deck() if irregular else fluids.bench6_deck()
     eos = lib.eos_from_deck(deck)
     sat = fluids.make_satfunc()
     rock = fluids.make_rock()
     NC = eos.nc
-    grid = workloads.cartesian_grid(nx, ny, nz, seed=seed)
+    grid = (workloads.faulted_grid(nx, ny, nz, seed=seed, frac=0.05) if irregular
+            else workloads.cartesian_grid(nx, ny, nz, seed=seed))
     nb = grid.nb
     P, z, Sw = workloads.cell_inputs(nb, NC, seed=seed + 1)
     fields = ["P", "T", "S", "v", "depth", "poroInit"]
@@ -87,11 +113,15 @@ def setup(nx, ny, nz, seed=2024):
     ctx.download(pb)
     pb.P[:] += rng.uniform(-5.0, 5.0, nb)
     ctx.upload(pb)
+    wells = None
+    if irregular:
+        wells = irregular_wells(grid, pb.P, NC, seed + 3)
+        ctx.set_wells(wells)
     ctx.calkrpc_rock()
     ctx.calres(DT, norms=False)
     ctx.save_flash_inputs()
     ctx.synchronize()
-    return ctx, grid, nb, NC
+    return ctx, grid, nb, NC, wells
 
 
 def step(ctx):
@@ -162,6 +192,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--nz", type=int, default=85, help="layers of the 60 x 220 x nz grid")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--irregular", action="store_true",
+                    help="the irregular-stencil configuration: 10-component fluid, faulted grid with "
+                         "non-neighbour connections, 40 wells (a parity/measurement case, not the headline)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
@@ -169,7 +202,7 @@ def main():
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
     nx, ny, nz = 60, 220, a.nz
-    ctx, grid, nb, NC = setup(nx, ny, nz)
+    ctx, grid, nb, NC, wells = setup(nx, ny, nz, irregular=a.irregular)
     for _ in range(a.warmup):
         step(ctx)
     ctx.synchronize()
@@ -210,14 +243,25 @@ def main():
         r["frac"] = r["achieved"] / r["peak"]
     roof = roof_split if per["flash_split"] >= per["assemble"] else roof_asm
     other = roof_asm if roof is roof_split else roof_split
+    if a.irregular:
+        # the flop tally was measured on the 6-component workload only
+        roof, other = roof_asm, None
     cpu = None
-    if rank == 0 and not a.no_cpu:
+    if rank == 0 and not a.no_cpu and not a.irregular:
         try:
             cpu = cpu_baseline()
         except Exception as e:  # the baseline is informative only
             log(f"[bench] cpu baseline failed: {e}")
+    if a.irregular:
+        workload = (f"irregular stencil: {nx}x{ny}x{nz} faulted grid ({grid.nconn} connections incl. "
+                    f"non-neighbour), 10-comp PR, {wells.nw} wells / {wells.nperf} perforations, warm "
+                    f"Newton pass, {ncol}x{ncol} BSR")
+    else:
+        workload = (f"standalone flash+assembly, {nx}x{ny}x{nz} grid, {nb} cells, "
+                    f"{grid.nconn} connections, 6-comp PR, warm Newton pass, 8x8 BSR")
     out = {
-        "metric": "cell flash+Jacobian updates/s (1.1M cells, 6-comp)",
+        "metric": ("cell flash+Jacobian updates/s (1.1M cells, 10-comp, irregular, wells)" if a.irregular
+                   else "cell flash+Jacobian updates/s (1.1M cells, 6-comp)"),
         "value": value,
         "unit": "cell-updates/s",
         "n_gpus": 1,
@@ -229,8 +273,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"standalone flash+assembly, {nx}x{ny}x{nz} grid, {nb} cells, "
-                               f"{grid.nconn} connections, 6-comp PR, warm Newton pass, 8x8 BSR",
+        "config": {"workload": workload,
                    "cells": nb, "connections": grid.nconn, "block": ncol,
                    "kernel_ms": per, "dominant_kernel": dominant, "flash_ms": flash_ms,
                    "flash_flops_per_cell": WARM_FLOPS_PER_CELL, "asm_bytes_per_launch": asm_bytes,

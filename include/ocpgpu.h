@@ -317,6 +317,29 @@ int ocpg_get_well_flux(ocpg_ctx* ctx, double* perfQi, double* wellQi, double* pr
 /* Rows of the assembled system: bulk cells plus open wells.                  */
 int64_t ocpg_bsr_rows(ocpg_ctx* ctx);
 
+/* The learned form (no reference counterpart).  FlashFIM as ocpg_flash_fim,
+ * with a learned initial guess in device memory: nb rows of ld floats
+ * (ln K_1 .. ln K_NC, stability logit).  A cell whose logit is positive first
+ * tries the guessed K in the stability test; the split starts from it.  The
+ * FP64 stability test and split still decide every cell.                   */
+int ocpg_flash_fim_guided(ocpg_ctx* ctx, const float* guess, int ld, ocpg_flash_stats* stats);
+/* Building blocks of the network's training and inference on MI355X (BF16
+ * MFMA products, FP32 accumulation): C = A B (+ C) with strided FP32 A, B
+ * (rounded to BF16), the K loop split in splitK parts summed in order
+ * (partial: splitK*M*N floats when splitK > 1); bias + GELU; GELU';
+ * column sums; the loss (masked ln K MSE + weighted stability BCE, dY / B);
+ * Adam.  stream is a hipStream_t.                                           */
+int ocpg_mlp_gemm(void* stream, int M, int N, int K, const float* A, int64_t sAm, int64_t sAk, const float* B,
+                  int64_t sBk, int64_t sBn, float* C, int64_t sCm, int64_t sCn, int accumulate, int splitK,
+                  float* partial);
+int ocpg_mlp_bias_act(void* stream, int M, int N, float* Z, const float* b, float* H);
+int ocpg_mlp_act_backward(void* stream, int64_t n, const float* Z, float* dH);
+int ocpg_mlp_colsum(void* stream, int M, int N, const float* X, float* out);
+int ocpg_mlp_loss(void* stream, int B, int nK, int ldY, const float* Y, const float* lnK, const float* label,
+                  const float* mask, float wStab, float* dY, float* loss);
+int ocpg_mlp_adam(void* stream, int64_t n, float* w, const float* g, float* m, float* v, float lr, float b1,
+                  float b2, float eps, int step);
+
 /* One Newton iteration of IsoT_FIM on the resident state (the inputs saved by
  * ocpg_save_flash_inputs restored, then CalFlash, CalKrPc, CalRock, the
  * wells' CalTrans/CalFlux, CalRes and AssembleMat: src/OCPFluidMethod.cpp:

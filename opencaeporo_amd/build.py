@@ -53,6 +53,7 @@ def _jobs(ncs):
     jobs.append(("fim_kernels.hip", "fim_kernels.o", []))
     jobs.append(("ocpgpu_capi.cpp", "ocpgpu_capi.o", []))
     jobs.append(("fim_capi.cpp", "fim_capi.o", []))
+    jobs.append(("mlp_kernels.hip", "mlp_kernels.o", []))
     return jobs
 
 

@@ -69,7 +69,7 @@ struct DevBulk {
 
 // Scratch of the staged flash (cells in SoA with the bulk's ld; lists of
 // cell indices with their lengths in count[4]).
-constexpr int kFlashLists = 7;
+constexpr int kFlashLists = 8;
 
 struct FlashWork {
     double* zi;      // NC per cell: feed mole fractions
@@ -87,6 +87,10 @@ struct FlashWork {
     uint8_t* flags;
     int32_t* list[kFlashLists];
     int32_t* count;  // list lengths [0, 8), work-queue cursors [8, 16)
+    // learned initial guess (NULL: none): per cell, ln K_i (NC values) and a
+    // stability logit at guess[n * guessLd + ...]
+    const float* guess;
+    int guessLd;
 };
 
 // Totals gathered by the flash kernels (one atomic per wavefront).

@@ -113,6 +113,10 @@ __device__ __forceinline__ double dZden(double aj, double bj, double zj)
 
 // Cap on PhaseSplit's SSM/NR restarts (none in the reference).
 constexpr int kMaxSplitRestarts = 100;
+// Successive-substitution iterations of a stability trial started from a
+// learned K (an extra trial: when it does not prove instability within
+// these, the reference's trials run as always).
+constexpr int kGuidedTrialMaxIt = 20;
 
 // Iteration counters, in the reference's order.
 enum { IT_SSMSTA = 0, IT_NRSTA, IT_SSMSP, IT_NRSP, IT_RR };
@@ -146,6 +150,10 @@ struct FlashCell {
     double Asta, Bsta, Zsta;
     // convergence bookkeeping (ePEC)
     bool ssmStaConflag, nrStaConflag, ssmSpConflag, nrSpConflag;
+    // K-values proposed by the learned model (NULL: none), and whether a
+    // trial started from them proved the feed unstable
+    const double* Kguide = nullptr;
+    bool guidedProof = false;
     int it[5];
 
     __device__ __forceinline__ explicit FlashCell(const DevEos& e, double* sqLds = nullptr) : E(e), sqA(sqLds) {}
@@ -365,6 +373,57 @@ struct FlashCell {
         return true;
     }
 
+    // One trial of StableSSM from the unnormalised trial composition Y:
+    // successive substitution, then Newton (src/MixtureComp.cpp:1505-1585).
+    // True when the trial proves the feed unstable.
+    __device__ __forceinline__ bool stabilityTrial(double* Y, const double* di, int maxIt)
+    {
+        const double Stol = E.ssmStaTol2;
+        const double eYt = E.ssmStaEYt;
+        double phiSta[NC], fugSta[NC];
+        Yt = 0;
+#pragma unroll
+        for (int i = 0; i < NC; ++i) Yt += Y[i];
+        {
+            const double s = 1 / Yt;
+#pragma unroll
+            for (int i = 0; i < NC; ++i) Y[i] = s * Y[i];
+        }
+        calFugPhi<true>(phiSta, fugSta, Y);
+        double Se = 0;
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            const double l = log(fugSta[i] / fug[0][i] * Yt);
+            Se += l * l;
+        }
+        int iter = 0;
+        while (Se > Stol) {
+            Yt = 0;
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                Y[i] = di[i] / phiSta[i];
+                Yt += Y[i];
+            }
+            {
+                const double s = 1 / Yt;
+#pragma unroll
+                for (int i = 0; i < NC; ++i) Y[i] = s * Y[i];
+            }
+            calFugPhi<true>(phiSta, fugSta, Y);
+            Se = 0;
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                const double l = log(fugSta[i] / fug[0][i] * Yt);
+                Se += l * l;
+            }
+            iter++;
+            if (iter > maxIt) break;
+        }
+        it[IT_SSMSTA] += iter;
+        const bool flag = stableNR(fug[0], Y, fugSta);
+        return flag && Yt > 1 + eYt;
+    }
+
     // MixtureComp::StableSSM (strict, 4 Wilson trials), src/MixtureComp.cpp:1492-1588.
     __device__ __forceinline__ bool stableSSM()
     {
@@ -421,6 +480,41 @@ struct FlashCell {
             it[IT_SSMSTA] += iter;
             const bool flag = stableNR(fug[0], Y, fugSta);
             if (flag && Yt > 1 + eYt) {
+                ssmStaConflag = true;
+                return false;
+            }
+        }
+        return true;
+    }
+
+    // StableSSM with K-values a learned model proposes (Kguide): two extra
+    // trials (x K and x / K) run first; a trial that proves instability
+    // settles it, otherwise the reference's four trials decide as always.
+    // A separate function from stableSSM so the plain flash keeps its own
+    // code.
+    __device__ __forceinline__ bool stableSSMGuided()
+    {
+        const double* xj = x[0];
+        calFugPhi<true>(phi[0], fug[0], xj);
+        double di[NC];
+#pragma unroll
+        for (int i = 0; i < NC; ++i) di[i] = phi[0][i] * xj[i];
+        double Y[NC];
+        if (Kguide) {
+            for (int k = 0; k < 2; ++k) {
+#pragma unroll
+                for (int i = 0; i < NC; ++i) Y[i] = k == 0 ? xj[i] * Kguide[i] : xj[i] / Kguide[i];
+                if (stabilityTrial(Y, di, kGuidedTrialMaxIt)) {
+                    ssmStaConflag = true;
+                    guidedProof = true;
+                    return false;
+                }
+            }
+        }
+        for (int k = 0; k < 4; ++k) {
+#pragma unroll
+            for (int i = 0; i < NC; ++i) Y[i] = xj[i] / kw(k, i);
+            if (stabilityTrial(Y, di, E.ssmStaMaxIt)) {
                 ssmStaConflag = true;
                 return false;
             }
@@ -502,6 +596,7 @@ struct FlashCell {
     }
 
     // MixtureComp::PhaseStable, src/MixtureComp.cpp:1355-1390 (NP = 1).
+    template <bool Guided = false>
     __device__ __forceinline__ bool phaseStable()
     {
         ssmStaConflag = false;
@@ -509,12 +604,14 @@ struct FlashCell {
         bool flag;
         int tmpNP = NP;
         if (lNP == 0) {
-            flag = stableSSM();
+            if constexpr (Guided) flag = stableSSMGuided();
+            else flag = stableSSM();
         } else {
             flag = stableSSM01();
             if (!flag) tmpNP++;
             if (tmpNP != lNP) {
-                flag = stableSSM();
+                if constexpr (Guided) flag = stableSSMGuided();
+                else flag = stableSSM();
                 flagSkip = false;
             }
         }

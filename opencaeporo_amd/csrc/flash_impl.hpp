@@ -520,8 +520,9 @@ __device__ __forceinline__ int calFtypeFIM(const FlashCell<NC>& F, const DevBulk
 // FlashWork buffers (SoA, leading dimension ld).
 // ---------------------------------------------------------------------------
 
-enum { FL_SKIP = 1, FL_SSMSTA = 2, FL_NRSTA = 4, FL_SSMSP = 8, FL_NRSP = 16 };
-enum { LIST_STAB = 0, LIST_SPLIT = 1, LIST_NP1 = 2, LIST_NP2 = 3, LIST_C1 = 4, LIST_C2 = 5, LIST_NR = 6 };
+enum { FL_SKIP = 1, FL_SSMSTA = 2, FL_NRSTA = 4, FL_SSMSP = 8, FL_NRSP = 16, FL_GUIDED = 32 };
+enum { LIST_STAB = 0, LIST_SPLIT = 1, LIST_NP1 = 2, LIST_NP2 = 3, LIST_C1 = 4, LIST_C2 = 5, LIST_NR = 6,
+       LIST_STABG = 7 };
 // Blocks of a work-queue stage: 256 CUs x 4 SIMDs x 2 waves, in blocks of
 // kFlashBlock / 64 waves.
 constexpr unsigned kQueueBlocks = 256 * 4 * 2 / (kFlashBlock / 64);
@@ -529,6 +530,10 @@ constexpr unsigned kQueueBlocks = 256 * 4 * 2 / (kFlashBlock / 64);
 // warm pass needs 8 or fewer in half of its two-phase cells, 15 at the 90th
 // percentile); the third slice runs to the end.
 constexpr int kSplitBudget0 = 8;
+// Whether the split starts from the learned K where a trial from it proved
+// the feed unstable (measured on the 1.1M-cell cold pass: fewer split
+// iterations in total but a longer tail than Wilson's start, so off).
+constexpr bool kSplitFromGuess = false;
 constexpr int kSplitBudget1 = 24;
 
 // Append the active lanes' cells to list k: one atomic per wavefront.
@@ -624,7 +629,11 @@ __global__ void __launch_bounds__(kFlashBlock) flashClassifyKernel(const DevEos*
             for (int q = 0; q < 5; ++q) B.iters[q * ld + n] = 0;
         }
     }
-    appendList(W, LIST_STAB, active && ftype == 0, (int32_t)n);
+    // cells the learned prior calls unstable get their own stability list, so
+    // the waves that try the guessed K first are not mixed with full tests
+    const bool guided = active && W.guess && W.guess[(int64_t)n * W.guessLd + NC] > 0.f;
+    appendList(W, LIST_STAB, active && ftype == 0 && !guided, (int32_t)n);
+    appendList(W, LIST_STABG, active && ftype == 0 && guided, (int32_t)n);
     appendList(W, LIST_SPLIT, active && ftype == 2, (int32_t)n);
     appendList(W, LIST_NP1, active && ftype == 1, (int32_t)n);
     if (ctr) {
@@ -648,21 +657,31 @@ __device__ __forceinline__ void loadFeed(FlashCell<NC>& F, const DevBulk& B, con
 }
 
 // Stage 2: PhaseEquilibrium, ftype 0 part up to the stability verdict.
-template <int NC>
+// Guided (a learned K guess is present): one work queue over LIST_STAB
+// followed by LIST_STABG (the cells the learned prior calls unstable), each
+// padded to whole wavefront chunks so a wave works on one list only; the
+// cells of LIST_STABG try the guessed K first.  The plain flash runs the
+// Guided = false instance, which never touches the guess.
+template <int NC, bool Guided>
 __global__ void __launch_bounds__(kFlashBlock) flashStabilityKernel(const DevEos* __restrict__ eos, DevBulk B,
                                                                     FlashWork W, DevFlashCounters* ctr)
 {
     __shared__ double sSq[FlashCell<NC>::NT * kFlashBlock];
-    const int total = W.count[LIST_STAB];
+    const int nA = W.count[LIST_STAB];
+    const int padA = Guided ? (nA + 63) & ~63 : nA;
+    const int nG = Guided ? W.count[LIST_STABG] : 0;
+    const int total = padA + nG;
     for (int base = nextChunk(W, LIST_STAB); base < total; base = nextChunk(W, LIST_STAB)) {
-    const int idx = base + (threadIdx.x & 63);
-    const bool active = idx < total;
+    const bool guidedChunk = Guided && base >= padA;
+    const int in = guidedChunk ? LIST_STABG : LIST_STAB;
+    const int idx = base - (guidedChunk ? padA : 0) + (threadIdx.x & 63);
+    const bool active = idx < (guidedChunk ? nG : nA);
     bool stable = false;
     int it0 = 0, it1 = 0;
     bool nonconv = false;
     int32_t n = 0;
     if (active) {
-        n = W.list[LIST_STAB][idx];
+        n = W.list[in][idx];
         const int64_t ld = B.ld;
         FlashCell<NC> F(*eos, &sSq[threadIdx.x]);
         loadFeed<NC>(F, B, W, n);
@@ -671,10 +690,22 @@ __global__ void __launch_bounds__(kFlashBlock) flashStabilityKernel(const DevEos
         F.flagSkip = true;
         F.singlePhaseFeed();
         F.calKwilson();
-        stable = F.phaseStable();
+        int fl = 0;
+        if constexpr (Guided) {
+            double Kg[NC];
+            if (guidedChunk) {
+#pragma unroll
+                for (int i = 0; i < NC; ++i) Kg[i] = exp((double)W.guess[(int64_t)n * W.guessLd + i]);
+                F.Kguide = Kg;
+            }
+            stable = F.template phaseStable<true>();
+            if (F.guidedProof) fl |= FL_GUIDED;
+        } else {
+            stable = F.phaseStable();
+        }
         it0 = F.it[IT_SSMSTA];
         it1 = F.it[IT_NRSTA];
-        int fl = F.flagSkip ? FL_SKIP : 0;
+        if (F.flagSkip) fl |= FL_SKIP;
         if (F.ssmStaConflag) fl |= FL_SSMSTA;
         if (F.nrStaConflag) fl |= FL_NRSTA;
         W.flags[n] = (uint8_t)fl;
@@ -731,6 +762,9 @@ __global__ void __launch_bounds__(kFlashBlock) flashSplitSSMKernel(const DevEos*
 #pragma unroll
                 for (int i = 0; i < NC; ++i)
                     F.Ks[i] = B.xij[(0 * nc + i) * ld + n] / B.xij[(1 * nc + i) * ld + n];
+            } else if (W.guess && kSplitFromGuess && (fl & FL_GUIDED)) {
+#pragma unroll
+                for (int i = 0; i < NC; ++i) F.Ks[i] = exp((double)W.guess[(int64_t)n * W.guessLd + i]);
             } else {
                 F.calKwilson();
 #pragma unroll
@@ -925,7 +959,10 @@ hipError_t launchFlashT(const DevEos* eos, const DevBulk& B, const FlashWork& W,
     hipLaunchKernelGGL(flashResetKernel<NC>, dim3(1), dim3(64), 0, s, W, ctr);
     hipLaunchKernelGGL(flashClassifyKernel<NC>, dim3(grid), dim3(kFlashBlock), 0, s, eos, B, W, ctr, zin);
     mark(0);
-    hipLaunchKernelGGL(flashStabilityKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, ctr);
+    if (W.guess)
+        hipLaunchKernelGGL((flashStabilityKernel<NC, true>), dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, ctr);
+    else
+        hipLaunchKernelGGL((flashStabilityKernel<NC, false>), dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, ctr);
     mark(1);
     mark(2);
     hipLaunchKernelGGL(flashSplitSSMKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, (int)LIST_SPLIT,

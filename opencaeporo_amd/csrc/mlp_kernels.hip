@@ -1,0 +1,239 @@
+// The learned form's network: a small MLP (P, T, z -> ln K_i and a
+// stability logit) trained and run on the GPU with BF16 MFMA products and
+// FP32 accumulation.  It only supplies the flash's initial K-values and a
+// stability prior (ocpg_flash_fim_guided); the FP64 stability test and split
+// still decide every cell.
+//
+//   mlpGemmKernel   : C = A * B (+ C) with A, B read in FP32 through
+//                     arbitrary strides, rounded to BF16 and multiplied on
+//                     v_mfma_f32_16x16x16_bf16 (FP32 accumulators); one
+//                     wavefront per 16 x 16 tile of C, the K loop optionally
+//                     split across gridDim.z with FP32 partial tiles summed in
+//                     a fixed order (deterministic).
+//   bias/GELU, GELU', column sums, loss and Adam: elementwise / reductions.
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "../../include/ocpgpu.h"
+
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// FP32 -> BF16 bits, round to nearest even (NaN kept quiet).
+__device__ __forceinline__ short toBf16(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (short)((u >> 16) | 0x40);
+    const uint32_t r = u + 0x7fffu + ((u >> 16) & 1u);
+    return (short)(r >> 16);
+}
+
+// One 16 x 16 tile of C per wavefront.  MFMA 16x16x16 operand layout
+// (lane l, j = 0..3): A[i = l % 16][k = 4 (l / 16) + j], B[k = 4 (l / 16) + j]
+// [n = l % 16]; result: C[m = 4 (l / 16) + j][n = l % 16].
+__global__ void __launch_bounds__(64) mlpGemmKernel(int M, int N, int K, const float* __restrict__ A, int64_t sAm,
+                                                    int64_t sAk, const float* __restrict__ B, int64_t sBk,
+                                                    int64_t sBn, float* __restrict__ C, int64_t sCm, int64_t sCn,
+                                                    int accumulate, int kChunk, float* __restrict__ partial)
+{
+    const int lane = threadIdx.x;
+    const int m0 = blockIdx.x * 16, n0 = blockIdx.y * 16;
+    const int kb = blockIdx.z * kChunk;
+    const int ke = min(K, kb + kChunk);
+    const int row = lane & 15;
+    const int kq = (lane >> 4) * 4;
+    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int k0 = kb; k0 < ke; k0 += 16) {
+        bf16x4 a, b;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = k0 + kq + j;
+            const int m = m0 + row, n = n0 + row;
+            const float av = (m < M && k < ke) ? A[(int64_t)m * sAm + (int64_t)k * sAk] : 0.f;
+            const float bv = (n < N && k < ke) ? B[(int64_t)k * sBk + (int64_t)n * sBn] : 0.f;
+            a[j] = toBf16(av);
+            b[j] = toBf16(bv);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    const int n = n0 + row;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int m = m0 + kq + j;
+        if (m >= M || n >= N) continue;
+        if (partial) {
+            partial[((int64_t)blockIdx.z * M + m) * N + n] = acc[j];
+        } else {
+            float* c = &C[(int64_t)m * sCm + (int64_t)n * sCn];
+            *c = accumulate ? *c + acc[j] : acc[j];
+        }
+    }
+}
+
+// C (+)= sum over z of partial[z] (fixed order).
+__global__ void mlpSplitSumKernel(int M, int N, int Z, const float* __restrict__ partial, float* __restrict__ C,
+                                  int64_t sCm, int64_t sCn, int accumulate)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)M * N) return;
+    const int m = (int)(t / N), n = (int)(t % N);
+    float s = 0.f;
+    for (int z = 0; z < Z; ++z) s += partial[((int64_t)z * M + m) * N + n];
+    float* c = &C[(int64_t)m * sCm + (int64_t)n * sCn];
+    *c = accumulate ? *c + s : s;
+}
+
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.f + erff(x * 0.70710678118654752f)); }
+__device__ __forceinline__ float geluGrad(float x)
+{
+    const float cdf = 0.5f * (1.f + erff(x * 0.70710678118654752f));
+    const float pdf = 0.39894228040143268f * expf(-0.5f * x * x);
+    return cdf + x * pdf;
+}
+
+// Z[m][n] += b[n]; H = GELU(Z) (H may be NULL: bias only).
+__global__ void mlpBiasActKernel(int M, int N, float* __restrict__ Z, const float* __restrict__ b,
+                                 float* __restrict__ H)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)M * N) return;
+    const float z = Z[t] + b[t % N];
+    Z[t] = z;
+    if (H) H[t] = gelu(z);
+}
+
+// dH *= GELU'(Z) in place.
+__global__ void mlpActBackKernel(int64_t n, const float* __restrict__ Z, float* __restrict__ dH)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dH[t] *= geluGrad(Z[t]);
+}
+
+// out[n] = sum_m X[m][n], one block per column, fixed reduction order.
+__global__ void mlpColSumKernel(int M, int N, const float* __restrict__ X, float* __restrict__ out)
+{
+    __shared__ float s[256];
+    const int n = blockIdx.x;
+    float v = 0.f;
+    for (int m = threadIdx.x; m < M; m += blockDim.x) v += X[(int64_t)m * N + n];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[n] = s[0];
+}
+
+// Loss per sample: mask * mean_i (Y_i - lnK_i)^2 + wStab * BCE(logit, label);
+// dY (B x (nK + 1)) is its gradient divided by the batch size.  loss[b] gets
+// the per-sample value.
+__global__ void mlpLossKernel(int B, int nK, int ldY, const float* __restrict__ Y, const float* __restrict__ lnK,
+                              const float* __restrict__ label, const float* __restrict__ mask, float wStab,
+                              float* __restrict__ dY, float* __restrict__ loss)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const float* y = Y + (int64_t)b * ldY;
+    float* d = dY + (int64_t)b * ldY;
+    const float mk = mask[b];
+    float l = 0.f;
+    for (int i = 0; i < nK; ++i) {
+        const float e = y[i] - lnK[(int64_t)b * nK + i];
+        l += mk * e * e / nK;
+        d[i] = mk * 2.f * e / nK / B;
+    }
+    const float z = y[nK];
+    const float p = 1.f / (1.f + expf(-z));
+    const float t = label[b];
+    l += wStab * (fmaxf(z, 0.f) - z * t + log1pf(expf(-fabsf(z))));
+    d[nK] = wStab * (p - t) / B;
+    for (int i = nK + 1; i < ldY; ++i) d[i] = 0.f;
+    loss[b] = l;
+}
+
+__global__ void mlpAdamKernel(int64_t n, float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                              float* __restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float gi = g[t];
+    const float mi = b1 * m[t] + (1.f - b1) * gi;
+    const float vi = b2 * v[t] + (1.f - b2) * gi * gi;
+    m[t] = mi;
+    v[t] = vi;
+    w[t] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+inline int status(hipError_t e) { return e == hipSuccess ? OCPG_OK : OCPG_EHIP; }
+inline unsigned blocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+extern "C" {
+
+int ocpg_mlp_gemm(void* stream, int M, int N, int K, const float* A, int64_t sAm, int64_t sAk, const float* B,
+                  int64_t sBk, int64_t sBn, float* C, int64_t sCm, int64_t sCn, int accumulate, int splitK,
+                  float* partial)
+{
+    if (M <= 0 || N <= 0 || K < 0 || !A || !B || !C || splitK < 1 || (splitK > 1 && !partial)) return OCPG_EINVAL;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int kChunk = ((K + splitK - 1) / splitK + 15) / 16 * 16;
+    const int Z = (K + kChunk - 1) / kChunk > 0 ? (K + kChunk - 1) / kChunk : 1;
+    dim3 grid((M + 15) / 16, (N + 15) / 16, Z);
+    hipLaunchKernelGGL(mlpGemmKernel, grid, dim3(64), 0, s, M, N, K, A, sAm, sAk, B, sBk, sBn, C, sCm, sCn,
+                       accumulate, kChunk, Z > 1 ? partial : nullptr);
+    if (Z > 1)
+        hipLaunchKernelGGL(mlpSplitSumKernel, dim3(blocks((int64_t)M * N, 256)), dim3(256), 0, s, M, N, Z, partial,
+                           C, sCm, sCn, accumulate);
+    return status(hipGetLastError());
+}
+
+int ocpg_mlp_bias_act(void* stream, int M, int N, float* Z, const float* b, float* H)
+{
+    if (M <= 0 || N <= 0 || !Z || !b) return OCPG_EINVAL;
+    hipLaunchKernelGGL(mlpBiasActKernel, dim3(blocks((int64_t)M * N, 256)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), M, N, Z, b, H);
+    return status(hipGetLastError());
+}
+
+int ocpg_mlp_act_backward(void* stream, int64_t n, const float* Z, float* dH)
+{
+    if (n <= 0 || !Z || !dH) return OCPG_EINVAL;
+    hipLaunchKernelGGL(mlpActBackKernel, dim3(blocks(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), n,
+                       Z, dH);
+    return status(hipGetLastError());
+}
+
+int ocpg_mlp_colsum(void* stream, int M, int N, const float* X, float* out)
+{
+    if (M <= 0 || N <= 0 || !X || !out) return OCPG_EINVAL;
+    hipLaunchKernelGGL(mlpColSumKernel, dim3(N), dim3(256), 0, static_cast<hipStream_t>(stream), M, N, X, out);
+    return status(hipGetLastError());
+}
+
+int ocpg_mlp_loss(void* stream, int B, int nK, int ldY, const float* Y, const float* lnK, const float* label,
+                  const float* mask, float wStab, float* dY, float* loss)
+{
+    if (B <= 0 || nK <= 0 || ldY < nK + 1 || !Y || !lnK || !label || !mask || !dY || !loss) return OCPG_EINVAL;
+    hipLaunchKernelGGL(mlpLossKernel, dim3(blocks(B, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), B, nK,
+                       ldY, Y, lnK, label, mask, wStab, dY, loss);
+    return status(hipGetLastError());
+}
+
+int ocpg_mlp_adam(void* stream, int64_t n, float* w, const float* g, float* m, float* v, float lr, float b1,
+                  float b2, float eps, int step)
+{
+    if (n <= 0 || !w || !g || !m || !v || step < 1) return OCPG_EINVAL;
+    const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+    hipLaunchKernelGGL(mlpAdamKernel, dim3(blocks(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), n, w,
+                       g, m, v, lr, b1, b2, eps, c1, c2);
+    return status(hipGetLastError());
+}
+
+}  // extern "C"

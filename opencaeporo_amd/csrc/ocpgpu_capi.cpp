@@ -550,6 +550,22 @@ int ocpg_flash_fim(ocpg_ctx* c, ocpg_flash_stats* stats)
     return OCPG_OK;
 }
 
+// FlashFIM with a learned initial guess (no reference counterpart): guess is
+// device memory, nb rows of ld floats (ln K_1..ln K_NC, stability logit).
+// The stability test of a cell whose logit is positive first tries the
+// guessed K; the split starts from it where the reference starts from Wilson.
+int ocpg_flash_fim_guided(ocpg_ctx* c, const float* guess, int ld, ocpg_flash_stats* stats)
+{
+    if (!c || !c->cur) return OCPG_ESTATE;
+    if (!guess || ld < c->NC + 1) return OCPG_EINVAL;
+    c->work.guess = guess;
+    c->work.guessLd = ld;
+    const int rc = ocpg_flash_fim(c, stats);
+    c->work.guess = nullptr;
+    c->work.guessLd = 0;
+    return rc;
+}
+
 // IsoT_FIM::InitFlash (src/OCPFluidMethod.cpp:1095-1106): InitFlashFIM from
 // P, T, Sw (S column 2), rockVp and the feed z (host, nb x NC row-major).
 // Ni becomes an output.

@@ -46,6 +46,15 @@ def load():
         "ocpg_bsr_nnzb": ([P], C.c_int64),
         "ocpg_bsr_rows": ([P], C.c_int64),
         "ocpg_capture_iteration": ([P, C.c_double], C.c_int),
+        "ocpg_flash_fim_guided": ([P, P, C.c_int, P], C.c_int),
+        "ocpg_mlp_gemm": ([P, C.c_int, C.c_int, C.c_int, P, C.c_int64, C.c_int64, P, C.c_int64, C.c_int64,
+                           P, C.c_int64, C.c_int64, C.c_int, C.c_int, P], C.c_int),
+        "ocpg_mlp_bias_act": ([P, C.c_int, C.c_int, P, P, P], C.c_int),
+        "ocpg_mlp_act_backward": ([P, C.c_int64, P, P], C.c_int),
+        "ocpg_mlp_colsum": ([P, C.c_int, C.c_int, P, P], C.c_int),
+        "ocpg_mlp_loss": ([P, C.c_int, C.c_int, C.c_int, P, P, P, P, C.c_float, P, P], C.c_int),
+        "ocpg_mlp_adam": ([P, C.c_int64, P, P, P, P, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int],
+                          C.c_int),
         "ocpg_launch_iteration": ([P], C.c_int),
         "ocpg_iteration_done": ([P], C.c_int),
         "ocpg_set_wells": ([P, P], C.c_int),
@@ -166,6 +175,14 @@ class Context:
 
     def rows(self):
         return int(self.lib.ocpg_bsr_rows(self.h))
+
+    def flash_fim_guided(self, guess, ld):
+        """FlashFIM with a learned guess: `guess` is a device tensor of nb rows
+        of ld float32 (ln K_1..ln K_NC, stability logit)."""
+        st = abi.FlashStats()
+        check(self.lib.ocpg_flash_fim_guided(self.h, C.c_void_p(guess.data_ptr()), ld, C.byref(st)),
+              "ocpg_flash_fim_guided")
+        return st
 
     def capture_iteration(self, dt):
         """Capture one Newton iteration (restore, flash, kr/Pc + rock, CalRes,

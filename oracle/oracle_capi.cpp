@@ -148,4 +148,16 @@ int orc_count_flash_fim(const ocpg_eos* eos, ocpg_bulk* bk, uint64_t* out)
 }
 #endif
 
+#ifdef OCP_ULP_NOISE
+// One serial CalFlash pass with every transcendental perturbed by up to one
+// ulp from the stream `seed` (nonzero).
+int orc_noise_flash_fim(const ocpg_eos* eos, ocpg_bulk* bk, uint64_t seed)
+{
+    if (!eos || !bk || !seed || eos->nc < 1 || eos->nc > OCPG_MAXNC) return OCPG_EINVAL;
+    oracle::g_noise = seed;
+    oracle::CalFlash(*eos, *bk, 1);
+    return OCPG_OK;
+}
+#endif
+
 }  // extern "C"

@@ -12,6 +12,9 @@ namespace oracle {
 
 #ifdef OCP_COUNT_FLOPS
 thread_local FlopTally g_tally;
+#ifdef OCP_ULP_NOISE
+thread_local uint64_t g_noise = 88172645463325252ull;
+#endif
 #endif
 
 static inline real deltaij(int i, int j) { return i == j ? 1.0 : 0.0; }

@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "build", "liboracle.so")
 COUNT_SO = os.path.join(HERE, "build", "liboracle_count.so")
 EXACT_SO = os.path.join(HERE, "build", "liboracle_exact.so")
+NOISE_SO = os.path.join(HERE, "build", "liboracle_noise.so")
 REF_SO = os.path.join(HERE, "_ref", "libocpref.so")
 
 
@@ -176,6 +177,19 @@ class FlopCounter(Oracle):
         assert self.lib.orc_count_flash_fim(C.byref(eos), C.byref(s), out.ctypes.data) == 0
         keys = ("flops", "trans", "sta_flops", "sta_trans", "split_flops", "split_trans")
         return {k: int(v) for k, v in zip(keys, out)}
+
+
+class NoisyOracle(Oracle):
+    """The restatement with every transcendental perturbed by up to one ulp
+    (build/liboracle_noise.so): a stand-in for another correct FP64 libm."""
+
+    def __init__(self):
+        super().__init__(NOISE_SO)
+
+    def flash_fim(self, eos, bulk, seed=1):
+        s = bulk.cstruct()
+        assert self.lib.orc_noise_flash_fim(C.byref(eos), C.byref(s), C.c_uint64(seed)) == 0
+        return bulk
 
 
 class Reference:

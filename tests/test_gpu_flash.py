@@ -35,6 +35,10 @@ def test_flash_matches_oracle_and_reference(orc, name, phase):
     gpu_util.say("  vs oracle:", {k: v for k, v in rep.items() if k != "cells_beyond_tol"})
     gpu_util.say("  cells beyond 1e-10 (each within 10x its rounding sensitivity):",
                  rep["cells_beyond_tol"][:8])
+    # the GPU takes the oracle's path: a different iteration count is a
+    # last-bit difference deciding a convergence test, so it stays rare
+    # (a wrong term in a Newton Jacobian would shift most cells)
+    assert rep["iters_mismatch_cells"] <= 0.1 * b.nb, rep["iters_mismatch_cells"]
     rep2 = parity.compare_state(g, d, f"{phase}_out_", 1e-8, frac=0.97, loose=1e-4)
     gpu_util.say("  vs reference: within 1e-8", rep2["frac_within_tol"])
     assert stats.cells == b.nb
