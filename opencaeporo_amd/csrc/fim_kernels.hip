@@ -339,6 +339,7 @@ __global__ void __launch_bounds__(kAsmBlock) assembleKernel(DevBulk B, FimDevice
     __shared__ double sDF[nc * W];              // own side's dFdXp | dFdXs
     __shared__ double sRec[2][R::LEN];          // [0] own cell, [1] current neighbour
     __shared__ double sDpc[9];                  // own dPcj_dS
+    __shared__ double sPh[2][3];                // per phase of the current connection: dP, transJ
 
     RecSrc src[R::perLane];
 #pragma unroll
@@ -421,13 +422,31 @@ __global__ void __launch_bounds__(kAsmBlock) assembleKernel(DevBulk B, FimDevice
             upRho[j] = F.upRho[c * 3 + j];
         }
 
+        // ---- per phase: whether the upwind cell has it, dP and transJ ----
+        // (lanes 0..2; every entry below reads them)
+        bool live[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const bool upB = ownB ? upOwn[j] : !upOwn[j];
+            live[j] = upB ? exB[j] : exE[j];
+        }
+        if (lane < 3) {
+            const int j = lane;
+            const double* rU = upOwn[j] ? sRec[0] : sRec[1];
+            sPh[0][j] = rB[R::PJ + j] - rE[R::PJ + j] - upRho[j] * dGamma;
+            sPh[1][j] = Akd * rU[R::KR + j] / rU[R::MU + j];
+        }
+        __syncthreads();
+
         // ---- own side's dFdXp column and dFdXs, entry by entry ----
-        for (int q = lane; q < nc * (ncolO + 1); q += kAsmBlock) {
-            const int i = q / (ncolO + 1);
-            const int col = q % (ncolO + 1) - 1;  // -1: pressure column of dFdXp
-            // decode the secondary column: saturation of phase j1S, or x of phase jX, comp kX
-            int j1S = -1, jX = -1, kX = -1;
-            if (col >= 0 && col < nsat) {
+        // Pressure and saturation columns: every phase contributes, in the
+        // reference's phase order.
+        const int nPS = 1 + nsat;
+        for (int q = lane; q < nc * nPS; q += kAsmBlock) {
+            const int i = q / nPS;
+            const int col = q % nPS - 1;  // -1: pressure column of dFdXp
+            int j1S = -1;
+            {
                 int s = 0;
 #pragma unroll
                 for (int j1 = 0; j1 < 3; ++j1)
@@ -435,34 +454,20 @@ __global__ void __launch_bounds__(kAsmBlock) assembleKernel(DevBulk B, FimDevice
                         if (s == col) j1S = j1;
                         s++;
                     }
-            } else if (col >= nsat) {
-                int start = nsat;
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    if (col >= start && col < start + pvO[j]) {
-                        jX = j;
-                        kX = col - start;
-                    }
-                    start += pvO[j];
-                }
             }
             double v = 0;
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
-                // the upwind cell is bId or eId; upB: it is bId
+                if (!live[j]) continue;
                 const bool upB = ownB ? upOwn[j] : !upOwn[j];
-                const bool exU = upB ? exB[j] : exE[j];
-                if (!exU) continue;
                 const double* rU = upOwn[j] ? sRec[0] : sRec[1];
-                const double dP = rB[R::PJ + j] - rE[R::PJ + j] - upRho[j] * dGamma;
+                const double dP = sPh[0][j];
+                const double transJ = sPh[1][j];
                 const double xi = rU[R::XI + j];
-                const double kr = rU[R::KR + j];
                 const double mu = rU[R::MU + j];
-                const double transJ = Akd * kr / mu;
                 const double xij = rU[R::XIJ + j * nc + i];
                 const double transIJ = xij * xi * transJ;
                 if (col < 0) {
-                    // pressure column
                     const double muP = rU[R::MUP + j];
                     const double xiP = rU[R::XIP + j];
                     const double rhoP = rU[R::RHOP + j];
@@ -489,7 +494,7 @@ __global__ void __launch_bounds__(kAsmBlock) assembleKernel(DevBulk B, FimDevice
                             if (!upB) v += tmp;
                         }
                     }
-                } else if (j1S >= 0) {
+                } else {
                     // saturation column of phase j1S
                     const double dpc = sDpc[j * 3 + j1S];
                     if (ownB) {
@@ -499,26 +504,49 @@ __global__ void __launch_bounds__(kAsmBlock) assembleKernel(DevBulk B, FimDevice
                         v -= transIJ * dpc;
                         if (!upB) v += Akd * xij * xi / mu * rU[R::DKR + j * 3 + j1S] * dP;
                     }
-                } else if (jX == j) {
-                    // x_{kX, j} column (only phase j's own block is touched in pass j)
-                    const int kk = kX;
-                    const bool ownUp = ownB ? upB : !upB;
-                    const bool otherEx = ownB ? exE[j] : exB[j];
-                    if (ownUp) {
-                        const double rhox = otherEx ? rO[R::RHOX + j * nc + kk] / 2 : rU[R::RHOX + j * nc + kk];
-                        const double xix = rU[R::XIX + j * nc + kk];
-                        const double mux = rU[R::MUX + j * nc + kk];
-                        double tmp = -transIJ * rhox * dGamma;
-                        tmp += xij * transJ * xix * dP;
-                        tmp += -transIJ * mux / mu * dP;
-                        v += tmp;
-                        if (kk == i) v += xi * transJ * dP;
-                    } else if (otherEx) {
-                        v += -transIJ * rO[R::RHOX + j * nc + kk] / 2 * dGamma;
-                    }
                 }
             }
             sDF[i * W + col + 1] = v;
+        }
+        // x_{kX, jX} columns: only phase jX's pass touches them.
+        const int nX = max(ncolO - nsat, 0);
+        for (int q = lane; q < nc * nX; q += kAsmBlock) {
+            const int i = q / nX;
+            int kk = q % nX, j = 0;
+            if (kk >= pvO[0]) {
+                kk -= pvO[0];
+                j = 1;
+                if (kk >= pvO[1]) {
+                    kk -= pvO[1];
+                    j = 2;
+                }
+            }
+            double v = 0;
+            if (live[j]) {
+                const bool upB = ownB ? upOwn[j] : !upOwn[j];
+                const double* rU = upOwn[j] ? sRec[0] : sRec[1];
+                const double dP = sPh[0][j];
+                const double transJ = sPh[1][j];
+                const double xi = rU[R::XI + j];
+                const double mu = rU[R::MU + j];
+                const double xij = rU[R::XIJ + j * nc + i];
+                const double transIJ = xij * xi * transJ;
+                const bool ownUp = ownB ? upB : !upB;
+                const bool otherEx = ownB ? exE[j] : exB[j];
+                if (ownUp) {
+                    const double rhox = otherEx ? rO[R::RHOX + j * nc + kk] / 2 : rU[R::RHOX + j * nc + kk];
+                    const double xix = rU[R::XIX + j * nc + kk];
+                    const double mux = rU[R::MUX + j * nc + kk];
+                    double tmp = -transIJ * rhox * dGamma;
+                    tmp += xij * transJ * xix * dP;
+                    tmp += -transIJ * mux / mu * dP;
+                    v += tmp;
+                    if (kk == i) v += xi * transJ * dP;
+                } else if (otherEx) {
+                    v += -transIJ * rO[R::RHOX + j * nc + kk] / 2 * dGamma;
+                }
+            }
+            sDF[i * W + 1 + nsat + q % nX] = v;
         }
         __syncthreads();
 
