@@ -11,7 +11,9 @@ components + water + pressure).  Synthetic, seeded inputs.
 One step = one Newton iteration of the reference's FIM loop on every cell:
 restore the iteration's input state (P, N, S, phase count, x_ij, skip memory;
 device-to-device), FlashFIM + PassFlashValue, CalKrPc + CalRock, CalRes,
-AssembleMat.  The input state is the "warm" one: the cold flash result with
+AssembleMat -- by default through ocpg_update_assemble, which overlaps the
+flash's slowest cells with the rest of the iteration (--serial: the four
+calls one after the other; identical results).  The input state is the "warm" one: the cold flash result with
 P perturbed by U[-5, 5] psia.  Inputs are resident in HBM before timing.
 
 Prints one JSON line (see the keys below).
@@ -45,31 +47,6 @@ WARM_SPLIT_TRANS_PER_CELL = 292374158 / 1122000
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-def irregular_wells(grid, P, NC, seed, n_wells=40):
-    """40 wells with 5-20 perforations each at random cells: injectors (rate
-    control, C1-rich gas) and producers (BHP or oil-rate control)."""
-    rng = np.random.default_rng(seed)
-    nc = NC + 1
-    inj_z = np.zeros(nc)
-    inj_z[0], inj_z[1] = 0.9, 0.1
-    wells = []
-    for w in range(n_wells):
-        perf = rng.choice(grid.nb, size=int(rng.integers(5, 21)), replace=False).astype(np.int32)
-        inj = w % 4 == 0
-        Pp = P[perf]
-        d = dict(type=abi.WELL_INJ if inj else abi.WELL_PROD,
-                 mode=abi.WELL_RATE if (inj or w % 4 == 1) else abi.WELL_BHP, perf=perf,
-                 bhp=(Pp.max() + 300.0) if inj else (Pp.min() - 300.0), ctrlRate=2000.0 if inj else 300.0,
-                 WI=rng.uniform(5.0, 50.0, len(perf)), dG=rng.uniform(-5.0, 5.0, len(perf)),
-                 prodPhaseWeight=[1.0, 0.0, 0.0])
-        d["ctrlBHP"] = d["bhp"] + (25.0 if inj else -25.0)
-        if inj:
-            d["injZi"] = inj_z
-            d["perfXi"] = rng.uniform(0.4, 0.6, len(perf))
-        wells.append(d)
-    return abi.Wells(nc, wells)
 
 
 def setup(nx, ny, nz, seed=2024, irregular=False):
@@ -115,7 +92,7 @@ def setup(nx, ny, nz, seed=2024, irregular=False):
     ctx.upload(pb)
     wells = None
     if irregular:
-        wells = irregular_wells(grid, pb.P, NC, seed + 3)
+        wells = workloads.irregular_wells(grid, pb.P, NC, seed + 3)
         ctx.set_wells(wells)
     ctx.calkrpc_rock()
     ctx.calres(DT, norms=False)
@@ -124,12 +101,15 @@ def setup(nx, ny, nz, seed=2024, irregular=False):
     return ctx, grid, nb, NC, wells
 
 
-def step(ctx):
+def step(ctx, serial=False):
     ctx.restore_flash_inputs()
-    ctx.flash_fim(stats=False)
-    ctx.calkrpc_rock()
-    ctx.calres(DT, norms=False)
-    ctx.assemble(DT)
+    if serial:
+        ctx.flash_fim(stats=False)
+        ctx.calkrpc_rock()
+        ctx.calres(DT, norms=False)
+        ctx.assemble(DT)
+    else:
+        ctx.update_assemble(DT)
 
 
 def cpu_baseline(nthreads=None, sample_layers=6, seed=2024):
@@ -192,6 +172,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--nz", type=int, default=85, help="layers of the 60 x 220 x nz grid")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--serial", action="store_true",
+                    help="one Newton iteration as four serial calls (flash, kr/Pc + rock, CalRes, assembly) "
+                         "instead of ocpg_update_assemble")
     ap.add_argument("--irregular", action="store_true",
                     help="the irregular-stencil configuration: 10-component fluid, faulted grid with "
                          "non-neighbour connections, 40 wells (a parity/measurement case, not the headline)")
@@ -204,18 +187,19 @@ def main():
     nx, ny, nz = 60, 220, a.nz
     ctx, grid, nb, NC, wells = setup(nx, ny, nz, irregular=a.irregular)
     for _ in range(a.warmup):
-        step(ctx)
+        step(ctx, a.serial)
     ctx.synchronize()
     ctx.timing(True)
     ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(a.steps):
-        step(ctx)
+        step(ctx, a.serial)
     ctx.synchronize()
     t1 = time.perf_counter()
     ms_step = (t1 - t0) * 1e3 / a.steps
     kern = {k: ctx.timer(k) for k in ("restore", "flash", "flash_stability", "flash_split", "flash_finish",
-                                      "krpc_rock", "calres", "assemble")}
+                                      "flash_tail", "krpc_rock", "calres", "assemble", "fixup",
+                                      "update_assemble")}
     ctx.timing(False)
     per = {k: (v[0] / max(v[1], 1)) for k, v in kern.items()}
     log("[bench] per-launch ms: " + ", ".join(f"{k} {v:.3f}" for k, v in per.items()))
@@ -227,7 +211,7 @@ def main():
     asm_bytes = nnzb * ncol * ncol * 8 + nnzb * 4 * 3 + (nb + 1) * 4 + nb * 8 * (
         1 + 3 * 3 + 2 * 3 * (NC + 1) + 2 * 9 + 2 + 3 + (3 + 2 * NC) * ncol) * 1.0 + grid.nconn * 3 * 12
     flash_ms = per["flash"]
-    launches = {k: v for k, v in per.items() if k not in ("flash",)}
+    launches = {k: v for k, v in per.items() if k not in ("flash", "update_assemble")}
     dominant = max(launches, key=launches.get)
     # roofline of the dominant kernel: the split (FP64-bound; vector and
     # matrix FP64 peaks coincide on MI355X) or the assembly (HBM-bound)

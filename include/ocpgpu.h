@@ -294,6 +294,16 @@ int ocpg_calres(ocpg_ctx* ctx, double dt, ocpg_resnorm* norms);
  * (:1569-1839) and AssembleMatWellsNew (:2417-2733) into the device BSR.
  * Reads the upwinding and well rates of the preceding ocpg_calres.          */
 int ocpg_assemble(ocpg_ctx* ctx, double dt);
+/* IsoT_FIM::UpdateProperty (CalFlash, CalKrPc, CalRock, the wells'
+ * CalTrans/CalFlux, CalRes; src/OCPFluidMethod.cpp:877-897) followed by
+ * IsoT_FIM::AssembleMat (:860-875): the results of ocpg_flash_fim,
+ * ocpg_calkrpc_rock, ocpg_calres and ocpg_assemble in one call, which lets
+ * the flash's slowest cells (long stability tests, split restarts) run on
+ * side streams while the other cells' properties, residual and Jacobian are
+ * formed, and then recomputes the rows that read the slow cells.  Identical
+ * results to the four calls.  norms and stats may be NULL (then the call is
+ * asynchronous).  Needs ocpg_set_conn and the kr/Pc tables.               */
+int ocpg_update_assemble(ocpg_ctx* ctx, double dt, ocpg_resnorm* norms, ocpg_flash_stats* stats);
 /* The assembled system as VectorFaspSolver::AssembleMat copies it into a
  * dBSRmat (src/FaspSolver.cpp:362-429): IA, JA, val (row-major blocks), rhs.
  * Any pointer may be NULL.                                                    */
@@ -362,7 +372,10 @@ int ocpg_save_flash_inputs(ocpg_ctx* ctx);
 int ocpg_restore_flash_inputs(ocpg_ctx* ctx);
 
 /* Per-kernel HIP-event timers on the context's stream.  Names: "flash",
- * "krpc_rock", "calres", "assemble", "init_flash", "restore".  No reference
+ * "krpc_rock", "calres", "assemble", "init_flash", "restore", the flash's
+ * stages "flash_stability", "flash_split", "flash_finish", and for
+ * ocpg_update_assemble "flash_tail" (side stream C), "fixup" and
+ * "update_assemble" (the whole call).  No reference
  * counterpart (the reference times whole phases with GetWallTime,
  * include/UtilTiming.hpp:31-62).                                             */
 int ocpg_timing_enable(ocpg_ctx* ctx, int on);

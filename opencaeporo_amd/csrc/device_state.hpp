@@ -68,8 +68,27 @@ struct DevBulk {
 };
 
 // Scratch of the staged flash (cells in SoA with the bulk's ld; lists of
-// cell indices with their lengths in count[4]).
-constexpr int kFlashLists = 8;
+// cell indices, their lengths in count[0, kFlashLists) and their work-queue
+// cursors in count[kFlashLists, 2 * kFlashLists)).
+constexpr int kFlashLists = 13;
+
+// The staged flash's cell lists.
+enum {
+    LIST_STAB = 0,     // ftype 0: stability test
+    LIST_SPLIT = 1,    // split from the start
+    LIST_NP1 = 2,      // finished with one hydrocarbon phase
+    LIST_NP2 = 3,      // finished with two
+    LIST_C1 = 4,       // split continues after the first slice
+    LIST_C2 = 5,       // ... after the second
+    LIST_NR = 6,       // successive substitution done: Newton stage
+    LIST_STABG = 7,    // stability test trying a learned guess first
+    LIST_SPLITB = 8,   // side stream B: unstable cells of its stability test
+    LIST_TSSM = 9,     // side stream C: split slices' leftovers
+    LIST_TNR = 10,     // side stream C: cells whose first Newton split failed
+    LIST_NP1T = 11,    // side cells finished with one hydrocarbon phase
+    LIST_NP2T = 12,    // ... with two
+};
+static_assert(LIST_NP2T < kFlashLists, "FlashWork holds too few lists");
 
 struct FlashWork {
     double* zi;      // NC per cell: feed mole fractions
@@ -82,11 +101,12 @@ struct FlashWork {
     double* Se;      // its last residual
     int32_t* ssmIt;  // its iterations so far
     int32_t* rrIt;   // Rachford-Rice iterations so far
+    int32_t* nrIt;   // the split's Newton iterations so far (a cell whose restarts are deferred)
     int8_t* ftype;
     int8_t* lNP;
     uint8_t* flags;
     int32_t* list[kFlashLists];
-    int32_t* count;  // list lengths [0, 8), work-queue cursors [8, 16)
+    int32_t* count;  // list lengths, then work-queue cursors
     // learned initial guess (NULL: none): per cell, ln K_i (NC values) and a
     // stability logit at guess[n * guessLd + ...]
     const float* guess;

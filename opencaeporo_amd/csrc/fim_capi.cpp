@@ -27,6 +27,7 @@ const DevSat* ctxSat(ocpg_ctx* c);
 const DevRock& ctxRock(ocpg_ctx* c);
 void ctxTimerBegin(ocpg_ctx* c, int slot);
 void ctxTimerEnd(ocpg_ctx* c, int slot);
+int fimReadNorms(ocpg_ctx* c, ocpg_resnorm* norms);
 }  // namespace ocpg
 
 // timer slots of ocpgpu_capi.cpp
@@ -265,6 +266,45 @@ static int buildPattern(ocpg_ctx* c)
     return uploadWellState(c);
 }
 
+namespace ocpg {
+// The relative-residual maxima of the last CalRes (waits for the stream).
+int fimReadNorms(ocpg_ctx* c, ocpg_resnorm* norms)
+{
+    FimDevice& F = ctxFim(c);
+    WellDevice& W = ctxWells(c);
+    {
+        std::vector<double> v((size_t)F.normBlocks * 2);
+        std::vector<int64_t> id((size_t)F.normBlocks * 2);
+        HIPCHK2(hipMemcpyAsync(v.data(), F.normPart, v.size() * sizeof(double), hipMemcpyDeviceToHost,
+                               ctxStream(c)));
+        HIPCHK2(hipMemcpyAsync(id.data(), F.normIdx, id.size() * sizeof(int64_t), hipMemcpyDeviceToHost,
+                               ctxStream(c)));
+        HIPCHK2(hipStreamSynchronize(ctxStream(c)));
+        ocpg_resnorm r = {0, 0, 0, 0, 0};
+        if (W.nw > 0) {
+            std::vector<double> rel(W.nw);
+            HIPCHK2(hipMemcpyAsync(rel.data(), W.wellRelRes, rel.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                   ctxStream(c)));
+            HIPCHK2(hipStreamSynchronize(ctxStream(c)));
+            for (double v : rel) r.maxWellRelRes_mol = std::max(r.maxWellRelRes_mol, v);
+        }
+        for (int b = 0; b < F.normBlocks; b++) {
+            if (r.maxRelRes_V < v[2 * b]) {
+                r.maxRelRes_V = v[2 * b];
+                r.maxId_V = id[2 * b];
+            }
+            if (r.maxRelRes_N < v[2 * b + 1]) {
+                r.maxRelRes_N = v[2 * b + 1];
+                r.maxId_N = id[2 * b + 1];
+            }
+        }
+        *norms = r;
+    }
+    return OCPG_OK;
+}
+
+}  // namespace ocpg
+
 extern "C" {
 
 // BulkConn::Setup/CalAkd output (src/BulkConn.cpp:31-117) and the BSR
@@ -453,34 +493,7 @@ int ocpg_calres(ocpg_ctx* c, double dt, ocpg_resnorm* norms)
     if (W.nw > 0) HIPCHK2(launchWellFlux(ctxNC(c), ctxEos(c), ctxBulk(c), F, W, ctxStream(c)));
     HIPCHK2(launchCalRes(ctxNC(c), ctxBulk(c), F, W, dt, ctxStream(c)));
     ctxTimerEnd(c, T_RES);
-    if (norms) {
-        std::vector<double> v((size_t)F.normBlocks * 2);
-        std::vector<int64_t> id((size_t)F.normBlocks * 2);
-        HIPCHK2(hipMemcpyAsync(v.data(), F.normPart, v.size() * sizeof(double), hipMemcpyDeviceToHost,
-                               ctxStream(c)));
-        HIPCHK2(hipMemcpyAsync(id.data(), F.normIdx, id.size() * sizeof(int64_t), hipMemcpyDeviceToHost,
-                               ctxStream(c)));
-        HIPCHK2(hipStreamSynchronize(ctxStream(c)));
-        ocpg_resnorm r = {0, 0, 0, 0, 0};
-        if (W.nw > 0) {
-            std::vector<double> rel(W.nw);
-            HIPCHK2(hipMemcpyAsync(rel.data(), W.wellRelRes, rel.size() * sizeof(double), hipMemcpyDeviceToHost,
-                                   ctxStream(c)));
-            HIPCHK2(hipStreamSynchronize(ctxStream(c)));
-            for (double v : rel) r.maxWellRelRes_mol = std::max(r.maxWellRelRes_mol, v);
-        }
-        for (int b = 0; b < F.normBlocks; b++) {
-            if (r.maxRelRes_V < v[2 * b]) {
-                r.maxRelRes_V = v[2 * b];
-                r.maxId_V = id[2 * b];
-            }
-            if (r.maxRelRes_N < v[2 * b + 1]) {
-                r.maxRelRes_N = v[2 * b + 1];
-                r.maxId_N = id[2 * b + 1];
-            }
-        }
-        *norms = r;
-    }
+    if (norms) return fimReadNorms(c, norms);
     return OCPG_OK;
 }
 

@@ -2,8 +2,13 @@
 //   krPcRockKernel : IsoT_FIM::CalKrPc (FlowUnit_ODGW01::CalKrPcDeriv, Stone II)
 //                    and IsothermalMethod::CalRock (Rock_Linear)
 //   calResKernel   : IsoT_FIM::CalRes, bulk part (accumulation + upwinded
-//                    TPFA fluxes), relative-residual maxima, negation
+//                    TPFA fluxes), negation; resNormKernel: its relative-
+//                    residual maxima
 //   assembleKernel : IsoT_FIM::AssembleMatBulksNew into the reference's BSR
+// plus list-driven variants for the rows the overlapped Newton iteration
+// recomputes.
+#include <algorithm>
+
 #include "fim_kernels.hpp"
 
 namespace ocpg {
@@ -43,11 +48,9 @@ __device__ __forceinline__ void tableEval4(const double (*tab)[4], int nRow, dou
 // FlowUnit_ODGW01::CalKrPcDeriv + CalKro_Stone2Der (src/FlowUnit.cpp:235-328),
 // Pj = P + Pc (src/OCPFluidMethod.cpp:1189-1199); Rock_Linear::CalPoro
 // (src/Rock.cpp:19-27) and rockVp = v * poro (src/OCPFluidMethod.cpp:25-32).
-__global__ void __launch_bounds__(256) krPcRockKernel(DevBulk B, const DevSat* __restrict__ sat, DevRock rock,
-                                                      int krpc, int rockOn)
+__device__ __forceinline__ void krPcRockCell(const DevBulk& B, const DevSat* __restrict__ sat, const DevRock& rock,
+                                             int krpc, int rockOn, int64_t n)
 {
-    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= B.nb) return;
     const int64_t ld = B.ld;
     if (krpc) {
         double data[4], cdata[4];
@@ -98,6 +101,101 @@ __global__ void __launch_bounds__(256) krPcRockKernel(DevBulk B, const DevSat* _
     }
 }
 
+__global__ void __launch_bounds__(256) krPcRockKernel(DevBulk B, const DevSat* __restrict__ sat, DevRock rock,
+                                                      int krpc, int rockOn)
+{
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= B.nb) return;
+    krPcRockCell(B, sat, rock, krpc, rockOn, n);
+}
+
+// kr/Pc and rock of the cells in two lists (the overlapped flash's side cells).
+__global__ void __launch_bounds__(256) krPcRockListKernel(DevBulk B, const DevSat* __restrict__ sat, DevRock rock,
+                                                          const int32_t* __restrict__ L0,
+                                                          const int32_t* __restrict__ L1,
+                                                          const int32_t* __restrict__ c0,
+                                                          const int32_t* __restrict__ c1)
+{
+    const int64_t n0 = *c0, total = n0 + *c1;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x)
+        krPcRockCell(B, sat, rock, 1, 1, i < n0 ? L0[i] : L1[i - n0]);
+}
+
+hipError_t launchKrPcRockList(const DevBulk& B, const DevSat* sat, const DevRock& rock, const int32_t* const* lists,
+                              const int32_t* const* counts, hipStream_t s)
+{
+    hipLaunchKernelGGL(krPcRockListKernel, dim3(64), dim3(256), 0, s, B, sat, rock, lists[0], lists[1], counts[0],
+                       counts[1]);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Rows the overlapped form recomputes once its side streams have joined:
+// what CalRes and the assembly computed on the main stream is final for a
+// row whose cell and neighbours all finished there, and is recomputed for
+// the side cells (bit 2) and their neighbours (bit 1); CalRes also for the
+// perforated cells (bit 4: the wells' rates read the side cells too).
+// mark[] is all zero between calls: the compaction clears what it reads.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) fixupMarkKernel(FimDevice F, const int32_t* __restrict__ L0,
+                                                       const int32_t* __restrict__ L1,
+                                                       const int32_t* __restrict__ c0,
+                                                       const int32_t* __restrict__ c1, uint32_t* mark,
+                                                       int32_t* counts)
+{
+    if (blockIdx.x == 0 && threadIdx.x < 2) counts[threadIdx.x] = 0;
+    const int64_t n0 = *c0, total = n0 + *c1;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t t = i < n0 ? L0[i] : L1[i - n0];
+        atomicOr(&mark[t], 2u);
+        for (int k = F.IA[t] + 1; k < F.bulkEnd[t]; ++k) atomicOr(&mark[F.JA[k]], 1u);
+    }
+}
+
+__global__ void __launch_bounds__(256) fixupMarkPerfKernel(WellDevice W, uint32_t* mark)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < W.npcell) atomicOr(&mark[W.pcell[i]], 4u);
+}
+
+__device__ __forceinline__ void appendRow(int32_t* list, int32_t* count, bool pred, int32_t row)
+{
+    const unsigned long long m = __ballot(pred);
+    if (m == 0) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(count, __popcll(m));
+    base = __shfl(base, leader, 64);
+    if (pred) base += __popcll(lane == 0 ? 0ull : (m & ((1ull << lane) - 1ull)));
+    if (pred) list[base] = row;
+}
+
+__global__ void __launch_bounds__(256) fixupCompactKernel(int64_t nb, uint32_t* mark, int32_t* resRows,
+                                                          int32_t* asmRows, int32_t* counts)
+{
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t m = 0;
+    if (n < nb) {
+        m = mark[n];
+        if (m) mark[n] = 0;
+    }
+    appendRow(resRows, counts + 0, m != 0, (int32_t)n);
+    appendRow(asmRows, counts + 1, (m & 3u) != 0, (int32_t)n);
+}
+
+hipError_t launchFixupRows(const FimDevice& F, const WellDevice& W, int64_t nb, const int32_t* const* lists,
+                           const int32_t* const* counts, const FixupRows& R, hipStream_t s)
+{
+    hipLaunchKernelGGL(fixupMarkKernel, dim3(64), dim3(256), 0, s, F, lists[0], lists[1], counts[0], counts[1],
+                       R.mark, R.counts);
+    if (W.npcell > 0)
+        hipLaunchKernelGGL(fixupMarkPerfKernel, dim3((W.npcell + 255) / 256), dim3(256), 0, s, W, R.mark);
+    hipLaunchKernelGGL(fixupCompactKernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, nb, R.mark,
+                       R.resRows, R.asmRows, R.counts);
+    return hipGetLastError();
+}
+
 hipError_t launchKrPcRock(const DevBulk& B, const DevSat* sat, const DevRock& rock, bool krpc, bool rockOn,
                           hipStream_t s)
 {
@@ -117,15 +215,13 @@ hipError_t launchKrPcRock(const DevBulk& B, const DevSat* sat, const DevRock& ro
 constexpr int kResBlock = 256;
 
 template <int NC>
-__global__ void __launch_bounds__(kResBlock) calResKernel(DevBulk B, FimDevice F, WellDevice W, double dt)
+__device__ __forceinline__ void calResCell(const DevBulk& B, const FimDevice& F, const WellDevice& W, double dt,
+                                           int64_t n)
 {
     constexpr int nc = NC + 1;
     constexpr int len = nc + 1;
-    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t ld = B.ld;
-    double vmax = -1.0, nmax = -1.0;
-    int64_t vidx = 0, nidx = 0;
-    if (n < B.nb) {
+    {
         double r[len];
         r[0] = B.rockVp[n] - B.vf[n];
 #pragma unroll
@@ -186,10 +282,43 @@ __global__ void __launch_bounds__(kResBlock) calResKernel(DevBulk B, FimDevice F
                 for (int i = 0; i < nc; ++i) r[1 + i] += qi[i] * dt;
             }
         }
+#pragma unroll
+        for (int e = 0; e < len; ++e) F.res[n * len + e] = -r[e];
+    }
+}
+
+template <int NC>
+__global__ void __launch_bounds__(kResBlock) calResKernel(DevBulk B, FimDevice F, WellDevice W, double dt)
+{
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < B.nb) calResCell<NC>(B, F, W, dt, n);
+}
+
+// CalRes of the rows in rows[0, *count) (the overlapped form's recomputation).
+template <int NC>
+__global__ void __launch_bounds__(kResBlock) calResListKernel(DevBulk B, FimDevice F, WellDevice W, double dt,
+                                                              const int32_t* __restrict__ rows,
+                                                              const int32_t* __restrict__ count)
+{
+    const int64_t total = *count;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x)
+        calResCell<NC>(B, F, W, dt, rows[i]);
+}
+
+// CalRes's relative-residual maxima (src/OCPFluidMethod.cpp:1362-1387) from
+// the stored right-hand side (the negated residual: |-r / v| = |r / v|).
+template <int NC>
+__global__ void __launch_bounds__(kResBlock) resNormKernel(DevBulk B, FimDevice F)
+{
+    constexpr int len = NC + 2;
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double vmax = -1.0, nmax = -1.0;
+    int64_t vidx = 0, nidx = 0;
+    if (n < B.nb) {
         const double rvp = B.rockVp[n], Nt = B.Nt[n];
 #pragma unroll
         for (int e = 0; e < len; ++e) {
-            const double t = fabs(r[e] / rvp);
+            const double t = fabs(F.res[n * len + e] / rvp);
             if (vmax < t) {
                 vmax = t;
                 vidx = n;
@@ -197,14 +326,12 @@ __global__ void __launch_bounds__(kResBlock) calResKernel(DevBulk B, FimDevice F
         }
 #pragma unroll
         for (int e = 1; e < len; ++e) {
-            const double t = fabs(r[e] / Nt);
+            const double t = fabs(F.res[n * len + e] / Nt);
             if (nmax < t) {
                 nmax = t;
                 nidx = n;
             }
         }
-#pragma unroll
-        for (int e = 0; e < len; ++e) F.res[n * len + e] = -r[e];
     }
     // block maxima, ties to the lowest cell (the reference scans cells upward
     // and keeps the first strict maximum)
@@ -320,7 +447,20 @@ __device__ __forceinline__ double loadRec(const RecSrc& s, int64_t cell)
 }
 
 template <int NC>
-__global__ void __launch_bounds__(kAsmBlock) assembleKernel(DevBulk B, FimDevice F, double dt)
+struct AsmShared {
+    using D = AsmDims<NC>;
+    using R = AsmRec<NC>;
+    double sSec[D::maxRows * D::ncol];  // own dSec_dPri (compact rows x ncol)
+    double sDF[D::nc * D::dfStride];    // own side's dFdXp | dFdXs
+    double sRec[2][R::LEN];             // [0] own cell, [1] current neighbour
+    double sDpc[9];                     // own dPcj_dS
+    double sPh[2][3];                   // per phase of the current connection: dP, transJ
+};
+
+// Row n of the bulk assembly, by one wavefront.
+template <int NC>
+__device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F, double dt, int64_t n,
+                                            AsmShared<NC>& sh)
 {
     using D = AsmDims<NC>;
     using R = AsmRec<NC>;
@@ -328,18 +468,15 @@ __global__ void __launch_bounds__(kAsmBlock) assembleKernel(DevBulk B, FimDevice
     constexpr int ncol = D::ncol;
     constexpr int bsize = D::bsize;
     constexpr int W = D::dfStride;
-    // workgroup b runs on XCD b % kXcds; XCD x takes rows [x*chunk, (x+1)*chunk)
-    const int64_t chunk = gridDim.x / kXcds;
-    const int64_t n = (int64_t)(blockIdx.x % kXcds) * chunk + blockIdx.x / kXcds;
-    if (n >= B.nb) return;
     const int64_t ld = B.ld;
     const int lane = threadIdx.x;
-
-    __shared__ double sSec[D::maxRows * ncol];  // own dSec_dPri (compact rows x ncol)
-    __shared__ double sDF[nc * W];              // own side's dFdXp | dFdXs
-    __shared__ double sRec[2][R::LEN];          // [0] own cell, [1] current neighbour
-    __shared__ double sDpc[9];                  // own dPcj_dS
-    __shared__ double sPh[2][3];                // per phase of the current connection: dP, transJ
+    double* sSec = sh.sSec;
+    double* sDF = sh.sDF;
+    double (*sRec)[R::LEN] = sh.sRec;
+    double* sDpc = sh.sDpc;
+    double (*sPh)[3] = sh.sPh;
+    // a previous row of this wavefront may still read the shared arrays
+    __syncthreads();
 
     RecSrc src[R::perLane];
 #pragma unroll
@@ -586,10 +723,60 @@ __global__ void __launch_bounds__(kAsmBlock) assembleKernel(DevBulk B, FimDevice
 }
 
 template <int NC>
-static hipError_t launchCalResT(const DevBulk& B, FimDevice& F, const WellDevice& W, double dt, hipStream_t s)
+__global__ void __launch_bounds__(kAsmBlock) assembleKernel(DevBulk B, FimDevice F, double dt)
+{
+    __shared__ AsmShared<NC> sh;
+    // workgroup b runs on XCD b % kXcds; XCD x takes rows [x*chunk, (x+1)*chunk)
+    const int64_t chunk = gridDim.x / kXcds;
+    const int64_t n = (int64_t)(blockIdx.x % kXcds) * chunk + blockIdx.x / kXcds;
+    if (n >= B.nb) return;
+    assembleRow<NC>(B, F, dt, n, sh);
+}
+
+// The rows in rows[0, *count) (the overlapped form's recomputation).
+template <int NC>
+__global__ void __launch_bounds__(kAsmBlock) assembleListKernel(DevBulk B, FimDevice F, double dt,
+                                                                const int32_t* __restrict__ rows,
+                                                                const int32_t* __restrict__ count)
+{
+    __shared__ AsmShared<NC> sh;
+    const int64_t total = *count;
+    for (int64_t i = blockIdx.x; i < total; i += gridDim.x) assembleRow<NC>(B, F, dt, rows[i], sh);
+}
+
+template <int NC>
+static hipError_t launchCalResT(const DevBulk& B, FimDevice& F, const WellDevice& W, double dt, bool norms,
+                                hipStream_t s)
 {
     const unsigned grid = (unsigned)((B.nb + kResBlock - 1) / kResBlock);
     hipLaunchKernelGGL(calResKernel<NC>, dim3(grid), dim3(kResBlock), 0, s, B, F, W, dt);
+    if (norms) hipLaunchKernelGGL(resNormKernel<NC>, dim3(grid), dim3(kResBlock), 0, s, B, F);
+    return hipGetLastError();
+}
+
+template <int NC>
+static hipError_t launchCalResRowsT(const DevBulk& B, FimDevice& F, const WellDevice& W, double dt,
+                                    const int32_t* rows, const int32_t* count, hipStream_t s)
+{
+    const unsigned grid = (unsigned)std::min<int64_t>((B.nb + kResBlock - 1) / kResBlock, 1024);
+    hipLaunchKernelGGL(calResListKernel<NC>, dim3(grid), dim3(kResBlock), 0, s, B, F, W, dt, rows, count);
+    return hipGetLastError();
+}
+
+template <int NC>
+static hipError_t launchResNormT(const DevBulk& B, FimDevice& F, hipStream_t s)
+{
+    const unsigned grid = (unsigned)((B.nb + kResBlock - 1) / kResBlock);
+    hipLaunchKernelGGL(resNormKernel<NC>, dim3(grid), dim3(kResBlock), 0, s, B, F);
+    return hipGetLastError();
+}
+
+template <int NC>
+static hipError_t launchAssembleRowsT(const DevBulk& B, FimDevice& F, double dt, const int32_t* rows,
+                                      const int32_t* count, hipStream_t s)
+{
+    const unsigned grid = (unsigned)std::min<int64_t>(B.nb, 4096);
+    hipLaunchKernelGGL(assembleListKernel<NC>, dim3(grid), dim3(kAsmBlock), 0, s, B, F, dt, rows, count);
     return hipGetLastError();
 }
 
@@ -603,11 +790,44 @@ static hipError_t launchAssembleT(const DevBulk& B, FimDevice& F, double dt, hip
 
 #define OCPG_FIM_NC(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
 
-hipError_t launchCalRes(int NC, const DevBulk& B, FimDevice& F, const WellDevice& W, double dt, hipStream_t s)
+hipError_t launchCalRes(int NC, const DevBulk& B, FimDevice& F, const WellDevice& W, double dt, hipStream_t s,
+                        bool norms)
 {
     if (B.nb == 0) return hipSuccess;
 #define X(N) \
-    if (NC == N) return launchCalResT<N>(B, F, W, dt, s);
+    if (NC == N) return launchCalResT<N>(B, F, W, dt, norms, s);
+    OCPG_FIM_NC(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launchCalResRows(int NC, const DevBulk& B, FimDevice& F, const WellDevice& W, double dt,
+                            const int32_t* rows, const int32_t* count, hipStream_t s)
+{
+    if (B.nb == 0) return hipSuccess;
+#define X(N) \
+    if (NC == N) return launchCalResRowsT<N>(B, F, W, dt, rows, count, s);
+    OCPG_FIM_NC(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launchResNorm(int NC, const DevBulk& B, FimDevice& F, hipStream_t s)
+{
+    if (B.nb == 0) return hipSuccess;
+#define X(N) \
+    if (NC == N) return launchResNormT<N>(B, F, s);
+    OCPG_FIM_NC(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launchAssembleRows(int NC, const DevBulk& B, FimDevice& F, double dt, const int32_t* rows,
+                              const int32_t* count, hipStream_t s)
+{
+    if (B.nb == 0) return hipSuccess;
+#define X(N) \
+    if (NC == N) return launchAssembleRowsT<N>(B, F, dt, rows, count, s);
     OCPG_FIM_NC(X)
 #undef X
     return hipErrorInvalidValue;

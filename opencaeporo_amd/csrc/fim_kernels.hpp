@@ -44,7 +44,32 @@ size_t fimBytes(const FimDevice& f);
 
 hipError_t launchKrPcRock(const DevBulk& B, const DevSat* sat, const DevRock& rock, bool krpc, bool rockOn,
                           hipStream_t s);
-hipError_t launchCalRes(int NC, const DevBulk& B, FimDevice& F, const WellDevice& W, double dt, hipStream_t s);
+// CalRes over every cell; with norms, also the per-block maxima of the
+// relative residuals (normPart / normIdx).
+hipError_t launchCalRes(int NC, const DevBulk& B, FimDevice& F, const WellDevice& W, double dt, hipStream_t s,
+                        bool norms = true);
+hipError_t launchResNorm(int NC, const DevBulk& B, FimDevice& F, hipStream_t s);
 hipError_t launchAssemble(int NC, const DevBulk& B, FimDevice& F, double dt, hipStream_t s);
+
+// The overlapped Newton iteration's recomputation (fim_kernels.hip): rows
+// touched by the cells the flash finished on its side streams.
+struct FixupRows {
+    uint32_t* mark = nullptr;     // nb flags, all zero between calls
+    int32_t* resRows = nullptr;   // nb: rows whose CalRes is recomputed
+    int32_t* asmRows = nullptr;   // nb: rows whose assembly is recomputed
+    int32_t* counts = nullptr;    // [0] resRows, [1] asmRows
+    void* mem = nullptr;
+    int64_t nb = 0;
+};
+// kr/Pc + rock of the cells in lists[0..1] (lengths at counts[0..1], device).
+hipError_t launchKrPcRockList(const DevBulk& B, const DevSat* sat, const DevRock& rock, const int32_t* const* lists,
+                              const int32_t* const* counts, hipStream_t s);
+// The rows to recompute for the side cells in lists[0..1].
+hipError_t launchFixupRows(const FimDevice& F, const WellDevice& W, int64_t nb, const int32_t* const* lists,
+                           const int32_t* const* counts, const FixupRows& R, hipStream_t s);
+hipError_t launchCalResRows(int NC, const DevBulk& B, FimDevice& F, const WellDevice& W, double dt,
+                            const int32_t* rows, const int32_t* count, hipStream_t s);
+hipError_t launchAssembleRows(int NC, const DevBulk& B, FimDevice& F, double dt, const int32_t* rows,
+                              const int32_t* count, hipStream_t s);
 
 }  // namespace ocpg

@@ -743,11 +743,12 @@ struct FlashCell {
         return true;
     }
 
-    // The rest of PhaseSplit after its first SplitSSM2: Newton, the restart
-    // loop and CheckSplit.
-    __device__ __forceinline__ void phaseSplitAfterSSM()
+    // PhaseSplit after its first SplitSSM2 and first SplitNR: the restart
+    // loop (which does not run when that Newton split converged) and
+    // CheckSplit.  A restart depends only on Ks, the feed and the counters,
+    // so a cell can leave after its first SplitNR and run this later.
+    __device__ __forceinline__ void splitRestarts()
     {
-        splitNR();
         int restarts = 0;
         while (!nrSpConflag) {
             splitSSM2(true);
@@ -757,6 +758,14 @@ struct FlashCell {
             if (++restarts >= kMaxSplitRestarts) break;
         }
         checkSplit();
+    }
+
+    // The rest of PhaseSplit after its first SplitSSM2: Newton, the restart
+    // loop and CheckSplit.
+    __device__ __forceinline__ void phaseSplitAfterSSM()
+    {
+        splitNR();
+        splitRestarts();
     }
 
     // d ln f_ij / d n_kj of phase j (CalFugNAll, src/MixtureComp.cpp:2168-2250),

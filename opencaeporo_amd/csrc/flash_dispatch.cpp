@@ -17,13 +17,30 @@ bool flashSupportsNC(int nc)
 }
 
 hipError_t launchFlash(int NC, const DevEos* eos, const DevBulk& B, const FlashWork& W, DevFlashCounters* ctr,
-                       const double* zin, hipStream_t s, hipEvent_t* ev)
+                       const double* zin, hipStream_t s, hipEvent_t* ev, const FlashSide* side)
 {
 #define X(N) \
-    if (NC == N) return launchFlashT<N>(eos, B, W, ctr, zin, s, ev);
+    if (NC == N) return launchFlashT<N>(eos, B, W, ctr, zin, s, ev, side);
     OCPG_NC_LIST(X)
 #undef X
     return hipErrorInvalidValue;
+}
+
+hipError_t launchFlashTailFinish(int NC, const DevEos* eos, const DevBulk& B, const FlashWork& W, hipStream_t s)
+{
+#define X(N) \
+    if (NC == N) return launchFlashTailFinishT<N>(eos, B, W, s);
+    OCPG_NC_LIST(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+void flashTailListsOf(const FlashWork& W, const int32_t** lists, const int32_t** counts)
+{
+    lists[0] = W.list[LIST_NP1T];
+    lists[1] = W.list[LIST_NP2T];
+    counts[0] = W.count + LIST_NP1T;
+    counts[1] = W.count + LIST_NP2T;
 }
 
 hipError_t launchWellFlux(int NC, const DevEos* eos, const DevBulk& B, FimDevice& F, WellDevice& W, hipStream_t s)

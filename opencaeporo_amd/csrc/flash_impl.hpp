@@ -518,11 +518,20 @@ __device__ __forceinline__ int calFtypeFIM(const FlashCell<NC>& F, const DevBulk
 // regroup cells so that a wavefront runs one kind of work, and each kernel
 // carries only its own working set.  State between stages lives in the
 // FlashWork buffers (SoA, leading dimension ld).
+//
+// Overlapped form (launchFlashT with side streams): the few cells whose
+// stability test or split takes far longer than the rest (hundreds of
+// iterations, Newton restarts) would hold the whole pass.  There the
+// stability test and the split of the cells it finds unstable run on side
+// stream B, the split slices' leftovers and the cells whose first Newton
+// split fails run to the end on side stream C, and the main stream goes on
+// to finish the other cells and to the caller's next kernels.  The side
+// cells end in LIST_NP1T / LIST_NP2T; the caller joins the side streams,
+// finishes those cells (launchFlashTailFinishT) and recomputes what read
+// them.  Every cell still runs the same arithmetic on the same inputs.
 // ---------------------------------------------------------------------------
 
 enum { FL_SKIP = 1, FL_SSMSTA = 2, FL_NRSTA = 4, FL_SSMSP = 8, FL_NRSP = 16, FL_GUIDED = 32 };
-enum { LIST_STAB = 0, LIST_SPLIT = 1, LIST_NP1 = 2, LIST_NP2 = 3, LIST_C1 = 4, LIST_C2 = 5, LIST_NR = 6,
-       LIST_STABG = 7 };
 // Blocks of a work-queue stage: 256 CUs x 4 SIMDs x 2 waves, in blocks of
 // kFlashBlock / 64 waves.
 constexpr unsigned kQueueBlocks = 256 * 4 * 2 / (kFlashBlock / 64);
@@ -535,6 +544,13 @@ constexpr int kSplitBudget0 = 8;
 // iterations in total but a longer tail than Wilson's start, so off).
 constexpr bool kSplitFromGuess = false;
 constexpr int kSplitBudget1 = 24;
+// Overlapped form: the third slice stops after this many more iterations
+// (the warm pass: 0.3% of the two-phase cells need more than 96 in all) and
+// hands the rest to side stream C.
+constexpr int kSplitBudget2 = 64;
+// Side-stream work queues are small: one block per 64 cells would do, this
+// many at most.
+constexpr unsigned kSideBlocks = 256;
 
 // Append the active lanes' cells to list k: one atomic per wavefront.
 __device__ __forceinline__ void appendList(const FlashWork& W, int k, bool pred, int32_t cell)
@@ -557,7 +573,7 @@ __device__ __forceinline__ void appendList(const FlashWork& W, int k, bool pred,
 __device__ __forceinline__ int nextChunk(const FlashWork& W, int k)
 {
     int base = 0;
-    if ((threadIdx.x & 63) == 0) base = atomicAdd(&W.count[8 + k], 64);
+    if ((threadIdx.x & 63) == 0) base = atomicAdd(&W.count[kFlashLists + k], 64);
     return __shfl(base, 0, 64);
 }
 
@@ -578,7 +594,7 @@ __device__ __forceinline__ void addCounter(unsigned long long* dst, unsigned lon
 template <int NC>
 __global__ void flashResetKernel(FlashWork W, DevFlashCounters* ctr)
 {
-    if (threadIdx.x < 16) W.count[threadIdx.x] = 0;
+    if (threadIdx.x < 2 * kFlashLists) W.count[threadIdx.x] = 0;
     if (ctr && threadIdx.x < sizeof(DevFlashCounters) / sizeof(unsigned long long))
         reinterpret_cast<unsigned long long*>(ctr)[threadIdx.x] = 0;
 }
@@ -661,10 +677,12 @@ __device__ __forceinline__ void loadFeed(FlashCell<NC>& F, const DevBulk& B, con
 // followed by LIST_STABG (the cells the learned prior calls unstable), each
 // padded to whole wavefront chunks so a wave works on one list only; the
 // cells of LIST_STABG try the guessed K first.  The plain flash runs the
-// Guided = false instance, which never touches the guess.
+// Guided = false instance, which never touches the guess.  Unstable cells go
+// to list outSplit, stable ones to outNP1.
 template <int NC, bool Guided>
 __global__ void __launch_bounds__(kFlashBlock) flashStabilityKernel(const DevEos* __restrict__ eos, DevBulk B,
-                                                                    FlashWork W, DevFlashCounters* ctr)
+                                                                    FlashWork W, DevFlashCounters* ctr,
+                                                                    int outSplit, int outNP1)
 {
     __shared__ double sSq[FlashCell<NC>::NT * kFlashBlock];
     const int nA = W.count[LIST_STAB];
@@ -716,13 +734,35 @@ __global__ void __launch_bounds__(kFlashBlock) flashStabilityKernel(const DevEos
             B.iters[IT_NRSTA * ld + n] = it1;
         }
     }
-    appendList(W, LIST_NP1, active && stable, n);
-    appendList(W, LIST_SPLIT, active && !stable, n);
+    appendList(W, outNP1, active && stable, n);
+    appendList(W, outSplit, active && !stable, n);
     if (ctr) {
         addCounter(&ctr->iters[IT_SSMSTA], (unsigned long long)it0);
         addCounter(&ctr->iters[IT_NRSTA], (unsigned long long)it1);
         addCounter(&ctr->nonconverged, nonconv ? 1 : 0);
     }
+    }
+}
+
+// The split's starting K (SplitSSM2 with restart = false, src/MixtureComp.cpp:
+// 1858-1870): the previous phase compositions' ratio when the cell had two
+// hydrocarbon phases, else Wilson's.
+template <int NC>
+__device__ __forceinline__ void splitStartK(FlashCell<NC>& F, const DevBulk& B, const FlashWork& W, int64_t n,
+                                            int fl)
+{
+    constexpr int nc = NC + 1;
+    const int64_t ld = B.ld;
+    if (F.lNP == 2) {
+#pragma unroll
+        for (int i = 0; i < NC; ++i) F.Ks[i] = B.xij[(0 * nc + i) * ld + n] / B.xij[(1 * nc + i) * ld + n];
+    } else if (W.guess && kSplitFromGuess && (fl & FL_GUIDED)) {
+#pragma unroll
+        for (int i = 0; i < NC; ++i) F.Ks[i] = exp((double)W.guess[(int64_t)n * W.guessLd + i]);
+    } else {
+        F.calKwilson();
+#pragma unroll
+        for (int i = 0; i < NC; ++i) F.Ks[i] = F.Kw0[i];
     }
 }
 
@@ -739,7 +779,6 @@ __global__ void __launch_bounds__(kFlashBlock) flashSplitSSMKernel(const DevEos*
                                                                    int first)
 {
     __shared__ double sSq[FlashCell<NC>::NT * kFlashBlock];
-    constexpr int nc = NC + 1;
     const int64_t ld = B.ld;
     const int total = W.count[in];
     for (int base = nextChunk(W, in); base < total; base = nextChunk(W, in)) {
@@ -758,18 +797,7 @@ __global__ void __launch_bounds__(kFlashBlock) flashSplitSSMKernel(const DevEos*
         int iter;
         int fl = W.flags[n];
         if (first) {
-            if (F.lNP == 2) {
-#pragma unroll
-                for (int i = 0; i < NC; ++i)
-                    F.Ks[i] = B.xij[(0 * nc + i) * ld + n] / B.xij[(1 * nc + i) * ld + n];
-            } else if (W.guess && kSplitFromGuess && (fl & FL_GUIDED)) {
-#pragma unroll
-                for (int i = 0; i < NC; ++i) F.Ks[i] = exp((double)W.guess[(int64_t)n * W.guessLd + i]);
-            } else {
-                F.calKwilson();
-#pragma unroll
-                for (int i = 0; i < NC; ++i) F.Ks[i] = F.Kw0[i];
-            }
+            splitStartK<NC>(F, B, W, n, fl);
             Se = 1;
             iter = 0;
             F.it[IT_RR] = 0;
@@ -803,13 +831,53 @@ __global__ void __launch_bounds__(kFlashBlock) flashSplitSSMKernel(const DevEos*
     }
 }
 
+// A cell's split is over: its flags, phase count and compositions go to the
+// work buffers, its iteration counts to B.iters (the end of the Newton stage
+// below, shared with the side streams' kernel).
+template <int NC>
+__device__ __forceinline__ void splitDone(FlashCell<NC>& F, const DevBulk& B, const FlashWork& W, int64_t n,
+                                          int fl, int* its, bool& nonconv)
+{
+    const int64_t ld = B.ld;
+    const int NP = F.NP;
+    if (F.ftype == 0) {
+        if (NP > 1) fl &= ~FL_SKIP;
+        nonconv = (NP == 1) ? !(F.nrStaConflag || F.ssmStaConflag) : !(F.nrSpConflag || F.ssmSpConflag);
+    } else {
+        nonconv = !(F.nrSpConflag || F.ssmSpConflag);
+    }
+    W.flags[n] = (uint8_t)fl;
+    if (NP == 2) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            W.nu[j * ld + n] = F.nu[j];
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                W.x[(j * NC + i) * ld + n] = F.x[j][i];
+                W.n[(j * NC + i) * ld + n] = F.n[j][i];
+            }
+        }
+    }
+    its[0] = F.it[IT_SSMSP];
+    its[1] = F.it[IT_NRSP];
+    its[2] = F.it[IT_RR];
+    if (B.iters) {
+        B.iters[IT_SSMSP * ld + n] = its[0];
+        B.iters[IT_NRSP * ld + n] = its[1];
+        B.iters[IT_RR * ld + n] = its[2];
+    }
+}
+
 // The split's Newton stage and its restarts (PhaseSplit after the first
 // SplitSSM2), from the state the last slice left: the compositions and
 // fractions of its last Rachford-Rice solve and the fugacities of those
 // compositions (recomputed: the same arithmetic on the same inputs).
+// deferList >= 0 (overlapped form): a cell whose first Newton split fails
+// leaves for that list (side stream C runs its restarts) with its Newton
+// iteration count; Ks and the other counters are already in the buffers.
 template <int NC>
 __global__ void __launch_bounds__(kFlashBlock) flashSplitNRKernel(const DevEos* __restrict__ eos, DevBulk B,
-                                                                  FlashWork W, DevFlashCounters* ctr)
+                                                                  FlashWork W, DevFlashCounters* ctr, int deferList)
 {
     __shared__ double sSq[FlashCell<NC>::NT * kFlashBlock];
     const int64_t ld = B.ld;
@@ -820,6 +888,7 @@ __global__ void __launch_bounds__(kFlashBlock) flashSplitNRKernel(const DevEos* 
     int NP = 1;
     int its[3] = {0, 0, 0};
     bool nonconv = false;
+    bool deferred = false;
     int32_t n = 0;
     if (active) {
         n = W.list[LIST_NR][idx];
@@ -847,37 +916,101 @@ __global__ void __launch_bounds__(kFlashBlock) flashSplitNRKernel(const DevEos* 
         F.it[IT_NRSP] = 0;
         F.it[IT_RR] = W.rrIt[n];
         F.calFugPhiAll();
-        F.phaseSplitAfterSSM();
-        NP = F.NP;
-        if (F.ftype == 0) {
-            if (NP > 1) fl &= ~FL_SKIP;
-            nonconv = (NP == 1) ? !(F.nrStaConflag || F.ssmStaConflag) : !(F.nrSpConflag || F.ssmSpConflag);
+        F.splitNR();
+        if (deferList >= 0 && !F.nrSpConflag) {
+            deferred = true;
+            W.nrIt[n] = F.it[IT_NRSP];
         } else {
-            nonconv = !(F.nrSpConflag || F.ssmSpConflag);
-        }
-        W.flags[n] = (uint8_t)fl;
-        if (NP == 2) {
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                W.nu[j * ld + n] = F.nu[j];
-#pragma unroll
-                for (int i = 0; i < NC; ++i) {
-                    W.x[(j * NC + i) * ld + n] = F.x[j][i];
-                    W.n[(j * NC + i) * ld + n] = F.n[j][i];
-                }
-            }
-        }
-        its[0] = F.it[IT_SSMSP];
-        its[1] = F.it[IT_NRSP];
-        its[2] = F.it[IT_RR];
-        if (B.iters) {
-            B.iters[IT_SSMSP * ld + n] = its[0];
-            B.iters[IT_NRSP * ld + n] = its[1];
-            B.iters[IT_RR * ld + n] = its[2];
+            F.splitRestarts();
+            NP = F.NP;
+            splitDone<NC>(F, B, W, n, fl, its, nonconv);
         }
     }
-    appendList(W, LIST_NP2, active && NP == 2, n);
-    appendList(W, LIST_NP1, active && NP == 1, n);
+    appendList(W, LIST_NP2, active && !deferred && NP == 2, n);
+    appendList(W, LIST_NP1, active && !deferred && NP == 1, n);
+    if (deferList >= 0) appendList(W, deferList, active && deferred, n);
+    if (ctr) {
+        addCounter(&ctr->iters[IT_SSMSP], (unsigned long long)its[0]);
+        addCounter(&ctr->iters[IT_NRSP], (unsigned long long)its[1]);
+        addCounter(&ctr->iters[IT_RR], (unsigned long long)its[2]);
+        addCounter(&ctr->nonconverged, nonconv ? 1 : 0);
+        addCounter(&ctr->twoPhase, (active && !deferred && NP == 2) ? 1 : 0);
+    }
+    }
+}
+
+// Side streams of the overlapped form: a split run to its end, from
+//   TAIL_FULL    the start (cells the side stability test found unstable),
+//   TAIL_SSM     where the main stream's last slice stopped, or
+//   TAIL_RESTART the failed first Newton split (the restart loop),
+// with the same arithmetic as the slices and the Newton stage.  Finished
+// cells go to LIST_NP1T / LIST_NP2T.
+enum { TAIL_FULL = 0, TAIL_SSM = 1, TAIL_RESTART = 2 };
+
+template <int NC>
+__global__ void __launch_bounds__(kFlashBlock) flashSplitTailKernel(const DevEos* __restrict__ eos, DevBulk B,
+                                                                    FlashWork W, DevFlashCounters* ctr, int in,
+                                                                    int mode)
+{
+    __shared__ double sSq[FlashCell<NC>::NT * kFlashBlock];
+    const int64_t ld = B.ld;
+    const int total = W.count[in];
+    for (int base = nextChunk(W, in); base < total; base = nextChunk(W, in)) {
+    const int idx = base + (threadIdx.x & 63);
+    const bool active = idx < total;
+    int NP = 1;
+    int its[3] = {0, 0, 0};
+    bool nonconv = false;
+    int32_t n = 0;
+    if (active) {
+        n = W.list[in][idx];
+        FlashCell<NC> F(*eos, &sSq[threadIdx.x]);
+        loadFeed<NC>(F, B, W, n);
+        const int fl = W.flags[n];
+        F.ssmStaConflag = fl & FL_SSMSTA;
+        F.nrStaConflag = fl & FL_NRSTA;
+        F.nrSpConflag = false;
+        F.Yt = W.Yt[n];
+        F.NP = 2;
+        F.calAiBi();
+        F.it[IT_SSMSTA] = 0;
+        F.it[IT_NRSTA] = 0;
+        F.it[IT_NRSP] = 0;
+        if (mode == TAIL_RESTART) {
+#pragma unroll
+            for (int i = 0; i < NC; ++i) F.Ks[i] = W.Ks[i * ld + n];
+            F.it[IT_SSMSP] = W.ssmIt[n];
+            F.it[IT_NRSP] = W.nrIt[n];
+            F.it[IT_RR] = W.rrIt[n];
+            F.ssmSpConflag = (fl & FL_SSMSP) != 0;
+            F.splitRestarts();
+        } else {
+            double Se;
+            int iter;
+            if (mode == TAIL_FULL) {
+                splitStartK<NC>(F, B, W, n, fl);
+                Se = 1;
+                iter = 0;
+                F.it[IT_RR] = 0;
+                F.ssmSpConflag = true;
+            } else {
+#pragma unroll
+                for (int i = 0; i < NC; ++i) F.Ks[i] = W.Ks[i * ld + n];
+                Se = W.Se[n];
+                iter = W.ssmIt[n];
+                F.it[IT_RR] = W.rrIt[n];
+                F.ssmSpConflag = (fl & FL_SSMSP) != 0;
+            }
+            F.splitSSM2Slice(Se, iter, -1);
+            F.it[IT_SSMSP] = iter;
+            F.calFugPhiAll();
+            F.phaseSplitAfterSSM();
+        }
+        NP = F.NP;
+        splitDone<NC>(F, B, W, n, fl, its, nonconv);
+    }
+    appendList(W, LIST_NP2T, active && NP == 2, n);
+    appendList(W, LIST_NP1T, active && NP == 1, n);
     if (ctr) {
         addCounter(&ctr->iters[IT_SSMSP], (unsigned long long)its[0]);
         addCounter(&ctr->iters[IT_NRSP], (unsigned long long)its[1]);
@@ -888,15 +1021,15 @@ __global__ void __launch_bounds__(kFlashBlock) flashSplitNRKernel(const DevEos* 
     }
 }
 
-// Stage 4: the rest of FlashFIM / InitFlashFIM for cells ending with NPT
-// hydrocarbon phases.  The EoS state the split ended with (Aj, Bj, Zj of the
-// final compositions) is re-derived from the stored compositions.
+// Stage 4: the rest of FlashFIM / InitFlashFIM for the cells of `list`,
+// which end with NPT hydrocarbon phases.  The EoS state the split ended with
+// (Aj, Bj, Zj of the final compositions) is re-derived from the stored
+// compositions.
 template <int NC, int NPT>
 __global__ void __launch_bounds__(kFlashBlock) flashFinishKernel(const DevEos* __restrict__ eos, DevBulk B,
-                                                                 FlashWork W, int init)
+                                                                 FlashWork W, int init, int list)
 {
     __shared__ double sSq[FlashCell<NC>::NT * kFlashBlock];
-    const int list = (NPT == 1) ? LIST_NP1 : LIST_NP2;
     const int total = W.count[list];
     for (int base = nextChunk(W, list); base < total; base = nextChunk(W, list)) {
     const int idx = base + (threadIdx.x & 63);
@@ -946,38 +1079,105 @@ __global__ void __launch_bounds__(kFlashBlock) flashFinishKernel(const DevEos* _
 
 template <int NC>
 hipError_t launchFlashT(const DevEos* eos, const DevBulk& B, const FlashWork& W, DevFlashCounters* ctr,
-                        const double* zin, hipStream_t s, hipEvent_t* ev)
+                        const double* zin, hipStream_t s, hipEvent_t* ev, const FlashSide* side)
 {
     if (B.nb == 0) return hipSuccess;
     const unsigned grid = (unsigned)((B.nb + kFlashBlock - 1) / kFlashBlock);
     // the list stages run as work queues over a grid that fills the chip
     // (idle waves leave as soon as the queue is empty)
     const unsigned qgrid = std::min<unsigned>(grid, kQueueBlocks);
-    auto mark = [&](int k) {
-        if (ev) (void)hipEventRecord(ev[k], s);
+    const unsigned sgrid = std::min<unsigned>(grid, kSideBlocks);
+    auto mark = [&](int k, hipStream_t st) {
+        if (ev) (void)hipEventRecord(ev[k], st);
     };
+    const int init = zin ? 1 : 0;
+    hipError_t e;
     hipLaunchKernelGGL(flashResetKernel<NC>, dim3(1), dim3(64), 0, s, W, ctr);
     hipLaunchKernelGGL(flashClassifyKernel<NC>, dim3(grid), dim3(kFlashBlock), 0, s, eos, B, W, ctr, zin);
-    mark(0);
-    if (W.guess)
-        hipLaunchKernelGGL((flashStabilityKernel<NC, true>), dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, ctr);
-    else
-        hipLaunchKernelGGL((flashStabilityKernel<NC, false>), dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, ctr);
-    mark(1);
-    mark(2);
+    if (!side) {
+        mark(0, s);
+        if (W.guess)
+            hipLaunchKernelGGL((flashStabilityKernel<NC, true>), dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, ctr,
+                               (int)LIST_SPLIT, (int)LIST_NP1);
+        else
+            hipLaunchKernelGGL((flashStabilityKernel<NC, false>), dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W,
+                               ctr, (int)LIST_SPLIT, (int)LIST_NP1);
+        mark(1, s);
+        mark(2, s);
+        hipLaunchKernelGGL(flashSplitSSMKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W,
+                           (int)LIST_SPLIT, (int)LIST_C1, kSplitBudget0, 1);
+        hipLaunchKernelGGL(flashSplitSSMKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, (int)LIST_C1,
+                           (int)LIST_C2, kSplitBudget1, 0);
+        hipLaunchKernelGGL(flashSplitSSMKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, (int)LIST_C2,
+                           -1, -1, 0);
+        hipLaunchKernelGGL(flashSplitNRKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, ctr, -1);
+        mark(3, s);
+        mark(4, s);
+        hipLaunchKernelGGL((flashFinishKernel<NC, 1>), dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, init,
+                           (int)LIST_NP1);
+        hipLaunchKernelGGL((flashFinishKernel<NC, 2>), dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, init,
+                           (int)LIST_NP2);
+        mark(5, s);
+        return hipGetLastError();
+    }
+    // overlapped form (not with a learned guess, nor for InitFlashFIM)
+    if (W.guess || zin) return hipErrorInvalidValue;
+    if ((e = hipEventRecord(side->classified, s)) != hipSuccess) return e;
+    // side stream B: the stability test and the split of the cells it finds unstable
+    if ((e = hipStreamWaitEvent(side->sB, side->classified, 0)) != hipSuccess) return e;
+    mark(0, side->sB);
+    hipLaunchKernelGGL((flashStabilityKernel<NC, false>), dim3(sgrid), dim3(kFlashBlock), 0, side->sB, eos, B, W,
+                       ctr, (int)LIST_SPLITB, (int)LIST_NP1T);
+    hipLaunchKernelGGL(flashSplitTailKernel<NC>, dim3(sgrid), dim3(kFlashBlock), 0, side->sB, eos, B, W, ctr,
+                       (int)LIST_SPLITB, (int)TAIL_FULL);
+    mark(1, side->sB);
+    if ((e = hipEventRecord(side->doneB, side->sB)) != hipSuccess) return e;
+    // main stream: the slices, the third one bounded
+    mark(2, s);
     hipLaunchKernelGGL(flashSplitSSMKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, (int)LIST_SPLIT,
                        (int)LIST_C1, kSplitBudget0, 1);
     hipLaunchKernelGGL(flashSplitSSMKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, (int)LIST_C1,
                        (int)LIST_C2, kSplitBudget1, 0);
-    hipLaunchKernelGGL(flashSplitSSMKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, (int)LIST_C2, -1,
-                       -1, 0);
-    hipLaunchKernelGGL(flashSplitNRKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, ctr);
-    mark(3);
-    mark(4);
-    const int init = zin ? 1 : 0;
-    hipLaunchKernelGGL((flashFinishKernel<NC, 1>), dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, init);
-    hipLaunchKernelGGL((flashFinishKernel<NC, 2>), dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, init);
-    mark(5);
+    hipLaunchKernelGGL(flashSplitSSMKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, (int)LIST_C2,
+                       (int)LIST_TSSM, kSplitBudget2, 0);
+    if ((e = hipEventRecord(side->sliced, s)) != hipSuccess) return e;
+    // side stream C: the slices' leftovers run to their end
+    if ((e = hipStreamWaitEvent(side->sC, side->sliced, 0)) != hipSuccess) return e;
+    mark(6, side->sC);
+    hipLaunchKernelGGL(flashSplitTailKernel<NC>, dim3(sgrid), dim3(kFlashBlock), 0, side->sC, eos, B, W, ctr,
+                       (int)LIST_TSSM, (int)TAIL_SSM);
+    // main stream: Newton stage; cells that would restart go to stream C
+    hipLaunchKernelGGL(flashSplitNRKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, ctr,
+                       (int)LIST_TNR);
+    mark(3, s);
+    if ((e = hipEventRecord(side->newton, s)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(side->sC, side->newton, 0)) != hipSuccess) return e;
+    hipLaunchKernelGGL(flashSplitTailKernel<NC>, dim3(sgrid), dim3(kFlashBlock), 0, side->sC, eos, B, W, ctr,
+                       (int)LIST_TNR, (int)TAIL_RESTART);
+    mark(7, side->sC);
+    if ((e = hipEventRecord(side->doneC, side->sC)) != hipSuccess) return e;
+    // main stream: finish the cells it holds
+    mark(4, s);
+    hipLaunchKernelGGL((flashFinishKernel<NC, 1>), dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, 0,
+                       (int)LIST_NP1);
+    hipLaunchKernelGGL((flashFinishKernel<NC, 2>), dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, 0,
+                       (int)LIST_NP2);
+    mark(5, s);
+    return hipGetLastError();
+}
+
+// Overlapped form, after the caller has made stream s wait for side->doneB
+// and side->doneC: finish the side streams' cells.
+template <int NC>
+hipError_t launchFlashTailFinishT(const DevEos* eos, const DevBulk& B, const FlashWork& W, hipStream_t s)
+{
+    if (B.nb == 0) return hipSuccess;
+    const unsigned grid = (unsigned)((B.nb + kFlashBlock - 1) / kFlashBlock);
+    const unsigned sgrid = std::min<unsigned>(grid, kSideBlocks);
+    hipLaunchKernelGGL((flashFinishKernel<NC, 1>), dim3(sgrid), dim3(kFlashBlock), 0, s, eos, B, W, 0,
+                       (int)LIST_NP1T);
+    hipLaunchKernelGGL((flashFinishKernel<NC, 2>), dim3(sgrid), dim3(kFlashBlock), 0, s, eos, B, W, 0,
+                       (int)LIST_NP2T);
     return hipGetLastError();
 }
 

@@ -12,11 +12,28 @@ namespace ocpg {
 constexpr int kFlashBlock = 128;
 
 bool flashSupportsNC(int nc);
+
+// Side streams and events of the overlapped flash (launchFlash with side
+// non-NULL).  Streams at the default priority.
+struct FlashSide {
+    hipStream_t sB, sC;
+    hipEvent_t classified, sliced, newton;  // recorded on the main stream
+    hipEvent_t doneB, doneC;                // recorded on sB / sC: the caller waits for both
+};
+
 // FlashFIM over every cell (zin == NULL) or InitFlashFIM from the feed zin
 // (SoA, leading dimension B.ld).  With ev non-NULL, ev[0..5] are recorded
-// around the stability, split and finishing stages (begin, end pairs).
+// around the stability, split and finishing stages (begin, end pairs) and,
+// with side non-NULL, ev[6..7] around side stream C's work.  With side
+// non-NULL the slow cells continue on the side streams (see flash_impl.hpp):
+// the caller makes s wait for side->doneB and side->doneC and then calls
+// launchFlashTailFinish before anything reads those cells.
 hipError_t launchFlash(int NC, const DevEos* eos, const DevBulk& B, const FlashWork& W, DevFlashCounters* ctr,
-                       const double* zin, hipStream_t s, hipEvent_t* ev = nullptr);
+                       const double* zin, hipStream_t s, hipEvent_t* ev = nullptr, const FlashSide* side = nullptr);
+hipError_t launchFlashTailFinish(int NC, const DevEos* eos, const DevBulk& B, const FlashWork& W, hipStream_t s);
+// The side streams' cells: lists[2] (one / two hydrocarbon phases) and the
+// device addresses of their lengths.
+void flashTailListsOf(const FlashWork& W, const int32_t** lists, const int32_t** counts);
 // Well::CalTrans + CalFlux + CalProdWeight and the well rows of CalRes.
 hipError_t launchWellFlux(int NC, const DevEos* eos, const DevBulk& B, FimDevice& F, WellDevice& W, hipStream_t s);
 // The well blocks of AssembleMatWellsNew (after the bulk assembly).
@@ -27,7 +44,9 @@ hipError_t launchWellAssemble(int NC, const DevBulk& B, FimDevice& F, WellDevice
 namespace ocpg {
 template <int NC>
 hipError_t launchFlashT(const DevEos* eos, const DevBulk& B, const FlashWork& W, DevFlashCounters* ctr,
-                        const double* zin, hipStream_t s, hipEvent_t* ev);
+                        const double* zin, hipStream_t s, hipEvent_t* ev, const FlashSide* side);
+template <int NC>
+hipError_t launchFlashTailFinishT(const DevEos* eos, const DevBulk& B, const FlashWork& W, hipStream_t s);
 template <int NC>
 hipError_t launchWellFluxT(const DevEos* eos, const DevBulk& B, FimDevice& F, WellDevice& W, hipStream_t s);
 template <int NC>

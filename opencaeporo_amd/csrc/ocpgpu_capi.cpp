@@ -3,6 +3,7 @@
 // that stand in for IsoT_FIM's property pass and assembly.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -67,9 +68,13 @@ constexpr int kNumFields = sizeof(kFields) / sizeof(kFields[0]);
 size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 // Timer slots of ocpg_timing_get.
-enum TimerSlot { T_FLASH = 0, T_KRPC, T_RES, T_ASM, T_INIT, T_RESTORE, T_STAB, T_SPLIT, T_FINISH, kTimerSlots };
-const char* kTimerNames[kTimerSlots] = {"flash",   "krpc_rock",       "calres",      "assemble",    "init_flash",
-                                        "restore", "flash_stability", "flash_split", "flash_finish"};
+enum TimerSlot {
+    T_FLASH = 0, T_KRPC, T_RES, T_ASM, T_INIT, T_RESTORE, T_STAB, T_SPLIT, T_FINISH, T_TAIL, T_FIXUP, T_UPDASM,
+    kTimerSlots
+};
+const char* kTimerNames[kTimerSlots] = {"flash",        "krpc_rock",       "calres",      "assemble",
+                                        "init_flash",   "restore",         "flash_stability", "flash_split",
+                                        "flash_finish", "flash_tail",      "fixup",       "update_assemble"};
 
 // Fields the flash reads (and overwrites): what one Newton iteration starts from.
 bool isFlashInput(size_t hostOff)
@@ -110,6 +115,10 @@ struct ocpg_ctx {
     // copy of the flash inputs (ocpg_save_flash_inputs)
     char* snap = nullptr;
     size_t snapBytes = 0;
+    // overlapped Newton iteration (ocpg_update_assemble): side streams of the
+    // flash and the rows recomputed after they join
+    FlashSide side{};
+    FixupRows fix{};
     // one Newton iteration captured as a graph (ocpg_capture_iteration)
     hipGraphExec_t iterGraph = nullptr;
     hipEvent_t iterDone = nullptr;
@@ -219,6 +228,43 @@ static void timerClear(ocpg_ctx* c)
     }
 }
 
+namespace ocpg {
+int fimReadNorms(ocpg_ctx* c, ocpg_resnorm* norms);
+}
+
+static void destroySide(ocpg_ctx* c)
+{
+    if (c->side.sB) (void)hipStreamSynchronize(c->side.sB);
+    if (c->side.sC) (void)hipStreamSynchronize(c->side.sC);
+    hipEvent_t evs[] = {c->side.classified, c->side.sliced, c->side.newton, c->side.doneB, c->side.doneC};
+    for (hipEvent_t e : evs)
+        if (e) (void)hipEventDestroy(e);
+    if (c->side.sB) (void)hipStreamDestroy(c->side.sB);
+    if (c->side.sC) (void)hipStreamDestroy(c->side.sC);
+    c->side = FlashSide{};
+}
+
+// The recomputation lists of the overlapped iteration, for the current cells.
+static int ensureFixup(ocpg_ctx* c)
+{
+    const int64_t nb = c->B.nb;
+    if (c->fix.mem && c->fix.nb == nb) return OCPG_OK;
+    if (c->fix.mem) HIPCHK(hipFree(c->fix.mem));
+    c->fix = FixupRows{};
+    const size_t L = (size_t)std::max<int64_t>(nb, 1);
+    const size_t bytes = align256(L * 4) * 3 + 256;
+    HIPCHK(hipMalloc(&c->fix.mem, bytes));
+    char* p = static_cast<char*>(c->fix.mem);
+    c->fix.mark = reinterpret_cast<uint32_t*>(p);
+    c->fix.resRows = reinterpret_cast<int32_t*>(p + align256(L * 4));
+    c->fix.asmRows = reinterpret_cast<int32_t*>(p + 2 * align256(L * 4));
+    c->fix.counts = reinterpret_cast<int32_t*>(p + 3 * align256(L * 4));
+    c->fix.nb = nb;
+    // the marks start (and stay, between calls) all zero
+    HIPCHK(hipMemsetAsync(c->fix.mem, 0, bytes, c->stream));
+    return OCPG_OK;
+}
+
 static bool tablesValid(int rows, int cols, const double* tab)
 {
     if (rows < 1 || rows > OCPG_MAXTAB) return false;
@@ -324,6 +370,15 @@ int ocpg_create(ocpg_ctx** out, const ocpg_eos* eos, const ocpg_satfunc* sat, co
     do {
         if ((rc = hipStatus(hipSetDevice(device)))) break;
         if ((rc = hipStatus(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)))) break;
+        if ((rc = hipStatus(hipStreamCreateWithFlags(&c->side.sB, hipStreamNonBlocking)))) break;
+        if ((rc = hipStatus(hipStreamCreateWithFlags(&c->side.sC, hipStreamNonBlocking)))) break;
+        {
+            hipEvent_t* evs[] = {&c->side.classified, &c->side.sliced, &c->side.newton, &c->side.doneB,
+                                 &c->side.doneC};
+            for (hipEvent_t* e : evs)
+                if ((rc = hipStatus(hipEventCreateWithFlags(e, hipEventDisableTiming)))) break;
+            if (rc) break;
+        }
         DevEos de;
         buildDevEos(*eos, de);
         if ((rc = hipStatus(hipMalloc(&c->dEos, sizeof(DevEos))))) break;
@@ -349,6 +404,7 @@ int ocpg_create(ocpg_ctx** out, const ocpg_eos* eos, const ocpg_satfunc* sat, co
         if (c->dEos) (void)hipFree(c->dEos);
         if (c->dCtr) (void)hipFree(c->dCtr);
         if (c->dSat) (void)hipFree(c->dSat);
+        destroySide(c);
         if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return rc;
@@ -374,6 +430,8 @@ int ocpg_destroy(ocpg_ctx* c)
     if (c->dEos) (void)hipFree(c->dEos);
     if (c->dCtr) (void)hipFree(c->dCtr);
     if (c->dSat) (void)hipFree(c->dSat);
+    if (c->fix.mem) (void)hipFree(c->fix.mem);
+    destroySide(c);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return OCPG_OK;
@@ -407,15 +465,15 @@ int ocpg_set_cells(ocpg_ctx* c, int64_t nb)
     HIPCHK(hipMalloc(&c->last, c->lastBytes));
     {
         // FlashWork: doubles zi(NC), Nh, Yt, x(2NC), n(2NC), nu(2), Ks(NC),
-        // Se; int32 ssmIt, rrIt; bytes ftype, lNP, flags; the int32 lists;
-        // the counters.
+        // Se; int32 ssmIt, rrIt, nrIt; bytes ftype, lNP, flags; the int32
+        // lists; the counters.
         const size_t L = (size_t)(ld > 0 ? ld : 64);
         const int NC = c->NC;
         const size_t dbl[8] = {(size_t)NC, 1, 1, (size_t)2 * NC, (size_t)2 * NC, 2, (size_t)NC, 1};
         size_t w = 0;
-        size_t offD[8], offI[2], offB[3], offL[kFlashLists], offC;
+        size_t offD[8], offI[3], offB[3], offL[kFlashLists], offC;
         for (int q = 0; q < 8; q++) { offD[q] = w; w += align256(dbl[q] * L * sizeof(double)); }
-        for (int q = 0; q < 2; q++) { offI[q] = w; w += align256(L * sizeof(int32_t)); }
+        for (int q = 0; q < 3; q++) { offI[q] = w; w += align256(L * sizeof(int32_t)); }
         for (int q = 0; q < 3; q++) { offB[q] = w; w += align256(L); }
         for (int q = 0; q < kFlashLists; q++) { offL[q] = w; w += align256(L * sizeof(int32_t)); }
         offC = w;
@@ -434,6 +492,7 @@ int ocpg_set_cells(ocpg_ctx* c, int64_t nb)
         c->work.Se = reinterpret_cast<double*>(b + offD[7]);
         c->work.ssmIt = reinterpret_cast<int32_t*>(b + offI[0]);
         c->work.rrIt = reinterpret_cast<int32_t*>(b + offI[1]);
+        c->work.nrIt = reinterpret_cast<int32_t*>(b + offI[2]);
         c->work.ftype = reinterpret_cast<int8_t*>(b + offB[0]);
         c->work.lNP = reinterpret_cast<int8_t*>(b + offB[1]);
         c->work.flags = reinterpret_cast<uint8_t*>(b + offB[2]);
@@ -522,7 +581,7 @@ int ocpg_flash_fim(ocpg_ctx* c, ocpg_flash_stats* stats)
 {
     if (!c || !c->cur) return OCPG_ESTATE;
     HIPCHK(hipSetDevice(c->device));
-    hipEvent_t ev[6] = {};
+    hipEvent_t ev[8] = {};
     bool stageEv = c->timing;
     for (int k = 0; k < 6 && stageEv; k++) stageEv = hipEventCreate(&ev[k]) == hipSuccess;
     timerBegin(c, T_FLASH);
@@ -564,6 +623,83 @@ int ocpg_flash_fim_guided(ocpg_ctx* c, const float* guess, int ld, ocpg_flash_st
     c->work.guess = nullptr;
     c->work.guessLd = 0;
     return rc;
+}
+
+// IsoT_FIM::UpdateProperty (CalFlash, CalKrPc, CalRock, CalRes;
+// src/OCPFluidMethod.cpp:877-897) followed by AssembleMat (:860-875): the
+// device work of one Newton iteration, with the flash's slowest cells
+// overlapped with the rest.  The stability test and the splits that run far
+// longer than the others continue on two side streams while the main stream
+// finishes the other cells, runs kr/Pc, rock, CalRes and the assembly on
+// every row; after the side streams join, their cells are finished and the
+// rows that read them (those cells, their neighbours and, with wells, the
+// perforated cells) are computed again.  Every row ends as the serial calls
+// leave it.  Asynchronous unless norms or stats is non-NULL.
+int ocpg_update_assemble(ocpg_ctx* c, double dt, ocpg_resnorm* norms, ocpg_flash_stats* stats)
+{
+    if (!c || !c->cur) return OCPG_ESTATE;
+    FimDevice& F = c->fim;
+    if (F.nb <= 0 || F.nb != c->B.nb) return OCPG_ESTATE;
+    if (!c->dSat) return OCPG_ESTATE;
+    HIPCHK(hipSetDevice(c->device));
+    int rc = ensureFixup(c);
+    if (rc != OCPG_OK) return rc;
+    WellDevice& W = c->wells;
+    hipEvent_t ev[8] = {};
+    bool stageEv = c->timing;
+    for (int k = 0; k < 8 && stageEv; k++) stageEv = hipEventCreate(&ev[k]) == hipSuccess;
+    timerBegin(c, T_UPDASM);
+    timerBegin(c, T_FLASH);
+    HIPCHK(launchFlash(c->NC, c->dEos, c->B, c->work, stats ? c->dCtr : nullptr, nullptr, c->stream,
+                       stageEv ? ev : nullptr, &c->side));
+    timerEnd(c, T_FLASH);
+    timerBegin(c, T_KRPC);
+    HIPCHK(launchKrPcRock(c->B, c->dSat, c->rock, true, true, c->stream));
+    timerEnd(c, T_KRPC);
+    timerBegin(c, T_RES);
+    HIPCHK(launchCalRes(c->NC, c->B, F, W, dt, c->stream, false));
+    timerEnd(c, T_RES);
+    timerBegin(c, T_ASM);
+    HIPCHK(launchAssemble(c->NC, c->B, F, dt, c->stream));
+    timerEnd(c, T_ASM);
+    // join the side streams, finish their cells, recompute what read them
+    HIPCHK(hipStreamWaitEvent(c->stream, c->side.doneB, 0));
+    HIPCHK(hipStreamWaitEvent(c->stream, c->side.doneC, 0));
+    timerBegin(c, T_FIXUP);
+    HIPCHK(launchFlashTailFinish(c->NC, c->dEos, c->B, c->work, c->stream));
+    const int32_t* lists[2];
+    const int32_t* counts[2];
+    flashTailListsOf(c->work, lists, counts);
+    HIPCHK(launchKrPcRockList(c->B, c->dSat, c->rock, lists, counts, c->stream));
+    HIPCHK(launchFixupRows(F, W, c->B.nb, lists, counts, c->fix, c->stream));
+    if (W.nw > 0) HIPCHK(launchWellFlux(c->NC, c->dEos, c->B, F, W, c->stream));
+    HIPCHK(launchCalResRows(c->NC, c->B, F, W, dt, c->fix.resRows, c->fix.counts + 0, c->stream));
+    HIPCHK(launchAssembleRows(c->NC, c->B, F, dt, c->fix.asmRows, c->fix.counts + 1, c->stream));
+    if (W.nw > 0) HIPCHK(launchWellAssemble(c->NC, c->B, F, W, dt, c->stream));
+    HIPCHK(launchResNorm(c->NC, c->B, F, c->stream));
+    timerEnd(c, T_FIXUP);
+    timerEnd(c, T_UPDASM);
+    if (stageEv) {
+        c->events[T_STAB].push_back({ev[0], ev[1]});
+        c->events[T_SPLIT].push_back({ev[2], ev[3]});
+        c->events[T_FINISH].push_back({ev[4], ev[5]});
+        c->events[T_TAIL].push_back({ev[6], ev[7]});
+    } else {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+    if (norms && (rc = fimReadNorms(c, norms)) != OCPG_OK) return rc;
+    if (stats) {
+        DevFlashCounters h;
+        HIPCHK(hipMemcpyAsync(&h, c->dCtr, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        stats->cells = (int64_t)h.cells;
+        for (int q = 0; q < 3; q++) stats->ftype_count[q] = (int64_t)h.ftype[q];
+        stats->two_phase = (int64_t)h.twoPhase;
+        for (int q = 0; q < 5; q++) stats->iters[q] = (int64_t)h.iters[q];
+        stats->nonconverged = (int64_t)h.nonconverged;
+    }
+    return OCPG_OK;
 }
 
 // IsoT_FIM::InitFlash (src/OCPFluidMethod.cpp:1095-1106): InitFlashFIM from

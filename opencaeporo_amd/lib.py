@@ -65,6 +65,7 @@ def load():
         "ocpg_calkrpc_rock": ([P], C.c_int),
         "ocpg_calres": ([P, C.c_double, P], C.c_int),
         "ocpg_assemble": ([P, C.c_double], C.c_int),
+        "ocpg_update_assemble": ([P, C.c_double, P, P], C.c_int),
         "ocpg_get_bsr": ([P, P, P, P, P], C.c_int),
         "ocpg_get_upwind": ([P, P, P, P], C.c_int),
         "ocpg_save_flash_inputs": ([P], C.c_int),
@@ -172,6 +173,16 @@ class Context:
 
     def assemble(self, dt):
         check(self.lib.ocpg_assemble(self.h, dt), "ocpg_assemble")
+
+    def update_assemble(self, dt, norms=False, stats=False):
+        """flash_fim + calkrpc_rock + calres + assemble in one call, the
+        flash's slowest cells overlapped with the rest (ocpg_update_assemble).
+        Returns (norms or None, stats or None)."""
+        nr = abi.ResNorm() if norms else None
+        st = abi.FlashStats() if stats else None
+        check(self.lib.ocpg_update_assemble(self.h, dt, C.byref(nr) if nr else None,
+                                            C.byref(st) if st else None), "ocpg_update_assemble")
+        return nr, st
 
     def rows(self):
         return int(self.lib.ocpg_bsr_rows(self.h))
