@@ -32,17 +32,19 @@ sys.path.insert(0, ROOT)
 from opencaeporo_amd import abi, fluids, lib, workloads  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
-FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 DT = 5.0
 # Algorithmic FP64 work of the warm flash pass per cell (add, sub, mul, div,
 # sqrt; transcendental calls counted apart), measured with the oracle's
 # counting build (oracle/liboracle_count.so, FlopCounter.count_flash_stages)
 # on this exact workload, all 1,122,000 cells: 21,584,690,202 flops and
-# 542,748,484 transcendentals in total, of which PhaseSplit 13,922,832,587
-# flops and 292,374,158 transcendentals.
+# 542,748,484 transcendentals in total.
 WARM_FLOPS_PER_CELL = 21584690202 / 1122000
-WARM_SPLIT_FLOPS_PER_CELL = 13922832587 / 1122000
-WARM_SPLIT_TRANS_PER_CELL = 292374158 / 1122000
+# HBM bytes per launch of the assembly and pack kernels on the default
+# workload: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, scaled by what the
+# counters read for a device-to-device copy of known size
+# (profiles/assembly_pack_1p1M_6comp_traffic.md)
+ASM_TRAFFIC_BYTES = 5.089e9 + 3.983e9
+PACK_TRAFFIC_BYTES = 2.378e9 + 2.165e9
 
 
 def log(*a):
@@ -198,38 +200,57 @@ def main():
     t1 = time.perf_counter()
     ms_step = (t1 - t0) * 1e3 / a.steps
     kern = {k: ctx.timer(k) for k in ("restore", "flash", "flash_stability", "flash_split", "flash_finish",
-                                      "flash_tail", "krpc_rock", "calres", "assemble", "fixup",
-                                      "update_assemble")}
+                                      "flash_tail", "flash_restarts", "krpc_rock", "calres", "pack", "assemble",
+                                      "fixup", "update_assemble")}
     ctx.timing(False)
     per = {k: (v[0] / max(v[1], 1)) for k, v in kern.items()}
     log("[bench] per-launch ms: " + ", ".join(f"{k} {v:.3f}" for k, v in per.items()))
     value = nb / (ms_step / 1e3)
     nnzb = ctx.nnzb()
-    ncol = NC + 2
-    # roofline of the assembly kernel (HBM-bound): algorithmic bytes per launch
-    c_bar = grid.nconn / nb
-    asm_bytes = nnzb * ncol * ncol * 8 + nnzb * 4 * 3 + (nb + 1) * 4 + nb * 8 * (
-        1 + 3 * 3 + 2 * 3 * (NC + 1) + 2 * 9 + 2 + 3 + (3 + 2 * NC) * ncol) * 1.0 + grid.nconn * 3 * 12
+    nc, ncol = NC + 1, NC + 2
+    # compact dSec_dPri rows per cell (3 + 2 NC with two hydrocarbon phases,
+    # 2 + NC with one) from the last iteration's state
+    rb = abi.Bulk(nb, NC, fields=["bRowSizedSdP"])
+    ctx.download(rb)
+    rows_mean = float(rb.bRowSizedSdP.mean())
+    # Algorithmic bytes of one launch of the assembly kernel (every value it
+    # needs read once, every block written once): the BSR blocks, the
+    # pattern (JA, connection and mirror slot per block; IA, bulkEnd per
+    # row), each cell's connection-side record (30 + 12 nc doubles: Pj, xi,
+    # kr, mu, muP, xiP, rhoP per phase, dKr_dS, xij, rhox, xix, mux, depth;
+    # three phase flags), its own-row record (dSec_dPri rows x ncol, dPcj_dS,
+    # vfi, vfP, v, poroP, 7 counts) and per connection Akd and its upwinding
+    # (3 cells, 3 densities).
+    rec_bytes = (30 + 12 * nc + 1) * 8 + 3
+    own_bytes = (rows_mean * ncol + 9 + nc + 3) * 8 + 7 * 4
+    asm_bytes = (nnzb * ncol * ncol * 8 + nnzb * 12 + nb * 8 + nb * (rec_bytes + own_bytes)
+                 + grid.nconn * (8 + 3 * 4 + 3 * 8))
+    # the pack kernel reads the same cell values from the structure-of-arrays
+    # state and writes them as per-cell records (doubles)
+    pack_bytes = nb * (rec_bytes + own_bytes) + nb * ((30 + 12 * nc + 4) + rows_mean * ncol + 9 + nc + 10) * 8
     flash_ms = per["flash"]
-    launches = {k: v for k, v in per.items() if k not in ("flash", "update_assemble")}
+    launches = {k: v for k, v in per.items() if k not in ("flash", "update_assemble", "flash_stability",
+                                                          "flash_split", "flash_finish", "flash_tail",
+                                                          "flash_restarts")}
     dominant = max(launches, key=launches.get)
-    # roofline of the dominant kernel: the split (FP64-bound; vector and
-    # matrix FP64 peaks coincide on MI355X) or the assembly (HBM-bound)
-    split_flops = WARM_SPLIT_FLOPS_PER_CELL * nb
-    roof_split = dict(kernel="flashSplitKernel", bound="mfma", note="FP64 VALU (no MFMA): vector and matrix "
-                      "FP64 peaks coincide on MI355X; flops exclude the transcendental calls",
-                      achieved=split_flops / (per["flash_split"] / 1e3) / 1e12, peak=FP64_PEAK_TFLOPS,
-                      unit="TFLOP/s", traffic=None, flops_per_launch=split_flops,
-                      transcendentals_per_launch=WARM_SPLIT_TRANS_PER_CELL * nb)
-    roof_asm = dict(kernel="assembleKernel", bound="hbm", achieved=asm_bytes / (per["assemble"] / 1e3) / 1e9,
-                    peak=HBM_PEAK_GBS, unit="GB/s", traffic=None, bytes_per_launch=asm_bytes)
-    for r in (roof_split, roof_asm):
+    # The dominant kernel: the assembly, the longest launch on the main
+    # stream (the flash stages are several launches each, and the side
+    # streams' kernels run beside the main stream).  Its bound by algorithmic
+    # work is HBM; it reaches a fraction of it because it issues ~2400 VALU
+    # instructions per row (rocprofv3 SQ_INSTS_VALU).
+    roof = dict(kernel="assembleKernel", bound="hbm", achieved=asm_bytes / (per["assemble"] / 1e3) / 1e9,
+                peak=HBM_PEAK_GBS, unit="GB/s", traffic=None, bytes_per_launch=asm_bytes)
+    roof_pack = dict(kernel="packRecordsKernel", bound="hbm", achieved=pack_bytes / (per["pack"] / 1e3) / 1e9,
+                     peak=HBM_PEAK_GBS, unit="GB/s", traffic=None, bytes_per_launch=pack_bytes)
+    for r in (roof, roof_pack):
         r["frac"] = r["achieved"] / r["peak"]
-    roof = roof_split if per["flash_split"] >= per["assemble"] else roof_asm
-    other = roof_asm if roof is roof_split else roof_split
-    if a.irregular:
-        # the flop tally was measured on the 6-component workload only
-        roof, other = roof_asm, None
+    if not a.irregular and nz == 85:
+        # HBM bytes per launch measured by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+        # on this workload, scaled by the copy-kernel calibration
+        # (profiles/assembly_pack_1p1M_6comp_traffic.md)
+        roof["traffic"] = ASM_TRAFFIC_BYTES
+        roof_pack["traffic"] = PACK_TRAFFIC_BYTES
+    other = [roof_pack]
     cpu = None
     if rank == 0 and not a.no_cpu and not a.irregular:
         try:
@@ -261,7 +282,7 @@ def main():
                    "cells": nb, "connections": grid.nconn, "block": ncol,
                    "kernel_ms": per, "dominant_kernel": dominant, "flash_ms": flash_ms,
                    "flash_flops_per_cell": WARM_FLOPS_PER_CELL, "asm_bytes_per_launch": asm_bytes,
-                   "c_bar": c_bar, "roofline_other": other},
+                   "c_bar": grid.nconn / nb, "dsec_rows_mean": rows_mean, "roofline_other": other},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

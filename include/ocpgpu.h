@@ -304,6 +304,29 @@ int ocpg_assemble(ocpg_ctx* ctx, double dt);
  * results to the four calls.  norms and stats may be NULL (then the call is
  * asynchronous).  Needs ocpg_set_conn and the kr/Pc tables.               */
 int ocpg_update_assemble(ocpg_ctx* ctx, double dt, ocpg_resnorm* norms, ocpg_flash_stats* stats);
+/* IsoT_FIM::GetSolution (src/OCPFluidMethod.cpp:2735-2857) on the resident
+ * state: the Newton update of P, Ni, S and x_ij with the saturation chop
+ * (dSmaxlim: OCPControl's NRdSmax, src/OCPControl.cpp:37) and of the open
+ * wells' BHP, from the linear system's solution u (rows x ncol, bulk rows
+ * then open wells; host memory, or device memory with u_on_device = 1).
+ * out (may be NULL: then asynchronous) gets Bulk::NRdPmax / NRdNmax.       */
+typedef struct {
+    double NRdPmax;  /* the first cell's dP of largest magnitude             */
+    double NRdNmax;  /* the first dNi / Nt of largest magnitude              */
+} ocpg_nr_update;
+int ocpg_get_solution(ocpg_ctx* ctx, const double* u, int u_on_device, double dSmaxlim, ocpg_nr_update* out);
+/* After the flash that follows ocpg_get_solution: IsoT_FIM::PassFlashValue's
+ * dSNR / maxNRdSSP (src/OCPFluidMethod.cpp:1135-1146) and Bulk::CalNRdSmax
+ * (src/Bulk.cpp:1235-1247), which IsoT_FIM::FinishNR (:937-960) reads.      */
+typedef struct {
+    double NRdSmax;          /* signed saturation change of largest magnitude */
+    int64_t NRdSmax_cell;
+    double maxNRdSSP;        /* largest flash-vs-Newton saturation difference */
+    int64_t maxNRdSSP_cell;
+} ocpg_nr_change_t;
+int ocpg_nr_change(ocpg_ctx* ctx, ocpg_nr_change_t* out);
+/* Well::BHP of every well as held on the device (nw doubles).              */
+int ocpg_get_well_bhp(ocpg_ctx* ctx, double* bhp);
 /* The assembled system as VectorFaspSolver::AssembleMat copies it into a
  * dBSRmat (src/FaspSolver.cpp:362-429): IA, JA, val (row-major blocks), rhs.
  * Any pointer may be NULL.                                                    */
@@ -374,8 +397,10 @@ int ocpg_restore_flash_inputs(ocpg_ctx* ctx);
 /* Per-kernel HIP-event timers on the context's stream.  Names: "flash",
  * "krpc_rock", "calres", "assemble", "init_flash", "restore", the flash's
  * stages "flash_stability", "flash_split", "flash_finish", and for
- * ocpg_update_assemble "flash_tail" (side stream C), "fixup" and
- * "update_assemble" (the whole call).  No reference
+ * ocpg_update_assemble "flash_tail" (side stream C), "flash_restarts" (side
+ * stream D), "fixup" and "update_assemble" (the whole call); "pack" (the
+ * per-cell records the assembly reads; "assemble" is the assembly kernel
+ * alone, with the wells' blocks).  No reference
  * counterpart (the reference times whole phases with GetWallTime,
  * include/UtilTiming.hpp:31-62).                                             */
 int ocpg_timing_enable(ocpg_ctx* ctx, int on);

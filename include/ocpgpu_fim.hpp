@@ -8,6 +8,9 @@
  *   AssembleMat    -> IsoT_FIM::AssembleMat    (bulk + well blocks, rhs)
  *   UpdateProperty -> IsoT_FIM::UpdateProperty (CalFlash, CalKrPc, CalRock,
  *                                               well CalTrans/CalFlux, CalRes)
+ *   UpdatePropertyAssemble -> both of the above, overlapped on the device
+ *   GetSolution    -> IsoT_FIM::GetSolution    (Newton update of the state)
+ *   NRChange       -> the dSNR / NRdSmax / maxNRdSSP that FinishNR reads
  *   UpdateLastTimeStep / ResetToLastTimeStep
  *
  * The caller keeps its Bulk / BulkConn / Well arrays in the reference's own
@@ -107,6 +110,36 @@ public:
 
     /* IsoT_FIM::AssembleMat (bulk and well blocks, rhs = -residual). */
     void AssembleMat(double dt) { check(ocpg_assemble(ctx_, dt), "ocpg_assemble"); }
+
+    /* UpdateProperty followed by AssembleMat in one call: the same results,
+     * with the flash's slowest cells overlapped with the other cells'
+     * properties, residual and Jacobian.  Returns the norms. */
+    ocpg_resnorm UpdatePropertyAssemble(double dt)
+    {
+        ocpg_resnorm r{};
+        check(ocpg_update_assemble(ctx_, dt, &r, nullptr), "ocpg_update_assemble");
+        return r;
+    }
+
+    /* IsoT_FIM::GetSolution on the device state from the linear solver's
+     * solution u (LinearSystem's layout: bulk rows, then open wells; host
+     * memory); dSmaxlim is OCPControl's NRdSmax.  Updates P, Ni, S, x_ij and
+     * the wells' BHP; returns Bulk::NRdPmax / NRdNmax. */
+    ocpg_nr_update GetSolution(const double* u, double dSmaxlim)
+    {
+        ocpg_nr_update r{};
+        check(ocpg_get_solution(ctx_, u, 0, dSmaxlim, &r), "ocpg_get_solution");
+        return r;
+    }
+
+    /* After the UpdateProperty that follows GetSolution: Bulk::CalNRdSmax and
+     * PassFlashValue's maxNRdSSP, as IsoT_FIM::FinishNR reads them. */
+    ocpg_nr_change_t NRChange()
+    {
+        ocpg_nr_change_t r{};
+        check(ocpg_nr_change(ctx_, &r), "ocpg_nr_change");
+        return r;
+    }
 
     /* The system as LinearSystem::AssembleMatLinearSolver hands it to FASP. */
     void GetSystem(BsrSystem& s) const

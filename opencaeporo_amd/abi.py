@@ -120,6 +120,15 @@ class ResNorm(C.Structure):
                 ("maxId_V", C.c_int64), ("maxId_N", C.c_int64), ("maxWellRelRes_mol", C.c_double)]
 
 
+class NRUpdate(C.Structure):
+    _fields_ = [("NRdPmax", C.c_double), ("NRdNmax", C.c_double)]
+
+
+class NRChange(C.Structure):
+    _fields_ = [("NRdSmax", C.c_double), ("NRdSmax_cell", C.c_int64), ("maxNRdSSP", C.c_double),
+                ("maxNRdSSP_cell", C.c_int64)]
+
+
 WELL_INJ, WELL_PROD = 0, 1
 WELL_BHP, WELL_RATE = 0, 1
 

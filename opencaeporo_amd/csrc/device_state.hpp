@@ -70,7 +70,7 @@ struct DevBulk {
 // Scratch of the staged flash (cells in SoA with the bulk's ld; lists of
 // cell indices, their lengths in count[0, kFlashLists) and their work-queue
 // cursors in count[kFlashLists, 2 * kFlashLists)).
-constexpr int kFlashLists = 13;
+constexpr int kFlashLists = 17;
 
 // The staged flash's cell lists.
 enum {
@@ -84,11 +84,42 @@ enum {
     LIST_STABG = 7,    // stability test trying a learned guess first
     LIST_SPLITB = 8,   // side stream B: unstable cells of its stability test
     LIST_TSSM = 9,     // side stream C: split slices' leftovers
-    LIST_TNR = 10,     // side stream C: cells whose first Newton split failed
-    LIST_NP1T = 11,    // side cells finished with one hydrocarbon phase
-    LIST_NP2T = 12,    // ... with two
+    LIST_TNR = 10,     // side stream D: cells whose first Newton split failed
+    LIST_NP1B = 11,    // side stream B's cells with one hydrocarbon phase
+    LIST_NP2B = 12,    // ... with two
+    LIST_NP1C = 13,    // side stream C's
+    LIST_NP2C = 14,
+    LIST_NP1D = 15,    // side stream D's
+    LIST_NP2D = 16,
 };
-static_assert(LIST_NP2T < kFlashLists, "FlashWork holds too few lists");
+static_assert(LIST_NP2D < kFlashLists, "FlashWork holds too few lists");
+
+// The side streams' finished cells (device lists and the device addresses of
+// their lengths), for the rows recomputed after the side streams join.
+constexpr int kSideLists = 6;
+struct CellLists {
+    const int32_t* list[kSideLists];
+    const int32_t* count[kSideLists];
+};
+
+// Cell i of the concatenated lists (i < cellTotal).
+__device__ __forceinline__ int64_t cellTotal(const CellLists& L)
+{
+    int64_t t = 0;
+#pragma unroll
+    for (int q = 0; q < kSideLists; ++q) t += *L.count[q];
+    return t;
+}
+__device__ __forceinline__ int32_t cellAt(const CellLists& L, int64_t i)
+{
+#pragma unroll
+    for (int q = 0; q < kSideLists; ++q) {
+        const int64_t c = *L.count[q];
+        if (i < c) return L.list[q][i];
+        i -= c;
+    }
+    return 0;
+}
 
 struct FlashWork {
     double* zi;      // NC per cell: feed mole fractions

@@ -28,10 +28,9 @@ const DevRock& ctxRock(ocpg_ctx* c);
 void ctxTimerBegin(ocpg_ctx* c, int slot);
 void ctxTimerEnd(ocpg_ctx* c, int slot);
 int fimReadNorms(ocpg_ctx* c, ocpg_resnorm* norms);
+NewtonDevice& ctxNewton(ocpg_ctx* c);
 }  // namespace ocpg
 
-// timer slots of ocpgpu_capi.cpp
-enum { T_FLASH = 0, T_KRPC, T_RES, T_ASM };
 
 using namespace ocpg;
 
@@ -343,6 +342,8 @@ int ocpg_set_conn(ocpg_ctx* c, const ocpg_conn* conn)
     HIPCHK2(devAlloc(F.upblock, nconn * 3, tot));
     HIPCHK2(devAlloc(F.upRho, nconn * 3, tot));
     HIPCHK2(devAlloc(F.upVel, nconn * 3, tot));
+    HIPCHK2(devAlloc(F.rec, (size_t)nb * recordDoubles(ctxNC(c)), tot));
+    HIPCHK2(devAlloc(F.own, (size_t)nb * ownDoubles(ctxNC(c)), tot));
     HIPCHK2(devAlloc(F.normPart, (size_t)F.normBlocks * 2, tot));
     HIPCHK2(devAlloc(F.normIdx, (size_t)F.normBlocks * 2, tot));
     F.bytes = tot;
@@ -497,6 +498,133 @@ int ocpg_calres(ocpg_ctx* c, double dt, ocpg_resnorm* norms)
     return OCPG_OK;
 }
 
+// The device arrays of GetSolution for the current cells (allocated on
+// first use, kept while the cell count stays).
+static int ensureNewton(ocpg_ctx* c)
+{
+    NewtonDevice& N = ctxNewton(c);
+    const DevBulk& B = ctxBulk(c);
+    if (N.mem && N.nb == B.nb && N.ld == B.ld) return OCPG_OK;
+    if (N.mem) HIPCHK2(hipFree(N.mem));
+    N.mem = nullptr;
+    N.nb = B.nb;
+    N.ld = B.ld;
+    N.blocks = (int)std::max<int64_t>((B.nb + 255) / 256, 1);
+    N.valid = false;
+    const size_t L = (size_t)std::max<int64_t>(B.ld, 64);
+    const size_t bD = align256w(3 * L * sizeof(double)), bB = align256w(3 * L);
+    const size_t bP = align256w((size_t)N.blocks * 4 * sizeof(double)), bI = align256w((size_t)N.blocks * 2 * 8);
+    HIPCHK2(hipMalloc(&N.mem, 2 * bD + bB + bP + bI));
+    char* p = static_cast<char*>(N.mem);
+    N.dSNR = reinterpret_cast<double*>(p);
+    N.dSNRP = reinterpret_cast<double*>(p + bD);
+    N.pexOld = reinterpret_cast<uint8_t*>(p + 2 * bD);
+    N.part = reinterpret_cast<double*>(p + 2 * bD + bB);
+    N.partIdx = reinterpret_cast<int64_t*>(p + 2 * bD + bB + bP);
+    return OCPG_OK;
+}
+
+// Host reduction of the blocks' first maxima (slot 0 or 1): the value at
+// the largest magnitude, lowest index on ties; (0, 0) when none is nonzero.
+static int firstMaxOverBlocks(ocpg_ctx* c, int slot, double* val, int64_t* idx)
+{
+    NewtonDevice& N = ctxNewton(c);
+    std::vector<double> part((size_t)N.blocks * 4);
+    std::vector<int64_t> pidx((size_t)N.blocks * 2);
+    HIPCHK2(hipMemcpyAsync(part.data(), N.part, part.size() * sizeof(double), hipMemcpyDeviceToHost, ctxStream(c)));
+    HIPCHK2(hipMemcpyAsync(pidx.data(), N.partIdx, pidx.size() * sizeof(int64_t), hipMemcpyDeviceToHost,
+                           ctxStream(c)));
+    HIPCHK2(hipStreamSynchronize(ctxStream(c)));
+    double bv = 0, bk = 0;
+    int64_t bi = 0;
+    bool any = false;
+    for (int b = 0; b < N.blocks; b++) {
+        const double v = part[4 * b + 2 * slot], k = part[4 * b + 2 * slot + 1];
+        const int64_t i = pidx[2 * b + slot];
+        if (k > bk || (any && k == bk && k > 0 && i < bi)) {
+            bv = v;
+            bk = k;
+            bi = i;
+            any = true;
+        }
+    }
+    *val = bv;
+    if (idx) *idx = any ? bi : 0;
+    return OCPG_OK;
+}
+
+// IsoT_FIM::GetSolution (src/OCPFluidMethod.cpp:2735-2857) on the resident
+// state: u is the linear system's solution, rows x ncol in the LinearSystem
+// layout (bulk rows, then open wells), in host memory or (u_on_device)
+// device memory.  Updates P, Ni, S, x_ij and the open wells' BHP; keeps the
+// saturations before the update and the predicted changes for
+// ocpg_nr_change.  out may be NULL (then the call is asynchronous).
+int ocpg_get_solution(ocpg_ctx* c, const double* u, int u_on_device, double dSmaxlim, ocpg_nr_update* out)
+{
+    if (!c || !u || !(dSmaxlim > 0)) return OCPG_EINVAL;
+    const DevBulk& B = ctxBulk(c);
+    FimDevice& F = ctxFim(c);
+    if (B.nb <= 0 || F.nb != B.nb) return OCPG_ESTATE;
+    HIPCHK2(hipSetDevice(ctxDevice(c)));
+    int rc = ensureNewton(c);
+    if (rc != OCPG_OK) return rc;
+    NewtonDevice& N = ctxNewton(c);
+    WellDevice& W = ctxWells(c);
+    const int64_t ncol = ctxNC(c) + 2;
+    const int64_t len = (B.nb + W.nwOpen) * ncol;
+    const double* du = u;
+    if (!u_on_device) {
+        if (N.uLen < len) {
+            if (N.u) HIPCHK2(hipFree(N.u));
+            N.u = nullptr;
+            N.uLen = 0;
+            HIPCHK2(hipMalloc(&N.u, (size_t)len * sizeof(double)));
+            N.uLen = len;
+        }
+        HIPCHK2(hipMemcpyAsync(N.u, u, (size_t)len * sizeof(double), hipMemcpyHostToDevice, ctxStream(c)));
+        du = N.u;
+    }
+    HIPCHK2(launchGetSolution(ctxNC(c), B, N, W, B.nb, du, dSmaxlim, ctxStream(c)));
+    N.valid = true;
+    if (out) {
+        if ((rc = firstMaxOverBlocks(c, 0, &out->NRdPmax, nullptr)) != OCPG_OK) return rc;
+        if ((rc = firstMaxOverBlocks(c, 1, &out->NRdNmax, nullptr)) != OCPG_OK) return rc;
+    }
+    return OCPG_OK;
+}
+
+// After the flash that followed ocpg_get_solution: PassFlashValue's dSNR
+// and maxNRdSSP (src/OCPFluidMethod.cpp:1135-1146) and Bulk::CalNRdSmax
+// (src/Bulk.cpp:1235-1247).  Waits for the stream.
+int ocpg_nr_change(ocpg_ctx* c, ocpg_nr_change_t* out)
+{
+    if (!c || !out) return OCPG_EINVAL;
+    NewtonDevice& N = ctxNewton(c);
+    const DevBulk& B = ctxBulk(c);
+    if (!N.valid || N.nb != B.nb) return OCPG_ESTATE;
+    HIPCHK2(hipSetDevice(ctxDevice(c)));
+    HIPCHK2(launchNRChange(B, N, ctxStream(c)));
+    N.valid = false;
+    int rc;
+    int64_t k = 0;
+    if ((rc = firstMaxOverBlocks(c, 0, &out->NRdSmax, &k)) != OCPG_OK) return rc;
+    out->NRdSmax_cell = k / 3;
+    if ((rc = firstMaxOverBlocks(c, 1, &out->maxNRdSSP, &out->maxNRdSSP_cell)) != OCPG_OK) return rc;
+    return OCPG_OK;
+}
+
+// The open and shut wells' bottom-hole pressures held on the device (nw).
+int ocpg_get_well_bhp(ocpg_ctx* c, double* bhp)
+{
+    if (!c || !bhp) return OCPG_EINVAL;
+    WellDevice& W = ctxWells(c);
+    if (W.nw <= 0) return OCPG_ESTATE;
+    HIPCHK2(hipSetDevice(ctxDevice(c)));
+    HIPCHK2(hipMemcpyAsync(bhp, W.bhp, (size_t)W.nw * sizeof(double), hipMemcpyDeviceToHost, ctxStream(c)));
+    HIPCHK2(hipStreamSynchronize(ctxStream(c)));
+    return OCPG_OK;
+}
+
 // IsoT_FIM::AssembleMat (src/OCPFluidMethod.cpp:860-875), bulk blocks; the
 // right-hand side is the residual CalRes left on the device.
 int ocpg_assemble(ocpg_ctx* c, double dt)
@@ -506,6 +634,9 @@ int ocpg_assemble(ocpg_ctx* c, double dt)
     if (F.nb <= 0 || F.nb != ctxBulk(c).nb) return OCPG_ESTATE;
     HIPCHK2(hipSetDevice(ctxDevice(c)));
     WellDevice& W = ctxWells(c);
+    ctxTimerBegin(c, T_PACK);
+    HIPCHK2(launchPackRecords(ctxNC(c), ctxBulk(c), F, ctxStream(c)));
+    ctxTimerEnd(c, T_PACK);
     ctxTimerBegin(c, T_ASM);
     HIPCHK2(launchAssemble(ctxNC(c), ctxBulk(c), F, dt, ctxStream(c)));
     if (W.nw > 0) HIPCHK2(launchWellAssemble(ctxNC(c), ctxBulk(c), F, W, dt, ctxStream(c)));

@@ -8,6 +8,12 @@
 
 namespace ocpg {
 
+// Timer slots of ocpg_timing_get (names in ocpgpu_capi.cpp).
+enum TimerSlot {
+    T_FLASH = 0, T_KRPC, T_RES, T_ASM, T_INIT, T_RESTORE, T_STAB, T_SPLIT, T_FINISH, T_TAIL, T_FIXUP, T_UPDASM,
+    T_TAILNR, T_PACK, kTimerSlots
+};
+
 struct WellsHost {
     int nw = 0;
     std::vector<int32_t> type, open, mode, unweight, perfStart, perfCell;

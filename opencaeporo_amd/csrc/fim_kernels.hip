@@ -109,23 +109,28 @@ __global__ void __launch_bounds__(256) krPcRockKernel(DevBulk B, const DevSat* _
     krPcRockCell(B, sat, rock, krpc, rockOn, n);
 }
 
-// kr/Pc and rock of the cells in two lists (the overlapped flash's side cells).
-__global__ void __launch_bounds__(256) krPcRockListKernel(DevBulk B, const DevSat* __restrict__ sat, DevRock rock,
-                                                          const int32_t* __restrict__ L0,
-                                                          const int32_t* __restrict__ L1,
-                                                          const int32_t* __restrict__ c0,
-                                                          const int32_t* __restrict__ c1)
+hipError_t launchKrPcRock(const DevBulk& B, const DevSat* sat, const DevRock& rock, bool krpc, bool rockOn,
+                          hipStream_t s)
 {
-    const int64_t n0 = *c0, total = n0 + *c1;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x)
-        krPcRockCell(B, sat, rock, 1, 1, i < n0 ? L0[i] : L1[i - n0]);
+    if (B.nb == 0) return hipSuccess;
+    const unsigned grid = (unsigned)((B.nb + 255) / 256);
+    hipLaunchKernelGGL(krPcRockKernel, dim3(grid), dim3(256), 0, s, B, sat, rock, krpc ? 1 : 0, rockOn ? 1 : 0);
+    return hipGetLastError();
 }
 
-hipError_t launchKrPcRockList(const DevBulk& B, const DevSat* sat, const DevRock& rock, const int32_t* const* lists,
-                              const int32_t* const* counts, hipStream_t s)
+// kr/Pc and rock of the overlapped flash's side cells.
+__global__ void __launch_bounds__(256) krPcRockListKernel(DevBulk B, const DevSat* __restrict__ sat, DevRock rock,
+                                                          CellLists L)
 {
-    hipLaunchKernelGGL(krPcRockListKernel, dim3(64), dim3(256), 0, s, B, sat, rock, lists[0], lists[1], counts[0],
-                       counts[1]);
+    const int64_t total = cellTotal(L);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x)
+        krPcRockCell(B, sat, rock, 1, 1, cellAt(L, i));
+}
+
+hipError_t launchKrPcRockList(const DevBulk& B, const DevSat* sat, const DevRock& rock, const CellLists& L,
+                              hipStream_t s)
+{
+    hipLaunchKernelGGL(krPcRockListKernel, dim3(64), dim3(256), 0, s, B, sat, rock, L);
     return hipGetLastError();
 }
 
@@ -134,74 +139,52 @@ hipError_t launchKrPcRockList(const DevBulk& B, const DevSat* sat, const DevRock
 // what CalRes and the assembly computed on the main stream is final for a
 // row whose cell and neighbours all finished there, and is recomputed for
 // the side cells (bit 2) and their neighbours (bit 1); CalRes also for the
-// perforated cells (bit 4: the wells' rates read the side cells too).
-// mark[] is all zero between calls: the compaction clears what it reads.
+// perforated cells (bit 4: the wells' rates read the side cells too).  A
+// row joins a list when its first bit of that list is set, so each row is
+// listed once.  mark[] is all zero between calls (fixupClearKernel).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) fixupMarkKernel(FimDevice F, const int32_t* __restrict__ L0,
-                                                       const int32_t* __restrict__ L1,
-                                                       const int32_t* __restrict__ c0,
-                                                       const int32_t* __restrict__ c1, uint32_t* mark,
-                                                       int32_t* counts)
+__device__ __forceinline__ void markRow(uint32_t* mark, int32_t row, uint32_t bit, const FixupRows& R)
 {
-    if (blockIdx.x == 0 && threadIdx.x < 2) counts[threadIdx.x] = 0;
-    const int64_t n0 = *c0, total = n0 + *c1;
+    const uint32_t old = atomicOr(&mark[row], bit);
+    if (old == 0) R.resRows[atomicAdd(&R.counts[0], 1)] = row;
+    if ((bit & 3u) && (old & 3u) == 0) R.asmRows[atomicAdd(&R.counts[1], 1)] = row;
+}
+
+__global__ void __launch_bounds__(256) fixupMarkKernel(FimDevice F, CellLists L, FixupRows R)
+{
+    const int64_t total = cellTotal(L);
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t t = i < n0 ? L0[i] : L1[i - n0];
-        atomicOr(&mark[t], 2u);
-        for (int k = F.IA[t] + 1; k < F.bulkEnd[t]; ++k) atomicOr(&mark[F.JA[k]], 1u);
+        const int32_t t = cellAt(L, i);
+        markRow(R.mark, t, 2u, R);
+        for (int k = F.IA[t] + 1; k < F.bulkEnd[t]; ++k) markRow(R.mark, F.JA[k], 1u, R);
     }
 }
 
-__global__ void __launch_bounds__(256) fixupMarkPerfKernel(WellDevice W, uint32_t* mark)
+__global__ void __launch_bounds__(256) fixupMarkPerfKernel(WellDevice W, FixupRows R)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < W.npcell) atomicOr(&mark[W.pcell[i]], 4u);
+    if (i < W.npcell) markRow(R.mark, W.pcell[i], 4u, R);
 }
 
-__device__ __forceinline__ void appendRow(int32_t* list, int32_t* count, bool pred, int32_t row)
+__global__ void __launch_bounds__(256) fixupClearKernel(FixupRows R)
 {
-    const unsigned long long m = __ballot(pred);
-    if (m == 0) return;
-    const int lane = threadIdx.x & 63;
-    const int leader = __ffsll((long long)m) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(count, __popcll(m));
-    base = __shfl(base, leader, 64);
-    if (pred) base += __popcll(lane == 0 ? 0ull : (m & ((1ull << lane) - 1ull)));
-    if (pred) list[base] = row;
+    const int64_t total = R.counts[0];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x)
+        R.mark[R.resRows[i]] = 0;
 }
 
-__global__ void __launch_bounds__(256) fixupCompactKernel(int64_t nb, uint32_t* mark, int32_t* resRows,
-                                                          int32_t* asmRows, int32_t* counts)
+__global__ void fixupResetKernel(FixupRows R)
 {
-    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t m = 0;
-    if (n < nb) {
-        m = mark[n];
-        if (m) mark[n] = 0;
-    }
-    appendRow(resRows, counts + 0, m != 0, (int32_t)n);
-    appendRow(asmRows, counts + 1, (m & 3u) != 0, (int32_t)n);
+    if (threadIdx.x < 2) R.counts[threadIdx.x] = 0;
 }
 
-hipError_t launchFixupRows(const FimDevice& F, const WellDevice& W, int64_t nb, const int32_t* const* lists,
-                           const int32_t* const* counts, const FixupRows& R, hipStream_t s)
+hipError_t launchFixupRows(const FimDevice& F, const WellDevice& W, const CellLists& L, const FixupRows& R,
+                           hipStream_t s)
 {
-    hipLaunchKernelGGL(fixupMarkKernel, dim3(64), dim3(256), 0, s, F, lists[0], lists[1], counts[0], counts[1],
-                       R.mark, R.counts);
-    if (W.npcell > 0)
-        hipLaunchKernelGGL(fixupMarkPerfKernel, dim3((W.npcell + 255) / 256), dim3(256), 0, s, W, R.mark);
-    hipLaunchKernelGGL(fixupCompactKernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, nb, R.mark,
-                       R.resRows, R.asmRows, R.counts);
-    return hipGetLastError();
-}
-
-hipError_t launchKrPcRock(const DevBulk& B, const DevSat* sat, const DevRock& rock, bool krpc, bool rockOn,
-                          hipStream_t s)
-{
-    if (B.nb == 0) return hipSuccess;
-    const unsigned grid = (unsigned)((B.nb + 255) / 256);
-    hipLaunchKernelGGL(krPcRockKernel, dim3(grid), dim3(256), 0, s, B, sat, rock, krpc ? 1 : 0, rockOn ? 1 : 0);
+    hipLaunchKernelGGL(fixupResetKernel, dim3(1), dim3(64), 0, s, R);
+    hipLaunchKernelGGL(fixupMarkKernel, dim3(64), dim3(256), 0, s, F, L, R);
+    if (W.npcell > 0) hipLaunchKernelGGL(fixupMarkPerfKernel, dim3((W.npcell + 255) / 256), dim3(256), 0, s, W, R);
+    hipLaunchKernelGGL(fixupClearKernel, dim3(64), dim3(256), 0, s, R);
     return hipGetLastError();
 }
 
@@ -409,6 +392,31 @@ struct AsmRec {
     static constexpr int perLane = (LEN + kAsmBlock - 1) / kAsmBlock;
 };
 
+// The own-row data of a row's assembly: compact dSec_dPri rows, dPcj_dS,
+// the accumulation terms and the secondary-column bookkeeping (counts stored
+// as doubles, exactly).
+template <int NC>
+struct AsmOwn {
+    using D = AsmDims<NC>;
+    static constexpr int nc = NC + 1;
+    static constexpr int DSEC = 0;                    // [r * ncol + c], r < rows
+    static constexpr int DPC = D::maxRows * D::ncol;  // dPcj_dS[9]
+    static constexpr int VFI = DPC + 9;               // vfi[nc]
+    static constexpr int VFP = VFI + nc, V = VFP + 1, POROP = V + 1;
+    static constexpr int SDEX = POROP + 1;            // pSderExist[3]
+    static constexpr int PVN = SDEX + 3;              // pVnumCom[3]
+    static constexpr int ROWS = PVN + 3;              // bRowSizedSdP
+    static constexpr int LEN = ROWS + 1;
+};
+
+// Both records are packed per cell (AoS, rows padded to whole 128-byte
+// lines), so a wavefront reads a neighbour's record as one contiguous
+// ~1 KB span instead of one 8-byte value from each of ~120 arrays.
+template <int NC>
+constexpr int recStride() { return (AsmRec<NC>::LEN + 15) / 16 * 16; }
+template <int NC>
+constexpr int ownStride() { return (AsmOwn<NC>::LEN + 15) / 16 * 16; }
+
 // Where record entry q lives: a double field advanced to its entry, or (for
 // the phase flags) a byte field.
 struct RecSrc {
@@ -446,6 +454,91 @@ __device__ __forceinline__ double loadRec(const RecSrc& s, int64_t cell)
     return 0.0;
 }
 
+// Own-row entry q of cell n (q >= DSEC + rows * ncol inside the dSec part is
+// never read and not stored).
+template <int NC>
+__device__ __forceinline__ double ownValue(const DevBulk& B, int q, int64_t n)
+{
+    using O = AsmOwn<NC>;
+    const int64_t ld = B.ld;
+    if (q < O::DPC) return B.dSec[(int64_t)q * ld + n];
+    if (q < O::VFI) return B.dPcj_dS[(int64_t)(q - O::DPC) * ld + n];
+    if (q < O::VFP) return B.vfi[(int64_t)(q - O::VFI) * ld + n];
+    if (q == O::VFP) return B.vfP[n];
+    if (q == O::V) return B.v[n];
+    if (q == O::POROP) return B.poroP[n];
+    if (q < O::PVN) return B.pSderExist[(int64_t)(q - O::SDEX) * ld + n] ? 1.0 : 0.0;
+    if (q < O::ROWS) return (double)B.pVnumCom[(int64_t)(q - O::PVN) * ld + n];
+    return (double)B.bRowSizedSdP[n];
+}
+
+// Pack both records of kPackTile consecutive cells: read field by field
+// (each field's kPackTile values are one 128-byte line), transpose in LDS,
+// write the cells' records as contiguous spans.
+constexpr int kPackTile = 16;
+constexpr int kPackBlock = 256;
+
+template <int NC>
+__global__ void __launch_bounds__(kPackBlock) packRecordsKernel(DevBulk B, FimDevice F)
+{
+    using R = AsmRec<NC>;
+    using O = AsmOwn<NC>;
+    constexpr int ncol = AsmDims<NC>::ncol;
+    constexpr int RS = recStride<NC>(), OS = ownStride<NC>();
+    constexpr int L = (R::LEN > O::LEN ? R::LEN : O::LEN) + 1;  // odd LDS row: no bank conflicts
+    __shared__ double t[kPackTile * L];
+    __shared__ int rows[kPackTile];
+    const int64_t n0 = (int64_t)blockIdx.x * kPackTile;
+    const int tile = (int)min<int64_t>(kPackTile, B.nb - n0);
+    if (threadIdx.x < kPackTile)
+        rows[threadIdx.x] = threadIdx.x < tile ? min(max(B.bRowSizedSdP[n0 + threadIdx.x], 0),
+                                                     AsmDims<NC>::maxRows) : 0;
+    // neighbour record
+    for (int i = threadIdx.x; i < R::LEN * kPackTile; i += kPackBlock) {
+        const int q = i / kPackTile, c = i % kPackTile;
+        if (c < tile) t[c * L + q] = loadRec(recSource<NC>(B, q), n0 + c);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < RS * kPackTile; i += kPackBlock) {
+        const int c = i / RS, q = i % RS;
+        if (c < tile && q < R::LEN) F.rec[(n0 + c) * RS + q] = t[c * L + q];
+    }
+    __syncthreads();
+    // own-row record
+    for (int i = threadIdx.x; i < O::LEN * kPackTile; i += kPackBlock) {
+        const int q = i / kPackTile, c = i % kPackTile;
+        if (c < tile && (q >= O::DPC || q < rows[c] * ncol)) t[c * L + q] = ownValue<NC>(B, q, n0 + c);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < OS * kPackTile; i += kPackBlock) {
+        const int c = i / OS, q = i % OS;
+        if (c < tile && q < O::LEN && (q >= O::DPC || q < rows[c] * ncol)) F.own[(n0 + c) * OS + q] = t[c * L + q];
+    }
+}
+
+// Both records of the overlapped flash's side cells: one thread per entry.
+template <int NC>
+__global__ void __launch_bounds__(256) packRecordsListKernel(DevBulk B, FimDevice F, CellLists L)
+{
+    using R = AsmRec<NC>;
+    using O = AsmOwn<NC>;
+    constexpr int ncol = AsmDims<NC>::ncol;
+    constexpr int RS = recStride<NC>(), OS = ownStride<NC>();
+    constexpr int W = R::LEN + O::LEN;
+    const int64_t total = cellTotal(L) * W;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t n = cellAt(L, i / W);
+        const int q = (int)(i % W);
+        if (q < R::LEN) {
+            F.rec[n * RS + q] = loadRec(recSource<NC>(B, q), n);
+        } else {
+            const int qo = q - R::LEN;
+            const int rows = min(max(B.bRowSizedSdP[n], 0), AsmDims<NC>::maxRows);
+            if (qo >= O::DPC || qo < rows * ncol) F.own[n * OS + qo] = ownValue<NC>(B, qo, n);
+        }
+    }
+}
+
 template <int NC>
 struct AsmShared {
     using D = AsmDims<NC>;
@@ -478,18 +571,17 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
     // a previous row of this wavefront may still read the shared arrays
     __syncthreads();
 
-    RecSrc src[R::perLane];
-#pragma unroll
-    for (int t = 0; t < R::perLane; ++t) src[t] = recSource<NC>(B, lane + t * kAsmBlock);
-
+    using O = AsmOwn<NC>;
+    constexpr int RS = recStride<NC>(), OS = ownStride<NC>();
+    const double* __restrict__ own = F.own + n * OS;
 #pragma unroll
     for (int t = 0; t < R::perLane; ++t) {
         const int q = lane + t * kAsmBlock;
-        if (q < R::LEN) sRec[0][q] = loadRec(src[t], n);
+        if (q < R::LEN) sRec[0][q] = F.rec[n * RS + q];
     }
-    const int rowsOwn = min(max(B.bRowSizedSdP[n], 0), D::maxRows);
-    for (int q = lane; q < rowsOwn * ncol; q += kAsmBlock) sSec[q] = B.dSec[(int64_t)q * ld + n];
-    if (lane < 9) sDpc[lane] = B.dPcj_dS[(int64_t)lane * ld + n];
+    const int rowsOwn = min(max((int)own[O::ROWS], 0), D::maxRows);
+    for (int q = lane; q < rowsOwn * ncol; q += kAsmBlock) sSec[q] = own[O::DSEC + q];
+    if (lane < 9) sDpc[lane] = own[O::DPC + lane];
 
     // secondary columns of this row's cell: existing saturations, then the
     // x_ij of each phase with derivatives
@@ -498,8 +590,8 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
     int nsat = 0;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-        sdO[j] = B.pSderExist[j * ld + n];
-        pvO[j] = min(max(B.pVnumCom[j * ld + n], 0), NC);
+        sdO[j] = own[O::SDEX + j] != 0.0;
+        pvO[j] = min(max((int)own[O::PVN + j], 0), NC);
         nsat += sdO[j] ? 1 : 0;
     }
     const int ncolO = min(nsat + pvO[0] + pvO[1] + pvO[2], D::maxRows);
@@ -513,9 +605,9 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
         double v = 0;
         if (e < bsize) {
             if (r == 0 && cc == 0)
-                v = B.v[n] * B.poroP[n] - B.vfP[n];
+                v = own[O::V] * own[O::POROP] - own[O::VFP];
             else if (r == 0)
-                v = -B.vfi[(cc - 1) * ld + n];
+                v = -own[O::VFI + cc - 1];
             else if (r == cc)
                 v = 1;
         }
@@ -527,7 +619,10 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
     if (k0 < k1) {
         const int64_t m = F.JA[k0];
 #pragma unroll
-        for (int t = 0; t < R::perLane; ++t) pre[t] = loadRec(src[t], m);
+        for (int t = 0; t < R::perLane; ++t) {
+            const int q = lane + t * kAsmBlock;
+            pre[t] = q < R::LEN ? F.rec[m * RS + q] : 0.0;
+        }
     }
     for (int k = k0; k < k1; ++k) {
         const int64_t m = F.JA[k];
@@ -541,7 +636,10 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
         if (k + 1 < k1) {
             const int64_t m2 = F.JA[k + 1];
 #pragma unroll
-            for (int t = 0; t < R::perLane; ++t) pre[t] = loadRec(src[t], m2);
+            for (int t = 0; t < R::perLane; ++t) {
+                const int q = lane + t * kAsmBlock;
+                pre[t] = q < R::LEN ? F.rec[m2 * RS + q] : 0.0;
+            }
         }
         const bool ownB = n < m;  // this row is the connection's bId side
         const double* rB = ownB ? sRec[0] : sRec[1];
@@ -578,10 +676,13 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
         // ---- own side's dFdXp column and dFdXs, entry by entry ----
         // Pressure and saturation columns: every phase contributes, in the
         // reference's phase order.
-        const int nPS = 1 + nsat;
-        for (int q = lane; q < nc * nPS; q += kAsmBlock) {
-            const int i = q / nPS;
-            const int col = q % nPS - 1;  // -1: pressure column of dFdXp
+        // Lane (i, col) = (lane / 4, lane % 4 - 1): nc <= 16 rows of at most
+        // four columns, no division by a run-time count.
+        static_assert(nc * 4 <= kAsmBlock, "pressure/saturation entries exceed a wavefront");
+        {
+            const int i = lane >> 2;
+            const int col = (lane & 3) - 1;  // -1: pressure column of dFdXp
+            if (i < nc && col < nsat) {
             int j1S = -1;
             {
                 int s = 0;
@@ -644,12 +745,19 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
                 }
             }
             sDF[i * W + col + 1] = v;
+            }
         }
-        // x_{kX, jX} columns: only phase jX's pass touches them.
+        // x_{kX, jX} columns: only phase jX's pass touches them.  Entry
+        // (i, kx) at q = i * XS + kx with the compile-time stride XS >= nX.
         const int nX = max(ncolO - nsat, 0);
-        for (int q = lane; q < nc * nX; q += kAsmBlock) {
-            const int i = q / nX;
-            int kk = q % nX, j = 0;
+        constexpr int XS = 2 * NC;
+#pragma unroll
+        for (int t = 0; t < (nc * XS + kAsmBlock - 1) / kAsmBlock; ++t) {
+            const int q = lane + t * kAsmBlock;
+            const int i = q / XS;
+            const int kx = q % XS;
+            if (i >= nc || kx >= nX) continue;
+            int kk = kx, j = 0;
             if (kk >= pvO[0]) {
                 kk -= pvO[0];
                 j = 1;
@@ -683,7 +791,7 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
                     v += -transIJ * rO[R::RHOX + j * nc + kk] / 2 * dGamma;
                 }
             }
-            sDF[i * W + 1 + nsat + q % nX] = v;
+            sDF[i * W + 1 + nsat + kx] = v;
         }
         __syncthreads();
 
@@ -699,7 +807,11 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
                 if (r > 0) {
                     const double* row = &sDF[(r - 1) * W];
                     v = (cc == 0) ? row[0] : 0.0;
-                    for (int kq = 0; kq < ncolO; ++kq) v = v + row[1 + kq] * sSec[kq * ncol + cc];
+#pragma unroll
+                    for (int kq = 0; kq < D::maxRows; ++kq) {
+                        if (kq >= ncolO) break;
+                        v = v + row[1 + kq] * sSec[kq * ncol + cc];
+                    }
                 }
                 v = dt * v;
                 if (ownB) {
@@ -788,7 +900,194 @@ static hipError_t launchAssembleT(const DevBulk& B, FimDevice& F, double dt, hip
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// IsoT_FIM::GetSolution (src/OCPFluidMethod.cpp:2735-2857), one lane per
+// cell, in the reference's operation order (DaAxpby, src/DenseMat.cpp:
+// 203-219, for dSec_dPri u).  The signed first maxima NRdPmax / NRdNmax
+// (largest magnitude, lowest index on ties, as the reference's sequential
+// scan finds them) reduce per block here and over blocks on the host.
+// ---------------------------------------------------------------------------
+constexpr int kNewtonBlock = 256;
+
+// First maximum by magnitude: (key, value, index); key 0 never wins (the
+// reference starts from 0 and replaces on a strictly larger magnitude).
+struct FirstMax {
+    double key, val;
+    int64_t idx;
+};
+__device__ __forceinline__ void fmTake(FirstMax& a, const FirstMax& b)
+{
+    if (b.key > a.key || (b.key == a.key && b.key > 0 && b.idx < a.idx)) a = b;
+}
+
+__device__ __forceinline__ void fmBlockReduce(FirstMax* sh, FirstMax v, double* part, int64_t* partIdx, int slot)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int st = kNewtonBlock / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) fmTake(sh[threadIdx.x], sh[threadIdx.x + st]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[4 * blockIdx.x + 2 * slot + 0] = sh[0].val;
+        part[4 * blockIdx.x + 2 * slot + 1] = sh[0].key;
+        partIdx[2 * blockIdx.x + slot] = sh[0].idx;
+    }
+    __syncthreads();
+}
+
+template <int NC>
+__global__ void __launch_bounds__(kNewtonBlock) getSolutionKernel(DevBulk B, NewtonDevice N,
+                                                                  const double* __restrict__ u, double dSmaxlim)
+{
+    constexpr int nc = NC + 1;
+    constexpr int col = nc + 1;
+    constexpr int maxRows = 3 + 2 * NC;
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t ld = B.ld;
+    FirstMax bp{0.0, 0.0, INT64_MAX}, bn{0.0, 0.0, INT64_MAX};
+    if (n < B.nb) {
+#pragma unroll
+        for (int j = 0; j < kNP; ++j) {
+            N.dSNR[j * ld + n] = B.S[j * ld + n];
+            N.pexOld[j * ld + n] = B.phaseExist[j * ld + n];
+        }
+        const int rows = min(max(B.bRowSizedSdP[n], 0), maxRows);
+        double dtmp[maxRows];
+#pragma unroll
+        for (int r = 0; r < maxRows; ++r) {
+            double y = 0;
+            if (r < rows) {
+                y = 1.0 * y;
+#pragma unroll
+                for (int c = 0; c < col; ++c) y += 1.0 * B.dSec[(int64_t)(r * col + c) * ld + n] * u[n * col + c];
+            }
+            dtmp[r] = y;
+        }
+        double chopmin = 1;
+        int js = 0;
+#pragma unroll
+        for (int j = 0; j < kNP; ++j) {
+            if (!B.pSderExist[j * ld + n]) continue;
+            const double S = B.S[j * ld + n];
+            double d = 0;
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+                if (r == js) d = dtmp[r];
+            double choptmp = 1;
+            if (fabs(d) > dSmaxlim) {
+                choptmp = dSmaxlim / fabs(d);
+            } else if (S + d < 0.0) {
+                choptmp = 0.9 * S / fabs(d);
+            }
+            chopmin = (choptmp < chopmin) ? choptmp : chopmin;
+            js++;
+        }
+        js = 0;
+#pragma unroll
+        for (int j = 0; j < kNP; ++j) {
+            if (!B.pSderExist[j * ld + n]) {
+                N.dSNRP[j * ld + n] = 0;
+                continue;
+            }
+            double d = 0;
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+                if (r == js) d = dtmp[r];
+            const double dS = chopmin * d;
+            N.dSNRP[j * ld + n] = dS;
+            B.S[j * ld + n] += dS;
+            js++;
+        }
+        if (B.phaseNum[n] >= 3) {
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int i = 0; i < NC; ++i) {
+                    double d = 0;
+#pragma unroll
+                    for (int r = 0; r < maxRows; ++r)
+                        if (r == js) d = dtmp[r];
+                    B.xij[(int64_t)(j * nc + i) * ld + n] += chopmin * d;
+                    js++;
+                }
+        }
+        const double dP = u[n * col];
+        bp = FirstMax{fabs(dP), dP, n};
+        if (!(bp.key > 0)) bp = FirstMax{0.0, 0.0, INT64_MAX};
+        B.P[n] += dP;
+        const double Nt = B.Nt[n];
+#pragma unroll
+        for (int i = 0; i < nc; ++i) {
+            const double dN = u[n * col + 1 + i] * chopmin;
+            if (bn.key < fabs(dN) / Nt) bn = FirstMax{fabs(dN) / Nt, dN / Nt, n * nc + i};
+            B.Ni[(int64_t)i * ld + n] += dN;
+        }
+    }
+    __shared__ FirstMax sh[kNewtonBlock];
+    fmBlockReduce(sh, bp, N.part, N.partIdx, 0);
+    fmBlockReduce(sh, bn, N.part, N.partIdx, 1);
+}
+
+// Well::SetBHP(BHP() + u) of the open wells (src/OCPFluidMethod.cpp:2846-2854).
+__global__ void wellBhpUpdateKernel(WellDevice W, const double* __restrict__ u, int64_t nb, int col)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < W.nw && W.row[w] >= 0) W.bhp[w] += u[(nb + W.row[w]) * col];
+}
+
+// PassFlashValue's dSNR = S - dSNR and maxNRdSSP (src/OCPFluidMethod.cpp:
+// 1135-1146) and Bulk::CalNRdSmax (src/Bulk.cpp:1235-1247).
+__global__ void __launch_bounds__(kNewtonBlock) nrChangeKernel(DevBulk B, NewtonDevice N)
+{
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t ld = B.ld;
+    FirstMax bs{0.0, 0.0, INT64_MAX}, bd{0.0, 0.0, INT64_MAX};
+    if (n < B.nb) {
+#pragma unroll
+        for (int j = 0; j < kNP; ++j) {
+            const double d = B.S[j * ld + n] - N.dSNR[j * ld + n];
+            N.dSNR[j * ld + n] = d;
+            if (N.pexOld[j * ld + n]) {
+                const double e = d - N.dSNRP[j * ld + n];
+                if (bs.key < fabs(e)) bs = FirstMax{fabs(e), e, n};
+            }
+            if (bd.key < fabs(d)) bd = FirstMax{fabs(d), d, n * kNP + j};
+        }
+    }
+    __shared__ FirstMax sh[kNewtonBlock];
+    fmBlockReduce(sh, bd, N.part, N.partIdx, 0);
+    fmBlockReduce(sh, bs, N.part, N.partIdx, 1);
+}
+
+template <int NC>
+static hipError_t launchGetSolutionT(const DevBulk& B, NewtonDevice& N, const WellDevice& W, int64_t nb,
+                                     const double* u, double dSmaxlim, hipStream_t s)
+{
+    hipLaunchKernelGGL(getSolutionKernel<NC>, dim3(N.blocks), dim3(kNewtonBlock), 0, s, B, N, u, dSmaxlim);
+    if (W.nw > 0)
+        hipLaunchKernelGGL(wellBhpUpdateKernel, dim3((W.nw + 63) / 64), dim3(64), 0, s, W, u, nb, NC + 2);
+    return hipGetLastError();
+}
+
+hipError_t launchNRChange(const DevBulk& B, NewtonDevice& N, hipStream_t s)
+{
+    hipLaunchKernelGGL(nrChangeKernel, dim3(N.blocks), dim3(kNewtonBlock), 0, s, B, N);
+    return hipGetLastError();
+}
+
 #define OCPG_FIM_NC(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
+
+hipError_t launchGetSolution(int NC, const DevBulk& B, NewtonDevice& N, const WellDevice& W, int64_t nb,
+                             const double* u, double dSmaxlim, hipStream_t s)
+{
+    if (B.nb == 0) return hipSuccess;
+#define X(N_) \
+    if (NC == N_) return launchGetSolutionT<N_>(B, N, W, nb, u, dSmaxlim, s);
+    OCPG_FIM_NC(X)
+#undef X
+    return hipErrorInvalidValue;
+}
 
 hipError_t launchCalRes(int NC, const DevBulk& B, FimDevice& F, const WellDevice& W, double dt, hipStream_t s,
                         bool norms)
@@ -833,6 +1132,59 @@ hipError_t launchAssembleRows(int NC, const DevBulk& B, FimDevice& F, double dt,
     return hipErrorInvalidValue;
 }
 
+template <int NC>
+static hipError_t launchPackT(const DevBulk& B, FimDevice& F, hipStream_t s)
+{
+    const unsigned grid = (unsigned)((B.nb + kPackTile - 1) / kPackTile);
+    hipLaunchKernelGGL(packRecordsKernel<NC>, dim3(grid), dim3(kPackBlock), 0, s, B, F);
+    return hipGetLastError();
+}
+
+template <int NC>
+static hipError_t launchPackListT(const DevBulk& B, FimDevice& F, const CellLists& L, hipStream_t s)
+{
+    hipLaunchKernelGGL(packRecordsListKernel<NC>, dim3(256), dim3(256), 0, s, B, F, L);
+    return hipGetLastError();
+}
+
+int64_t recordDoubles(int NC)
+{
+#define X(N) \
+    if (NC == N) return recStride<N>();
+    OCPG_FIM_NC(X)
+#undef X
+    return 0;
+}
+
+int64_t ownDoubles(int NC)
+{
+#define X(N) \
+    if (NC == N) return ownStride<N>();
+    OCPG_FIM_NC(X)
+#undef X
+    return 0;
+}
+
+hipError_t launchPackRecords(int NC, const DevBulk& B, FimDevice& F, hipStream_t s)
+{
+    if (B.nb == 0) return hipSuccess;
+#define X(N) \
+    if (NC == N) return launchPackT<N>(B, F, s);
+    OCPG_FIM_NC(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launchPackRecordsList(int NC, const DevBulk& B, FimDevice& F, const CellLists& L, hipStream_t s)
+{
+    if (B.nb == 0) return hipSuccess;
+#define X(N) \
+    if (NC == N) return launchPackListT<N>(B, F, L, s);
+    OCPG_FIM_NC(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
 hipError_t launchAssemble(int NC, const DevBulk& B, FimDevice& F, double dt, hipStream_t s)
 {
     if (B.nb == 0) return hipSuccess;
@@ -846,7 +1198,7 @@ hipError_t launchAssemble(int NC, const DevBulk& B, FimDevice& F, double dt, hip
 void fimFree(FimDevice& f)
 {
     void* ptrs[] = {f.bId, f.eId, f.area, f.IA, f.JA, f.slotConn, f.slotMirror, f.bulkEnd, f.upblock,
-                    f.upRho, f.upVel, f.res, f.val, f.normPart, f.normIdx};
+                    f.upRho, f.upVel, f.res, f.val, f.normPart, f.normIdx, f.rec, f.own};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     f = FimDevice{};

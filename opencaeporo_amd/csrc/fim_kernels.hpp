@@ -32,6 +32,9 @@ struct FimDevice {
     int64_t nrow = 0;               // rows: bulk cells, then open wells
     double* res = nullptr;          // nrow * ncol, cell-major (LinearSystem rhs layout), negated
     double* val = nullptr;          // nnzb * ncol * ncol, row-major blocks
+    // per-cell records the assembly reads, packed (AoS) by launchPackRecords
+    double* rec = nullptr;          // nb * recStride: a connection side's cell values
+    double* own = nullptr;          // nb * ownStride: a row's own dSec_dPri, dPcj_dS, accumulation terms
     double* normPart = nullptr;     // per-block partial maxima of CalRes norms
     int64_t* normIdx = nullptr;
     int normBlocks = 0;
@@ -50,6 +53,31 @@ hipError_t launchCalRes(int NC, const DevBulk& B, FimDevice& F, const WellDevice
                         bool norms = true);
 hipError_t launchResNorm(int NC, const DevBulk& B, FimDevice& F, hipStream_t s);
 hipError_t launchAssemble(int NC, const DevBulk& B, FimDevice& F, double dt, hipStream_t s);
+// Doubles per cell of the packed records (FimDevice::rec, ::own), and their
+// packing from the SoA state (every cell, or the cells of two lists).
+int64_t recordDoubles(int NC);
+int64_t ownDoubles(int NC);
+hipError_t launchPackRecords(int NC, const DevBulk& B, FimDevice& F, hipStream_t s);
+hipError_t launchPackRecordsList(int NC, const DevBulk& B, FimDevice& F, const CellLists& L, hipStream_t s);
+
+// IsoT_FIM::GetSolution on the device (fim_kernels.hip) and the Newton-
+// change bookkeeping of the next PassFlashValue / Bulk::CalNRdSmax.
+struct NewtonDevice {
+    int64_t nb = 0, ld = 0;
+    double* dSNR = nullptr;     // np per cell (SoA): S before the update; after ocpg_nr_change, its change
+    double* dSNRP = nullptr;    // np per cell: the update's saturation changes
+    uint8_t* pexOld = nullptr;  // np per cell: phaseExist before the flash that follows
+    double* part = nullptr;     // per block: 2 (value, magnitude) pairs of first maxima
+    int64_t* partIdx = nullptr; // per block: their indices
+    int blocks = 0;
+    double* u = nullptr;        // staging of a solution vector from the host
+    int64_t uLen = 0;
+    bool valid = false;         // a GetSolution happened since the last flash bookkeeping
+    void* mem = nullptr;
+};
+hipError_t launchGetSolution(int NC, const DevBulk& B, NewtonDevice& N, const WellDevice& W, int64_t nb,
+                             const double* u, double dSmaxlim, hipStream_t s);
+hipError_t launchNRChange(const DevBulk& B, NewtonDevice& N, hipStream_t s);
 
 // The overlapped Newton iteration's recomputation (fim_kernels.hip): rows
 // touched by the cells the flash finished on its side streams.
@@ -61,12 +89,12 @@ struct FixupRows {
     void* mem = nullptr;
     int64_t nb = 0;
 };
-// kr/Pc + rock of the cells in lists[0..1] (lengths at counts[0..1], device).
-hipError_t launchKrPcRockList(const DevBulk& B, const DevSat* sat, const DevRock& rock, const int32_t* const* lists,
-                              const int32_t* const* counts, hipStream_t s);
-// The rows to recompute for the side cells in lists[0..1].
-hipError_t launchFixupRows(const FimDevice& F, const WellDevice& W, int64_t nb, const int32_t* const* lists,
-                           const int32_t* const* counts, const FixupRows& R, hipStream_t s);
+// kr/Pc + rock of the cells in L.
+hipError_t launchKrPcRockList(const DevBulk& B, const DevSat* sat, const DevRock& rock, const CellLists& L,
+                              hipStream_t s);
+// The rows to recompute for the side cells in L (and the perforated cells).
+hipError_t launchFixupRows(const FimDevice& F, const WellDevice& W, const CellLists& L, const FixupRows& R,
+                           hipStream_t s);
 hipError_t launchCalResRows(int NC, const DevBulk& B, FimDevice& F, const WellDevice& W, double dt,
                             const int32_t* rows, const int32_t* count, hipStream_t s);
 hipError_t launchAssembleRows(int NC, const DevBulk& B, FimDevice& F, double dt, const int32_t* rows,

@@ -26,21 +26,25 @@ hipError_t launchFlash(int NC, const DevEos* eos, const DevBulk& B, const FlashW
     return hipErrorInvalidValue;
 }
 
-hipError_t launchFlashTailFinish(int NC, const DevEos* eos, const DevBulk& B, const FlashWork& W, hipStream_t s)
+
+hipError_t launchFlashSideFinish(int NC, const DevEos* eos, const DevBulk& B, const FlashWork& W, hipStream_t s)
 {
 #define X(N) \
-    if (NC == N) return launchFlashTailFinishT<N>(eos, B, W, s);
+    if (NC == N) return launchFlashSideFinishT<N>(eos, B, W, s);
     OCPG_NC_LIST(X)
 #undef X
     return hipErrorInvalidValue;
 }
 
-void flashTailListsOf(const FlashWork& W, const int32_t** lists, const int32_t** counts)
+CellLists flashSideCells(const FlashWork& W)
 {
-    lists[0] = W.list[LIST_NP1T];
-    lists[1] = W.list[LIST_NP2T];
-    counts[0] = W.count + LIST_NP1T;
-    counts[1] = W.count + LIST_NP2T;
+    const int ids[kSideLists] = {LIST_NP1B, LIST_NP2B, LIST_NP1C, LIST_NP2C, LIST_NP1D, LIST_NP2D};
+    CellLists L;
+    for (int q = 0; q < kSideLists; q++) {
+        L.list[q] = W.list[ids[q]];
+        L.count[q] = W.count + ids[q];
+    }
+    return L;
 }
 
 hipError_t launchWellFlux(int NC, const DevEos* eos, const DevBulk& B, FimDevice& F, WellDevice& W, hipStream_t s)

@@ -3,6 +3,7 @@
 // :1095-1106), one lane per cell over the SoA state of device_state.hpp.
 #pragma once
 
+#include "flash_coop.hpp"
 #include "flash_device.hpp"
 #include "flash_kernel.hpp"
 
@@ -523,15 +524,17 @@ __device__ __forceinline__ int calFtypeFIM(const FlashCell<NC>& F, const DevBulk
 // stability test or split takes far longer than the rest (hundreds of
 // iterations, Newton restarts) would hold the whole pass.  There the
 // stability test and the split of the cells it finds unstable run on side
-// stream B, the split slices' leftovers and the cells whose first Newton
-// split fails run to the end on side stream C, and the main stream goes on
-// to finish the other cells and to the caller's next kernels.  The side
-// cells end in LIST_NP1T / LIST_NP2T; the caller joins the side streams,
-// finishes those cells (launchFlashTailFinishT) and recomputes what read
-// them.  Every cell still runs the same arithmetic on the same inputs.
+// stream B, the split slices' leftovers on side stream C and the cells
+// whose first Newton split fails on side stream D, each to the end of its
+// split, while the main stream finishes the other cells and goes on to the
+// caller's next kernels.  The caller joins the side streams, finishes their
+// cells (launchFlashSideFinishT: one small launch once the chip is free,
+// where a side-stream launch would wait for the main stream's kernels to
+// drain) and recomputes what read them (flashSideCells).  Every cell still
+// runs the same arithmetic on the same inputs.
 // ---------------------------------------------------------------------------
 
-enum { FL_SKIP = 1, FL_SSMSTA = 2, FL_NRSTA = 4, FL_SSMSP = 8, FL_NRSP = 16, FL_GUIDED = 32 };
+enum { FL_SKIP = 1, FL_SSMSTA = 2, FL_NRSTA = 4, FL_SSMSP = 8, FL_NRSP = 16, FL_GUIDED = 32, FL_DEFER = 64 };
 // Blocks of a work-queue stage: 256 CUs x 4 SIMDs x 2 waves, in blocks of
 // kFlashBlock / 64 waves.
 constexpr unsigned kQueueBlocks = 256 * 4 * 2 / (kFlashBlock / 64);
@@ -548,9 +551,15 @@ constexpr int kSplitBudget1 = 24;
 // (the warm pass: 0.3% of the two-phase cells need more than 96 in all) and
 // hands the rest to side stream C.
 constexpr int kSplitBudget2 = 64;
-// Side-stream work queues are small: one block per 64 cells would do, this
-// many at most.
-constexpr unsigned kSideBlocks = 256;
+// Longest lists the side streams take (longer ones stay on the main
+// stream), and their grids: the per-lane side kernels hold a few waves;
+// the one-cell-per-wavefront kernels start every cell of a short list at
+// once.
+constexpr int kSideMaxStab = 4096;
+constexpr int kSideMaxTail = 2048;
+constexpr int kSideMaxRestart = 1 << 30;  // restarts are the slowest cells: always beside
+constexpr unsigned kSideBlocks = 32;
+constexpr unsigned kCoopBlocks = 256;
 
 // Append the active lanes' cells to list k: one atomic per wavefront.
 __device__ __forceinline__ void appendList(const FlashWork& W, int k, bool pred, int32_t cell)
@@ -575,6 +584,15 @@ __device__ __forceinline__ int nextChunk(const FlashWork& W, int k)
     int base = 0;
     if ((threadIdx.x & 63) == 0) base = atomicAdd(&W.count[kFlashLists + k], 64);
     return __shfl(base, 0, 64);
+}
+
+// Overlapped form: a list is worked by a side stream while it is short and
+// by the main stream once it is long (side streams hold only a few waves).
+// gate > 0: run only when the list has at most gate cells; gate < 0: only
+// when it has more than -gate; 0: always.
+__device__ __forceinline__ bool gateOpen(int gate, int count)
+{
+    return gate == 0 || (gate > 0 ? count <= gate : count > -gate);
 }
 
 __device__ __forceinline__ unsigned long long waveSum(unsigned long long v)
@@ -682,13 +700,14 @@ __device__ __forceinline__ void loadFeed(FlashCell<NC>& F, const DevBulk& B, con
 template <int NC, bool Guided>
 __global__ void __launch_bounds__(kFlashBlock) flashStabilityKernel(const DevEos* __restrict__ eos, DevBulk B,
                                                                     FlashWork W, DevFlashCounters* ctr,
-                                                                    int outSplit, int outNP1)
+                                                                    int outSplit, int outNP1, int gate)
 {
     __shared__ double sSq[FlashCell<NC>::NT * kFlashBlock];
     const int nA = W.count[LIST_STAB];
     const int padA = Guided ? (nA + 63) & ~63 : nA;
     const int nG = Guided ? W.count[LIST_STABG] : 0;
     const int total = padA + nG;
+    if (!gateOpen(gate, total)) return;
     for (int base = nextChunk(W, LIST_STAB); base < total; base = nextChunk(W, LIST_STAB)) {
     const bool guidedChunk = Guided && base >= padA;
     const int in = guidedChunk ? LIST_STABG : LIST_STAB;
@@ -770,17 +789,18 @@ __device__ __forceinline__ void splitStartK(FlashCell<NC>& F, const DevBulk& B, 
 // as successive-substitution slices and a Newton stage.  A slice runs at most
 // `budget` SplitSSM2 iterations per cell (budget < 0: to the end); cells
 // still iterating go to list `cont` with their state (Ks, Se, iteration
-// counts), finished ones to LIST_NR.  Bounding the iterations per slice keeps
+// counts), finished ones to list nrOut.  Bounding the iterations per slice keeps
 // a wavefront from idling on its slowest lane: the cells that need many
 // iterations continue together in the next slice.
 template <int NC>
 __global__ void __launch_bounds__(kFlashBlock) flashSplitSSMKernel(const DevEos* __restrict__ eos, DevBulk B,
                                                                    FlashWork W, int in, int cont, int budget,
-                                                                   int first)
+                                                                   int first, int nrOut, int gate)
 {
     __shared__ double sSq[FlashCell<NC>::NT * kFlashBlock];
     const int64_t ld = B.ld;
     const int total = W.count[in];
+    if (!gateOpen(gate, total)) return;
     for (int base = nextChunk(W, in); base < total; base = nextChunk(W, in)) {
     const int idx = base + (threadIdx.x & 63);
     const bool active = idx < total;
@@ -826,7 +846,7 @@ __global__ void __launch_bounds__(kFlashBlock) flashSplitSSMKernel(const DevEos*
             }
         }
     }
-    appendList(W, LIST_NR, active && done, n);
+    appendList(W, nrOut, active && done, n);
     if (cont >= 0) appendList(W, cont, active && !done, n);
     }
 }
@@ -873,16 +893,18 @@ __device__ __forceinline__ void splitDone(FlashCell<NC>& F, const DevBulk& B, co
 // fractions of its last Rachford-Rice solve and the fugacities of those
 // compositions (recomputed: the same arithmetic on the same inputs).
 // deferList >= 0 (overlapped form): a cell whose first Newton split fails
-// leaves for that list (side stream C runs its restarts) with its Newton
-// iteration count; Ks and the other counters are already in the buffers.
+// leaves for that list (FL_DEFER set; a side stream runs its restarts) with
+// its Newton iteration count; Ks and the other counters are already in the
+// buffers.
 template <int NC>
 __global__ void __launch_bounds__(kFlashBlock) flashSplitNRKernel(const DevEos* __restrict__ eos, DevBulk B,
-                                                                  FlashWork W, DevFlashCounters* ctr, int deferList)
+                                                                  FlashWork W, DevFlashCounters* ctr, int deferList,
+                                                                  int in, int outNP1, int outNP2)
 {
     __shared__ double sSq[FlashCell<NC>::NT * kFlashBlock];
     const int64_t ld = B.ld;
-    const int total = W.count[LIST_NR];
-    for (int base = nextChunk(W, LIST_NR); base < total; base = nextChunk(W, LIST_NR)) {
+    const int total = W.count[in];
+    for (int base = nextChunk(W, in); base < total; base = nextChunk(W, in)) {
     const int idx = base + (threadIdx.x & 63);
     const bool active = idx < total;
     int NP = 1;
@@ -891,7 +913,7 @@ __global__ void __launch_bounds__(kFlashBlock) flashSplitNRKernel(const DevEos* 
     bool deferred = false;
     int32_t n = 0;
     if (active) {
-        n = W.list[LIST_NR][idx];
+        n = W.list[in][idx];
         FlashCell<NC> F(*eos, &sSq[threadIdx.x]);
         loadFeed<NC>(F, B, W, n);
         int fl = W.flags[n];
@@ -920,14 +942,15 @@ __global__ void __launch_bounds__(kFlashBlock) flashSplitNRKernel(const DevEos* 
         if (deferList >= 0 && !F.nrSpConflag) {
             deferred = true;
             W.nrIt[n] = F.it[IT_NRSP];
+            W.flags[n] = (uint8_t)(fl | FL_DEFER);
         } else {
             F.splitRestarts();
             NP = F.NP;
             splitDone<NC>(F, B, W, n, fl, its, nonconv);
         }
     }
-    appendList(W, LIST_NP2, active && !deferred && NP == 2, n);
-    appendList(W, LIST_NP1, active && !deferred && NP == 1, n);
+    appendList(W, outNP2, active && !deferred && NP == 2, n);
+    appendList(W, outNP1, active && !deferred && NP == 1, n);
     if (deferList >= 0) appendList(W, deferList, active && deferred, n);
     if (ctr) {
         addCounter(&ctr->iters[IT_SSMSP], (unsigned long long)its[0]);
@@ -944,17 +967,18 @@ __global__ void __launch_bounds__(kFlashBlock) flashSplitNRKernel(const DevEos* 
 //   TAIL_SSM     where the main stream's last slice stopped, or
 //   TAIL_RESTART the failed first Newton split (the restart loop),
 // with the same arithmetic as the slices and the Newton stage.  Finished
-// cells go to LIST_NP1T / LIST_NP2T.
+// cells go to lists outNP1 / outNP2.
 enum { TAIL_FULL = 0, TAIL_SSM = 1, TAIL_RESTART = 2 };
 
 template <int NC>
 __global__ void __launch_bounds__(kFlashBlock) flashSplitTailKernel(const DevEos* __restrict__ eos, DevBulk B,
                                                                     FlashWork W, DevFlashCounters* ctr, int in,
-                                                                    int mode)
+                                                                    int mode, int outNP1, int outNP2, int gate)
 {
     __shared__ double sSq[FlashCell<NC>::NT * kFlashBlock];
     const int64_t ld = B.ld;
     const int total = W.count[in];
+    if (!gateOpen(gate, total)) return;
     for (int base = nextChunk(W, in); base < total; base = nextChunk(W, in)) {
     const int idx = base + (threadIdx.x & 63);
     const bool active = idx < total;
@@ -1009,8 +1033,8 @@ __global__ void __launch_bounds__(kFlashBlock) flashSplitTailKernel(const DevEos
         NP = F.NP;
         splitDone<NC>(F, B, W, n, fl, its, nonconv);
     }
-    appendList(W, LIST_NP2T, active && NP == 2, n);
-    appendList(W, LIST_NP1T, active && NP == 1, n);
+    appendList(W, outNP2, active && NP == 2, n);
+    appendList(W, outNP1, active && NP == 1, n);
     if (ctr) {
         addCounter(&ctr->iters[IT_SSMSP], (unsigned long long)its[0]);
         addCounter(&ctr->iters[IT_NRSP], (unsigned long long)its[1]);
@@ -1018,6 +1042,79 @@ __global__ void __launch_bounds__(kFlashBlock) flashSplitTailKernel(const DevEos
         addCounter(&ctr->nonconverged, nonconv ? 1 : 0);
         addCounter(&ctr->twoPhase, (active && NP == 2) ? 1 : 0);
     }
+    }
+}
+
+// Side streams, one wavefront per cell (flash_coop.hpp): the few cells
+// whose split runs past the slices (as TAIL_SSM) or into Newton restarts
+// (FL_DEFER set: as TAIL_RESTART) continue with their successive
+// substitution spread over the wavefront's lanes; SplitNR and CheckSplit
+// run on every lane alike.  Same results as flashSplitTailKernel, bit for
+// bit.
+__device__ __forceinline__ int nextCell(const FlashWork& W, int k)
+{
+    int idx = 0;
+    if ((threadIdx.x & 63) == 0) idx = atomicAdd(&W.count[kFlashLists + k], 1);
+    return __shfl(idx, 0, 64);
+}
+
+// (one wavefront per SIMD: the few waves of this kernel may use the whole
+// register file instead of spilling to scratch memory)
+template <int NC>
+__global__ void __launch_bounds__(kFlashBlock) __attribute__((amdgpu_waves_per_eu(1, 1)))
+flashSplitTailCoopKernel(const DevEos* __restrict__ eos, DevBulk B, FlashWork W, DevFlashCounters* ctr, int in,
+                         int outNP1, int outNP2, int gate)
+{
+    __shared__ double sSq[FlashCell<NC>::NT * kFlashBlock];
+    const int64_t ld = B.ld;
+    const int total = W.count[in];
+    if (!gateOpen(gate, total)) return;
+    const bool lead = (threadIdx.x & 63) == 0;
+    for (int idx = nextCell(W, in); idx < total; idx = nextCell(W, in)) {
+        const int32_t n = W.list[in][idx];
+        FlashCell<NC> F(*eos, &sSq[threadIdx.x]);
+        loadFeed<NC>(F, B, W, n);
+        const int fl = W.flags[n];
+        F.ssmStaConflag = fl & FL_SSMSTA;
+        F.nrStaConflag = fl & FL_NRSTA;
+        F.nrSpConflag = false;
+        F.Yt = W.Yt[n];
+        F.NP = 2;
+        F.calAiBi();
+        F.it[IT_SSMSTA] = 0;
+        F.it[IT_NRSTA] = 0;
+        F.it[IT_NRSP] = 0;
+#pragma unroll
+        for (int i = 0; i < NC; ++i) F.Ks[i] = W.Ks[i * ld + n];
+        F.it[IT_RR] = W.rrIt[n];
+        F.ssmSpConflag = (fl & FL_SSMSP) != 0;
+        if (fl & FL_DEFER) {
+            F.it[IT_SSMSP] = W.ssmIt[n];
+            F.it[IT_NRSP] = W.nrIt[n];
+            CoopSplit<NC> C(F);
+            C.splitRestarts();
+        } else {
+            double Se = W.Se[n];
+            int iter = W.ssmIt[n];
+            CoopSplit<NC> C(F);
+            C.splitSSM2Rest(Se, iter);
+            F.it[IT_SSMSP] = iter;
+            F.splitNR();
+            C.splitRestarts();
+        }
+        const int NP = F.NP;
+        int its[3] = {0, 0, 0};
+        bool nonconv = false;
+        if (lead) splitDone<NC>(F, B, W, n, fl, its, nonconv);
+        appendList(W, outNP2, lead && NP == 2, n);
+        appendList(W, outNP1, lead && NP == 1, n);
+        if (ctr) {
+            addCounter(&ctr->iters[IT_SSMSP], (unsigned long long)its[0]);
+            addCounter(&ctr->iters[IT_NRSP], (unsigned long long)its[1]);
+            addCounter(&ctr->iters[IT_RR], (unsigned long long)its[2]);
+            addCounter(&ctr->nonconverged, nonconv ? 1 : 0);
+            addCounter(&ctr->twoPhase, (lead && NP == 2) ? 1 : 0);
+        }
     }
 }
 
@@ -1077,6 +1174,85 @@ __global__ void __launch_bounds__(kFlashBlock) flashFinishKernel(const DevEos* _
     }
 }
 
+// The finishing stage of the side streams' cells, one launch for both
+// phase counts: wavefront chunks of the one-phase lists first, then of the
+// two-phase lists.
+template <int NC>
+__global__ void __launch_bounds__(kFlashBlock) flashFinishSideKernel(const DevEos* __restrict__ eos, DevBulk B,
+                                                                     FlashWork W)
+{
+    __shared__ double sSq[FlashCell<NC>::NT * kFlashBlock];
+    const int l1[3] = {LIST_NP1B, LIST_NP1C, LIST_NP1D};
+    const int l2[3] = {LIST_NP2B, LIST_NP2C, LIST_NP2D};
+    int c1[3], c2[3], chunks1 = 0, chunks = 0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        c1[q] = W.count[l1[q]];
+        c2[q] = W.count[l2[q]];
+        chunks1 += (c1[q] + 63) / 64;
+    }
+    chunks = chunks1;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) chunks += (c2[q] + 63) / 64;
+    const int64_t ld = B.ld;
+    constexpr int nc = NC + 1;
+    for (int ch = (int)((blockIdx.x * blockDim.x + threadIdx.x) / 64); ch < chunks;
+         ch += (int)(gridDim.x * blockDim.x / 64)) {
+        const bool two = ch >= chunks1;
+        int r = two ? ch - chunks1 : ch, list = -1, idx = 0;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const int cnt = two ? c2[q] : c1[q];
+            const int nch = (cnt + 63) / 64;
+            if (list < 0 && r < nch) {
+                list = two ? l2[q] : l1[q];
+                idx = r * 64 + (int)(threadIdx.x & 63);
+                if (idx >= cnt) list = -2;
+            } else if (list == -1) {
+                r -= nch;
+            }
+        }
+        if (list < 0) continue;
+        const int32_t n = W.list[list][idx];
+        FlashCell<NC> F(*eos, &sSq[threadIdx.x]);
+        loadFeed<NC>(F, B, W, n);
+        F.flagSkip = (W.flags[n] & FL_SKIP) != 0;
+#pragma unroll
+        for (int i = 0; i < nc; ++i) F.Ni[i] = B.Ni[i * ld + n];
+        F.Nh = W.Nh[n];
+        F.calAiBi();
+        if (!two) {
+            F.NP = 1;
+            F.nu[0] = 1;
+#pragma unroll
+            for (int i = 0; i < NC; ++i) F.x[0][i] = F.zi[i];
+            F.calAjBj(F.Aj[0], F.Bj[0], F.x[0]);
+            F.Zj[0] = solEoS(F.Aj[0], F.Bj[0]);
+            F.nu[0] = F.Nh * F.nu[0];
+            FlashKernel<NC>::template finish<1>(F, *eos, B, n, false, 0.0, 0.0);
+        } else {
+            F.NP = 2;
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                F.nu[j] = W.nu[j * ld + n];
+#pragma unroll
+                for (int i = 0; i < NC; ++i) {
+                    F.x[j][i] = W.x[(j * NC + i) * ld + n];
+                    F.n[j][i] = W.n[(j * NC + i) * ld + n];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                F.calAjBj(F.Aj[j], F.Bj[j], F.x[j]);
+                F.Zj[j] = solEoS(F.Aj[j], F.Bj[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < 2; ++j) F.nu[j] = F.Nh * F.nu[j];
+            FlashKernel<NC>::template finish<2>(F, *eos, B, n, false, 0.0, 0.0);
+        }
+    }
+}
+
 template <int NC>
 hipError_t launchFlashT(const DevEos* eos, const DevBulk& B, const FlashWork& W, DevFlashCounters* ctr,
                         const double* zin, hipStream_t s, hipEvent_t* ev, const FlashSide* side)
@@ -1098,19 +1274,20 @@ hipError_t launchFlashT(const DevEos* eos, const DevBulk& B, const FlashWork& W,
         mark(0, s);
         if (W.guess)
             hipLaunchKernelGGL((flashStabilityKernel<NC, true>), dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, ctr,
-                               (int)LIST_SPLIT, (int)LIST_NP1);
+                               (int)LIST_SPLIT, (int)LIST_NP1, 0);
         else
             hipLaunchKernelGGL((flashStabilityKernel<NC, false>), dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W,
-                               ctr, (int)LIST_SPLIT, (int)LIST_NP1);
+                               ctr, (int)LIST_SPLIT, (int)LIST_NP1, 0);
         mark(1, s);
         mark(2, s);
         hipLaunchKernelGGL(flashSplitSSMKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W,
-                           (int)LIST_SPLIT, (int)LIST_C1, kSplitBudget0, 1);
+                           (int)LIST_SPLIT, (int)LIST_C1, kSplitBudget0, 1, (int)LIST_NR, 0);
         hipLaunchKernelGGL(flashSplitSSMKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, (int)LIST_C1,
-                           (int)LIST_C2, kSplitBudget1, 0);
+                           (int)LIST_C2, kSplitBudget1, 0, (int)LIST_NR, 0);
         hipLaunchKernelGGL(flashSplitSSMKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, (int)LIST_C2,
-                           -1, -1, 0);
-        hipLaunchKernelGGL(flashSplitNRKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, ctr, -1);
+                           -1, -1, 0, (int)LIST_NR, 0);
+        hipLaunchKernelGGL(flashSplitNRKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, ctr, -1,
+                           (int)LIST_NR, (int)LIST_NP1, (int)LIST_NP2);
         mark(3, s);
         mark(4, s);
         hipLaunchKernelGGL((flashFinishKernel<NC, 1>), dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, init,
@@ -1120,43 +1297,60 @@ hipError_t launchFlashT(const DevEos* eos, const DevBulk& B, const FlashWork& W,
         mark(5, s);
         return hipGetLastError();
     }
-    // overlapped form (not with a learned guess, nor for InitFlashFIM)
+    // Overlapped form (not with a learned guess, nor for InitFlashFIM).
+    // Each side list is worked on its side stream while it is short (a
+    // warm Newton pass: a few hundred cells) and on the main stream, in the
+    // serial order, when it is long (a cold pass puts every cell through the
+    // stability test): a kernel pair gated on the list's length on the
+    // device, exactly one of which works.
     if (W.guess || zin) return hipErrorInvalidValue;
     if ((e = hipEventRecord(side->classified, s)) != hipSuccess) return e;
     // side stream B: the stability test and the split of the cells it finds unstable
     if ((e = hipStreamWaitEvent(side->sB, side->classified, 0)) != hipSuccess) return e;
     mark(0, side->sB);
     hipLaunchKernelGGL((flashStabilityKernel<NC, false>), dim3(sgrid), dim3(kFlashBlock), 0, side->sB, eos, B, W,
-                       ctr, (int)LIST_SPLITB, (int)LIST_NP1T);
+                       ctr, (int)LIST_SPLITB, (int)LIST_NP1B, kSideMaxStab);
     hipLaunchKernelGGL(flashSplitTailKernel<NC>, dim3(sgrid), dim3(kFlashBlock), 0, side->sB, eos, B, W, ctr,
-                       (int)LIST_SPLITB, (int)TAIL_FULL);
+                       (int)LIST_SPLITB, (int)TAIL_FULL, (int)LIST_NP1B, (int)LIST_NP2B, 0);
     mark(1, side->sB);
     if ((e = hipEventRecord(side->doneB, side->sB)) != hipSuccess) return e;
-    // main stream: the slices, the third one bounded
+    // main stream: a long stability list, then the slices, the third one bounded
+    hipLaunchKernelGGL((flashStabilityKernel<NC, false>), dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, ctr,
+                       (int)LIST_SPLIT, (int)LIST_NP1, -kSideMaxStab);
     mark(2, s);
     hipLaunchKernelGGL(flashSplitSSMKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, (int)LIST_SPLIT,
-                       (int)LIST_C1, kSplitBudget0, 1);
+                       (int)LIST_C1, kSplitBudget0, 1, (int)LIST_NR, 0);
     hipLaunchKernelGGL(flashSplitSSMKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, (int)LIST_C1,
-                       (int)LIST_C2, kSplitBudget1, 0);
+                       (int)LIST_C2, kSplitBudget1, 0, (int)LIST_NR, 0);
     hipLaunchKernelGGL(flashSplitSSMKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, (int)LIST_C2,
-                       (int)LIST_TSSM, kSplitBudget2, 0);
+                       (int)LIST_TSSM, kSplitBudget2, 0, (int)LIST_NR, 0);
+    // a long leftover list stays on the main stream
+    hipLaunchKernelGGL(flashSplitSSMKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, (int)LIST_TSSM,
+                       -1, -1, 0, (int)LIST_NR, -kSideMaxTail);
     if ((e = hipEventRecord(side->sliced, s)) != hipSuccess) return e;
-    // side stream C: the slices' leftovers run to their end
+    // side stream C: a short leftover list runs to its end, one wavefront
+    // per cell
     if ((e = hipStreamWaitEvent(side->sC, side->sliced, 0)) != hipSuccess) return e;
     mark(6, side->sC);
-    hipLaunchKernelGGL(flashSplitTailKernel<NC>, dim3(sgrid), dim3(kFlashBlock), 0, side->sC, eos, B, W, ctr,
-                       (int)LIST_TSSM, (int)TAIL_SSM);
-    // main stream: Newton stage; cells that would restart go to stream C
-    hipLaunchKernelGGL(flashSplitNRKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, ctr,
-                       (int)LIST_TNR);
-    mark(3, s);
-    if ((e = hipEventRecord(side->newton, s)) != hipSuccess) return e;
-    if ((e = hipStreamWaitEvent(side->sC, side->newton, 0)) != hipSuccess) return e;
-    hipLaunchKernelGGL(flashSplitTailKernel<NC>, dim3(sgrid), dim3(kFlashBlock), 0, side->sC, eos, B, W, ctr,
-                       (int)LIST_TNR, (int)TAIL_RESTART);
+    hipLaunchKernelGGL(flashSplitTailCoopKernel<NC>, dim3(kCoopBlocks), dim3(kFlashBlock), 0, side->sC, eos, B,
+                       W, ctr, (int)LIST_TSSM, (int)LIST_NP1C, (int)LIST_NP2C, kSideMaxTail);
     mark(7, side->sC);
     if ((e = hipEventRecord(side->doneC, side->sC)) != hipSuccess) return e;
-    // main stream: finish the cells it holds
+    // main stream: Newton stage; cells that would restart go to list TNR
+    hipLaunchKernelGGL(flashSplitNRKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, ctr,
+                       (int)LIST_TNR, (int)LIST_NR, (int)LIST_NP1, (int)LIST_NP2);
+    mark(3, s);
+    if ((e = hipEventRecord(side->newton, s)) != hipSuccess) return e;
+    // side stream D: a short restart list, one wavefront per cell
+    if ((e = hipStreamWaitEvent(side->sD, side->newton, 0)) != hipSuccess) return e;
+    mark(8, side->sD);
+    hipLaunchKernelGGL(flashSplitTailCoopKernel<NC>, dim3(kCoopBlocks), dim3(kFlashBlock), 0, side->sD, eos, B,
+                       W, ctr, (int)LIST_TNR, (int)LIST_NP1D, (int)LIST_NP2D, kSideMaxRestart);
+    mark(9, side->sD);
+    if ((e = hipEventRecord(side->doneD, side->sD)) != hipSuccess) return e;
+    // main stream: a long restart list, then finish the cells it holds
+    hipLaunchKernelGGL(flashSplitTailKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, ctr,
+                       (int)LIST_TNR, (int)TAIL_RESTART, (int)LIST_NP1, (int)LIST_NP2, -kSideMaxRestart);
     mark(4, s);
     hipLaunchKernelGGL((flashFinishKernel<NC, 1>), dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, 0,
                        (int)LIST_NP1);
@@ -1166,18 +1360,15 @@ hipError_t launchFlashT(const DevEos* eos, const DevBulk& B, const FlashWork& W,
     return hipGetLastError();
 }
 
-// Overlapped form, after the caller has made stream s wait for side->doneB
-// and side->doneC: finish the side streams' cells.
+// Overlapped form, once the caller's stream s has joined the side streams:
+// the finishing stage of their cells.
 template <int NC>
-hipError_t launchFlashTailFinishT(const DevEos* eos, const DevBulk& B, const FlashWork& W, hipStream_t s)
+hipError_t launchFlashSideFinishT(const DevEos* eos, const DevBulk& B, const FlashWork& W, hipStream_t s)
 {
     if (B.nb == 0) return hipSuccess;
     const unsigned grid = (unsigned)((B.nb + kFlashBlock - 1) / kFlashBlock);
-    const unsigned sgrid = std::min<unsigned>(grid, kSideBlocks);
-    hipLaunchKernelGGL((flashFinishKernel<NC, 1>), dim3(sgrid), dim3(kFlashBlock), 0, s, eos, B, W, 0,
-                       (int)LIST_NP1T);
-    hipLaunchKernelGGL((flashFinishKernel<NC, 2>), dim3(sgrid), dim3(kFlashBlock), 0, s, eos, B, W, 0,
-                       (int)LIST_NP2T);
+    hipLaunchKernelGGL(flashFinishSideKernel<NC>, dim3(std::min<unsigned>(grid, 256)), dim3(kFlashBlock), 0, s, eos,
+                       B, W);
     return hipGetLastError();
 }
 

@@ -16,24 +16,25 @@ bool flashSupportsNC(int nc);
 // Side streams and events of the overlapped flash (launchFlash with side
 // non-NULL).  Streams at the default priority.
 struct FlashSide {
-    hipStream_t sB, sC;
+    hipStream_t sB, sC, sD;
     hipEvent_t classified, sliced, newton;  // recorded on the main stream
-    hipEvent_t doneB, doneC;                // recorded on sB / sC: the caller waits for both
+    hipEvent_t doneB, doneC, doneD;         // recorded on sB / sC / sD: the caller waits for all
 };
 
 // FlashFIM over every cell (zin == NULL) or InitFlashFIM from the feed zin
-// (SoA, leading dimension B.ld).  With ev non-NULL, ev[0..5] are recorded
-// around the stability, split and finishing stages (begin, end pairs) and,
-// with side non-NULL, ev[6..7] around side stream C's work.  With side
-// non-NULL the slow cells continue on the side streams (see flash_impl.hpp):
-// the caller makes s wait for side->doneB and side->doneC and then calls
-// launchFlashTailFinish before anything reads those cells.
+// (SoA, leading dimension B.ld).  With ev non-NULL, begin/end event pairs
+// are recorded: ev[0..1] around the stability test (side stream B's work
+// with side non-NULL), ev[2..3] the split, ev[4..5] the finishing stage and,
+// with side non-NULL, ev[6..7] side stream C's work, ev[8..9] side stream
+// D's.  With side non-NULL the slow cells continue on the side streams (see
+// flash_impl.hpp): the caller makes s wait for side->doneB, doneC and doneD
+// and calls launchFlashSideFinish before anything reads those cells.
 hipError_t launchFlash(int NC, const DevEos* eos, const DevBulk& B, const FlashWork& W, DevFlashCounters* ctr,
                        const double* zin, hipStream_t s, hipEvent_t* ev = nullptr, const FlashSide* side = nullptr);
-hipError_t launchFlashTailFinish(int NC, const DevEos* eos, const DevBulk& B, const FlashWork& W, hipStream_t s);
-// The side streams' cells: lists[2] (one / two hydrocarbon phases) and the
-// device addresses of their lengths.
-void flashTailListsOf(const FlashWork& W, const int32_t** lists, const int32_t** counts);
+// After the join: the finishing stage of the side streams' cells, and
+// those cells.
+hipError_t launchFlashSideFinish(int NC, const DevEos* eos, const DevBulk& B, const FlashWork& W, hipStream_t s);
+CellLists flashSideCells(const FlashWork& W);
 // Well::CalTrans + CalFlux + CalProdWeight and the well rows of CalRes.
 hipError_t launchWellFlux(int NC, const DevEos* eos, const DevBulk& B, FimDevice& F, WellDevice& W, hipStream_t s);
 // The well blocks of AssembleMatWellsNew (after the bulk assembly).
@@ -46,7 +47,8 @@ template <int NC>
 hipError_t launchFlashT(const DevEos* eos, const DevBulk& B, const FlashWork& W, DevFlashCounters* ctr,
                         const double* zin, hipStream_t s, hipEvent_t* ev, const FlashSide* side);
 template <int NC>
-hipError_t launchFlashTailFinishT(const DevEos* eos, const DevBulk& B, const FlashWork& W, hipStream_t s);
+hipError_t launchFlashSideFinishT(const DevEos* eos, const DevBulk& B, const FlashWork& W, hipStream_t s);
+
 template <int NC>
 hipError_t launchWellFluxT(const DevEos* eos, const DevBulk& B, FimDevice& F, WellDevice& W, hipStream_t s);
 template <int NC>

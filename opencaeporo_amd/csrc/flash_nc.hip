@@ -10,7 +10,7 @@
 namespace ocpg {
 template hipError_t launchFlashT<OCPG_NC>(const DevEos*, const DevBulk&, const FlashWork&, DevFlashCounters*,
                                            const double*, hipStream_t, hipEvent_t*, const FlashSide*);
-template hipError_t launchFlashTailFinishT<OCPG_NC>(const DevEos*, const DevBulk&, const FlashWork&, hipStream_t);
+template hipError_t launchFlashSideFinishT<OCPG_NC>(const DevEos*, const DevBulk&, const FlashWork&, hipStream_t);
 template hipError_t launchWellFluxT<OCPG_NC>(const DevEos*, const DevBulk&, FimDevice&, WellDevice&, hipStream_t);
 template hipError_t launchWellAssembleT<OCPG_NC>(const DevBulk&, FimDevice&, WellDevice&, double, hipStream_t);
 }  // namespace ocpg

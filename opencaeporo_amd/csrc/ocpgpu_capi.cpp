@@ -68,13 +68,10 @@ constexpr int kNumFields = sizeof(kFields) / sizeof(kFields[0]);
 size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 // Timer slots of ocpg_timing_get.
-enum TimerSlot {
-    T_FLASH = 0, T_KRPC, T_RES, T_ASM, T_INIT, T_RESTORE, T_STAB, T_SPLIT, T_FINISH, T_TAIL, T_FIXUP, T_UPDASM,
-    kTimerSlots
-};
 const char* kTimerNames[kTimerSlots] = {"flash",        "krpc_rock",       "calres",      "assemble",
                                         "init_flash",   "restore",         "flash_stability", "flash_split",
-                                        "flash_finish", "flash_tail",      "fixup",       "update_assemble"};
+                                        "flash_finish", "flash_tail",      "fixup",       "update_assemble",
+                                        "flash_restarts", "pack"};
 
 // Fields the flash reads (and overwrites): what one Newton iteration starts from.
 bool isFlashInput(size_t hostOff)
@@ -119,6 +116,8 @@ struct ocpg_ctx {
     // flash and the rows recomputed after they join
     FlashSide side{};
     FixupRows fix{};
+    // IsoT_FIM::GetSolution on the device (ocpg_get_solution)
+    NewtonDevice newton{};
     // one Newton iteration captured as a graph (ocpg_capture_iteration)
     hipGraphExec_t iterGraph = nullptr;
     hipEvent_t iterDone = nullptr;
@@ -234,13 +233,15 @@ int fimReadNorms(ocpg_ctx* c, ocpg_resnorm* norms);
 
 static void destroySide(ocpg_ctx* c)
 {
-    if (c->side.sB) (void)hipStreamSynchronize(c->side.sB);
-    if (c->side.sC) (void)hipStreamSynchronize(c->side.sC);
-    hipEvent_t evs[] = {c->side.classified, c->side.sliced, c->side.newton, c->side.doneB, c->side.doneC};
+    hipStream_t ss[] = {c->side.sB, c->side.sC, c->side.sD};
+    for (hipStream_t st : ss)
+        if (st) (void)hipStreamSynchronize(st);
+    hipEvent_t evs[] = {c->side.classified, c->side.sliced, c->side.newton, c->side.doneB, c->side.doneC,
+                        c->side.doneD};
     for (hipEvent_t e : evs)
         if (e) (void)hipEventDestroy(e);
-    if (c->side.sB) (void)hipStreamDestroy(c->side.sB);
-    if (c->side.sC) (void)hipStreamDestroy(c->side.sC);
+    for (hipStream_t st : ss)
+        if (st) (void)hipStreamDestroy(st);
     c->side = FlashSide{};
 }
 
@@ -290,6 +291,8 @@ int64_t ocpg_sizeof(const char* name)
     if (!std::strcmp(name, "ocpg_resnorm")) return sizeof(ocpg_resnorm);
     if (!std::strcmp(name, "ocpg_wells")) return sizeof(ocpg_wells);
     if (!std::strcmp(name, "ocpg_flash_stats")) return sizeof(ocpg_flash_stats);
+    if (!std::strcmp(name, "ocpg_nr_update")) return sizeof(ocpg_nr_update);
+    if (!std::strcmp(name, "ocpg_nr_change_t")) return sizeof(ocpg_nr_change_t);
     return -1;
 }
 
@@ -372,9 +375,10 @@ int ocpg_create(ocpg_ctx** out, const ocpg_eos* eos, const ocpg_satfunc* sat, co
         if ((rc = hipStatus(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)))) break;
         if ((rc = hipStatus(hipStreamCreateWithFlags(&c->side.sB, hipStreamNonBlocking)))) break;
         if ((rc = hipStatus(hipStreamCreateWithFlags(&c->side.sC, hipStreamNonBlocking)))) break;
+        if ((rc = hipStatus(hipStreamCreateWithFlags(&c->side.sD, hipStreamNonBlocking)))) break;
         {
             hipEvent_t* evs[] = {&c->side.classified, &c->side.sliced, &c->side.newton, &c->side.doneB,
-                                 &c->side.doneC};
+                                 &c->side.doneC, &c->side.doneD};
             for (hipEvent_t* e : evs)
                 if ((rc = hipStatus(hipEventCreateWithFlags(e, hipEventDisableTiming)))) break;
             if (rc) break;
@@ -431,6 +435,8 @@ int ocpg_destroy(ocpg_ctx* c)
     if (c->dCtr) (void)hipFree(c->dCtr);
     if (c->dSat) (void)hipFree(c->dSat);
     if (c->fix.mem) (void)hipFree(c->fix.mem);
+    if (c->newton.mem) (void)hipFree(c->newton.mem);
+    if (c->newton.u) (void)hipFree(c->newton.u);
     destroySide(c);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -645,9 +651,9 @@ int ocpg_update_assemble(ocpg_ctx* c, double dt, ocpg_resnorm* norms, ocpg_flash
     int rc = ensureFixup(c);
     if (rc != OCPG_OK) return rc;
     WellDevice& W = c->wells;
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[10] = {};
     bool stageEv = c->timing;
-    for (int k = 0; k < 8 && stageEv; k++) stageEv = hipEventCreate(&ev[k]) == hipSuccess;
+    for (int k = 0; k < 10 && stageEv; k++) stageEv = hipEventCreate(&ev[k]) == hipSuccess;
     timerBegin(c, T_UPDASM);
     timerBegin(c, T_FLASH);
     HIPCHK(launchFlash(c->NC, c->dEos, c->B, c->work, stats ? c->dCtr : nullptr, nullptr, c->stream,
@@ -659,21 +665,24 @@ int ocpg_update_assemble(ocpg_ctx* c, double dt, ocpg_resnorm* norms, ocpg_flash
     timerBegin(c, T_RES);
     HIPCHK(launchCalRes(c->NC, c->B, F, W, dt, c->stream, false));
     timerEnd(c, T_RES);
+    timerBegin(c, T_PACK);
+    HIPCHK(launchPackRecords(c->NC, c->B, F, c->stream));
+    timerEnd(c, T_PACK);
     timerBegin(c, T_ASM);
     HIPCHK(launchAssemble(c->NC, c->B, F, dt, c->stream));
     timerEnd(c, T_ASM);
     // join the side streams, finish their cells, recompute what read them
     HIPCHK(hipStreamWaitEvent(c->stream, c->side.doneB, 0));
     HIPCHK(hipStreamWaitEvent(c->stream, c->side.doneC, 0));
+    HIPCHK(hipStreamWaitEvent(c->stream, c->side.doneD, 0));
     timerBegin(c, T_FIXUP);
-    HIPCHK(launchFlashTailFinish(c->NC, c->dEos, c->B, c->work, c->stream));
-    const int32_t* lists[2];
-    const int32_t* counts[2];
-    flashTailListsOf(c->work, lists, counts);
-    HIPCHK(launchKrPcRockList(c->B, c->dSat, c->rock, lists, counts, c->stream));
-    HIPCHK(launchFixupRows(F, W, c->B.nb, lists, counts, c->fix, c->stream));
+    HIPCHK(launchFlashSideFinish(c->NC, c->dEos, c->B, c->work, c->stream));
+    const CellLists sideCells = flashSideCells(c->work);
+    HIPCHK(launchKrPcRockList(c->B, c->dSat, c->rock, sideCells, c->stream));
+    HIPCHK(launchFixupRows(F, W, sideCells, c->fix, c->stream));
     if (W.nw > 0) HIPCHK(launchWellFlux(c->NC, c->dEos, c->B, F, W, c->stream));
     HIPCHK(launchCalResRows(c->NC, c->B, F, W, dt, c->fix.resRows, c->fix.counts + 0, c->stream));
+    HIPCHK(launchPackRecordsList(c->NC, c->B, F, sideCells, c->stream));
     HIPCHK(launchAssembleRows(c->NC, c->B, F, dt, c->fix.asmRows, c->fix.counts + 1, c->stream));
     if (W.nw > 0) HIPCHK(launchWellAssemble(c->NC, c->B, F, W, dt, c->stream));
     HIPCHK(launchResNorm(c->NC, c->B, F, c->stream));
@@ -684,6 +693,7 @@ int ocpg_update_assemble(ocpg_ctx* c, double dt, ocpg_resnorm* norms, ocpg_flash
         c->events[T_SPLIT].push_back({ev[2], ev[3]});
         c->events[T_FINISH].push_back({ev[4], ev[5]});
         c->events[T_TAIL].push_back({ev[6], ev[7]});
+        c->events[T_TAILNR].push_back({ev[8], ev[9]});
     } else {
         for (hipEvent_t e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -902,6 +912,7 @@ DevBulk& ctxBulk(ocpg_ctx* c) { return c->B; }
 FimDevice& ctxFim(ocpg_ctx* c) { return c->fim; }
 WellDevice& ctxWells(ocpg_ctx* c) { return c->wells; }
 FimHost& ctxFimHost(ocpg_ctx* c) { return c->fimHost; }
+NewtonDevice& ctxNewton(ocpg_ctx* c) { return c->newton; }
 const DevEos* ctxEos(ocpg_ctx* c) { return c->dEos; }
 hipStream_t ctxStream(ocpg_ctx* c) { return c->stream; }
 int ctxNC(ocpg_ctx* c) { return c->NC; }

@@ -11,7 +11,7 @@ import numpy as np
 from . import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libocpgpu.so")
+LIB_PATH = os.environ.get("OCPG_LIBRARY") or os.path.join(HERE, "libocpgpu.so")
 
 _lib = None
 
@@ -66,6 +66,9 @@ def load():
         "ocpg_calres": ([P, C.c_double, P], C.c_int),
         "ocpg_assemble": ([P, C.c_double], C.c_int),
         "ocpg_update_assemble": ([P, C.c_double, P, P], C.c_int),
+        "ocpg_get_solution": ([P, P, C.c_int, C.c_double, P], C.c_int),
+        "ocpg_nr_change": ([P, P], C.c_int),
+        "ocpg_get_well_bhp": ([P, P], C.c_int),
         "ocpg_get_bsr": ([P, P, P, P, P], C.c_int),
         "ocpg_get_upwind": ([P, P, P, P], C.c_int),
         "ocpg_save_flash_inputs": ([P], C.c_int),
@@ -183,6 +186,33 @@ class Context:
         check(self.lib.ocpg_update_assemble(self.h, dt, C.byref(nr) if nr else None,
                                             C.byref(st) if st else None), "ocpg_update_assemble")
         return nr, st
+
+    def get_solution(self, u, dSmaxlim, device_ptr=None):
+        """IsoT_FIM::GetSolution on the device from the solution u (host
+        array of rows * ncol, or device_ptr: an address in device memory).
+        Returns (NRdPmax, NRdNmax)."""
+        out = abi.NRUpdate()
+        if device_ptr is not None:
+            check(self.lib.ocpg_get_solution(self.h, C.c_void_p(device_ptr), 1, dSmaxlim, C.byref(out)),
+                  "ocpg_get_solution")
+        else:
+            u = np.ascontiguousarray(u, dtype=np.float64)
+            check(self.lib.ocpg_get_solution(self.h, u.ctypes.data, 0, dSmaxlim, C.byref(out)),
+                  "ocpg_get_solution")
+        return out.NRdPmax, out.NRdNmax
+
+    def nr_change(self):
+        """After the flash that followed get_solution: NRdSmax and maxNRdSSP
+        with their cells."""
+        out = abi.NRChange()
+        check(self.lib.ocpg_nr_change(self.h, C.byref(out)), "ocpg_nr_change")
+        return dict(NRdSmax=out.NRdSmax, NRdSmax_cell=out.NRdSmax_cell, maxNRdSSP=out.maxNRdSSP,
+                    maxNRdSSP_cell=out.maxNRdSSP_cell)
+
+    def get_well_bhp(self):
+        out = np.zeros(self.wells.nw)
+        check(self.lib.ocpg_get_well_bhp(self.h, out.ctypes.data), "ocpg_get_well_bhp")
+        return out
 
     def rows(self):
         return int(self.lib.ocpg_bsr_rows(self.h))

@@ -130,6 +130,25 @@ double orc_time_flash_fim(const ocpg_eos* eos, ocpg_bulk* bk, int nthreads)
     return std::chrono::duration<double>(t1 - t0).count();
 }
 
+// IsoT_FIM::GetSolution, bulk part; out[2] = {NRdPmax, NRdNmax}.
+int orc_get_solution(int nc, ocpg_bulk* bk, const double* u, double dSmaxlim, double* dSNR, double* dSNRP,
+                     uint8_t* phaseExistOld, double* out)
+{
+    if (!bk || !u || !dSNR || !dSNRP || !phaseExistOld || !out) return OCPG_EINVAL;
+    oracle::GetSolution(nc, *bk, u, dSmaxlim, dSNR, dSNRP, phaseExistOld, &out[0], &out[1]);
+    return OCPG_OK;
+}
+
+// The Newton-change bookkeeping after the flash; out[4] = {NRdSmax, its
+// cell, maxNRdSSP, its cell}.
+int orc_nr_change(int64_t nb, const double* S, double* dSNR, const double* dSNRP, const uint8_t* phaseExistOld,
+                  double* out)
+{
+    if (!S || !dSNR || !dSNRP || !phaseExistOld || !out) return OCPG_EINVAL;
+    oracle::NRChange(nb, S, dSNR, dSNRP, phaseExistOld, out);
+    return OCPG_OK;
+}
+
 #ifdef OCP_COUNT_FLOPS
 // Flop tally of one serial CalFlash pass: out[0] = add/sub/mul/div/sqrt,
 // out[1] = transcendental calls, out[2..3] the same within PhaseStable,

@@ -586,4 +586,114 @@ void AssembleBulks(int nc, const ocpg_conn& conn, const ocpg_bulk& bk, double dt
     (void)JA;
 }
 
+// DaAxpby, src/DenseMat.cpp:203-219: y = a A x + b y (A m x n, row-major).
+static void DaAxpby(int m, int n, double a, const double* A, const double* x, double b, double* y)
+{
+    for (int i = 0; i < m; i++) {
+        y[i] = b * y[i];
+        for (int j = 0; j < n; j++) y[i] += a * A[i * n + j] * x[j];
+    }
+}
+
+// IsoT_FIM::GetSolution, bulk part (src/OCPFluidMethod.cpp:2735-2844): the
+// secondary changes dSec_dPri u, the saturation chop, the updates of S,
+// x_ij (two hydrocarbon phases), P and Ni, and NRdPmax / NRdNmax (signed,
+// the first cell / component at the largest magnitude).  Also keeps what
+// the next PassFlashValue reads (:1135-1146): the saturations before the
+// update (dSNR), the predicted changes (dSNRP) and the phase flags.
+void GetSolution(int nc, const ocpg_bulk& bk, const double* u, double dSmaxlim, double* dSNR, double* dSNRP,
+                 uint8_t* phaseExistOld, double* NRdPmax, double* NRdNmax)
+{
+    const int np = NPH;
+    const int NC = nc - 1;
+    const int col = nc + 1;
+    const int row = np * (nc + 1);
+    const int lenSdP = maxLendSdP(nc);
+    std::vector<double> dtmp(row, 0.0);
+    *NRdPmax = 0;
+    *NRdNmax = 0;
+    for (int64_t k = 0; k < bk.nb * np; k++) {
+        dSNR[k] = bk.S[k];
+        phaseExistOld[k] = bk.phaseExist[k];
+    }
+    for (int64_t n = 0; n < bk.nb; n++) {
+        double chopmin = 1;
+        std::fill(dtmp.begin(), dtmp.end(), 0.0);
+        DaAxpby(bk.bRowSizedSdP[n], col, 1, &bk.dSec_dPri[n * lenSdP], u + n * col, 1, dtmp.data());
+        int js = 0;
+        for (int j = 0; j < np; j++) {
+            if (!bk.pSderExist[n * np + j]) continue;
+            double choptmp = 1;
+            if (std::fabs(dtmp[js]) > dSmaxlim) {
+                choptmp = dSmaxlim / std::fabs(dtmp[js]);
+            } else if (bk.S[n * np + j] + dtmp[js] < 0.0) {
+                choptmp = 0.9 * bk.S[n * np + j] / std::fabs(dtmp[js]);
+            }
+            chopmin = std::min(chopmin, choptmp);
+            js++;
+        }
+        js = 0;
+        for (int j = 0; j < np; j++) {
+            if (!bk.pSderExist[n * np + j]) {
+                dSNRP[n * np + j] = 0;
+                continue;
+            }
+            dSNRP[n * np + j] = chopmin * dtmp[js];
+            bk.S[n * np + j] += dSNRP[n * np + j];
+            js++;
+        }
+        if (bk.phaseNum[n] >= 3) {
+            for (int j = 0; j < 2; j++) {
+                const int64_t bId = n * np * nc + j * nc;
+                for (int i = 0; i < NC; i++) {
+                    bk.xij[bId + i] += chopmin * dtmp[js];
+                    js++;
+                }
+            }
+        }
+        const double dP = u[n * col];
+        if (std::fabs(*NRdPmax) < std::fabs(dP)) *NRdPmax = dP;
+        bk.P[n] += dP;
+        for (int i = 0; i < nc; i++) {
+            const double dN = u[n * col + 1 + i] * chopmin;
+            if (std::fabs(*NRdNmax) < std::fabs(dN) / bk.Nt[n]) *NRdNmax = dN / bk.Nt[n];
+            bk.Ni[n * nc + i] += dN;
+        }
+    }
+}
+
+// After the next flash: PassFlashValue's dSNR = S - dSNR and maxNRdSSP
+// (src/OCPFluidMethod.cpp:1135-1146, over the phases that existed before
+// the flash) and Bulk::CalNRdSmax (src/Bulk.cpp:1235-1247).  out = {NRdSmax,
+// its cell, maxNRdSSP, its cell}.
+void NRChange(int64_t nb, const double* S, double* dSNR, const double* dSNRP, const uint8_t* phaseExistOld,
+              double* out)
+{
+    const int np = NPH;
+    double maxSSP = 0;
+    int64_t iSSP = 0;
+    for (int64_t n = 0; n < nb; n++)
+        for (int j = 0; j < np; j++) {
+            const int64_t k = n * np + j;
+            dSNR[k] = S[k] - dSNR[k];
+            if (phaseExistOld[k]) {
+                if (std::fabs(maxSSP) < std::fabs(dSNR[k] - dSNRP[k])) {
+                    maxSSP = dSNR[k] - dSNRP[k];
+                    iSSP = n;
+                }
+            }
+        }
+    double dSmax = 0;
+    int64_t index = 0;
+    for (int64_t k = 0; k < nb * np; k++)
+        if (std::fabs(dSmax) < std::fabs(dSNR[k])) {
+            dSmax = dSNR[k];
+            index = k;
+        }
+    out[0] = dSmax;
+    out[1] = (double)(index / np);
+    out[2] = maxSSP;
+    out[3] = (double)iSSP;
+}
+
 }  // namespace oracle

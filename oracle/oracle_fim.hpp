@@ -37,6 +37,13 @@ void AssembleBulks(int nc, const ocpg_conn& conn, const ocpg_bulk& bk, double dt
                    const int32_t* upblock, const double* upblock_Rho, const int32_t* IA,
                    const int32_t* JA, double* val);
 
+// IsoT_FIM::GetSolution (bulk part) and the Newton-change bookkeeping of
+// the next PassFlashValue / Bulk::CalNRdSmax.
+void GetSolution(int nc, const ocpg_bulk& bk, const double* u, double dSmaxlim, double* dSNR, double* dSNRP,
+                 uint8_t* phaseExistOld, double* NRdPmax, double* NRdNmax);
+void NRChange(int64_t nb, const double* S, double* dSNR, const double* dSNRP, const uint8_t* phaseExistOld,
+              double* out);
+
 }  // namespace oracle
 
 #endif

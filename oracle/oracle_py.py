@@ -42,10 +42,37 @@ class Oracle:
         L.orc_calres_wells.argtypes = [P, P, P, P, C.c_double, P, P, P, P, P]
         L.orc_bsr_pattern_wells.argtypes = [C.c_int64, P, P, P, P]
         L.orc_assemble_wells.argtypes = [P, P, P, P, C.c_double, P, P, P, P, P]
+        L.orc_get_solution.argtypes = [C.c_int, P, P, C.c_double, P, P, P, P]
+        L.orc_nr_change.argtypes = [C.c_int64, P, P, P, P, P]
         L.orc_time_flash_fim.argtypes = [P, P, C.c_int]
         L.orc_time_flash_fim.restype = C.c_double
         if hasattr(L, "orc_count_flash_fim"):
             L.orc_count_flash_fim.argtypes = [P, P, P]
+
+    def get_solution(self, nc, bulk, u, dSmaxlim):
+        """IsoT_FIM::GetSolution on `bulk` (updated in place) from the
+        linear solution u (nb * (nc + 1)).  Returns the bookkeeping for
+        nr_change and (NRdPmax, NRdNmax)."""
+        nb = bulk.nb
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        book = dict(dSNR=np.zeros(nb * abi.NP), dSNRP=np.zeros(nb * abi.NP),
+                    pexOld=np.zeros(nb * abi.NP, dtype=np.uint8))
+        out = np.zeros(2)
+        s = bulk.cstruct()
+        rc = self.lib.orc_get_solution(nc, C.byref(s), u.ctypes.data, dSmaxlim, book["dSNR"].ctypes.data,
+                                       book["dSNRP"].ctypes.data, book["pexOld"].ctypes.data, out.ctypes.data)
+        assert rc == 0
+        return book, (out[0], out[1])
+
+    def nr_change(self, bulk, book):
+        """dSNR (S change over the Newton iteration), NRdSmax and maxNRdSSP
+        with their cells, after the flash that followed get_solution."""
+        dSNR = book["dSNR"].copy()
+        out = np.zeros(4)
+        rc = self.lib.orc_nr_change(bulk.nb, bulk.S.ctypes.data, dSNR.ctypes.data, book["dSNRP"].ctypes.data,
+                                    book["pexOld"].ctypes.data, out.ctypes.data)
+        assert rc == 0
+        return dSNR, dict(NRdSmax=out[0], NRdSmax_cell=int(out[1]), maxNRdSSP=out[2], maxNRdSSP_cell=int(out[3]))
 
     def flash_fim(self, eos, bulk, nthreads=1):
         s = bulk.cstruct()

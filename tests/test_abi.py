@@ -33,7 +33,8 @@ def test_library_exports_every_declared_symbol():
                                       ("ocpg_satfunc", abi.SatFunc), ("ocpg_rock", abi.Rock),
                                       ("ocpg_bulk", abi.BulkC), ("ocpg_conn", abi.Conn),
                                       ("ocpg_resnorm", abi.ResNorm), ("ocpg_wells", abi.WellsC),
-                                      ("ocpg_flash_stats", abi.FlashStats)])
+                                      ("ocpg_flash_stats", abi.FlashStats), ("ocpg_nr_update", abi.NRUpdate),
+                                      ("ocpg_nr_change_t", abi.NRChange)])
 def test_struct_sizes_match(name, cls):
     assert lib.load().ocpg_sizeof(name.encode()) == C.sizeof(cls)
 
