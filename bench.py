@@ -174,6 +174,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--nz", type=int, default=85, help="layers of the 60 x 220 x nz grid")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--handoff-steps", type=int, default=3,
+                    help="steps timed again with the BSR copied to page-locked host arrays (PCIe-inclusive rate)")
     ap.add_argument("--serial", action="store_true",
                     help="one Newton iteration as four serial calls (flash, kr/Pc + rock, CalRes, assembly) "
                          "instead of ocpg_update_assemble")
@@ -204,6 +206,20 @@ def main():
                                       "fixup", "update_assemble")}
     ctx.timing(False)
     per = {k: (v[0] / max(v[1], 1)) for k, v in kern.items()}
+    # the same iteration with the system handed to the host (FASP's
+    # dBSRmat arrays, page-locked) each step: the PCIe-inclusive rate
+    handoff_ms = None
+    if not a.serial and a.handoff_steps > 0:
+        arrays = ctx.pinned_bsr()
+        ctx.synchronize()
+        t0h = time.perf_counter()
+        for _ in range(a.handoff_steps):
+            step(ctx, False)
+            ctx.get_bsr_async(arrays)
+        ctx.synchronize()
+        handoff_ms = (time.perf_counter() - t0h) * 1e3 / a.handoff_steps
+        ctx.free_pinned(arrays)
+        log(f"[bench] with the BSR handed to pinned host arrays: {handoff_ms:.2f} ms per step")
     log("[bench] per-launch ms: " + ", ".join(f"{k} {v:.3f}" for k, v in per.items()))
     value = nb / (ms_step / 1e3)
     nnzb = ctx.nnzb()
@@ -282,7 +298,9 @@ def main():
                    "cells": nb, "connections": grid.nconn, "block": ncol,
                    "kernel_ms": per, "dominant_kernel": dominant, "flash_ms": flash_ms,
                    "flash_flops_per_cell": WARM_FLOPS_PER_CELL, "asm_bytes_per_launch": asm_bytes,
-                   "c_bar": grid.nconn / nb, "dsec_rows_mean": rows_mean, "roofline_other": other},
+                   "c_bar": grid.nconn / nb, "dsec_rows_mean": rows_mean, "roofline_other": other,
+                   "pcie_inclusive_ms_per_step": handoff_ms,
+                   "pcie_inclusive_cell_updates_per_s": (nb / (handoff_ms / 1e3)) if handoff_ms else None},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

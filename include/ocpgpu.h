@@ -331,6 +331,14 @@ int ocpg_get_well_bhp(ocpg_ctx* ctx, double* bhp);
  * dBSRmat (src/FaspSolver.cpp:362-429): IA, JA, val (row-major blocks), rhs.
  * Any pointer may be NULL.                                                    */
 int ocpg_get_bsr(ocpg_ctx* ctx, int32_t* IA, int32_t* JA, double* val, double* rhs);
+/* The same copies queued on the context's stream without waiting (the
+ * caller waits with ocpg_synchronize before the solver reads them), into
+ * page-locked arrays from ocpg_host_alloc for the full link rate: the
+ * hand-off of the reference's LinearSystem -> VectorFaspSolver::AssembleMat
+ * (src/FaspSolver.cpp:362-429) straight into the dBSRmat's arrays.         */
+int ocpg_get_bsr_async(ocpg_ctx* ctx, int32_t* IA, int32_t* JA, double* val, double* rhs);
+void* ocpg_host_alloc(int64_t bytes);
+int ocpg_host_free(void* p);
 /* Wells: AllWells as IsoT_FIM::UpdateProperty, CalRes and
  * AssembleMatWellsNew read it (src/OCPFluidMethod.cpp:914-935, 1284-1352,
  * 2417-2733).  ocpg_set_wells fixes the wells and perforations and rebuilds

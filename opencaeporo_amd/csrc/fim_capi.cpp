@@ -662,6 +662,39 @@ int ocpg_get_bsr(ocpg_ctx* c, int32_t* IA, int32_t* JA, double* val, double* rhs
     return OCPG_OK;
 }
 
+// The same hand-off without waiting: the copies are queued on the
+// context's stream behind the assembly and overlap with host work; the
+// caller waits with ocpg_synchronize before FASP reads the arrays.  With
+// page-locked destinations (ocpg_host_alloc) they run at the link's rate.
+int ocpg_get_bsr_async(ocpg_ctx* c, int32_t* IA, int32_t* JA, double* val, double* rhs)
+{
+    if (!c) return OCPG_EINVAL;
+    FimDevice& F = ctxFim(c);
+    if (F.nb <= 0) return OCPG_ESTATE;
+    HIPCHK2(hipSetDevice(ctxDevice(c)));
+    const size_t bs = (size_t)F.ncol * F.ncol;
+    hipStream_t s = ctxStream(c);
+    if (IA) HIPCHK2(hipMemcpyAsync(IA, F.IA, (F.nrow + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    if (JA) HIPCHK2(hipMemcpyAsync(JA, F.JA, F.nnzb * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    if (val) HIPCHK2(hipMemcpyAsync(val, F.val, F.nnzb * bs * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (rhs) HIPCHK2(hipMemcpyAsync(rhs, F.res, F.nrow * F.ncol * sizeof(double), hipMemcpyDeviceToHost, s));
+    return OCPG_OK;
+}
+
+// Page-locked host memory for the hand-off arrays (NULL on failure).
+void* ocpg_host_alloc(int64_t bytes)
+{
+    void* p = nullptr;
+    if (bytes <= 0 || hipHostMalloc(&p, (size_t)bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
+    return p;
+}
+
+int ocpg_host_free(void* p)
+{
+    if (!p) return OCPG_OK;
+    return hipHostFree(p) == hipSuccess ? OCPG_OK : OCPG_EHIP;
+}
+
 // BulkConn upwinding written by CalRes: upblock, upblock_Rho,
 // upblock_Velocity (nconn*np each).  Any pointer may be NULL.
 int ocpg_get_upwind(ocpg_ctx* c, int32_t* upblock, double* upRho, double* upVel)

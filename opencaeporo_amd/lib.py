@@ -70,6 +70,9 @@ def load():
         "ocpg_nr_change": ([P, P], C.c_int),
         "ocpg_get_well_bhp": ([P, P], C.c_int),
         "ocpg_get_bsr": ([P, P, P, P, P], C.c_int),
+        "ocpg_get_bsr_async": ([P, P, P, P, P], C.c_int),
+        "ocpg_host_alloc": ([C.c_int64], C.c_void_p),
+        "ocpg_host_free": ([P], C.c_int),
         "ocpg_get_upwind": ([P, P, P, P], C.c_int),
         "ocpg_save_flash_inputs": ([P], C.c_int),
         "ocpg_restore_flash_inputs": ([P], C.c_int),
@@ -266,6 +269,31 @@ class Context:
         check(self.lib.ocpg_get_bsr(self.h, IA.ctypes.data, JA.ctypes.data, val.ctypes.data,
                                     rhs.ctypes.data), "ocpg_get_bsr")
         return IA, JA, val, rhs
+
+    def pinned_bsr(self):
+        """Page-locked host arrays (IA, JA, val, rhs) for get_bsr_async; freed
+        with free_pinned."""
+        rows, nnzb, ncol = self.rows(), self.nnzb(), self.ncol
+        specs = [(rows + 1, np.int32), (nnzb, np.int32), (nnzb * ncol * ncol, np.float64),
+                 (rows * ncol, np.float64)]
+        out = []
+        for n, dt in specs:
+            nbytes = max(int(n) * np.dtype(dt).itemsize, 1)
+            p = self.lib.ocpg_host_alloc(nbytes)
+            if not p:
+                raise OcpgError("ocpg_host_alloc failed")
+            buf = (C.c_char * nbytes).from_address(p)
+            out.append((p, np.frombuffer(buf, dtype=dt, count=int(n))))
+        return out
+
+    def free_pinned(self, arrays):
+        for p, _ in arrays:
+            self.lib.ocpg_host_free(C.c_void_p(p))
+
+    def get_bsr_async(self, arrays):
+        """Queue the BSR hand-off into pinned arrays (pinned_bsr); wait with
+        synchronize()."""
+        check(self.lib.ocpg_get_bsr_async(self.h, *[C.c_void_p(p) for p, _ in arrays]), "ocpg_get_bsr_async")
 
     def get_upwind(self):
         n = self.grid.nconn * abi.NP

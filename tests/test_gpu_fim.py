@@ -91,3 +91,26 @@ def test_property_pass_and_assembly_from_primary_variables():
     assert rerr.max() <= 1e-10
     assert np.mean(err <= 1e-10) >= 0.99 and err.max() <= 1e-6
     ctx.close()
+
+
+def test_async_handoff_into_pinned_arrays():
+    """ocpg_get_bsr_async into page-locked arrays gives what ocpg_get_bsr gives."""
+    gpu_util.say("[fim] asynchronous BSR hand-off")
+    case = fim_case.build_case(8, 6, 5, seed=13)
+    b = case["bulk"]
+    ctx = lib.Context(case["eos"], case["sat"], case["rock"])
+    ctx.set_cells(b.nb)
+    ctx.upload(b)
+    ctx.set_conn(case["grid"])
+    ctx.calres(DT, norms=False)
+    ctx.assemble(DT)
+    IA, JA, val, rhs = ctx.get_bsr()
+    arrays = ctx.pinned_bsr()
+    try:
+        ctx.get_bsr_async(arrays)
+        ctx.synchronize()
+        for ref, (_, got) in zip((IA, JA, val, rhs), arrays):
+            assert np.array_equal(ref, got)
+    finally:
+        ctx.free_pinned(arrays)
+    ctx.close()
