@@ -378,6 +378,11 @@ struct AsmDims {
     static constexpr int maxRows = 3 + 2 * NC;          // compact secondary rows
     static constexpr int perLane = (bsize + kAsmBlock - 1) / kAsmBlock;
     static constexpr int dfStride = maxRows + 1;        // [i][0] = dFdXp, [i][1+k] = dFdXs
+    // lane stride of the x_ij columns in the assembly's second pass: a power
+    // of two >= 2 NC when that takes no more passes over the wavefront
+    static constexpr int xPow2 = 2 * NC <= 8 ? 8 : 2 * NC <= 16 ? 16 : 2 * NC <= 32 ? 32 : 64;
+    static constexpr int xStride =
+        (nc * xPow2 + kAsmBlock - 1) / kAsmBlock == (nc * 2 * NC + kAsmBlock - 1) / kAsmBlock ? xPow2 : 2 * NC;
 };
 
 // Entry offsets of the per-cell record (phase j, component i).
@@ -539,6 +544,25 @@ __global__ void __launch_bounds__(256) packRecordsListKernel(DevBulk B, FimDevic
     }
 }
 
+// A value that is the same on every lane, moved to scalar registers.
+__device__ __forceinline__ double uniformD(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b & 0xffffffffull));
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// v != 0.0 for a value that is the same on every lane, on its bits in
+// scalar registers (-0.0 is zero, a NaN is not, as for the comparison).
+__device__ __forceinline__ bool uniformNonzero(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b & 0xffffffffull));
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+    return (lo | (hi & 0x7fffffffu)) != 0;
+}
+
 template <int NC>
 struct AsmShared {
     using D = AsmDims<NC>;
@@ -615,47 +639,71 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
     }
 
     const int k0 = F.IA[n] + 1, k1 = F.bulkEnd[n];
+    // Slot indices and connection values are the same on every lane: they
+    // live in scalar registers, and the branches on them are scalar.
     double pre[R::perLane];
+    int64_t mNext = 0, cNext = 0;
     if (k0 < k1) {
-        const int64_t m = F.JA[k0];
+        mNext = __builtin_amdgcn_readfirstlane(F.JA[k0]);
+        cNext = __builtin_amdgcn_readfirstlane(F.slotConn[k0]);
 #pragma unroll
         for (int t = 0; t < R::perLane; ++t) {
             const int q = lane + t * kAsmBlock;
-            pre[t] = q < R::LEN ? F.rec[m * RS + q] : 0.0;
+            pre[t] = q < R::LEN ? F.rec[mNext * RS + q] : 0.0;
         }
     }
     for (int k = k0; k < k1; ++k) {
-        const int64_t m = F.JA[k];
-        const int64_t c = F.slotConn[k];
+        const int64_t m = mNext, c = cNext;
+        // One round trip for this connection's values and the next slot's
+        // indices, issued before the next neighbour's record is prefetched:
+        // loads complete in order, so waiting for these leaves that
+        // prefetch in flight while this connection is computed.
+        const double areaV = F.area[c];
+        int32_t upV[3];
+        double upRhoV[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            upV[j] = F.upblock[c * 3 + j];
+            upRhoV[j] = F.upRho[c * 3 + j];
+        }
+        const int32_t mirV = F.slotMirror[k];
+        const bool more = k + 1 < k1;
+        int32_t mNv = 0, cNv = 0;
+        if (more) {
+            mNv = F.JA[k + 1];
+            cNv = F.slotConn[k + 1];
+        }
 #pragma unroll
         for (int t = 0; t < R::perLane; ++t) {
             const int q = lane + t * kAsmBlock;
             if (q < R::LEN) sRec[1][q] = pre[t];
         }
         __syncthreads();
-        if (k + 1 < k1) {
-            const int64_t m2 = F.JA[k + 1];
+        mNext = __builtin_amdgcn_readfirstlane(mNv);
+        cNext = __builtin_amdgcn_readfirstlane(cNv);
+        if (more) {
 #pragma unroll
             for (int t = 0; t < R::perLane; ++t) {
                 const int q = lane + t * kAsmBlock;
-                pre[t] = q < R::LEN ? F.rec[m2 * RS + q] : 0.0;
+                pre[t] = q < R::LEN ? F.rec[mNext * RS + q] : 0.0;
             }
         }
         const bool ownB = n < m;  // this row is the connection's bId side
         const double* rB = ownB ? sRec[0] : sRec[1];
         const double* rE = ownB ? sRec[1] : sRec[0];
         const double* rO = sRec[0];
-        const double Akd = kCONV1f * kCONV2f * F.area[c];
-        const double dGamma = kGravityFactor * (rB[R::DEPTH] - rE[R::DEPTH]);
+        const double Akd = kCONV1f * kCONV2f * uniformD(areaV);
+        const double dGamma = kGravityFactor * (uniformD(rB[R::DEPTH]) - uniformD(rE[R::DEPTH]));
         bool exB[3], exE[3], upOwn[3];
         double upRho[3];
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            exB[j] = rB[R::EX + j] != 0.0;
-            exE[j] = rE[R::EX + j] != 0.0;
-            upOwn[j] = F.upblock[c * 3 + j] == (int32_t)n;
-            upRho[j] = F.upRho[c * 3 + j];
+            exB[j] = uniformNonzero(rB[R::EX + j]);
+            exE[j] = uniformNonzero(rE[R::EX + j]);
+            upOwn[j] = __builtin_amdgcn_readfirstlane(upV[j]) == (int32_t)n;
+            upRho[j] = uniformD(upRhoV[j]);
         }
+        const int64_t mirror = __builtin_amdgcn_readfirstlane(mirV);
 
         // ---- per phase: whether the upwind cell has it, dP and transJ ----
         // (lanes 0..2; every entry below reads them)
@@ -693,6 +741,8 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
                         s++;
                     }
             }
+            // saturation lanes read their column's dPc/dKr; the pressure lane reads column 0
+            const int s1 = j1S < 0 ? 0 : j1S;
             double v = 0;
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
@@ -705,12 +755,17 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
                 const double mu = rU[R::MU + j];
                 const double xij = rU[R::XIJ + j * nc + i];
                 const double transIJ = xij * xi * transJ;
+                const double muP = rU[R::MUP + j];
+                // The pressure entry's -transIJ * muP / mu and a saturation
+                // entry's Akd * xij * xi / mu share one division: the lanes
+                // pick their numerator (same operands, same rounding as the
+                // two separate expressions).
+                const double q = (col < 0 ? -transIJ * muP : Akd * xij * xi) / mu;
                 if (col < 0) {
-                    const double muP = rU[R::MUP + j];
                     const double xiP = rU[R::XIP + j];
                     const double rhoP = rU[R::RHOP + j];
                     double tmp = xij * transJ * xiP * dP;
-                    tmp += -transIJ * muP / mu * dP;
+                    tmp += q * dP;
                     if (ownB) {
                         v += transIJ;
                         if (!exE[j]) {
@@ -733,14 +788,14 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
                         }
                     }
                 } else {
-                    // saturation column of phase j1S
-                    const double dpc = sDpc[j * 3 + j1S];
+                    // saturation column of phase j1S (= s1 on these lanes)
+                    const double dpc = sDpc[j * 3 + s1];
                     if (ownB) {
                         v += transIJ * dpc;
-                        if (upB) v += Akd * xij * xi / mu * rU[R::DKR + j * 3 + j1S] * dP;
+                        if (upB) v += q * rU[R::DKR + j * 3 + s1] * dP;
                     } else {
                         v -= transIJ * dpc;
-                        if (!upB) v += Akd * xij * xi / mu * rU[R::DKR + j * 3 + j1S] * dP;
+                        if (!upB) v += q * rU[R::DKR + j * 3 + s1] * dP;
                     }
                 }
             }
@@ -750,7 +805,17 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
         // x_{kX, jX} columns: only phase jX's pass touches them.  Entry
         // (i, kx) at q = i * XS + kx with the compile-time stride XS >= nX.
         const int nX = max(ncolO - nsat, 0);
-        constexpr int XS = 2 * NC;
+        // per-phase flags as bit masks: the lanes of this pass index them by
+        // their own phase
+        unsigned liveM = 0, upOwnM = 0, exBM = 0, exEM = 0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            liveM |= live[j] ? 1u << j : 0u;
+            upOwnM |= upOwn[j] ? 1u << j : 0u;
+            exBM |= exB[j] ? 1u << j : 0u;
+            exEM |= exE[j] ? 1u << j : 0u;
+        }
+        constexpr int XS = D::xStride;
 #pragma unroll
         for (int t = 0; t < (nc * XS + kAsmBlock - 1) / kAsmBlock; ++t) {
             const int q = lane + t * kAsmBlock;
@@ -767,9 +832,10 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
                 }
             }
             double v = 0;
-            if (live[j]) {
-                const bool upB = ownB ? upOwn[j] : !upOwn[j];
-                const double* rU = upOwn[j] ? sRec[0] : sRec[1];
+            if ((liveM >> j) & 1u) {
+                const bool upOwnJ = (upOwnM >> j) & 1u;
+                const bool upB = ownB ? upOwnJ : !upOwnJ;
+                const double* rU = upOwnJ ? sRec[0] : sRec[1];
                 const double dP = sPh[0][j];
                 const double transJ = sPh[1][j];
                 const double xi = rU[R::XI + j];
@@ -777,7 +843,7 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
                 const double xij = rU[R::XIJ + j * nc + i];
                 const double transIJ = xij * xi * transJ;
                 const bool ownUp = ownB ? upB : !upB;
-                const bool otherEx = ownB ? exE[j] : exB[j];
+                const bool otherEx = (((ownB ? exEM : exBM) >> j) & 1u) != 0;
                 if (ownUp) {
                     const double rhox = otherEx ? rO[R::RHOX + j * nc + kk] / 2 : rU[R::RHOX + j * nc + kk];
                     const double xix = rU[R::XIX + j * nc + kk];
@@ -796,7 +862,6 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
         __syncthreads();
 
         // ---- block = (dFdXp + dFdXs * dSec_own) * dt, then scatter ----
-        const int64_t mirror = F.slotMirror[k];
         double* dst = F.val + mirror * bsize;
 #pragma unroll
         for (int t = 0; t < D::perLane; ++t) {
