@@ -51,25 +51,36 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def setup(nx, ny, nz, seed=2024, irregular=False):
+def setup(nx, ny, nz, seed=2024, irregular=False, model=None):
+    """model: a deck.Model (the workload read from an input deck) instead of
+    the synthetic generators."""
     t0 = time.time()
-    deck = fluids.ten_deck() if irregular else fluids.bench6_deck()
-    eos = lib.eos_from_deck(deck)
-    sat = fluids.make_satfunc()
-    rock = fluids.make_rock()
+    if model is not None:
+        eos = lib.eos_from_deck(model.eos_deck)
+        sat, rock, grid = model.sat, model.rock, model.grid
+    else:
+        deck = fluids.ten_deck() if irregular else fluids.bench6_deck()
+        eos = lib.eos_from_deck(deck)
+        sat = fluids.make_satfunc()
+        rock = fluids.make_rock()
+        grid = (workloads.faulted_grid(nx, ny, nz, seed=seed, frac=0.05) if irregular
+                else workloads.cartesian_grid(nx, ny, nz, seed=seed))
     NC = eos.nc
-    grid = (workloads.faulted_grid(nx, ny, nz, seed=seed, frac=0.05) if irregular
-            else workloads.cartesian_grid(nx, ny, nz, seed=seed))
     nb = grid.nb
-    P, z, Sw = workloads.cell_inputs(nb, NC, seed=seed + 1)
     fields = ["P", "T", "S", "v", "depth", "poroInit"]
     b = abi.Bulk(nb, NC, fields=fields)
-    b.P[:] = P
-    b.T[:] = workloads.T_F
-    b.S.reshape(nb, 3)[:, 2] = Sw
-    b.v[:] = grid.v
-    b.depth[:] = grid.depth
-    b.poroInit[:] = 0.3
+    if model is not None:
+        z = model.z
+        for f in fields:
+            getattr(b, f)[:] = getattr(model.bulk, f)
+    else:
+        P, z, Sw = workloads.cell_inputs(nb, NC, seed=seed + 1)
+        b.P[:] = P
+        b.T[:] = workloads.T_F
+        b.S.reshape(nb, 3)[:, 2] = Sw
+        b.v[:] = grid.v
+        b.depth[:] = grid.depth
+        b.poroInit[:] = 0.3
     ctx = lib.Context(eos, sat, rock)
     ctx.set_cells(nb)
     ctx.upload(b)
@@ -93,8 +104,11 @@ def setup(nx, ny, nz, seed=2024, irregular=False):
     pb.P[:] += rng.uniform(-5.0, 5.0, nb)
     ctx.upload(pb)
     wells = None
-    if irregular:
+    if model is not None:
+        wells = model.wells
+    elif irregular:
         wells = workloads.irregular_wells(grid, pb.P, NC, seed + 3)
+    if wells is not None:
         ctx.set_wells(wells)
     ctx.calkrpc_rock()
     ctx.calres(DT, norms=False)
@@ -182,6 +196,9 @@ def main():
     ap.add_argument("--irregular", action="store_true",
                     help="the irregular-stencil configuration: 10-component fluid, faulted grid with "
                          "non-neighbour connections, 40 wells (a parity/measurement case, not the headline)")
+    ap.add_argument("--deck", default=None,
+                    help="read the workload from an input deck (opencaeporo_amd/deck.py), e.g. "
+                         "tests/golden/decks/spe5refine/spe5-70x70x30-2y.data (a measurement case, not the headline)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
@@ -189,7 +206,13 @@ def main():
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
     nx, ny, nz = 60, 220, a.nz
-    ctx, grid, nb, NC, wells = setup(nx, ny, nz, irregular=a.irregular)
+    model = None
+    if a.deck:
+        from opencaeporo_amd import deck
+        model = deck.load(a.deck)
+        nx, ny, nz = model.deck.dims
+    side_case = a.irregular or model is not None
+    ctx, grid, nb, NC, wells = setup(nx, ny, nz, irregular=a.irregular, model=model)
     for _ in range(a.warmup):
         step(ctx, a.serial)
     ctx.synchronize()
@@ -260,7 +283,7 @@ def main():
                      peak=HBM_PEAK_GBS, unit="GB/s", traffic=None, bytes_per_launch=pack_bytes)
     for r in (roof, roof_pack):
         r["frac"] = r["achieved"] / r["peak"]
-    if not a.irregular and nz == 85:
+    if not side_case and nz == 85:
         # HBM bytes per launch measured by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
         # on this workload, scaled by the copy-kernel calibration
         # (profiles/assembly_pack_1p1M_6comp_traffic.md)
@@ -268,12 +291,16 @@ def main():
         roof_pack["traffic"] = PACK_TRAFFIC_BYTES
     other = [roof_pack]
     cpu = None
-    if rank == 0 and not a.no_cpu and not a.irregular:
+    if rank == 0 and not a.no_cpu and not side_case:
         try:
             cpu = cpu_baseline()
         except Exception as e:  # the baseline is informative only
             log(f"[bench] cpu baseline failed: {e}")
-    if a.irregular:
+    if model is not None:
+        workload = (f"input deck {os.path.basename(a.deck)}: {nx}x{ny}x{nz} grid, {nb} cells, "
+                    f"{grid.nconn} connections, {NC}-comp PR, {wells.nw if wells else 0} wells / "
+                    f"{wells.nperf if wells else 0} perforations, warm Newton pass, {ncol}x{ncol} BSR")
+    elif a.irregular:
         workload = (f"irregular stencil: {nx}x{ny}x{nz} faulted grid ({grid.nconn} connections incl. "
                     f"non-neighbour), 10-comp PR, {wells.nw} wells / {wells.nperf} perforations, warm "
                     f"Newton pass, {ncol}x{ncol} BSR")
@@ -281,7 +308,8 @@ def main():
         workload = (f"standalone flash+assembly, {nx}x{ny}x{nz} grid, {nb} cells, "
                     f"{grid.nconn} connections, 6-comp PR, warm Newton pass, 8x8 BSR")
     out = {
-        "metric": ("cell flash+Jacobian updates/s (1.1M cells, 10-comp, irregular, wells)" if a.irregular
+        "metric": ("cell flash+Jacobian updates/s (input deck)" if model is not None
+                   else "cell flash+Jacobian updates/s (1.1M cells, 10-comp, irregular, wells)" if a.irregular
                    else "cell flash+Jacobian updates/s (1.1M cells, 6-comp)"),
         "value": value,
         "unit": "cell-updates/s",
@@ -293,7 +321,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic",
+        "data": "input deck" if model is not None else "synthetic",
         "config": {"workload": workload,
                    "cells": nb, "connections": grid.nconn, "block": ncol,
                    "kernel_ms": per, "dominant_kernel": dominant, "flash_ms": flash_ms,

@@ -1,0 +1,140 @@
+"""The deck reader (opencaeporo_amd/deck.py) on the reference's SPE5 decks,
+pinned by what the decks and the mathematics fix: the fluid equals the
+SPE5 constants the flash fixtures were made with (tests/golden, from the
+reference's own EoS reader), cell geometry and transmissibilities equal
+their closed forms on these uniform blocks, Peaceman's well index its
+closed form for an isotropic horizontal permeability, and the table
+interpolation the two-point line through the rows around the depth."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from opencaeporo_amd import abi, deck, fluids
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SPE5 = os.path.join(HERE, "golden", "decks", "spe5", "spe5.data")
+REFINE = os.path.join(HERE, "golden", "decks", "spe5refine", "spe5-70x70x30-2y.data")
+
+
+def test_tokens_and_defaults():
+    assert deck._tokens("  'PROD1'   2*   3   3   1*   0.5   3*   / trailing text") == \
+        ["PROD1", "2*", "3", "3", "1*", "0.5", "3*", "/"]
+    assert deck._tokens("-- a comment") == [] and deck._tokens("# another") == []
+    assert deck._tokens("70 70 30 / -- origin") == ["70", "70", "30", "/"]
+    assert deck._defaults(["A", "2*", "3*7", "x"]) == ["A", None, None, "7", "7", "7", "x"]
+
+
+def test_spe5_fluid_and_tables_are_the_pinned_ones():
+    m = deck.load(SPE5)
+    assert m.NC == 6 and m.deck.names == fluids.SPE5["names"]
+    for k in ("Tc", "Pc", "Zc", "MW", "Acf"):
+        assert m.comp[k] == fluids.SPE5[k], k
+    assert m.bic == fluids.SPE5["BIC_lower"]
+    assert m.controls == fluids.EOS_CONTROLS
+    assert m.pvtw == fluids.PVTW_SPE5
+    assert m.deck.tables["SWOF"] == fluids.SWOF_SPE5
+    assert m.deck.tables["SGOF"] == fluids.SGOF_SPE5
+    assert m.deck.rock == fluids.ROCK_SPE5
+    # the EosDeck the library reads is the one the SPE5 fixtures use
+    ref = fluids.spe5_deck()
+    for f, _ in abi.EosDeck._fields_:
+        a, b = getattr(m.eos_deck, f), getattr(ref, f)
+        if hasattr(a, "__len__"):
+            assert list(np.ravel(a)) == list(np.ravel(b)), f
+        else:
+            assert a == b, f
+
+
+def test_spe5_grid_closed_forms():
+    m = deck.load(SPE5)
+    g = m.grid
+    nx, ny, nz = 7, 7, 3
+    assert (g.nx, g.ny, g.nz) == (nx, ny, nz)
+    dz = np.repeat([20.0, 30.0, 50.0], nx * ny)
+    np.testing.assert_array_equal(g.v, 500.0 * 500.0 * dz)
+    np.testing.assert_array_equal(g.depth, np.repeat([8335.0, 8360.0, 8400.0], nx * ny))
+    # 6x7x3 + 7x6x3 lateral and 7x7x2 vertical connections
+    assert g.nconn == 2 * 6 * 7 * 3 + 7 * 7 * 2
+    kx = np.repeat([500.0, 50.0, 200.0], nx * ny)
+    kz = np.repeat([50.0, 50.0, 25.0], nx * ny)
+    for b_, e_, a in zip(g.bId, g.eId, g.area):
+        if e_ - b_ in (1, nx):             # lateral: T = k * dz (both halves the same)
+            T = kx[b_] * 2 * 500.0 * dz[b_] / 500.0
+            assert a == pytest.approx(T / 2, rel=1e-15)
+        else:                               # vertical: series of two half-cells
+            T1 = kz[b_] * 2 * 500.0 * 500.0 / dz[b_]
+            T2 = kz[e_] * 2 * 500.0 * 500.0 / dz[e_]
+            assert a == pytest.approx(T1 * T2 / (T1 + T2), rel=1e-14)
+    np.testing.assert_array_equal(m.bulk.poroInit, 0.3)
+    np.testing.assert_array_equal(m.bulk.P, 4000.0)
+    np.testing.assert_array_equal(m.bulk.T, 160.0)
+    np.testing.assert_array_equal(m.bulk.S.reshape(-1, 3)[:, 2], 0.2)
+    # ZMFVD is flat in depth here
+    np.testing.assert_array_equal(m.z, np.tile([0.5, 0.03, 0.07, 0.2, 0.15, 0.05], (g.nb, 1)))
+
+
+def test_spe5_well_from_the_first_step():
+    m = deck.load(SPE5)
+    w = m.wells
+    assert w.nw == 1 and list(w.perfCell) == [m.cell(7, 7, 3)] == [146]
+    assert w.type[0] == abi.WELL_PROD and w.mode[0] == abi.WELL_RATE
+    assert w.ctrlRate[0] == 12000.0 and w.ctrlBHP[0] == 1000.0 and w.bhp[0] == 4000.0
+    np.testing.assert_array_equal(w.prodPhaseWeight, [1.0, 0.0, 0.0])
+    # Peaceman, vertical well, kx = ky = k: r_o = 0.14 sqrt(dx^2 + dy^2), kh = k dz
+    ro = 0.14 * math.hypot(500.0, 500.0)
+    wi = 1.12712e-3 * 2 * 3.141592653 * (200.0 * 50.0) / math.log(ro / 0.25)
+    assert w.perfWI[0] == pytest.approx(wi, rel=1e-14)
+
+
+def test_refined_deck():
+    m = deck.load(REFINE)
+    nx, ny, nz = 70, 70, 30
+    g = m.grid
+    assert g.nb == 147000 and g.nconn == 2 * 69 * 70 * 30 + 70 * 70 * 29
+    layer = np.arange(g.nb) // (nx * ny)
+    np.testing.assert_array_equal(g.v, 50.0 * 50.0 * np.where(layer < 10, 2.0, np.where(layer < 20, 3.0, 5.0)))
+    assert g.depth[0] == 8326.0 and g.depth[-1] == pytest.approx(8325 + 20 + 30 + 50 - 2.5, rel=1e-15)
+    w = m.wells
+    assert list(w.perfCell) == [m.cell(70, 70, k) for k in range(21, 31)]
+    ro = 0.14 * math.hypot(50.0, 50.0)
+    wi = 1.12712e-3 * 2 * 3.141592653 * (200.0 * 5.0) / math.log(ro / 0.25)
+    np.testing.assert_allclose(w.perfWI, wi, rtol=1e-14)
+    np.testing.assert_array_equal(m.z[0], [0.5, 0.03, 0.07, 0.2, 0.15, 0.05])
+
+
+def test_table_evaluation():
+    t = [[1000.0, 0.1, 5.0], [2000.0, 0.3, 4.0], [4000.0, 0.7, 0.0]]
+    np.testing.assert_array_equal(deck._eval0(t, 500.0), [0.1, 5.0])
+    np.testing.assert_array_equal(deck._eval0(t, 4000.0), [0.7, 0.0])
+    np.testing.assert_array_equal(deck._eval0(t, 9000.0), [0.7, 0.0])
+    for x in (1000.0, 1500.0, 2500.0, 3999.0):
+        np.testing.assert_allclose(deck._eval0(t, x),
+                                   [np.interp(x, [r[0] for r in t], [r[c] for r in t]) for c in (1, 2)],
+                                   rtol=1e-15, atol=1e-15)
+
+
+def _write(tmp_path, text, eos=None):
+    """A deck in tmp_path beside a copy of the SPE5 EGOIL.in (INCLUDE takes
+    a relative name: '/' ends a record)."""
+    inc = os.path.join(os.path.dirname(SPE5), "EGOIL.in")
+    (tmp_path / "EGOIL.in").write_text(eos if eos is not None else open(inc).read())
+    p = tmp_path / "d.data"
+    p.write_text(text)
+    return str(p)
+
+
+def test_unsupported_decks_are_rejected(tmp_path):
+    base = open(SPE5, encoding="latin-1").read()
+    with pytest.raises(deck.DeckError, match="corner-point"):
+        deck.load(_write(tmp_path, base.replace("GRID ", "GRID\nCOORD\n1 2 3 /\n/\n", 1)))
+    with pytest.raises(deck.DeckError, match="ACTNUM"):
+        deck.load(_write(tmp_path, base.replace("GRID ", "GRID\nACTNUM\n147*1 /\n", 1)))
+    with pytest.raises(deck.DeckError, match="inactive"):
+        deck.load(_write(tmp_path, base.replace("'PORO'   0.3   4*  1  3", "'PORO'   0.0   4*  1  3")))
+    eos = open(os.path.join(os.path.dirname(SPE5), "EGOIL.in")).read()
+    with pytest.raises(deck.DeckError, match="NRSP"):
+        deck.load(_write(tmp_path, base, eos=eos[:eos.index("NRSP")]))
+    # the unchanged deck from the same place reads
+    assert deck.load(_write(tmp_path, base)).grid.nb == 147
