@@ -383,6 +383,8 @@ struct AsmDims {
     static constexpr int xPow2 = 2 * NC <= 8 ? 8 : 2 * NC <= 16 ? 16 : 2 * NC <= 32 ? 32 : 64;
     static constexpr int xStride =
         (nc * xPow2 + kAsmBlock - 1) / kAsmBlock == (nc * 2 * NC + kAsmBlock - 1) / kAsmBlock ? xPow2 : 2 * NC;
+    // the same for rows whose cell has one hydrocarbon phase (NC x columns)
+    static constexpr int xStride1 = NC <= 8 ? 8 : NC <= 16 ? 16 : 32;
 };
 
 // Entry offsets of the per-cell record (phase j, component i).
@@ -803,7 +805,9 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
             }
         }
         // x_{kX, jX} columns: only phase jX's pass touches them.  Entry
-        // (i, kx) at q = i * XS + kx with the compile-time stride XS >= nX.
+        // (i, kx) at q = i * XS + kx with a compile-time stride XS >= nX:
+        // NC rounded up to a power of two when the row's cell has one
+        // hydrocarbon phase (nX = NC), the wider stride otherwise (nX = 2 NC).
         const int nX = max(ncolO - nsat, 0);
         // per-phase flags as bit masks: the lanes of this pass index them by
         // their own phase
@@ -815,13 +819,7 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
             exBM |= exB[j] ? 1u << j : 0u;
             exEM |= exE[j] ? 1u << j : 0u;
         }
-        constexpr int XS = D::xStride;
-#pragma unroll
-        for (int t = 0; t < (nc * XS + kAsmBlock - 1) / kAsmBlock; ++t) {
-            const int q = lane + t * kAsmBlock;
-            const int i = q / XS;
-            const int kx = q % XS;
-            if (i >= nc || kx >= nX) continue;
+        auto xEntry = [&](int i, int kx) {
             int kk = kx, j = 0;
             if (kk >= pvO[0]) {
                 kk -= pvO[0];
@@ -858,6 +856,23 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
                 }
             }
             sDF[i * W + 1 + nsat + kx] = v;
+        };
+        if (nX <= NC) {
+            constexpr int XS = D::xStride1;
+#pragma unroll
+            for (int t = 0; t < (nc * XS + kAsmBlock - 1) / kAsmBlock; ++t) {
+                const int q = lane + t * kAsmBlock;
+                const int i = q / XS, kx = q % XS;
+                if (i < nc && kx < nX) xEntry(i, kx);
+            }
+        } else {
+            constexpr int XS = D::xStride;
+#pragma unroll
+            for (int t = 0; t < (nc * XS + kAsmBlock - 1) / kAsmBlock; ++t) {
+                const int q = lane + t * kAsmBlock;
+                const int i = q / XS, kx = q % XS;
+                if (i < nc && kx < nX) xEntry(i, kx);
+            }
         }
         __syncthreads();
 
@@ -900,7 +915,7 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
 }
 
 template <int NC>
-__global__ void __launch_bounds__(kAsmBlock) assembleKernel(DevBulk B, FimDevice F, double dt)
+__global__ void __launch_bounds__(kAsmBlock) __attribute__((amdgpu_waves_per_eu(NC <= 6 ? 6 : 4))) assembleKernel(DevBulk B, FimDevice F, double dt)
 {
     __shared__ AsmShared<NC> sh;
     // workgroup b runs on XCD b % kXcds; XCD x takes rows [x*chunk, (x+1)*chunk)
