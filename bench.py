@@ -43,7 +43,7 @@ WARM_FLOPS_PER_CELL = 21584690202 / 1122000
 # workload: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, scaled by what the
 # counters read for a device-to-device copy of known size
 # (profiles/assembly_pack_1p1M_6comp_traffic.md)
-ASM_TRAFFIC_BYTES = 5.089e9 + 3.983e9
+ASM_TRAFFIC_BYTES = 5.120e9 + 3.983e9
 PACK_TRAFFIC_BYTES = 2.378e9 + 2.165e9
 
 
@@ -196,6 +196,9 @@ def main():
     ap.add_argument("--irregular", action="store_true",
                     help="the irregular-stencil configuration: 10-component fluid, faulted grid with "
                          "non-neighbour connections, 40 wells (a parity/measurement case, not the headline)")
+    ap.add_argument("--serving-requests", type=int, default=96,
+                    help="requests of the serving-form leg (p50/p99 latency, continuous batching vs one at a "
+                         "time, 32 realisations of 1e3..1e5 cells); 0 skips it")
     ap.add_argument("--deck", default=None,
                     help="read the workload from an input deck (opencaeporo_amd/deck.py), e.g. "
                          "tests/golden/decks/spe5refine/spe5-70x70x30-2y.data (a measurement case, not the headline)")
@@ -332,8 +335,26 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
     ctx.close()
+    if rank == 0 and a.serving_requests > 0 and not side_case and not a.serial:
+        # the serving form beside the headline: one Newton iteration per
+        # request of one of many resident realisations (opencaeporo_amd/serving.py)
+        try:
+            from opencaeporo_amd import serving
+            # offered load 1.2x what one-at-a-time service sustains: the
+            # regime where overlapping realisations decides the latency
+            sv = serving.benchmark(n_real=32, cells_lo=1e3, cells_hi=1e5, n_req=a.serving_requests,
+                                   budget_gb=8.0, log=log, load=1.2)
+            out["config"]["serving"] = {
+                "realisations": 32, "cells": "log-uniform in [1e3, 1e5]", "requests": a.serving_requests,
+                "offered_rate_per_s": sv["config"]["rate_per_s"], "offered_load_vs_one_at_a_time": 1.2,
+                "continuous": {k: sv["continuous"][k] for k in ("p50_ms", "p99_ms", "throughput_req_s",
+                                                                "cell_updates_s", "max_concurrent")},
+                "one_at_a_time": {k: sv["one_at_a_time"][k] for k in ("p50_ms", "p99_ms", "throughput_req_s",
+                                                                      "cell_updates_s")}}
+        except Exception as e:  # informative only
+            log(f"[bench] serving leg failed: {e}")
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -229,7 +229,9 @@ def summarize(reqs, ens):
 
 
 def benchmark(n_real=64, cells_lo=1e3, cells_hi=2e5, n_req=256, rate=None, budget_gb=16.0, seed=7,
-              log=lambda *a: None):
+              log=lambda *a: None, load=0.8):
+    """load: the offered arrival rate as a fraction of the one-at-a-time
+    capacity measured on the same ensemble (used when rate is None)."""
     deck = fluids.bench6_deck()
     eos = lib.eos_from_deck(deck)
     sat = fluids.make_satfunc()
@@ -253,15 +255,15 @@ def benchmark(n_real=64, cells_lo=1e3, cells_hi=2e5, n_req=256, rate=None, budge
     probe.evict()
     log(f"[serving] largest realisation {probe.nb} cells: {largest_ms:.2f} ms per iteration")
     if rate is None:
-        # offered load: 80% of what one-at-a-time service sustains on this
-        # ensemble (measured with every request queued at once)
+        # offered load: a fraction of what one-at-a-time service sustains on
+        # this ensemble (measured with every request queued at once)
         for r in ens:
             r.busy = False
         burst = [serving_request for serving_request in poisson_requests(n_req, 1e9, n_real, seed + 5)]
         srv = Server(ens, budget_gb * 1e9, max_inflight=1)
         srv.run(burst)
         cap = summarize(burst, {r.rid: r for r in ens})["throughput_req_s"]
-        rate = 0.8 * cap
+        rate = load * cap
         log(f"[serving] one-at-a-time capacity {cap:.1f} requests/s; offered load {rate:.1f}/s")
     out = {}
     for mode, mi in (("continuous", None), ("one_at_a_time", 1)):
@@ -281,7 +283,7 @@ def benchmark(n_real=64, cells_lo=1e3, cells_hi=2e5, n_req=256, rate=None, budge
         if r.resident:
             r.evict()
     out["config"] = dict(realisations=n_real, cells_lo=cells_lo, cells_hi=cells_hi, total_cells=total,
-                         requests=n_req, rate_per_s=rate, budget_gb=budget_gb)
+                         requests=n_req, rate_per_s=rate, budget_gb=budget_gb, load=load)
     return out
 
 
