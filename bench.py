@@ -204,8 +204,9 @@ def main():
                          "tests/golden/decks/spe5refine/spe5-70x70x30-2y.data (a measurement case, not the headline)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
-              "(see DESIGN.md for how the path would shard)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (across GPUs each would "
+              "take a slab of cells and exchange the records of its boundary layer once per Newton "
+              "iteration)", file=sys.stderr)
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
     nx, ny, nz = 60, 220, a.nz
