@@ -148,3 +148,17 @@ def test_overlapped_with_wells_ten_components():
     ctx.set_wells(wells)
     compare(ctx, grid, NC, wells=wells, label="wells")
     ctx.close()
+
+
+def test_overlapped_irregular_full_size():
+    """The irregular-stencil configuration at full size (1,122,000 cells,
+    faulted 10-component field, 40 wells): its Newton-restart list is long
+    (thousands of cells), and side stream D works it one wavefront per cell
+    over the whole chip while the main stream's kernels wait for CUs."""
+    gpu_util.say("[overlap] irregular stencil, faulted 60x220x85, 10-comp, 40 wells")
+    ctx, grid, NC = make_field(60, 220, 85, seed=2024, irregular=True)
+    P = warm_up(ctx, grid, NC, seed=2026)
+    wells = workloads.irregular_wells(grid, P, NC, seed=2027)
+    ctx.set_wells(wells)
+    compare(ctx, grid, NC, wells=wells, label="irregular")
+    ctx.close()
