@@ -500,26 +500,82 @@ __global__ void __launch_bounds__(kPackBlock) packRecordsKernel(DevBulk B, FimDe
     if (threadIdx.x < kPackTile)
         rows[threadIdx.x] = threadIdx.x < tile ? min(max(B.bRowSizedSdP[n0 + threadIdx.x], 0),
                                                      AsmDims<NC>::maxRows) : 0;
-    // neighbour record
-    for (int i = threadIdx.x; i < R::LEN * kPackTile; i += kPackBlock) {
-        const int q = i / kPackTile, c = i % kPackTile;
-        if (c < tile) t[c * L + q] = loadRec(recSource<NC>(B, q), n0 + c);
+    // Each field's base for this block's cells, once per block: the loads
+    // below are then branch-free with compile-time trip counts, so a
+    // thread's loads are all in flight before the first is waited for.
+    // Entries past the tile (c >= tile) read the padding up to ld and are
+    // not written back.
+    __shared__ const double* srcR[R::EX];
+    __shared__ const double* srcO[O::SDEX];
+    for (int q = threadIdx.x; q < R::EX; q += kPackBlock) srcR[q] = recSource<NC>(B, q).d + n0;
+    for (int q = threadIdx.x; q < O::SDEX; q += kPackBlock) {
+        const int64_t ld = B.ld;
+        const double* p;
+        if (q < O::DPC) p = B.dSec + (int64_t)q * ld;
+        else if (q < O::VFI) p = B.dPcj_dS + (int64_t)(q - O::DPC) * ld;
+        else if (q < O::VFP) p = B.vfi + (int64_t)(q - O::VFI) * ld;
+        else if (q == O::VFP) p = B.vfP;
+        else if (q == O::V) p = B.v;
+        else p = B.poroP;
+        srcO[q] = p + n0;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < RS * kPackTile; i += kPackBlock) {
+    // neighbour record: the double fields, then the three phase flags
+    {
+        // all loads first, then all LDS stores (generic loads may alias LDS:
+        // interleaved, each load would be waited for before the next store)
+        constexpr int K = (R::EX * kPackTile + kPackBlock - 1) / kPackBlock;
+        double v[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int i = threadIdx.x + k * kPackBlock;
+            v[k] = i < R::EX * kPackTile ? srcR[i / kPackTile][i % kPackTile] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int i = threadIdx.x + k * kPackBlock;
+            if (i < R::EX * kPackTile) t[(i % kPackTile) * L + i / kPackTile] = v[k];
+        }
+    }
+    if (threadIdx.x < 3 * kPackTile) {
+        const int q = R::EX + threadIdx.x / kPackTile, c = threadIdx.x % kPackTile;
+        t[c * L + q] = B.phaseExist[(int64_t)(q - R::EX) * B.ld + n0 + c] ? 1.0 : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < (RS * kPackTile + kPackBlock - 1) / kPackBlock; ++k) {
+        const int i = threadIdx.x + k * kPackBlock;
         const int c = i / RS, q = i % RS;
-        if (c < tile && q < R::LEN) F.rec[(n0 + c) * RS + q] = t[c * L + q];
+        if (i < RS * kPackTile && c < tile && q < R::LEN) F.rec[(n0 + c) * RS + q] = t[c * L + q];
     }
     __syncthreads();
-    // own-row record
-    for (int i = threadIdx.x; i < O::LEN * kPackTile; i += kPackBlock) {
-        const int q = i / kPackTile, c = i % kPackTile;
-        if (c < tile && (q >= O::DPC || q < rows[c] * ncol)) t[c * L + q] = ownValue<NC>(B, q, n0 + c);
+    // own-row record: the double fields (unused dSec rows are read and not
+    // written back), then the seven counts
+    {
+        constexpr int K = (O::SDEX * kPackTile + kPackBlock - 1) / kPackBlock;
+        double v[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int i = threadIdx.x + k * kPackBlock;
+            v[k] = i < O::SDEX * kPackTile ? srcO[i / kPackTile][i % kPackTile] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int i = threadIdx.x + k * kPackBlock;
+            if (i < O::SDEX * kPackTile) t[(i % kPackTile) * L + i / kPackTile] = v[k];
+        }
+    }
+    if (threadIdx.x < (O::LEN - O::SDEX) * kPackTile) {
+        const int q = O::SDEX + threadIdx.x / kPackTile, c = threadIdx.x % kPackTile;
+        if (c < tile) t[c * L + q] = ownValue<NC>(B, q, n0 + c);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < OS * kPackTile; i += kPackBlock) {
+#pragma unroll
+    for (int k = 0; k < (OS * kPackTile + kPackBlock - 1) / kPackBlock; ++k) {
+        const int i = threadIdx.x + k * kPackBlock;
         const int c = i / OS, q = i % OS;
-        if (c < tile && q < O::LEN && (q >= O::DPC || q < rows[c] * ncol)) F.own[(n0 + c) * OS + q] = t[c * L + q];
+        if (i < OS * kPackTile && c < tile && q < O::LEN && (q >= O::DPC || q < rows[c] * ncol))
+            F.own[(n0 + c) * OS + q] = t[c * L + q];
     }
 }
 
