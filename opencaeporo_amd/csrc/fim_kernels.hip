@@ -479,11 +479,18 @@ __device__ __forceinline__ double ownValue(const DevBulk& B, int q, int64_t n)
     return (double)B.bRowSizedSdP[n];
 }
 
-// Pack both records of kPackTile consecutive cells: read field by field
-// (each field's kPackTile values are one 128-byte line), transpose in LDS,
+// Pack both records of packTile<NC>() consecutive cells: read field by field
+// (each field's kPackTile values are contiguous), transpose in LDS,
 // write the cells' records as contiguous spans.
-constexpr int kPackTile = 16;
+// Cells per block: 32 for the 6-component records; 16 for larger ones
+// (their tile in LDS and the loads in flight per thread grow with NC).
+template <int NC>
+constexpr int packTile() { return NC <= 6 ? 32 : 16; }
 constexpr int kPackBlock = 256;
+// every tile ends inside the padded leading dimension (a multiple of 64),
+// and the flag and count passes take one pass over the block
+static_assert(64 % packTile<6>() == 0 && 64 % packTile<10>() == 0, "a tile must not cross the padding of ld");
+static_assert(7 * packTile<6>() <= kPackBlock && 7 * packTile<10>() <= kPackBlock, "flag/count pass exceeds the block");
 
 template <int NC>
 __global__ void __launch_bounds__(kPackBlock) packRecordsKernel(DevBulk B, FimDevice F)
@@ -492,6 +499,7 @@ __global__ void __launch_bounds__(kPackBlock) packRecordsKernel(DevBulk B, FimDe
     using O = AsmOwn<NC>;
     constexpr int ncol = AsmDims<NC>::ncol;
     constexpr int RS = recStride<NC>(), OS = ownStride<NC>();
+    constexpr int kPackTile = packTile<NC>();
     constexpr int L = (R::LEN > O::LEN ? R::LEN : O::LEN) + 1;  // odd LDS row: no bank conflicts
     __shared__ double t[kPackTile * L];
     __shared__ int rows[kPackTile];
@@ -1271,7 +1279,7 @@ hipError_t launchAssembleRows(int NC, const DevBulk& B, FimDevice& F, double dt,
 template <int NC>
 static hipError_t launchPackT(const DevBulk& B, FimDevice& F, hipStream_t s)
 {
-    const unsigned grid = (unsigned)((B.nb + kPackTile - 1) / kPackTile);
+    const unsigned grid = (unsigned)((B.nb + packTile<NC>() - 1) / packTile<NC>());
     hipLaunchKernelGGL(packRecordsKernel<NC>, dim3(grid), dim3(kPackBlock), 0, s, B, F);
     return hipGetLastError();
 }
