@@ -44,7 +44,7 @@ WARM_FLOPS_PER_CELL = 21584690202 / 1122000
 # counters read for a device-to-device copy of known size
 # (profiles/assembly_pack_1p1M_6comp_traffic.md)
 ASM_TRAFFIC_BYTES = 5.120e9 + 3.983e9
-PACK_TRAFFIC_BYTES = 2.378e9 + 2.165e9
+PACK_TRAFFIC_BYTES = 2.329e9 + 2.165e9
 
 
 def log(*a):
