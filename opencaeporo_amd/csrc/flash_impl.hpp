@@ -21,7 +21,7 @@ struct FlashKernel {
     // CalMuPXLBC_partial (:3037-3213), pSderExist/pVnumCom (:559-570),
     // CalSkipForNextStep (:5155-5254), then PassFlashValue's stores.
     template <int NPT>
-    __device__ static void finish(FlashCell<NC>& F, const DevEos& E, const DevBulk& B, int64_t cell,
+    __device__ __forceinline__ static void finish(FlashCell<NC>& F, const DevEos& E, const DevBulk& B, int64_t cell,
                                   bool init, double Sw_in, double Vpore)
     {
         const int64_t ld = B.ld;
