@@ -20,9 +20,29 @@ struct FlashKernel {
     // CaldXsdXpAPI02p (:4386-4493), CalXiPNX_partial / CalRhoPX_partial /
     // CalMuPXLBC_partial (:3037-3213), pSderExist/pVnumCom (:559-570),
     // CalSkipForNextStep (:5155-5254), then PassFlashValue's stores.
+    //
+    // Inlined into its kernels for NC <= 6 (the cell object then stays in
+    // registers); the larger instantiations keep it out of line.
     template <int NPT>
     __device__ __forceinline__ static void finish(FlashCell<NC>& F, const DevEos& E, const DevBulk& B, int64_t cell,
-                                  bool init, double Sw_in, double Vpore)
+                                                  bool init, double Sw_in, double Vpore)
+    {
+        if constexpr (NC <= 6)
+            finishBody<NPT>(F, E, B, cell, init, Sw_in, Vpore);
+        else
+            finishCall<NPT>(F, E, B, cell, init, Sw_in, Vpore);
+    }
+
+    template <int NPT>
+    __device__ __noinline__ static void finishCall(FlashCell<NC>& F, const DevEos& E, const DevBulk& B,
+                                                   int64_t cell, bool init, double Sw_in, double Vpore)
+    {
+        finishBody<NPT>(F, E, B, cell, init, Sw_in, Vpore);
+    }
+
+    template <int NPT>
+    __device__ __forceinline__ static void finishBody(FlashCell<NC>& F, const DevEos& E, const DevBulk& B,
+                                                      int64_t cell, bool init, double Sw_in, double Vpore)
     {
         const int64_t ld = B.ld;
         constexpr int NP = NPT;
