@@ -29,6 +29,7 @@ void ctxTimerBegin(ocpg_ctx* c, int slot);
 void ctxTimerEnd(ocpg_ctx* c, int slot);
 int fimReadNorms(ocpg_ctx* c, ocpg_resnorm* norms);
 NewtonDevice& ctxNewton(ocpg_ctx* c);
+void ctxDropGraph(ocpg_ctx* c);
 }  // namespace ocpg
 
 
@@ -103,6 +104,7 @@ static int buildPattern(ocpg_ctx* c)
     const FimHost& H = ctxFimHost(c);
     const WellsHost& WH = H.wells;
     const int64_t nb = F.nb, nconn = F.nconn;
+    ctxDropGraph(c);
     const int nw = WH.nw;
     int nwOpen = 0;
     std::vector<int32_t> wrow(nw, -1);
@@ -330,6 +332,7 @@ int ocpg_set_conn(ocpg_ctx* c, const ocpg_conn* conn)
     H.eId.assign(conn->eId, conn->eId + nconn);
     H.wells = WellsHost{};
     FimDevice& F = ctxFim(c);
+    ctxDropGraph(c);
     fimFree(F);
     F.nb = nb;
     F.nconn = nconn;

@@ -246,6 +246,16 @@ static void destroySide(ocpg_ctx* c)
 }
 
 // The recomputation lists of the overlapped iteration, for the current cells.
+// The captured Newton iteration holds the device addresses it was captured
+// with; every call that frees or moves a buffer it reads drops it, so a later
+// ocpg_launch_iteration returns OCPG_ESTATE instead of replaying stale
+// addresses.
+static void dropIterGraph(ocpg_ctx* c)
+{
+    if (c->iterGraph) (void)hipGraphExecDestroy(c->iterGraph);
+    c->iterGraph = nullptr;
+}
+
 static int ensureFixup(ocpg_ctx* c)
 {
     const int64_t nb = c->B.nb;
@@ -450,6 +460,7 @@ int ocpg_set_cells(ocpg_ctx* c, int64_t nb)
     if (!c || nb < 0 || nb > (int64_t(1) << 31)) return OCPG_EINVAL;
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipStreamSynchronize(c->stream));
+    dropIterGraph(c);
     if (c->cur) HIPCHK(hipFree(c->cur));
     if (c->last) HIPCHK(hipFree(c->last));
     c->cur = c->last = nullptr;
@@ -767,6 +778,8 @@ int ocpg_save_flash_inputs(ocpg_ctx* c)
         if (isFlashInput(kFields[f].hostOff))
             need += align256((size_t)kFields[f].elem * cellEntries(c, kFields[f].cnt) * c->B.ld);
     if (need != c->snapBytes) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        dropIterGraph(c);
         if (c->snap) HIPCHK(hipFree(c->snap));
         c->snap = nullptr;
         HIPCHK(hipMalloc(&c->snap, need));
@@ -850,10 +863,7 @@ int ocpg_capture_iteration(ocpg_ctx* c, double dt)
     if (c->fim.nb != c->B.nb || c->fim.nb <= 0) return OCPG_ESTATE;
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (c->iterGraph) {
-        HIPCHK(hipGraphExecDestroy(c->iterGraph));
-        c->iterGraph = nullptr;
-    }
+    dropIterGraph(c);
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
     int rc = ocpg_restore_flash_inputs(c);
     if (rc == OCPG_OK) rc = ocpg_flash_fim(c, nullptr);
@@ -921,4 +931,5 @@ const DevSat* ctxSat(ocpg_ctx* c) { return c->dSat; }
 const DevRock& ctxRock(ocpg_ctx* c) { return c->rock; }
 void ctxTimerBegin(ocpg_ctx* c, int slot) { timerBegin(c, slot); }
 void ctxTimerEnd(ocpg_ctx* c, int slot) { timerEnd(c, slot); }
+void ctxDropGraph(ocpg_ctx* c) { dropIterGraph(c); }
 }  // namespace ocpg
