@@ -373,6 +373,18 @@ int ocpg_flash_fim_guided(ocpg_ctx* ctx, const float* guess, int ld, ocpg_flash_
 int ocpg_mlp_gemm(void* stream, int M, int N, int K, const float* A, int64_t sAm, int64_t sAk, const float* B,
                   int64_t sBk, int64_t sBn, float* C, int64_t sCm, int64_t sCn, int accumulate, int splitK,
                   float* partial);
+/* ocpg_mlp_gemm without a workspace argument and without allocating: the
+ * split-K partial tiles live in a static buffer of the library, one per GPU,
+ * of OCPG_MLP_SCRATCH_FLOATS floats (16 MiB); a call needing more returns
+ * OCPG_ETOOBIG.  No reference counterpart: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops that
+ * forbid allocation).  Calls on one GPU share that buffer, so calls that
+ * overlap on one GPU (other streams, other host threads) race; calls
+ * ordered on one stream do not.                                            */
+#define OCPG_MLP_SCRATCH_FLOATS (1 << 22)
+int ocpg_mlp_gemm_noalloc(void* stream, int M, int N, int K, const float* A, int64_t sAm, int64_t sAk,
+                          const float* B, int64_t sBk, int64_t sBn, float* C, int64_t sCm, int64_t sCn,
+                          int accumulate, int splitK);
 int ocpg_mlp_bias_act(void* stream, int M, int N, float* Z, const float* b, float* H);
 int ocpg_mlp_act_backward(void* stream, int64_t n, const float* Z, float* dH);
 int ocpg_mlp_colsum(void* stream, int M, int N, const float* X, float* out);

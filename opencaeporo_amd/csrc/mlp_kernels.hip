@@ -16,6 +16,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 
 #include "../../include/ocpgpu.h"
 
@@ -175,6 +176,46 @@ inline unsigned blocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace
 
+// Split-K partial tiles of ocpg_mlp_gemm_noalloc, one copy per GPU (each
+// device holds its own instance of this array from the first launch of the
+// library's code on it until the process ends).  The host only ever passes
+// the address hipGetSymbolAddress returns for the current device; the name
+// itself is a host-side stand-in, not a device address.
+static __device__ __align__(256) float scratch[OCPG_MLP_SCRATCH_FLOATS];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex gScratchMu;
+float* gScratchPtr[kMaxDevices] = {};
+
+// The current device's address of `scratch`, looked up once per device.
+int scratchOfCurrentDevice(float** out)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return OCPG_ENODEV;
+    if (dev >= kMaxDevices) return OCPG_EUNSUP;
+    std::lock_guard<std::mutex> g(gScratchMu);
+    if (!gScratchPtr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return OCPG_EHIP;
+        gScratchPtr[dev] = static_cast<float*>(p);
+    }
+    *out = gScratchPtr[dev];
+    return OCPG_OK;
+}
+
+// Grid of mlpGemmKernel: the K chunk per z-slice and the number of slices.
+void gemmSplit(int K, int splitK, int* kChunk, int* Z)
+{
+    *kChunk = ((K + splitK - 1) / splitK + 15) / 16 * 16;
+    const int z = *kChunk > 0 ? (K + *kChunk - 1) / *kChunk : 0;
+    *Z = z > 0 ? z : 1;
+}
+
+}  // namespace
+
 extern "C" {
 
 int ocpg_mlp_gemm(void* stream, int M, int N, int K, const float* A, int64_t sAm, int64_t sAk, const float* B,
@@ -183,8 +224,8 @@ int ocpg_mlp_gemm(void* stream, int M, int N, int K, const float* A, int64_t sAm
 {
     if (M <= 0 || N <= 0 || K < 0 || !A || !B || !C || splitK < 1 || (splitK > 1 && !partial)) return OCPG_EINVAL;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const int kChunk = ((K + splitK - 1) / splitK + 15) / 16 * 16;
-    const int Z = (K + kChunk - 1) / kChunk > 0 ? (K + kChunk - 1) / kChunk : 1;
+    int kChunk, Z;
+    gemmSplit(K, splitK, &kChunk, &Z);
     dim3 grid((M + 15) / 16, (N + 15) / 16, Z);
     hipLaunchKernelGGL(mlpGemmKernel, grid, dim3(64), 0, s, M, N, K, A, sAm, sAk, B, sBk, sBn, C, sCm, sCn,
                        accumulate, kChunk, Z > 1 ? partial : nullptr);
@@ -192,6 +233,26 @@ int ocpg_mlp_gemm(void* stream, int M, int N, int K, const float* A, int64_t sAm
         hipLaunchKernelGGL(mlpSplitSumKernel, dim3(blocks((int64_t)M * N, 256)), dim3(256), 0, s, M, N, Z, partial,
                            C, sCm, sCn, accumulate);
     return status(hipGetLastError());
+}
+
+// ocpg_mlp_gemm with the split-K partial tiles in the library's static
+// per-device buffer: allocates nothing.  The partial tiles need
+// Z * M * N <= OCPG_MLP_SCRATCH_FLOATS floats (Z = the K slices actually
+// used); larger requests are refused with OCPG_ETOOBIG before any launch.
+int ocpg_mlp_gemm_noalloc(void* stream, int M, int N, int K, const float* A, int64_t sAm, int64_t sAk,
+                          const float* B, int64_t sBk, int64_t sBn, float* C, int64_t sCm, int64_t sCn,
+                          int accumulate, int splitK)
+{
+    if (M <= 0 || N <= 0 || K < 0 || !A || !B || !C || splitK < 1) return OCPG_EINVAL;
+    int kChunk, Z;
+    gemmSplit(K, splitK, &kChunk, &Z);
+    if (Z > 1 && (int64_t)Z * M * N > (int64_t)OCPG_MLP_SCRATCH_FLOATS) return OCPG_ETOOBIG;
+    float* part = nullptr;
+    if (Z > 1) {
+        const int rc = scratchOfCurrentDevice(&part);
+        if (rc != OCPG_OK) return rc;
+    }
+    return ocpg_mlp_gemm(stream, M, N, K, A, sAm, sAk, B, sBk, sBn, C, sCm, sCn, accumulate, splitK, part);
 }
 
 int ocpg_mlp_bias_act(void* stream, int M, int N, float* Z, const float* b, float* H)

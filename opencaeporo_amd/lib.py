@@ -49,6 +49,8 @@ def load():
         "ocpg_flash_fim_guided": ([P, P, C.c_int, P], C.c_int),
         "ocpg_mlp_gemm": ([P, C.c_int, C.c_int, C.c_int, P, C.c_int64, C.c_int64, P, C.c_int64, C.c_int64,
                            P, C.c_int64, C.c_int64, C.c_int, C.c_int, P], C.c_int),
+        "ocpg_mlp_gemm_noalloc": ([P, C.c_int, C.c_int, C.c_int, P, C.c_int64, C.c_int64, P, C.c_int64,
+                                   C.c_int64, P, C.c_int64, C.c_int64, C.c_int, C.c_int], C.c_int),
         "ocpg_mlp_bias_act": ([P, C.c_int, C.c_int, P, P, P], C.c_int),
         "ocpg_mlp_act_backward": ([P, C.c_int64, P, P], C.c_int),
         "ocpg_mlp_colsum": ([P, C.c_int, C.c_int, P, P], C.c_int),
