@@ -358,6 +358,72 @@ int ocpg_get_well_flux(ocpg_ctx* ctx, double* perfQi, double* wellQi, double* pr
 /* Rows of the assembled system: bulk cells plus open wells.                  */
 int64_t ocpg_bsr_rows(ocpg_ctx* ctx);
 
+/* What the wells' per-step preparation reads beyond ocpg_wells (Well::depth,
+ * WellOpt::injTemp / injProdPhase / initOptMode; src/Well.cpp:41-92,
+ * include/WellOpt.hpp:77-113).  nw entries each; the perforations' depths are
+ * their cells' depths (Well::Setup, src/Well.cpp:69).  Set after
+ * ocpg_set_wells (which clears it).                                          */
+typedef struct ocpg_well_setup {
+    const double* depth;      /* Well::depth, the BHP reference depth (ft)    */
+    const double* injTemp;    /* injected fluid temperature (F), injectors     */
+    const int32_t* injPhase;  /* injectors' phase: 1 gas (solvent), 2 water    */
+    const int32_t* initMode;  /* OCPG_WELL_BHP / OCPG_WELL_RATE as scheduled   */
+} ocpg_well_setup;
+int ocpg_set_well_setup(ocpg_ctx* ctx, const ocpg_well_setup* setup);
+/* AllWells::PrepareWell (src/AllWells.cpp:201-214) on the device, once per
+ * time step: for each open well Well::CalTrans, Well::CaldG (CalInjdG /
+ * CalProddG01, src/Well.cpp:357-555: the perforations' hydrostatic offsets,
+ * integrated in 10 ft segments with the injected fluid's density or the
+ * flashed perforation stream's), Well::CheckOptMode (:893-944: BHP or rate
+ * control from the rate at the BHP limit) and Well::CalFlux's ReCalXi branch
+ * (:204-228: the injected fluid's molar density at each perforation).
+ * Updates the wells' mode, bhp, dG and perforation densities on the device
+ * (ocpg_get_well_state reads them).  Needs ocpg_set_well_setup.             */
+int ocpg_prepare_wells(ocpg_ctx* ctx);
+/* AllWells::CheckP (src/AllWells.cpp:335-369; Well::CheckP and
+ * Well::CheckCrossFlow, src/Well.cpp:946-1066) on the state of the last
+ * ocpg_calres: *status = OCPG_WELL_SUCCESS, OCPG_WELL_NEGATIVE_PRESSURE
+ * (a BHP or open perforation pressure below zero) or OCPG_WELL_SWITCH (a
+ * rate control switched to BHP, or a perforation closed by cross flow; the
+ * reference's WELL_SWITCH_TO_BHPMODE / WELL_CROSSFLOW).  Switches and
+ * closures are applied to the device state as the reference applies them. */
+#define OCPG_WELL_SUCCESS 0
+#define OCPG_WELL_NEGATIVE_PRESSURE 1
+#define OCPG_WELL_SWITCH 2
+int ocpg_check_wells(ocpg_ctx* ctx, int32_t* status);
+/* The per-step well state held on the device: control mode (nw), bhp (nw),
+ * perforation dG (nperf), injected fluid molar density (nperf) and
+ * multiplier (nperf; 0 for a perforation closed by cross flow).  Any pointer
+ * may be NULL.                                                              */
+int ocpg_get_well_state(ocpg_ctx* ctx, int32_t* mode, double* bhp, double* perfDG, double* perfXi,
+                        double* perfMult);
+/* MixtureComp::XiPhase and MixtureComp::RhoPhase (src/MixtureComp.cpp:
+ * 711-768): molar (lbmol/ft3) and mass (lb/ft3) density of the single phase
+ * of feed zi (nc host values) at P (psia), T (F); phase 2 = water (PVTW),
+ * otherwise hydrocarbon (one EoS root).  SetupWellOpt's injection factor is
+ * 1000 * xi at surface conditions (src/MixtureComp.cpp:790-795).  Blocking. */
+int ocpg_phase_density(ocpg_ctx* ctx, double P, double T, const double* zi, int phase, double* xi, double* rho);
+
+/* Bulk::CheckNi then Bulk::CheckP (src/Bulk.cpp:1250-1308), as
+ * OCPControl::Check(rs, {"BulkNi", "BulkP"}) runs them in
+ * IsoT_FIM::UpdateProperty (src/OCPFluidMethod.cpp:877-885): *status =
+ * 0, OCPG_BULK_NEGATIVE_COMPONENTS_MOLES or OCPG_BULK_NEGATIVE_PRESSURE
+ * (the reference's BULK_* values, include/Bulk.hpp:39-44).                  */
+#define OCPG_BULK_NEGATIVE_PRESSURE (-1)
+#define OCPG_BULK_NEGATIVE_COMPONENTS_MOLES (-3)
+int ocpg_check_bulk(ocpg_ctx* ctx, int32_t* status);
+/* Bulk::CalMaxChange (src/Bulk.cpp:1335-1387) against the last time step
+ * and AllWells::CalMaxBHPChange (src/AllWells.cpp:398-406), which
+ * OCPControl::CalNextTimeStep reads (src/OCPControl.cpp:292-336).           */
+typedef struct {
+    double dPmax;    /* max |P - lP|                                         */
+    double dNmax;    /* max |Ni - lNi| / |max(Ni, lNi)| over |max| > 1e-8    */
+    double dSmax;    /* max |S - lS|                                         */
+    double eVmax;    /* max |vf - rockVp| / rockVp                           */
+    double dBHPmax;  /* max |bhp - lbhp| over open wells                     */
+} ocpg_max_change_t;
+int ocpg_max_change(ocpg_ctx* ctx, ocpg_max_change_t* out);
+
 /* The learned form (no reference counterpart).  FlashFIM as ocpg_flash_fim,
  * with a learned initial guess in device memory: nb rows of ld floats
  * (ln K_1 .. ln K_NC, stability logit).  A cell whose logit is positive first

@@ -198,6 +198,37 @@ class Wells:
         return int(np.count_nonzero(self.open[:w]))
 
 
+class WellSetupC(C.Structure):
+    _fields_ = [("depth", _P), ("injTemp", _P), ("injPhase", _P), ("initMode", _P)]
+
+
+class WellSetup:
+    """Numpy-backed ocpg_well_setup: per well its BHP reference depth, the
+    injected fluid's temperature (F) and phase (1 gas, 2 water) and the
+    scheduled control mode."""
+
+    def __init__(self, depth, injTemp, injPhase, initMode):
+        self.depth = np.ascontiguousarray(depth, dtype=np.float64)
+        self.injTemp = np.ascontiguousarray(injTemp, dtype=np.float64)
+        self.injPhase = np.ascontiguousarray(injPhase, dtype=np.int32)
+        self.initMode = np.ascontiguousarray(initMode, dtype=np.int32)
+
+    def cstruct(self):
+        s = WellSetupC()
+        for f in ("depth", "injTemp", "injPhase", "initMode"):
+            setattr(s, f, getattr(self, f).ctypes.data)
+        return s
+
+
+class MaxChange(C.Structure):
+    _fields_ = [("dPmax", C.c_double), ("dNmax", C.c_double), ("dSmax", C.c_double), ("eVmax", C.c_double),
+                ("dBHPmax", C.c_double)]
+
+
+WELL_SUCCESS, WELL_NEGATIVE_PRESSURE, WELL_SWITCH = 0, 1, 2
+BULK_NEGATIVE_PRESSURE, BULK_NEGATIVE_COMPONENTS_MOLES = -1, -3
+
+
 class FlashStats(C.Structure):
     _fields_ = [("cells", C.c_int64), ("ftype_count", C.c_int64 * 3), ("two_phase", C.c_int64),
                 ("iters", C.c_int64 * 5), ("nonconverged", C.c_int64)]

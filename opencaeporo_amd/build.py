@@ -50,7 +50,9 @@ def _jobs(ncs):
     nc_list = " ".join(f"X({n})" for n in ncs)
     jobs = []
     for n in ncs:
-        jobs.append(("flash_nc.hip", f"flash_nc{n}.o", [f"-DOCPG_NC={n}"] + ([NC_OPT[n]] if n in NC_OPT else [])))
+        opt = [NC_OPT[n]] if n in NC_OPT else []
+        jobs.append(("flash_nc.hip", f"flash_nc{n}.o", [f"-DOCPG_NC={n}"] + opt))
+        jobs.append(("well_nc.hip", f"well_nc{n}.o", [f"-DOCPG_NC={n}"] + opt))
     jobs.append(("flash_dispatch.cpp", "flash_dispatch.o", [f"-DOCPG_NC_LIST(X)={nc_list}"]))
     jobs.append(("fim_kernels.hip", "fim_kernels.o", []))
     jobs.append(("ocpgpu_capi.cpp", "ocpgpu_capi.o", []))

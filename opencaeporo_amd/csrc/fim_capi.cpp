@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -30,6 +31,8 @@ void ctxTimerEnd(ocpg_ctx* c, int slot);
 int fimReadNorms(ocpg_ctx* c, ocpg_resnorm* norms);
 NewtonDevice& ctxNewton(ocpg_ctx* c);
 void ctxDropGraph(ocpg_ctx* c);
+const DevBulk& ctxLastBulk(ocpg_ctx* c);
+bool ctxLastValid(ocpg_ctx* c);
 }  // namespace ocpg
 
 
@@ -56,6 +59,21 @@ static hipError_t devAlloc(T*& p, size_t count, size_t& total)
 }
 
 static size_t align256w(size_t b) { return (b + 255) & ~size_t(255); }
+static int ensureNewton(ocpg_ctx* c);
+
+// A producer's WellOpt::prodPhaseWeight (3 per well) is usable for a rate
+// control when given and not all zero: with zero weights the rate equation
+// is a constant and the well row of the Jacobian vanishes.
+static bool positiveWeights(const double* pw, int w)
+{
+    if (!pw) return false;
+    double s = 0;
+    for (int j = 0; j < 3; j++) {
+        if (!(pw[(size_t)w * 3 + j] >= 0)) return false;
+        s += pw[(size_t)w * 3 + j];
+    }
+    return s > 0;
+}
 
 // The per-step / per-iteration well fields (bhp, control, dG, perforation
 // molar densities of injectors).
@@ -229,6 +247,9 @@ static int buildPattern(ocpg_ctx* c)
         {(void**)&W.cellPerfPtr, (size_t)(nb + 1) * 4}, {(void**)&W.cellPerfIdx, cellPerfIdx.size() * 4},
         {(void**)&W.pcell, std::max<size_t>(pcell.size(), 1) * 4}, {(void**)&W.perfBlock, NP * bsize * 8},
         {(void**)&W.perfDQW, NP * nc * 8}, {(void**)&W.slotBW, NP * 4}, {(void**)&W.slotWB, NP * 4},
+        {(void**)&W.lbhp, NW * 8}, {(void**)&W.initMode, NW * 4}, {(void**)&W.injPhase, NW * 4},
+        {(void**)&W.wellDepth, NW * 8}, {(void**)&W.injTemp, NW * 8}, {(void**)&W.perfState, NP * 4},
+        {(void**)&W.perfP, NP * 8}, {(void**)&W.status, NW * 4}, {(void**)&W.perfTmp, NP * 8},
     };
     size_t total = 0;
     for (const Part& q : parts) total += align256w(q.bytes);
@@ -264,7 +285,17 @@ static int buildPattern(ocpg_ctx* c)
     HIPCHK2(up(W.pcell, pcell.data(), pcell.size() * 4));
     HIPCHK2(up(W.slotBW, slotBW.data(), (size_t)nperf * 4));
     HIPCHK2(up(W.slotWB, slotWB.data(), (size_t)nperf * 4));
-    return uploadWellState(c);
+    {
+        // every perforation starts open (Well::Setup, src/Well.cpp:64-73)
+        std::vector<int32_t> st((size_t)nperf, 1);
+        HIPCHK2(up(W.perfState, st.data(), (size_t)nperf * 4));
+    }
+    const int rc = uploadWellState(c);
+    if (rc != OCPG_OK) return rc;
+    // the last-step BHP starts as the given one (UpdateLastTimeStepBHP of
+    // IsoT_FIM::InitReservoir, src/OCPFluidMethod.cpp:836-850)
+    HIPCHK2(hipMemcpy(W.lbhp, W.bhp, NW * 8, hipMemcpyDeviceToDevice));
+    return OCPG_OK;
 }
 
 namespace ocpg {
@@ -384,7 +415,7 @@ int ocpg_set_wells(ocpg_ctx* c, const ocpg_wells* w)
             if (w->type[i] != OCPG_WELL_INJ && w->type[i] != OCPG_WELL_PROD) return OCPG_EINVAL;
             if (w->mode[i] != OCPG_WELL_BHP && w->mode[i] != OCPG_WELL_RATE) return OCPG_EINVAL;
             if (w->type[i] == OCPG_WELL_INJ && (!w->injZi || !w->perfXi)) return OCPG_EINVAL;
-            if (w->type[i] == OCPG_WELL_PROD && w->mode[i] == OCPG_WELL_RATE && !w->prodPhaseWeight)
+            if (w->type[i] == OCPG_WELL_PROD && w->mode[i] == OCPG_WELL_RATE && !positiveWeights(w->prodPhaseWeight, i))
                 return OCPG_EINVAL;
         }
         const int nperf = w->perfStart[nw];
@@ -402,7 +433,9 @@ int ocpg_set_wells(ocpg_ctx* c, const ocpg_wells* w)
         if (w->perfMult) WH.perfMult.assign(w->perfMult, w->perfMult + nperf);
         WH.injZi.assign((size_t)nw * nc, 0.0);
         WH.prodPW.assign((size_t)nw * 3, 0.0);
+        WH.hasPW.assign(nw, 0);
         for (int i = 0; i < nw; i++) {
+            if (w->type[i] == OCPG_WELL_PROD) WH.hasPW[i] = positiveWeights(w->prodPhaseWeight, i) ? 1 : 0;
             if (w->type[i] == OCPG_WELL_INJ)
                 std::copy(w->injZi + (size_t)i * nc, w->injZi + (size_t)(i + 1) * nc, WH.injZi.begin() + (size_t)i * nc);
             else if (w->prodPhaseWeight)
@@ -430,12 +463,178 @@ int ocpg_set_well_state(ocpg_ctx* c, const ocpg_wells* w)
     for (int i = 0; i < WH.nw; i++) {
         if (w->mode[i] != OCPG_WELL_BHP && w->mode[i] != OCPG_WELL_RATE) return OCPG_EINVAL;
         if (WH.type[i] == OCPG_WELL_INJ && !w->perfXi) return OCPG_EINVAL;
-        if (WH.type[i] == OCPG_WELL_PROD && w->mode[i] == OCPG_WELL_RATE && WH.prodPW.empty()) return OCPG_EINVAL;
+        if (WH.type[i] == OCPG_WELL_PROD && w->mode[i] == OCPG_WELL_RATE) {
+            // new weights may come with the switch to a rate control
+            const bool given = w->prodPhaseWeight && positiveWeights(w->prodPhaseWeight, i);
+            if (!given && !WH.hasPW[i]) return OCPG_EINVAL;
+        }
+    }
+    if (w->prodPhaseWeight) {
+        for (int i = 0; i < WH.nw; i++) {
+            if (WH.type[i] != OCPG_WELL_PROD || !positiveWeights(w->prodPhaseWeight, i)) continue;
+            std::copy(w->prodPhaseWeight + (size_t)i * 3, w->prodPhaseWeight + (size_t)(i + 1) * 3,
+                      WH.prodPW.begin() + (size_t)i * 3);
+            WH.hasPW[i] = 1;
+        }
     }
     setWellState(WH, *w);
     HIPCHK2(hipSetDevice(ctxDevice(c)));
     HIPCHK2(hipStreamSynchronize(ctxStream(c)));
-    return uploadWellState(c);
+    const int rc = uploadWellState(c);
+    if (rc != OCPG_OK) return rc;
+    WellDevice& W = ctxWells(c);
+    HIPCHK2(hipMemcpy(W.prodPW, WH.prodPW.data(), (size_t)WH.nw * 3 * 8, hipMemcpyHostToDevice));
+    return OCPG_OK;
+}
+
+// Well::depth, WellOpt::injTemp / injProdPhase / initOptMode for
+// ocpg_prepare_wells and the CaldG of ocpg_reset_to_last_step.
+int ocpg_set_well_setup(ocpg_ctx* c, const ocpg_well_setup* su)
+{
+    if (!c || !su) return OCPG_EINVAL;
+    WellDevice& W = ctxWells(c);
+    const WellsHost& WH = ctxFimHost(c).wells;
+    if (W.nw <= 0) return OCPG_ESTATE;
+    if (!su->depth || !su->initMode) return OCPG_EINVAL;
+    const int nw = W.nw;
+    std::vector<double> temp(nw, 0.0);
+    std::vector<int32_t> phase(nw, 1);
+    for (int w = 0; w < nw; w++) {
+        if (su->initMode[w] != OCPG_WELL_BHP && su->initMode[w] != OCPG_WELL_RATE) return OCPG_EINVAL;
+        if (!std::isfinite(su->depth[w])) return OCPG_EINVAL;
+        if (WH.type[w] == OCPG_WELL_INJ) {
+            if (!su->injTemp || !su->injPhase) return OCPG_EINVAL;
+            if (su->injPhase[w] != 1 && su->injPhase[w] != 2) return OCPG_EINVAL;
+            temp[w] = su->injTemp[w];
+            phase[w] = su->injPhase[w];
+        }
+        if (WH.type[w] == OCPG_WELL_PROD && su->initMode[w] == OCPG_WELL_RATE && !WH.hasPW[w])
+            return OCPG_EINVAL;
+    }
+    HIPCHK2(hipSetDevice(ctxDevice(c)));
+    HIPCHK2(hipStreamSynchronize(ctxStream(c)));
+    HIPCHK2(hipMemcpy(W.wellDepth, su->depth, nw * 8, hipMemcpyHostToDevice));
+    HIPCHK2(hipMemcpy(W.initMode, su->initMode, nw * 4, hipMemcpyHostToDevice));
+    HIPCHK2(hipMemcpy(W.injTemp, temp.data(), nw * 8, hipMemcpyHostToDevice));
+    HIPCHK2(hipMemcpy(W.injPhase, phase.data(), nw * 4, hipMemcpyHostToDevice));
+    W.hasSetup = true;
+    ctxDropGraph(c);
+    return OCPG_OK;
+}
+
+// AllWells::PrepareWell on the device (asynchronous).
+int ocpg_prepare_wells(ocpg_ctx* c)
+{
+    if (!c) return OCPG_EINVAL;
+    WellDevice& W = ctxWells(c);
+    if (W.nw <= 0 || !W.hasSetup) return OCPG_ESTATE;
+    HIPCHK2(hipSetDevice(ctxDevice(c)));
+    HIPCHK2(launchWellPrepare(ctxNC(c), ctxEos(c), ctxBulk(c), W, 0, ctxStream(c)));
+    return OCPG_OK;
+}
+
+// AllWells::CheckP on the device; waits for the result.
+int ocpg_check_wells(ocpg_ctx* c, int32_t* status)
+{
+    if (!c || !status) return OCPG_EINVAL;
+    WellDevice& W = ctxWells(c);
+    if (W.nw <= 0) return OCPG_ESTATE;
+    HIPCHK2(hipSetDevice(ctxDevice(c)));
+    HIPCHK2(launchWellCheck(ctxNC(c), ctxBulk(c), W, ctxStream(c)));
+    HIPCHK2(hipMemcpyAsync(status, W.status, sizeof(int32_t), hipMemcpyDeviceToHost, ctxStream(c)));
+    HIPCHK2(hipStreamSynchronize(ctxStream(c)));
+    return OCPG_OK;
+}
+
+int ocpg_get_well_state(ocpg_ctx* c, int32_t* mode, double* bhp, double* perfDG, double* perfXi, double* perfMult)
+{
+    if (!c) return OCPG_EINVAL;
+    WellDevice& W = ctxWells(c);
+    if (W.nw <= 0) return OCPG_ESTATE;
+    HIPCHK2(hipSetDevice(ctxDevice(c)));
+    HIPCHK2(hipStreamSynchronize(ctxStream(c)));
+    const size_t NW = W.nw, NP = W.nperf;
+    if (mode) HIPCHK2(hipMemcpy(mode, W.mode, NW * 4, hipMemcpyDeviceToHost));
+    if (bhp) HIPCHK2(hipMemcpy(bhp, W.bhp, NW * 8, hipMemcpyDeviceToHost));
+    if (NP && perfDG) HIPCHK2(hipMemcpy(perfDG, W.perfDG, NP * 8, hipMemcpyDeviceToHost));
+    if (NP && perfXi) HIPCHK2(hipMemcpy(perfXi, W.perfXi, NP * 8, hipMemcpyDeviceToHost));
+    if (NP && perfMult) HIPCHK2(hipMemcpy(perfMult, W.perfMult, NP * 8, hipMemcpyDeviceToHost));
+    return OCPG_OK;
+}
+
+// MixtureComp::XiPhase / RhoPhase of one feed.
+int ocpg_phase_density(ocpg_ctx* c, double P, double T, const double* zi, int phase, double* xi, double* rho)
+{
+    if (!c || !zi || !(P > 0) || !std::isfinite(T)) return OCPG_EINVAL;
+    if (!flashSupportsNC(ctxNC(c))) return OCPG_EUNSUP;
+    const int nc = ctxNC(c) + 1;
+    HIPCHK2(hipSetDevice(ctxDevice(c)));
+    double* d = nullptr;
+    HIPCHK2(hipMalloc(&d, (size_t)(nc + 2) * sizeof(double)));
+    double out[2] = {0, 0};
+    hipError_t e = hipMemcpy(d, zi, nc * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launchPhaseDensity(ctxNC(c), ctxEos(c), P, T, d, phase == 2 ? 2 : 1, d + nc, ctxStream(c));
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d + nc, 2 * sizeof(double), hipMemcpyDeviceToHost, ctxStream(c));
+    if (e == hipSuccess) e = hipStreamSynchronize(ctxStream(c));
+    (void)hipFree(d);
+    HIPCHK2(e);
+    if (xi) *xi = out[0];
+    if (rho) *rho = out[1];
+    return OCPG_OK;
+}
+
+// Bulk::CheckNi, then Bulk::CheckP; waits for the result.
+int ocpg_check_bulk(ocpg_ctx* c, int32_t* status)
+{
+    if (!c || !status) return OCPG_EINVAL;
+    const DevBulk& B = ctxBulk(c);
+    if (B.nb <= 0) return OCPG_ESTATE;
+    HIPCHK2(hipSetDevice(ctxDevice(c)));
+    int rc = ensureNewton(c);
+    if (rc != OCPG_OK) return rc;
+    NewtonDevice& N = ctxNewton(c);
+    HIPCHK2(hipMemsetAsync(N.chk, 0, sizeof(int32_t), ctxStream(c)));
+    HIPCHK2(launchBulkCheck(B, ctxNC(c) + 1, N.chk, ctxStream(c)));
+    int32_t f = 0;
+    HIPCHK2(hipMemcpyAsync(&f, N.chk, sizeof(int32_t), hipMemcpyDeviceToHost, ctxStream(c)));
+    HIPCHK2(hipStreamSynchronize(ctxStream(c)));
+    *status = (f & 2) ? OCPG_BULK_NEGATIVE_COMPONENTS_MOLES : (f & 1) ? OCPG_BULK_NEGATIVE_PRESSURE : 0;
+    return OCPG_OK;
+}
+
+// Bulk::CalMaxChange + AllWells::CalMaxBHPChange; waits for the result.
+int ocpg_max_change(ocpg_ctx* c, ocpg_max_change_t* out)
+{
+    if (!c || !out) return OCPG_EINVAL;
+    const DevBulk& B = ctxBulk(c);
+    if (B.nb <= 0 || !ctxLastValid(c)) return OCPG_ESTATE;
+    HIPCHK2(hipSetDevice(ctxDevice(c)));
+    int rc = ensureNewton(c);
+    if (rc != OCPG_OK) return rc;
+    NewtonDevice& N = ctxNewton(c);
+    HIPCHK2(launchMaxChange(B, ctxLastBulk(c), ctxNC(c) + 1, N.chg, N.blocks, ctxStream(c)));
+    std::vector<double> part((size_t)N.blocks * 4);
+    HIPCHK2(hipMemcpyAsync(part.data(), N.chg, part.size() * sizeof(double), hipMemcpyDeviceToHost, ctxStream(c)));
+    WellDevice& W = ctxWells(c);
+    std::vector<double> bhp(W.nw), lbhp(W.nw);
+    std::vector<int32_t> open(W.nw);
+    if (W.nw > 0) {
+        HIPCHK2(hipMemcpyAsync(bhp.data(), W.bhp, W.nw * 8, hipMemcpyDeviceToHost, ctxStream(c)));
+        HIPCHK2(hipMemcpyAsync(lbhp.data(), W.lbhp, W.nw * 8, hipMemcpyDeviceToHost, ctxStream(c)));
+        HIPCHK2(hipMemcpyAsync(open.data(), W.open, W.nw * 4, hipMemcpyDeviceToHost, ctxStream(c)));
+    }
+    HIPCHK2(hipStreamSynchronize(ctxStream(c)));
+    ocpg_max_change_t r = {0, 0, 0, 0, 0};
+    for (int b = 0; b < N.blocks; b++) {
+        r.dPmax = std::max(r.dPmax, part[4 * b]);
+        r.dNmax = std::max(r.dNmax, part[4 * b + 1]);
+        r.dSmax = std::max(r.dSmax, part[4 * b + 2]);
+        r.eVmax = std::max(r.eVmax, part[4 * b + 3]);
+    }
+    for (int w = 0; w < W.nw; w++)
+        if (open[w]) r.dBHPmax = std::max(r.dBHPmax, std::fabs(bhp[w] - lbhp[w]));
+    *out = r;
+    return OCPG_OK;
 }
 
 // Well::CalFlux / CalProdWeight results of the last ocpg_calres.
@@ -517,13 +716,16 @@ static int ensureNewton(ocpg_ctx* c)
     const size_t L = (size_t)std::max<int64_t>(B.ld, 64);
     const size_t bD = align256w(3 * L * sizeof(double)), bB = align256w(3 * L);
     const size_t bP = align256w((size_t)N.blocks * 4 * sizeof(double)), bI = align256w((size_t)N.blocks * 2 * 8);
-    HIPCHK2(hipMalloc(&N.mem, 2 * bD + bB + bP + bI));
+    const size_t bK = 256;
+    HIPCHK2(hipMalloc(&N.mem, 2 * bD + bB + 2 * bP + bI + bK));
     char* p = static_cast<char*>(N.mem);
     N.dSNR = reinterpret_cast<double*>(p);
     N.dSNRP = reinterpret_cast<double*>(p + bD);
     N.pexOld = reinterpret_cast<uint8_t*>(p + 2 * bD);
     N.part = reinterpret_cast<double*>(p + 2 * bD + bB);
     N.partIdx = reinterpret_cast<int64_t*>(p + 2 * bD + bB + bP);
+    N.chg = reinterpret_cast<double*>(p + 2 * bD + bB + bP + bI);
+    N.chk = reinterpret_cast<int32_t*>(p + 2 * bD + bB + 2 * bP + bI);
     return OCPG_OK;
 }
 

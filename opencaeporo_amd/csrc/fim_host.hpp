@@ -18,6 +18,7 @@ struct WellsHost {
     int nw = 0;
     std::vector<int32_t> type, open, mode, unweight, perfStart, perfCell;
     std::vector<double> bhp, ctrlBHP, ctrlRate, injZi, prodPW, perfWI, perfMult, perfDG, perfXi;
+    std::vector<int32_t> hasPW;  // per well: production phase weights given, with a positive sum
     double Psurf = 0, Tsurf = 0;
 };
 

@@ -1220,6 +1220,65 @@ hipError_t launchNRChange(const DevBulk& B, NewtonDevice& N, hipStream_t s)
     return hipGetLastError();
 }
 
+// Bulk::CheckNi and Bulk::CheckP (src/Bulk.cpp:1250-1308): bit 1 of *flag
+// when a component's moles are negative, bit 0 when a pressure is.
+__global__ void __launch_bounds__(kNewtonBlock) bulkCheckKernel(DevBulk B, int nc, int32_t* flag)
+{
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int f = 0;
+    if (n < B.nb) {
+        if (B.P[n] < 0) f |= 1;
+        for (int i = 0; i < nc; ++i)
+            if (B.Ni[(int64_t)i * B.ld + n] < 0) f |= 2;
+    }
+    if (f) atomicOr(flag, f);
+}
+
+// Bulk::CalMaxChange (src/Bulk.cpp:1335-1387) against the last time step:
+// per block the maxima of |dP|, the relative |dNi|, |dS| and the relative
+// volume error, into part[4 * block + {0, 1, 2, 3}].
+__global__ void __launch_bounds__(kNewtonBlock) maxChangeKernel(DevBulk B, DevBulk L, int nc, double* part)
+{
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t ld = B.ld;
+    double m[4] = {0, 0, 0, 0};
+    if (n < B.nb) {
+        m[0] = fabs(B.P[n] - L.P[n]);
+        for (int j = 0; j < kNP; ++j) m[2] = fmax(m[2], fabs(B.S[j * ld + n] - L.S[j * ld + n]));
+        for (int i = 0; i < nc; ++i) {
+            const double a = B.Ni[(int64_t)i * ld + n], b = L.Ni[(int64_t)i * ld + n];
+            const double t = fabs(fmax(a, b));
+            if (t > 1E-8) m[1] = fmax(m[1], fabs(a - b) / t);
+        }
+        m[3] = fabs(B.vf[n] - B.rockVp[n]) / B.rockVp[n];
+    }
+    __shared__ double sh[4][kNewtonBlock];
+    for (int q = 0; q < 4; ++q) sh[q][threadIdx.x] = m[q];
+    __syncthreads();
+    for (int st = kNewtonBlock / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int q = 0; q < 4; ++q) sh[q][threadIdx.x] = fmax(sh[q][threadIdx.x], sh[q][threadIdx.x + st]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 4; ++q) part[4 * (int64_t)blockIdx.x + q] = sh[q][0];
+}
+
+hipError_t launchBulkCheck(const DevBulk& B, int nc, int32_t* flag, hipStream_t s)
+{
+    if (B.nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(bulkCheckKernel, dim3((unsigned)((B.nb + kNewtonBlock - 1) / kNewtonBlock)),
+                       dim3(kNewtonBlock), 0, s, B, nc, flag);
+    return hipGetLastError();
+}
+
+hipError_t launchMaxChange(const DevBulk& B, const DevBulk& L, int nc, double* part, int blocks, hipStream_t s)
+{
+    if (B.nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(maxChangeKernel, dim3(blocks), dim3(kNewtonBlock), 0, s, B, L, nc, part);
+    return hipGetLastError();
+}
+
 #define OCPG_FIM_NC(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
 
 hipError_t launchGetSolution(int NC, const DevBulk& B, NewtonDevice& N, const WellDevice& W, int64_t nb,

@@ -73,11 +73,18 @@ struct NewtonDevice {
     double* u = nullptr;        // staging of a solution vector from the host
     int64_t uLen = 0;
     bool valid = false;         // a GetSolution happened since the last flash bookkeeping
+    int32_t* chk = nullptr;     // flags of ocpg_check_bulk
+    double* chg = nullptr;      // per block: 4 maxima of ocpg_max_change
     void* mem = nullptr;
 };
 hipError_t launchGetSolution(int NC, const DevBulk& B, NewtonDevice& N, const WellDevice& W, int64_t nb,
                              const double* u, double dSmaxlim, hipStream_t s);
 hipError_t launchNRChange(const DevBulk& B, NewtonDevice& N, hipStream_t s);
+// Bulk::CheckNi / CheckP into *flag (bit 1 / bit 0; *flag zeroed by the
+// caller) and Bulk::CalMaxChange's per-block maxima against the last-step
+// state L (dP, relative dN, dS, relative volume error).
+hipError_t launchBulkCheck(const DevBulk& B, int nc, int32_t* flag, hipStream_t s);
+hipError_t launchMaxChange(const DevBulk& B, const DevBulk& L, int nc, double* part, int blocks, hipStream_t s);
 
 // The overlapped Newton iteration's recomputation (fim_kernels.hip): rows
 // touched by the cells the flash finished on its side streams.

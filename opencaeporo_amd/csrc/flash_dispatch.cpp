@@ -65,4 +65,32 @@ hipError_t launchWellAssemble(int NC, const DevBulk& B, FimDevice& F, WellDevice
     return hipErrorInvalidValue;
 }
 
+hipError_t launchWellPrepare(int NC, const DevEos* eos, const DevBulk& B, WellDevice& W, int what, hipStream_t s)
+{
+#define X(N) \
+    if (NC == N) return launchWellPrepareT<N>(eos, B, W, what, s);
+    OCPG_NC_LIST(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launchWellCheck(int NC, const DevBulk& B, WellDevice& W, hipStream_t s)
+{
+#define X(N) \
+    if (NC == N) return launchWellCheckT<N>(B, W, s);
+    OCPG_NC_LIST(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launchPhaseDensity(int NC, const DevEos* eos, double P, double TF, const double* zi, int phase, double* out,
+                              hipStream_t s)
+{
+#define X(N) \
+    if (NC == N) return launchPhaseDensityT<N>(eos, P, TF, zi, phase, out, s);
+    OCPG_NC_LIST(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
 }  // namespace ocpg

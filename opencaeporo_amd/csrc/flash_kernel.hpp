@@ -39,6 +39,13 @@ CellLists flashSideCells(const FlashWork& W);
 hipError_t launchWellFlux(int NC, const DevEos* eos, const DevBulk& B, FimDevice& F, WellDevice& W, hipStream_t s);
 // The well blocks of AssembleMatWellsNew (after the bulk assembly).
 hipError_t launchWellAssemble(int NC, const DevBulk& B, FimDevice& F, WellDevice& W, double dt, hipStream_t s);
+// AllWells::PrepareWell (what = 0) or the well part of ResetToLastTimeStep
+// (what = 1); AllWells::CheckP into W.status[0]; MixtureComp::XiPhase /
+// RhoPhase of one feed into out[0], out[1] (device memory).
+hipError_t launchWellPrepare(int NC, const DevEos* eos, const DevBulk& B, WellDevice& W, int what, hipStream_t s);
+hipError_t launchWellCheck(int NC, const DevBulk& B, WellDevice& W, hipStream_t s);
+hipError_t launchPhaseDensity(int NC, const DevEos* eos, double P, double TF, const double* zi, int phase, double* out,
+                              hipStream_t s);
 
 }  // namespace ocpg
 
@@ -53,4 +60,11 @@ template <int NC>
 hipError_t launchWellFluxT(const DevEos* eos, const DevBulk& B, FimDevice& F, WellDevice& W, hipStream_t s);
 template <int NC>
 hipError_t launchWellAssembleT(const DevBulk& B, FimDevice& F, WellDevice& W, double dt, hipStream_t s);
+template <int NC>
+hipError_t launchWellPrepareT(const DevEos* eos, const DevBulk& B, WellDevice& W, int what, hipStream_t s);
+template <int NC>
+hipError_t launchWellCheckT(const DevBulk& B, WellDevice& W, hipStream_t s);
+template <int NC>
+hipError_t launchPhaseDensityT(const DevEos* eos, double P, double TF, const double* zi, int phase, double* out,
+                               hipStream_t s);
 }  // namespace ocpg

@@ -93,8 +93,9 @@ struct ocpg_ctx {
     DevSat* dSat = nullptr;
     DevRock rock{};
     DevFlashCounters* dCtr = nullptr;
-    // cells
+    // cells, and the last-step copies of the fields IsoT_FIM keeps (lP, lS, ...)
     DevBulk B{};
+    DevBulk Blast{};
     char* cur = nullptr;
     char* last = nullptr;
     size_t lastBytes = 0;  // bytes of the last-step block at the front of `cur`
@@ -303,6 +304,8 @@ int64_t ocpg_sizeof(const char* name)
     if (!std::strcmp(name, "ocpg_flash_stats")) return sizeof(ocpg_flash_stats);
     if (!std::strcmp(name, "ocpg_nr_update")) return sizeof(ocpg_nr_update);
     if (!std::strcmp(name, "ocpg_nr_change_t")) return sizeof(ocpg_nr_change_t);
+    if (!std::strcmp(name, "ocpg_well_setup")) return sizeof(ocpg_well_setup);
+    if (!std::strcmp(name, "ocpg_max_change_t")) return sizeof(ocpg_max_change_t);
     return -1;
 }
 
@@ -529,6 +532,14 @@ int ocpg_set_cells(ocpg_ctx* c, int64_t nb)
     // IsoT_FIM::CalRes reads the last-step moles as lNi.
     for (int f = 0; f < kNumFields; f++)
         if (kFields[f].hostOff == HB(Ni)) c->B.lNi = reinterpret_cast<double*>(c->last + c->fieldOff[f]);
+    std::memset(&c->Blast, 0, sizeof(DevBulk));
+    c->Blast.nb = nb;
+    c->Blast.ld = ld;
+    for (int f = 0; f < kNumFields; f++) {
+        if (!kFields[f].lastStep) continue;
+        void* p = c->last + c->fieldOff[f];
+        std::memcpy(reinterpret_cast<char*>(&c->Blast) + kFields[f].devOff, &p, sizeof(void*));
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
     return OCPG_OK;
 }
@@ -753,16 +764,24 @@ int ocpg_update_last_step(ocpg_ctx* c)
     if (!c || !c->cur) return OCPG_ESTATE;
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipMemcpyAsync(c->last, c->cur, c->lastBytes, hipMemcpyDeviceToDevice, c->stream));
+    // AllWells::UpdateLastTimeStepBHP
+    if (c->wells.nw > 0)
+        HIPCHK(hipMemcpyAsync(c->wells.lbhp, c->wells.bhp, (size_t)c->wells.nw * sizeof(double),
+                              hipMemcpyDeviceToDevice, c->stream));
     c->lastValid = true;
     return OCPG_OK;
 }
 
-// IsoT_FIM::ResetToLastTimeStep's Bulk part (src/OCPFluidMethod.cpp:2859-2895).
+// IsoT_FIM::ResetToLastTimeStep (src/OCPFluidMethod.cpp:2859-2913): the
+// Bulk and skip-stability state, then the wells' AllWells::ResetBHP,
+// CalTrans and (with ocpg_set_well_setup) CaldG on the restored state; the
+// caller's ocpg_calres does CalFlux and CalRes.
 int ocpg_reset_to_last_step(ocpg_ctx* c)
 {
     if (!c || !c->cur || !c->lastValid) return OCPG_ESTATE;
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipMemcpyAsync(c->cur, c->last, c->lastBytes, hipMemcpyDeviceToDevice, c->stream));
+    if (c->wells.nw > 0) HIPCHK(launchWellPrepare(c->NC, c->dEos, c->B, c->wells, 1, c->stream));
     return OCPG_OK;
 }
 
@@ -919,6 +938,8 @@ int64_t ocpg_device_bytes(ocpg_ctx* c)
 // Accessors used by the FIM half of the C-ABI (fim_capi.cpp).
 namespace ocpg {
 DevBulk& ctxBulk(ocpg_ctx* c) { return c->B; }
+const DevBulk& ctxLastBulk(ocpg_ctx* c) { return c->Blast; }
+bool ctxLastValid(ocpg_ctx* c) { return c->lastValid; }
 FimDevice& ctxFim(ocpg_ctx* c) { return c->fim; }
 WellDevice& ctxWells(ocpg_ctx* c) { return c->wells; }
 FimHost& ctxFimHost(ocpg_ctx* c) { return c->fimHost; }

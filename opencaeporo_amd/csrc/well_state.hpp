@@ -21,6 +21,17 @@ struct WellDevice {
     double *bhp = nullptr, *ctrlBHP = nullptr, *ctrlRate = nullptr;
     double *injZi = nullptr, *prodPW = nullptr;  // nw*nc, nw*3
     double *perfWI = nullptr, *perfMult = nullptr, *perfDG = nullptr, *perfXi = nullptr;
+    double* lbhp = nullptr;  // nw: Well::lbhp, BHP at the last time step
+    // AllWells::PrepareWell / CheckP inputs and state (ocpg_set_well_setup)
+    bool hasSetup = false;
+    int32_t* initMode = nullptr;   // nw: WellOpt::initOptMode (0 BHP, 1 rate)
+    int32_t* injPhase = nullptr;   // nw: WellOpt::injProdPhase of injectors (1 gas, 2 water)
+    double* wellDepth = nullptr;   // nw: Well::depth
+    double* injTemp = nullptr;     // nw: WellOpt::injTemp (F)
+    int32_t* perfState = nullptr;  // nperf: WellPerf::state (1 open, 0 closed)
+    double* perfP = nullptr;       // nperf: WellPerf::P as the last CaldG / CalFlux left it
+    int32_t* status = nullptr;     // [0]: AllWells::CheckP of the last ocpg_check_wells
+    double* perfTmp = nullptr;     // nperf: per-interval pressure increments of CaldG
     // Well::CalTrans / CalFlux / CalProdWeight results
     double* perfTransj = nullptr;  // nperf*3
     double* perfQi = nullptr;      // nperf*nc

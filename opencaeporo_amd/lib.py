@@ -61,6 +61,13 @@ def load():
         "ocpg_iteration_done": ([P], C.c_int),
         "ocpg_set_wells": ([P, P], C.c_int),
         "ocpg_set_well_state": ([P, P], C.c_int),
+        "ocpg_set_well_setup": ([P, P], C.c_int),
+        "ocpg_prepare_wells": ([P], C.c_int),
+        "ocpg_check_wells": ([P, P], C.c_int),
+        "ocpg_get_well_state": ([P, P, P, P, P, P], C.c_int),
+        "ocpg_phase_density": ([P, C.c_double, C.c_double, P, C.c_int, P, P], C.c_int),
+        "ocpg_check_bulk": ([P, P], C.c_int),
+        "ocpg_max_change": ([P, P], C.c_int),
         "ocpg_get_well_flux": ([P, P, P, P], C.c_int),
         "ocpg_calkrpc": ([P], C.c_int),
         "ocpg_calrock": ([P], C.c_int),
@@ -253,6 +260,49 @@ class Context:
     def set_well_state(self, wells):
         ws = wells.cstruct()
         check(self.lib.ocpg_set_well_state(self.h, C.byref(ws)), "ocpg_set_well_state")
+
+    def set_well_setup(self, setup):
+        """abi.WellSetup -> ocpg_set_well_setup (after set_wells)."""
+        su = setup.cstruct()
+        check(self.lib.ocpg_set_well_setup(self.h, C.byref(su)), "ocpg_set_well_setup")
+
+    def prepare_wells(self):
+        """AllWells::PrepareWell on the device."""
+        check(self.lib.ocpg_prepare_wells(self.h), "ocpg_prepare_wells")
+
+    def check_wells(self):
+        """AllWells::CheckP: abi.WELL_SUCCESS / WELL_NEGATIVE_PRESSURE / WELL_SWITCH."""
+        st = C.c_int32()
+        check(self.lib.ocpg_check_wells(self.h, C.byref(st)), "ocpg_check_wells")
+        return st.value
+
+    def get_well_state(self):
+        """(mode, bhp, perfDG, perfXi, perfMult) held on the device."""
+        w = self.wells
+        out = (np.zeros(w.nw, dtype=np.int32), np.zeros(w.nw), np.zeros(max(w.nperf, 1)),
+               np.zeros(max(w.nperf, 1)), np.zeros(max(w.nperf, 1)))
+        check(self.lib.ocpg_get_well_state(self.h, *[a.ctypes.data for a in out]), "ocpg_get_well_state")
+        return tuple(a[:w.nperf] if i >= 2 else a for i, a in enumerate(out))
+
+    def phase_density(self, P, T, zi, phase):
+        """MixtureComp::XiPhase / RhoPhase: (xi, rho)."""
+        zi = np.ascontiguousarray(zi, dtype=np.float64)
+        xi, rho = C.c_double(), C.c_double()
+        check(self.lib.ocpg_phase_density(self.h, P, T, zi.ctypes.data, phase, C.byref(xi), C.byref(rho)),
+              "ocpg_phase_density")
+        return xi.value, rho.value
+
+    def check_bulk(self):
+        """Bulk::CheckNi then CheckP: 0 or abi.BULK_NEGATIVE_*."""
+        st = C.c_int32()
+        check(self.lib.ocpg_check_bulk(self.h, C.byref(st)), "ocpg_check_bulk")
+        return st.value
+
+    def max_change(self):
+        """Bulk::CalMaxChange + AllWells::CalMaxBHPChange against the last step."""
+        out = abi.MaxChange()
+        check(self.lib.ocpg_max_change(self.h, C.byref(out)), "ocpg_max_change")
+        return dict(dPmax=out.dPmax, dNmax=out.dNmax, dSmax=out.dSmax, eVmax=out.eVmax, dBHPmax=out.dBHPmax)
 
     def get_well_flux(self):
         w = self.wells

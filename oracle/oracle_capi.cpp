@@ -43,6 +43,45 @@ int orc_calres(int nc, const ocpg_conn* conn, ocpg_bulk* bk, double dt, double* 
     return OCPG_OK;
 }
 
+// AllWells::PrepareWell (what 0) / ResetToLastTimeStep's well part (what 1)
+// on the wells' arrays in place; setup may be NULL (no CaldG / CheckOptMode).
+int orc_prepare_wells(const ocpg_eos* eos, ocpg_wells* w, const ocpg_well_setup* setup, ocpg_bulk* bk,
+                      double* perfTransj, double* perfP, int32_t* perfState, const double* lbhp, int what)
+{
+    if (!eos || !w || !bk || !perfTransj || !perfP || !perfState || (what == 1 && !lbhp)) return OCPG_EINVAL;
+    oracle::WellStep st{perfTransj, perfP, perfState, lbhp};
+    oracle::PrepareWells(*eos, *w, setup, *bk, st, what);
+    return OCPG_OK;
+}
+
+int orc_check_wells(ocpg_wells* w, ocpg_bulk* bk, double* perfTransj, double* perfP, int32_t* perfState,
+                    int32_t* status)
+{
+    if (!w || !bk || !perfTransj || !perfP || !perfState || !status) return OCPG_EINVAL;
+    oracle::WellStep st{perfTransj, perfP, perfState, nullptr};
+    *status = oracle::CheckWells(*w, *bk, st);
+    return OCPG_OK;
+}
+
+int orc_phase_density(const ocpg_eos* eos, double P, double T, const double* zi, int phase, double* out)
+{
+    if (!eos || !zi || !out) return OCPG_EINVAL;
+    oracle::Mixture mix(*eos);
+    mix.PhaseDensity(P, T, zi, phase == 2 ? 2 : 1, &out[0], &out[1]);
+    return OCPG_OK;
+}
+
+int orc_flash_phases(const ocpg_eos* eos, double P, double T, const double* Ni, double* out)
+{
+    // out: ex[3], vj[3], xi[3], rho[3], Nt
+    if (!eos || !Ni || !out) return OCPG_EINVAL;
+    oracle::Mixture mix(*eos);
+    bool ex[oracle::NPH];
+    out[12] = mix.FlashPhases(P, T, Ni, ex, out + 3, out + 6, out + 9);
+    for (int j = 0; j < oracle::NPH; j++) out[j] = ex[j] ? 1.0 : 0.0;
+    return OCPG_OK;
+}
+
 int orc_bsr_pattern(int64_t nb, const ocpg_conn* conn, int32_t* IA, int32_t* JA)
 {
     if (!conn || !IA || !JA) return OCPG_EINVAL;

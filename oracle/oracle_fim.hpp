@@ -25,6 +25,21 @@ struct WellFlux {
 };
 int OpenWells(const ocpg_wells* w);
 void CalWellFlux(const ocpg_eos& eos, const ocpg_wells& w, const ocpg_bulk& bk, WellFlux& out);
+// Per-step well state the preparation and checks change, besides the
+// fields of ocpg_wells they update in place (mode, bhp, perfDG, perfXi,
+// perfMult).
+struct WellStep {
+    double* perfTransj;  // nperf * 3
+    double* perfP;       // nperf: WellPerf::P
+    int32_t* perfState;  // nperf: 1 open, 0 closed
+    const double* lbhp;  // nw, for what = 1
+};
+// AllWells::PrepareWell (what = 0) or the well part of ResetToLastTimeStep
+// (what = 1: ResetBHP, CalTrans, CaldG, the perforation pressures).
+void PrepareWells(const ocpg_eos& eos, ocpg_wells& w, const ocpg_well_setup* setup, const ocpg_bulk& bk,
+                  WellStep& st, int what);
+// AllWells::CheckP: 0 success, 1 negative pressure, 2 switch / cross flow.
+int CheckWells(ocpg_wells& w, const ocpg_bulk& bk, WellStep& st);
 // With wells, resAbs holds (nb + open wells) * (nc + 1) entries.
 void CalRes(int nc, const ocpg_conn& conn, const ocpg_bulk& bk, double dt, double* resAbs,
             int32_t* upblock, double* upblock_Rho, double* upblock_Velocity, ocpg_resnorm* norms,

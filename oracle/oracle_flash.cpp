@@ -365,6 +365,65 @@ double Mixture::ProdWeight(double Pin, double Tin, const double* Niin, const dou
     return static_cast<double>(tmp);
 }
 
+double Mixture::FlashPhases(double Pin, double Tin, const double* Niin, bool* ex, double* vjo, double* xio,
+                            double* rhoo)
+{
+    ftype = 0;
+    lNP = 0;
+    InitPTN(Pin, Tin + CONV5, Niin);
+    CalFlash();
+    for (int j = 0; j < NPH - 1; j++) {
+        ex[j] = phaseExist[j];
+        vjo[j] = ex[j] ? static_cast<double>(vj[j]) : 0.0;
+        xio[j] = ex[j] ? static_cast<double>(xi[j]) : 0.0;
+        rhoo[j] = ex[j] ? static_cast<double>(rho[j]) : 0.0;
+    }
+    const int Wpid = numPhase - 1;
+    const int Wcid = numCom - 1;
+    real data[5], cdata[5];
+    PVTWEval(P, data, cdata);
+    const real bw = data[1] * (1 - data[2] * (P - data[0]));
+    ex[Wpid] = true;
+    xio[Wpid] = static_cast<double>(1 / (CONV1 * bw));
+    rhoo[Wpid] = static_cast<double>(std_RhoW / bw);
+    vjo[Wpid] = static_cast<double>(CONV1 * Ni[Wcid] * bw);
+    return static_cast<double>(Nh + Ni[Wcid]);
+}
+
+void Mixture::ProdRate(double Pin, double Tin, const double* Niin, double* rate)
+{
+    bool ex[NPH];
+    double v[NPH], x_[NPH], r[NPH];
+    FlashPhases(Pin, Tin, Niin, ex, v, x_, r);
+    rate[0] = v[0] / CONV1;
+    rate[1] = v[1] / 1000;
+    rate[2] = v[2] * x_[2];
+}
+
+void Mixture::PhaseDensity(double Pin, double Tin, const double* Ziin, int phase, double* xio, double* rhoo)
+{
+    if (phase == NPH - 1) {
+        real data[5], cdata[5];
+        PVTWEval(Pin, data, cdata);
+        const real bw = data[1] * (1 - data[2] * (Pin - data[0]));
+        *xio = static_cast<double>(1 / (CONV1 * bw));
+        *rhoo = static_cast<double>(std_RhoW / bw);
+        return;
+    }
+    InitPTZ(Pin, Tin + CONV5, Ziin);
+    NP = 1;
+    CalAiBi();
+    CalAjBj(Aj[0], Bj[0], zi);
+    SolEoS(Zj[0], Aj[0], Bj[0]);
+    real vtmp = Zj[0] * GAS_CONSTANT * T / P;
+    for (int i = 0; i < NC; i++) vtmp -= zi[i] * Vshift[i];
+    const real xitmp = 1 / vtmp;
+    real mw = 0;
+    for (int i = 0; i < NC; i++) mw += zi[i] * MWC[i];
+    *xio = static_cast<double>(xitmp);
+    *rhoo = static_cast<double>(mw * xitmp);
+}
+
 // MixtureComp::SolEoS, src/MixtureComp.cpp:889-914.
 void Mixture::SolEoS(real& ZjT, const real& AjT, const real& BjT)
 {

@@ -54,6 +54,15 @@ public:
     // flash does not produce contributes no volume (the reference reads what
     // its shared flash object last left in vj for it).
     double ProdWeight(double Pin, double Tin, const double* Niin, const double* prodPhase);
+    // MixtureComp::Flash (src/MixtureComp.cpp:167-193): per phase whether it
+    // exists, its volume, molar and mass density (zeros for a phase the flash
+    // does not produce); returns Nt.
+    double FlashPhases(double Pin, double Tin, const double* Niin, bool* ex, double* vjo, double* xio,
+                       double* rhoo);
+    // MixtureComp::CalProdRate (:855-865), absent phases contributing zero.
+    void ProdRate(double Pin, double Tin, const double* Niin, double* rate);
+    // MixtureComp::XiPhase / RhoPhase (:711-768); phase 2 is water.
+    void PhaseDensity(double Pin, double Tin, const double* Ziin, int phase, double* xio, double* rhoo);
 
     int NC, numCom, numPhase;
     // Outputs (Mixture base members, include/Mixture.hpp:241-292).

@@ -43,6 +43,10 @@ class Oracle:
         L.orc_bsr_pattern_wells.argtypes = [C.c_int64, P, P, P, P]
         L.orc_assemble_wells.argtypes = [P, P, P, P, C.c_double, P, P, P, P, P]
         L.orc_get_solution.argtypes = [C.c_int, P, P, C.c_double, P, P, P, P]
+        L.orc_prepare_wells.argtypes = [P, P, P, P, P, P, P, P, C.c_int]
+        L.orc_check_wells.argtypes = [P, P, P, P, P, P]
+        L.orc_phase_density.argtypes = [P, C.c_double, C.c_double, P, C.c_int, P]
+        L.orc_flash_phases.argtypes = [P, C.c_double, C.c_double, P, P]
         L.orc_nr_change.argtypes = [C.c_int64, P, P, P, P, P]
         L.orc_time_flash_fim.argtypes = [P, P, C.c_int]
         L.orc_time_flash_fim.restype = C.c_double
@@ -131,6 +135,39 @@ class Oracle:
 
 
     # ---- wells ----
+    def phase_density(self, eos, P, T, zi, phase):
+        """MixtureComp::XiPhase / RhoPhase: (xi, rho)."""
+        zi = np.ascontiguousarray(zi, dtype=np.float64)
+        out = np.zeros(2)
+        assert self.lib.orc_phase_density(C.byref(eos), P, T, zi.ctypes.data, phase, out.ctypes.data) == 0
+        return float(out[0]), float(out[1])
+
+    def flash_phases(self, eos, P, T, Ni):
+        """MixtureComp::Flash: (exist[3], vj[3], xi[3], rho[3], Nt)."""
+        Ni = np.ascontiguousarray(Ni, dtype=np.float64)
+        out = np.zeros(13)
+        assert self.lib.orc_flash_phases(C.byref(eos), P, T, Ni.ctypes.data, out.ctypes.data) == 0
+        return out[0:3] > 0, out[3:6], out[6:9], out[9:12], float(out[12])
+
+    def prepare_wells(self, eos, wells, bulk, step, setup=None, what=0):
+        """AllWells::PrepareWell (what 0) or the well part of
+        ResetToLastTimeStep (what 1) on ``wells`` in place; ``step`` is a
+        WellStepState (perfTransj, perfP, perfState, lbhp)."""
+        ws, s = wells.cstruct(), bulk.cstruct()
+        su = setup.cstruct() if setup is not None else None
+        rc = self.lib.orc_prepare_wells(C.byref(eos), C.byref(ws), C.byref(su) if su is not None else None,
+                                        C.byref(s), step.perfTransj.ctypes.data, step.perfP.ctypes.data,
+                                        step.perfState.ctypes.data, step.lbhp.ctypes.data, what)
+        assert rc == 0
+
+    def check_wells(self, wells, bulk, step):
+        """AllWells::CheckP: 0 success, 1 negative pressure, 2 switch / cross flow."""
+        ws, s = wells.cstruct(), bulk.cstruct()
+        st = np.zeros(1, dtype=np.int32)
+        rc = self.lib.orc_check_wells(C.byref(ws), C.byref(s), step.perfTransj.ctypes.data, step.perfP.ctypes.data,
+                                      step.perfState.ctypes.data, st.ctypes.data)
+        assert rc == 0
+        return int(st[0])
     def prod_weight(self, eos, Psurf, Tsurf, Ni, prod_phase):
         Ni = np.ascontiguousarray(Ni, dtype=np.float64)
         pp = np.ascontiguousarray(prod_phase, dtype=np.float64)
@@ -187,6 +224,18 @@ class Oracle:
         return val
 
 
+class WellStepState:
+    """The per-step perforation state the oracle's well preparation and
+    checks keep beside ocpg_wells: phase transmissibilities, perforation
+    pressures, open/closed state, and the last-step BHP."""
+
+    def __init__(self, wells):
+        self.perfTransj = np.zeros(max(wells.nperf, 1) * 3)
+        self.perfP = np.zeros(max(wells.nperf, 1))
+        self.perfState = np.ones(max(wells.nperf, 1), dtype=np.int32)
+        self.lbhp = wells.bhp.copy()
+
+
 class FlopCounter(Oracle):
     def __init__(self):
         super().__init__(COUNT_SO)
@@ -240,6 +289,8 @@ class Reference:
         L.ref_flash_fim.argtypes = [P, P]
         L.ref_init_flash_fim.argtypes = [P, P, P]
         L.ref_prod_weight.argtypes = [P, C.c_double, C.c_double, P, P, P]
+        L.ref_phase_density.argtypes = [P, C.c_double, C.c_double, P, C.c_int, P]
+        L.ref_flash_phases.argtypes = [P, C.c_double, C.c_double, P, P]
         self.deck = deck
         self.h = L.ref_create(C.byref(deck), nb)
 
@@ -264,6 +315,18 @@ class Reference:
         assert self.lib.ref_prod_weight(self.h, Psurf, Tsurf, Ni.ctypes.data, pp.ctypes.data,
                                         out.ctypes.data) == 0
         return float(out[0])
+
+    def phase_density(self, P, T, zi, phase):
+        zi = np.ascontiguousarray(zi, dtype=np.float64)
+        out = np.zeros(2)
+        assert self.lib.ref_phase_density(self.h, P, T, zi.ctypes.data, phase, out.ctypes.data) == 0
+        return float(out[0]), float(out[1])
+
+    def flash_phases(self, P, T, Ni):
+        Ni = np.ascontiguousarray(Ni, dtype=np.float64)
+        out = np.zeros(13)
+        assert self.lib.ref_flash_phases(self.h, P, T, Ni.ctypes.data, out.ctypes.data) == 0
+        return out[0:3] > 0, out[3:6], out[6:9], out[9:12], float(out[12])
 
     def __del__(self):
         try:

@@ -6,6 +6,11 @@
 //   AssembleWells : IsoT_FIM::AssembleMatWellsNew (src/OCPFluidMethod.cpp:
 //                   2417-2436) -> AssembleMatInjWellsNew (:2438-2580),
 //                   AssembleMatProdWellsNew (:2582-2733)
+//   PrepareWells  : AllWells::PrepareWell (src/AllWells.cpp:201-214):
+//                   Well::CalTrans, CaldG (CalInjdG :367-435, CalProddG01
+//                   :438-555), CheckOptMode (:893-944), CalFlux's ReCalXi
+//   CheckWells    : AllWells::CheckP (src/AllWells.cpp:335-369), Well::CheckP
+//                   and Well::CheckCrossFlow (src/Well.cpp:946-1066)
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -116,6 +121,274 @@ void CalWellFlux(const ocpg_eos& eos, const ocpg_wells& W, const ocpg_bulk& bk, 
             }
         }
     }
+}
+
+// Well::CalTrans, src/Well.cpp:164-202 (transj per perforation and phase).
+static void WellTrans(const ocpg_wells& W, const ocpg_bulk& bk, int w, double* transj)
+{
+    for (int p = W.perfStart[w]; p < W.perfStart[w + 1]; p++) {
+        const int64_t k = W.perfCell[p];
+        const double temp = CONV1 * W.perfWI[p] * W.perfMult[p];
+        for (int j = 0; j < NPH; j++) {
+            transj[(size_t)p * NPH + j] = 0;
+            const int64_t id = k * NPH + j;
+            if (bk.phaseExist[id]) transj[(size_t)p * NPH + j] = temp * bk.kr[id] / bk.mu[id];
+        }
+    }
+}
+
+static double TransInj(const ocpg_wells& W, const ocpg_bulk& bk, const double* transj, int w, int p)
+{
+    if (W.unweight && W.unweight[w]) return W.perfWI[p];
+    double t = 0;
+    const int64_t k = W.perfCell[p];
+    for (int j = 0; j < NPH; j++)
+        if (bk.phaseExist[k * NPH + j]) t += transj[(size_t)p * NPH + j];
+    return t;
+}
+
+// Well::CaldG -> CalInjdG / CalProddG01 (src/Well.cpp:357-555).
+static void WellCaldG(Mixture& mix, const ocpg_wells& W, const ocpg_well_setup& S, const ocpg_bulk& bk, int w,
+                      WellStep& st)
+{
+    const int nc = mix.numCom;
+    const int p0 = W.perfStart[w];
+    const int numPerf = W.perfStart[w + 1] - p0;
+    if (numPerf <= 0) return;
+    double* dG = const_cast<double*>(W.perfDG) + p0;
+    const double bhp = W.bhp[w];
+    const double depth = S.depth[w];
+    const double maxlen = 10;
+    std::vector<double> dGperf(numPerf, 0.0);
+    std::vector<double> tmpNi(nc, 0.0);
+    auto pdepth = [&](int q) { return bk.depth[W.perfCell[p0 + q]]; };
+    const bool inj = W.type[w] == OCPG_WELL_INJ;
+    const double* zInj = W.injZi + (size_t)w * nc;
+    // one segment's pressure change: the injected fluid's density, or the
+    // volume-weighted density of the flashed perforation stream
+    auto rhoAt = [&](double Ptmp, int64_t n) {
+        if (inj) {
+            double xi, rho;
+            mix.PhaseDensity(Ptmp, S.injTemp[w], zInj, S.injPhase[w] == 2 ? 2 : 1, &xi, &rho);
+            return rho;
+        }
+        bool ex[NPH];
+        double vj[NPH], xi[NPH], rho[NPH];
+        mix.FlashPhases(Ptmp, bk.T[n], tmpNi.data(), ex, vj, xi, rho);
+        double qtacc = 0, rhoacc = 0;
+        for (int j = 0; j < NPH; j++) {
+            if (!ex[j]) continue;
+            qtacc += vj[j];
+            rhoacc += vj[j] * rho[j];
+        }
+        return rhoacc / qtacc;
+    };
+    auto stream = [&](int p, int64_t n) {
+        for (int j = 0; j < NPH; j++) {
+            const int64_t id = n * NPH + j;
+            if (!bk.phaseExist[id]) continue;
+            for (int k = 0; k < nc; k++)
+                tmpNi[k] += (bk.P[n] - st.perfP[p]) * st.perfTransj[(size_t)p * NPH + j] * bk.xi[id] *
+                            bk.xij[id * nc + k];
+        }
+        double tmpSum = 0;
+        for (int k = 0; k < nc; k++) tmpSum += std::fabs(tmpNi[k]);
+        if (tmpSum < TINY)
+            for (int i = 0; i < nc; i++) tmpNi[i] = bk.Ni[n * nc + i];
+    };
+    if (depth <= pdepth(0)) {
+        // well above its perforations
+        for (int q = numPerf - 1; q >= 0; q--) {
+            const double dz = q == 0 ? pdepth(0) - depth : pdepth(q) - pdepth(q - 1);
+            const int seg_num = (int)std::ceil(std::fabs(dz / maxlen));
+            if (seg_num == 0) continue;
+            const double seg_len = dz / seg_num;
+            const int p = p0 + q;
+            const int64_t n = W.perfCell[p];
+            st.perfP[p] = bhp + dG[q];
+            const double Pperf = st.perfP[p];
+            double Ptmp = Pperf;
+            if (!inj) stream(p, n);
+            for (int i = 0; i < seg_num; i++) Ptmp -= rhoAt(Ptmp, n) * GRAVITY_FACTOR * seg_len;
+            dGperf[q] = Pperf - Ptmp;
+        }
+        dG[0] = dGperf[0];
+        for (int q = 1; q < numPerf; q++) dG[q] = dG[q - 1] + dGperf[q];
+    } else if (depth >= pdepth(numPerf - 1)) {
+        // well below its perforations
+        for (int q = 0; q < numPerf; q++) {
+            const double dz = q == numPerf - 1 ? depth - pdepth(numPerf - 1) : pdepth(q + 1) - pdepth(q);
+            const int seg_num = (int)std::ceil(std::fabs(dz / maxlen));
+            if (seg_num == 0) continue;
+            const double seg_len = dz / seg_num;
+            const int p = p0 + q;
+            const int64_t n = W.perfCell[p];
+            st.perfP[p] = bhp + dG[q];
+            const double Pperf = st.perfP[p];
+            double Ptmp = Pperf;
+            if (!inj) stream(p, n);
+            for (int i = 0; i < seg_num; i++) Ptmp += rhoAt(Ptmp, n) * GRAVITY_FACTOR * seg_len;
+            dGperf[q] = Ptmp - Pperf;
+        }
+        dG[numPerf - 1] = dGperf[numPerf - 1];
+        for (int q = numPerf - 2; q >= 0; q--) dG[q] = dG[q + 1] + dGperf[q];
+    }
+}
+
+// Well::CheckOptMode (src/Well.cpp:893-944).
+static void WellCheckOptMode(Mixture& mix, ocpg_wells& W, const ocpg_well_setup& S, const ocpg_bulk& bk, int w,
+                             WellStep& st)
+{
+    const int nc = mix.numCom;
+    int32_t* mode = const_cast<int32_t*>(W.mode);
+    double* bhp = const_cast<double*>(W.bhp);
+    const int p0 = W.perfStart[w], p1 = W.perfStart[w + 1];
+    const bool inj = W.type[w] == OCPG_WELL_INJ;
+    const int initMode = S.initMode[w];
+    if (inj) {
+        // Well::CalInjRateMaxBHP (:263-284)
+        double q = 0;
+        for (int p = p0; p < p1; p++) {
+            const double Pperf = W.ctrlBHP[w] + W.perfDG[p];
+            const int64_t k = W.perfCell[p];
+            double xi, rho;
+            mix.PhaseDensity(Pperf, S.injTemp[w], W.injZi + (size_t)w * nc, S.injPhase[w] == 2 ? 2 : 1, &xi, &rho);
+            const double dP = Pperf - bk.P[k];
+            q += TransInj(W, bk, st.perfTransj, w, p) * xi * dP;
+        }
+        if (q > W.ctrlRate[w]) {
+            mode[w] = initMode == OCPG_WELL_BHP ? OCPG_WELL_RATE : initMode;
+        } else {
+            mode[w] = OCPG_WELL_BHP;
+            bhp[w] = W.ctrlBHP[w];
+        }
+        return;
+    }
+    if (initMode == OCPG_WELL_BHP) {
+        mode[w] = OCPG_WELL_BHP;
+        bhp[w] = W.ctrlBHP[w];
+        return;
+    }
+    // Well::CalProdRateMinBHP (:289-320)
+    std::vector<double> qi(nc, 0.0);
+    for (int p = p0; p < p1; p++) {
+        const double Pperf = W.ctrlBHP[w] + W.perfDG[p];
+        const int64_t k = W.perfCell[p];
+        for (int j = 0; j < NPH; j++) {
+            const int64_t id = k * NPH + j;
+            if (!bk.phaseExist[id]) continue;
+            const double dP = bk.Pj[id] - Pperf;
+            const double temp = st.perfTransj[(size_t)p * NPH + j] * bk.xi[id] * dP;
+            for (int i = 0; i < nc; i++) qi[i] += bk.xij[id * nc + i] * temp;
+        }
+    }
+    double rate[NPH];
+    mix.ProdRate(W.Psurf, W.Tsurf, qi.data(), rate);
+    double q = 0;
+    for (int j = 0; j < NPH; j++) q += rate[j] * W.prodPhaseWeight[(size_t)w * 3 + j];
+    if (q > W.ctrlRate[w]) {
+        mode[w] = initMode;
+    } else {
+        mode[w] = OCPG_WELL_BHP;
+        bhp[w] = W.ctrlBHP[w];
+    }
+}
+
+void PrepareWells(const ocpg_eos& eos, ocpg_wells& W, const ocpg_well_setup* S, const ocpg_bulk& bk,
+                  WellStep& st, int what)
+{
+    Mixture mix(eos);
+    const int nc = eos.nc + 1;
+    double* bhp = const_cast<double*>(W.bhp);
+    double* perfXi = const_cast<double*>(W.perfXi);
+    for (int w = 0; w < W.nw; w++) {
+        if (!W.open[w]) continue;
+        if (what == 1) {
+            // AllWells::ResetBHP: bhp = lbhp, Well::CorrectBHP
+            bhp[w] = st.lbhp[w];
+            if (W.mode[w] == OCPG_WELL_BHP) bhp[w] = W.ctrlBHP[w];
+        }
+        WellTrans(W, bk, w, st.perfTransj);
+        if (S) WellCaldG(mix, W, *S, bk, w, st);
+        if (what == 0 && S) WellCheckOptMode(mix, W, *S, bk, w, st);
+        for (int p = W.perfStart[w]; p < W.perfStart[w + 1]; p++) {
+            st.perfP[p] = bhp[w] + W.perfDG[p];
+            if (what == 0 && S && W.type[w] == OCPG_WELL_INJ) {
+                double xi, rho;
+                mix.PhaseDensity(st.perfP[p], S->injTemp[w], W.injZi + (size_t)w * nc, S->injPhase[w] == 2 ? 2 : 1,
+                                 &xi, &rho);
+                perfXi[p] = xi;
+            }
+        }
+    }
+}
+
+int CheckWells(ocpg_wells& W, const ocpg_bulk& bk, WellStep& st)
+{
+    int32_t* mode = const_cast<int32_t*>(W.mode);
+    double* bhp = const_cast<double*>(W.bhp);
+    double* mult = const_cast<double*>(W.perfMult);
+    bool flagSwitch = false, flagCross = false;
+    for (int w = 0; w < W.nw; w++) {
+        if (!W.open[w]) continue;
+        const int p0 = W.perfStart[w], p1 = W.perfStart[w + 1];
+        const bool inj = W.type[w] == OCPG_WELL_INJ;
+        // Well::CheckP
+        if (bhp[w] < 0) return 1;
+        for (int p = p0; p < p1; p++)
+            if (st.perfState[p] == 1 && st.perfP[p] < 0) return 1;
+        if (inj) {
+            if (mode[w] != OCPG_WELL_BHP && bhp[w] > W.ctrlBHP[w]) {
+                mode[w] = OCPG_WELL_BHP;
+                bhp[w] = W.ctrlBHP[w];
+                flagSwitch = true;
+                continue;
+            }
+        } else if (mode[w] != OCPG_WELL_BHP && bhp[w] < W.ctrlBHP[w]) {
+            mode[w] = OCPG_WELL_BHP;
+            bhp[w] = W.ctrlBHP[w];
+            flagSwitch = true;
+            continue;
+        }
+        // Well::CheckCrossFlow
+        bool flagC = true;
+        for (int p = p0; p < p1; p++) {
+            const double Pk = bk.P[W.perfCell[p]];
+            if (!inj) {
+                if (st.perfState[p] == 1 && Pk < st.perfP[p]) {
+                    st.perfState[p] = 0;
+                    mult[p] = 0;
+                    flagC = false;
+                    break;
+                } else if (st.perfState[p] == 0 && Pk > st.perfP[p]) {
+                    st.perfState[p] = 1;
+                    mult[p] = 1;
+                }
+            } else {
+                if (st.perfState[p] == 1 && Pk > st.perfP[p]) {
+                    st.perfState[p] = 0;
+                    mult[p] = 0;
+                    flagC = false;
+                    break;
+                } else if (st.perfState[p] == 0 && Pk < st.perfP[p]) {
+                    st.perfState[p] = 1;
+                    mult[p] = 1;
+                }
+            }
+        }
+        bool flag = false;
+        for (int p = p0; p < p1; p++)
+            if (st.perfState[p] == 1) flag = true;
+        if (!flag && p1 > p0) {
+            st.perfState[p1 - 1] = 1;
+            mult[p1 - 1] = 1;
+        }
+        if (!flagC) {
+            WellTrans(W, bk, w, st.perfTransj);
+            flagCross = true;
+        }
+    }
+    return (flagSwitch || flagCross) ? 2 : 0;
 }
 
 // DaABpbC, src/DenseMat.cpp:75-94 (row-major, accumulated over k in order).

@@ -277,4 +277,36 @@ int ref_prod_weight(void* p, double Psurf, double Tsurf, const double* Ni, const
     return OCPG_OK;
 }
 
+// MixtureComp::XiPhase and MixtureComp::RhoPhase of one feed: out[0] xi,
+// out[1] rho (phase 1 gas, 2 water).
+int ref_phase_density(void* p, double P, double T, const double* zi, int phase, double* out)
+{
+    RefHandle* h = static_cast<RefHandle*>(p);
+    if (!h || !zi || !out) return OCPG_EINVAL;
+    Quiet q;
+    const USI ph = phase == 2 ? WATER : GAS;
+    out[0] = h->mix->XiPhase(P, T, zi, ph);
+    out[1] = h->mix->RhoPhase(P, 0, T, zi, ph);
+    return OCPG_OK;
+}
+
+// MixtureComp::Flash of the moles Ni (nc): out = phaseExist[3], vj[3],
+// xi[3], rho[3] (zeros where a phase does not exist), Nt.
+int ref_flash_phases(void* p, double P, double T, const double* Ni, double* out)
+{
+    RefHandle* h = static_cast<RefHandle*>(p);
+    if (!h || !Ni || !out) return OCPG_EINVAL;
+    Quiet q;
+    h->mix->Flash(P, T, Ni);
+    for (int j = 0; j < 3; j++) {
+        const bool ex = h->mix->phaseExist[j];
+        out[j] = ex ? 1.0 : 0.0;
+        out[3 + j] = ex ? h->mix->vj[j] : 0.0;
+        out[6 + j] = ex ? h->mix->xi[j] : 0.0;
+        out[9 + j] = ex ? h->mix->rho[j] : 0.0;
+    }
+    out[12] = h->mix->Nt;
+    return OCPG_OK;
+}
+
 }  // extern "C"

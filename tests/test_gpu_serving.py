@@ -79,3 +79,25 @@ def test_result_is_the_same_alone_and_in_the_mix():
         alone.evict()
         for a, b in zip(mixed, ref):
             np.testing.assert_array_equal(a, b)
+
+
+def test_reallocation_drops_the_captured_iteration():
+    """A captured iteration holds the addresses it was captured with: after
+    set_cells / set_conn reallocate, launching it is refused (ESTATE)."""
+    gpu_util.say("[serving] captured graph dropped on reallocation")
+    r = models(1, 2000, 2000, seed=4)[0]
+    r.initialize()
+    ctx = lib.Context(r.eos, r.sat, r.rock)
+    for realloc in ("set_conn", "set_cells"):
+        ctx.set_cells(r.nb)
+        ctx.upload(r.host)
+        ctx.set_conn(r.grid)
+        ctx.save_flash_inputs()
+        ctx.capture_iteration(serving.DT)
+        if realloc == "set_conn":
+            ctx.set_conn(r.grid)
+        else:
+            ctx.set_cells(r.nb)
+        with pytest.raises(lib.OcpgError, match="code -4"):
+            ctx.launch_iteration()
+    ctx.close()
