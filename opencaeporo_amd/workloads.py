@@ -184,3 +184,60 @@ def irregular_wells(grid, P, NC, seed, n_wells=40):
             d["perfXi"] = rng.uniform(0.4, 0.6, len(perf))
         wells.append(d)
     return abi.Wells(nc, wells)
+
+
+def peaceman_wi_z(dx, dy, dz, kx, ky, radius=0.25, skin=0.0):
+    """Well::CalWI_Peaceman (src/Well.cpp:93-161) for a vertical (z)
+    perforation: equivalent radius from the cell's dx, dy and kx/ky, kh from
+    dz and sqrt(kx ky); CONV2 * 2 PI * kh / (ln(ro / rw) + skin)."""
+    conv2, pi = 1.12712e-3, 3.141592653
+    kxky = kx * ky
+    kx_ky = kx / ky
+    ro = 0.28 * (dx * dx * (1 / kx_ky) ** 0.5 + dy * dy * kx_ky ** 0.5) ** 0.5
+    ro /= kx_ky ** 0.25 + (1 / kx_ky) ** 0.25
+    kh = dz * kxky ** 0.5
+    return conv2 * (2 * pi) * kh / (np.log(ro / radius) + skin)
+
+
+# The solvent of examples/spe5/spe5.data (WELLSTRE): C1 0.77, C3 0.20, C6 0.03.
+SOLVENT6 = (0.77, 0.20, 0.03, 0.0, 0.0, 0.0)
+
+
+def config1(seed=20241204, inj_mscf=20000.0, prod_stb=5000.0, max_bhp=10000.0, min_bhp=1000.0, sw=0.2):
+    """The plain method's 147-cell case: a 7 x 7 x 3 Cartesian grid of
+    500 x 500 x (20, 30, 50) ft cells (tops 8325 ft), porosity 0.3,
+    ln k ~ N(ln 100 mD, 0.5^2) isotropic, P = 4000 psia, T = 160 F, Sw = 0.2,
+    z ~ Dirichlet(1) per cell, a gas injector (rate control, BHP limit) in
+    column (1, 1) and an oil-rate producer (BHP limit) in column (7, 7), both
+    perforated in every layer.
+
+    Returns dict(grid, P, z, Sw, wells=spec list, setup=(depth, injTemp,
+    injPhase, initMode), inj_mscf).  The injector's rate is in Mscf/day:
+    the caller converts it with MixtureComp::SetupWellOpt's factor
+    1000 * XiPhase(Psurf, Tsurf, zi, GAS) (src/MixtureComp.cpp:790-795)."""
+    nx, ny, nz = 7, 7, 3
+    rng = np.random.default_rng(seed)
+    nb = nx * ny * nz
+    k = np.exp(rng.normal(np.log(100.0), 0.5, nb))
+    dz = np.repeat([20.0, 30.0, 50.0], nx * ny)
+    grid = Grid(nx, ny, nz, 500.0, 500.0, dz, 8325.0, (k, k, k))
+    P = np.full(nb, 4000.0)
+    z = rng.dirichlet(np.ones(6), nb)
+    Sw = np.full(nb, sw)
+    nxny = nx * ny
+    col = lambda i, j: np.array([(kk * ny + j) * nx + i for kk in range(nz)], dtype=np.int32)
+    inj_cells, prod_cells = col(0, 0), col(nx - 1, ny - 1)
+    wi = lambda cells: np.array([peaceman_wi_z(500.0, 500.0, dz[c], k[c], k[c]) for c in cells])
+    nc = 7
+    zi = np.zeros(nc)
+    zi[:6] = SOLVENT6
+    wells = [
+        dict(type=abi.WELL_INJ, mode=abi.WELL_RATE, perf=inj_cells, bhp=float(P[inj_cells[0]]), ctrlBHP=max_bhp,
+             ctrlRate=0.0, WI=wi(inj_cells), dG=np.zeros(nz), injZi=zi, perfXi=np.zeros(nz)),
+        dict(type=abi.WELL_PROD, mode=abi.WELL_RATE, perf=prod_cells, bhp=float(P[prod_cells[0]]),
+             ctrlBHP=min_bhp, ctrlRate=prod_stb, WI=wi(prod_cells), dG=np.zeros(nz),
+             prodPhaseWeight=[1.0, 0.0, 0.0]),
+    ]
+    depth = [float(grid.depth[inj_cells[0]]), float(grid.depth[prod_cells[0]])]
+    setup = (depth, [160.0, 160.0], [1, 1], [abi.WELL_RATE, abi.WELL_RATE])
+    return dict(grid=grid, P=P, z=z, Sw=Sw, wells=wells, setup=setup, inj_mscf=inj_mscf, nxny=nxny)
