@@ -32,6 +32,7 @@ sys.path.insert(0, ROOT)
 from opencaeporo_amd import abi, fluids, lib, workloads  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
+FP64_PEAK_TFS = 78.6      # MI355X FP64 vector
 DT = 5.0
 # Algorithmic FP64 work of the warm flash pass per cell (add, sub, mul, div,
 # sqrt; transcendental calls counted apart), measured with the oracle's
@@ -119,6 +120,10 @@ def setup(nx, ny, nz, seed=2024, irregular=False, model=None):
 
 def step(ctx, serial=False):
     ctx.restore_flash_inputs()
+    step_body(ctx, serial)
+
+
+def step_body(ctx, serial=False):
     if serial:
         ctx.flash_fim(stats=False)
         ctx.calkrpc_rock()
@@ -128,57 +133,102 @@ def step(ctx, serial=False):
         ctx.update_assemble(DT)
 
 
-def cpu_baseline(nthreads=None, sample_layers=6, seed=2024):
-    """The oracle (CPU restatement) on the host: the same warm pass on a
-    60 x 220 x sample_layers slice; flash with OpenMP over cells, the rest
-    single-threaded as the restatement runs it."""
+# Fields a warm Newton pass starts from (what ocpg_save_flash_inputs keeps,
+# plus the static cell data and the last step's moles CalRes reads).
+PASS_INPUTS = ["P", "T", "Ni", "lNi", "S", "phaseNum", "xij", "skipFlag", "skipMinEig", "skipP", "skipT", "skipZi",
+               "v", "depth", "poroInit"]
+
+
+def host_cpu():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit():
+        threads = min(threads, int(omp))
+    return model, os.cpu_count() or 1, max(threads, 1)
+
+
+def cpu_baseline(ctx, grid, NC, reps=5, slice_layers=2, seed=2024):
+    """The oracle (CPU restatement, OpenMP over cells for the flash and over
+    rows for CalRes and the assembly) timed on the host on the GPU's own warm
+    input state, downloaded from the device: every host thread the process
+    may use (the box grants OMP_NUM_THREADS) on all cells, and one thread on
+    the first slice_layers layers (the same cell distribution; the all-thread
+    rate on that slice is reported beside it).  A cold pass is the same input
+    with no previous phase information and empty skip memory."""
     from oracle.oracle_py import Oracle
     orc = Oracle()
-    nthreads = nthreads or min(os.cpu_count() or 1, 16)
-    deck = fluids.bench6_deck()
-    eos = lib.eos_from_deck(deck)
+    eos = ctx.eos
     sat = fluids.make_satfunc()
     rock = fluids.make_rock()
-    NC = eos.nc
-    grid = workloads.cartesian_grid(60, 220, sample_layers, seed=seed)
+    model, nproc, threads = host_cpu()
     nb = grid.nb
-    P, z, Sw = workloads.cell_inputs(nb, NC, seed=seed + 1)
-    b = abi.Bulk(nb, NC)
-    b.P[:] = P
-    b.T[:] = workloads.T_F
-    b.S.reshape(nb, 3)[:, 2] = Sw
-    b.v[:] = grid.v
-    b.depth[:] = grid.depth
-    b.poroInit[:] = 0.3
-    orc.calrock(rock, b, nthreads)
-    orc.init_flash_fim(eos, b, z, nthreads)
-    orc.calkrpc(sat, b, nthreads)
-    b.lNi[:] = b.Ni
-    b.phaseNum[:] = 0
-    b.skipFlag[:] = 0
-    orc.flash_fim(eos, b, nthreads)
-    rng = np.random.default_rng(seed + 2)
-    b.P[:] += rng.uniform(-5.0, 5.0, nb)
-    orc.calkrpc(sat, b, nthreads)
-    orc.calrock(rock, b, nthreads)
-    warm = b.copy()
-    times = []
-    for _ in range(3):
-        w = warm.copy()
-        t0 = time.perf_counter()
-        orc.flash_fim(eos, w, nthreads)
-        orc.calkrpc(sat, w, nthreads)
-        orc.calrock(rock, w, nthreads)
-        nc = NC + 1
-        res, up, urho, _, _ = orc.calres(nc, grid, w, DT)
-        IA, JA = orc.bsr_pattern(grid)
-        orc.assemble(nc, grid, w, DT, up, urho, IA, JA)
-        times.append(time.perf_counter() - t0)
-    t = float(np.median(times))
-    return dict(value=nb / t, unit="cell-updates/s", cores=nthreads, kind="port",
-                sample=f"oracle warm pass on 60x220x{sample_layers} = {nb} cells "
-                       f"(flash + kr/Pc + rock with {nthreads} OpenMP threads, CalRes + "
-                       f"assembly 1 thread), median of {len(times)}")
+    nc = NC + 1
+    ctx.restore_flash_inputs()
+    warm = abi.Bulk(nb, NC)
+    inp = abi.Bulk(nb, NC, fields=PASS_INPUTS)
+    ctx.download(inp)
+    for f in PASS_INPUTS:
+        getattr(warm, f)[:] = getattr(inp, f)
+    del inp
+    IA, JA = orc.bsr_pattern(grid)
+    val = np.zeros(int(IA[-1]) * (nc + 1) ** 2)
+
+    def sub(b, n1):
+        """The first n1 cells (whole layers) with their own connections."""
+        s = abi.Bulk(n1, NC)
+        for k, a in b.arrays.items():
+            per = a.size // nb
+            s.arrays[k][:] = a[:n1 * per]
+        m = grid.eId < n1
+        g = type("G", (), {})()
+        g.nb, g.bId, g.eId, g.area = n1, grid.bId[m], grid.eId[m], grid.area[m]
+        g.nconn = int(m.sum())
+        return s, g
+
+    def timed(b0, g, nt, cold, n_rep, vbuf):
+        IA_, JA_ = (IA, JA) if g is grid else orc.bsr_pattern(g)
+        if vbuf is None or vbuf.size != int(IA_[-1]) * (nc + 1) ** 2:
+            vbuf = np.zeros(int(IA_[-1]) * (nc + 1) ** 2)
+        times = []
+        for _ in range(n_rep):
+            w = b0.copy()
+            if cold:
+                w.phaseNum[:] = 0
+                w.skipFlag[:] = 0
+            t0 = time.perf_counter()
+            orc.flash_fim(eos, w, nt)
+            orc.calkrpc(sat, w, nt)
+            orc.calrock(rock, w, nt)
+            res, up, urho, _, _ = orc.calres(nc, g, w, DT, nthreads=nt)
+            orc.assemble(nc, g, w, DT, up, urho, IA_, JA_, nthreads=nt, out=vbuf)
+            times.append(time.perf_counter() - t0)
+        return g.nb / float(np.median(times)), times
+
+    warm_all, t_wa = timed(warm, grid, threads, False, reps, val)
+    n1 = 60 * 220 * slice_layers
+    sb, sg = sub(warm, n1)
+    del val
+    cold_all, t_ca = timed(sb, sg, threads, True, reps, None)
+    warm_slice_all, _ = timed(sb, sg, threads, False, reps, None)
+    warm_one, t_w1 = timed(sb, sg, 1, False, reps, None)
+    cold_one, t_c1 = timed(sb, sg, 1, True, reps, None)
+    return dict(value=warm_all, unit="cell-updates/s", cores=threads, kind="port",
+                sample=(f"oracle warm Newton pass (FlashFIM + kr/Pc + rock + CalRes + assembly) on the GPU's own "
+                        f"warm input, all {nb} cells with {threads} OpenMP threads, median of {reps}; the cold "
+                        f"pass and the one-thread rates on the first {slice_layers} layers ({n1} cells)"),
+                cpu_model=model, nproc=nproc, threads=threads,
+                warm_all_threads=warm_all, cold_all_threads_slice=cold_all,
+                warm_all_threads_slice=warm_slice_all, warm_one_thread_slice=warm_one,
+                cold_one_thread_slice=cold_one, reps=reps,
+                seconds=dict(warm_all=t_wa, cold_all_slice=t_ca, warm_one_slice=t_w1, cold_one_slice=t_c1))
 
 
 def main():
@@ -190,6 +240,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--handoff-steps", type=int, default=3,
                     help="steps timed again with the BSR copied to page-locked host arrays (PCIe-inclusive rate)")
+    ap.add_argument("--cold-steps", type=int, default=3,
+                    help="cold passes (no previous phase information) timed after the warm steps")
     ap.add_argument("--serial", action="store_true",
                     help="one Newton iteration as four serial calls (flash, kr/Pc + rock, CalRes, assembly) "
                          "instead of ocpg_update_assemble")
@@ -248,6 +300,22 @@ def main():
         ctx.free_pinned(arrays)
         log(f"[bench] with the BSR handed to pinned host arrays: {handoff_ms:.2f} ms per step")
     log("[bench] per-launch ms: " + ", ".join(f"{k} {v:.3f}" for k, v in per.items()))
+    # the cold pass (no previous phase information, empty skip memory: every
+    # cell takes the full stability test) on the same input, timed per step
+    cold_ms = None
+    if not side_case and a.cold_steps > 0:
+        flags = abi.Bulk(nb, NC, fields=["phaseNum", "skipFlag"])
+        ts = []
+        for _ in range(a.cold_steps):
+            ctx.restore_flash_inputs()
+            ctx.upload(flags)
+            ctx.synchronize()
+            t0c = time.perf_counter()
+            step_body(ctx, a.serial)
+            ctx.synchronize()
+            ts.append((time.perf_counter() - t0c) * 1e3)
+        cold_ms = float(np.median(ts))
+        log(f"[bench] cold pass: {cold_ms:.2f} ms per step (median of {len(ts)})")
     value = nb / (ms_step / 1e3)
     nnzb = ctx.nnzb()
     nc, ncol = NC + 1, NC + 2
@@ -294,10 +362,26 @@ def main():
         roof["traffic"] = ASM_TRAFFIC_BYTES
         roof_pack["traffic"] = PACK_TRAFFIC_BYTES
     other = [roof_pack]
+    # the flash against FP64 vector peak (its algorithmic flops per warm cell,
+    # over the main stream's flash time), and the whole step against the
+    # step's roofline time: flash flops at FP64 peak plus the assembly's
+    # algorithmic bytes at HBM peak
+    flash_roof = step_roof = None
+    if not side_case and nz == 85:
+        flops = WARM_FLOPS_PER_CELL * nb
+        flash_roof = dict(bound="fp64", flops_per_launch=flops, achieved=flops / (per["flash"] / 1e3) / 1e12,
+                          peak=FP64_PEAK_TFS, unit="TFLOP/s")
+        flash_roof["frac"] = flash_roof["achieved"] / flash_roof["peak"]
+        t_roof = flops / (FP64_PEAK_TFS * 1e12) + asm_bytes / (HBM_PEAK_GBS * 1e9)
+        step_roof = dict(t_roof_ms=t_roof * 1e3, t_measured_ms=ms_step, frac=t_roof * 1e3 / ms_step)
     cpu = None
     if rank == 0 and not a.no_cpu and not side_case:
         try:
-            cpu = cpu_baseline()
+            t0c = time.time()
+            cpu = cpu_baseline(ctx, grid, NC)
+            log(f"[bench] cpu baseline ({cpu['threads']} threads, {cpu['cpu_model']}): warm "
+                f"{cpu['warm_all_threads']:.4g} cell-updates/s all threads, one thread "
+                f"{cpu['warm_one_thread_slice']:.4g} ({time.time() - t0c:.1f}s)")
         except Exception as e:  # the baseline is informative only
             log(f"[bench] cpu baseline failed: {e}")
     if model is not None:
@@ -331,6 +415,10 @@ def main():
                    "kernel_ms": per, "dominant_kernel": dominant, "flash_ms": flash_ms,
                    "flash_flops_per_cell": WARM_FLOPS_PER_CELL, "asm_bytes_per_launch": asm_bytes,
                    "c_bar": grid.nconn / nb, "dsec_rows_mean": rows_mean, "roofline_other": other,
+                   "cold_ms_per_step": cold_ms,
+                   "cold_cell_updates_per_s": (nb / (cold_ms / 1e3)) if cold_ms else None,
+                   "roofline_flash": flash_roof,
+                   "roofline_step": step_roof,
                    "pcie_inclusive_ms_per_step": handoff_ms,
                    "pcie_inclusive_cell_updates_per_s": (nb / (handoff_ms / 1e3)) if handoff_ms else None},
         "roofline": roof,

@@ -43,6 +43,16 @@ int orc_calres(int nc, const ocpg_conn* conn, ocpg_bulk* bk, double dt, double* 
     return OCPG_OK;
 }
 
+// The same with nthreads OpenMP threads (identical results).
+int orc_calres_mt(int nc, const ocpg_conn* conn, ocpg_bulk* bk, double dt, double* resAbs, int32_t* upblock,
+                  double* upblock_Rho, double* upblock_Velocity, ocpg_resnorm* norms, int nthreads)
+{
+    if (!conn || !bk || !resAbs || !upblock || !upblock_Rho || !upblock_Velocity) return OCPG_EINVAL;
+    oracle::CalRes(nc, *conn, *bk, dt, resAbs, upblock, upblock_Rho, upblock_Velocity, norms, nullptr, nullptr,
+                   nthreads);
+    return OCPG_OK;
+}
+
 // AllWells::PrepareWell (what 0) / ResetToLastTimeStep's well part (what 1)
 // on the wells' arrays in place; setup may be NULL (no CaldG / CheckOptMode).
 int orc_prepare_wells(const ocpg_eos* eos, ocpg_wells* w, const ocpg_well_setup* setup, ocpg_bulk* bk,
@@ -95,6 +105,15 @@ int orc_assemble_bulks(int nc, const ocpg_conn* conn, ocpg_bulk* bk, double dt,
 {
     if (!conn || !bk || !upblock || !upblock_Rho || !IA || !JA || !val) return OCPG_EINVAL;
     oracle::AssembleBulks(nc, *conn, *bk, dt, upblock, upblock_Rho, IA, JA, val);
+    return OCPG_OK;
+}
+
+int orc_assemble_bulks_mt(int nc, const ocpg_conn* conn, ocpg_bulk* bk, double dt, const int32_t* upblock,
+                          const double* upblock_Rho, const int32_t* IA, const int32_t* JA, double* val,
+                          int nthreads)
+{
+    if (!conn || !bk || !upblock || !upblock_Rho || !IA || !JA || !val) return OCPG_EINVAL;
+    oracle::AssembleBulks(nc, *conn, *bk, dt, upblock, upblock_Rho, IA, JA, val, nthreads);
     return OCPG_OK;
 }
 

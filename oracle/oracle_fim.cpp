@@ -231,29 +231,64 @@ void CalRock(const ocpg_rock& rock, const ocpg_bulk& bk, int nthreads)
     }
 }
 
+// Each cell's connections in ascending index order (CSR), and each
+// connection's position in its two cells' lists: the slot order in which
+// LinearSystem::NewOffDiag fills a row while the connection loop runs.
+CellConn CellConnections(int64_t nb, const ocpg_conn& conn)
+{
+    CellConn cc;
+    const int64_t nconn = conn.nconn;
+    cc.ptr.assign(nb + 1, 0);
+    for (int64_t c = 0; c < nconn; c++) {
+        cc.ptr[conn.bId[c] + 1]++;
+        cc.ptr[conn.eId[c] + 1]++;
+    }
+    for (int64_t n = 0; n < nb; n++) cc.ptr[n + 1] += cc.ptr[n];
+    cc.conn.resize(cc.ptr[nb]);
+    cc.posB.resize(nconn);
+    cc.posE.resize(nconn);
+    std::vector<int64_t> fill(cc.ptr.begin(), cc.ptr.end() - 1);
+    for (int64_t c = 0; c < nconn; c++) {
+        const int64_t b = conn.bId[c], e = conn.eId[c];
+        cc.posE[c] = (int32_t)(fill[e] - cc.ptr[e]);
+        cc.conn[fill[e]++] = c;
+        cc.posB[c] = (int32_t)(fill[b] - cc.ptr[b]);
+        cc.conn[fill[b]++] = c;
+    }
+    return cc;
+}
+
 // IsoT_FIM::CalRes (src/OCPFluidMethod.cpp:1201-1389): bulk-to-bulk part
 // serial in connection order as the reference accumulates, then the well
 // sources and well equations (:1284-1352), norms and negation.
 void CalRes(int nc, const ocpg_conn& conn, const ocpg_bulk& bk, double dt, double* resAbs,
             int32_t* upblock, double* upblock_Rho, double* upblock_Velocity, ocpg_resnorm* norms,
-            const ocpg_wells* wells, const WellFlux* wf)
+            const ocpg_wells* wells, const WellFlux* wf, int nthreads)
 {
     const int np = NPH;
     const int len = nc + 1;
     const int64_t nb = bk.nb;
     const int64_t nrow = nb + OpenWells(wells);
     std::memset(resAbs, 0, sizeof(double) * nrow * len);
+#pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1) schedule(static)
     for (int64_t n = 0; n < nb; n++) {
         const int64_t bId = n * len;
         const int64_t bIdb = n * nc;
         resAbs[bId] = bk.rockVp[n] - bk.vf[n];
         for (int i = 0; i < nc; i++) resAbs[bId + 1 + i] = bk.Ni[bIdb + i] - bk.lNi[bIdb + i];
     }
-    for (int64_t c = 0; c < conn.nconn; c++) {
+    // Per connection (independent): upwinding, and per phase the flux factor
+    // dt Akd xi kr / mu dP of the upstream cell (flag: the phase flows).
+    const int64_t nconn = conn.nconn;
+    std::vector<double> fac((size_t)nconn * np);
+    std::vector<uint8_t> flows((size_t)nconn * np);
+#pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1) schedule(static)
+    for (int64_t c = 0; c < nconn; c++) {
         const int64_t bId = conn.bId[c];
         const int64_t eId = conn.eId[c];
         const double Akd = CONV1 * CONV2 * conn.area[c];
         for (int j = 0; j < np; j++) {
+            flows[c * np + j] = 0;
             const int64_t bId_np_j = bId * np + j;
             const int64_t eId_np_j = eId * np + j;
             const bool exbegin = bk.phaseExist[bId_np_j];
@@ -283,11 +318,32 @@ void CalRes(int nc, const ocpg_conn& conn, const ocpg_bulk& bk, double dt, doubl
                 upblock_Velocity[c * np + j] = 0;
                 continue;
             }
-            const double tmp = dt * Akd * bk.xi[uId_np_j] * bk.kr[uId_np_j] / bk.mu[uId_np_j] * dP;
-            for (int i = 0; i < nc; i++) {
-                const double dNi = tmp * bk.xij[uId_np_j * nc + i];
-                resAbs[bId * len + 1 + i] += dNi;
-                resAbs[eId * len + 1 + i] -= dNi;
+            fac[c * np + j] = dt * Akd * bk.xi[uId_np_j] * bk.kr[uId_np_j] / bk.mu[uId_np_j] * dP;
+            flows[c * np + j] = 1;
+        }
+    }
+    // Per row: its connections in the order the reference's connection loop
+    // reaches them (ascending index), each adding (bId side) or subtracting
+    // (eId side) its components' fluxes -- the same sums in the same order
+    // as the connection-ordered scatter, so the rows are independent.
+    const CellConn cc = CellConnections(nb, conn);
+#pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1) schedule(static)
+    for (int64_t n = 0; n < nb; n++) {
+        double* r = &resAbs[n * len + 1];
+        for (int64_t q = cc.ptr[n]; q < cc.ptr[n + 1]; q++) {
+            const int64_t c = cc.conn[q];
+            const bool isB = conn.bId[c] == n;
+            for (int j = 0; j < np; j++) {
+                if (!flows[c * np + j]) continue;
+                const double tmp = fac[c * np + j];
+                const int64_t uId_np_j = (int64_t)upblock[c * np + j] * np + j;
+                for (int i = 0; i < nc; i++) {
+                    const double dNi = tmp * bk.xij[uId_np_j * nc + i];
+                    if (isB)
+                        r[i] += dNi;
+                    else
+                        r[i] -= dNi;
+                }
             }
         }
     }
@@ -337,6 +393,7 @@ void CalRes(int nc, const ocpg_conn& conn, const ocpg_bulk& bk, double dt, doubl
             }
         }
     }
+#pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1) schedule(static)
     for (int64_t k = 0; k < nrow * len; k++) resAbs[k] = -resAbs[k];
     if (norms) *norms = nr;
 }
@@ -392,7 +449,7 @@ static void DaABpbC(int m, int n, int k, const double* A, const double* B, doubl
 // whose pattern BsrPattern() produced.  val has IA[nb]*ncol*ncol entries.
 void AssembleBulks(int nc, const ocpg_conn& conn, const ocpg_bulk& bk, double dt,
                    const int32_t* upblock, const double* upblock_Rho, const int32_t* IA,
-                   const int32_t* JA, double* val)
+                   const int32_t* JA, double* val, int nthreads)
 {
     const int np = NPH;
     const int ncol = nc + 1;
@@ -401,21 +458,27 @@ void AssembleBulks(int nc, const ocpg_conn& conn, const ocpg_bulk& bk, double dt
     const int bsize2 = ncol * ncol2;
     const int lendSdP = maxLendSdP(nc);
     const int64_t nb = bk.nb;
-    std::vector<int32_t> fill(nb);
-    for (int64_t n = 0; n < nb; n++) fill[n] = IA[n] + 1;
-    std::memset(val, 0, sizeof(double) * (size_t)IA[nb] * bsize);
-
-    std::vector<double> bmat(bsize, 0);
-    for (int i = 1; i < ncol; i++) bmat[i * ncol + i] = 1;
+    const int nt = nthreads > 0 ? nthreads : 1;
+    const CellConn cc = CellConnections(nb, conn);
+#pragma omp parallel for num_threads(nt) schedule(static)
     for (int64_t n = 0; n < nb; n++) {
-        bmat[0] = bk.v[n] * bk.poroP[n] - bk.vfP[n];
-        for (int i = 0; i < nc; i++) bmat[i + 1] = -bk.vfi[n * nc + i];
-        std::memcpy(&val[(size_t)IA[n] * bsize], bmat.data(), sizeof(double) * bsize);
+        double* d = &val[(size_t)IA[n] * bsize];
+        std::memset(d, 0, sizeof(double) * (size_t)(IA[n + 1] - IA[n]) * bsize);
+        for (int i = 1; i < ncol; i++) d[i * ncol + i] = 1;
+        d[0] = bk.v[n] * bk.poroP[n] - bk.vfP[n];
+        for (int i = 0; i < nc; i++) d[i + 1] = -bk.vfi[n * nc + i];
     }
 
+    // Per connection (independent): both sides' condensed blocks, written to
+    // the two off-diagonal slots the reference's NewOffDiag calls fill
+    // ((eId, bId): -b side, (bId, eId): e side).  The diagonals follow.
+#pragma omp parallel num_threads(nt)
+    {
+    std::vector<double> bmat(bsize, 0);
     std::vector<double> dFdXpB(bsize), dFdXpE(bsize), dFdXsB(bsize2), dFdXsE(bsize2);
     bool phaseExistB[NPH], phaseExistE[NPH], phasedS_B[NPH], phasedS_E[NPH];
     int pVnumComB[NPH], pVnumComE[NPH];
+#pragma omp for schedule(static)
     for (int64_t c = 0; c < conn.nconn; c++) {
         const int64_t bId = conn.bId[c];
         const int64_t eId = conn.eId[c];
@@ -567,9 +630,7 @@ void AssembleBulks(int nc, const ocpg_conn& conn, const ocpg_bulk& bk, double dt
         DaABpbC(ncol, ncol, ncolB, dFdXsB.data(), &bk.dSec_dPri[bId * lendSdP], bmat.data());
         for (auto& v : bmat) v = dt * v;
         {
-            double* d = &val[(size_t)IA[bId] * bsize];
-            for (int q = 0; q < bsize; q++) d[q] += bmat[q];
-            double* o = &val[(size_t)fill[eId]++ * bsize];
+            double* o = &val[(size_t)(IA[eId] + 1 + cc.posE[c]) * bsize];
             for (int q = 0; q < bsize; q++) o[q] = -1.0 * bmat[q];
         }
         // e side: NewOffDiag(bId, eId, bmat) ; AddDiag(eId, -bmat)
@@ -577,10 +638,25 @@ void AssembleBulks(int nc, const ocpg_conn& conn, const ocpg_bulk& bk, double dt
         DaABpbC(ncol, ncol, ncolE, dFdXsE.data(), &bk.dSec_dPri[eId * lendSdP], bmat.data());
         for (auto& v : bmat) v = dt * v;
         {
-            double* o = &val[(size_t)fill[bId]++ * bsize];
+            double* o = &val[(size_t)(IA[bId] + 1 + cc.posB[c]) * bsize];
             for (int q = 0; q < bsize; q++) o[q] = bmat[q];
-            double* d = &val[(size_t)IA[eId] * bsize];
-            for (int q = 0; q < bsize; q++) d[q] += -1.0 * bmat[q];
+        }
+    }
+    }
+    // Per row: AddDiag in the connection loop's order.  The b side added
+    // bmat, the block its (eId, bId) slot holds negated; the e side added
+    // -bmat, its (bId, eId) slot negated: either way the negation of the
+    // mirror block, so the diagonal is the reference's sum term by term.
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (int64_t n = 0; n < nb; n++) {
+        double* d = &val[(size_t)IA[n] * bsize];
+        for (int64_t q = cc.ptr[n]; q < cc.ptr[n + 1]; q++) {
+            const int64_t c = cc.conn[q];
+            const bool isB = conn.bId[c] == n;
+            const int64_t m = isB ? conn.eId[c] : conn.bId[c];
+            const int32_t pos = isB ? cc.posE[c] : cc.posB[c];
+            const double* o = &val[(size_t)(IA[m] + 1 + pos) * bsize];
+            for (int k = 0; k < bsize; k++) d[k] += -o[k];
         }
     }
     (void)JA;

@@ -41,16 +41,26 @@ void PrepareWells(const ocpg_eos& eos, ocpg_wells& w, const ocpg_well_setup* set
 // AllWells::CheckP: 0 success, 1 negative pressure, 2 switch / cross flow.
 int CheckWells(ocpg_wells& w, const ocpg_bulk& bk, WellStep& st);
 // With wells, resAbs holds (nb + open wells) * (nc + 1) entries.
+// Rows are computed in parallel (nthreads OpenMP threads), each summing its
+// connections' terms in the reference's connection order: the result does
+// not depend on nthreads.
 void CalRes(int nc, const ocpg_conn& conn, const ocpg_bulk& bk, double dt, double* resAbs,
             int32_t* upblock, double* upblock_Rho, double* upblock_Velocity, ocpg_resnorm* norms,
-            const ocpg_wells* wells = nullptr, const WellFlux* wf = nullptr);
+            const ocpg_wells* wells = nullptr, const WellFlux* wf = nullptr, int nthreads = 1);
+// Each cell's connections in ascending index order; posB / posE: a
+// connection's rank in its bId / eId cell's list.
+struct CellConn {
+    std::vector<int64_t> ptr, conn;
+    std::vector<int32_t> posB, posE;
+};
+CellConn CellConnections(int64_t nb, const ocpg_conn& conn);
 void BsrPattern(int64_t nb, const ocpg_conn& conn, int32_t* IA, int32_t* JA,
                 const ocpg_wells* wells = nullptr);
 void AssembleWells(int nc, const ocpg_conn& conn, const ocpg_wells& w, const WellFlux& wf,
                    const ocpg_bulk& bk, double dt, const int32_t* IA, double* val);
 void AssembleBulks(int nc, const ocpg_conn& conn, const ocpg_bulk& bk, double dt,
                    const int32_t* upblock, const double* upblock_Rho, const int32_t* IA,
-                   const int32_t* JA, double* val);
+                   const int32_t* JA, double* val, int nthreads = 1);
 
 // IsoT_FIM::GetSolution (bulk part) and the Newton-change bookkeeping of
 // the next PassFlashValue / Bulk::CalNRdSmax.

@@ -37,6 +37,8 @@ class Oracle:
         L.orc_calres.argtypes = [C.c_int, P, P, C.c_double, P, P, P, P, P]
         L.orc_bsr_pattern.argtypes = [C.c_int64, P, P, P]
         L.orc_assemble_bulks.argtypes = [C.c_int, P, P, C.c_double, P, P, P, P, P]
+        L.orc_calres_mt.argtypes = [C.c_int, P, P, C.c_double, P, P, P, P, P, C.c_int]
+        L.orc_assemble_bulks_mt.argtypes = [C.c_int, P, P, C.c_double, P, P, P, P, P, C.c_int]
         L.orc_well_flux.argtypes = [P, P, P, P, P, P, P]
         L.orc_prod_weight.argtypes = [P, C.c_double, C.c_double, P, P, P]
         L.orc_calres_wells.argtypes = [P, P, P, P, C.c_double, P, P, P, P, P]
@@ -101,7 +103,7 @@ class Oracle:
         s = bulk.cstruct()
         assert self.lib.orc_calrock(C.byref(rock), C.byref(s), nthreads) == 0
 
-    def calres(self, nc, grid, bulk, dt):
+    def calres(self, nc, grid, bulk, dt, nthreads=1):
         conn = abi.conn_struct(grid.bId, grid.eId, grid.area)
         res = np.zeros(bulk.nb * (nc + 1))
         up = np.zeros(grid.nconn * abi.NP, dtype=np.int32)
@@ -109,9 +111,9 @@ class Oracle:
         uvel = np.zeros(grid.nconn * abi.NP)
         norms = abi.ResNorm()
         s = bulk.cstruct()
-        rc = self.lib.orc_calres(nc, C.byref(conn), C.byref(s), dt, res.ctypes.data,
-                                 up.ctypes.data, urho.ctypes.data, uvel.ctypes.data,
-                                 C.byref(norms))
+        rc = self.lib.orc_calres_mt(nc, C.byref(conn), C.byref(s), dt, res.ctypes.data,
+                                    up.ctypes.data, urho.ctypes.data, uvel.ctypes.data,
+                                    C.byref(norms), nthreads)
         assert rc == 0
         return res, up, urho, uvel, norms
 
@@ -122,14 +124,14 @@ class Oracle:
         assert self.lib.orc_bsr_pattern(grid.nb, C.byref(conn), IA.ctypes.data, JA.ctypes.data) == 0
         return IA, JA
 
-    def assemble(self, nc, grid, bulk, dt, up, urho, IA, JA):
+    def assemble(self, nc, grid, bulk, dt, up, urho, IA, JA, nthreads=1, out=None):
         conn = abi.conn_struct(grid.bId, grid.eId, grid.area)
         ncol = nc + 1
-        val = np.zeros(int(IA[-1]) * ncol * ncol)
+        val = out if out is not None else np.zeros(int(IA[-1]) * ncol * ncol)
         s = bulk.cstruct()
-        rc = self.lib.orc_assemble_bulks(nc, C.byref(conn), C.byref(s), dt, up.ctypes.data,
-                                         urho.ctypes.data, IA.ctypes.data, JA.ctypes.data,
-                                         val.ctypes.data)
+        rc = self.lib.orc_assemble_bulks_mt(nc, C.byref(conn), C.byref(s), dt, up.ctypes.data,
+                                            urho.ctypes.data, IA.ctypes.data, JA.ctypes.data,
+                                            val.ctypes.data, nthreads)
         assert rc == 0
         return val
 
