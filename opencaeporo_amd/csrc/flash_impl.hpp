@@ -50,6 +50,20 @@ struct FlashKernel {
         // ---- CalMW, CalVfXiRho, CalViscoLBC (per internal phase) ----
         double MW[2], vC[2], xiC[2], rhoC[2], muC[2], muA[2][5];
         double vf = 0;
+        // The LBC terms' powers of Tc, Pc and Tr: the same expressions the
+        // reference evaluates per phase and, in the derivatives, per (k, i)
+        // pair, evaluated once (pow is a pure function: the same values).
+        double visT[NC], tc16[NC], pc23[NC];
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            tc16[i] = pow(E.Tc[i], 1.0 / 6);
+            pc23[i] = pow(E.Pc[i], 2.0 / 3);
+            const double Tri = T / E.Tc[i];
+            if (Tri <= 1.5)
+                visT[i] = 34 * 1E-5 * pow(Tri, 0.94);
+            else
+                visT[i] = 17.78 * 1E-5 * pow((4.58 * Tri - 1.67), 0.625);
+        }
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             if (j >= NP) break;
@@ -68,6 +82,7 @@ struct FlashKernel {
             xiC[j] = 1 / tmp;
             rhoC[j] = MW[j] * xiC[j];
         }
+        double xT16[2], xP23[2];
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             if (j >= NP) break;
@@ -75,12 +90,8 @@ struct FlashKernel {
             const double sMW = sqrt(MW[j]);
 #pragma unroll
             for (int i = 0; i < NC; ++i) {
-                double tmp = 5.4402 * pow(E.Tc[i], 1.0 / 6) / sMW / pow(E.Pc[i], 2.0 / 3);
-                const double Tri = T / E.Tc[i];
-                if (Tri <= 1.5)
-                    tmp = 34 * 1E-5 * pow(Tri, 0.94) / tmp;
-                else
-                    tmp = 17.78 * 1E-5 * pow((4.58 * Tri - 1.67), 0.625) / tmp;
+                double tmp = 5.4402 * tc16[i] / sMW / pc23[i];
+                tmp = visT[i] / tmp;
                 m0 += F.x[j][i] * E.sqrtMWi[i] * tmp;
                 m1 += F.x[j][i] * E.sqrtMWi[i];
                 xT += F.x[j][i] * E.Tc[i];
@@ -89,7 +100,9 @@ struct FlashKernel {
             }
             muA[j][0] = m0;
             muA[j][1] = m1;
-            muA[j][2] = 5.4402 * pow(xT, 1.0 / 6) / sMW / pow(xP, 2.0 / 3);
+            xT16[j] = pow(xT, 1.0 / 6);
+            xP23[j] = pow(xP, 2.0 / 3);
+            muA[j][2] = 5.4402 * xT16[j] / sMW / xP23[j];
             muA[j][3] = xiC[j] * xV;
             const double a3 = muA[j][3];
             const double* L = E.LBCcoef;
@@ -345,6 +358,7 @@ struct FlashKernel {
             B.rhoP[s * ld + n] = rhoP;
             B.xiP[s * ld + n] = xiP;
             B.muP[s * ld + n] = muP;
+            const double xTm56 = pow(xTj, -5.0 / 6);
 #pragma unroll
             for (int k = 0; k < NC; ++k) {
                 // der1IJ of the reference is -(1 / 2) * ... = 0 (integer division)
@@ -353,21 +367,16 @@ struct FlashKernel {
                 for (int i = 0; i < NC; ++i) {
                     const double val1IJ = E.muAux1I[i] / sMW;
                     const double der1IJ = 0 * E.muAux1I[i] * pMW15 * E.MWC[k];
-                    const double Tri = T / E.Tc[i];
-                    double tmp;
-                    if (Tri <= 1.5)
-                        tmp = 34 * 1E-5 * pow(Tri, 0.94);
-                    else
-                        tmp = 17.78 * 1E-5 * pow(4.58 * Tri - 1.67, 0.625);
+                    const double tmp = visT[i];
                     const double val2IJ = tmp / val1IJ;
                     const double der2IJ = -tmp * der1IJ / (val1IJ * val1IJ);
                     der3J += F.x[j][i] * E.sqrtMWi[i] * der2IJ + (i == k ? 1.0 : 0.0) * E.sqrtMWi[k] * val2IJ;
                 }
                 const double der4J = E.sqrtMWi[k];
                 const double der6J = 5.4402 *
-                                     (1.0 / 6 * pow(xTj, -5.0 / 6) * E.Tc[k] -
-                                      pow(xTj, 1.0 / 6) * (0.5 / MW[j] * E.MWC[k] + 2.0 / 3 / xPj * E.Pc[k])) /
-                                     (sMW * pow(xPj, 2.0 / 3));
+                                     (1.0 / 6 * xTm56 * E.Tc[k] -
+                                      xT16[j] * (0.5 / MW[j] * E.MWC[k] + 2.0 / 3 / xPj * E.Pc[k])) /
+                                     (sMW * xP23[j]);
                 const double der7J = xix[k] * xVj + xiC[j] * E.Vcvis[k];
                 const double der8J = der7J * poly;
                 const double mux = (der3J * muA[j][1] - muA[j][0] * der4J) / (muA[j][1] * muA[j][1]) +
@@ -453,8 +462,9 @@ struct FlashKernel {
             }
         }
 
-        // ---- CalSkipForNextStep ----
-        if (F.flagSkip && F.ftype == 0) {
+        // ---- CalSkipForNextStep (only a one-phase result keeps the skip
+        // flag: PhaseEquilibrium clears it when NP > 1) ----
+        if constexpr (NPT == 1) if (F.flagSkip && F.ftype == 0) {
             // CalPhiNSkip (src/MixtureComp.cpp:5155-5217) + AssembleSkipMatSTA
             const double aj = F.Aj[0], bj = F.Bj[0], zj = F.Zj[0], nu0 = F.nu[0];
             const double* xj = F.x[0];
