@@ -474,6 +474,34 @@ int ocpg_iteration_done(ocpg_ctx* ctx);
 int ocpg_get_upwind(ocpg_ctx* ctx, int32_t* upblock, double* upblock_Rho,
                     double* upblock_Velocity);
 
+/* Serving pool: many realisations (ensemble members of their own sizes) in
+ * one context, each at a reserved range of cells, connection slots and BSR
+ * blocks, and a per-cell activity mask; the captured iteration then works
+ * on the realisations whose mask is on, in one set of launches.  A pool's
+ * realisations carry no wells.  No reference counterpart: the reference runs
+ * one reservoir per process.
+ *   ocpg_pool_init: capacity cells of at most max_neighbours neighbours each
+ *     (reallocates like ocpg_set_cells; drops a captured iteration);
+ *   ocpg_pool_place: a realisation's cell state (the layout of
+ *     ocpg_upload_bulk, state->nb cells) and connections (local indices, the
+ *     rules of ocpg_set_conn) at cells [base, base + nb); its iteration
+ *     inputs are saved at the same time; the range must be free;
+ *   ocpg_pool_activate: whether it takes part in the iterations launched
+ *     after the call on the context's stream (stream-ordered);
+ *   ocpg_pool_system: its rows of the last assembly in its own indexing (what
+ *     ocpg_get_bsr returns for it alone); NULL outputs are skipped;
+ *   ocpg_pool_download: its cell state, the saved iteration inputs
+ *     (saved != 0) or the current values;
+ *   ocpg_pool_remove: frees its range.
+ * Placing and removing keep a captured iteration valid.                      */
+int ocpg_pool_init(ocpg_ctx* ctx, int64_t capacity, int32_t max_neighbours);
+int ocpg_pool_place(ocpg_ctx* ctx, int64_t base, const ocpg_bulk* state, const ocpg_conn* conn);
+int ocpg_pool_activate(ocpg_ctx* ctx, int64_t base, int on);
+int ocpg_pool_system(ocpg_ctx* ctx, int64_t base, int64_t* nnzb, int32_t* IA, int32_t* JA, double* val,
+                     double* rhs);
+int ocpg_pool_download(ocpg_ctx* ctx, int64_t base, ocpg_bulk* state, int saved);
+int ocpg_pool_remove(ocpg_ctx* ctx, int64_t base);
+
 /* Copy / restore of the fields the flash starts from (P, Ni, S, phaseNum,
  * xij, skip-stability memory), to re-run one Newton iteration from the same
  * state (benchmarks, retries).  No reference counterpart.                    */

@@ -65,7 +65,12 @@ struct DevBulk {
     double *kr, *Pc, *Pj, *dKr_dS, *dPcj_dS;
     double *v, *depth, *poroInit, *poro, *poroP, *rockVp;
     int32_t *ftype, *iters;
+    // serving pool: per cell 1 when its realisation takes part in the
+    // iteration (NULL: every cell does)
+    const uint8_t* mask;
 };
+
+__device__ __forceinline__ bool cellOn(const DevBulk& B, int64_t n) { return !B.mask || B.mask[n]; }
 
 // Scratch of the staged flash (cells in SoA with the bulk's ld; lists of
 // cell indices, their lengths in count[0, kFlashLists) and their work-queue

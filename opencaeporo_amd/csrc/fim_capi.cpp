@@ -33,6 +33,10 @@ NewtonDevice& ctxNewton(ocpg_ctx* c);
 void ctxDropGraph(ocpg_ctx* c);
 const DevBulk& ctxLastBulk(ocpg_ctx* c);
 bool ctxLastValid(ocpg_ctx* c);
+uint8_t* ctxEnsureMask(ocpg_ctx* c);
+int ctxDropMask(ocpg_ctx* c);
+int ctxRangeCopy(ocpg_ctx* c, int64_t base, ocpg_bulk* h, bool toDevice, bool saved);
+int ctxSaveInputsRange(ocpg_ctx* c, int64_t base, int64_t nb);
 }  // namespace ocpg
 
 
@@ -362,6 +366,11 @@ int ocpg_set_conn(ocpg_ctx* c, const ocpg_conn* conn)
     H.bId.assign(conn->bId, conn->bId + nconn);
     H.eId.assign(conn->eId, conn->eId + nconn);
     H.wells = WellsHost{};
+    H.pool = PoolHost{};
+    {
+        const int rc = ctxDropMask(c);
+        if (rc != OCPG_OK) return rc;
+    }
     FimDevice& F = ctxFim(c);
     ctxDropGraph(c);
     fimFree(F);
@@ -391,6 +400,252 @@ int ocpg_set_conn(ocpg_ctx* c, const ocpg_conn* conn)
     return buildPattern(c);
 }
 
+// ---------------------------------------------------------------------------
+// Serving pool: many realisations in one context, each at a reserved range of
+// cells, connection slots and BSR blocks, and an activity mask that selects
+// the realisations an iteration (the captured one included) works on.  The
+// kernels are the ones of an ordinary context; a realisation's rows only
+// reach its own cells, so realisations never see each other.
+
+// Capacity cells with at most maxNb neighbours each; every cell starts
+// outside any realisation (diagonal-only rows, mask off).
+int ocpg_pool_init(ocpg_ctx* c, int64_t capacity, int32_t maxNb)
+{
+    if (!c || capacity <= 0 || maxNb < 1 || maxNb > 64) return OCPG_EINVAL;
+    const int64_t bpc = 1 + (int64_t)maxNb;
+    if (capacity * bpc > (int64_t)INT32_MAX) return OCPG_ETOOBIG;
+    int rc = ocpg_set_cells(c, capacity);
+    if (rc != OCPG_OK) return rc;
+    HIPCHK2(hipSetDevice(ctxDevice(c)));
+    FimHost& H = ctxFimHost(c);
+    H.bId.clear();
+    H.eId.clear();
+    H.wells = WellsHost{};
+    H.pool = PoolHost{};
+    H.pool.capacity = capacity;
+    H.pool.maxNb = maxNb;
+    FimDevice& F = ctxFim(c);
+    WellDevice& W = ctxWells(c);
+    ctxDropGraph(c);
+    fimFree(F);
+    if (W.mem) HIPCHK2(hipFree(W.mem));
+    W = WellDevice{};
+    const int NC = ctxNC(c);
+    const int64_t nconn = capacity * maxNb, nnzb = capacity * bpc;
+    F.nb = capacity;
+    F.nconn = nconn;
+    F.ncol = NC + 2;
+    F.normBlocks = (int)((capacity + 255) / 256);
+    F.nnzb = nnzb;
+    F.nrow = capacity;
+    const int ncol = F.ncol;
+    size_t tot = 0;
+    HIPCHK2(devAlloc(F.bId, nconn, tot));
+    HIPCHK2(devAlloc(F.eId, nconn, tot));
+    HIPCHK2(devAlloc(F.area, nconn, tot));
+    HIPCHK2(devAlloc(F.upblock, nconn * 3, tot));
+    HIPCHK2(devAlloc(F.upRho, nconn * 3, tot));
+    HIPCHK2(devAlloc(F.upVel, nconn * 3, tot));
+    HIPCHK2(devAlloc(F.rec, (size_t)capacity * recordDoubles(NC), tot));
+    HIPCHK2(devAlloc(F.own, (size_t)capacity * ownDoubles(NC), tot));
+    HIPCHK2(devAlloc(F.normPart, (size_t)F.normBlocks * 2, tot));
+    HIPCHK2(devAlloc(F.normIdx, (size_t)F.normBlocks * 2, tot));
+    const size_t before = tot;
+    HIPCHK2(devAlloc(F.IA, capacity + 1, tot));
+    HIPCHK2(devAlloc(F.JA, nnzb, tot));
+    HIPCHK2(devAlloc(F.slotConn, nnzb, tot));
+    HIPCHK2(devAlloc(F.slotMirror, nnzb, tot));
+    HIPCHK2(devAlloc(F.bulkEnd, capacity, tot));
+    HIPCHK2(devAlloc(F.res, capacity * ncol, tot));
+    HIPCHK2(devAlloc(F.val, (size_t)nnzb * ncol * ncol, tot));
+    F.patternBytes = tot - before;
+    F.bytes = tot;
+    HIPCHK2(hipMemset(F.bId, 0, nconn * sizeof(int32_t)));
+    HIPCHK2(hipMemset(F.eId, 0, nconn * sizeof(int32_t)));
+    HIPCHK2(hipMemset(F.area, 0, nconn * sizeof(double)));
+    HIPCHK2(hipMemset(F.upblock, 0, nconn * 3 * sizeof(int32_t)));
+    HIPCHK2(hipMemset(F.upRho, 0, nconn * 3 * sizeof(double)));
+    HIPCHK2(hipMemset(F.upVel, 0, nconn * 3 * sizeof(double)));
+    HIPCHK2(hipMemset(F.res, 0, capacity * ncol * sizeof(double)));
+    {
+        std::vector<int32_t> IA(capacity + 1), JA(nnzb, 0), sc(nnzb, -1), sm(nnzb), be(capacity);
+        for (int64_t n = 0; n <= capacity; n++) IA[n] = (int32_t)(n * bpc);
+        for (int64_t n = 0; n < capacity; n++) {
+            for (int64_t k = IA[n]; k < IA[n] + bpc; k++) {
+                JA[k] = (int32_t)n;
+                sm[k] = (int32_t)k;
+            }
+            be[n] = IA[n] + 1;
+        }
+        HIPCHK2(hipMemcpy(F.IA, IA.data(), IA.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK2(hipMemcpy(F.JA, JA.data(), JA.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK2(hipMemcpy(F.slotConn, sc.data(), sc.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK2(hipMemcpy(F.slotMirror, sm.data(), sm.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK2(hipMemcpy(F.bulkEnd, be.data(), be.size() * 4, hipMemcpyHostToDevice));
+    }
+    if (!ctxEnsureMask(c)) return OCPG_EHIP;
+    // the saved iteration inputs, filled per realisation by ocpg_pool_place
+    return ocpg_save_flash_inputs(c);
+}
+
+// Places a realisation: its cell state (h->nb cells, the layout of
+// ocpg_upload_bulk) and its connections (local cell indices, the order and
+// rules of ocpg_set_conn) at cells [base, base + h->nb).  The range must not
+// overlap a placed realisation.  The state's iteration inputs are also saved
+// (the starting point of every iteration over it); its mask stays off until
+// ocpg_pool_activate.
+int ocpg_pool_place(ocpg_ctx* c, int64_t base, const ocpg_bulk* h, const ocpg_conn* conn)
+{
+    if (!c || !h || !conn || (conn->nconn > 0 && (!conn->bId || !conn->eId || !conn->area))) return OCPG_EINVAL;
+    PoolHost& P = ctxFimHost(c).pool;
+    if (P.capacity <= 0) return OCPG_ESTATE;
+    const int64_t nb = h->nb, nconn = conn->nconn, D = P.maxNb, bpc = 1 + D;
+    if (nb <= 0 || base < 0 || base + nb > P.capacity || nconn > nb * D) return OCPG_EINVAL;
+    {
+        auto it = P.slots.lower_bound(base);
+        if (it != P.slots.end() && it->first < base + nb) return OCPG_EINVAL;
+        if (it != P.slots.begin()) {
+            --it;
+            if (it->first + it->second.nb > base) return OCPG_EINVAL;
+        }
+    }
+    std::vector<int64_t> cnt(nb, 1);
+    for (int64_t k = 0; k < nconn; k++) {
+        const int64_t b = conn->bId[k], e = conn->eId[k];
+        if (b < 0 || e >= nb || b >= e) return OCPG_EINVAL;
+        if (k > 0 && (conn->bId[k - 1] > b || (conn->bId[k - 1] == b && conn->eId[k - 1] >= e)))
+            return OCPG_EINVAL;
+        cnt[b]++;
+        cnt[e]++;
+    }
+    for (int64_t n = 0; n < nb; n++)
+        if (cnt[n] > bpc) return OCPG_EINVAL;
+    // no wait for the stream: a free range has its mask off, so no iteration
+    // in flight reads or writes it
+    HIPCHK2(hipSetDevice(ctxDevice(c)));
+    // the realisation's rows as buildPattern lays them out, then moved to its
+    // reserved ranges
+    PoolSlot S;
+    S.nb = nb;
+    S.nconn = nconn;
+    S.IA.assign(nb + 1, 0);
+    for (int64_t n = 0; n < nb; n++) S.IA[n + 1] = S.IA[n] + (int32_t)cnt[n];
+    S.nnzb = S.IA[nb];
+    const int64_t cb = base * D, kb = base * bpc;
+    std::vector<int32_t> IA(nb), JA(S.nnzb), sc(S.nnzb), sm(S.nnzb), be(nb), bId(nconn), eId(nconn);
+    std::vector<int64_t> fill(nb);
+    for (int64_t n = 0; n < nb; n++) {
+        IA[n] = (int32_t)(kb + S.IA[n]);
+        JA[S.IA[n]] = (int32_t)(base + n);
+        sc[S.IA[n]] = -1;
+        sm[S.IA[n]] = IA[n];
+        be[n] = (int32_t)(kb + S.IA[n + 1]);
+        fill[n] = S.IA[n] + 1;
+    }
+    for (int64_t k = 0; k < nconn; k++) {
+        const int32_t b = conn->bId[k], e = conn->eId[k];
+        const int64_t se = fill[e]++, sb = fill[b]++;
+        JA[se] = (int32_t)(base + b);
+        JA[sb] = (int32_t)(base + e);
+        sc[se] = sc[sb] = (int32_t)(cb + k);
+        sm[se] = (int32_t)(kb + sb);
+        sm[sb] = (int32_t)(kb + se);
+        bId[k] = (int32_t)(base + b);
+        eId[k] = (int32_t)(base + e);
+    }
+    FimDevice& F = ctxFim(c);
+    HIPCHK2(hipMemcpy(F.IA + base, IA.data(), nb * 4, hipMemcpyHostToDevice));
+    HIPCHK2(hipMemcpy(F.bulkEnd + base, be.data(), nb * 4, hipMemcpyHostToDevice));
+    HIPCHK2(hipMemcpy(F.JA + kb, JA.data(), S.nnzb * 4, hipMemcpyHostToDevice));
+    HIPCHK2(hipMemcpy(F.slotConn + kb, sc.data(), S.nnzb * 4, hipMemcpyHostToDevice));
+    HIPCHK2(hipMemcpy(F.slotMirror + kb, sm.data(), S.nnzb * 4, hipMemcpyHostToDevice));
+    if (nconn > 0) {
+        HIPCHK2(hipMemcpy(F.bId + cb, bId.data(), nconn * 4, hipMemcpyHostToDevice));
+        HIPCHK2(hipMemcpy(F.eId + cb, eId.data(), nconn * 4, hipMemcpyHostToDevice));
+        HIPCHK2(hipMemcpy(F.area + cb, conn->area, nconn * 8, hipMemcpyHostToDevice));
+    }
+    int rc = ctxRangeCopy(c, base, const_cast<ocpg_bulk*>(h), true, false);
+    if (rc == OCPG_OK) rc = ctxSaveInputsRange(c, base, nb);  // in stream order
+    if (rc != OCPG_OK) return rc;
+    P.slots[base] = std::move(S);
+    return OCPG_OK;
+}
+
+// Forgets the realisation at base (its mask off); its ranges may be reused.
+// Waits for the stream.
+int ocpg_pool_remove(ocpg_ctx* c, int64_t base)
+{
+    if (!c) return OCPG_EINVAL;
+    PoolHost& P = ctxFimHost(c).pool;
+    auto it = P.slots.find(base);
+    if (P.capacity <= 0 || it == P.slots.end()) return OCPG_EINVAL;
+    HIPCHK2(hipSetDevice(ctxDevice(c)));
+    uint8_t* m = ctxEnsureMask(c);
+    if (!m) return OCPG_EHIP;
+    HIPCHK2(launchMaskFill(m, base, it->second.nb, 0, ctxStream(c)));
+    // the range is free once every iteration that may include it is done
+    HIPCHK2(hipStreamSynchronize(ctxStream(c)));
+    P.slots.erase(it);
+    return OCPG_OK;
+}
+
+// Sets whether the realisation at base takes part in the iterations launched
+// after this call on the context's stream (a kernel, in stream order).
+int ocpg_pool_activate(ocpg_ctx* c, int64_t base, int on)
+{
+    if (!c) return OCPG_EINVAL;
+    PoolHost& P = ctxFimHost(c).pool;
+    auto it = P.slots.find(base);
+    if (P.capacity <= 0 || it == P.slots.end()) return OCPG_EINVAL;
+    HIPCHK2(hipSetDevice(ctxDevice(c)));
+    uint8_t* m = ctxEnsureMask(c);
+    if (!m) return OCPG_EHIP;
+    HIPCHK2(launchMaskFill(m, base, it->second.nb, on ? 1 : 0, ctxStream(c)));
+    return OCPG_OK;
+}
+
+// The realisation's rows of the last assembly in its own indexing (the
+// system ocpg_get_bsr returns for the realisation alone): IA (nb + 1), JA
+// and val (nnzb blocks), rhs (nb x ncol).  Waits for the stream.  NULL
+// outputs are skipped; *nnzb (if given) receives the block count.
+int ocpg_pool_system(ocpg_ctx* c, int64_t base, int64_t* nnzb, int32_t* IA, int32_t* JA, double* val, double* rhs)
+{
+    if (!c) return OCPG_EINVAL;
+    PoolHost& P = ctxFimHost(c).pool;
+    auto it = P.slots.find(base);
+    if (P.capacity <= 0 || it == P.slots.end()) return OCPG_EINVAL;
+    const PoolSlot& S = it->second;
+    if (nnzb) *nnzb = S.nnzb;
+    HIPCHK2(hipSetDevice(ctxDevice(c)));
+    HIPCHK2(hipStreamSynchronize(ctxStream(c)));
+    const FimDevice& F = ctxFim(c);
+    const int64_t kb = base * (1 + (int64_t)P.maxNb);
+    const size_t bsize = (size_t)F.ncol * F.ncol;
+    if (IA) std::memcpy(IA, S.IA.data(), S.IA.size() * 4);
+    if (JA) {
+        HIPCHK2(hipMemcpy(JA, F.JA + kb, S.nnzb * 4, hipMemcpyDeviceToHost));
+        for (int64_t k = 0; k < S.nnzb; k++) JA[k] -= (int32_t)base;
+    }
+    if (val) HIPCHK2(hipMemcpy(val, F.val + kb * bsize, S.nnzb * bsize * 8, hipMemcpyDeviceToHost));
+    if (rhs) HIPCHK2(hipMemcpy(rhs, F.res + base * F.ncol, S.nb * F.ncol * 8, hipMemcpyDeviceToHost));
+    return OCPG_OK;
+}
+
+// The realisation's cell state into h (h->nb must be its cell count):
+// saved != 0 gives the state its next iteration starts from (the saved
+// iteration inputs, with the other fields as they are); otherwise the
+// current state.  Waits for the stream.
+int ocpg_pool_download(ocpg_ctx* c, int64_t base, ocpg_bulk* h, int saved)
+{
+    if (!c || !h) return OCPG_EINVAL;
+    PoolHost& P = ctxFimHost(c).pool;
+    auto it = P.slots.find(base);
+    if (P.capacity <= 0 || it == P.slots.end() || it->second.nb != h->nb) return OCPG_EINVAL;
+    HIPCHK2(hipSetDevice(ctxDevice(c)));
+    HIPCHK2(hipStreamSynchronize(ctxStream(c)));
+    return ctxRangeCopy(c, base, h, false, saved != 0);
+}
+
 int64_t ocpg_bsr_nnzb(ocpg_ctx* c) { return c ? ctxFim(c).nnzb : -1; }
 int64_t ocpg_bsr_rows(ocpg_ctx* c) { return c ? ctxFim(c).nrow : -1; }
 
@@ -398,6 +653,7 @@ int64_t ocpg_bsr_rows(ocpg_ctx* c) { return c ? ctxFim(c).nrow : -1; }
 int ocpg_set_wells(ocpg_ctx* c, const ocpg_wells* w)
 {
     if (!c || !w || w->nw < 0) return OCPG_EINVAL;
+    if (ctxFimHost(c).pool.capacity > 0) return OCPG_ESTATE;  // a pool's realisations carry no wells
     FimDevice& F = ctxFim(c);
     if (F.nb <= 0 || !F.bId) return OCPG_ESTATE;
     if (!flashSupportsNC(ctxNC(c))) return OCPG_EUNSUP;

@@ -4,6 +4,7 @@
 
 #include <stdint.h>
 
+#include <map>
 #include <vector>
 
 namespace ocpg {
@@ -22,9 +23,24 @@ struct WellsHost {
     double Psurf = 0, Tsurf = 0;
 };
 
+// A serving pool (ocpg_pool_init): realisations placed at reserved ranges of
+// one context.  The realisation at cell base b owns cells [b, b + nb),
+// connection slots [b * maxNb, ...) and BSR blocks [b * (1 + maxNb), ...),
+// so placing one never moves another.
+struct PoolSlot {
+    int64_t nb = 0, nconn = 0, nnzb = 0;
+    std::vector<int32_t> IA;  // the realisation's own row starts (nb + 1, from 0)
+};
+struct PoolHost {
+    int64_t capacity = 0;   // 0: not a pool
+    int maxNb = 0;          // most neighbours of a cell
+    std::map<int64_t, PoolSlot> slots;  // by cell base
+};
+
 struct FimHost {
     std::vector<int32_t> bId, eId;
     WellsHost wells;
+    PoolHost pool;
 };
 
 }  // namespace ocpg

@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(256) krPcRockKernel(DevBulk B, const DevSat* _
                                                       int krpc, int rockOn)
 {
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= B.nb) return;
+    if (n >= B.nb || !cellOn(B, n)) return;
     krPcRockCell(B, sat, rock, krpc, rockOn, n);
 }
 
@@ -274,7 +274,7 @@ template <int NC>
 __global__ void __launch_bounds__(kResBlock) calResKernel(DevBulk B, FimDevice F, WellDevice W, double dt)
 {
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n < B.nb) calResCell<NC>(B, F, W, dt, n);
+    if (n < B.nb && cellOn(B, n)) calResCell<NC>(B, F, W, dt, n);
 }
 
 // CalRes of the rows in rows[0, *count) (the overlapped form's recomputation).
@@ -297,7 +297,7 @@ __global__ void __launch_bounds__(kResBlock) resNormKernel(DevBulk B, FimDevice 
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double vmax = -1.0, nmax = -1.0;
     int64_t vidx = 0, nidx = 0;
-    if (n < B.nb) {
+    if (n < B.nb && cellOn(B, n)) {
         const double rvp = B.rockVp[n], Nt = B.Nt[n];
 #pragma unroll
         for (int e = 0; e < len; ++e) {
@@ -505,6 +505,16 @@ __global__ void __launch_bounds__(kPackBlock) packRecordsKernel(DevBulk B, FimDe
     __shared__ int rows[kPackTile];
     const int64_t n0 = (int64_t)blockIdx.x * kPackTile;
     const int tile = (int)min<int64_t>(kPackTile, B.nb - n0);
+    __shared__ int on[kPackTile];
+    __shared__ int anyOn;
+    if (threadIdx.x == 0) anyOn = 0;
+    __syncthreads();
+    if (threadIdx.x < kPackTile) {
+        on[threadIdx.x] = threadIdx.x < tile && cellOn(B, n0 + threadIdx.x);
+        if (on[threadIdx.x]) atomicOr(&anyOn, 1);
+    }
+    __syncthreads();
+    if (!anyOn) return;  // the whole block leaves: a pool's idle realisations
     if (threadIdx.x < kPackTile)
         rows[threadIdx.x] = threadIdx.x < tile ? min(max(B.bRowSizedSdP[n0 + threadIdx.x], 0),
                                                      AsmDims<NC>::maxRows) : 0;
@@ -554,7 +564,7 @@ __global__ void __launch_bounds__(kPackBlock) packRecordsKernel(DevBulk B, FimDe
     for (int k = 0; k < (RS * kPackTile + kPackBlock - 1) / kPackBlock; ++k) {
         const int i = threadIdx.x + k * kPackBlock;
         const int c = i / RS, q = i % RS;
-        if (i < RS * kPackTile && c < tile && q < R::LEN) F.rec[(n0 + c) * RS + q] = t[c * L + q];
+        if (i < RS * kPackTile && c < tile && q < R::LEN && on[c]) F.rec[(n0 + c) * RS + q] = t[c * L + q];
     }
     __syncthreads();
     // own-row record: the double fields (unused dSec rows are read and not
@@ -582,7 +592,7 @@ __global__ void __launch_bounds__(kPackBlock) packRecordsKernel(DevBulk B, FimDe
     for (int k = 0; k < (OS * kPackTile + kPackBlock - 1) / kPackBlock; ++k) {
         const int i = threadIdx.x + k * kPackBlock;
         const int c = i / OS, q = i % OS;
-        if (i < OS * kPackTile && c < tile && q < O::LEN && (q >= O::DPC || q < rows[c] * ncol))
+        if (i < OS * kPackTile && c < tile && q < O::LEN && (q >= O::DPC || q < rows[c] * ncol) && on[c])
             F.own[(n0 + c) * OS + q] = t[c * L + q];
     }
 }
@@ -985,7 +995,7 @@ __global__ void __launch_bounds__(kAsmBlock) __attribute__((amdgpu_waves_per_eu(
     // workgroup b runs on XCD b % kXcds; XCD x takes rows [x*chunk, (x+1)*chunk)
     const int64_t chunk = gridDim.x / kXcds;
     const int64_t n = (int64_t)(blockIdx.x % kXcds) * chunk + blockIdx.x / kXcds;
-    if (n >= B.nb) return;
+    if (n >= B.nb || !cellOn(B, n)) return;
     assembleRow<NC>(B, F, dt, n, sh);
 }
 
@@ -1276,6 +1286,49 @@ hipError_t launchMaxChange(const DevBulk& B, const DevBulk& L, int nc, double* p
 {
     if (B.nb == 0) return hipSuccess;
     hipLaunchKernelGGL(maxChangeKernel, dim3(blocks), dim3(kNewtonBlock), 0, s, B, L, nc, part);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Serving pool: the activity mask and the restore of the active cells' saved
+// iteration inputs (a kernel instead of whole-array copies, so an iteration
+// over a few realisations of a large pool copies only theirs).
+__global__ void __launch_bounds__(256) maskFillKernel(uint8_t* mask, int64_t base, int64_t n, uint8_t v)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) mask[base + i] = v;
+}
+
+hipError_t launchMaskFill(uint8_t* mask, int64_t base, int64_t n, uint8_t v, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(maskFillKernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, mask, base, n, v);
+    return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256) maskedCopyKernel(const uint8_t* __restrict__ mask, int64_t nb, int64_t ld,
+                                                         MaskedCopy d)
+{
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nb || !mask[n]) return;
+    for (int f = 0; f < d.nfields; ++f) {
+        const int e = d.elem[f];
+        for (int r = 0; r < d.rows[f]; ++r) {
+            const int64_t o = ((int64_t)r * ld + n) * e;
+            if (e == 8)
+                *reinterpret_cast<double*>(d.dst[f] + o) = *reinterpret_cast<const double*>(d.src[f] + o);
+            else if (e == 4)
+                *reinterpret_cast<int32_t*>(d.dst[f] + o) = *reinterpret_cast<const int32_t*>(d.src[f] + o);
+            else
+                d.dst[f][o] = d.src[f][o];
+        }
+    }
+}
+
+hipError_t launchMaskedCopy(const uint8_t* mask, int64_t nb, int64_t ld, const MaskedCopy& d, hipStream_t s)
+{
+    if (nb <= 0) return hipSuccess;
+    hipLaunchKernelGGL(maskedCopyKernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, mask, nb, ld, d);
     return hipGetLastError();
 }
 

@@ -107,4 +107,18 @@ hipError_t launchCalResRows(int NC, const DevBulk& B, FimDevice& F, const WellDe
 hipError_t launchAssembleRows(int NC, const DevBulk& B, FimDevice& F, double dt, const int32_t* rows,
                               const int32_t* count, hipStream_t s);
 
+// Serving pool: mask[base, base + n) = v; and a copy of fields (each
+// rows x ld entries of elem bytes, the SoA layout) for the cells whose mask
+// is set.
+constexpr int kMaxMaskedCopy = 12;
+struct MaskedCopy {
+    const char* src[kMaxMaskedCopy];
+    char* dst[kMaxMaskedCopy];
+    int elem[kMaxMaskedCopy];
+    int rows[kMaxMaskedCopy];
+    int nfields;
+};
+hipError_t launchMaskFill(uint8_t* mask, int64_t base, int64_t n, uint8_t v, hipStream_t s);
+hipError_t launchMaskedCopy(const uint8_t* mask, int64_t nb, int64_t ld, const MaskedCopy& d, hipStream_t s);
+
 }  // namespace ocpg

@@ -655,7 +655,7 @@ __global__ void __launch_bounds__(kFlashBlock) flashClassifyKernel(const DevEos*
                                                                    const double* __restrict__ zin)
 {
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = n < B.nb;
+    const bool active = n < B.nb && cellOn(B, n);
     const int64_t ld = B.ld;
     constexpr int nc = NC + 1;
     int ftype = 0;

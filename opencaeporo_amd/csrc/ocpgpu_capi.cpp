@@ -113,6 +113,8 @@ struct ocpg_ctx {
     // copy of the flash inputs (ocpg_save_flash_inputs)
     char* snap = nullptr;
     size_t snapBytes = 0;
+    // serving pool: per-cell activity mask (B.mask), NULL outside a pool
+    uint8_t* mask = nullptr;
     // overlapped Newton iteration (ocpg_update_assemble): side streams of the
     // flash and the rows recomputed after they join
     FlashSide side{};
@@ -441,6 +443,7 @@ int ocpg_destroy(ocpg_ctx* c)
     fimFree(c->fim);
     if (c->wells.mem) (void)hipFree(c->wells.mem);
     if (c->snap) (void)hipFree(c->snap);
+    if (c->mask) (void)hipFree(c->mask);
     if (c->workMem) (void)hipFree(c->workMem);
     if (c->cur) (void)hipFree(c->cur);
     if (c->last) (void)hipFree(c->last);
@@ -466,7 +469,9 @@ int ocpg_set_cells(ocpg_ctx* c, int64_t nb)
     dropIterGraph(c);
     if (c->cur) HIPCHK(hipFree(c->cur));
     if (c->last) HIPCHK(hipFree(c->last));
+    if (c->mask) HIPCHK(hipFree(c->mask));
     c->cur = c->last = nullptr;
+    c->mask = nullptr;
     c->lastValid = false;
     const int64_t ld = (nb + 63) / 64 * 64;
     size_t off = 0;
@@ -819,6 +824,24 @@ int ocpg_restore_flash_inputs(ocpg_ctx* c)
     if (!c || !c->cur || !c->snap) return OCPG_ESTATE;
     HIPCHK(hipSetDevice(c->device));
     timerBegin(c, T_RESTORE);
+    if (c->mask) {
+        // a serving pool: only the cells of the realisations taking part
+        MaskedCopy d{};
+        size_t off = 0;
+        for (int f = 0; f < kNumFields; f++) {
+            if (!isFlashInput(kFields[f].hostOff)) continue;
+            const size_t b = (size_t)kFields[f].elem * cellEntries(c, kFields[f].cnt) * c->B.ld;
+            d.src[d.nfields] = c->snap + off;
+            d.dst[d.nfields] = c->cur + c->fieldOff[f];
+            d.elem[d.nfields] = kFields[f].elem;
+            d.rows[d.nfields] = (int)cellEntries(c, kFields[f].cnt);
+            d.nfields++;
+            off += align256(b);
+        }
+        HIPCHK(launchMaskedCopy(c->mask, c->B.nb, c->B.ld, d, c->stream));
+        timerEnd(c, T_RESTORE);
+        return OCPG_OK;
+    }
     size_t off = 0;
     for (int f = 0; f < kNumFields; f++) {
         if (!isFlashInput(kFields[f].hostOff)) continue;
@@ -953,4 +976,113 @@ const DevRock& ctxRock(ocpg_ctx* c) { return c->rock; }
 void ctxTimerBegin(ocpg_ctx* c, int slot) { timerBegin(c, slot); }
 void ctxTimerEnd(ocpg_ctx* c, int slot) { timerEnd(c, slot); }
 void ctxDropGraph(ocpg_ctx* c) { dropIterGraph(c); }
+
+// The serving pool's activity mask (allocated zero on first use; B.mask then
+// points at it, which changes what a captured iteration reads: the graph is
+// dropped).
+uint8_t* ctxEnsureMask(ocpg_ctx* c)
+{
+    if (c->mask) return c->mask;
+    const size_t n = (size_t)std::max<int64_t>(c->B.ld, 64);
+    if (hipMalloc(&c->mask, n) != hipSuccess) return nullptr;
+    if (hipMemset(c->mask, 0, n) != hipSuccess) return nullptr;
+    c->B.mask = c->mask;
+    dropIterGraph(c);
+    return c->mask;
+}
+
+// Back to an ordinary context: every cell takes part.
+int ctxDropMask(ocpg_ctx* c)
+{
+    if (!c->mask) return OCPG_OK;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipFree(c->mask));
+    c->mask = nullptr;
+    c->B.mask = nullptr;
+    dropIterGraph(c);
+    return OCPG_OK;
+}
+
+// Copies of the fields of cells [base, base + h->nb) between the host (the
+// reference's cell-major layout, h->nb cells) and the device: toDevice
+// uploads every non-NULL field; otherwise, with saved, the iteration inputs
+// come from the saved copy (ocpg_save_flash_inputs) and the rest from the
+// current state, and without it everything from the current state.
+// Transposes cnt x nb entries of E bytes between the host's cell-major
+// layout (src/dst[n * cnt + k]) and a field-major staging buffer.
+template <int E>
+static void transposeRange(char* dev, char* host, int64_t nb, int64_t cnt, bool toStage)
+{
+    struct alignas(E) V { char b[E]; };
+    V* d = reinterpret_cast<V*>(dev);
+    V* h = reinterpret_cast<V*>(host);
+    if (toStage) {
+        for (int64_t n = 0; n < nb; n++)
+            for (int64_t k = 0; k < cnt; k++) d[k * nb + n] = h[n * cnt + k];
+    } else {
+        for (int64_t n = 0; n < nb; n++)
+            for (int64_t k = 0; k < cnt; k++) h[n * cnt + k] = d[k * nb + n];
+    }
+}
+
+static void transposeAny(int elem, char* stage, char* host, int64_t nb, int64_t cnt, bool toStage)
+{
+    if (elem == 8) transposeRange<8>(stage, host, nb, cnt, toStage);
+    else if (elem == 4) transposeRange<4>(stage, host, nb, cnt, toStage);
+    else transposeRange<1>(stage, host, nb, cnt, toStage);
+}
+
+int ctxRangeCopy(ocpg_ctx* c, int64_t base, ocpg_bulk* h, bool toDevice, bool saved)
+{
+    const int64_t nb = h->nb, ld = c->B.ld;
+    if (base < 0 || nb < 0 || base + nb > c->B.nb) return OCPG_EINVAL;
+    if (nb == 0) return OCPG_OK;
+    std::vector<double> stage;
+    size_t snapOff = 0;
+    auto move = [&](char* devBase, char* hp, int elem, int64_t cnt) -> hipError_t {
+        stage.resize(((size_t)elem * cnt * nb + 7) / 8);
+        char* st = reinterpret_cast<char*>(stage.data());
+        const size_t w = (size_t)elem * nb, pitch = (size_t)elem * ld;
+        char* dp = devBase + (size_t)base * elem;
+        if (toDevice) {
+            transposeAny(elem, st, hp, nb, cnt, true);
+            return hipMemcpy2D(dp, pitch, st, w, w, cnt, hipMemcpyHostToDevice);
+        }
+        const hipError_t e = hipMemcpy2D(st, w, dp, pitch, w, cnt, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) transposeAny(elem, st, hp, nb, cnt, false);
+        return e;
+    };
+    for (int f = 0; f < kNumFields; f++) {
+        const FieldDesc& d = kFields[f];
+        const int64_t cnt = cellEntries(c, d.cnt);
+        const size_t fieldBytes = (size_t)d.elem * cnt * ld;
+        const bool input = isFlashInput(d.hostOff);
+        char* devBase = c->cur + c->fieldOff[f];
+        if (input && !toDevice && saved) devBase = c->snap + snapOff;
+        if (input) snapOff += align256(fieldBytes);
+        char* hp;
+        std::memcpy(&hp, reinterpret_cast<char*>(h) + d.hostOff, sizeof(void*));
+        if (hp) HIPCHK(move(devBase, hp, d.elem, cnt));
+    }
+    if (h->lNi) HIPCHK(move(reinterpret_cast<char*>(c->B.lNi), reinterpret_cast<char*>(h->lNi), 8, c->NC + 1));
+    return OCPG_OK;
+}
+
+// Saved iteration inputs of cells [base, base + nb): current -> saved copy.
+int ctxSaveInputsRange(ocpg_ctx* c, int64_t base, int64_t nb)
+{
+    if (!c->snap) return OCPG_ESTATE;
+    const int64_t ld = c->B.ld;
+    size_t off = 0;
+    for (int f = 0; f < kNumFields; f++) {
+        const FieldDesc& d = kFields[f];
+        if (!isFlashInput(d.hostOff)) continue;
+        const int64_t cnt = cellEntries(c, d.cnt);
+        const size_t o = (size_t)base * d.elem, pitch = (size_t)ld * d.elem;
+        HIPCHK(hipMemcpy2DAsync(c->snap + off + o, pitch, c->cur + c->fieldOff[f] + o, pitch, (size_t)nb * d.elem,
+                                cnt, hipMemcpyDeviceToDevice, c->stream));
+        off += align256((size_t)d.elem * cnt * ld);
+    }
+    return OCPG_OK;
+}
 }  // namespace ocpg

@@ -83,6 +83,12 @@ def load():
         "ocpg_host_alloc": ([C.c_int64], C.c_void_p),
         "ocpg_host_free": ([P], C.c_int),
         "ocpg_get_upwind": ([P, P, P, P], C.c_int),
+        "ocpg_pool_init": ([P, C.c_int64, C.c_int32], C.c_int),
+        "ocpg_pool_place": ([P, C.c_int64, P, P], C.c_int),
+        "ocpg_pool_activate": ([P, C.c_int64, C.c_int], C.c_int),
+        "ocpg_pool_system": ([P, C.c_int64, P, P, P, P, P], C.c_int),
+        "ocpg_pool_download": ([P, C.c_int64, P, C.c_int], C.c_int),
+        "ocpg_pool_remove": ([P, C.c_int64], C.c_int),
         "ocpg_save_flash_inputs": ([P], C.c_int),
         "ocpg_restore_flash_inputs": ([P], C.c_int),
         "ocpg_timing_enable": ([P, C.c_int], C.c_int),
@@ -355,6 +361,41 @@ class Context:
         check(self.lib.ocpg_get_upwind(self.h, up.ctypes.data, urho.ctypes.data, uvel.ctypes.data),
               "ocpg_get_upwind")
         return up, urho, uvel
+
+    # ---- serving pool ----
+    def pool_init(self, capacity, max_neighbours=6):
+        check(self.lib.ocpg_pool_init(self.h, int(capacity), int(max_neighbours)), "ocpg_pool_init")
+        self.nb = int(capacity)
+        self.grid = None
+
+    def pool_place(self, base, bulk, grid):
+        s = bulk.cstruct()
+        conn = abi.conn_struct(grid.bId, grid.eId, grid.area)
+        check(self.lib.ocpg_pool_place(self.h, int(base), C.byref(s), C.byref(conn)), "ocpg_pool_place")
+
+    def pool_activate(self, base, on=True):
+        check(self.lib.ocpg_pool_activate(self.h, int(base), 1 if on else 0), "ocpg_pool_activate")
+
+    def pool_system(self, base, nb):
+        """(IA, JA, val, rhs) of the realisation at base, in its own indexing."""
+        nnzb = C.c_int64()
+        check(self.lib.ocpg_pool_system(self.h, int(base), C.byref(nnzb), None, None, None, None),
+              "ocpg_pool_system")
+        k = nnzb.value
+        IA = np.zeros(nb + 1, dtype=np.int32)
+        JA = np.zeros(k, dtype=np.int32)
+        val = np.zeros(k * self.ncol * self.ncol)
+        rhs = np.zeros(nb * self.ncol)
+        check(self.lib.ocpg_pool_system(self.h, int(base), None, IA.ctypes.data, JA.ctypes.data,
+                                        val.ctypes.data, rhs.ctypes.data), "ocpg_pool_system")
+        return IA, JA, val, rhs
+
+    def pool_download(self, base, bulk, saved=True):
+        s = bulk.cstruct()
+        check(self.lib.ocpg_pool_download(self.h, int(base), C.byref(s), 1 if saved else 0), "ocpg_pool_download")
+
+    def pool_remove(self, base):
+        check(self.lib.ocpg_pool_remove(self.h, int(base)), "ocpg_pool_remove")
 
     def save_flash_inputs(self):
         check(self.lib.ocpg_save_flash_inputs(self.h), "ocpg_save_flash_inputs")

@@ -206,6 +206,168 @@ class Server:
         return requests
 
 
+class Pool:
+    """One device context holding many realisations at reserved cell ranges
+    (ocpg_pool_*), with one captured Newton iteration over every active
+    realisation.  Ranges are handed out first fit."""
+
+    def __init__(self, eos, sat, rock, capacity, max_neighbours=6):
+        self.ctx = lib.Context(eos, sat, rock)
+        self.ctx.pool_init(capacity, max_neighbours)
+        self.ctx.capture_iteration(DT)
+        self.capacity = int(capacity)
+        self.free = [[0, self.capacity]]      # sorted [base, length]
+        self.bytes = self.ctx.device_bytes()
+
+    def alloc(self, nb):
+        for k, (b, n) in enumerate(self.free):
+            if n >= nb:
+                if n == nb:
+                    self.free.pop(k)
+                else:
+                    self.free[k] = [b + nb, n - nb]
+                return b
+        return None
+
+    def release(self, base, nb):
+        self.free.append([base, nb])
+        self.free.sort()
+        merged = []
+        for b, n in self.free:
+            if merged and merged[-1][0] + merged[-1][1] == b:
+                merged[-1][1] += n
+            else:
+                merged.append([b, n])
+        self.free = merged
+
+    def close(self):
+        self.ctx.close()
+
+
+class BatchServer:
+    """Continuous batching on a Pool: every request that has arrived when
+    the GPU finishes a batch joins the next one (one request per realisation
+    per batch; a second request for the same realisation waits for the
+    next), and the whole batch is one replay of the captured iteration over
+    the active realisations.  While a batch runs, the realisations of the
+    waiting requests are admitted (placed into free ranges, evicting the
+    least recently used idle ones when the pool is full), so their uploads
+    overlap the GPU's work.  max_batch = 1 serves one request per launch on
+    the same pool."""
+
+    def __init__(self, realisations, pool, max_batch=None):
+        self.real = {r.rid: r for r in realisations}
+        self.pool = pool
+        self.max_batch = max_batch or len(realisations)
+        self.base = {}               # rid -> cell base while resident
+        self.active = set()          # rids whose mask is on
+        self.inflight = set()        # rids in the running batch
+        self.last_used = {}
+        self.admissions = 0
+        self.evictions = 0
+        self.batches = 0
+        self.max_batch_seen = 0
+
+    def _evict(self, rid):
+        r = self.real[rid]
+        ctx = self.pool.ctx
+        if rid in self.active:
+            ctx.pool_activate(self.base[rid], False)
+            self.active.discard(rid)
+        ctx.pool_download(self.base[rid], r.host, saved=True)
+        ctx.pool_remove(self.base[rid])
+        self.pool.release(self.base.pop(rid), r.nb)
+        self.evictions += 1
+
+    def _admit(self, rid, keep, evict=True):
+        """Place realisation rid; evicts idle residents outside `keep` (least
+        recently used first) until a range fits.  False if none does.  An
+        eviction waits for the batch in flight, so the overlapped admissions
+        take free ranges only (evict=False)."""
+        r = self.real[rid]
+        if r.nb > self.pool.capacity:
+            raise ValueError(f"realisation {rid} ({r.nb} cells) exceeds the pool")
+        base = self.pool.alloc(r.nb)
+        while base is None:
+            if not evict:
+                return False
+            idle = [q for q in self.base if q not in keep and q not in self.inflight]
+            if not idle:
+                return False
+            self._evict(min(idle, key=lambda q: self.last_used.get(q, 0.0)))
+            base = self.pool.alloc(r.nb)
+        self.pool.ctx.pool_place(base, r.host, r.grid)
+        self.base[rid] = base
+        self.admissions += 1
+        return True
+
+    def run(self, requests, clock=time.perf_counter):
+        """Serve `requests` (sorted by arrival, seconds from the start) in real
+        time; fills in their start/end times."""
+        ctx = self.pool.ctx
+        pending = collections.deque()
+        batch = []
+        t0 = clock()
+        i, n, done = 0, len(requests), 0
+        while done < n:
+            now = clock() - t0
+            while i < n and requests[i].arrival <= now:
+                pending.append(requests[i])
+                i += 1
+            if batch and ctx.iteration_done():
+                end = clock() - t0
+                for q in batch:
+                    q.end = end
+                    self.last_used[q.rid] = end
+                done += len(batch)
+                batch = []
+                self.inflight = set()
+            if not batch and pending:
+                chosen, seen = [], set()
+                for q in pending:
+                    if len(chosen) >= self.max_batch:
+                        break
+                    if q.rid in seen:
+                        continue
+                    if q.rid not in self.base and not self._admit(q.rid, keep=seen):
+                        continue
+                    chosen.append(q)
+                    seen.add(q.rid)
+                if chosen:
+                    for q in chosen:
+                        pending.remove(q)
+                    for rid in self.active - seen:
+                        ctx.pool_activate(self.base[rid], False)
+                    for rid in seen - self.active:
+                        ctx.pool_activate(self.base[rid], True)
+                    self.active = set(seen)
+                    ctx.launch_iteration()
+                    start = clock() - t0
+                    for q in chosen:
+                        q.start = start
+                    batch = chosen
+                    self.inflight = set(seen)
+                    self.batches += 1
+                    self.max_batch_seen = max(self.max_batch_seen, len(chosen))
+                    continue
+            if batch:
+                # overlap: admit one waiting realisation while the GPU works
+                for q in pending:
+                    if q.rid not in self.base:
+                        if not self._admit(q.rid, keep=self.inflight, evict=False):
+                            time.sleep(0.0001)
+                        break
+                else:
+                    time.sleep(0.0001)
+            elif i < n:
+                time.sleep(min(0.0002, max(0.0, requests[i].arrival - (clock() - t0))))
+        return requests
+
+    def system(self, rid):
+        """The realisation's system from its last iteration (own indexing)."""
+        return self.pool.ctx.pool_system(self.base[rid], self.real[rid].nb)
+
+
 def make_ensemble(n, cells_lo, cells_hi, seed, eos, sat, rock):
     rng = np.random.default_rng(seed)
     sizes = np.exp(rng.uniform(np.log(cells_lo), np.log(cells_hi), n)).astype(int)

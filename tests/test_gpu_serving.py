@@ -101,3 +101,57 @@ def test_reallocation_drops_the_captured_iteration():
         with pytest.raises(lib.OcpgError, match="code -4"):
             ctx.launch_iteration()
     ctx.close()
+
+
+def _alone(r):
+    """The realisation's iteration on a context of its own (the plain path)."""
+    ctx = lib.Context(r.eos, r.sat, r.rock)
+    ctx.set_cells(r.nb)
+    ctx.upload(r.host)
+    ctx.set_conn(r.grid)
+    ctx.save_flash_inputs()
+    ctx.restore_flash_inputs()
+    ctx.flash_fim(stats=False)
+    ctx.calkrpc_rock()
+    ctx.calres(serving.DT, norms=False)
+    ctx.assemble(serving.DT)
+    out = ctx.get_bsr()
+    ctx.close()
+    return out
+
+
+def test_pool_batches_give_each_realisation_its_own_result():
+    """Continuous batching on one pooled context: realisations of uneven
+    size placed at reserved ranges, admitted and evicted by the pool's
+    capacity, batched into replays of one captured iteration over the active
+    ones.  Each realisation's system equals, bit for bit, its iteration on a
+    context of its own."""
+    gpu_util.say("[serving] pooled batches vs alone")
+    ens = models(8, 1000, 20000, seed=11)
+    for r in ens:
+        r.initialize()
+    ref = {r.rid: _alone(r) for r in ens}
+    total = sum(r.nb for r in ens)
+    pool = serving.Pool(ens[0].eos, ens[0].sat, ens[0].rock, capacity=int(0.6 * total))
+    srv = serving.BatchServer(ens, pool)
+    reqs = serving.poisson_requests(40, rate=3000.0, n_real=len(ens), seed=12)
+    srv.run(reqs)
+    assert all(q.end is not None and q.end >= q.start >= q.arrival for q in reqs)
+    gpu_util.say(f"  batches {srv.batches} (largest {srv.max_batch_seen}), admissions {srv.admissions}, "
+                 f"evictions {srv.evictions}")
+    assert srv.max_batch_seen > 1 and srv.evictions > 0
+    checked = set()
+    for _ in range(len(ens)):
+        for rid in list(srv.base):
+            if rid in checked or rid not in srv.active:
+                continue
+            got = srv.system(rid)
+            for a, b in zip(got, ref[rid]):
+                np.testing.assert_array_equal(a, b)
+            checked.add(rid)
+        rest = [r.rid for r in ens if r.rid not in checked]
+        if not rest:
+            break
+        srv.run([serving.Request(rid, 0.0) for rid in rest])
+    assert checked == {r.rid for r in ens}
+    pool.close()
