@@ -248,9 +248,9 @@ def main():
     ap.add_argument("--irregular", action="store_true",
                     help="the irregular-stencil configuration: 10-component fluid, faulted grid with "
                          "non-neighbour connections, 40 wells (a parity/measurement case, not the headline)")
-    ap.add_argument("--serving-requests", type=int, default=96,
-                    help="requests of the serving-form leg (p50/p99 latency, continuous batching vs one at a "
-                         "time, 32 realisations of 1e3..1e5 cells); 0 skips it")
+    ap.add_argument("--serving-requests", type=int, default=384,
+                    help="requests of the serving-form leg (p50/p99 latency and capacity, continuous batching "
+                         "vs one at a time, 256 realisations of 1e3..1e6 cells); 0 skips it")
     ap.add_argument("--deck", default=None,
                     help="read the workload from an input deck (opencaeporo_amd/deck.py), e.g. "
                          "tests/golden/decks/spe5refine/spe5-70x70x30-2y.data (a measurement case, not the headline)")
@@ -426,21 +426,24 @@ def main():
     }
     ctx.close()
     if rank == 0 and a.serving_requests > 0 and not side_case and not a.serial:
-        # the serving form beside the headline: one Newton iteration per
-        # request of one of many resident realisations (opencaeporo_amd/serving.py)
+        # the serving form beside the headline: 256 ensemble realisations of
+        # 1e3..1e6 cells, one Newton iteration per request, admission by a
+        # 200 GB device-memory budget (opencaeporo_amd/serving.py)
         try:
             from opencaeporo_amd import serving
-            # offered load 1.2x what one-at-a-time service sustains: the
-            # regime where overlapping realisations decides the latency
-            sv = serving.benchmark(n_real=32, cells_lo=1e3, cells_hi=1e5, n_req=a.serving_requests,
-                                   budget_gb=8.0, log=log, load=1.2)
+            sv = serving.benchmark(n_real=256, cells_lo=1e3, cells_hi=1e6, n_req=a.serving_requests,
+                                   budget_gb=200.0, log=log, load=0.8)
+            keys = ("p50_ms", "p99_ms", "mean_ms", "throughput_req_s", "cell_updates_s", "admissions", "evictions")
             out["config"]["serving"] = {
-                "realisations": 32, "cells": "log-uniform in [1e3, 1e5]", "requests": a.serving_requests,
-                "offered_rate_per_s": sv["config"]["rate_per_s"], "offered_load_vs_one_at_a_time": 1.2,
-                "continuous": {k: sv["continuous"][k] for k in ("p50_ms", "p99_ms", "throughput_req_s",
-                                                                "cell_updates_s", "max_concurrent")},
-                "one_at_a_time": {k: sv["one_at_a_time"][k] for k in ("p50_ms", "p99_ms", "throughput_req_s",
-                                                                      "cell_updates_s")}}
+                "realisations": 256, "cells": "log-uniform in [1e3, 1e6]",
+                "total_cells": sv["config"]["total_cells"], "requests": a.serving_requests,
+                "budget_gb": 200.0, "pool_cells": sv["config"]["pool_cells"],
+                "offered_rate_per_s": sv["config"]["rate_per_s"], "offered_load_vs_one_at_a_time": 0.8,
+                "capacity": sv["capacity"],
+                "one_at_a_time": {k: sv["one_at_a_time"][k] for k in keys},
+                "continuous": {k: sv["continuous"][k] for k in keys + ("mean_batch", "max_batch")},
+                "pool_one_at_a_time": {k: sv["pool_one_at_a_time"][k] for k in keys},
+                "gain": sv["gain"]}
         except Exception as e:  # informative only
             log(f"[bench] serving leg failed: {e}")
     print(json.dumps(out), flush=True)

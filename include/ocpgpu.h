@@ -492,8 +492,18 @@ int ocpg_get_upwind(ocpg_ctx* ctx, int32_t* upblock, double* upblock_Rho,
  *     ocpg_get_bsr returns for it alone); NULL outputs are skipped;
  *   ocpg_pool_download: its cell state, the saved iteration inputs
  *     (saved != 0) or the current values;
- *   ocpg_pool_remove: frees its range.
- * Placing and removing keep a captured iteration valid.                      */
+ *   ocpg_pool_remove: frees its range;
+ *   ocpg_pool_image_bytes / ocpg_pool_export / ocpg_pool_import: the
+ *     realisation as one host block in the device's layout (state, fixed
+ *     fields and pattern); export with all == 0 refreshes only the state
+ *     (saved iteration inputs, last-step moles) of an image it filled
+ *     before; import places it at any free base.  Both run on the pool's
+ *     copy stream, overlapping the iterations (page-locked images copy
+ *     without staging); ocpg_pool_sync_copies waits for them.  Export and
+ *     remove need the realisation switched off.
+ * The pool keeps no last-step copy beyond the moles CalRes reads
+ * (ocpg_update_last_step / ocpg_reset_to_last_step return OCPG_ESTATE).
+ * Placing, importing and removing keep a captured iteration valid.           */
 int ocpg_pool_init(ocpg_ctx* ctx, int64_t capacity, int32_t max_neighbours);
 int ocpg_pool_place(ocpg_ctx* ctx, int64_t base, const ocpg_bulk* state, const ocpg_conn* conn);
 int ocpg_pool_activate(ocpg_ctx* ctx, int64_t base, int on);
@@ -501,6 +511,10 @@ int ocpg_pool_system(ocpg_ctx* ctx, int64_t base, int64_t* nnzb, int32_t* IA, in
                      double* rhs);
 int ocpg_pool_download(ocpg_ctx* ctx, int64_t base, ocpg_bulk* state, int saved);
 int ocpg_pool_remove(ocpg_ctx* ctx, int64_t base);
+int64_t ocpg_pool_image_bytes(ocpg_ctx* ctx, int64_t base);
+int ocpg_pool_export(ocpg_ctx* ctx, int64_t base, void* image, int all);
+int ocpg_pool_import(ocpg_ctx* ctx, int64_t base, const void* image);
+int ocpg_pool_sync_copies(ocpg_ctx* ctx);
 
 /* Copy / restore of the fields the flash starts from (P, Ni, S, phaseNum,
  * xij, skip-stability memory), to re-run one Newton iteration from the same

@@ -36,7 +36,14 @@ bool ctxLastValid(ocpg_ctx* c);
 uint8_t* ctxEnsureMask(ocpg_ctx* c);
 int ctxDropMask(ocpg_ctx* c);
 int ctxRangeCopy(ocpg_ctx* c, int64_t base, ocpg_bulk* h, bool toDevice, bool saved);
-int ctxSaveInputsRange(ocpg_ctx* c, int64_t base, int64_t nb);
+int ctxSaveInputsRange(ocpg_ctx* c, int64_t base, int64_t nb, hipStream_t stream);
+int ctxPoolTrimLast(ocpg_ctx* c);
+hipStream_t ctxCopyStream(ocpg_ctx* c);
+int ctxCopyAfterMain(ocpg_ctx* c);
+int ctxMainAfterCopy(ocpg_ctx* c);
+uint64_t ctxLaunches(ocpg_ctx* c);
+uint64_t ctxLaunchesDone(ocpg_ctx* c);
+void ctxImageSegments(ocpg_ctx* c, int64_t base, int64_t nb, bool state, bool fixed, std::vector<ImageSeg>& out);
 }  // namespace ocpg
 
 
@@ -412,11 +419,13 @@ int ocpg_set_conn(ocpg_ctx* c, const ocpg_conn* conn)
 int ocpg_pool_init(ocpg_ctx* c, int64_t capacity, int32_t maxNb)
 {
     if (!c || capacity <= 0 || maxNb < 1 || maxNb > 64) return OCPG_EINVAL;
-    const int64_t bpc = 1 + (int64_t)maxNb;
+    const int64_t bpc = 1 + (int64_t)maxNb, cpc = (maxNb + 1) / 2;
     if (capacity * bpc > (int64_t)INT32_MAX) return OCPG_ETOOBIG;
     int rc = ocpg_set_cells(c, capacity);
+    if (rc == OCPG_OK) rc = ctxPoolTrimLast(c);
     if (rc != OCPG_OK) return rc;
     HIPCHK2(hipSetDevice(ctxDevice(c)));
+    if (!ctxCopyStream(c)) return OCPG_EHIP;
     FimHost& H = ctxFimHost(c);
     H.bId.clear();
     H.eId.clear();
@@ -424,6 +433,7 @@ int ocpg_pool_init(ocpg_ctx* c, int64_t capacity, int32_t maxNb)
     H.pool = PoolHost{};
     H.pool.capacity = capacity;
     H.pool.maxNb = maxNb;
+    H.pool.connPerCell = (int)cpc;
     FimDevice& F = ctxFim(c);
     WellDevice& W = ctxWells(c);
     ctxDropGraph(c);
@@ -431,7 +441,7 @@ int ocpg_pool_init(ocpg_ctx* c, int64_t capacity, int32_t maxNb)
     if (W.mem) HIPCHK2(hipFree(W.mem));
     W = WellDevice{};
     const int NC = ctxNC(c);
-    const int64_t nconn = capacity * maxNb, nnzb = capacity * bpc;
+    const int64_t nconn = capacity * cpc, nnzb = capacity * bpc;
     F.nb = capacity;
     F.nconn = nconn;
     F.ncol = NC + 2;
@@ -499,8 +509,8 @@ int ocpg_pool_place(ocpg_ctx* c, int64_t base, const ocpg_bulk* h, const ocpg_co
     if (!c || !h || !conn || (conn->nconn > 0 && (!conn->bId || !conn->eId || !conn->area))) return OCPG_EINVAL;
     PoolHost& P = ctxFimHost(c).pool;
     if (P.capacity <= 0) return OCPG_ESTATE;
-    const int64_t nb = h->nb, nconn = conn->nconn, D = P.maxNb, bpc = 1 + D;
-    if (nb <= 0 || base < 0 || base + nb > P.capacity || nconn > nb * D) return OCPG_EINVAL;
+    const int64_t nb = h->nb, nconn = conn->nconn, D = P.maxNb, bpc = 1 + D, cpc = P.connPerCell;
+    if (nb <= 0 || base < 0 || base + nb > P.capacity || nconn > nb * cpc) return OCPG_EINVAL;
     {
         auto it = P.slots.lower_bound(base);
         if (it != P.slots.end() && it->first < base + nb) return OCPG_EINVAL;
@@ -520,9 +530,11 @@ int ocpg_pool_place(ocpg_ctx* c, int64_t base, const ocpg_bulk* h, const ocpg_co
     }
     for (int64_t n = 0; n < nb; n++)
         if (cnt[n] > bpc) return OCPG_EINVAL;
-    // no wait for the stream: a free range has its mask off, so no iteration
-    // in flight reads or writes it
+    // no wait for the context's stream: a free range has its mask off, so no
+    // iteration in flight reads or writes it; copies of a former occupant
+    // (and the waits ocpg_pool_remove queued for it) finish first
     HIPCHK2(hipSetDevice(ctxDevice(c)));
+    HIPCHK2(hipStreamSynchronize(ctxCopyStream(c)));
     // the realisation's rows as buildPattern lays them out, then moved to its
     // reserved ranges
     PoolSlot S;
@@ -531,7 +543,7 @@ int ocpg_pool_place(ocpg_ctx* c, int64_t base, const ocpg_bulk* h, const ocpg_co
     S.IA.assign(nb + 1, 0);
     for (int64_t n = 0; n < nb; n++) S.IA[n + 1] = S.IA[n] + (int32_t)cnt[n];
     S.nnzb = S.IA[nb];
-    const int64_t cb = base * D, kb = base * bpc;
+    const int64_t cb = base * cpc, kb = base * bpc;
     std::vector<int32_t> IA(nb), JA(S.nnzb), sc(S.nnzb), sm(S.nnzb), be(nb), bId(nconn), eId(nconn);
     std::vector<int64_t> fill(nb);
     for (int64_t n = 0; n < nb; n++) {
@@ -564,15 +576,19 @@ int ocpg_pool_place(ocpg_ctx* c, int64_t base, const ocpg_bulk* h, const ocpg_co
         HIPCHK2(hipMemcpy(F.eId + cb, eId.data(), nconn * 4, hipMemcpyHostToDevice));
         HIPCHK2(hipMemcpy(F.area + cb, conn->area, nconn * 8, hipMemcpyHostToDevice));
     }
+    // the saved inputs on the copy stream: before any export of this
+    // realisation, and before the context's later work
     int rc = ctxRangeCopy(c, base, const_cast<ocpg_bulk*>(h), true, false);
-    if (rc == OCPG_OK) rc = ctxSaveInputsRange(c, base, nb);  // in stream order
+    if (rc == OCPG_OK) rc = ctxSaveInputsRange(c, base, nb, ctxCopyStream(c));
+    if (rc == OCPG_OK) rc = ctxMainAfterCopy(c);
     if (rc != OCPG_OK) return rc;
     P.slots[base] = std::move(S);
     return OCPG_OK;
 }
 
 // Forgets the realisation at base (its mask off); its ranges may be reused.
-// Waits for the stream.
+// Does not wait: what comes next in the range is ordered after the
+// iterations that may still include this realisation.
 int ocpg_pool_remove(ocpg_ctx* c, int64_t base)
 {
     if (!c) return OCPG_EINVAL;
@@ -580,11 +596,20 @@ int ocpg_pool_remove(ocpg_ctx* c, int64_t base)
     auto it = P.slots.find(base);
     if (P.capacity <= 0 || it == P.slots.end()) return OCPG_EINVAL;
     HIPCHK2(hipSetDevice(ctxDevice(c)));
-    uint8_t* m = ctxEnsureMask(c);
-    if (!m) return OCPG_EHIP;
-    HIPCHK2(launchMaskFill(m, base, it->second.nb, 0, ctxStream(c)));
-    // the range is free once every iteration that may include it is done
-    HIPCHK2(hipStreamSynchronize(ctxStream(c)));
+    PoolSlot& S = it->second;
+    if (S.active) {
+        uint8_t* m = ctxEnsureMask(c);
+        if (!m) return OCPG_EHIP;
+        HIPCHK2(launchMaskFill(m, base, S.nb, 0, ctxStream(c)));
+        S.active = false;
+        S.offAt = ctxLaunches(c);
+    }
+    // the range's next occupant is written once every iteration that may
+    // include this one is done: the copy stream waits for them
+    if (S.offAt > ctxLaunchesDone(c)) {
+        const int rc = ctxCopyAfterMain(c);
+        if (rc != OCPG_OK) return rc;
+    }
     P.slots.erase(it);
     return OCPG_OK;
 }
@@ -601,6 +626,9 @@ int ocpg_pool_activate(ocpg_ctx* c, int64_t base, int on)
     uint8_t* m = ctxEnsureMask(c);
     if (!m) return OCPG_EHIP;
     HIPCHK2(launchMaskFill(m, base, it->second.nb, on ? 1 : 0, ctxStream(c)));
+    PoolSlot& S = it->second;
+    if (S.active && !on) S.offAt = ctxLaunches(c);
+    S.active = on != 0;
     return OCPG_OK;
 }
 
@@ -643,7 +671,181 @@ int ocpg_pool_download(ocpg_ctx* c, int64_t base, ocpg_bulk* h, int saved)
     if (P.capacity <= 0 || it == P.slots.end() || it->second.nb != h->nb) return OCPG_EINVAL;
     HIPCHK2(hipSetDevice(ctxDevice(c)));
     HIPCHK2(hipStreamSynchronize(ctxStream(c)));
+    HIPCHK2(hipStreamSynchronize(ctxCopyStream(c)));
     return ctxRangeCopy(c, base, h, false, saved != 0);
+}
+
+// Realisation images: a placed realisation's device state and pattern as
+// one host block (a header, then each piece in the device's own layout, 256
+// byte aligned), so that readmitting it is a few straight copies (no
+// transposition, no pattern build) on the pool's copy stream, which
+// overlap the iterations running on the context's stream.
+namespace {
+constexpr uint64_t kImageMagic = 0x474D493147504F43ull;  // "OCPG1IMG"
+struct ImageHeader {
+    uint64_t magic;
+    int32_t NC, maxNb;
+    int64_t nb, nconn, nnzb, base0;
+};
+static_assert(sizeof(ImageHeader) <= 256, "image header");
+
+// Every piece of the realisation at base, in image order (the state part
+// first: stateSegs of them).
+void imagePieces(ocpg_ctx* c, int64_t base, int64_t nb, int64_t nconn, int64_t nnzb, std::vector<ImageSeg>& seg,
+                 size_t& stateSegs)
+{
+    const PoolHost& P = ctxFimHost(c).pool;
+    FimDevice& F = ctxFim(c);
+    const int64_t kb = base * (1 + (int64_t)P.maxNb), cb = base * P.connPerCell;
+    seg.clear();
+    ctxImageSegments(c, base, nb, true, false, seg);
+    stateSegs = seg.size();
+    ctxImageSegments(c, base, nb, false, true, seg);
+    auto add = [&](void* p, size_t bytes) { seg.push_back({static_cast<char*>(p), bytes, bytes, 1}); };
+    add(F.IA + base, (size_t)nb * 4);
+    add(F.bulkEnd + base, (size_t)nb * 4);
+    add(F.JA + kb, (size_t)nnzb * 4);
+    add(F.slotConn + kb, (size_t)nnzb * 4);
+    add(F.slotMirror + kb, (size_t)nnzb * 4);
+    add(F.bId + cb, (size_t)nconn * 4);
+    add(F.eId + cb, (size_t)nconn * 4);
+    add(F.area + cb, (size_t)nconn * 8);
+}
+
+size_t imageOffsets(const std::vector<ImageSeg>& seg, std::vector<size_t>& off)
+{
+    size_t o = 256;
+    off.resize(seg.size());
+    for (size_t i = 0; i < seg.size(); i++) {
+        off[i] = o;
+        o += align256w(seg[i].width * seg[i].rows);
+    }
+    return o;
+}
+}  // namespace
+
+// Bytes of the image of the realisation at base.
+int64_t ocpg_pool_image_bytes(ocpg_ctx* c, int64_t base)
+{
+    if (!c) return OCPG_EINVAL;
+    PoolHost& P = ctxFimHost(c).pool;
+    auto it = P.slots.find(base);
+    if (P.capacity <= 0 || it == P.slots.end()) return OCPG_EINVAL;
+    std::vector<ImageSeg> seg;
+    std::vector<size_t> off;
+    size_t ns;
+    imagePieces(c, base, it->second.nb, it->second.nconn, it->second.nnzb, seg, ns);
+    return (int64_t)imageOffsets(seg, off);
+}
+
+// Copies the realisation at base (switched off) into image: everything with
+// all != 0, otherwise only its state (saved iteration inputs and last-step
+// moles) into an image it filled before.  Asynchronous on the copy stream
+// (ocpg_pool_sync_copies waits; page-locked images copy without staging).
+int ocpg_pool_export(ocpg_ctx* c, int64_t base, void* image, int all)
+{
+    if (!c || !image) return OCPG_EINVAL;
+    PoolHost& P = ctxFimHost(c).pool;
+    auto it = P.slots.find(base);
+    if (P.capacity <= 0 || it == P.slots.end() || it->second.active) return OCPG_EINVAL;
+    const PoolSlot& S = it->second;
+    ImageHeader* hd = static_cast<ImageHeader*>(image);
+    if (!all && (hd->magic != kImageMagic || hd->nb != S.nb || hd->nconn != S.nconn || hd->nnzb != S.nnzb))
+        return OCPG_EINVAL;
+    HIPCHK2(hipSetDevice(ctxDevice(c)));
+    hipStream_t cs = ctxCopyStream(c);
+    if (!cs) return OCPG_EHIP;
+    if (S.offAt > ctxLaunchesDone(c)) {
+        const int rc = ctxCopyAfterMain(c);
+        if (rc != OCPG_OK) return rc;
+    }
+    std::vector<ImageSeg> seg;
+    std::vector<size_t> off;
+    size_t ns;
+    imagePieces(c, base, S.nb, S.nconn, S.nnzb, seg, ns);
+    imageOffsets(seg, off);
+    char* img = static_cast<char*>(image);
+    const size_t n = all ? seg.size() : ns;
+    for (size_t i = 0; i < n; i++)
+        HIPCHK2(hipMemcpy2DAsync(img + off[i], seg[i].width, seg[i].dev, seg[i].pitch, seg[i].width, seg[i].rows,
+                                 hipMemcpyDeviceToHost, cs));
+    if (all) {
+        ImageHeader h{kImageMagic, ctxNC(c), P.maxNb, S.nb, S.nconn, S.nnzb, base};
+        std::memcpy(image, &h, sizeof(h));
+    }
+    return OCPG_OK;
+}
+
+// Places a realisation from its image at cells [base, base + nb) (a free
+// range): the copies and the pattern's move to the new base run on the copy
+// stream, and the context's later work waits for them.  The image must stay
+// as it is until ocpg_pool_sync_copies.
+int ocpg_pool_import(ocpg_ctx* c, int64_t base, const void* image)
+{
+    if (!c || !image) return OCPG_EINVAL;
+    PoolHost& P = ctxFimHost(c).pool;
+    if (P.capacity <= 0) return OCPG_ESTATE;
+    ImageHeader h;
+    std::memcpy(&h, image, sizeof(h));
+    if (h.magic != kImageMagic || h.NC != ctxNC(c) || h.maxNb != P.maxNb || h.nb <= 0 || h.nconn < 0 ||
+        h.nconn > h.nb * P.connPerCell || h.nnzb < h.nb || h.nnzb > h.nb * (1 + (int64_t)P.maxNb))
+        return OCPG_EINVAL;
+    const int64_t nb = h.nb;
+    if (base < 0 || base + nb > P.capacity) return OCPG_EINVAL;
+    {
+        auto it = P.slots.lower_bound(base);
+        if (it != P.slots.end() && it->first < base + nb) return OCPG_EINVAL;
+        if (it != P.slots.begin()) {
+            --it;
+            if (it->first + it->second.nb > base) return OCPG_EINVAL;
+        }
+    }
+    HIPCHK2(hipSetDevice(ctxDevice(c)));
+    hipStream_t cs = ctxCopyStream(c);
+    if (!cs) return OCPG_EHIP;
+    std::vector<ImageSeg> seg;
+    std::vector<size_t> off;
+    size_t ns;
+    imagePieces(c, base, nb, h.nconn, h.nnzb, seg, ns);
+    imageOffsets(seg, off);
+    const char* img = static_cast<const char*>(image);
+    // the realisation's own row starts, from the image's (at its old base)
+    const int64_t bpc = 1 + (int64_t)P.maxNb;
+    PoolSlot S;
+    S.nb = nb;
+    S.nconn = h.nconn;
+    S.nnzb = h.nnzb;
+    S.IA.resize(nb + 1);
+    const int32_t* IAimg = reinterpret_cast<const int32_t*>(img + off[seg.size() - 8]);
+    for (int64_t n = 0; n < nb; n++) S.IA[n] = (int32_t)(IAimg[n] - h.base0 * bpc);
+    S.IA[nb] = (int32_t)h.nnzb;
+    if (S.IA[0] != 0) return OCPG_EINVAL;
+    for (int64_t n = 0; n < nb; n++)
+        if (S.IA[n + 1] <= S.IA[n] || S.IA[n + 1] - S.IA[n] > bpc) return OCPG_EINVAL;
+    for (size_t i = 0; i < seg.size(); i++)
+        HIPCHK2(hipMemcpy2DAsync(seg[i].dev, seg[i].pitch, img + off[i], seg[i].width, seg[i].width, seg[i].rows,
+                                 hipMemcpyHostToDevice, cs));
+    const int64_t d = base - h.base0;
+    if (d != 0) {
+        FimDevice& F = ctxFim(c);
+        const int64_t kb = base * bpc, cb = base * P.connPerCell;
+        HIPCHK2(launchPoolRebase(F.IA + base, F.bulkEnd + base, nb, F.JA + kb, F.slotConn + kb, F.slotMirror + kb,
+                                 h.nnzb, F.bId + cb, F.eId + cb, h.nconn, d, d * P.connPerCell, d * bpc, cs));
+    }
+    const int rc = ctxMainAfterCopy(c);
+    if (rc != OCPG_OK) return rc;
+    P.slots[base] = std::move(S);
+    return OCPG_OK;
+}
+
+// Waits for the pool's copies (exports and imports).
+int ocpg_pool_sync_copies(ocpg_ctx* c)
+{
+    if (!c) return OCPG_EINVAL;
+    if (ctxFimHost(c).pool.capacity <= 0) return OCPG_ESTATE;
+    HIPCHK2(hipSetDevice(ctxDevice(c)));
+    HIPCHK2(hipStreamSynchronize(ctxCopyStream(c)));
+    return OCPG_OK;
 }
 
 int64_t ocpg_bsr_nnzb(ocpg_ctx* c) { return c ? ctxFim(c).nnzb : -1; }

@@ -30,10 +30,13 @@ struct WellsHost {
 struct PoolSlot {
     int64_t nb = 0, nconn = 0, nnzb = 0;
     std::vector<int32_t> IA;  // the realisation's own row starts (nb + 1, from 0)
+    bool active = false;
+    uint64_t offAt = 0;       // iteration launches queued when it was last switched off
 };
 struct PoolHost {
     int64_t capacity = 0;   // 0: not a pool
     int maxNb = 0;          // most neighbours of a cell
+    int connPerCell = 0;    // connection slots reserved per cell: ceil(maxNb / 2)
     std::map<int64_t, PoolSlot> slots;  // by cell base
 };
 

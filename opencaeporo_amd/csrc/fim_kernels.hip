@@ -1306,6 +1306,38 @@ hipError_t launchMaskFill(uint8_t* mask, int64_t base, int64_t n, uint8_t v, hip
     return hipGetLastError();
 }
 
+__global__ void __launch_bounds__(256) poolRebaseKernel(int32_t* IA, int32_t* bulkEnd, int64_t nb, int32_t* JA,
+                                                         int32_t* slotConn, int32_t* slotMirror, int64_t nnzb,
+                                                         int32_t* bId, int32_t* eId, int64_t nconn, int32_t dCell,
+                                                         int32_t dConn, int32_t dBlock)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nb) {
+        IA[i] += dBlock;
+        bulkEnd[i] += dBlock;
+    }
+    if (i < nnzb) {
+        JA[i] += dCell;
+        if (slotConn[i] >= 0) slotConn[i] += dConn;
+        slotMirror[i] += dBlock;
+    }
+    if (i < nconn) {
+        bId[i] += dCell;
+        eId[i] += dCell;
+    }
+}
+
+hipError_t launchPoolRebase(int32_t* IA, int32_t* bulkEnd, int64_t nb, int32_t* JA, int32_t* slotConn,
+                            int32_t* slotMirror, int64_t nnzb, int32_t* bId, int32_t* eId, int64_t nconn,
+                            int64_t dCell, int64_t dConn, int64_t dBlock, hipStream_t s)
+{
+    const int64_t n = std::max(nb, std::max(nnzb, nconn));
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(poolRebaseKernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, IA, bulkEnd, nb, JA,
+                       slotConn, slotMirror, nnzb, bId, eId, nconn, (int32_t)dCell, (int32_t)dConn, (int32_t)dBlock);
+    return hipGetLastError();
+}
+
 __global__ void __launch_bounds__(256) maskedCopyKernel(const uint8_t* __restrict__ mask, int64_t nb, int64_t ld,
                                                          MaskedCopy d)
 {

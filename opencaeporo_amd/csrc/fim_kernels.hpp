@@ -119,6 +119,18 @@ struct MaskedCopy {
     int nfields;
 };
 hipError_t launchMaskFill(uint8_t* mask, int64_t base, int64_t n, uint8_t v, hipStream_t s);
+// A realisation's pattern moved by dCell cells (its connection slots by
+// dConn, its blocks by dBlock): IA and bulkEnd (nb), JA, slotConn (where
+// >= 0) and slotMirror (nnzb), bId and eId (nconn).
+hipError_t launchPoolRebase(int32_t* IA, int32_t* bulkEnd, int64_t nb, int32_t* JA, int32_t* slotConn,
+                            int32_t* slotMirror, int64_t nnzb, int32_t* bId, int32_t* eId, int64_t nconn,
+                            int64_t dCell, int64_t dConn, int64_t dBlock, hipStream_t s);
+// One strided piece of device memory a realisation image holds: rows rows
+// of width bytes, pitch bytes apart.
+struct ImageSeg {
+    char* dev;
+    size_t pitch, width, rows;
+};
 hipError_t launchMaskedCopy(const uint8_t* mask, int64_t nb, int64_t ld, const MaskedCopy& d, hipStream_t s);
 
 }  // namespace ocpg

@@ -113,8 +113,16 @@ struct ocpg_ctx {
     // copy of the flash inputs (ocpg_save_flash_inputs)
     char* snap = nullptr;
     size_t snapBytes = 0;
-    // serving pool: per-cell activity mask (B.mask), NULL outside a pool
+    // serving pool: per-cell activity mask (B.mask), NULL outside a pool;
+    // a pool keeps no last-step copy but the moles (poolLNi), and moves
+    // realisation images on a stream of its own
     uint8_t* mask = nullptr;
+    bool pool = false;
+    double* poolLNi = nullptr;
+    hipStream_t copyStream = nullptr;
+    hipEvent_t copyEv = nullptr, mainEv = nullptr;
+    // replays of the captured iteration launched / known to have finished
+    uint64_t launches = 0, launchesDone = 0;
     // overlapped Newton iteration (ocpg_update_assemble): side streams of the
     // flash and the rows recomputed after they join
     FlashSide side{};
@@ -444,6 +452,13 @@ int ocpg_destroy(ocpg_ctx* c)
     if (c->wells.mem) (void)hipFree(c->wells.mem);
     if (c->snap) (void)hipFree(c->snap);
     if (c->mask) (void)hipFree(c->mask);
+    if (c->poolLNi) (void)hipFree(c->poolLNi);
+    if (c->copyStream) {
+        (void)hipStreamSynchronize(c->copyStream);
+        (void)hipStreamDestroy(c->copyStream);
+    }
+    if (c->copyEv) (void)hipEventDestroy(c->copyEv);
+    if (c->mainEv) (void)hipEventDestroy(c->mainEv);
     if (c->workMem) (void)hipFree(c->workMem);
     if (c->cur) (void)hipFree(c->cur);
     if (c->last) (void)hipFree(c->last);
@@ -470,8 +485,12 @@ int ocpg_set_cells(ocpg_ctx* c, int64_t nb)
     if (c->cur) HIPCHK(hipFree(c->cur));
     if (c->last) HIPCHK(hipFree(c->last));
     if (c->mask) HIPCHK(hipFree(c->mask));
+    if (c->copyStream) HIPCHK(hipStreamSynchronize(c->copyStream));
+    if (c->poolLNi) HIPCHK(hipFree(c->poolLNi));
     c->cur = c->last = nullptr;
     c->mask = nullptr;
+    c->poolLNi = nullptr;
+    c->pool = false;
     c->lastValid = false;
     const int64_t ld = (nb + 63) / 64 * 64;
     size_t off = 0;
@@ -766,7 +785,7 @@ int ocpg_init_flash_fim(ocpg_ctx* c, const double* zi)
 // skip-stability memory: device-to-device copy into the last-step block.
 int ocpg_update_last_step(ocpg_ctx* c)
 {
-    if (!c || !c->cur) return OCPG_ESTATE;
+    if (!c || !c->cur || c->pool) return OCPG_ESTATE;
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipMemcpyAsync(c->last, c->cur, c->lastBytes, hipMemcpyDeviceToDevice, c->stream));
     // AllWells::UpdateLastTimeStepBHP
@@ -783,7 +802,7 @@ int ocpg_update_last_step(ocpg_ctx* c)
 // caller's ocpg_calres does CalFlux and CalRes.
 int ocpg_reset_to_last_step(ocpg_ctx* c)
 {
-    if (!c || !c->cur || !c->lastValid) return OCPG_ESTATE;
+    if (!c || !c->cur || !c->lastValid || c->pool) return OCPG_ESTATE;
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipMemcpyAsync(c->cur, c->last, c->lastBytes, hipMemcpyDeviceToDevice, c->stream));
     if (c->wells.nw > 0) HIPCHK(launchWellPrepare(c->NC, c->dEos, c->B, c->wells, 1, c->stream));
@@ -891,6 +910,7 @@ int ocpg_synchronize(ocpg_ctx* c)
     if (!c) return OCPG_EINVAL;
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipStreamSynchronize(c->stream));
+    c->launchesDone = c->launches;
     return OCPG_OK;
 }
 
@@ -933,6 +953,7 @@ int ocpg_launch_iteration(ocpg_ctx* c)
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipGraphLaunch(c->iterGraph, c->stream));
     HIPCHK(hipEventRecord(c->iterDone, c->stream));
+    c->launches++;
     return OCPG_OK;
 }
 
@@ -941,7 +962,10 @@ int ocpg_iteration_done(ocpg_ctx* c)
 {
     if (!c || !c->iterDone) return OCPG_ESTATE;
     const hipError_t e = hipEventQuery(c->iterDone);
-    if (e == hipSuccess) return 1;
+    if (e == hipSuccess) {
+        c->launchesDone = c->launches;
+        return 1;
+    }
     if (e == hipErrorNotReady) return 0;
     return hipStatus(e);
 }
@@ -989,6 +1013,82 @@ uint8_t* ctxEnsureMask(ocpg_ctx* c)
     c->B.mask = c->mask;
     dropIterGraph(c);
     return c->mask;
+}
+
+// A pool's cell state: the last-step copy shrinks to the moles CalRes reads
+// (a pool runs single Newton iterations, not time steps).
+int ctxPoolTrimLast(ocpg_ctx* c)
+{
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->last) HIPCHK(hipFree(c->last));
+    c->last = nullptr;
+    c->lastBytes = 0;
+    c->lastValid = false;
+    std::memset(&c->Blast, 0, sizeof(DevBulk));
+    const size_t n = (size_t)(c->NC + 1) * (size_t)std::max<int64_t>(c->B.ld, 64);
+    HIPCHK(hipMalloc(&c->poolLNi, n * sizeof(double)));
+    c->B.lNi = c->poolLNi;
+    c->pool = true;
+    return OCPG_OK;
+}
+
+// The pool's copy stream (created on first use) and the orderings between
+// it and the context's stream, by events: copies after the work queued so
+// far on the context's stream, and the context's later work after the
+// copies queued so far.
+hipStream_t ctxCopyStream(ocpg_ctx* c)
+{
+    if (!c->copyStream) {
+        if (hipStreamCreateWithFlags(&c->copyStream, hipStreamNonBlocking) != hipSuccess) return nullptr;
+        if (hipEventCreateWithFlags(&c->copyEv, hipEventDisableTiming) != hipSuccess) return nullptr;
+        if (hipEventCreateWithFlags(&c->mainEv, hipEventDisableTiming) != hipSuccess) return nullptr;
+    }
+    return c->copyStream;
+}
+int ctxCopyAfterMain(ocpg_ctx* c)
+{
+    HIPCHK(hipEventRecord(c->mainEv, c->stream));
+    HIPCHK(hipStreamWaitEvent(c->copyStream, c->mainEv, 0));
+    return OCPG_OK;
+}
+int ctxMainAfterCopy(ocpg_ctx* c)
+{
+    HIPCHK(hipEventRecord(c->copyEv, c->copyStream));
+    HIPCHK(hipStreamWaitEvent(c->stream, c->copyEv, 0));
+    return OCPG_OK;
+}
+uint64_t ctxLaunches(ocpg_ctx* c) { return c->launches; }
+uint64_t ctxLaunchesDone(ocpg_ctx* c) { return c->launchesDone; }
+
+// Device segments of the cell state of cells [base, base + nb) that a
+// realisation image holds, in image order: with `state`, the saved
+// iteration inputs and the last-step moles (what an iteration may change
+// between admissions); with `fixed`, T, v, depth and poroInit.
+void ctxImageSegments(ocpg_ctx* c, int64_t base, int64_t nb, bool state, bool fixed, std::vector<ImageSeg>& out)
+{
+    const int64_t ld = c->B.ld;
+    size_t snapOff = 0;
+    for (int f = 0; f < kNumFields; f++) {
+        const FieldDesc& d = kFields[f];
+        const int64_t cnt = cellEntries(c, d.cnt);
+        const size_t fieldBytes = (size_t)d.elem * cnt * ld;
+        if (isFlashInput(d.hostOff)) {
+            if (state)
+                out.push_back({c->snap + snapOff + (size_t)base * d.elem, (size_t)ld * d.elem, (size_t)nb * d.elem,
+                               (size_t)cnt});
+            snapOff += align256(fieldBytes);
+        }
+    }
+    if (state)
+        out.push_back({reinterpret_cast<char*>(c->B.lNi + base), (size_t)ld * 8, (size_t)nb * 8,
+                       (size_t)(c->NC + 1)});
+    if (fixed)
+        for (int f = 0; f < kNumFields; f++) {
+            const FieldDesc& d = kFields[f];
+            if (d.hostOff == HB(T) || d.hostOff == HB(v) || d.hostOff == HB(depth) || d.hostOff == HB(poroInit))
+                out.push_back({c->cur + c->fieldOff[f] + (size_t)base * d.elem, (size_t)ld * d.elem,
+                               (size_t)nb * d.elem, (size_t)cellEntries(c, d.cnt)});
+        }
 }
 
 // Back to an ordinary context: every cell takes part.
@@ -1068,8 +1168,9 @@ int ctxRangeCopy(ocpg_ctx* c, int64_t base, ocpg_bulk* h, bool toDevice, bool sa
     return OCPG_OK;
 }
 
-// Saved iteration inputs of cells [base, base + nb): current -> saved copy.
-int ctxSaveInputsRange(ocpg_ctx* c, int64_t base, int64_t nb)
+// Saved iteration inputs of cells [base, base + nb): current -> saved copy,
+// on `stream`.
+int ctxSaveInputsRange(ocpg_ctx* c, int64_t base, int64_t nb, hipStream_t stream)
 {
     if (!c->snap) return OCPG_ESTATE;
     const int64_t ld = c->B.ld;
@@ -1080,7 +1181,7 @@ int ctxSaveInputsRange(ocpg_ctx* c, int64_t base, int64_t nb)
         const int64_t cnt = cellEntries(c, d.cnt);
         const size_t o = (size_t)base * d.elem, pitch = (size_t)ld * d.elem;
         HIPCHK(hipMemcpy2DAsync(c->snap + off + o, pitch, c->cur + c->fieldOff[f] + o, pitch, (size_t)nb * d.elem,
-                                cnt, hipMemcpyDeviceToDevice, c->stream));
+                                cnt, hipMemcpyDeviceToDevice, stream));
         off += align256((size_t)d.elem * cnt * ld);
     }
     return OCPG_OK;

@@ -89,6 +89,10 @@ def load():
         "ocpg_pool_system": ([P, C.c_int64, P, P, P, P, P], C.c_int),
         "ocpg_pool_download": ([P, C.c_int64, P, C.c_int], C.c_int),
         "ocpg_pool_remove": ([P, C.c_int64], C.c_int),
+        "ocpg_pool_image_bytes": ([P, C.c_int64], C.c_int64),
+        "ocpg_pool_export": ([P, C.c_int64, P, C.c_int], C.c_int),
+        "ocpg_pool_import": ([P, C.c_int64, P], C.c_int),
+        "ocpg_pool_sync_copies": ([P], C.c_int),
         "ocpg_save_flash_inputs": ([P], C.c_int),
         "ocpg_restore_flash_inputs": ([P], C.c_int),
         "ocpg_timing_enable": ([P, C.c_int], C.c_int),
@@ -396,6 +400,21 @@ class Context:
 
     def pool_remove(self, base):
         check(self.lib.ocpg_pool_remove(self.h, int(base)), "ocpg_pool_remove")
+
+    def pool_image_bytes(self, base):
+        n = int(self.lib.ocpg_pool_image_bytes(self.h, int(base)))
+        if n < 0:
+            raise OcpgError(f"ocpg_pool_image_bytes failed with code {n}")
+        return n
+
+    def pool_export(self, base, image_ptr, all=True):
+        check(self.lib.ocpg_pool_export(self.h, int(base), image_ptr, 1 if all else 0), "ocpg_pool_export")
+
+    def pool_import(self, base, image_ptr):
+        check(self.lib.ocpg_pool_import(self.h, int(base), image_ptr), "ocpg_pool_import")
+
+    def pool_sync_copies(self):
+        check(self.lib.ocpg_pool_sync_copies(self.h), "ocpg_pool_sync_copies")
 
     def save_flash_inputs(self):
         check(self.lib.ocpg_save_flash_inputs(self.h), "ocpg_save_flash_inputs")

@@ -1,31 +1,41 @@
 """Serving form: continuous batching of Newton-iteration requests from many
 ensemble realisations on one GPU.
 
-Each realisation is a reservoir model of its own size (its own grid, fluid
-state and wells) driven through the reference's FIM Newton loop; a request
-asks for one Newton iteration of one realisation (flash + kr/Pc + rock +
-CalRes + AssembleMat on its resident state).  The server
+Each realisation is a reservoir model of its own size (its own grid and
+fluid state, no wells) driven through the reference's FIM Newton loop; a
+request asks for one Newton iteration of one realisation (flash + kr/Pc +
+rock + CalRes + AssembleMat on its resident state, from its saved iteration
+inputs).
 
-* keeps a realisation's state resident in HBM between its iterations (one
-  C-ABI context per realisation, `ocpg_ctx`, with its own stream);
-* admits a realisation by device-memory budget: when the budget is full, the
-  least recently used idle realisation is evicted (its primary variables and
-  last-step state go to host memory) and readmitted on its next request;
-* runs each iteration as one replay of a HIP graph captured at admission
-  (`ocpg_capture_iteration`), so a request costs one launch;
-* keeps admitting while others run: requests that arrive while the GPU is
-  busy start as soon as their realisation is free and memory allows, and
-  their kernels overlap with the ones already on the device (each context's
-  stream runs independently).
+Continuous batching (`Pool`, `BatchServer`):
 
-`run()` replays a Poisson arrival sequence in real time and reports
-throughput and the p50/p99 of request latency (arrival to completion), for
-continuous batching and for one-at-a-time service of the same arrivals.
+* one C-ABI context is a pool of cells (`ocpg_pool_init`); a realisation is
+  placed at a free range of it (`ocpg_pool_place`) and stays resident
+  between its iterations;
+* admission is by the pool's capacity, i.e. the device-memory budget: when
+  no free range fits, the least recently used idle realisation is evicted
+  (its state goes back to host memory) and readmitted on its next request;
+* a batch is every request that has arrived when the GPU finishes the
+  previous one; its realisations' masks are switched on
+  (`ocpg_pool_activate`) and the batch runs as ONE replay of an iteration
+  graph captured once for the whole pool, so a batch of many small
+  realisations costs one set of launches;
+* while a batch runs, the realisations of waiting requests are placed into
+  free ranges, overlapping their uploads with the GPU's work.
 
-    python -m opencaeporo_amd.serving --realisations 64 --rate 200
+One at a time (`Server` with max_inflight=1): the plain GPU path, each
+realisation on a context of its own with its own captured iteration,
+admitted and evicted by the same budget, one request per launch.
+
+`benchmark()` measures both modes' capacity (all requests queued at once)
+and their p50/p99 latency (arrival to completion) for the same Poisson
+arrivals.
+
+    python -m opencaeporo_amd.serving --realisations 256 --cells-hi 1e6
 """
 import argparse
 import collections
+import ctypes as C
 import json
 import sys
 import time
@@ -57,6 +67,8 @@ class Realisation:
         self.NC = eos.nc
         self.ctx = None
         self.host = None      # state while not resident
+        self.image = None     # page-locked image for a pool (ocpg_pool_export)
+        self.image_bytes = 0
         self.bytes = 0
         self.busy = False
         self.last_used = 0.0
@@ -229,6 +241,33 @@ class Pool:
                 return b
         return None
 
+    def prime(self, realisations, log=lambda *a: None):
+        """Builds each realisation's image (placed once through the cell
+        layout, then exported), so that admissions are image imports; the
+        pool is left empty."""
+        ctx = self.ctx
+        for k, r in enumerate(realisations):
+            if r.image is not None:
+                continue
+            if r.nb > self.capacity:
+                raise ValueError(f"realisation {r.rid} ({r.nb} cells) exceeds the pool")
+            ctx.pool_place(0, r.host, r.grid)
+            r.image_bytes = ctx.pool_image_bytes(0)
+            p = ctx.lib.ocpg_host_alloc(r.image_bytes)
+            if not p:
+                raise lib.OcpgError(f"ocpg_host_alloc({r.image_bytes}) failed")
+            r.image = p
+            ctx.pool_export(0, p, all=True)
+            ctx.pool_sync_copies()
+            ctx.pool_remove(0)
+        ctx.synchronize()
+
+    def free_images(self, realisations):
+        for r in realisations:
+            if r.image is not None:
+                self.ctx.lib.ocpg_host_free(C.c_void_p(r.image))
+                r.image = None
+
     def release(self, base, nb):
         self.free.append([base, nb])
         self.free.sort()
@@ -256,6 +295,8 @@ class BatchServer:
     the same pool."""
 
     def __init__(self, realisations, pool, max_batch=None):
+        """pool: an empty Pool on which every realisation has an image
+        (Pool.prime)."""
         self.real = {r.rid: r for r in realisations}
         self.pool = pool
         self.max_batch = max_batch or len(realisations)
@@ -269,14 +310,17 @@ class BatchServer:
         self.max_batch_seen = 0
 
     def _evict(self, rid):
+        """Back to host memory: the state part of its image is refreshed (on
+        the pool's copy stream) and its range freed."""
         r = self.real[rid]
         ctx = self.pool.ctx
+        base = self.base.pop(rid)
         if rid in self.active:
-            ctx.pool_activate(self.base[rid], False)
+            ctx.pool_activate(base, False)
             self.active.discard(rid)
-        ctx.pool_download(self.base[rid], r.host, saved=True)
-        ctx.pool_remove(self.base[rid])
-        self.pool.release(self.base.pop(rid), r.nb)
+        ctx.pool_export(base, r.image, all=False)
+        ctx.pool_remove(base)
+        self.pool.release(base, r.nb)
         self.evictions += 1
 
     def _admit(self, rid, keep, evict=True):
@@ -296,7 +340,7 @@ class BatchServer:
                 return False
             self._evict(min(idle, key=lambda q: self.last_used.get(q, 0.0)))
             base = self.pool.alloc(r.nb)
-        self.pool.ctx.pool_place(base, r.host, r.grid)
+        self.pool.ctx.pool_import(base, r.image)
         self.base[rid] = base
         self.admissions += 1
         return True
@@ -367,6 +411,12 @@ class BatchServer:
         """The realisation's system from its last iteration (own indexing)."""
         return self.pool.ctx.pool_system(self.base[rid], self.real[rid].nb)
 
+    def clear(self):
+        """Evicts every resident realisation (the pool is empty again)."""
+        for rid in list(self.base):
+            self._evict(rid)
+        self.pool.ctx.pool_sync_copies()
+
 
 def make_ensemble(n, cells_lo, cells_hi, seed, eos, sat, rock):
     rng = np.random.default_rng(seed)
@@ -390,76 +440,128 @@ def summarize(reqs, ens):
                 makespan_s=span)
 
 
-def benchmark(n_real=64, cells_lo=1e3, cells_hi=2e5, n_req=256, rate=None, budget_gb=16.0, seed=7,
+def _reset(ens):
+    for r in ens:
+        if r.resident:
+            r.evict()
+        r.busy = False
+
+
+def pool_bytes_per_cell(eos, sat, rock, probe=1 << 16):
+    """Device bytes per cell of a pool (measured on a small one)."""
+    p = Pool(eos, sat, rock, probe)
+    b = p.bytes / probe
+    p.close()
+    return b
+
+
+def benchmark(n_real=256, cells_lo=1e3, cells_hi=1e6, n_req=1024, rate=None, budget_gb=96.0, seed=7,
               log=lambda *a: None, load=0.8):
-    """load: the offered arrival rate as a fraction of the one-at-a-time
-    capacity measured on the same ensemble (used when rate is None)."""
+    """The serving form on an ensemble of n_real realisations with cell counts
+    log-uniform in [cells_lo, cells_hi], within a device-memory budget:
+
+    * one_at_a_time: the plain GPU path, one request at a time (each
+      realisation on a context of its own with its captured iteration,
+      admitted and evicted by the budget);
+    * continuous: continuous batching on one pool of the same budget
+      (BatchServer), admissions by image import;
+    * pool_one_at_a_time: the same pool and admissions, one request per
+      launch (what batching alone gains).
+
+    Each mode's capacity (every request queued at once) is measured, and
+    each serves the same Poisson arrivals at `load` x the one-at-a-time
+    capacity (or at `rate` requests/s)."""
     deck = fluids.bench6_deck()
     eos = lib.eos_from_deck(deck)
     sat = fluids.make_satfunc()
     rock = fluids.make_rock()
     ens = make_ensemble(n_real, cells_lo, cells_hi, seed, eos, sat, rock)
     t0 = time.time()
-    for r in ens:
+    for k, r in enumerate(ens):
         r.initialize()
+        if (k + 1) % 32 == 0:
+            log(f"[serving] initialised {k + 1}/{n_real} realisations ({time.time() - t0:.1f}s)")
     total = sum(r.nb for r in ens)
-    log(f"[serving] {n_real} realisations, {total} cells, initial states in {time.time() - t0:.1f}s")
-    # service time of one request on an idle GPU, to set the arrival rate
-    probe = max(ens, key=lambda r: r.nb)
-    probe.admit()
-    probe.ctx.launch_iteration()
-    probe.ctx.synchronize()
-    t1 = time.perf_counter()
-    for _ in range(3):
-        probe.ctx.launch_iteration()
-    probe.ctx.synchronize()
-    largest_ms = (time.perf_counter() - t1) / 3 * 1e3
-    probe.evict()
-    log(f"[serving] largest realisation {probe.nb} cells: {largest_ms:.2f} ms per iteration")
-    if rate is None:
-        # offered load: a fraction of what one-at-a-time service sustains on
-        # this ensemble (measured with every request queued at once)
-        for r in ens:
-            r.busy = False
-        burst = [serving_request for serving_request in poisson_requests(n_req, 1e9, n_real, seed + 5)]
-        srv = Server(ens, budget_gb * 1e9, max_inflight=1)
-        srv.run(burst)
-        cap = summarize(burst, {r.rid: r for r in ens})["throughput_req_s"]
-        rate = load * cap
-        log(f"[serving] one-at-a-time capacity {cap:.1f} requests/s; offered load {rate:.1f}/s")
+    by_id = {r.rid: r for r in ens}
+    budget = budget_gb * 1e9
+    per_cell = pool_bytes_per_cell(eos, sat, rock)
+    capacity = int(min(budget / per_cell, (1 << 31) - 1))
+    capacity = max(capacity, max(r.nb for r in ens))
+    log(f"[serving] {n_real} realisations, {total} cells (largest {max(r.nb for r in ens)}), initial states in "
+        f"{time.time() - t0:.1f}s; budget {budget_gb:g} GB = pool of {capacity} cells ({per_cell:.0f} B/cell)")
     out = {}
-    for mode, mi in (("continuous", None), ("one_at_a_time", 1)):
-        for r in ens:
-            if r.resident:
-                r.evict()
-            r.busy = False
-        reqs = poisson_requests(n_req, rate, n_real, seed + 5)
-        srv = Server(ens, budget_gb * 1e9, max_inflight=mi)
+
+    def one_at_a_time(reqs):
+        _reset(ens)
+        srv = Server(ens, budget, max_inflight=1)
         srv.run(reqs)
-        s = summarize(reqs, {r.rid: r for r in ens})
-        s.update(admissions=srv.admissions, evictions=srv.evictions, max_concurrent=srv.max_concurrent)
-        out[mode] = s
-        log(f"[serving] {mode}: " + ", ".join(f"{k} {v:.4g}" if isinstance(v, float) else f"{k} {v}"
-                                              for k, v in s.items()))
-    for r in ens:
-        if r.resident:
-            r.evict()
+        s = summarize(reqs, by_id)
+        s.update(admissions=srv.admissions, evictions=srv.evictions)
+        _reset(ens)
+        return s
+
+    pool = None
+
+    def batched(reqs, max_batch=None):
+        srv = BatchServer(ens, pool, max_batch=max_batch)
+        srv.run(reqs)
+        s = summarize(reqs, by_id)
+        s.update(admissions=srv.admissions, evictions=srv.evictions, batches=srv.batches,
+                 mean_batch=len(reqs) / max(srv.batches, 1), max_batch=srv.max_batch_seen)
+        srv.clear()
+        return s
+
+    burst = lambda: poisson_requests(n_req, 1e9, n_real, seed + 5)
+    # the per-realisation contexts first: they and the pool never share the GPU
+    cap1 = one_at_a_time(burst())
+    if rate is None:
+        rate = load * cap1["throughput_req_s"]
+    arrivals = lambda: poisson_requests(n_req, rate, n_real, seed + 5)
+    out["one_at_a_time"] = one_at_a_time(arrivals())
+    t1 = time.time()
+    pool = Pool(eos, sat, rock, capacity)
+    pool.prime(ens)
+    log(f"[serving] pool of {capacity} cells ({pool.bytes / 1e9:.1f} GB) and {n_real} images "
+        f"({sum(r.image_bytes for r in ens) / 1e9:.1f} GB page-locked) in {time.time() - t1:.1f}s")
+    capB = batched(burst())
+    capP = batched(burst(), max_batch=1)
+    log(f"[serving] capacity (all {n_req} requests queued at once): one at a time {cap1['throughput_req_s']:.1f} "
+        f"req/s, pool one at a time {capP['throughput_req_s']:.1f} req/s, continuous batching "
+        f"{capB['throughput_req_s']:.1f} req/s")
+    out["capacity"] = {"one_at_a_time_req_s": cap1["throughput_req_s"],
+                       "pool_one_at_a_time_req_s": capP["throughput_req_s"],
+                       "continuous_req_s": capB["throughput_req_s"],
+                       "one_at_a_time_cell_updates_s": cap1["cell_updates_s"],
+                       "continuous_cell_updates_s": capB["cell_updates_s"],
+                       "continuous_mean_batch": capB["mean_batch"]}
+    out["continuous"] = batched(arrivals())
+    out["pool_one_at_a_time"] = batched(arrivals(), max_batch=1)
+    for mode in ("one_at_a_time", "continuous", "pool_one_at_a_time"):
+        log(f"[serving] {mode} at {rate:.1f} req/s: " + ", ".join(
+            f"{k} {v:.4g}" if isinstance(v, float) else f"{k} {v}" for k, v in out[mode].items()))
+    out["gain"] = {"p99": out["one_at_a_time"]["p99_ms"] / out["continuous"]["p99_ms"],
+                   "p50": out["one_at_a_time"]["p50_ms"] / out["continuous"]["p50_ms"],
+                   "throughput_at_capacity": capB["throughput_req_s"] / cap1["throughput_req_s"],
+                   "batching_alone_throughput_at_capacity": capB["throughput_req_s"] / capP["throughput_req_s"]}
+    pool.free_images(ens)
+    pool.close()
     out["config"] = dict(realisations=n_real, cells_lo=cells_lo, cells_hi=cells_hi, total_cells=total,
-                         requests=n_req, rate_per_s=rate, budget_gb=budget_gb, load=load)
+                         requests=n_req, rate_per_s=rate, budget_gb=budget_gb, pool_cells=capacity, load=load)
     return out
 
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--realisations", type=int, default=64)
+    ap.add_argument("--realisations", type=int, default=256)
     ap.add_argument("--cells-lo", type=float, default=1e3)
-    ap.add_argument("--cells-hi", type=float, default=2e5)
-    ap.add_argument("--requests", type=int, default=256)
+    ap.add_argument("--cells-hi", type=float, default=1e6)
+    ap.add_argument("--requests", type=int, default=1024)
     ap.add_argument("--rate", type=float, default=None)
-    ap.add_argument("--budget-gb", type=float, default=16.0)
+    ap.add_argument("--budget-gb", type=float, default=96.0)
+    ap.add_argument("--load", type=float, default=0.8)
     a = ap.parse_args()
     res = benchmark(a.realisations, a.cells_lo, a.cells_hi, a.requests, a.rate, a.budget_gb,
-                    log=lambda *x: print(*x, file=sys.stderr, flush=True))
+                    log=lambda *x: print(*x, file=sys.stderr, flush=True), load=a.load)
     print(json.dumps(res), flush=True)
 
 

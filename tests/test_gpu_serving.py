@@ -133,6 +133,7 @@ def test_pool_batches_give_each_realisation_its_own_result():
     ref = {r.rid: _alone(r) for r in ens}
     total = sum(r.nb for r in ens)
     pool = serving.Pool(ens[0].eos, ens[0].sat, ens[0].rock, capacity=int(0.6 * total))
+    pool.prime(ens)
     srv = serving.BatchServer(ens, pool)
     reqs = serving.poisson_requests(40, rate=3000.0, n_real=len(ens), seed=12)
     srv.run(reqs)
@@ -154,4 +155,45 @@ def test_pool_batches_give_each_realisation_its_own_result():
             break
         srv.run([serving.Request(rid, 0.0) for rid in rest])
     assert checked == {r.rid for r in ens}
+    srv.clear()
+    pool.free_images(ens)
+    pool.close()
+
+
+def test_pool_image_moves_a_realisation():
+    """A realisation placed from its cell state, exported, removed and
+    imported at another base gives the same system; the state part of the
+    image survives an export/import round trip."""
+    gpu_util.say("[serving] pool image export / import")
+    a, b = models(2, 3000, 6000, seed=21)
+    for r in (a, b):
+        r.initialize()
+    pool = serving.Pool(a.eos, a.sat, a.rock, capacity=a.nb + b.nb + 100)
+    pool.prime([a, b])
+    ctx = pool.ctx
+    ctx.pool_import(0, b.image)            # b first, so a lands at a new base
+    ctx.pool_import(b.nb + 37, a.image)
+    for base in (0, b.nb + 37):
+        ctx.pool_activate(base, True)
+    ctx.launch_iteration()
+    ctx.synchronize()
+    for base, r in ((0, b), (b.nb + 37, a)):
+        got = ctx.pool_system(base, r.nb)
+        for x, y in zip(got, _alone(r)):
+            np.testing.assert_array_equal(x, y)
+        back = r.host.copy()
+        ctx.pool_download(base, back, saved=True)
+        for f in serving.STATE_FIELDS:
+            np.testing.assert_array_equal(getattr(back, f), getattr(r.host, f), err_msg=f)
+    with pytest.raises(lib.OcpgError):
+        ctx.pool_export(0, b.image, all=False)      # still switched on
+    for base in (0, b.nb + 37):
+        ctx.pool_activate(base, False)
+    ctx.pool_export(0, b.image, all=False)
+    ctx.pool_remove(0)
+    ctx.pool_remove(b.nb + 37)
+    ctx.pool_sync_copies()
+    with pytest.raises(lib.OcpgError):
+        ctx.reset_to_last_step()                    # a pool keeps no last step
+    pool.free_images([a, b])
     pool.close()
