@@ -118,6 +118,57 @@ def setup(nx, ny, nz, seed=2024, irregular=False, model=None):
     return ctx, grid, nb, NC, wells
 
 
+def learned_leg(ctx, nb, NC, reps=3):
+    """The learned form on the benchmark's own field: the flash alone
+    (ocpg_flash_fim) against ocpg_flash_fim_learned (the network's fused
+    inference from the cells' P, T and moles, then the guided flash), cold
+    (no previous phase information) and warm (the timed Newton pass's
+    input), with the network trained on the GPU on 10M flash results
+    (python -m opencaeporo_amd.learned --out ...).  Medians of reps timed
+    passes; the agreement of the converged results."""
+    from opencaeporo_amd import learned
+    path = learned.weights_path(NC)
+    if not os.path.exists(path):
+        return None
+    d = np.load(path, allow_pickle=False)
+    ctx.set_learned([d[f"W{l}"] for l in range(4)], [d[f"b{l}"] for l in range(4)])
+    cold = abi.Bulk(nb, NC, fields=["phaseNum", "skipFlag"])
+    out = {"weights": os.path.relpath(path, os.path.dirname(os.path.abspath(__file__)))}
+    for kind in ("cold", "warm"):
+        res, got = {}, {}
+        for mode in ("plain", "learned"):
+            ts = []
+            for rep in range(reps + 1):
+                ctx.restore_flash_inputs()
+                if kind == "cold":
+                    ctx.upload(cold)
+                ctx.synchronize()
+                ctx.timing(rep == reps)
+                t0 = time.perf_counter()
+                st = ctx.flash_fim(stats=True) if mode == "plain" else ctx.flash_fim_learned(stats=True)
+                ctx.synchronize()
+                if rep > 0:
+                    ts.append((time.perf_counter() - t0) * 1e3)
+            stages = {k: ctx.timer(k)[0] for k in ("flash_stability", "flash_split", "flash_finish", "mlp")}
+            ctx.timing(False)
+            o = abi.Bulk(nb, NC, fields=["phaseNum", "S", "xij"])
+            ctx.download(o)
+            got[mode] = o
+            res[mode] = {"flash_ms": float(np.median(ts)), "stage_ms": stages,
+                         "iters": dict(zip(["ssm_sta", "nr_sta", "ssm_sp", "nr_sp", "rr"], [int(v) for v in st.iters])),
+                         "two_phase": int(st.two_phase), "nonconverged": int(st.nonconverged)}
+            log(f"[bench] learned leg, {kind} {mode}: {res[mode]['flash_ms']:.2f} ms, iterations {res[mode]['iters']}")
+        same = got["plain"].phaseNum == got["learned"].phaseNum
+        dS = np.abs(got["plain"].S.reshape(nb, 3)[same] - got["learned"].S.reshape(nb, 3)[same])
+        res["agreement"] = {"phase_count_differs": int((~same).sum()), "max_abs_S_diff": float(dS.max()),
+                            "p99_abs_S_diff": float(np.quantile(dS.max(axis=1), 0.99))}
+        res["gain"] = res["plain"]["flash_ms"] / res["learned"]["flash_ms"]
+        res["inference_ms"] = res["learned"]["stage_ms"]["mlp"]
+        out[kind] = res
+    ctx.set_learned(None)
+    return out
+
+
 def step(ctx, serial=False):
     ctx.restore_flash_inputs()
     step_body(ctx, serial)
@@ -316,6 +367,12 @@ def main():
             ts.append((time.perf_counter() - t0c) * 1e3)
         cold_ms = float(np.median(ts))
         log(f"[bench] cold pass: {cold_ms:.2f} ms per step (median of {len(ts)})")
+    learned_res = None
+    if not side_case:
+        try:
+            learned_res = learned_leg(ctx, nb, NC)
+        except Exception as e:  # informative only
+            log(f"[bench] learned leg failed: {e}")
     value = nb / (ms_step / 1e3)
     nnzb = ctx.nnzb()
     nc, ncol = NC + 1, NC + 2
@@ -416,6 +473,7 @@ def main():
                    "flash_flops_per_cell": WARM_FLOPS_PER_CELL, "asm_bytes_per_launch": asm_bytes,
                    "c_bar": grid.nconn / nb, "dsec_rows_mean": rows_mean, "roofline_other": other,
                    "cold_ms_per_step": cold_ms,
+                   "learned": learned_res,
                    "cold_cell_updates_per_s": (nb / (cold_ms / 1e3)) if cold_ms else None,
                    "roofline_flash": flash_roof,
                    "roofline_step": step_roof,

@@ -430,6 +430,14 @@ int ocpg_max_change(ocpg_ctx* ctx, ocpg_max_change_t* out);
  * tries the guessed K in the stability test; the split starts from it.  The
  * FP64 stability test and split still decide every cell.                   */
 int ocpg_flash_fim_guided(ocpg_ctx* ctx, const float* guess, int ld, ocpg_flash_stats* stats);
+/* The learned form in the product path: the network (host FP32 weights,
+ * nin = NC + 2 inputs, hidden width 64, nout = NC + 1 outputs) is copied to
+ * the context; ocpg_flash_fim_learned runs it on every cell's P, T and
+ * moles in one fused kernel and then ocpg_flash_fim_guided with its output.
+ * All-NULL weights remove the network.  No reference counterpart.         */
+int ocpg_set_learned(ocpg_ctx* ctx, int nin, int nout, const float* W1, const float* b1, const float* W2,
+                     const float* b2, const float* W3, const float* b3, const float* W4, const float* b4);
+int ocpg_flash_fim_learned(ocpg_ctx* ctx, ocpg_flash_stats* stats);
 /* Building blocks of the network's training and inference on MI355X (BF16
  * MFMA products, FP32 accumulation): C = A B (+ C) with strided FP32 A, B
  * (rounded to BF16), the K loop split in splitK parts summed in order
@@ -451,6 +459,14 @@ int ocpg_mlp_gemm(void* stream, int M, int N, int K, const float* A, int64_t sAm
 int ocpg_mlp_gemm_noalloc(void* stream, int M, int N, int K, const float* A, int64_t sAm, int64_t sAk,
                           const float* B, int64_t sBk, int64_t sBn, float* C, int64_t sCm, int64_t sCn,
                           int accumulate, int splitK);
+/* The network's whole inference (4 layers of hidden width 64, bias, GELU)
+ * in one kernel: Y = MLP(X), X (B x nin, ldX), Y (B x nout, ldY), W_l
+ * (in x out row-major FP32, rounded to BF16 in LDS), biases FP32; nin <= 32,
+ * nout <= 16.  The same roundings as ocpg_mlp_gemm + ocpg_mlp_bias_act layer
+ * by layer.  No reference counterpart (the learned form's own).           */
+int ocpg_mlp_forward(void* stream, int B, int nin, int nout, const float* X, int ldX, const float* W1,
+                     const float* b1, const float* W2, const float* b2, const float* W3, const float* b3,
+                     const float* W4, const float* b4, float* Y, int ldY);
 int ocpg_mlp_bias_act(void* stream, int M, int N, float* Z, const float* b, float* H);
 int ocpg_mlp_act_backward(void* stream, int64_t n, const float* Z, float* dH);
 int ocpg_mlp_colsum(void* stream, int M, int N, const float* X, float* out);

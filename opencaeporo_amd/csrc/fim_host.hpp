@@ -12,7 +12,7 @@ namespace ocpg {
 // Timer slots of ocpg_timing_get (names in ocpgpu_capi.cpp).
 enum TimerSlot {
     T_FLASH = 0, T_KRPC, T_RES, T_ASM, T_INIT, T_RESTORE, T_STAB, T_SPLIT, T_FINISH, T_TAIL, T_FIXUP, T_UPDASM,
-    T_TAILNR, T_PACK, kTimerSlots
+    T_TAILNR, T_PACK, T_MLP, kTimerSlots
 };
 
 struct WellsHost {

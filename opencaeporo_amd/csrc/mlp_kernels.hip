@@ -6,37 +6,50 @@
 //
 //   mlpGemmKernel   : C = A * B (+ C) with A, B read in FP32 through
 //                     arbitrary strides, rounded to BF16 and multiplied on
-//                     v_mfma_f32_16x16x16_bf16 (FP32 accumulators); one
-//                     wavefront per 16 x 16 tile of C, the K loop optionally
-//                     split across gridDim.z with FP32 partial tiles summed in
-//                     a fixed order (deterministic).
+//                     gfx950's v_mfma_f32_16x16x32_bf16 (FP32 accumulators);
+//                     one wavefront per 16 x 16 tile of C, the K loop
+//                     optionally split across gridDim.z with FP32 partial
+//                     tiles summed in a fixed order (deterministic).
+//   mlpForwardKernel: the whole inference (4 layers, bias, GELU) in one
+//                     kernel: the weights as BF16 MFMA operand fragments in
+//                     LDS, then in registers; activations stay in registers
+//                     between layers (the products are taken transposed, so
+//                     one layer's accumulator layout is the next one's
+//                     operand layout up to a fixed permutation of K).
 //   bias/GELU, GELU', column sums, loss and Adam: elementwise / reductions.
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <mutex>
 
 #include "../../include/ocpgpu.h"
+#include "mlp_kernels.hpp"
 
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-// FP32 -> BF16 bits, round to nearest even (NaN kept quiet).
-__device__ __forceinline__ short toBf16(float f)
+// FP32 -> BF16, round to nearest even (NaN kept quiet).
+__device__ __forceinline__ __bf16 toBf16(float f)
 {
     uint32_t u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (short)((u >> 16) | 0x40);
-    const uint32_t r = u + 0x7fffu + ((u >> 16) & 1u);
-    return (short)(r >> 16);
+    uint16_t h;
+    if ((u & 0x7fffffffu) > 0x7f800000u) {
+        h = (uint16_t)((u >> 16) | 0x40);
+    } else {
+        const uint32_t r = u + 0x7fffu + ((u >> 16) & 1u);
+        h = (uint16_t)(r >> 16);
+    }
+    return __builtin_bit_cast(__bf16, h);
 }
 
-// One 16 x 16 tile of C per wavefront.  MFMA 16x16x16 operand layout
-// (lane l, j = 0..3): A[i = l % 16][k = 4 (l / 16) + j], B[k = 4 (l / 16) + j]
-// [n = l % 16]; result: C[m = 4 (l / 16) + j][n = l % 16].
+// One 16 x 16 tile of C per wavefront.  MFMA 16x16x32 operand layout
+// (lane l, e = 0..7): A[i = l % 16][k = 8 (l / 16) + e], B[k = 8 (l / 16) + e]
+// [n = l % 16]; result (j = 0..3): C[m = 4 (l / 16) + j][n = l % 16].
 __global__ void __launch_bounds__(64) mlpGemmKernel(int M, int N, int K, const float* __restrict__ A, int64_t sAm,
                                                     int64_t sAk, const float* __restrict__ B, int64_t sBk,
                                                     int64_t sBn, float* __restrict__ C, int64_t sCm, int64_t sCn,
@@ -47,20 +60,21 @@ __global__ void __launch_bounds__(64) mlpGemmKernel(int M, int N, int K, const f
     const int kb = blockIdx.z * kChunk;
     const int ke = min(K, kb + kChunk);
     const int row = lane & 15;
+    const int k8 = (lane >> 4) * 8;
     const int kq = (lane >> 4) * 4;
     floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k0 = kb; k0 < ke; k0 += 16) {
-        bf16x4 a, b;
+    for (int k0 = kb; k0 < ke; k0 += 32) {
+        bf16x8 a, b;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int k = k0 + kq + j;
+        for (int e = 0; e < 8; ++e) {
+            const int k = k0 + k8 + e;
             const int m = m0 + row, n = n0 + row;
             const float av = (m < M && k < ke) ? A[(int64_t)m * sAm + (int64_t)k * sAk] : 0.f;
             const float bv = (n < N && k < ke) ? B[(int64_t)k * sBk + (int64_t)n * sBn] : 0.f;
-            a[j] = toBf16(av);
-            b[j] = toBf16(bv);
+            a[e] = toBf16(av);
+            b[e] = toBf16(bv);
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
     }
     const int n = n0 + row;
 #pragma unroll
@@ -171,10 +185,187 @@ __global__ void mlpAdamKernel(int64_t n, float* __restrict__ w, const float* __r
     w[t] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
+// ---------------------------------------------------------------------------
+// Fused inference.  Per 16-row batch tile a wavefront computes
+// Z_l^T = W_l^T H_{l-1}^T (M = the layer's outputs, N = 16 batch rows), so
+// its accumulators hold (lane l, tile t, j = 0..3) Z[16 t + 4 (l / 16) + j]
+// [row l % 16].  The next product's B operand needs, per K step s and lane
+// l, 8 consecutive K positions 8 (l / 16) + e: they are taken as the
+// hidden features f(g, e, s) = 16 (2 s + e / 4) + 4 g + e % 4 (g = l / 16),
+// i.e. the lane's own accumulators of tiles 2 s and 2 s + 1, and the weight
+// fragments (the A operand) are laid out with the same permutation.  The
+// first layer reads X in the plain layout, K = the inputs padded to 32.
+constexpr int kHid = 64;
+constexpr int kFwdWaves = 4;
+// A-operand fragments (64 lanes x 8 BF16) per layer: layer 1: 4 tiles x 1
+// step, layers 2 and 3: 4 x 2, layer 4: 1 x 2.
+constexpr int kFrag1 = 4, kFrag2 = 8, kFrag3 = 8, kFrag4 = 2;
+constexpr int kFrags = kFrag1 + kFrag2 + kFrag3 + kFrag4;
+
+__device__ __forceinline__ int hidOfK(int g, int e, int s) { return 16 * (2 * s + e / 4) + 4 * g + (e & 3); }
+
+struct MlpWeights {
+    const float *W1, *b1, *W2, *b2, *W3, *b3, *W4, *b4;  // W_l: (in x out) row-major
+};
+
+// The network's input: rows of X, or (X NULL) the features of cells of a
+// SoA cell state: P / 5000, T / 600 and the hydrocarbon mole fractions
+// Ni_i / sum_j |Ni_j| (j < NC), the flash's own feed.
+struct MlpInput {
+    const float* X;
+    int ldX;
+    const double *P, *T, *Ni;
+    int64_t ld;
+    int NC;
+};
+
+__device__ __forceinline__ float mlpFeature(const MlpInput& in, int64_t row, int k, int nin)
+{
+    if (k >= nin) return 0.f;
+    if (in.X) return in.X[row * in.ldX + k];
+    if (k == 0) return (float)(in.P[row] / 5000.0);
+    if (k == 1) return (float)(in.T[row] / 600.0);
+    double Nh = 0.0;
+    for (int i = 0; i < in.NC; ++i) Nh += fabs(in.Ni[i * in.ld + row]);
+    return (float)(in.Ni[(k - 2) * in.ld + row] / Nh);
+}
+
+__global__ void __launch_bounds__(64 * kFwdWaves) mlpForwardKernel(int B, int nin, int nout, MlpInput in,
+                                                                   MlpWeights w, float* __restrict__ Y, int ldY)
+{
+    __shared__ bf16x8 frag[kFrags][64];
+    __shared__ float bias[3 * kHid + 16];
+    // stage the fragments: fragment q, lane l, element e
+    for (int t = threadIdx.x; t < kFrags * 64; t += blockDim.x) {
+        const int q = t / 64, l = t % 64, g = l / 16, i = l % 16;
+        bf16x8 v;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            float x = 0.f;
+            if (q < kFrag1) {                                   // layer 1, tile q
+                const int k = 8 * g + e, o = 16 * q + i;
+                if (k < nin) x = w.W1[k * kHid + o];
+            } else if (q < kFrag1 + kFrag2 + kFrag3) {          // layers 2, 3: tile, step
+                const int r = q < kFrag1 + kFrag2 ? q - kFrag1 : q - kFrag1 - kFrag2;
+                const float* W = q < kFrag1 + kFrag2 ? w.W2 : w.W3;
+                const int tile = r / 2, s = r % 2, o = 16 * tile + i;
+                x = W[hidOfK(g, e, s) * kHid + o];
+            } else {                                            // layer 4: step
+                const int s = q - (kFrag1 + kFrag2 + kFrag3);
+                if (i < nout) x = w.W4[hidOfK(g, e, s) * nout + i];
+            }
+            v[e] = toBf16(x);
+        }
+        frag[q][l] = v;
+    }
+    for (int t = threadIdx.x; t < 3 * kHid + 16; t += blockDim.x) {
+        float x = 0.f;
+        if (t < kHid) x = w.b1[t];
+        else if (t < 2 * kHid) x = w.b2[t - kHid];
+        else if (t < 3 * kHid) x = w.b3[t - 2 * kHid];
+        else if (t - 3 * kHid < nout) x = w.b4[t - 3 * kHid];
+        bias[t] = x;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x % 64, g = lane / 16, r = lane % 16;
+    bf16x8 A[kFrags];
+#pragma unroll
+    for (int q = 0; q < kFrags; ++q) A[q] = frag[q][lane];
+    float bl[3][4][4], b4[4];
+#pragma unroll
+    for (int L = 0; L < 3; ++L)
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bl[L][t][j] = bias[L * kHid + 16 * t + 4 * g + j];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b4[j] = bias[3 * kHid + 4 * g + j];
+    const int tiles = (B + 15) / 16;
+    const int wave = blockIdx.x * kFwdWaves + threadIdx.x / 64, waves = gridDim.x * kFwdWaves;
+    for (int tile = wave; tile < tiles; tile += waves) {
+        const int row = tile * 16 + r;
+        const bool ok = row < B;
+        // layer 1: B operand = X^T, K = the inputs
+        bf16x8 xb;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) xb[e] = toBf16(ok ? mlpFeature(in, row, 8 * g + e, nin) : 0.f);
+        floatx4 z[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            z[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+            z[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[t], xb, z[t], 0, 0, 0);
+        }
+        // layers 2 and 3
+#pragma unroll
+        for (int L = 0; L < 2; ++L) {
+            bf16x8 hb[2];
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int t = 2 * s + e / 4, j = e & 3;
+                    hb[s][e] = toBf16(gelu(z[t][j] + bl[L][t][j]));
+                }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                z[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int s = 0; s < 2; ++s)
+                    z[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[kFrag1 + L * kFrag2 + 2 * t + s], hb[s], z[t],
+                                                                   0, 0, 0);
+            }
+        }
+        // layer 4
+        bf16x8 hb[2];
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int t = 2 * s + e / 4, j = e & 3;
+                hb[s][e] = toBf16(gelu(z[t][j] + bl[2][t][j]));
+            }
+        floatx4 y = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+            y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[kFrag1 + kFrag2 + kFrag3 + s], hb[s], y, 0, 0, 0);
+        if (ok) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int o = 4 * g + j;
+                if (o < nout) Y[(int64_t)row * ldY + o] = y[j] + b4[j];
+            }
+        }
+    }
+}
+
 inline int status(hipError_t e) { return e == hipSuccess ? OCPG_OK : OCPG_EHIP; }
 inline unsigned blocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace
+
+namespace ocpg {
+
+static hipError_t launchMlpForward(hipStream_t s, int B, int nin, int nout, const MlpInput& in,
+                                   const float* const wb[8], float* Y, int ldY)
+{
+    const int tiles = (B + 15) / 16;
+    // enough workgroups to fill the chip a few times over; each wavefront
+    // loops over batch tiles, so the fragment staging is amortised
+    const int wg = std::min((tiles + kFwdWaves - 1) / kFwdWaves, 2048);
+    MlpWeights w{wb[0], wb[1], wb[2], wb[3], wb[4], wb[5], wb[6], wb[7]};
+    hipLaunchKernelGGL(mlpForwardKernel, dim3(wg), dim3(64 * kFwdWaves), 0, s, B, nin, nout, in, w, Y, ldY);
+    return hipGetLastError();
+}
+
+hipError_t launchMlpForwardCells(hipStream_t s, int64_t nb, int64_t ld, int NC, const double* P, const double* T,
+                                 const double* Ni, const float* const wb[8], float* Y, int ldY)
+{
+    if (nb <= 0) return hipSuccess;
+    MlpInput in{nullptr, 0, P, T, Ni, ld, NC};
+    return launchMlpForward(s, (int)nb, NC + 2, NC + 1, in, wb, Y, ldY);
+}
+
+}  // namespace ocpg
 
 // Split-K partial tiles of ocpg_mlp_gemm_noalloc, one copy per GPU (each
 // device holds its own instance of this array from the first launch of the
@@ -209,7 +400,7 @@ int scratchOfCurrentDevice(float** out)
 // Grid of mlpGemmKernel: the K chunk per z-slice and the number of slices.
 void gemmSplit(int K, int splitK, int* kChunk, int* Z)
 {
-    *kChunk = ((K + splitK - 1) / splitK + 15) / 16 * 16;
+    *kChunk = ((K + splitK - 1) / splitK + 31) / 32 * 32;
     const int z = *kChunk > 0 ? (K + *kChunk - 1) / *kChunk : 0;
     *Z = z > 0 ? z : 1;
 }
@@ -253,6 +444,22 @@ int ocpg_mlp_gemm_noalloc(void* stream, int M, int N, int K, const float* A, int
         if (rc != OCPG_OK) return rc;
     }
     return ocpg_mlp_gemm(stream, M, N, K, A, sAm, sAk, B, sBk, sBn, C, sCm, sCn, accumulate, splitK, part);
+}
+
+// The network's inference in one kernel (see mlpForwardKernel): hidden
+// width 64, nin <= 32 inputs, nout <= 16 outputs; the same BF16 roundings as
+// the layer-by-layer path (operands rounded to BF16, FP32 accumulation, bias
+// and GELU in FP32).
+int ocpg_mlp_forward(void* stream, int B, int nin, int nout, const float* X, int ldX, const float* W1,
+                     const float* b1, const float* W2, const float* b2, const float* W3, const float* b3,
+                     const float* W4, const float* b4, float* Y, int ldY)
+{
+    if (B <= 0 || nin < 1 || nin > 32 || nout < 1 || nout > 16 || ldX < nin || ldY < nout || !X || !Y || !W1 ||
+        !b1 || !W2 || !b2 || !W3 || !b3 || !W4 || !b4)
+        return OCPG_EINVAL;
+    MlpInput in{X, ldX, nullptr, nullptr, nullptr, 0, 0};
+    const float* wb[8] = {W1, b1, W2, b2, W3, b3, W4, b4};
+    return status(ocpg::launchMlpForward(static_cast<hipStream_t>(stream), B, nin, nout, in, wb, Y, ldY));
 }
 
 int ocpg_mlp_bias_act(void* stream, int M, int N, float* Z, const float* b, float* H)

@@ -16,6 +16,7 @@
 #include "fim_host.hpp"
 #include "fim_kernels.hpp"
 #include "flash_kernel.hpp"
+#include "mlp_kernels.hpp"
 
 using namespace ocpg;
 
@@ -71,7 +72,7 @@ size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 const char* kTimerNames[kTimerSlots] = {"flash",        "krpc_rock",       "calres",      "assemble",
                                         "init_flash",   "restore",         "flash_stability", "flash_split",
                                         "flash_finish", "flash_tail",      "fixup",       "update_assemble",
-                                        "flash_restarts", "pack"};
+                                        "flash_restarts", "pack",          "mlp"};
 
 // Fields the flash reads (and overwrites): what one Newton iteration starts from.
 bool isFlashInput(size_t hostOff)
@@ -123,6 +124,12 @@ struct ocpg_ctx {
     hipEvent_t copyEv = nullptr, mainEv = nullptr;
     // replays of the captured iteration launched / known to have finished
     uint64_t launches = 0, launchesDone = 0;
+    // learned form (ocpg_set_learned): the network's weights on the device
+    // and its per-cell output
+    float* mlpMem = nullptr;
+    const float* mlpW[8] = {};
+    float* guessBuf = nullptr;
+    int64_t guessRows = 0;
     // overlapped Newton iteration (ocpg_update_assemble): side streams of the
     // flash and the rows recomputed after they join
     FlashSide side{};
@@ -453,6 +460,8 @@ int ocpg_destroy(ocpg_ctx* c)
     if (c->snap) (void)hipFree(c->snap);
     if (c->mask) (void)hipFree(c->mask);
     if (c->poolLNi) (void)hipFree(c->poolLNi);
+    if (c->mlpMem) (void)hipFree(c->mlpMem);
+    if (c->guessBuf) (void)hipFree(c->guessBuf);
     if (c->copyStream) {
         (void)hipStreamSynchronize(c->copyStream);
         (void)hipStreamDestroy(c->copyStream);
@@ -675,6 +684,69 @@ int ocpg_flash_fim_guided(ocpg_ctx* c, const float* guess, int ld, ocpg_flash_st
     c->work.guess = nullptr;
     c->work.guessLd = 0;
     return rc;
+}
+
+// The learned form's network for this context (no reference counterpart):
+// host FP32 arrays W1 (nin x 64), b1 (64), W2, W3 (64 x 64), b2, b3 (64),
+// W4 (64 x nout), b4 (nout), with nin = NC + 2 and nout = NC + 1; copied to
+// the device.  All NULL removes it.
+int ocpg_set_learned(ocpg_ctx* c, int nin, int nout, const float* W1, const float* b1, const float* W2,
+                     const float* b2, const float* W3, const float* b3, const float* W4, const float* b4)
+{
+    if (!c) return OCPG_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    const float* src[8] = {W1, b1, W2, b2, W3, b3, W4, b4};
+    bool none = true, all = true;
+    for (const float* p : src) {
+        none = none && !p;
+        all = all && p;
+    }
+    if (!none && !all) return OCPG_EINVAL;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->mlpMem) HIPCHK(hipFree(c->mlpMem));
+    c->mlpMem = nullptr;
+    for (auto& p : c->mlpW) p = nullptr;
+    if (none) return OCPG_OK;
+    if (nin != c->NC + 2 || nout != c->NC + 1) return OCPG_EINVAL;
+    constexpr int H = 64;
+    const size_t n[8] = {(size_t)nin * H, H, (size_t)H * H, H, (size_t)H * H, H, (size_t)H * nout, (size_t)nout};
+    size_t off[8], tot = 0;
+    for (int k = 0; k < 8; k++) {
+        off[k] = tot;
+        tot += (n[k] * sizeof(float) + 255) & ~size_t(255);
+    }
+    char* m = nullptr;
+    HIPCHK(hipMalloc(&m, tot));
+    c->mlpMem = reinterpret_cast<float*>(m);
+    for (int k = 0; k < 8; k++) {
+        HIPCHK(hipMemcpy(m + off[k], src[k], n[k] * sizeof(float), hipMemcpyHostToDevice));
+        c->mlpW[k] = reinterpret_cast<const float*>(m + off[k]);
+    }
+    return OCPG_OK;
+}
+
+// FlashFIM with the learned initial guess computed on the device from the
+// cells' own P, T and moles (ocpg_set_learned's network in one fused
+// kernel), then ocpg_flash_fim_guided with it.  Asynchronous unless stats
+// is non-NULL.
+int ocpg_flash_fim_learned(ocpg_ctx* c, ocpg_flash_stats* stats)
+{
+    if (!c || !c->cur || !c->mlpMem) return OCPG_ESTATE;
+    HIPCHK(hipSetDevice(c->device));
+    const int64_t nb = c->B.nb;
+    const int ldG = c->NC + 1;
+    if (c->guessRows != nb) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (c->guessBuf) HIPCHK(hipFree(c->guessBuf));
+        c->guessBuf = nullptr;
+        c->guessRows = 0;
+        HIPCHK(hipMalloc(&c->guessBuf, (size_t)std::max<int64_t>(nb, 1) * ldG * sizeof(float)));
+        c->guessRows = nb;
+    }
+    timerBegin(c, T_MLP);
+    HIPCHK(launchMlpForwardCells(c->stream, nb, c->B.ld, c->NC, c->B.P, c->B.T, c->B.Ni, c->mlpW, c->guessBuf, ldG));
+    timerEnd(c, T_MLP);
+    return ocpg_flash_fim_guided(c, c->guessBuf, ldG, stats);
 }
 
 // IsoT_FIM::UpdateProperty (CalFlash, CalKrPc, CalRock, CalRes;

@@ -47,11 +47,15 @@ def load():
         "ocpg_bsr_rows": ([P], C.c_int64),
         "ocpg_capture_iteration": ([P, C.c_double], C.c_int),
         "ocpg_flash_fim_guided": ([P, P, C.c_int, P], C.c_int),
+        "ocpg_set_learned": ([P, C.c_int, C.c_int, P, P, P, P, P, P, P, P], C.c_int),
+        "ocpg_flash_fim_learned": ([P, P], C.c_int),
         "ocpg_mlp_gemm": ([P, C.c_int, C.c_int, C.c_int, P, C.c_int64, C.c_int64, P, C.c_int64, C.c_int64,
                            P, C.c_int64, C.c_int64, C.c_int, C.c_int, P], C.c_int),
         "ocpg_mlp_gemm_noalloc": ([P, C.c_int, C.c_int, C.c_int, P, C.c_int64, C.c_int64, P, C.c_int64,
                                    C.c_int64, P, C.c_int64, C.c_int64, C.c_int, C.c_int], C.c_int),
         "ocpg_mlp_bias_act": ([P, C.c_int, C.c_int, P, P, P], C.c_int),
+        "ocpg_mlp_forward": ([P, C.c_int, C.c_int, C.c_int, P, C.c_int, P, P, P, P, P, P, P, P, P, C.c_int],
+                             C.c_int),
         "ocpg_mlp_act_backward": ([P, C.c_int64, P, P], C.c_int),
         "ocpg_mlp_colsum": ([P, C.c_int, C.c_int, P, P], C.c_int),
         "ocpg_mlp_loss": ([P, C.c_int, C.c_int, C.c_int, P, P, P, P, C.c_float, P, P], C.c_int),
@@ -245,6 +249,25 @@ class Context:
         st = abi.FlashStats()
         check(self.lib.ocpg_flash_fim_guided(self.h, C.c_void_p(guess.data_ptr()), ld, C.byref(st)),
               "ocpg_flash_fim_guided")
+        return st
+
+    def set_learned(self, W=None, b=None):
+        """The learned form's network (W_l (in x out), b_l float32 arrays, 4
+        layers); None removes it."""
+        if W is None:
+            check(self.lib.ocpg_set_learned(self.h, 0, 0, *([None] * 8)), "ocpg_set_learned")
+            self._learned = None
+            return
+        arrs = []
+        for l in range(4):
+            arrs += [np.ascontiguousarray(W[l], dtype=np.float32), np.ascontiguousarray(b[l], dtype=np.float32)]
+        self._learned = arrs
+        check(self.lib.ocpg_set_learned(self.h, arrs[0].shape[0], arrs[6].shape[1],
+                                        *[a.ctypes.data for a in arrs]), "ocpg_set_learned")
+
+    def flash_fim_learned(self, stats=True):
+        st = abi.FlashStats() if stats else None
+        check(self.lib.ocpg_flash_fim_learned(self.h, C.byref(st) if st else None), "ocpg_flash_fim_learned")
         return st
 
     def capture_iteration(self, dt):

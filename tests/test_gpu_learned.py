@@ -69,9 +69,36 @@ def test_training_and_guided_flash():
     assert hist[-1][1] < 0.5 * hist[0][1]
     ev = learned.evaluate(eos, net, nb=20_000, seed=77)
     gpu_util.say("  ", ev)
-    a = ev["agreement"]
-    assert a["phase_count_same"] >= 0.999
-    assert a["p99_abs_S_diff"] < 1e-10
-    it_p, it_l = ev["plain"]["iters"], ev["learned"]["iters"]
+    for kind in ("cold", "warm"):
+        a = ev[kind]["agreement"]
+        assert a["phase_count_same"] >= 0.999, kind
+        assert a["p99_abs_S_diff"] < 1e-10, kind
+    it_p, it_l = ev["cold"]["plain"]["iters"], ev["cold"]["learned"]["iters"]
     # the guessed K proves instability in fewer successive substitutions
     assert it_l["ssm_sta"] < it_p["ssm_sta"]
+
+
+def test_fused_inference_matches_layer_path_and_cpu():
+    """ocpg_mlp_forward (one kernel, weights as BF16 fragments in LDS,
+    activations in registers) against the layer-by-layer GEMM path and the
+    CPU reference with the same BF16 roundings: both differ from it only by
+    FP32 accumulation order (and the rare BF16 rounding it flips)."""
+    gpu_util.say("[learned] fused inference vs layer path")
+    t = torch()
+    rng = np.random.default_rng(9)
+    for NC, B in ((6, 1000), (10, 333), (6, 17)):
+        net = learned.MLP(NC, seed=NC)
+        for l in range(4):
+            net.b[l].copy_(t.tensor(rng.normal(0, 0.1, net.b[l].shape[0]).astype(np.float32), device="cuda:0"))
+        X = rng.uniform(0, 1, (B, NC + 2)).astype(np.float32)
+        Xd = t.tensor(X, device="cuda:0")
+        fused = net.infer(Xd).cpu().numpy()
+        layered = net.forward(Xd).cpu().numpy()
+        W = [w.cpu().numpy() for w in net.W]
+        b = [x.cpu().numpy() for x in net.b]
+        ref = mlp_ref.forward(W, b, X)[0]
+        scale = np.abs(ref).max()
+        for got in (fused, layered):
+            err = np.abs(got - ref) / scale
+            assert err.max() < 2e-2 and np.quantile(err, 0.99) < 2e-3, (NC, B, err.max())
+        assert np.abs(fused - layered).max() / scale < 2e-2
