@@ -102,3 +102,22 @@ def test_fused_inference_matches_layer_path_and_cpu():
             err = np.abs(got - ref) / scale
             assert err.max() < 2e-2 and np.quantile(err, 0.99) < 2e-3, (NC, B, err.max())
         assert np.abs(fused - layered).max() / scale < 2e-2
+
+
+def test_committed_network_on_the_full_field():
+    """The committed network (trained on 10M GPU flash results) on the
+    1.122M-cell 6-component field, cold: converged results unchanged but for
+    a handful of near-boundary cells where a trial from the guess finds an
+    instability Wilson's four trials miss, fewer successive substitutions in
+    the stability test."""
+    gpu_util.say("[learned] committed network on the 1.122M-cell field")
+    eos = lib.eos_from_deck(fluids.bench6_deck())
+    net = learned.MLP.load(learned.weights_path(6))
+    ev = learned.evaluate(eos, net, nb=1_122_000, warm=False)
+    c = ev["cold"]
+    gpu_util.say("  ", c["agreement"], "gain", c["gain"], "iters", c["plain"]["iters"], c["learned"]["iters"])
+    a = c["agreement"]
+    assert a["phase_count_differs"] <= 30
+    assert a["p99_abs_S_diff"] < 1e-10 and a["max_abs_S_diff"] < 1e-8
+    assert c["learned"]["iters"]["ssm_sta"] < 0.8 * c["plain"]["iters"]["ssm_sta"]
+    assert c["learned"]["nonconverged"] <= c["plain"]["nonconverged"]
