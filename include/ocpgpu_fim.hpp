@@ -95,6 +95,37 @@ public:
     }
     /* New BHPs (IsoT_FIM::GetSolution's well update) or controls. */
     void UpdateWellState(const ocpg_wells& wells) { check(ocpg_set_well_state(ctx_, &wells), "ocpg_set_well_state"); }
+    /* The wells' setup data (Well::Setup: depth, injected phase and
+     * temperature, initial mode), once after Prepare(wells); from then on
+     * AllWells::PrepareWell can run on the device. */
+    void SetWellSetup(const ocpg_well_setup& su) { check(ocpg_set_well_setup(ctx_, &su), "ocpg_set_well_setup"); }
+    /* AllWells::PrepareWell on the device (CalTrans, CaldG, the injectors'
+     * molar density, CheckOptMode); IsoT_FIM::Prepare then calls CalRes. */
+    void PrepareWells() { check(ocpg_prepare_wells(ctx_), "ocpg_prepare_wells"); }
+    /* AllWells::CheckP at the end of a converged Newton iteration
+     * (IsoT_FIM::FinishNR): OCPG_WELL_SUCCESS, _NEGATIVE_PRESSURE or _SWITCH. */
+    int CheckWells()
+    {
+        int32_t st = 0;
+        check(ocpg_check_wells(ctx_, &st), "ocpg_check_wells");
+        return st;
+    }
+    /* Bulk::CheckNi and Bulk::CheckP (IsoT_FIM::UpdateProperty): 0 or
+     * OCPG_BULK_NEGATIVE_COMPONENTS_MOLES / OCPG_BULK_NEGATIVE_PRESSURE. */
+    int CheckBulk()
+    {
+        int32_t st = 0;
+        check(ocpg_check_bulk(ctx_, &st), "ocpg_check_bulk");
+        return st;
+    }
+    /* The step's changes OCPControl::CalNextTimeStep reads (Bulk::CalMaxChange
+     * and the wells' BHP change). */
+    ocpg_max_change_t MaxChange()
+    {
+        ocpg_max_change_t r{};
+        check(ocpg_max_change(ctx_, &r), "ocpg_max_change");
+        return r;
+    }
 
     /* IsoT_FIM::UpdateProperty after the caller's CheckP/CheckNi: CalFlash,
      * CalKrPc, CalRock, the wells' CalTrans/CalFlux and CalRes on the
