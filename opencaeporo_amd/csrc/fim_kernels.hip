@@ -358,14 +358,31 @@ __global__ void __launch_bounds__(kResBlock) resNormKernel(DevBulk B, FimDevice 
 // neighbour's row at the slot the reference would have filled.  Every block
 // is computed once, nothing is accumulated across waves, no atomics.
 //
-// Memory plan: the cell values a connection side needs (its upwind phase
-// properties, pressures, depth, phase flags) form a record of AsmRec::LEN
-// doubles.  The wavefront gathers a record with one load per lane (each lane
-// owns a fixed record entry), keeps the own cell's record and the current
-// neighbour's in LDS, and prefetches the next neighbour's record into
-// registers while it works on the current connection.  Rows are handed to
-// workgroups so that each XCD walks a contiguous range of cells, which keeps
-// the neighbour records (cells n +- 1, n +- nx, ...) in that XCD's L2.
+// Arithmetic.  Every entry of the own side's dFdXp / dFdXs is a sum over
+// the phases j of x_ij(upwind of j) times a coefficient that does not depend
+// on the component i (the flux derivatives are transIJ = x_ij xi_j transJ_j
+// times per-phase factors), plus, in the x columns, a term on the
+// diagonal kk == i.  So per connection the wavefront computes
+//   C[j][col]  the coefficient of column col (pressure, the own cell's
+//              saturations, its x_kj) for phase j          (lanes (j, col))
+//   G[j][cc] = sum_col C[j][col] dSec_own[col][cc] (+ the pressure part)
+//                                                          (lanes (j, cc))
+//   block[i][cc] = dt (sum_j x_ij G[j][cc] + sum_j d_j dSec_own[x(j, i)][cc])
+//                                                          (lanes (i, cc))
+// instead of the nc x (1 + ncol_own) derivative entries and their product
+// with dSec one by one; the sums are the reference's, reassociated (the
+// blocks agree with the reference's to rounding).  kr/mu, xi/mu, muP/mu and
+// mux/mu are formed once per cell by the record packing, so the assembly
+// divides nothing.
+//
+// Memory plan: a neighbour contributes only its pressures, depth, phase
+// flags and, where it is upwind, its xi, kr/mu and x_ij: a record of
+// AsmRec::LEN doubles, read with one load per lane and prefetched into
+// registers while the previous connection is computed.  The row's own cell
+// also contributes its derivatives (AsmOwn, read once per row).  Rows are
+// handed to workgroups so that each XCD walks a contiguous range of cells,
+// which keeps the neighbour records (cells n +- 1, n +- nx, ...) in that
+// XCD's L2.
 // ---------------------------------------------------------------------------
 constexpr int kAsmBlock = 64;
 constexpr int kXcds = 8;
@@ -377,31 +394,27 @@ struct AsmDims {
     static constexpr int bsize = ncol * ncol;
     static constexpr int maxRows = 3 + 2 * NC;          // compact secondary rows
     static constexpr int perLane = (bsize + kAsmBlock - 1) / kAsmBlock;
-    static constexpr int dfStride = maxRows + 1;        // [i][0] = dFdXp, [i][1+k] = dFdXs
-    // lane stride of the x_ij columns in the assembly's second pass: a power
-    // of two >= 2 NC when that takes no more passes over the wavefront
-    static constexpr int xPow2 = 2 * NC <= 8 ? 8 : 2 * NC <= 16 ? 16 : 2 * NC <= 32 ? 32 : 64;
-    static constexpr int xStride =
-        (nc * xPow2 + kAsmBlock - 1) / kAsmBlock == (nc * 2 * NC + kAsmBlock - 1) / kAsmBlock ? xPow2 : 2 * NC;
-    // the same for rows whose cell has one hydrocarbon phase (NC x columns)
-    static constexpr int xStride1 = NC <= 8 ? 8 : NC <= 16 ? 16 : 32;
+    // columns per phase of the coefficient table: the own cell's secondary
+    // columns, then the pressure coefficient at CS - 1
+    static constexpr int CS = maxRows + 1 <= 16 ? 16 : 32;
+    static constexpr int cPasses = (3 * CS + kAsmBlock - 1) / kAsmBlock;
+    static_assert(3 * ncol <= kAsmBlock, "G needs a lane per (phase, column)");
 };
 
-// Entry offsets of the per-cell record (phase j, component i).
+// The record a row reads for each neighbour (and for itself): phase j,
+// component i.
 template <int NC>
 struct AsmRec {
     static constexpr int nc = NC + 1;
-    static constexpr int PJ = 0, XI = 3, KR = 6, MU = 9, MUP = 12, XIP = 15, RHOP = 18;
-    static constexpr int DKR = 21;  // dKr_dS[j*3+k]
-    static constexpr int XIJ = 30;  // xij[j*nc+i]
-    static constexpr int RHOX = XIJ + 3 * nc, XIX = RHOX + 3 * nc, MUX = XIX + 3 * nc;
-    static constexpr int DEPTH = MUX + 3 * nc, EX = DEPTH + 1, LEN = EX + 3;
+    static constexpr int PJ = 0, XI = 3, LAM = 6;   // LAM: kr / mu
+    static constexpr int XIJ = 9;                   // xij[j*nc+i]
+    static constexpr int DEPTH = XIJ + 3 * nc, EX = DEPTH + 1, LEN = EX + 3;
     static constexpr int perLane = (LEN + kAsmBlock - 1) / kAsmBlock;
 };
 
 // The own-row data of a row's assembly: compact dSec_dPri rows, dPcj_dS,
-// the accumulation terms and the secondary-column bookkeeping (counts stored
-// as doubles, exactly).
+// the accumulation terms, the secondary-column bookkeeping (counts stored
+// as doubles, exactly), then the own cell's derivatives (DER block).
 template <int NC>
 struct AsmOwn {
     using D = AsmDims<NC>;
@@ -413,51 +426,41 @@ struct AsmOwn {
     static constexpr int SDEX = POROP + 1;            // pSderExist[3]
     static constexpr int PVN = SDEX + 3;              // pVnumCom[3]
     static constexpr int ROWS = PVN + 3;              // bRowSizedSdP
-    static constexpr int LEN = ROWS + 1;
+    static constexpr int DER = ROWS + 1;
+    static constexpr int XIP = DER, RHOP = XIP + 3, MUPM = RHOP + 3;  // MUPM: muP / mu
+    static constexpr int XIM = MUPM + 3;                              // xi / mu
+    static constexpr int DKR = XIM + 3;                               // dKr_dS[j*3+k]
+    static constexpr int RHOX = DKR + 9, XIX = RHOX + 3 * nc;         // [j*nc+k]
+    static constexpr int MUXM = XIX + 3 * nc;                         // mux / mu
+    static constexpr int LEN = MUXM + 3 * nc;
+    static constexpr int perLane = (LEN + kAsmBlock - 1) / kAsmBlock;
 };
 
 // Both records are packed per cell (AoS, rows padded to whole 128-byte
-// lines), so a wavefront reads a neighbour's record as one contiguous
-// ~1 KB span instead of one 8-byte value from each of ~120 arrays.
+// lines), so a wavefront reads a neighbour's record as one contiguous span
+// instead of one 8-byte value from each of ~40 arrays.
 template <int NC>
 constexpr int recStride() { return (AsmRec<NC>::LEN + 15) / 16 * 16; }
 template <int NC>
 constexpr int ownStride() { return (AsmOwn<NC>::LEN + 15) / 16 * 16; }
 
-// Where record entry q lives: a double field advanced to its entry, or (for
-// the phase flags) a byte field.
-struct RecSrc {
-    const double* d;
-    const uint8_t* b;
-};
+__device__ __forceinline__ double ratioOrZero(double a, double b) { return b != 0.0 ? a / b : 0.0; }
 
+// Record entry q of cell n, straight from the SoA state.
 template <int NC>
-__device__ __forceinline__ RecSrc recSource(const DevBulk& B, int q)
+__device__ __forceinline__ double recValue(const DevBulk& B, int q, int64_t n)
 {
     using R = AsmRec<NC>;
     const int64_t ld = B.ld;
-    RecSrc s{nullptr, nullptr};
-    if (q < R::XI) s.d = B.Pj + (int64_t)(q - R::PJ) * ld;
-    else if (q < R::KR) s.d = B.xi + (int64_t)(q - R::XI) * ld;
-    else if (q < R::MU) s.d = B.kr + (int64_t)(q - R::KR) * ld;
-    else if (q < R::MUP) s.d = B.mu + (int64_t)(q - R::MU) * ld;
-    else if (q < R::XIP) s.d = B.muP + (int64_t)(q - R::MUP) * ld;
-    else if (q < R::RHOP) s.d = B.xiP + (int64_t)(q - R::XIP) * ld;
-    else if (q < R::DKR) s.d = B.rhoP + (int64_t)(q - R::RHOP) * ld;
-    else if (q < R::XIJ) s.d = B.dKr_dS + (int64_t)(q - R::DKR) * ld;
-    else if (q < R::RHOX) s.d = B.xij + (int64_t)(q - R::XIJ) * ld;
-    else if (q < R::XIX) s.d = B.rhox + (int64_t)(q - R::RHOX) * ld;
-    else if (q < R::MUX) s.d = B.xix + (int64_t)(q - R::XIX) * ld;
-    else if (q < R::DEPTH) s.d = B.mux + (int64_t)(q - R::MUX) * ld;
-    else if (q < R::EX) s.d = B.depth;
-    else if (q < R::LEN) s.b = B.phaseExist + (int64_t)(q - R::EX) * ld;
-    return s;
-}
-
-__device__ __forceinline__ double loadRec(const RecSrc& s, int64_t cell)
-{
-    if (s.d) return s.d[cell];
-    if (s.b) return s.b[cell] ? 1.0 : 0.0;
+    if (q < R::XI) return B.Pj[(int64_t)(q - R::PJ) * ld + n];
+    if (q < R::LAM) return B.xi[(int64_t)(q - R::XI) * ld + n];
+    if (q < R::XIJ) {
+        const int64_t o = (int64_t)(q - R::LAM) * ld + n;
+        return ratioOrZero(B.kr[o], B.mu[o]);
+    }
+    if (q < R::DEPTH) return B.xij[(int64_t)(q - R::XIJ) * ld + n];
+    if (q == R::DEPTH) return B.depth[n];
+    if (q < R::LEN) return B.phaseExist[(int64_t)(q - R::EX) * ld + n] ? 1.0 : 0.0;
     return 0.0;
 }
 
@@ -467,6 +470,7 @@ template <int NC>
 __device__ __forceinline__ double ownValue(const DevBulk& B, int q, int64_t n)
 {
     using O = AsmOwn<NC>;
+    constexpr int nc = O::nc;
     const int64_t ld = B.ld;
     if (q < O::DPC) return B.dSec[(int64_t)q * ld + n];
     if (q < O::VFI) return B.dPcj_dS[(int64_t)(q - O::DPC) * ld + n];
@@ -476,12 +480,33 @@ __device__ __forceinline__ double ownValue(const DevBulk& B, int q, int64_t n)
     if (q == O::POROP) return B.poroP[n];
     if (q < O::PVN) return B.pSderExist[(int64_t)(q - O::SDEX) * ld + n] ? 1.0 : 0.0;
     if (q < O::ROWS) return (double)B.pVnumCom[(int64_t)(q - O::PVN) * ld + n];
-    return (double)B.bRowSizedSdP[n];
+    if (q == O::ROWS) return (double)B.bRowSizedSdP[n];
+    if (q < O::RHOP) return B.xiP[(int64_t)(q - O::XIP) * ld + n];
+    if (q < O::MUPM) return B.rhoP[(int64_t)(q - O::RHOP) * ld + n];
+    if (q < O::XIM) {
+        const int64_t o = (int64_t)(q - O::MUPM) * ld + n;
+        return ratioOrZero(B.muP[o], B.mu[o]);
+    }
+    if (q < O::DKR) {
+        const int64_t o = (int64_t)(q - O::XIM) * ld + n;
+        return ratioOrZero(B.xi[o], B.mu[o]);
+    }
+    if (q < O::RHOX) return B.dKr_dS[(int64_t)(q - O::DKR) * ld + n];
+    if (q < O::XIX) return B.rhox[(int64_t)(q - O::RHOX) * ld + n];
+    if (q < O::MUXM) return B.xix[(int64_t)(q - O::XIX) * ld + n];
+    if (q < O::LEN) {
+        const int jk = q - O::MUXM;
+        return ratioOrZero(B.mux[(int64_t)jk * ld + n], B.mu[(int64_t)(jk / nc) * ld + n]);
+    }
+    return 0.0;
 }
 
 // Pack both records of packTile<NC>() consecutive cells: read field by field
-// (each field's kPackTile values are contiguous), transpose in LDS,
-// write the cells' records as contiguous spans.
+// (each field's kPackTile values are contiguous), transpose in LDS, form the
+// ratios, write the cells' records as contiguous spans.  Three passes over
+// one LDS tile: the neighbour record (raw kr in the LAM slots, mu after the
+// record), the own record's first part, its derivative block (raw muP, xi,
+// mux in the ratio slots, mu after the block).
 // Cells per block: 32 for the 6-component records; 16 for larger ones
 // (their tile in LDS and the loads in flight per thread grow with NC).
 template <int NC>
@@ -492,21 +517,50 @@ constexpr int kPackBlock = 256;
 static_assert(64 % packTile<6>() == 0 && 64 % packTile<10>() == 0, "a tile must not cross the padding of ld");
 static_assert(7 * packTile<6>() <= kPackBlock && 7 * packTile<10>() <= kPackBlock, "flag/count pass exceeds the block");
 
+// Loads Q raw double fields (src[q] already advanced to the tile's first
+// cell) of the tile's cells into t[c * L + q]: all loads first, then the LDS
+// stores (generic loads may alias LDS: interleaved, each load would be
+// waited for before the next store).  Entries past the tile read the
+// padding up to ld and are never written back.
+template <int Q, int TILE, int L>
+__device__ __forceinline__ void packLoad(const double* const* src, double* t)
+{
+    constexpr int K = (Q * TILE + kPackBlock - 1) / kPackBlock;
+    double v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i = threadIdx.x + k * kPackBlock;
+        v[k] = i < Q * TILE ? src[i / TILE][i % TILE] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i = threadIdx.x + k * kPackBlock;
+        if (i < Q * TILE) t[(i % TILE) * L + i / TILE] = v[k];
+    }
+}
+
 template <int NC>
 __global__ void __launch_bounds__(kPackBlock) packRecordsKernel(DevBulk B, FimDevice F)
 {
     using R = AsmRec<NC>;
     using O = AsmOwn<NC>;
+    constexpr int nc = NC + 1;
     constexpr int ncol = AsmDims<NC>::ncol;
     constexpr int RS = recStride<NC>(), OS = ownStride<NC>();
     constexpr int kPackTile = packTile<NC>();
-    constexpr int L = (R::LEN > O::LEN ? R::LEN : O::LEN) + 1;  // odd LDS row: no bank conflicts
+    // raw fields per pass: the record's doubles + mu; the own part before
+    // the flags; the derivative block + mu
+    constexpr int QR = R::EX + 3, QO = O::SDEX, QD = O::LEN - O::DER + 3;
+    constexpr int QMAX = QR > QO ? (QR > QD ? QR : QD) : (QO > QD ? QO : QD);
+    constexpr int L = (QMAX > O::DER ? QMAX : O::DER) + 1;  // odd LDS row: no bank conflicts
     __shared__ double t[kPackTile * L];
     __shared__ int rows[kPackTile];
-    const int64_t n0 = (int64_t)blockIdx.x * kPackTile;
-    const int tile = (int)min<int64_t>(kPackTile, B.nb - n0);
     __shared__ int on[kPackTile];
     __shared__ int anyOn;
+    __shared__ const double* src[QMAX];
+    const int64_t n0 = (int64_t)blockIdx.x * kPackTile;
+    const int tile = (int)min<int64_t>(kPackTile, B.nb - n0);
+    const int64_t ld = B.ld;
     if (threadIdx.x == 0) anyOn = 0;
     __syncthreads();
     if (threadIdx.x < kPackTile) {
@@ -518,46 +572,30 @@ __global__ void __launch_bounds__(kPackBlock) packRecordsKernel(DevBulk B, FimDe
     if (threadIdx.x < kPackTile)
         rows[threadIdx.x] = threadIdx.x < tile ? min(max(B.bRowSizedSdP[n0 + threadIdx.x], 0),
                                                      AsmDims<NC>::maxRows) : 0;
-    // Each field's base for this block's cells, once per block: the loads
-    // below are then branch-free with compile-time trip counts, so a
-    // thread's loads are all in flight before the first is waited for.
-    // Entries past the tile (c >= tile) read the padding up to ld and are
-    // not written back.
-    __shared__ const double* srcR[R::EX];
-    __shared__ const double* srcO[O::SDEX];
-    for (int q = threadIdx.x; q < R::EX; q += kPackBlock) srcR[q] = recSource<NC>(B, q).d + n0;
-    for (int q = threadIdx.x; q < O::SDEX; q += kPackBlock) {
-        const int64_t ld = B.ld;
+    // ---- the neighbour record ----
+    for (int q = threadIdx.x; q < QR; q += kPackBlock) {
         const double* p;
-        if (q < O::DPC) p = B.dSec + (int64_t)q * ld;
-        else if (q < O::VFI) p = B.dPcj_dS + (int64_t)(q - O::DPC) * ld;
-        else if (q < O::VFP) p = B.vfi + (int64_t)(q - O::VFI) * ld;
-        else if (q == O::VFP) p = B.vfP;
-        else if (q == O::V) p = B.v;
-        else p = B.poroP;
-        srcO[q] = p + n0;
+        if (q < R::XI) p = B.Pj + (int64_t)(q - R::PJ) * ld;
+        else if (q < R::LAM) p = B.xi + (int64_t)(q - R::XI) * ld;
+        else if (q < R::XIJ) p = B.kr + (int64_t)(q - R::LAM) * ld;
+        else if (q < R::DEPTH) p = B.xij + (int64_t)(q - R::XIJ) * ld;
+        else if (q == R::DEPTH) p = B.depth;
+        else p = B.mu + (int64_t)(q - R::EX) * ld;  // raw mu at EX..EX+2, the flags go in after
+        src[q] = p + n0;
     }
     __syncthreads();
-    // neighbour record: the double fields, then the three phase flags
-    {
-        // all loads first, then all LDS stores (generic loads may alias LDS:
-        // interleaved, each load would be waited for before the next store)
-        constexpr int K = (R::EX * kPackTile + kPackBlock - 1) / kPackBlock;
-        double v[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int i = threadIdx.x + k * kPackBlock;
-            v[k] = i < R::EX * kPackTile ? srcR[i / kPackTile][i % kPackTile] : 0.0;
-        }
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int i = threadIdx.x + k * kPackBlock;
-            if (i < R::EX * kPackTile) t[(i % kPackTile) * L + i / kPackTile] = v[k];
-        }
+    packLoad<QR, kPackTile, L>(src, t);
+    __syncthreads();
+    // kr -> kr / mu, then the flags over the raw mu
+    if (threadIdx.x < 3 * kPackTile) {
+        const int j = threadIdx.x / kPackTile, c = threadIdx.x % kPackTile;
+        double* row = t + c * L;
+        row[R::LAM + j] = ratioOrZero(row[R::LAM + j], row[R::EX + j]);
     }
+    __syncthreads();
     if (threadIdx.x < 3 * kPackTile) {
         const int q = R::EX + threadIdx.x / kPackTile, c = threadIdx.x % kPackTile;
-        t[c * L + q] = B.phaseExist[(int64_t)(q - R::EX) * B.ld + n0 + c] ? 1.0 : 0.0;
+        t[c * L + q] = B.phaseExist[(int64_t)(q - R::EX) * ld + n0 + c] ? 1.0 : 0.0;
     }
     __syncthreads();
 #pragma unroll
@@ -567,33 +605,76 @@ __global__ void __launch_bounds__(kPackBlock) packRecordsKernel(DevBulk B, FimDe
         if (i < RS * kPackTile && c < tile && q < R::LEN && on[c]) F.rec[(n0 + c) * RS + q] = t[c * L + q];
     }
     __syncthreads();
-    // own-row record: the double fields (unused dSec rows are read and not
-    // written back), then the seven counts
-    {
-        constexpr int K = (O::SDEX * kPackTile + kPackBlock - 1) / kPackBlock;
-        double v[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int i = threadIdx.x + k * kPackBlock;
-            v[k] = i < O::SDEX * kPackTile ? srcO[i / kPackTile][i % kPackTile] : 0.0;
-        }
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int i = threadIdx.x + k * kPackBlock;
-            if (i < O::SDEX * kPackTile) t[(i % kPackTile) * L + i / kPackTile] = v[k];
-        }
+    // ---- the own record, first part: the double fields (unused dSec rows
+    // are read and not written back), then the seven counts ----
+    for (int q = threadIdx.x; q < QO; q += kPackBlock) {
+        const double* p;
+        if (q < O::DPC) p = B.dSec + (int64_t)q * ld;
+        else if (q < O::VFI) p = B.dPcj_dS + (int64_t)(q - O::DPC) * ld;
+        else if (q < O::VFP) p = B.vfi + (int64_t)(q - O::VFI) * ld;
+        else if (q == O::VFP) p = B.vfP;
+        else if (q == O::V) p = B.v;
+        else p = B.poroP;
+        src[q] = p + n0;
     }
-    if (threadIdx.x < (O::LEN - O::SDEX) * kPackTile) {
+    __syncthreads();
+    packLoad<QO, kPackTile, L>(src, t);
+    if (threadIdx.x < (O::DER - O::SDEX) * kPackTile) {
         const int q = O::SDEX + threadIdx.x / kPackTile, c = threadIdx.x % kPackTile;
         if (c < tile) t[c * L + q] = ownValue<NC>(B, q, n0 + c);
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < (OS * kPackTile + kPackBlock - 1) / kPackBlock; ++k) {
+    for (int k = 0; k < (O::DER * kPackTile + kPackBlock - 1) / kPackBlock; ++k) {
         const int i = threadIdx.x + k * kPackBlock;
-        const int c = i / OS, q = i % OS;
-        if (i < OS * kPackTile && c < tile && q < O::LEN && (q >= O::DPC || q < rows[c] * ncol) && on[c])
+        const int c = i / O::DER, q = i % O::DER;
+        if (i < O::DER * kPackTile && c < tile && (q >= O::DPC || q < rows[c] * ncol) && on[c])
             F.own[(n0 + c) * OS + q] = t[c * L + q];
+    }
+    __syncthreads();
+    // ---- the own record's derivative block (tile index q - DER) ----
+    for (int q = threadIdx.x; q < QD; q += kPackBlock) {
+        const int o = O::DER + q;
+        const double* p;
+        if (o < O::RHOP) p = B.xiP + (int64_t)(o - O::XIP) * ld;
+        else if (o < O::MUPM) p = B.rhoP + (int64_t)(o - O::RHOP) * ld;
+        else if (o < O::XIM) p = B.muP + (int64_t)(o - O::MUPM) * ld;
+        else if (o < O::DKR) p = B.xi + (int64_t)(o - O::XIM) * ld;
+        else if (o < O::RHOX) p = B.dKr_dS + (int64_t)(o - O::DKR) * ld;
+        else if (o < O::XIX) p = B.rhox + (int64_t)(o - O::RHOX) * ld;
+        else if (o < O::MUXM) p = B.xix + (int64_t)(o - O::XIX) * ld;
+        else if (o < O::LEN) p = B.mux + (int64_t)(o - O::MUXM) * ld;
+        else p = B.mu + (int64_t)(o - O::LEN) * ld;  // raw mu after the block
+        src[q] = p + n0;
+    }
+    __syncthreads();
+    packLoad<QD, kPackTile, L>(src, t);
+    __syncthreads();
+    {
+        // the ratios: muP / mu, xi / mu (3 each), mux / mu (3 nc)
+        constexpr int NR = 6 + 3 * nc;
+        constexpr int MU = O::LEN - O::DER;
+        for (int i = threadIdx.x; i < NR * kPackTile; i += kPackBlock) {
+            const int r = i / kPackTile, c = i % kPackTile;
+            double* row = t + c * L;
+            int q, j;
+            if (r < 6) {
+                q = (r < 3 ? O::MUPM : O::XIM) - O::DER + r % 3;
+                j = r % 3;
+            } else {
+                q = O::MUXM - O::DER + (r - 6);
+                j = (r - 6) / nc;
+            }
+            row[q] = ratioOrZero(row[q], row[MU + j]);
+        }
+    }
+    __syncthreads();
+    constexpr int ND = O::LEN - O::DER;
+#pragma unroll
+    for (int k = 0; k < (ND * kPackTile + kPackBlock - 1) / kPackBlock; ++k) {
+        const int i = threadIdx.x + k * kPackBlock;
+        const int c = i / ND, q = i % ND;
+        if (i < ND * kPackTile && c < tile && on[c]) F.own[(n0 + c) * OS + O::DER + q] = t[c * L + q];
     }
 }
 
@@ -611,7 +692,7 @@ __global__ void __launch_bounds__(256) packRecordsListKernel(DevBulk B, FimDevic
         const int64_t n = cellAt(L, i / W);
         const int q = (int)(i % W);
         if (q < R::LEN) {
-            F.rec[n * RS + q] = loadRec(recSource<NC>(B, q), n);
+            F.rec[n * RS + q] = recValue<NC>(B, q, n);
         } else {
             const int qo = q - R::LEN;
             const int rows = min(max(B.bRowSizedSdP[n], 0), AsmDims<NC>::maxRows);
@@ -643,11 +724,12 @@ template <int NC>
 struct AsmShared {
     using D = AsmDims<NC>;
     using R = AsmRec<NC>;
-    double sSec[D::maxRows * D::ncol];  // own dSec_dPri (compact rows x ncol)
-    double sDF[D::nc * D::dfStride];    // own side's dFdXp | dFdXs
+    using O = AsmOwn<NC>;
+    double sOwn[O::LEN];                // own record
     double sRec[2][R::LEN];             // [0] own cell, [1] current neighbour
-    double sDpc[9];                     // own dPcj_dS
-    double sPh[2][3];                   // per phase of the current connection: dP, transJ
+    double sC[3][D::CS];                // per phase: column coefficients, [CS - 1] the pressure one
+    double sD[3];                       // per phase: the x columns' diagonal term
+    double sG[3][D::ncol];              // per phase: C * dSec_own
 };
 
 // Row n of the bulk assembly, by one wavefront.
@@ -657,34 +739,34 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
 {
     using D = AsmDims<NC>;
     using R = AsmRec<NC>;
+    using O = AsmOwn<NC>;
     constexpr int nc = D::nc;
     constexpr int ncol = D::ncol;
     constexpr int bsize = D::bsize;
-    constexpr int W = D::dfStride;
-    const int64_t ld = B.ld;
+    constexpr int CS = D::CS;
+    constexpr int RS = recStride<NC>(), OS = ownStride<NC>();
     const int lane = threadIdx.x;
-    double* sSec = sh.sSec;
-    double* sDF = sh.sDF;
+    double* sOwn = sh.sOwn;
     double (*sRec)[R::LEN] = sh.sRec;
-    double* sDpc = sh.sDpc;
-    double (*sPh)[3] = sh.sPh;
     // a previous row of this wavefront may still read the shared arrays
     __syncthreads();
 
-    using O = AsmOwn<NC>;
-    constexpr int RS = recStride<NC>(), OS = ownStride<NC>();
     const double* __restrict__ own = F.own + n * OS;
+    const int rowsOwn = min(max((int)own[O::ROWS], 0), D::maxRows);
+#pragma unroll
+    for (int t = 0; t < O::perLane; ++t) {
+        const int q = lane + t * kAsmBlock;
+        if (q < O::LEN && (q >= O::DPC || q < rowsOwn * ncol)) sOwn[q] = own[q];
+    }
 #pragma unroll
     for (int t = 0; t < R::perLane; ++t) {
         const int q = lane + t * kAsmBlock;
         if (q < R::LEN) sRec[0][q] = F.rec[n * RS + q];
     }
-    const int rowsOwn = min(max((int)own[O::ROWS], 0), D::maxRows);
-    for (int q = lane; q < rowsOwn * ncol; q += kAsmBlock) sSec[q] = own[O::DSEC + q];
-    if (lane < 9) sDpc[lane] = own[O::DPC + lane];
 
-    // secondary columns of this row's cell: existing saturations, then the
-    // x_ij of each phase with derivatives
+    // secondary columns of this row's cell: existing saturations (their
+    // phases in sph), then the x_ij of each phase with derivatives (phase j's
+    // from column xb[j])
     bool sdO[3];
     int pvO[3];
     int nsat = 0;
@@ -694,7 +776,24 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
         pvO[j] = min(max((int)own[O::PVN + j], 0), NC);
         nsat += sdO[j] ? 1 : 0;
     }
+    int sph[3] = {0, 0, 0};
+    {
+        int s = 0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            if (sdO[j]) sph[s++] = j;
+    }
     const int ncolO = min(nsat + pvO[0] + pvO[1] + pvO[2], D::maxRows);
+    int xb[3];
+    xb[0] = nsat;
+    xb[1] = xb[0] + pvO[0];
+    xb[2] = xb[1] + pvO[1];
+
+    // this lane's column of dSec_own for G (lanes (j, cc))
+    const int gcc = lane % ncol;
+    double dS[D::maxRows];
+#pragma unroll
+    for (int k = 0; k < D::maxRows; ++k) dS[k] = k < rowsOwn ? own[O::DSEC + k * ncol + gcc] : 0.0;
 
     // diagonal starts as the accumulation block (src/OCPFluidMethod.cpp:1586-1597)
     double acc[D::perLane];
@@ -767,190 +866,108 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
         const bool ownB = n < m;  // this row is the connection's bId side
         const double* rB = ownB ? sRec[0] : sRec[1];
         const double* rE = ownB ? sRec[1] : sRec[0];
-        const double* rO = sRec[0];
         const double Akd = kCONV1f * kCONV2f * uniformD(areaV);
         const double dGamma = kGravityFactor * (uniformD(rB[R::DEPTH]) - uniformD(rE[R::DEPTH]));
-        bool exB[3], exE[3], upOwn[3];
+        // per-phase flags as bit masks (scalar): the own cell has the phase,
+        // the other cell has it, the own cell is upwind, the upwind cell has
+        // it (the phase flows)
+        unsigned ownExM = 0, othExM = 0, upOwnM = 0, liveM = 0;
         double upRho[3];
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            exB[j] = uniformNonzero(rB[R::EX + j]);
-            exE[j] = uniformNonzero(rE[R::EX + j]);
-            upOwn[j] = __builtin_amdgcn_readfirstlane(upV[j]) == (int32_t)n;
+            const bool exO = uniformNonzero(sRec[0][R::EX + j]);
+            const bool exN = uniformNonzero(sRec[1][R::EX + j]);
+            const bool upO = __builtin_amdgcn_readfirstlane(upV[j]) == (int32_t)n;
+            ownExM |= exO ? 1u << j : 0u;
+            othExM |= exN ? 1u << j : 0u;
+            upOwnM |= upO ? 1u << j : 0u;
+            liveM |= (upO ? exO : exN) ? 1u << j : 0u;
             upRho[j] = uniformD(upRhoV[j]);
         }
         const int64_t mirror = __builtin_amdgcn_readfirstlane(mirV);
+        const double sgn = ownB ? 1.0 : -1.0;
 
-        // ---- per phase: whether the upwind cell has it, dP and transJ ----
-        // (lanes 0..2; every entry below reads them)
-        bool live[3];
+        // ---- coefficients C[j][col], the pressure one at CS - 1, d_j ----
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            const bool upB = ownB ? upOwn[j] : !upOwn[j];
-            live[j] = upB ? exB[j] : exE[j];
-        }
-        if (lane < 3) {
-            const int j = lane;
-            const double* rU = upOwn[j] ? sRec[0] : sRec[1];
-            sPh[0][j] = rB[R::PJ + j] - rE[R::PJ + j] - upRho[j] * dGamma;
-            sPh[1][j] = Akd * rU[R::KR + j] / rU[R::MU + j];
+        for (int t = 0; t < D::cPasses; ++t) {
+            const int q = lane + t * kAsmBlock;
+            const int j = q / CS, col = q % CS;
+            if (j < 3) {
+                double cv = 0.0, dv = 0.0;
+                if ((liveM >> j) & 1u) {
+                    const bool upO = (upOwnM >> j) & 1u;
+                    const bool othEx = (othExM >> j) & 1u;
+                    const bool ownEx = (ownExM >> j) & 1u;
+                    const double* rU = upO ? sRec[0] : sRec[1];
+                    const double dP = rB[R::PJ + j] - rE[R::PJ + j] - (j == 0 ? upRho[0] : j == 1 ? upRho[1] : upRho[2]) * dGamma;
+                    const double transJ = Akd * rU[R::LAM + j];
+                    const double A = rU[R::XI + j] * transJ;
+                    if (col == CS - 1) {
+                        // pressure: sgn A, the density term when the other
+                        // cell lacks the phase, half of it when both have it,
+                        // the own cell's xi / mu derivatives when it is upwind
+                        double v = sgn * A;
+                        if (upO || (ownEx && othEx)) {
+                            const double R1 = A * (-sOwn[O::RHOP + j] * dGamma);
+                            const double T = transJ * sOwn[O::XIP + j] * dP - A * sOwn[O::MUPM + j] * dP;
+                            if (!othEx) {
+                                v += R1 + T;
+                            } else if (ownEx) {
+                                v += R1 / 2;
+                                if (upO) v += T;
+                            }
+                        }
+                        cv = v;
+                        dv = upO ? sRec[0][R::XI + j] * transJ * dP : 0.0;
+                    } else if (col < nsat) {
+                        const int s1 = col == 0 ? sph[0] : col == 1 ? sph[1] : sph[2];
+                        double v = sgn * A * sOwn[O::DPC + j * 3 + s1];
+                        if (upO) v += Akd * sOwn[O::XIM + j] * sOwn[O::DKR + j * 3 + s1] * dP;
+                        cv = v;
+                    } else if (col < ncolO) {
+                        // x column (jx, kk) of the own cell: only phase jx's row
+                        int kk = col - nsat, jx = 0;
+                        if (kk >= pvO[0]) {
+                            kk -= pvO[0];
+                            jx = 1;
+                            if (kk >= pvO[1]) {
+                                kk -= pvO[1];
+                                jx = 2;
+                            }
+                        }
+                        if (jx == j) {
+                            const double rhox = sOwn[O::RHOX + j * nc + kk];
+                            if (upO) {
+                                const double rx = othEx ? rhox / 2 : rhox;
+                                cv = -A * rx * dGamma + transJ * sOwn[O::XIX + j * nc + kk] * dP -
+                                     A * sOwn[O::MUXM + j * nc + kk] * dP;
+                            } else {
+                                cv = -A * (rhox / 2) * dGamma;
+                            }
+                        }
+                    }
+                }
+                sh.sC[j][col] = cv;
+                if (col == CS - 1) sh.sD[j] = dv;
+            }
         }
         __syncthreads();
-
-        // ---- own side's dFdXp column and dFdXs, entry by entry ----
-        // Pressure and saturation columns: every phase contributes, in the
-        // reference's phase order.
-        // Lane (i, col) = (lane / 4, lane % 4 - 1): nc <= 16 rows of at most
-        // four columns, no division by a run-time count.
-        static_assert(nc * 4 <= kAsmBlock, "pressure/saturation entries exceed a wavefront");
-        {
-            const int i = lane >> 2;
-            const int col = (lane & 3) - 1;  // -1: pressure column of dFdXp
-            if (i < nc && col < nsat) {
-            int j1S = -1;
-            {
-                int s = 0;
-#pragma unroll
-                for (int j1 = 0; j1 < 3; ++j1)
-                    if (sdO[j1]) {
-                        if (s == col) j1S = j1;
-                        s++;
-                    }
-            }
-            // saturation lanes read their column's dPc/dKr; the pressure lane reads column 0
-            const int s1 = j1S < 0 ? 0 : j1S;
-            double v = 0;
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                if (!live[j]) continue;
-                const bool upB = ownB ? upOwn[j] : !upOwn[j];
-                const double* rU = upOwn[j] ? sRec[0] : sRec[1];
-                const double dP = sPh[0][j];
-                const double transJ = sPh[1][j];
-                const double xi = rU[R::XI + j];
-                const double mu = rU[R::MU + j];
-                const double xij = rU[R::XIJ + j * nc + i];
-                const double transIJ = xij * xi * transJ;
-                const double muP = rU[R::MUP + j];
-                // The pressure entry's -transIJ * muP / mu and a saturation
-                // entry's Akd * xij * xi / mu share one division: the lanes
-                // pick their numerator (same operands, same rounding as the
-                // two separate expressions).
-                const double q = (col < 0 ? -transIJ * muP : Akd * xij * xi) / mu;
-                if (col < 0) {
-                    const double xiP = rU[R::XIP + j];
-                    const double rhoP = rU[R::RHOP + j];
-                    double tmp = xij * transJ * xiP * dP;
-                    tmp += q * dP;
-                    if (ownB) {
-                        v += transIJ;
-                        if (!exE[j]) {
-                            tmp += transIJ * (-rhoP * dGamma);
-                            v += tmp;
-                        } else if (!exB[j]) {
-                        } else {
-                            v += transIJ * (-rO[R::RHOP + j] * dGamma) / 2;
-                            if (upB) v += tmp;
-                        }
-                    } else {
-                        v -= transIJ;
-                        if (!exE[j]) {
-                        } else if (!exB[j]) {
-                            tmp += transIJ * (-rhoP * dGamma);
-                            v += tmp;
-                        } else {
-                            v += transIJ * (-rO[R::RHOP + j] * dGamma) / 2;
-                            if (!upB) v += tmp;
-                        }
-                    }
-                } else {
-                    // saturation column of phase j1S (= s1 on these lanes)
-                    const double dpc = sDpc[j * 3 + s1];
-                    if (ownB) {
-                        v += transIJ * dpc;
-                        if (upB) v += q * rU[R::DKR + j * 3 + s1] * dP;
-                    } else {
-                        v -= transIJ * dpc;
-                        if (!upB) v += q * rU[R::DKR + j * 3 + s1] * dP;
-                    }
-                }
-            }
-            sDF[i * W + col + 1] = v;
-            }
-        }
-        // x_{kX, jX} columns: only phase jX's pass touches them.  Entry
-        // (i, kx) at q = i * XS + kx with a compile-time stride XS >= nX:
-        // NC rounded up to a power of two when the row's cell has one
-        // hydrocarbon phase (nX = NC), the wider stride otherwise (nX = 2 NC).
-        const int nX = max(ncolO - nsat, 0);
-        // per-phase flags as bit masks: the lanes of this pass index them by
-        // their own phase
-        unsigned liveM = 0, upOwnM = 0, exBM = 0, exEM = 0;
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            liveM |= live[j] ? 1u << j : 0u;
-            upOwnM |= upOwn[j] ? 1u << j : 0u;
-            exBM |= exB[j] ? 1u << j : 0u;
-            exEM |= exE[j] ? 1u << j : 0u;
-        }
-        auto xEntry = [&](int i, int kx) {
-            int kk = kx, j = 0;
-            if (kk >= pvO[0]) {
-                kk -= pvO[0];
-                j = 1;
-                if (kk >= pvO[1]) {
-                    kk -= pvO[1];
-                    j = 2;
-                }
-            }
-            double v = 0;
+        // ---- G[j][cc] = [cc == 0] C[j][pressure] + sum_col C[j][col] dSec[col][cc] ----
+        if (lane < 3 * ncol) {
+            const int j = lane / ncol;
+            double g = 0.0;
             if ((liveM >> j) & 1u) {
-                const bool upOwnJ = (upOwnM >> j) & 1u;
-                const bool upB = ownB ? upOwnJ : !upOwnJ;
-                const double* rU = upOwnJ ? sRec[0] : sRec[1];
-                const double dP = sPh[0][j];
-                const double transJ = sPh[1][j];
-                const double xi = rU[R::XI + j];
-                const double mu = rU[R::MU + j];
-                const double xij = rU[R::XIJ + j * nc + i];
-                const double transIJ = xij * xi * transJ;
-                const bool ownUp = ownB ? upB : !upB;
-                const bool otherEx = (((ownB ? exEM : exBM) >> j) & 1u) != 0;
-                if (ownUp) {
-                    const double rhox = otherEx ? rO[R::RHOX + j * nc + kk] / 2 : rU[R::RHOX + j * nc + kk];
-                    const double xix = rU[R::XIX + j * nc + kk];
-                    const double mux = rU[R::MUX + j * nc + kk];
-                    double tmp = -transIJ * rhox * dGamma;
-                    tmp += xij * transJ * xix * dP;
-                    tmp += -transIJ * mux / mu * dP;
-                    v += tmp;
-                    if (kk == i) v += xi * transJ * dP;
-                } else if (otherEx) {
-                    v += -transIJ * rO[R::RHOX + j * nc + kk] / 2 * dGamma;
-                }
-            }
-            sDF[i * W + 1 + nsat + kx] = v;
-        };
-        if (nX <= NC) {
-            constexpr int XS = D::xStride1;
+                // columns past the own cell's hold zero coefficients and
+                // zero dSec: a fixed trip count keeps dS in registers
+                g = gcc == 0 ? sh.sC[j][CS - 1] : 0.0;
 #pragma unroll
-            for (int t = 0; t < (nc * XS + kAsmBlock - 1) / kAsmBlock; ++t) {
-                const int q = lane + t * kAsmBlock;
-                const int i = q / XS, kx = q % XS;
-                if (i < nc && kx < nX) xEntry(i, kx);
+                for (int kq = 0; kq < D::maxRows; ++kq) g = fma(sh.sC[j][kq], dS[kq], g);
             }
-        } else {
-            constexpr int XS = D::xStride;
-#pragma unroll
-            for (int t = 0; t < (nc * XS + kAsmBlock - 1) / kAsmBlock; ++t) {
-                const int q = lane + t * kAsmBlock;
-                const int i = q / XS, kx = q % XS;
-                if (i < nc && kx < nX) xEntry(i, kx);
-            }
+            sh.sG[j][gcc] = g;
         }
         __syncthreads();
 
-        // ---- block = (dFdXp + dFdXs * dSec_own) * dt, then scatter ----
+        // ---- block = dt (sum_j x_ij G[j] + sum_j d_j dSec[x(j, i)]), then scatter ----
         double* dst = F.val + mirror * bsize;
 #pragma unroll
         for (int t = 0; t < D::perLane; ++t) {
@@ -959,12 +976,14 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
                 const int r = e / ncol, cc = e % ncol;
                 double v = 0;
                 if (r > 0) {
-                    const double* row = &sDF[(r - 1) * W];
-                    v = (cc == 0) ? row[0] : 0.0;
+                    const int i = r - 1;
 #pragma unroll
-                    for (int kq = 0; kq < D::maxRows; ++kq) {
-                        if (kq >= ncolO) break;
-                        v = v + row[1 + kq] * sSec[kq * ncol + cc];
+                    for (int j = 0; j < 3; ++j) {
+                        if (!((liveM >> j) & 1u)) continue;
+                        const double* rU = ((upOwnM >> j) & 1u) ? sRec[0] : sRec[1];
+                        v = fma(rU[R::XIJ + j * nc + i], sh.sG[j][cc], v);
+                        if (((upOwnM >> j) & 1u) && i < pvO[j])
+                            v = fma(sh.sD[j], sOwn[O::DSEC + (xb[j] + i) * ncol + cc], v);
                     }
                 }
                 v = dt * v;
@@ -977,8 +996,8 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
                 }
             }
         }
-        // the next iteration overwrites sRec[1] (read only before the barrier
-        // above) and sDF (after its own barrier)
+        // the next iteration overwrites sRec[1] and the tables only after
+        // its first barrier, which every lane reaches after these reads
     }
     double* diag = F.val + (int64_t)F.IA[n] * bsize;
 #pragma unroll
@@ -989,7 +1008,7 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
 }
 
 template <int NC>
-__global__ void __launch_bounds__(kAsmBlock) __attribute__((amdgpu_waves_per_eu(NC <= 6 ? 6 : 4))) assembleKernel(DevBulk B, FimDevice F, double dt)
+__global__ void __launch_bounds__(kAsmBlock) __attribute__((amdgpu_waves_per_eu(NC <= 6 ? 5 : 4))) assembleKernel(DevBulk B, FimDevice F, double dt)
 {
     __shared__ AsmShared<NC> sh;
     // workgroup b runs on XCD b % kXcds; XCD x takes rows [x*chunk, (x+1)*chunk)

@@ -64,7 +64,9 @@ def compare(case, wells, got, ref):
         loose[IA[r]:IA[r + 1]] = True
     gpu_util.say("  blocks", len(err), "exact", np.mean(err == 0), "max err (rate-producer rows)",
                  err[loose].max() if loose.any() else 0.0)
-    assert np.all(err[~loose] == 0), np.flatnonzero((err != 0) & ~loose)[:10]
+    # the bulk blocks are the reference's sums reassociated (rounding-level
+    # differences); the well blocks are the same arithmetic in the same order
+    assert np.all(err[~loose] <= 1e-13), np.flatnonzero((err > 1e-13) & ~loose)[:10]
     assert np.all(err[loose] <= 1e-12)
 
 
