@@ -40,12 +40,12 @@ DT = 5.0
 # on this exact workload, all 1,122,000 cells: 21,584,690,202 flops and
 # 542,748,484 transcendentals in total.
 WARM_FLOPS_PER_CELL = 21584690202 / 1122000
-# HBM bytes per launch of the assembly and pack kernels on the default
-# workload: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, scaled by what the
-# counters read for a device-to-device copy of known size
-# (profiles/assembly_pack_1p1M_6comp_traffic.md)
-ASM_TRAFFIC_BYTES = 5.120e9 + 3.983e9
-PACK_TRAFFIC_BYTES = 2.329e9 + 2.165e9
+# HBM bytes per launch of the assembly and of the CalRes + record-packing
+# launch on the default workload: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE,
+# scaled by what the counters read for a device-to-device copy of known size
+# (profiles/assembly_calres_1p1M_6comp_traffic.md)
+ASM_TRAFFIC_BYTES = 3.840e9 + 3.983e9
+CALRES_PACK_TRAFFIC_BYTES = 6.149e9 + 3.259e9
 
 
 def log(*a):
@@ -302,6 +302,7 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=384,
                     help="requests of the serving-form leg (p50/p99 latency and capacity, continuous batching "
                          "vs one at a time, 256 realisations of 1e3..1e6 cells); 0 skips it")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-form leg")
     ap.add_argument("--deck", default=None,
                     help="read the workload from an input deck (opencaeporo_amd/deck.py), e.g. "
                          "tests/golden/decks/spe5refine/spe5-70x70x30-2y.data (a measurement case, not the headline)")
@@ -368,7 +369,7 @@ def main():
         cold_ms = float(np.median(ts))
         log(f"[bench] cold pass: {cold_ms:.2f} ms per step (median of {len(ts)})")
     learned_res = None
-    if not side_case:
+    if not side_case and not a.no_learned:
         try:
             learned_res = learned_leg(ctx, nb, NC)
         except Exception as e:  # informative only
@@ -384,18 +385,25 @@ def main():
     # Algorithmic bytes of one launch of the assembly kernel (every value it
     # needs read once, every block written once): the BSR blocks, the
     # pattern (JA, connection and mirror slot per block; IA, bulkEnd per
-    # row), each cell's connection-side record (30 + 12 nc doubles: Pj, xi,
-    # kr, mu, muP, xiP, rhoP per phase, dKr_dS, xij, rhox, xix, mux, depth;
-    # three phase flags), its own-row record (dSec_dPri rows x ncol, dPcj_dS,
-    # vfi, vfP, v, poroP, 7 counts) and per connection Akd and its upwinding
-    # (3 cells, 3 densities).
-    rec_bytes = (30 + 12 * nc + 1) * 8 + 3
-    own_bytes = (rows_mean * ncol + 9 + nc + 3) * 8 + 7 * 4
+    # row), each cell's neighbour record (10 + 3 nc doubles: Pj, xi, kr/mu
+    # per phase, xij, depth; three phase flags), its own-row record (dSec_dPri
+    # rows x ncol, dPcj_dS, vfi, vfP, v, poroP, its derivatives xiP, rhoP,
+    # muP/mu, xi/mu, dKr_dS, rhox, xix, mux/mu; 7 counts) and per connection
+    # Akd and its upwinding (3 cells, 3 densities).
+    rec_bytes = (10 + 3 * nc) * 8 + 3
+    own_bytes = (rows_mean * ncol + 9 + nc + 3 + 21 + 9 * nc) * 8 + 7 * 4
     asm_bytes = (nnzb * ncol * ncol * 8 + nnzb * 12 + nb * 8 + nb * (rec_bytes + own_bytes)
                  + grid.nconn * (8 + 3 * 4 + 3 * 8))
-    # the pack kernel reads the same cell values from the structure-of-arrays
-    # state and writes them as per-cell records (doubles)
-    pack_bytes = nb * (rec_bytes + own_bytes) + nb * ((30 + 12 * nc + 4) + rows_mean * ncol + 9 + nc + 10) * 8
+    # The record packing reads the same cell values from the
+    # structure-of-arrays state (mu and xi once) and writes the records
+    # (doubles); CalRes reads each cell's pore and fluid volume, moles, last
+    # moles, densities, pressures, depth, flags and upwind properties once,
+    # writes the residual and per connection the upwinding (3 cells, 3
+    # densities, 3 velocities) from the area.
+    pack_bytes = (nb * (rec_bytes + own_bytes - 6 * 8)
+                  + nb * ((10 + 3 * nc + 3) + rows_mean * ncol + 9 + nc + 10 + 21 + 9 * nc) * 8)
+    calres_bytes = (nb * ((2 + 2 * nc + 3 + 3 + 1 + 9 + 3 * nc) * 8 + 3 + (nc + 1) * 8)
+                    + grid.nconn * (8 + 3 * 4 + 3 * 8 + 3 * 8))
     flash_ms = per["flash"]
     launches = {k: v for k, v in per.items() if k not in ("flash", "update_assemble", "flash_stability",
                                                           "flash_split", "flash_finish", "flash_tail",
@@ -404,21 +412,36 @@ def main():
     # The dominant kernel: the assembly, the longest launch on the main
     # stream (the flash stages are several launches each, and the side
     # streams' kernels run beside the main stream).  Its bound by algorithmic
-    # work is HBM; it reaches a fraction of it because it issues ~2400 VALU
-    # instructions per row (rocprofv3 SQ_INSTS_VALU).
+    # work is HBM; it reaches a fraction of it because each row's wavefront
+    # works through its connections one after the other (~1,190 VALU and
+    # ~990 SALU instructions per row, rocprofv3 SQ_INSTS_VALU / SQ_INSTS_SALU)
+    # at five wavefronts per SIMD.
     roof = dict(kernel="assembleKernel", bound="hbm", achieved=asm_bytes / (per["assemble"] / 1e3) / 1e9,
                 peak=HBM_PEAK_GBS, unit="GB/s", traffic=None, bytes_per_launch=asm_bytes)
-    roof_pack = dict(kernel="packRecordsKernel", bound="hbm", achieved=pack_bytes / (per["pack"] / 1e3) / 1e9,
-                     peak=HBM_PEAK_GBS, unit="GB/s", traffic=None, bytes_per_launch=pack_bytes)
-    for r in (roof, roof_pack):
-        r["frac"] = r["achieved"] / r["peak"]
-    if not side_case and nz == 85:
+    if a.serial:
+        # separate launches: CalRes (ocpg_calres), the packing (ocpg_assemble)
+        roof_other = [dict(kernel="calResKernel", bound="hbm", bytes_per_launch=calres_bytes, ms=per["calres"]),
+                      dict(kernel="packRecordsKernel", bound="hbm", bytes_per_launch=pack_bytes, ms=per["pack"])]
+    else:
+        roof_other = [dict(kernel="calResPackKernel", bound="hbm", bytes_per_launch=calres_bytes + pack_bytes,
+                           ms=per["calres"])]
+    other = []
+    for r in roof_other:
+        ms = r.pop("ms")
+        if ms > 0:
+            r.update(achieved=r["bytes_per_launch"] / (ms / 1e3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
+                     traffic=None)
+            r["frac"] = r["achieved"] / r["peak"]
+            other.append(r)
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    if not side_case and nz == 85 and not a.serial:
         # HBM bytes per launch measured by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
         # on this workload, scaled by the copy-kernel calibration
-        # (profiles/assembly_pack_1p1M_6comp_traffic.md)
+        # (profiles/assembly_calres_1p1M_6comp_traffic.md)
         roof["traffic"] = ASM_TRAFFIC_BYTES
-        roof_pack["traffic"] = PACK_TRAFFIC_BYTES
-    other = [roof_pack]
+        for r in other:
+            if r["kernel"] == "calResPackKernel":
+                r["traffic"] = CALRES_PACK_TRAFFIC_BYTES
     # the flash against FP64 vector peak (its algorithmic flops per warm cell,
     # over the main stream's flash time), and the whole step against the
     # step's roofline time: flash flops at FP64 peak plus the assembly's

@@ -540,7 +540,7 @@ __device__ __forceinline__ void packLoad(const double* const* src, double* t)
 }
 
 template <int NC>
-__global__ void __launch_bounds__(kPackBlock) packRecordsKernel(DevBulk B, FimDevice F)
+__device__ __forceinline__ void packTileBody(const DevBulk& B, const FimDevice& F, int64_t tileIdx)
 {
     using R = AsmRec<NC>;
     using O = AsmOwn<NC>;
@@ -558,7 +558,7 @@ __global__ void __launch_bounds__(kPackBlock) packRecordsKernel(DevBulk B, FimDe
     __shared__ int on[kPackTile];
     __shared__ int anyOn;
     __shared__ const double* src[QMAX];
-    const int64_t n0 = (int64_t)blockIdx.x * kPackTile;
+    const int64_t n0 = tileIdx * kPackTile;
     const int tile = (int)min<int64_t>(kPackTile, B.nb - n0);
     const int64_t ld = B.ld;
     if (threadIdx.x == 0) anyOn = 0;
@@ -675,6 +675,34 @@ __global__ void __launch_bounds__(kPackBlock) packRecordsKernel(DevBulk B, FimDe
         const int i = threadIdx.x + k * kPackBlock;
         const int c = i / ND, q = i % ND;
         if (i < ND * kPackTile && c < tile && on[c]) F.own[(n0 + c) * OS + O::DER + q] = t[c * L + q];
+    }
+}
+
+template <int NC>
+__global__ void __launch_bounds__(kPackBlock) packRecordsKernel(DevBulk B, FimDevice F)
+{
+    packTileBody<NC>(B, F, blockIdx.x);
+}
+
+// CalRes and the record packing in one launch (they read the same state and
+// neither needs the other): each group of 1 + tiles blocks is one CalRes
+// block and the packing of the same cells, so the two kinds of work share
+// the chip instead of running one after the other.
+template <int NC>
+constexpr int packTilesPerResBlock() { return kResBlock / packTile<NC>(); }
+
+template <int NC>
+__global__ void __launch_bounds__(kResBlock) calResPackKernel(DevBulk B, FimDevice F, WellDevice W, double dt)
+{
+    constexpr int PT = packTilesPerResBlock<NC>();
+    const int64_t group = blockIdx.x / (1 + PT);
+    const int r = (int)(blockIdx.x % (1 + PT));
+    if (r == 0) {
+        const int64_t n = group * kResBlock + threadIdx.x;
+        if (n < B.nb && cellOn(B, n)) calResCell<NC>(B, F, W, dt, n);
+    } else {
+        const int64_t tileIdx = group * PT + (r - 1);
+        if (tileIdx * packTile<NC>() < B.nb) packTileBody<NC>(B, F, tileIdx);
     }
 }
 
@@ -1448,6 +1476,15 @@ static hipError_t launchPackT(const DevBulk& B, FimDevice& F, hipStream_t s)
 }
 
 template <int NC>
+static hipError_t launchCalResPackT(const DevBulk& B, FimDevice& F, const WellDevice& W, double dt, hipStream_t s)
+{
+    const int64_t groups = (B.nb + kResBlock - 1) / kResBlock;
+    const unsigned grid = (unsigned)(groups * (1 + packTilesPerResBlock<NC>()));
+    hipLaunchKernelGGL(calResPackKernel<NC>, dim3(grid), dim3(kResBlock), 0, s, B, F, W, dt);
+    return hipGetLastError();
+}
+
+template <int NC>
 static hipError_t launchPackListT(const DevBulk& B, FimDevice& F, const CellLists& L, hipStream_t s)
 {
     hipLaunchKernelGGL(packRecordsListKernel<NC>, dim3(256), dim3(256), 0, s, B, F, L);
@@ -1477,6 +1514,16 @@ hipError_t launchPackRecords(int NC, const DevBulk& B, FimDevice& F, hipStream_t
     if (B.nb == 0) return hipSuccess;
 #define X(N) \
     if (NC == N) return launchPackT<N>(B, F, s);
+    OCPG_FIM_NC(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launchCalResPack(int NC, const DevBulk& B, FimDevice& F, const WellDevice& W, double dt, hipStream_t s)
+{
+    if (B.nb == 0) return hipSuccess;
+#define X(N) \
+    if (NC == N) return launchCalResPackT<N>(B, F, W, dt, s);
     OCPG_FIM_NC(X)
 #undef X
     return hipErrorInvalidValue;

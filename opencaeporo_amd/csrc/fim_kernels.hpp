@@ -58,6 +58,8 @@ hipError_t launchAssemble(int NC, const DevBulk& B, FimDevice& F, double dt, hip
 int64_t recordDoubles(int NC);
 int64_t ownDoubles(int NC);
 hipError_t launchPackRecords(int NC, const DevBulk& B, FimDevice& F, hipStream_t s);
+// CalRes (no norms) and the record packing of every cell in one launch.
+hipError_t launchCalResPack(int NC, const DevBulk& B, FimDevice& F, const WellDevice& W, double dt, hipStream_t s);
 hipError_t launchPackRecordsList(int NC, const DevBulk& B, FimDevice& F, const CellLists& L, hipStream_t s);
 
 // IsoT_FIM::GetSolution on the device (fim_kernels.hip) and the Newton-

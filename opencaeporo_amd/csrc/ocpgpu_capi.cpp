@@ -780,12 +780,11 @@ int ocpg_update_assemble(ocpg_ctx* c, double dt, ocpg_resnorm* norms, ocpg_flash
     timerBegin(c, T_KRPC);
     HIPCHK(launchKrPcRock(c->B, c->dSat, c->rock, true, true, c->stream));
     timerEnd(c, T_KRPC);
+    // CalRes and the records the assembly reads, side by side in one launch
+    // (timed as "calres"; "pack" stays empty here)
     timerBegin(c, T_RES);
-    HIPCHK(launchCalRes(c->NC, c->B, F, W, dt, c->stream, false));
+    HIPCHK(launchCalResPack(c->NC, c->B, F, W, dt, c->stream));
     timerEnd(c, T_RES);
-    timerBegin(c, T_PACK);
-    HIPCHK(launchPackRecords(c->NC, c->B, F, c->stream));
-    timerEnd(c, T_PACK);
     timerBegin(c, T_ASM);
     HIPCHK(launchAssemble(c->NC, c->B, F, dt, c->stream));
     timerEnd(c, T_ASM);
