@@ -168,6 +168,10 @@ typedef struct ocpg_bulk {
     /* diagnostics (no reference counterpart; NULL to skip)                    */
     int32_t* ftype;       /* nb: 0 full test, 1 skipped, 2 straight to split    */
     int32_t* iters;       /* nb*5: SSMSTA, NRSTA, SSMSP, NRSP, RR iterations    */
+    int32_t* nonconv;     /* nb: 1 where the last flash did not converge (the
+                             cell's MixtureComp::ePEC would read 1e8,
+                             src/MixtureComp.cpp:1301-1346), 2 where its
+                             results are not finite, else 0                    */
 } ocpg_bulk;
 
 /* Connection list in BulkConn order (src/BulkConn.cpp:31-117): bId < eId,

@@ -103,7 +103,7 @@ BULK_FIELDS = [
     ("dKr_dS", "f8", "nb*np*np"), ("dPcj_dS", "f8", "nb*np*np"),
     ("v", "f8", "nb"), ("depth", "f8", "nb"), ("poroInit", "f8", "nb"), ("poro", "f8", "nb"),
     ("poroP", "f8", "nb"), ("rockVp", "f8", "nb"),
-    ("ftype", "i4", "nb"), ("iters", "i4", "nb*5"),
+    ("ftype", "i4", "nb"), ("iters", "i4", "nb*5"), ("nonconv", "i4", "nb"),
 ]
 
 

@@ -64,7 +64,7 @@ struct DevBulk {
     double *skipMinEig, *skipP, *skipT, *skipZi;
     double *kr, *Pc, *Pj, *dKr_dS, *dPcj_dS;
     double *v, *depth, *poroInit, *poro, *poroP, *rockVp;
-    int32_t *ftype, *iters;
+    int32_t *ftype, *iters, *nonconv;
     // serving pool: per cell 1 when its realisation takes part in the
     // iteration (NULL: every cell does)
     const uint8_t* mask;

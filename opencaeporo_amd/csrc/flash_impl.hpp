@@ -284,6 +284,10 @@ struct FlashKernel {
         B.vfP[n] = vfP;
 #pragma unroll
         for (int j = 0; j < 3; ++j) B.S[j * ld + n] = S[j];
+        // non-finite results (a NaN or infinite input) are flagged apart
+        // from non-convergence
+        if (B.nonconv && !(isfinite(vf) && isfinite(S[0]) && isfinite(S[1]) && isfinite(S[2])))
+            B.nonconv[n] = 2;
 #pragma unroll
         for (int i = 0; i < nc; ++i) B.vfi[i * ld + n] = vfi[i];
         if (init) {
@@ -688,6 +692,7 @@ __global__ void __launch_bounds__(kFlashBlock) flashClassifyKernel(const DevEos*
         W.flags[n] = (ftype == 2) ? 0 : FL_SKIP;
         if (ftype == 2) W.Yt[n] = 1.01;
         if (B.ftype) B.ftype[n] = ftype;
+        if (B.nonconv) B.nonconv[n] = 0;
         if (B.iters) {
 #pragma unroll
             for (int q = 0; q < 5; ++q) B.iters[q * ld + n] = 0;
@@ -778,6 +783,7 @@ __global__ void __launch_bounds__(kFlashBlock) flashStabilityKernel(const DevEos
         W.flags[n] = (uint8_t)fl;
         if (!stable) W.Yt[n] = F.Yt;
         if (stable) nonconv = !(F.nrStaConflag || F.ssmStaConflag);
+        if (stable && B.nonconv) B.nonconv[n] = nonconv ? 1 : 0;
         if (B.iters) {
             B.iters[IT_SSMSTA * ld + n] = it0;
             B.iters[IT_NRSTA * ld + n] = it1;
@@ -896,6 +902,7 @@ __device__ __forceinline__ void splitDone(FlashCell<NC>& F, const DevBulk& B, co
     } else {
         nonconv = !(F.nrSpConflag || F.ssmSpConflag);
     }
+    if (B.nonconv) B.nonconv[n] = nonconv ? 1 : 0;
     W.flags[n] = (uint8_t)fl;
     if (NP == 2) {
 #pragma unroll

@@ -63,6 +63,7 @@ const FieldDesc kFields[] = {
     {HB(T), DB(T), 8, C1, false},               {HB(v), DB(v), 8, C1, false},
     {HB(depth), DB(depth), 8, C1, false},       {HB(poroInit), DB(poroInit), 8, C1, false},
     {HB(ftype), DB(ftype), 4, C1, false},       {HB(iters), DB(iters), 4, C5, false},
+    {HB(nonconv), DB(nonconv), 4, C1, false},
 };
 constexpr int kNumFields = sizeof(kFields) / sizeof(kFields[0]);
 

@@ -66,6 +66,15 @@ static void PassFlashValue(const Mixture& mix, const ocpg_bulk& bk, int64_t n)
     if (bk.ftype) bk.ftype[n] = mix.ftype;
     if (bk.iters)
         for (int q = 0; q < 5; q++) bk.iters[n * 5 + q] = (int32_t)mix.iters[q];
+    // MixtureComp's ePEC bookkeeping (src/MixtureComp.cpp:1301-1346): 1e8
+    // where neither the Newton nor the SSM stage of the outcome converged;
+    // 2 where the results are not finite (the product's own flag)
+    if (bk.nonconv) {
+        bk.nonconv[n] = (todbl(mix.ePEC) >= 1e8) ? 1 : 0;
+        bool fin = std::isfinite(bk.vf[n]);
+        for (int j = 0; j < np; j++) fin = fin && std::isfinite(bk.S[bIdp + j]);
+        if (!fin) bk.nonconv[n] = 2;
+    }
 }
 
 static void LoadSkip(const ocpg_bulk& bk, int NC, int64_t n, SkipState& s)

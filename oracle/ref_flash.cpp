@@ -209,6 +209,7 @@ static void pass_value(RefHandle* h, ocpg_bulk* bk, int64_t n, const unsigned lo
     const vector<OCP_DBL>& d = m.GetDXsDXp();
     for (int k = 0; k < len * (nc + 1); k++) bk->dSec_dPri[n * L + k] = d[k];
     if (bk->ftype) bk->ftype[n] = m.ftype;
+    if (bk->nonconv) bk->nonconv[n] = m.GetErrorPEC() >= 1e8 ? 1 : 0;
     if (bk->iters) {
         bk->iters[n * 5 + 0] = (int32_t)(m.itersSSMSTA - it0[0]);
         bk->iters[n * 5 + 1] = (int32_t)(m.itersNRSTA - it0[1]);

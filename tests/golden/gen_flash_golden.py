@@ -9,6 +9,7 @@ P + U[-5, 5] psia) every array PassFlashValue hands to the Bulk.
 
     python tests/golden/gen_flash_golden.py
 """
+import copy
 import ctypes as C
 import json
 import os
@@ -71,7 +72,15 @@ def snapshot(b, fields, prefix):
     return out
 
 
-def gen_flash(name, deck, nb, seeds, dp_seed):
+def gen_flash(name, deck, nb, seeds, dp_seed, limits=None):
+    """limits: (SSMSTA, NRSTA, SSMSP, NRSP) iteration caps replacing the
+    deck's (the non-convergence paths); such files also record the per-cell
+    non-convergence flag."""
+    out_fields = OUT_FIELDS
+    if limits is not None:
+        deck = copy.copy(deck)
+        deck.ssmsta_maxit, deck.nrsta_maxit, deck.ssmsp_maxit, deck.nrsp_maxit = limits
+        out_fields = OUT_FIELDS + ["nonconv"]
     NC = deck.nc
     ref = Reference(deck, nb)
     P, z, Sw = workloads.cell_inputs(nb, NC, seed=seeds[0])
@@ -85,12 +94,12 @@ def gen_flash(name, deck, nb, seeds, dp_seed):
     data.update(eos_arrays(ref.eos()))
     data.update(snapshot(b, IN_FIELDS, "cold_in_"))
     ref.flash_fim(b)
-    data.update(snapshot(b, OUT_FIELDS, "cold_out_"))
+    data.update(snapshot(b, out_fields, "cold_out_"))
     rng = np.random.default_rng(dp_seed)
     b.P[:] = P + rng.uniform(-5.0, 5.0, nb)
     data.update(snapshot(b, IN_FIELDS, "warm_in_"))
     ref.flash_fim(b)
-    data.update(snapshot(b, OUT_FIELDS, "warm_out_"))
+    data.update(snapshot(b, out_fields, "warm_out_"))
     path = os.path.join(HERE, name)
     np.savez_compressed(path, **data)
     two = np.mean(b.phaseNum == 3)
@@ -132,6 +141,10 @@ def main():
     gen_flash("flash_bench6.npz", fluids.bench6_deck(), 256, (201, 202), 203)
     gen_flash("flash_ten.npz", fluids.ten_deck(), 192, (301, 302), 303)
     gen_init("initflash_spe5.npz", fluids.spe5_deck(), 128, (401, 402))
+    # iteration caps: some cells stop unconverged, and with every cap at 0
+    # every cell does (MixtureComp's ePEC = 1e8 paths)
+    gen_flash("flash_limits_bench6.npz", fluids.bench6_deck(), 256, (501, 502), 503, limits=(10, 2, 8, 2))
+    gen_flash("flash_maxit0_bench6.npz", fluids.bench6_deck(), 128, (601, 602), 603, limits=(0, 0, 0, 0))
     meta = dict(reference="OpenCAEPoro 0.5.0 MixtureComp (src/MixtureComp.cpp) compiled by "
                           "oracle/Makefile 'ref'",
                 lapack=mkl_version() + " (sequential layer)",

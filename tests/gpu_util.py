@@ -48,7 +48,7 @@ def snapshot_dict(b, prefix):
     """Fixture-like dict of a bulk's state fields (for parity.compare_state)."""
     from tests import parity
     d = {}
-    for f in parity.STATE_FIELDS:
+    for f in parity.STATE_FIELDS + ["nonconv"]:
         a = getattr(b, f)
         if f == "dSec_dPri":
             L = abi.max_len_dsdp(b.nc)

@@ -155,6 +155,12 @@ def compare_state(b, d, prefix, tol, int_exact=True, skip_tol=1e-4, frac=1.0, lo
         report[f] = int(bad.sum())
         if int_exact:
             assert not bad.any(), f"{f}: {bad.sum()} cells differ, first {np.flatnonzero(bad)[:5]}"
+    if prefix + "nonconv" in d:
+        # the per-cell non-convergence flag (ePEC = 1e8) must match exactly
+        bad = b.nonconv != d[prefix + "nonconv"]
+        report["nonconv"] = int(bad.sum())
+        report["nonconv_cells"] = int(b.nonconv.sum())
+        assert not bad.any(), f"nonconv: {bad.sum()} cells differ, first {np.flatnonzero(bad)[:5]}"
     for f in FLOAT_FIELDS:
         ref = d[prefix + f]
         got = field(b, f, used=ref.shape[1] if f == "dSec_dPri" else None)

@@ -9,7 +9,10 @@ from tests import gpu_util, parity
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["flash_spe5.npz", "flash_bench6.npz", "flash_ten.npz"]
+CASES = ["flash_spe5.npz", "flash_bench6.npz", "flash_ten.npz",
+         # iteration caps (10, 2, 8, 2) and all 0: the non-convergence paths,
+         # with the per-cell flag compared exactly
+         "flash_limits_bench6.npz", "flash_maxit0_bench6.npz"]
 
 
 @pytest.fixture(scope="module")

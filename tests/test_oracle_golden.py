@@ -24,7 +24,10 @@ ORACLE_VS_REF_TOL = 1e-8
 ILL_COND_FRAC = 0.97
 ILL_COND_LOOSE = 1e-4
 
-CASES = ["flash_spe5.npz", "flash_bench6.npz", "flash_ten.npz"]
+CASES = ["flash_spe5.npz", "flash_bench6.npz", "flash_ten.npz",
+         # iteration caps (10, 2, 8, 2) and all 0: the non-convergence paths,
+         # with the per-cell flag compared exactly
+         "flash_limits_bench6.npz", "flash_maxit0_bench6.npz"]
 
 
 @pytest.fixture(scope="module")
