@@ -172,6 +172,12 @@ typedef struct ocpg_bulk {
                              cell's MixtureComp::ePEC would read 1e8,
                              src/MixtureComp.cpp:1301-1346), 2 where its
                              results are not finite, else 0                    */
+    /* IsoT_FIMn state (src/OCPFluidMethod.cpp:3111-3127): phase moles, the
+     * flash's secondary-variable correction in dSec_dPri's row order (its
+     * first bRowSizedSdP entries are set) and its volume-balance term       */
+    double* nj;           /* nb*np                                              */
+    double* res_n;        /* nb*np*(nc+1)                                       */
+    double* resPc;        /* nb                                                 */
 } ocpg_bulk;
 
 /* Connection list in BulkConn order (src/BulkConn.cpp:31-117): bId < eId,

@@ -104,6 +104,7 @@ BULK_FIELDS = [
     ("v", "f8", "nb"), ("depth", "f8", "nb"), ("poroInit", "f8", "nb"), ("poro", "f8", "nb"),
     ("poroP", "f8", "nb"), ("rockVp", "f8", "nb"),
     ("ftype", "i4", "nb"), ("iters", "i4", "nb*5"), ("nonconv", "i4", "nb"),
+    ("nj", "f8", "nb*np"), ("res_n", "f8", "nb*np*ncol"), ("resPc", "f8", "nb"),
 ]
 
 
@@ -250,7 +251,7 @@ class Bulk:
         self.nb = int(nb)
         self.NC = int(NC)
         self.nc = self.NC + 1
-        dims = {"nb": self.nb, "nc": self.nc, "np": NP, "NC": self.NC,
+        dims = {"nb": self.nb, "nc": self.nc, "np": NP, "NC": self.NC, "ncol": self.nc + 1,
                 "lend": max_len_dsdp(self.nc)}
         self.arrays = {}
         for name, dt, shape in BULK_FIELDS:

@@ -81,6 +81,15 @@ int orc_phase_density(const ocpg_eos* eos, double P, double T, const double* zi,
     return OCPG_OK;
 }
 
+// xi, rho, mu of one hydrocarbon phase of composition x at P, T.
+int orc_phase_props(const ocpg_eos* eos, double P, double T, const double* x, double* out)
+{
+    if (!eos || !x || !out) return OCPG_EINVAL;
+    oracle::Mixture mix(*eos);
+    mix.PhaseProps(P, T, x, out);
+    return OCPG_OK;
+}
+
 int orc_flash_phases(const ocpg_eos* eos, double P, double T, const double* Ni, double* out)
 {
     // out: ex[3], vj[3], xi[3], rho[3], Nt
@@ -176,6 +185,57 @@ int orc_assemble_wells(const ocpg_eos* eos, const ocpg_wells* wells, const ocpg_
     oracle::WellFlux wf;
     oracle::CalWellFlux(*eos, *wells, *bk, wf);
     oracle::AssembleWells(nc, *conn, *wells, wf, *bk, dt, IA, val);
+    return OCPG_OK;
+}
+
+// IsoT_FIMn: CalFlash / InitFlash (FlashFIMn / InitFlashFIMn + its
+// PassFlashValue; bk->nj, res_n, resPc are outputs too).
+int orc_flash_fimn(const ocpg_eos* eos, ocpg_bulk* bk, int nthreads)
+{
+    if (!eos || !bk || !bk->nj || !bk->res_n || !bk->resPc || eos->nc < 1 || eos->nc > OCPG_MAXNC)
+        return OCPG_EINVAL;
+    oracle::CalFlashN(*eos, *bk, nthreads);
+    return OCPG_OK;
+}
+
+int orc_init_flash_fimn(const ocpg_eos* eos, ocpg_bulk* bk, const double* zi, int nthreads)
+{
+    if (!eos || !bk || !zi || !bk->nj || !bk->res_n || !bk->resPc || eos->nc < 1 || eos->nc > OCPG_MAXNC)
+        return OCPG_EINVAL;
+    oracle::InitFlashN(*eos, *bk, zi, nthreads);
+    return OCPG_OK;
+}
+
+// IsoT_FIMn::AssembleMat: AssembleMatBulksNew, AssembleMatWellsNew (wells
+// may be NULL), then AssembleRhsAccumulate(resAbs): rhs ((nb + open wells)
+// * ncol) = the bulk and perforation terms + resAbs.
+int orc_assemble_fimn(const ocpg_eos* eos, const ocpg_wells* wells, const ocpg_conn* conn, ocpg_bulk* bk,
+                      double dt, const int32_t* upblock, const double* upblock_Rho, const int32_t* IA,
+                      const int32_t* JA, const double* resAbs, double* val, double* rhs)
+{
+    if (!eos || !conn || !bk || !upblock || !upblock_Rho || !IA || !JA || !resAbs || !val || !rhs)
+        return OCPG_EINVAL;
+    const int nc = eos->nc + 1;
+    const int ncol = nc + 1;
+    const int64_t rows = bk->nb + oracle::OpenWells(wells);
+    for (int64_t k = 0; k < rows * ncol; k++) rhs[k] = 0;
+    oracle::AssembleBulks(nc, *conn, *bk, dt, upblock, upblock_Rho, IA, JA, val, 1, true, rhs);
+    if (wells) {
+        oracle::WellFlux wf;
+        oracle::CalWellFlux(*eos, *wells, *bk, wf);
+        oracle::AssembleWells(nc, *conn, *wells, wf, *bk, dt, IA, val, true, rhs);
+    }
+    for (int64_t k = 0; k < rows * ncol; k++) rhs[k] += resAbs[k];
+    return OCPG_OK;
+}
+
+// IsoT_FIMn::GetSolution, bulk part; out[2] = {NRdPmax, NRdNmax}.
+int orc_get_solution_n(int nc, ocpg_bulk* bk, const double* u, double dSmaxlim, const double* scm, double* dSNR,
+                       double* dSNRP, uint8_t* phaseExistOld, double* out)
+{
+    if (!bk || !u || !scm || !dSNR || !dSNRP || !phaseExistOld || !out || !bk->nj || !bk->res_n)
+        return OCPG_EINVAL;
+    oracle::GetSolutionN(nc, *bk, u, dSmaxlim, scm, dSNR, dSNRP, phaseExistOld, &out[0], &out[1]);
     return OCPG_OK;
 }
 

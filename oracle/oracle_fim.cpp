@@ -136,6 +136,60 @@ void InitFlash(const ocpg_eos& eos, const ocpg_bulk& bk, const double* zi, int n
     }
 }
 
+// IsoT_FIMn::PassFlashValue, src/OCPFluidMethod.cpp:3179-3196: IsoT_FIM's,
+// then nj of each existing phase, res (bRowSizedSdP entries) and resPc.
+static void PassFlashValueN(const Mixture& mix, const ocpg_bulk& bk, int64_t n)
+{
+    PassFlashValue(mix, bk, n);
+    const int np = NPH;
+    const int nc = mix.numCom;
+    for (int j = 0; j < np; j++)
+        if (bk.phaseExist[n * np + j]) bk.nj[n * np + j] = todbl(mix.nj[j]);
+    const int len = bk.bRowSizedSdP[n];
+    for (int k = 0; k < len; k++) bk.res_n[n * (np * (nc + 1)) + k] = todbl(mix.resN[k]);
+    bk.resPc[n] = todbl(mix.resPc);
+}
+
+void CalFlashN(const ocpg_eos& eos, const ocpg_bulk& bk, int nthreads)
+{
+    const int NC = eos.nc;
+    const int nc = NC + 1;
+    const int np = NPH;
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : 1)
+    {
+        Mixture mix(eos);
+#pragma omp for schedule(dynamic, 256)
+        for (int64_t n = 0; n < bk.nb; n++) {
+            SkipState s;
+            LoadSkip(bk, NC, n, s);
+            mix.FlashFIMn(bk.P[n], bk.T[n], &bk.Ni[n * nc], &bk.S[n * np], bk.phaseNum[n],
+                          &bk.xij[n * np * nc], s);
+            StoreSkip(bk, NC, n, s);
+            PassFlashValueN(mix, bk, n);
+        }
+    }
+}
+
+void InitFlashN(const ocpg_eos& eos, const ocpg_bulk& bk, const double* zi, int nthreads)
+{
+    const int NC = eos.nc;
+    const int nc = NC + 1;
+    const int np = NPH;
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : 1)
+    {
+        Mixture mix(eos);
+#pragma omp for schedule(dynamic, 256)
+        for (int64_t n = 0; n < bk.nb; n++) {
+            SkipState s;
+            LoadSkip(bk, NC, n, s);
+            mix.InitFlashFIMn(bk.P[n], bk.T[n], &bk.S[n * np], bk.rockVp[n], &zi[n * NC], s);
+            StoreSkip(bk, NC, n, s);
+            for (int i = 0; i < nc; i++) bk.Ni[n * nc + i] = todbl(mix.Ni[i]);
+            PassFlashValueN(mix, bk, n);
+        }
+    }
+}
+
 // OCPTable::Eval_All on column 0 of a 4-column table.
 static void TableEval4(const double (*tab)[4], int nRow, double val, double* out, double* slope)
 {
@@ -456,9 +510,17 @@ static void DaABpbC(int m, int n, int k, const double* A, const double* B, doubl
 
 // IsoT_FIM::AssembleMatBulksNew, src/OCPFluidMethod.cpp:1569-1839, into a BSR
 // whose pattern BsrPattern() produced.  val has IA[nb]*ncol*ncol entries.
+static void DaAxpby(int m, int n, double a, const double* A, const double* x, double b, double* y);
+
+// With fimn, IsoT_FIMn::AssembleMatBulksNew (src/OCPFluidMethod.cpp:
+// 3198-3501): the secondary variables are S and n_ij (x_ij = n_ij / nj, two
+// more terms in the n_ij columns of the upwind side) and rhs (nb * ncol)
+// receives the bulk part of the right-hand side before resAbs is added:
+// -resPc on each volume row, then, in the connection loop's order, each
+// side's -dFdXs res_n where that cell has two hydrocarbon phases.
 void AssembleBulks(int nc, const ocpg_conn& conn, const ocpg_bulk& bk, double dt,
                    const int32_t* upblock, const double* upblock_Rho, const int32_t* IA,
-                   const int32_t* JA, double* val, int nthreads)
+                   const int32_t* JA, double* val, int nthreads, bool fimn, double* rhs)
 {
     const int np = NPH;
     const int ncol = nc + 1;
@@ -481,6 +543,14 @@ void AssembleBulks(int nc, const ocpg_conn& conn, const ocpg_bulk& bk, double dt
     // Per connection (independent): both sides' condensed blocks, written to
     // the two off-diagonal slots the reference's NewOffDiag calls fill
     // ((eId, bId): -b side, (bId, eId): e side).  The diagonals follow.
+    std::vector<double> rhsB, rhsE;  // per connection: each side's -dFdXs res_n
+    std::vector<uint8_t> hasB, hasE;
+    if (fimn) {
+        rhsB.assign((size_t)conn.nconn * ncol, 0.0);
+        rhsE.assign((size_t)conn.nconn * ncol, 0.0);
+        hasB.assign(conn.nconn, 0);
+        hasE.assign(conn.nconn, 0);
+    }
 #pragma omp parallel num_threads(nt)
     {
     std::vector<double> bmat(bsize, 0);
@@ -574,9 +644,12 @@ void AssembleBulks(int nc, const ocpg_conn& conn, const ocpg_bulk& bk, double dt
                             tmp = -transIJ * rhox * dGamma;
                             tmp += xij * transJ * xix * dP;
                             tmp += -transIJ * mux / mu * dP;
+                            if (fimn) tmp -= transIJ * dP / bk.nj[uId_np_j];
                             dFdXsB[(i + 1) * ncolB + jxB + k] += tmp;
                         }
-                        if (i < pVnumComB[j]) dFdXsB[(i + 1) * ncolB + jxB + i] += xi * transJ * dP;
+                        if (i < pVnumComB[j])
+                            dFdXsB[(i + 1) * ncolB + jxB + i] +=
+                                fimn ? xi * transJ * dP / bk.nj[uId_np_j] : xi * transJ * dP;
                     } else {
                         for (int k = 0; k < pVnumComB[j]; k++) {
                             const double rhox = bk.rhox[bId_np_j * nc + k] / 2;
@@ -585,11 +658,14 @@ void AssembleBulks(int nc, const ocpg_conn& conn, const ocpg_bulk& bk, double dt
                             tmp = -transIJ * rhox * dGamma;
                             tmp += xij * transJ * xix * dP;
                             tmp += -transIJ * mux / mu * dP;
+                            if (fimn) tmp -= transIJ * dP / bk.nj[uId_np_j];
                             dFdXsB[(i + 1) * ncolB + jxB + k] += tmp;
                             dFdXsE[(i + 1) * ncolE + jxE + k] +=
                                 -transIJ * bk.rhox[eId_np_j * nc + k] / 2 * dGamma;
                         }
-                        if (i < pVnumComB[j]) dFdXsB[(i + 1) * ncolB + jxB + i] += xi * transJ * dP;
+                        if (i < pVnumComB[j])
+                            dFdXsB[(i + 1) * ncolB + jxB + i] +=
+                                fimn ? xi * transJ * dP / bk.nj[uId_np_j] : xi * transJ * dP;
                     }
                 } else {
                     for (int j1 = 0; j1 < np; j1++) {
@@ -612,9 +688,12 @@ void AssembleBulks(int nc, const ocpg_conn& conn, const ocpg_bulk& bk, double dt
                             tmp = -transIJ * rhox * dGamma;
                             tmp += xij * transJ * xix * dP;
                             tmp += -transIJ * mux / mu * dP;
+                            if (fimn) tmp -= transIJ * dP / bk.nj[uId_np_j];
                             dFdXsE[(i + 1) * ncolE + jxE + k] += tmp;
                         }
-                        if (i < pVnumComE[j]) dFdXsE[(i + 1) * ncolE + jxE + i] += xi * transJ * dP;
+                        if (i < pVnumComE[j])
+                            dFdXsE[(i + 1) * ncolE + jxE + i] +=
+                                fimn ? xi * transJ * dP / bk.nj[uId_np_j] : xi * transJ * dP;
                     } else {
                         for (int k = 0; k < pVnumComE[j]; k++) {
                             const double rhox = bk.rhox[eId_np_j * nc + k] / 2;
@@ -623,16 +702,27 @@ void AssembleBulks(int nc, const ocpg_conn& conn, const ocpg_bulk& bk, double dt
                             tmp = -transIJ * rhox * dGamma;
                             tmp += xij * transJ * xix * dP;
                             tmp += -transIJ * mux / mu * dP;
+                            if (fimn) tmp -= transIJ * dP / bk.nj[uId_np_j];
                             dFdXsE[(i + 1) * ncolE + jxE + k] += tmp;
                             dFdXsB[(i + 1) * ncolB + jxB + k] +=
                                 -transIJ * bk.rhox[bId_np_j * nc + k] / 2 * dGamma;
                         }
-                        if (i < pVnumComE[j]) dFdXsE[(i + 1) * ncolE + jxE + i] += xi * transJ * dP;
+                        if (i < pVnumComE[j])
+                            dFdXsE[(i + 1) * ncolE + jxE + i] +=
+                                fimn ? xi * transJ * dP / bk.nj[uId_np_j] : xi * transJ * dP;
                     }
                 }
             }
             jxB += pVnumComB[j];
             jxE += pVnumComE[j];
+        }
+        if (fimn && bk.phaseNum[bId] > 2) {
+            DaAxpby(ncol, ncolB, -1.0, dFdXsB.data(), &bk.res_n[bId * ncol2], 1.0, &rhsB[(size_t)c * ncol]);
+            hasB[c] = 1;
+        }
+        if (fimn && bk.phaseNum[eId] > 2) {
+            DaAxpby(ncol, ncolE, -1.0, dFdXsE.data(), &bk.res_n[eId * ncol2], 1.0, &rhsE[(size_t)c * ncol]);
+            hasE[c] = 1;
         }
         // b side: AddDiag(bId, bmat) ; NewOffDiag(eId, bId, -bmat)
         bmat = dFdXpB;
@@ -666,6 +756,22 @@ void AssembleBulks(int nc, const ocpg_conn& conn, const ocpg_bulk& bk, double dt
             const int32_t pos = isB ? cc.posE[c] : cc.posB[c];
             const double* o = &val[(size_t)(IA[m] + 1 + pos) * bsize];
             for (int k = 0; k < bsize; k++) d[k] += -o[k];
+        }
+    }
+    // The right-hand side's bulk part, each row's terms in connection order.
+    if (fimn && rhs) {
+#pragma omp parallel for num_threads(nt) schedule(static)
+        for (int64_t n = 0; n < nb; n++) {
+            double* b = &rhs[n * ncol];
+            for (int i = 0; i < ncol; i++) b[i] = 0;
+            b[0] += -bk.resPc[n];
+            for (int64_t q = cc.ptr[n]; q < cc.ptr[n + 1]; q++) {
+                const int64_t c = cc.conn[q];
+                const bool isB = conn.bId[c] == n;
+                if (isB ? !hasB[c] : !hasE[c]) continue;
+                const double* t = isB ? &rhsB[(size_t)c * ncol] : &rhsE[(size_t)c * ncol];
+                for (int i = 0; i < ncol; i++) b[i] += t[i];
+            }
         }
     }
     (void)JA;
@@ -741,6 +847,84 @@ void GetSolution(int nc, const ocpg_bulk& bk, const double* u, double dSmaxlim, 
         bk.P[n] += dP;
         for (int i = 0; i < nc; i++) {
             const double dN = u[n * col + 1 + i] * chopmin;
+            if (std::fabs(*NRdNmax) < std::fabs(dN) / bk.Nt[n]) *NRdNmax = dN / bk.Nt[n];
+            bk.Ni[n * nc + i] += dN;
+        }
+    }
+}
+
+// IsoT_FIMn::GetSolution, bulk part (src/OCPFluidMethod.cpp:3847-3990):
+// the secondary changes res_n + dSec_dPri u, the saturation chop (limit,
+// critical saturations, non-negativity), the updates of nj and x_ij from
+// the phase moles, P and Ni.  Saturations are not updated (the next flash
+// sets them); dSNRP keeps the predicted changes.  The reference takes the
+// count of n_ij rows of phase j from bk.pVnumCom[j], i.e. from cell 0's
+// entry for every cell (:3933-3936); restated as such.
+void GetSolutionN(int nc, const ocpg_bulk& bk, const double* u, double dSmaxlim, const double* scm, double* dSNR,
+                  double* dSNRP, uint8_t* phaseExistOld, double* NRdPmax, double* NRdNmax)
+{
+    const int np = NPH;
+    const int row = np * (nc + 1);
+    const int col = nc + 1;
+    const int lenSdP = maxLendSdP(nc);
+    std::vector<double> dtmp(row, 0.0), tmpNij(np * nc, 0.0);
+    *NRdPmax = 0;
+    *NRdNmax = 0;
+    for (int64_t k = 0; k < bk.nb * np; k++) {
+        dSNR[k] = bk.S[k];
+        phaseExistOld[k] = bk.phaseExist[k];
+    }
+    for (int64_t n = 0; n < bk.nb; n++) {
+        const double dP = u[n * col];
+        bk.P[n] += dP;
+        if (std::fabs(*NRdPmax) < std::fabs(dP)) *NRdPmax = dP;
+        double dSmax = 0;
+        double chop = 1;
+        const int cNp = bk.phaseNum[n];
+        const int len = bk.bRowSizedSdP[n];
+        for (int k = 0; k < len; k++) dtmp[k] = bk.res_n[n * row + k];
+        DaAxpby(len, col, 1.0, &bk.dSec_dPri[n * lenSdP], u + n * col, 1.0, dtmp.data());
+        int js = 0;
+        for (int j = 0; j < np; j++) {
+            if (bk.pSderExist[n * np + j]) {
+                dSmax = std::max(std::fabs(dtmp[js]), dSmax);
+                js++;
+            }
+        }
+        if (dSmax > dSmaxlim) {
+            chop = std::min(chop, dSmaxlim / dSmax);
+            dSmax = dSmaxlim;
+        }
+        js = 0;
+        for (int j = 0; j < np; j++) {
+            const int64_t k = n * np + j;
+            if (bk.pSderExist[k]) {
+                if (std::fabs(bk.S[k] - scm[j]) > TINY && (bk.S[k] - scm[j]) / (chop * dtmp[js]) < 0)
+                    chop *= std::min(1.0, -((bk.S[k] - scm[j]) / (chop * dtmp[js])));
+                if (bk.S[k] + chop * dtmp[js] < 0) chop *= std::min(1.0, -bk.S[k] / (chop * dtmp[js]));
+                js++;
+            }
+        }
+        std::fill(tmpNij.begin(), tmpNij.end(), 0.0);
+        js = 0;
+        int jx = cNp;
+        for (int j = 0; j < np; j++) {
+            const int64_t k = n * np + j;
+            if (bk.pSderExist[k]) {
+                dSNRP[k] = chop * dtmp[js];
+                js++;
+                // Daxpy(nc, nj, xij, tmpNij)
+                for (int i = 0; i < nc; i++) tmpNij[j * nc + i] += bk.nj[k] * bk.xij[k * nc + i];
+                for (int i = 0; i < bk.pVnumCom[j]; i++) tmpNij[j * nc + i] += chop * dtmp[jx + i];
+                jx += bk.pVnumCom[j];
+                double s1 = 0;  // Dnorm1
+                for (int i = 0; i < nc; i++) s1 += std::fabs(tmpNij[j * nc + i]);
+                bk.nj[k] = s1;
+                for (int i = 0; i < nc; i++) bk.xij[k * nc + i] = tmpNij[j * nc + i] / bk.nj[k];
+            }
+        }
+        for (int i = 0; i < nc; i++) {
+            const double dN = u[n * col + 1 + i] * chop;
             if (std::fabs(*NRdNmax) < std::fabs(dN) / bk.Nt[n]) *NRdNmax = dN / bk.Nt[n];
             bk.Ni[n * nc + i] += dN;
         }

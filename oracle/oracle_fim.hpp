@@ -56,16 +56,29 @@ struct CellConn {
 CellConn CellConnections(int64_t nb, const ocpg_conn& conn);
 void BsrPattern(int64_t nb, const ocpg_conn& conn, int32_t* IA, int32_t* JA,
                 const ocpg_wells* wells = nullptr);
+// With fimn, IsoT_FIMn::AssembleMatWellsNew (src/OCPFluidMethod.cpp:
+// 3503-3845); rhs (nb * ncol, holding the bulk part) receives each
+// perforation's -dQdXs res_n where its cell has two hydrocarbon phases.
 void AssembleWells(int nc, const ocpg_conn& conn, const ocpg_wells& w, const WellFlux& wf,
-                   const ocpg_bulk& bk, double dt, const int32_t* IA, double* val);
+                   const ocpg_bulk& bk, double dt, const int32_t* IA, double* val, bool fimn = false,
+                   double* rhs = nullptr);
 void AssembleBulks(int nc, const ocpg_conn& conn, const ocpg_bulk& bk, double dt,
                    const int32_t* upblock, const double* upblock_Rho, const int32_t* IA,
-                   const int32_t* JA, double* val, int nthreads = 1);
+                   const int32_t* JA, double* val, int nthreads = 1, bool fimn = false,
+                   double* rhs = nullptr);
 
 // IsoT_FIM::GetSolution (bulk part) and the Newton-change bookkeeping of
 // the next PassFlashValue / Bulk::CalNRdSmax.
 void GetSolution(int nc, const ocpg_bulk& bk, const double* u, double dSmaxlim, double* dSNR, double* dSNRP,
                  uint8_t* phaseExistOld, double* NRdPmax, double* NRdNmax);
+// IsoT_FIMn::GetSolution, bulk part (src/OCPFluidMethod.cpp:3847-3990);
+// scm: the critical saturation of each phase (Bulk::satcm).
+void GetSolutionN(int nc, const ocpg_bulk& bk, const double* u, double dSmaxlim, const double* scm, double* dSNR,
+                  double* dSNRP, uint8_t* phaseExistOld, double* NRdPmax, double* NRdNmax);
+// IsoT_FIMn::CalFlash / InitFlash with IsoT_FIMn::PassFlashValue
+// (src/OCPFluidMethod.cpp:3129-3196).
+void CalFlashN(const ocpg_eos& eos, const ocpg_bulk& bk, int nthreads);
+void InitFlashN(const ocpg_eos& eos, const ocpg_bulk& bk, const double* zi, int nthreads);
 void NRChange(int64_t nb, const double* S, double* dSNR, const double* dSNRP, const uint8_t* phaseExistOld,
               double* out);
 

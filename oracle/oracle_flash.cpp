@@ -1884,7 +1884,7 @@ int SymIndefSolve(int n, real* a, real* b)
 int LUSolveColMajor(int n, int nrhs, real* a, real* b)
 {
 #define A_(r, c) a[(r) + (c)*n]
-    int ipiv[MAXNC];
+    int ipiv[NPH + NPMAX * MAXNC];
     int info = 0;
     for (int j = 0; j < n; j++) {
         int p = j + iamax(n - j, &A_(j, j), 1);

@@ -48,6 +48,15 @@ public:
     // MixtureComp::InitFlashFIM, src/MixtureComp.cpp:261-353.
     void InitFlashFIM(double Pin, double Tin, const double* Sjin, double Vpore,
                       const double* Ziin, SkipState& skip);
+    // MixtureComp::FlashFIMn, src/MixtureComp.cpp:575-695 (the IsoT_FIMn
+    // flash: secondary variables S and n_ij).  The reference reads its
+    // njin / phaseExistin arguments only for a branch it never takes
+    // (`inputNP == 1 || OCP_TRUE`, :592), so they are not arguments here.
+    void FlashFIMn(double Pin, double Tin, const double* Niin, const double* Sjin,
+                   int lastNP, const double* xijin, SkipState& skip);
+    // MixtureComp::InitFlashFIMn, src/MixtureComp.cpp:355-442.
+    void InitFlashFIMn(double Pin, double Tin, const double* Sjin, double Vpore,
+                       const double* Ziin, SkipState& skip);
 
     // MixtureComp::CalProdWeight, src/MixtureComp.cpp:830-853, on the
     // surface flash MixtureComp::Flash (:167-193).  A hydrocarbon phase the
@@ -61,6 +70,11 @@ public:
                        double* rhoo);
     // MixtureComp::CalProdRate (:855-865), absent phases contributing zero.
     void ProdRate(double Pin, double Tin, const double* Niin, double* rate);
+    // xi, rho and LBC mu of one hydrocarbon phase of composition x (HC
+    // mole fractions) at P, T (degF): the EoS root SolEoS picks, CalMW,
+    // CalVfXiRho, CalViscoLBC on that phase alone.  Test helper (the finite
+    // differences that pin the phase-property derivatives).
+    void PhaseProps(double Pin, double Tin, const double* x, double* out3);
     // MixtureComp::XiPhase / RhoPhase (:711-768); phase 2 is water.
     void PhaseDensity(double Pin, double Tin, const double* Ziin, int phase, double* xio, double* rhoo);
 
@@ -81,6 +95,11 @@ public:
     int pVnumCom[NPH];
     int ftype;
     real ePEC;
+    // IsoT_FIMn outputs (MixtureComp::res / resPc, src/MixtureComp.cpp:
+    // 4495-4632): the secondary variables' correction in dXsdXp's row order,
+    // and its volume-balance term.
+    real resN[NPH * (MAXNC + 2)];
+    real resPc;
     long iters[5];  // SSMSTA, NRSTA, SSMSP, NRSP, RR for the last call
 
 private:
@@ -126,6 +145,9 @@ private:
     real fugP[NPMAX][MAXNC], Zp[NPMAX];
     real JmatDer[MAXNC * MAXNC], rhsDer[MAXNC * (MAXNC + 1)];
     real vjp[NPH], vji[NPH][MAXNC + 1];
+    // CaldXsdXp03's system: dimension NP + 1 + NP * NC, NC + 3 right-hand sides
+    static constexpr int NDER = NPH + NPMAX * MAXNC;
+    real JmatDerN[NDER * NDER], JmatTmpN[NDER * NDER], rhsDerN[NDER * (MAXNC + 3)];
     real xixC[NPMAX * MAXNC], xiPC[NPMAX];
     bool flagSkip;
     real phiN[MAXNC * MAXNC];
@@ -179,6 +201,14 @@ private:
     void SetDerivFlags();
     void CalPhiNSkip();
     void CalSkipForNextStep(SkipState& skip);
+    // IsoT_FIMn derivatives, src/MixtureComp.cpp:2581-2757, 4495-4853.
+    void CalVjpVfpVfn_partial();
+    void CaldXsdXpAPI03();
+    void CaldXsdXp03();
+    void CalXiPn_partial();
+    void CalRhoPn_partial();
+    void CalMuPnLBC_partial();
+    void CalVfiVfp_full03();
     void PVTWEval(real Pv, real* data, real* cdata) const;
 };
 

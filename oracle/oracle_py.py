@@ -52,6 +52,11 @@ class Oracle:
         L.orc_nr_change.argtypes = [C.c_int64, P, P, P, P, P]
         L.orc_time_flash_fim.argtypes = [P, P, C.c_int]
         L.orc_time_flash_fim.restype = C.c_double
+        L.orc_phase_props.argtypes = [P, C.c_double, C.c_double, P, P]
+        L.orc_flash_fimn.argtypes = [P, P, C.c_int]
+        L.orc_init_flash_fimn.argtypes = [P, P, P, C.c_int]
+        L.orc_assemble_fimn.argtypes = [P, P, P, P, C.c_double, P, P, P, P, P, P, P]
+        L.orc_get_solution_n.argtypes = [C.c_int, P, P, C.c_double, P, P, P, P, P]
         if hasattr(L, "orc_count_flash_fim"):
             L.orc_count_flash_fim.argtypes = [P, P, P]
 
@@ -69,6 +74,58 @@ class Oracle:
                                        book["dSNRP"].ctypes.data, book["pexOld"].ctypes.data, out.ctypes.data)
         assert rc == 0
         return book, (out[0], out[1])
+
+    def get_solution_n(self, nc, bulk, u, dSmaxlim, scm):
+        """IsoT_FIMn::GetSolution on `bulk` (updated in place); scm: the
+        critical saturation per phase.  Returns the bookkeeping for
+        nr_change and (NRdPmax, NRdNmax)."""
+        nb = bulk.nb
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        scm = np.ascontiguousarray(scm, dtype=np.float64)
+        book = dict(dSNR=np.zeros(nb * abi.NP), dSNRP=np.zeros(nb * abi.NP),
+                    pexOld=np.zeros(nb * abi.NP, dtype=np.uint8))
+        out = np.zeros(2)
+        s = bulk.cstruct()
+        rc = self.lib.orc_get_solution_n(nc, C.byref(s), u.ctypes.data, dSmaxlim, scm.ctypes.data,
+                                         book["dSNR"].ctypes.data, book["dSNRP"].ctypes.data,
+                                         book["pexOld"].ctypes.data, out.ctypes.data)
+        assert rc == 0
+        return book, (out[0], out[1])
+
+    def phase_props(self, eos, P, T, x):
+        """(xi, rho, mu) of one hydrocarbon phase of composition x."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros(3)
+        assert self.lib.orc_phase_props(C.byref(eos), P, T, x.ctypes.data, out.ctypes.data) == 0
+        return out
+
+    def flash_fimn(self, eos, bulk, nthreads=1):
+        s = bulk.cstruct()
+        rc = self.lib.orc_flash_fimn(C.byref(eos), C.byref(s), nthreads)
+        assert rc == 0, rc
+
+    def init_flash_fimn(self, eos, bulk, zi, nthreads=1):
+        zi = np.ascontiguousarray(zi, dtype=np.float64)
+        s = bulk.cstruct()
+        rc = self.lib.orc_init_flash_fimn(C.byref(eos), C.byref(s), zi.ctypes.data, nthreads)
+        assert rc == 0, rc
+
+    def assemble_fimn(self, eos, grid, bulk, dt, up, urho, IA, JA, resAbs, wells=None):
+        """IsoT_FIMn::AssembleMat (bulks, then wells if given, then
+        AssembleRhsAccumulate(resAbs)): (val, rhs)."""
+        conn = abi.conn_struct(grid.bId, grid.eId, grid.area)
+        ncol = bulk.nc + 1
+        rows = len(IA) - 1
+        val = np.zeros(int(IA[-1]) * ncol * ncol)
+        rhs = np.zeros(rows * ncol)
+        resAbs = np.ascontiguousarray(resAbs, dtype=np.float64)
+        s = bulk.cstruct()
+        ws = wells.cstruct() if wells is not None else None
+        rc = self.lib.orc_assemble_fimn(C.byref(eos), C.byref(ws) if ws is not None else None, C.byref(conn),
+                                        C.byref(s), dt, up.ctypes.data, urho.ctypes.data, IA.ctypes.data,
+                                        JA.ctypes.data, resAbs.ctypes.data, val.ctypes.data, rhs.ctypes.data)
+        assert rc == 0
+        return val, rhs
 
     def nr_change(self, bulk, book):
         """dSNR (S change over the Newton iteration), NRdSmax and maxNRdSSP
@@ -293,6 +350,8 @@ class Reference:
         L.ref_prod_weight.argtypes = [P, C.c_double, C.c_double, P, P, P]
         L.ref_phase_density.argtypes = [P, C.c_double, C.c_double, P, C.c_int, P]
         L.ref_flash_phases.argtypes = [P, C.c_double, C.c_double, P, P]
+        L.ref_flash_fimn.argtypes = [P, P]
+        L.ref_init_flash_fimn.argtypes = [P, P, P]
         self.deck = deck
         self.h = L.ref_create(C.byref(deck), nb)
 
@@ -309,6 +368,15 @@ class Reference:
         zi = np.ascontiguousarray(zi, dtype=np.float64)
         s = bulk.cstruct()
         assert self.lib.ref_init_flash_fim(self.h, C.byref(s), zi.ctypes.data) == 0
+
+    def flash_fimn(self, bulk):
+        s = bulk.cstruct()
+        assert self.lib.ref_flash_fimn(self.h, C.byref(s)) == 0
+
+    def init_flash_fimn(self, bulk, zi):
+        zi = np.ascontiguousarray(zi, dtype=np.float64)
+        s = bulk.cstruct()
+        assert self.lib.ref_init_flash_fimn(self.h, C.byref(s), zi.ctypes.data) == 0
 
     def prod_weight(self, Psurf, Tsurf, Ni, prod_phase):
         Ni = np.ascontiguousarray(Ni, dtype=np.float64)

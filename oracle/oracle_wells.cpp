@@ -5,7 +5,8 @@
 //                   ReCalXi = false (:204-258), Well::CalProdWeight (:829-862)
 //   AssembleWells : IsoT_FIM::AssembleMatWellsNew (src/OCPFluidMethod.cpp:
 //                   2417-2436) -> AssembleMatInjWellsNew (:2438-2580),
-//                   AssembleMatProdWellsNew (:2582-2733)
+//                   AssembleMatProdWellsNew (:2582-2733); IsoT_FIMn's
+//                   (:3503-3845) with fimn
 //   PrepareWells  : AllWells::PrepareWell (src/AllWells.cpp:201-214):
 //                   Well::CalTrans, CaldG (CalInjdG :367-435, CalProddG01
 //                   :438-555), CheckOptMode (:893-944), CalFlux's ReCalXi
@@ -402,7 +403,7 @@ static void DaABpbCw(int m, int n, int k, const double* A, const double* B, doub
 }
 
 void AssembleWells(int nc, const ocpg_conn& conn, const ocpg_wells& W, const WellFlux& wf,
-                   const ocpg_bulk& bk, double dt, const int32_t* IA, double* val)
+                   const ocpg_bulk& bk, double dt, const int32_t* IA, double* val, bool fimn, double* rhs)
 {
     const int np = NPH;
     const int ncol = nc + 1;
@@ -503,14 +504,28 @@ void AssembleWells(int nc, const ocpg_conn& conn, const ocpg_wells& W, const Wel
                             }
                         }
                         for (int k = 0; k < pVnumComB[j]; k++) {
-                            const double tmp = dP * transj[j] * xij *
-                                               (bk.xix[n_np_j * nc + k] - xi / mu * bk.mux[n_np_j * nc + k]);
+                            double tmp = dP * transj[j] * xij *
+                                         (bk.xix[n_np_j * nc + k] - xi / mu * bk.mux[n_np_j * nc + k]);
+                            if (fimn) tmp -= transIJ * dP / bk.nj[n_np_j];
                             dQdXsB[(i + 1) * ncolB + jxB + k] += tmp;
                         }
-                        if (i < pVnumComB[j]) dQdXsB[(i + 1) * ncolB + jxB + i] += transj[j] * xi * dP;
+                        if (i < pVnumComB[j])
+                            dQdXsB[(i + 1) * ncolB + jxB + i] +=
+                                fimn ? transj[j] * xi * dP / bk.nj[n_np_j] : transj[j] * xi * dP;
                     }
                     jxB += pVnumComB[j];
                 }
+            }
+            // IsoT_FIMn: AddRhs(n, -dQdXsB res_n) where the cell has two
+            // hydrocarbon phases (src/OCPFluidMethod.cpp:3610-3616, 3780-3786)
+            if (fimn && rhs && bk.phaseNum[n] > 2) {
+                std::vector<double> tmpb(ncol, 0.0);
+                for (int r = 0; r < ncol; r++) {
+                    tmpb[r] = 1.0 * tmpb[r];
+                    for (int c = 0; c < ncolB; c++)
+                        tmpb[r] += -1.0 * dQdXsB[r * ncolB + c] * bk.res_n[n * ncol2 + c];
+                }
+                for (int r = 0; r < ncol; r++) rhs[n * ncol + r] += tmpb[r];
             }
             // Bulk - Bulk: AddDiag(n, dt * (dQdXpB + dQdXsB * dSec))
             bmat = dQdXpB;

@@ -261,6 +261,68 @@ int ref_init_flash_fim(void* p, ocpg_bulk* bk, const double* zi)
     return 0;
 }
 
+// IsoT_FIMn::PassFlashValue (src/OCPFluidMethod.cpp:3179-3196): IsoT_FIM's,
+// then nj, res and resPc.
+static void pass_value_n(RefHandle* h, ocpg_bulk* bk, int64_t n, const unsigned long long* it0)
+{
+    MixtureComp& m = *h->mix;
+    pass_value(h, bk, n, it0);
+    const int np = 3;
+    const int nc = h->NC + 1;
+    for (int j = 0; j < np; j++)
+        if (bk->phaseExist[n * np + j]) bk->nj[n * np + j] = m.GetNj(j);
+    const vector<OCP_DBL>& r = m.GetRes();
+    for (int k = 0; k < bk->bRowSizedSdP[n]; k++) bk->res_n[n * (np * (nc + 1)) + k] = r[k];
+    bk->resPc[n] = m.GetResPc();
+}
+
+// IsoT_FIMn::CalFlash (src/OCPFluidMethod.cpp:3152-3177) around the
+// reference's MixtureComp::FlashFIMn.
+int ref_flash_fimn(void* p, ocpg_bulk* bk)
+{
+    Quiet q;
+    RefHandle* h = static_cast<RefHandle*>(p);
+    if (bk->nb > h->nb || !bk->nj || !bk->res_n || !bk->resPc) return -1;
+    MixtureComp& m = *h->mix;
+    const int np = 3;
+    const int nc = h->NC + 1;
+    std::vector<USI> flagB(np, 0);
+    for (int64_t n = 0; n < bk->nb; n++) {
+        seed_skip(h, bk, n);
+        for (int j = 0; j < np; j++) flagB[j] = bk->phaseExist[n * np + j];
+        unsigned long long it0[5] = {m.itersSSMSTA, m.itersNRSTA, m.itersSSMSP, m.itersNRSP,
+                                     m.itersRR};
+        m.FlashFIMn(bk->P[n], bk->T[n], &bk->Ni[n * nc], &bk->S[n * np], &bk->xij[n * np * nc],
+                    &bk->nj[n * np], flagB.data(), (USI)bk->phaseNum[n], (OCP_USI)n);
+        read_skip(h, bk, n);
+        pass_value_n(h, bk, n, it0);
+    }
+    return 0;
+}
+
+// IsoT_FIMn::InitFlash (src/OCPFluidMethod.cpp:3129-3150).
+int ref_init_flash_fimn(void* p, ocpg_bulk* bk, const double* zi)
+{
+    Quiet q;
+    RefHandle* h = static_cast<RefHandle*>(p);
+    if (bk->nb > h->nb || !bk->nj || !bk->res_n || !bk->resPc) return -1;
+    MixtureComp& m = *h->mix;
+    const int np = 3;
+    const int NC = h->NC;
+    const int nc = NC + 1;
+    for (int64_t n = 0; n < bk->nb; n++) {
+        seed_skip(h, bk, n);
+        unsigned long long it0[5] = {m.itersSSMSTA, m.itersNRSTA, m.itersSSMSP, m.itersNRSP,
+                                     m.itersRR};
+        m.InitFlashFIMn(bk->P[n], 0.0, bk->T[n], &bk->S[n * np], bk->rockVp[n], &zi[n * NC],
+                        (OCP_USI)n);
+        for (int i = 0; i < nc; i++) bk->Ni[n * nc + i] = m.GetNi(i);
+        read_skip(h, bk, n);
+        pass_value_n(h, bk, n, it0);
+    }
+    return 0;
+}
+
 // MixtureComp::CalProdWeight (src/MixtureComp.cpp:830-853) on the
 // reference's own object.  The phase volumes vj start from zero, as in a
 // freshly allocated object, so a phase absent at surface conditions

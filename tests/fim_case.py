@@ -7,9 +7,10 @@ from opencaeporo_amd import abi, fluids, workloads
 from oracle.oracle_py import Oracle
 
 
-def build_case(nx=8, ny=6, nz=5, seed=7, deck=None, dP=20.0, dN=0.01, grid=None):
+def build_case(nx=8, ny=6, nz=5, seed=7, deck=None, dP=20.0, dN=0.01, grid=None, method="fim"):
     """grid: None (orthogonal 7-point) or "faulted" (workloads.faulted_grid,
-    non-neighbour connections across thrown columns)."""
+    non-neighbour connections across thrown columns).  method "fimn": the
+    IsoT_FIMn flash (InitFlashFIMn / FlashFIMn)."""
     orc = Oracle()
     from opencaeporo_amd import lib
     deck = deck or fluids.spe5_deck()
@@ -31,17 +32,18 @@ def build_case(nx=8, ny=6, nz=5, seed=7, deck=None, dP=20.0, dN=0.01, grid=None)
     b.depth[:] = grid.depth
     b.poroInit[:] = 0.3
     orc.calrock(rock, b)
-    orc.init_flash_fim(eos, b, z)
+    fimn = method == "fimn"
+    (orc.init_flash_fimn if fimn else orc.init_flash_fim)(eos, b, z)
     orc.calkrpc(sat, b)
     # last time step, then a Newton-like update of the primary variables
     b.lNi[:] = b.Ni
     rng = np.random.default_rng(seed + 2)
     b.P[:] = b.P + rng.uniform(-dP, dP, nb)
     b.Ni[:] = b.Ni * (1 + rng.uniform(-dN, dN, b.Ni.size))
-    orc.flash_fim(eos, b)
+    (orc.flash_fimn if fimn else orc.flash_fim)(eos, b)
     orc.calkrpc(sat, b)
     orc.calrock(rock, b)
-    return dict(eos=eos, sat=sat, rock=rock, grid=grid, bulk=b, NC=NC, orc=orc)
+    return dict(eos=eos, sat=sat, rock=rock, grid=grid, bulk=b, NC=NC, orc=orc, method=method)
 
 
 def add_wells(case, spec=None, seed=11):

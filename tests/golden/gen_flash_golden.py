@@ -7,7 +7,9 @@ deck, the EoS description the reference constructor derived, and for each
 pass (cold: lastNP = 0 and empty skip memory; warm: the cold result with
 P + U[-5, 5] psia) every array PassFlashValue hands to the Bulk.
 
-    python tests/golden/gen_flash_golden.py
+    python tests/golden/gen_flash_golden.py [--fimn]
+
+(--fimn writes only the IsoT_FIMn files.)
 """
 import copy
 import ctypes as C
@@ -72,15 +74,20 @@ def snapshot(b, fields, prefix):
     return out
 
 
-def gen_flash(name, deck, nb, seeds, dp_seed, limits=None):
+FIMN_FIELDS = ["nj", "res_n", "resPc"]
+
+
+def gen_flash(name, deck, nb, seeds, dp_seed, limits=None, method="fim"):
     """limits: (SSMSTA, NRSTA, SSMSP, NRSP) iteration caps replacing the
     deck's (the non-convergence paths); such files also record the per-cell
-    non-convergence flag."""
-    out_fields = OUT_FIELDS
+    non-convergence flag.  method "fimn": the IsoT_FIMn flash (FlashFIMn),
+    whose files also record nj, res_n and resPc."""
+    out_fields = OUT_FIELDS + (FIMN_FIELDS if method == "fimn" else [])
+    flash = "flash_fimn" if method == "fimn" else "flash_fim"
     if limits is not None:
         deck = copy.copy(deck)
         deck.ssmsta_maxit, deck.nrsta_maxit, deck.ssmsp_maxit, deck.nrsp_maxit = limits
-        out_fields = OUT_FIELDS + ["nonconv"]
+        out_fields = out_fields + ["nonconv"]
     NC = deck.nc
     ref = Reference(deck, nb)
     P, z, Sw = workloads.cell_inputs(nb, NC, seed=seeds[0])
@@ -93,12 +100,12 @@ def gen_flash(name, deck, nb, seeds, dp_seed, limits=None):
     data.update(deck_arrays(deck))
     data.update(eos_arrays(ref.eos()))
     data.update(snapshot(b, IN_FIELDS, "cold_in_"))
-    ref.flash_fim(b)
+    getattr(ref, flash)(b)
     data.update(snapshot(b, out_fields, "cold_out_"))
     rng = np.random.default_rng(dp_seed)
     b.P[:] = P + rng.uniform(-5.0, 5.0, nb)
     data.update(snapshot(b, IN_FIELDS, "warm_in_"))
-    ref.flash_fim(b)
+    getattr(ref, flash)(b)
     data.update(snapshot(b, out_fields, "warm_out_"))
     path = os.path.join(HERE, name)
     np.savez_compressed(path, **data)
@@ -106,7 +113,7 @@ def gen_flash(name, deck, nb, seeds, dp_seed, limits=None):
     print(f"{name}: {nb} cells NC={NC}, two-phase {two:.2f}, {os.path.getsize(path)} bytes")
 
 
-def gen_init(name, deck, nb, seeds):
+def gen_init(name, deck, nb, seeds, method="fim"):
     NC = deck.nc
     ref = Reference(deck, nb)
     P, z, Sw = workloads.cell_inputs(nb, NC, seed=seeds[0])
@@ -119,8 +126,12 @@ def gen_init(name, deck, nb, seeds):
     data = dict(seeds=np.array(seeds), init_z=z.copy(), init_rockVp=b.rockVp.copy())
     data.update(deck_arrays(deck))
     data.update(snapshot(b, ["P", "T", "S"], "init_in_"))
-    ref.init_flash_fim(b, z)
-    data.update(snapshot(b, OUT_FIELDS + ["Ni"], "init_out_"))
+    if method == "fimn":
+        ref.init_flash_fimn(b, z)
+        data.update(snapshot(b, OUT_FIELDS + FIMN_FIELDS + ["Ni"], "init_out_"))
+    else:
+        ref.init_flash_fim(b, z)
+        data.update(snapshot(b, OUT_FIELDS + ["Ni"], "init_out_"))
     path = os.path.join(HERE, name)
     np.savez_compressed(path, **data)
     print(f"{name}: {nb} cells NC={NC}, {os.path.getsize(path)} bytes")
@@ -134,9 +145,20 @@ def mkl_version():
         return "unknown"
 
 
+def gen_fimn():
+    """The IsoT_FIMn files (FlashFIMn / InitFlashFIMn)."""
+    gen_flash("flashn_spe5.npz", fluids.spe5_deck(), 256, (111, 112), 113, method="fimn")
+    gen_flash("flashn_ten.npz", fluids.ten_deck(), 128, (311, 312), 313, method="fimn")
+    gen_init("initflashn_spe5.npz", fluids.spe5_deck(), 128, (411, 412), method="fimn")
+
+
 def main():
     if not Reference.available():
         sys.exit("oracle/_ref/libocpref.so is not built (make -C oracle ref)")
+    if "--fimn" in sys.argv:
+        gen_fimn()
+        return
+    gen_fimn()
     gen_flash("flash_spe5.npz", fluids.spe5_deck(), 384, (101, 102), 103)
     gen_flash("flash_bench6.npz", fluids.bench6_deck(), 256, (201, 202), 203)
     gen_flash("flash_ten.npz", fluids.ten_deck(), 192, (301, 302), 303)

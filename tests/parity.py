@@ -139,6 +139,30 @@ def rounding_sensitivity(flash, bulk, perturbations=SENS_PERTURBATIONS):
     return sens
 
 
+def fimn_errs(b, d, prefix):
+    """Per-cell errors of the IsoT_FIMn fields: nj over the existing phases
+    relative to Nt; res_n relative to the larger of its own magnitude and
+    the variable's scale (one for the saturation rows, the first phaseNum;
+    Nt for the n_ij rows); resPc relative to the larger of |resPc| and vf.
+    At a converged split res_n is at rounding level and is compared on the
+    variables' scale; where the split is poorly converged it is large and
+    compared relatively."""
+    nb = b.nb
+    Nt = np.abs(d[prefix + "Nt"]).reshape(nb, 1)
+    vf = np.abs(d[prefix + "vf"])
+    ex = d[prefix + "phaseExist"].reshape(nb, -1) != 0
+    nj = np.where(ex, np.abs(b.nj.reshape(nb, -1) - d[prefix + "nj"].reshape(nb, -1)), 0.0) / Nt
+    rows = d[prefix + "bRowSizedSdP"].reshape(nb)
+    nS = d[prefix + "phaseNum"].reshape(nb)
+    rr = np.arange(b.res_n.size // nb)[None, :]
+    ref = d[prefix + "res_n"].reshape(nb, -1)
+    dr = np.abs(b.res_n.reshape(nb, -1) - ref)
+    dr = np.where(rr < rows[:, None], dr, 0.0)
+    dr = dr / np.maximum(np.abs(ref), np.where(rr < nS[:, None], 1.0, Nt))
+    rpc = np.abs(b.resPc - d[prefix + "resPc"]) / np.maximum(np.abs(d[prefix + "resPc"]), vf)
+    return {"nj": nj.max(axis=1), "res_n": dr.max(axis=1), "resPc": rpc}
+
+
 def compare_state(b, d, prefix, tol, int_exact=True, skip_tol=1e-4, frac=1.0, loose=None, cell_tol=None):
     """Integer fields must match in every cell.  A floating field must be
     within ``tol`` (per-cell scaled error) in at least ``frac`` of the cells
@@ -168,6 +192,14 @@ def compare_state(b, d, prefix, tol, int_exact=True, skip_tol=1e-4, frac=1.0, lo
         worst = np.maximum(worst, err)
         report[f] = float(err.max())
         assert err.max() <= loose, f"{f}: max scaled error {err.max():.3e} > {loose:.1e} at cell {err.argmax()}"
+    if prefix + "res_n" in d:
+        # IsoT_FIMn: nj of the existing phases and the flash's correction,
+        # scaled by the cell's moles (n_ij rows) or by one (saturation rows)
+        err = fimn_errs(b, d, prefix)
+        for f, e in err.items():
+            worst = np.maximum(worst, e)
+            report[f] = float(e.max())
+            assert e.max() <= loose, f"{f}: max scaled error {e.max():.3e} > {loose:.1e} at cell {e.argmax()}"
     within = float(np.mean(worst <= tol))
     report["frac_within_tol"] = within
     report["cells_beyond_tol"] = [int(i) for i in np.flatnonzero(worst > tol)]

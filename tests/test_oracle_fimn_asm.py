@@ -1,0 +1,169 @@
+"""Pins the oracle's IsoT_FIMn residual system (AssembleMatBulksNew /
+AssembleMatWellsNew of IsoT_FIMn, src/OCPFluidMethod.cpp:3198-3845, and
+GetSolution, :3847-3990) against mathematics and against the IsoT_FIM
+system:
+
+* each Jacobian column equals the central finite difference of the
+  residual (CalRes) when one primary variable is perturbed and the FIMn
+  flash, kr/Pc and rock are re-evaluated (exact-LBC variant, as for FIM);
+* the right-hand side's secondary-variable terms (-dFdXs res_n per
+  connection side and perforation, -resPc) are the Jacobian's own
+  secondary parts: with res_n replaced by a column c >= 1 of dSec_dPri
+  they equal that column of the off-diagonal blocks (signed by the side)
+  and of the perforation shares of the diagonal, over dt;
+* GetSolution moves n_ij by res_n + dSec_dPri u: the new phase moles
+  summed over the phases change by exactly the moles u adds, and a
+  one-phase cell's n_i follow N_i.
+"""
+import numpy as np
+import pytest
+
+from opencaeporo_amd import abi
+from oracle.oracle_py import EXACT_SO, Oracle
+from tests import fim_case
+
+DT = 5.0
+
+
+def props(orc, case, b):
+    w = b.copy()
+    orc.flash_fimn(case["eos"], w)
+    orc.calkrpc(case["sat"], w)
+    orc.calrock(case["rock"], w)
+    return w
+
+
+def system(orc, case, b, wells=None):
+    nc = case["NC"] + 1
+    ncol = nc + 1
+    grid = case["grid"]
+    if wells is None:
+        res, up, urho, _, _ = orc.calres(nc, grid, b, DT)
+        IA, JA = orc.bsr_pattern(grid)
+    else:
+        res, up, urho, _, _ = orc.calres_wells(case["eos"], wells, grid, b, DT)
+        IA, JA = orc.bsr_pattern_wells(grid, wells)
+    val, rhs = orc.assemble_fimn(case["eos"], grid, b, DT, up, urho, IA, JA, res, wells=wells)
+    return -res, IA, JA, val.reshape(-1, ncol, ncol), rhs
+
+
+@pytest.mark.parametrize("shape,seed", [((3, 3, 3), 5), ((4, 2, 3), 9)])
+def test_fimn_jacobian_matches_finite_differences(shape, seed):
+    case = fim_case.build_case(*shape, seed=seed, dP=2.0, dN=0.001, method="fimn")
+    orc = Oracle(EXACT_SO)
+    base = props(orc, case, case["bulk"])
+    assert np.any(base.phaseNum == 3) and np.any(base.phaseNum == 2)
+    R, IA, JA, val, _ = system(orc, case, base)
+    nb = base.nb
+    nc = case["NC"] + 1
+    ncol = nc + 1
+    worst = 0.0
+    for cell in range(nb):
+        for col in range(ncol):
+            xp, xm = base.copy(), base.copy()
+            if col == 0:
+                h = 1e-4 * base.P[cell]
+                xp.P[cell] += h
+                xm.P[cell] -= h
+            else:
+                k = cell * nc + col - 1
+                h = 1e-4 * max(base.Ni[k], 1e-3)
+                xp.Ni[k] += h
+                xm.Ni[k] -= h
+            fd = (system(orc, case, props(orc, case, xp))[0] -
+                  system(orc, case, props(orc, case, xm))[0]) / (2 * h)
+            fd = fd.reshape(nb, ncol)
+            floor = max(1e-6 * np.abs(fd).max(), 1e-12 * np.abs(R).max() / h)
+            for r in range(nb):
+                ks = [k for k in range(IA[r], IA[r + 1]) if JA[k] == cell]
+                if not ks:
+                    assert np.abs(fd[r]).max() <= 1e-8 * np.abs(fd).max()
+                    continue
+                j = val[ks[0]][:, col]
+                scale = max(np.abs(j).max(), np.abs(fd[r]).max(), floor, 1e-300)
+                worst = max(worst, np.abs(j - fd[r]).max() / scale)
+    assert worst < 1e-4, worst
+
+
+@pytest.mark.parametrize("with_wells", [False, True])
+def test_fimn_rhs_terms_are_the_jacobians_secondary_parts(with_wells):
+    case = fim_case.build_case(4, 3, 3, seed=13, method="fimn")
+    orc = Oracle()
+    b = case["bulk"].copy()
+    wells = fim_case.add_wells(case) if with_wells else None
+    nb, nc = b.nb, case["NC"] + 1
+    ncol = nc + 1
+    L = abi.max_len_dsdp(nc)
+    _, IA, JA, val, _ = system(orc, case, b, wells)
+    rows = len(IA) - 1
+    for c in (1, 3):
+        t = b.copy()
+        D = t.dSec_dPri.reshape(nb, L)
+        R = t.res_n.reshape(nb, -1)
+        R[:] = 0
+        for n in range(nb):
+            k = int(t.bRowSizedSdP[n])
+            R[n, :k] = D[n, :k * ncol].reshape(k, ncol)[:, c]
+        t.resPc[:] = 0
+        grid = case["grid"]
+        if wells is None:
+            _, up, urho, _, _ = orc.calres(nc, grid, t, DT)
+        else:
+            _, up, urho, _, _ = orc.calres_wells(case["eos"], wells, grid, t, DT)
+        val2, rhs = orc.assemble_fimn(case["eos"], grid, t, DT, up, urho, IA, JA, np.zeros(rows * ncol),
+                                      wells=wells)
+        assert np.array_equal(val2.reshape(val.shape), val)
+        rhs = rhs.reshape(rows, ncol)
+        vfi = t.vfi.reshape(nb, nc)
+        for n in range(nb):
+            acc = np.zeros((ncol, ncol))
+            acc[0, 0] = t.v[n] * t.poroP[n] - t.vfP[n]
+            acc[0, 1:] = -vfi[n]
+            acc[range(1, ncol), range(1, ncol)] = 1
+            want = np.zeros(ncol)
+            share = np.zeros((ncol, ncol))
+            for k in range(IA[n] + 1, IA[n + 1]):
+                m = JA[k]
+                if m >= nb:
+                    continue  # a well column
+                km = [q for q in range(IA[m], IA[m + 1]) if JA[q] == n][0]
+                share += -val[km]
+                want += (val[km][:, c] if m > n else -val[km][:, c]) / DT
+            perf = val[IA[n]] - acc - share
+            want -= perf[:, c] / DT
+            if t.phaseNum[n] <= 2:
+                want[:] = 0
+            scale = max(np.abs(val[IA[n]][:, c]).max() / DT, 1e-300)
+            assert np.abs(rhs[n] - want).max() <= 1e-9 * scale, (c, n, rhs[n], want)
+        assert np.all(rhs[nb:] == 0)
+
+
+def test_fimn_get_solution_moves_phase_moles_by_the_update():
+    case = fim_case.build_case(4, 3, 2, seed=21, method="fimn")
+    orc = Oracle()
+    b = case["bulk"].copy()
+    nb, nc = b.nb, case["NC"] + 1
+    NC, ncol = nc - 1, nc + 1
+    b.res_n[:] = 0
+    rng = np.random.default_rng(3)
+    u = np.zeros(nb * ncol)
+    uN = rng.uniform(-1e-3, 1e-3, (nb, NC)) * b.Ni.reshape(nb, nc)[:, :NC]
+    u.reshape(nb, ncol)[:, 1:1 + NC] = uN
+    before = b.copy()
+    scm = np.array([0.0, 0.0, 0.0])
+    orc.get_solution_n(nc, b, u, 10.0, scm)
+    nj0 = before.nj.reshape(nb, 3)
+    x0 = before.xij.reshape(nb, 3, nc)
+    nj1 = b.nj.reshape(nb, 3)
+    x1 = b.xij.reshape(nb, 3, nc)
+    ex = before.pSderExist.reshape(nb, 3)[:, :2]
+    pv0 = before.pVnumCom.reshape(nb, 3)[0]  # the reference's per-phase count (cell 0's)
+    for n in range(nb):
+        hc = [j for j in range(2) if ex[n, j] and pv0[j] == NC]
+        if len(hc) != int(ex[n].sum()):
+            continue
+        old = sum(nj0[n, j] * x0[n, j, :NC] for j in hc)
+        new = sum(nj1[n, j] * x1[n, j, :NC] for j in hc)
+        np.testing.assert_allclose(new - old, uN[n], rtol=1e-8, atol=1e-12 * np.abs(old).max())
+    assert np.array_equal(b.S, before.S)  # saturations wait for the next flash
+    np.testing.assert_allclose(b.Ni.reshape(nb, nc)[:, :NC], before.Ni.reshape(nb, nc)[:, :NC] + uN, rtol=1e-14)
