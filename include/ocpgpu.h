@@ -260,8 +260,27 @@ int ocpg_create(ocpg_ctx** ctx, const ocpg_eos* eos, const ocpg_satfunc* sat,
                 const ocpg_rock* rock, int device);
 int ocpg_destroy(ocpg_ctx* ctx);
 
+/* The FIM method of the context, as IsothermalSolver::SetupMethod chooses it
+ * (src/IsothermalSolver.cpp:15-34): IsoT_FIM (secondary variables S and
+ * x_ij, the default) or IsoT_FIMn (src/OCPFluidMethod.cpp:2962-4017:
+ * secondary variables S and the phase moles n_ij).  Call before
+ * ocpg_set_cells (OCPG_ESTATE after).  With IsoT_FIMn the flash, the
+ * assembly, the right-hand side, GetSolution and the last-step copy are
+ * IsoT_FIMn's (each entry point below says what changes); ocpg_bulk's nj,
+ * res_n and resPc carry its extra state.  The serving pool runs IsoT_FIM
+ * only (ocpg_pool_init returns OCPG_EUNSUP in an IsoT_FIMn context).        */
+#define OCPG_METHOD_FIM 0
+#define OCPG_METHOD_FIMN 1
+int ocpg_set_method(ocpg_ctx* ctx, int method);
+int ocpg_get_method(ocpg_ctx* ctx);
+/* Bulk::satcm (src/Bulk.cpp:350-373, FlowUnit_ODGW01's Scm,
+ * src/FlowUnit.cpp:196-202): each phase's critical saturation, from the
+ * context's tables; IsoT_FIMn::GetSolution's chop reads them.              */
+int ocpg_critical_saturations(ocpg_ctx* ctx, double* scm /* 3 */);
+
 /* Device storage for nb cells (IsoT_FIM::AllocateReservoir,
- * src/OCPFluidMethod.cpp:980-1084), zero-filled.                            */
+ * src/OCPFluidMethod.cpp:980-1084; IsoT_FIMn::AllocateReservoir, :3111-3127,
+ * adds nj, res_n, resPc and their last-step copies), zero-filled.          */
 int ocpg_set_cells(ocpg_ctx* ctx, int64_t nb);
 /* Host <-> device copies of every non-NULL ocpg_bulk field (reference layout).
  * lNi, when given, is the last-step copy that CalRes reads.                  */
@@ -271,15 +290,19 @@ int ocpg_download_bulk(ocpg_ctx* ctx, ocpg_bulk* bulk);
 /* IsoT_FIM::CalFlash (src/OCPFluidMethod.cpp:1108-1120): MixtureComp::
  * FlashFIM (src/MixtureComp.cpp:494-573) + PassFlashValue (:1122-1187) for
  * every cell, with the skip-stability memory of SkipStaAnaly.  stats may be
- * NULL (then the call is asynchronous).                                      */
+ * NULL (then the call is asynchronous).  IsoT_FIMn: IsoT_FIMn::CalFlash
+ * (:3152-3177), MixtureComp::FlashFIMn (src/MixtureComp.cpp:575-695) and
+ * its PassFlashValue (:3179-3196: also nj, res_n, resPc).                   */
 int ocpg_flash_fim(ocpg_ctx* ctx, ocpg_flash_stats* stats);
 /* IsoT_FIM::InitFlash (src/OCPFluidMethod.cpp:1095-1106): MixtureComp::
  * InitFlashFIM (src/MixtureComp.cpp:261-353) from P, T, Sw (column 2 of S),
- * rockVp and the feed zi (host array, nb x NC row-major); Ni is an output.  */
+ * rockVp and the feed zi (host array, nb x NC row-major); Ni is an output.
+ * IsoT_FIMn: IsoT_FIMn::InitFlash (:3129-3150, InitFlashFIMn :355-442).     */
 int ocpg_init_flash_fim(ocpg_ctx* ctx, const double* zi);
 
 /* IsoT_FIM::UpdateLastTimeStep / ResetToLastTimeStep, Bulk and skip-state
- * part (src/OCPFluidMethod.cpp:2859-2957, src/AcceleratePVT.cpp:122-138).   */
+ * part (src/OCPFluidMethod.cpp:2859-2957, src/AcceleratePVT.cpp:122-138).
+ * IsoT_FIMn's (:3999-4015) also keep nj, res_n and resPc.                  */
 int ocpg_update_last_step(ocpg_ctx* ctx);
 int ocpg_reset_to_last_step(ocpg_ctx* ctx);
 
@@ -302,7 +325,12 @@ int ocpg_calkrpc_rock(ocpg_ctx* ctx);
 int ocpg_calres(ocpg_ctx* ctx, double dt, ocpg_resnorm* norms);
 /* IsoT_FIM::AssembleMat (src/OCPFluidMethod.cpp:860-875): AssembleMatBulksNew
  * (:1569-1839) and AssembleMatWellsNew (:2417-2733) into the device BSR.
- * Reads the upwinding and well rates of the preceding ocpg_calres.          */
+ * Reads the upwinding and well rates of the preceding ocpg_calres.
+ * IsoT_FIMn: IsoT_FIMn::AssembleMat (:2998-3006): its AssembleMatBulksNew
+ * (:3198-3501) and AssembleMatWellsNew (:3503-3845), whose right-hand side
+ * (what ocpg_get_bsr returns as rhs) is -resPc and the -dFdXs res_n terms
+ * of the cells with two hydrocarbon phases, plus the residual
+ * (AssembleRhsAccumulate).                                                  */
 int ocpg_assemble(ocpg_ctx* ctx, double dt);
 /* IsoT_FIM::UpdateProperty (CalFlash, CalKrPc, CalRock, the wells'
  * CalTrans/CalFlux, CalRes; src/OCPFluidMethod.cpp:877-897) followed by
@@ -312,7 +340,9 @@ int ocpg_assemble(ocpg_ctx* ctx, double dt);
  * side streams while the other cells' properties, residual and Jacobian are
  * formed, and then recomputes the rows that read the slow cells.  Identical
  * results to the four calls.  norms and stats may be NULL (then the call is
- * asynchronous).  Needs ocpg_set_conn and the kr/Pc tables.               */
+ * asynchronous).  Needs ocpg_set_conn and the kr/Pc tables.  In an
+ * IsoT_FIMn context the four calls run one after the other (no side
+ * streams).                                                                 */
 int ocpg_update_assemble(ocpg_ctx* ctx, double dt, ocpg_resnorm* norms, ocpg_flash_stats* stats);
 /* IsoT_FIM::GetSolution (src/OCPFluidMethod.cpp:2735-2857) on the resident
  * state: the Newton update of P, Ni, S and x_ij with the saturation chop
@@ -324,6 +354,10 @@ typedef struct {
     double NRdPmax;  /* the first cell's dP of largest magnitude             */
     double NRdNmax;  /* the first dNi / Nt of largest magnitude              */
 } ocpg_nr_update;
+/* IsoT_FIMn: IsoT_FIMn::GetSolution (src/OCPFluidMethod.cpp:3847-3990):
+ * P, Ni, the phase moles nj and x_ij (from res_n + dSec_dPri u, with the
+ * chop against dSmaxlim and the critical saturations), the BHP; S waits for
+ * the next flash.                                                           */
 int ocpg_get_solution(ocpg_ctx* ctx, const double* u, int u_on_device, double dSmaxlim, ocpg_nr_update* out);
 /* After the flash that follows ocpg_get_solution: IsoT_FIM::PassFlashValue's
  * dSNR / maxNRdSSP (src/OCPFluidMethod.cpp:1135-1146) and Bulk::CalNRdSmax

@@ -131,6 +131,8 @@ class NRChange(C.Structure):
 
 
 WELL_INJ, WELL_PROD = 0, 1
+# ocpg_set_method: IsoT_FIM / IsoT_FIMn
+METHOD_FIM, METHOD_FIMN = 0, 1
 WELL_BHP, WELL_RATE = 0, 1
 
 

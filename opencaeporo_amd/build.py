@@ -54,6 +54,7 @@ def _jobs(ncs):
     for n in ncs:
         opt = [NC_OPT[n]] if n in NC_OPT else []
         jobs.append(("flash_nc.hip", f"flash_nc{n}.o", [f"-DOCPG_NC={n}"] + opt))
+        jobs.append(("flashn_nc.hip", f"flashn_nc{n}.o", [f"-DOCPG_NC={n}"] + opt))
         jobs.append(("well_nc.hip", f"well_nc{n}.o", [f"-DOCPG_NC={n}"] + opt))
     jobs.append(("flash_dispatch.cpp", "flash_dispatch.o", [f"-DOCPG_NC_LIST(X)={nc_list}"]))
     jobs.append(("fim_kernels.hip", "fim_kernels.o", []))

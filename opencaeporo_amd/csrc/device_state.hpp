@@ -65,6 +65,10 @@ struct DevBulk {
     double *kr, *Pc, *Pj, *dKr_dS, *dPcj_dS;
     double *v, *depth, *poroInit, *poro, *poroP, *rockVp;
     int32_t *ftype, *iters, *nonconv;
+    // IsoT_FIMn only (NULL otherwise): phase moles nj (np), the flash's
+    // secondary-variable correction res_n (np * (nc + 1) rows, the first
+    // bRowSizedSdP used) and its volume-balance term resPc
+    double *nj, *resN, *resPc;
     // serving pool: per cell 1 when its realisation takes part in the
     // iteration (NULL: every cell does)
     const uint8_t* mask;

@@ -17,6 +17,8 @@
 
 namespace ocpg {
 DevBulk& ctxBulk(ocpg_ctx* c);
+bool ctxFimn(ocpg_ctx* c);
+const double* ctxScm(ocpg_ctx* c);
 FimDevice& ctxFim(ocpg_ctx* c);
 WellDevice& ctxWells(ocpg_ctx* c);
 FimHost& ctxFimHost(ocpg_ctx* c);
@@ -211,11 +213,12 @@ static int buildPattern(ocpg_ctx* c)
     }
     // device arrays of the pattern (the connection arrays stay)
     const int ncol = F.ncol;
-    void* old[] = {F.IA, F.JA, F.slotConn, F.slotMirror, F.bulkEnd, F.res, F.val};
+    void* old[] = {F.IA, F.JA, F.slotConn, F.slotMirror, F.bulkEnd, F.res, F.val, F.rhsN};
     for (void* q : old)
         if (q) HIPCHK2(hipFree(q));
     F.IA = F.JA = F.slotConn = F.slotMirror = F.bulkEnd = nullptr;
-    F.res = F.val = nullptr;
+    F.res = F.val = F.rhsN = nullptr;
+    F.fimn = ctxFimn(c) ? 1 : 0;
     size_t tot = F.bytes - F.patternBytes;
     const size_t before = tot;
     HIPCHK2(devAlloc(F.IA, nrow + 1, tot));
@@ -225,6 +228,7 @@ static int buildPattern(ocpg_ctx* c)
     HIPCHK2(devAlloc(F.bulkEnd, nb, tot));
     HIPCHK2(devAlloc(F.res, nrow * ncol, tot));
     HIPCHK2(devAlloc(F.val, (size_t)nnzb * ncol * ncol, tot));
+    if (F.fimn) HIPCHK2(devAlloc(F.rhsN, nrow * ncol, tot));
     F.patternBytes = tot - before;
     F.bytes = tot;
     F.nnzb = nnzb;
@@ -235,6 +239,7 @@ static int buildPattern(ocpg_ctx* c)
     HIPCHK2(hipMemcpy(F.slotMirror, slotMirror.data(), nnzb * sizeof(int32_t), hipMemcpyHostToDevice));
     HIPCHK2(hipMemcpy(F.bulkEnd, bulkEnd.data(), nb * sizeof(int32_t), hipMemcpyHostToDevice));
     HIPCHK2(hipMemset(F.res, 0, nrow * ncol * sizeof(double)));
+    if (F.rhsN) HIPCHK2(hipMemset(F.rhsN, 0, nrow * ncol * sizeof(double)));
 
     // wells: one allocation for every per-well / per-perforation array
     if (W.mem) HIPCHK2(hipFree(W.mem));
@@ -261,6 +266,7 @@ static int buildPattern(ocpg_ctx* c)
         {(void**)&W.lbhp, NW * 8}, {(void**)&W.initMode, NW * 4}, {(void**)&W.injPhase, NW * 4},
         {(void**)&W.wellDepth, NW * 8}, {(void**)&W.injTemp, NW * 8}, {(void**)&W.perfState, NP * 4},
         {(void**)&W.perfP, NP * 8}, {(void**)&W.status, NW * 4}, {(void**)&W.perfTmp, NP * 8},
+        {(void**)&W.perfRhs, NP * ncol * 8},
     };
     size_t total = 0;
     for (const Part& q : parts) total += align256w(q.bytes);
@@ -277,6 +283,7 @@ static int buildPattern(ocpg_ctx* c)
     W.nw = nw;
     W.nwOpen = nwOpen;
     W.nperf = nperf;
+    W.fimn = F.fimn;
     W.npcell = (int)pcell.size();
     W.Psurf = WH.Psurf;
     W.Tsurf = WH.Tsurf;
@@ -419,6 +426,7 @@ int ocpg_set_conn(ocpg_ctx* c, const ocpg_conn* conn)
 int ocpg_pool_init(ocpg_ctx* c, int64_t capacity, int32_t maxNb)
 {
     if (!c || capacity <= 0 || maxNb < 1 || maxNb > 64) return OCPG_EINVAL;
+    if (ctxFimn(c)) return OCPG_EUNSUP;  // the serving pool runs IsoT_FIM
     const int64_t bpc = 1 + (int64_t)maxNb, cpc = (maxNb + 1) / 2;
     if (capacity * bpc > (int64_t)INT32_MAX) return OCPG_ETOOBIG;
     int rc = ocpg_set_cells(c, capacity);
@@ -1221,7 +1229,9 @@ static int firstMaxOverBlocks(ocpg_ctx* c, int slot, double* val, int64_t* idx)
 // layout (bulk rows, then open wells), in host memory or (u_on_device)
 // device memory.  Updates P, Ni, S, x_ij and the open wells' BHP; keeps the
 // saturations before the update and the predicted changes for
-// ocpg_nr_change.  out may be NULL (then the call is asynchronous).
+// ocpg_nr_change.  out may be NULL (then the call is asynchronous).  In an
+// IsoT_FIMn context IsoT_FIMn::GetSolution (:3847-3990): P, Ni, nj, x_ij
+// and the BHP (the saturations wait for the next flash).
 int ocpg_get_solution(ocpg_ctx* c, const double* u, int u_on_device, double dSmaxlim, ocpg_nr_update* out)
 {
     if (!c || !u || !(dSmaxlim > 0)) return OCPG_EINVAL;
@@ -1247,7 +1257,10 @@ int ocpg_get_solution(ocpg_ctx* c, const double* u, int u_on_device, double dSma
         HIPCHK2(hipMemcpyAsync(N.u, u, (size_t)len * sizeof(double), hipMemcpyHostToDevice, ctxStream(c)));
         du = N.u;
     }
-    HIPCHK2(launchGetSolution(ctxNC(c), B, N, W, B.nb, du, dSmaxlim, ctxStream(c)));
+    if (ctxFimn(c))
+        HIPCHK2(launchGetSolutionN(ctxNC(c), B, N, W, B.nb, du, dSmaxlim, ctxScm(c), ctxStream(c)));
+    else
+        HIPCHK2(launchGetSolution(ctxNC(c), B, N, W, B.nb, du, dSmaxlim, ctxStream(c)));
     N.valid = true;
     if (out) {
         if ((rc = firstMaxOverBlocks(c, 0, &out->NRdPmax, nullptr)) != OCPG_OK) return rc;
@@ -1289,7 +1302,9 @@ int ocpg_get_well_bhp(ocpg_ctx* c, double* bhp)
 }
 
 // IsoT_FIM::AssembleMat (src/OCPFluidMethod.cpp:860-875), bulk blocks; the
-// right-hand side is the residual CalRes left on the device.
+// right-hand side is the residual CalRes left on the device.  In an
+// IsoT_FIMn context IsoT_FIMn::AssembleMat, whose right-hand side adds the
+// flash correction's terms to the residual.
 int ocpg_assemble(ocpg_ctx* c, double dt)
 {
     if (!c) return OCPG_EINVAL;
@@ -1301,8 +1316,16 @@ int ocpg_assemble(ocpg_ctx* c, double dt)
     HIPCHK2(launchPackRecords(ctxNC(c), ctxBulk(c), F, ctxStream(c)));
     ctxTimerEnd(c, T_PACK);
     ctxTimerBegin(c, T_ASM);
-    HIPCHK2(launchAssemble(ctxNC(c), ctxBulk(c), F, dt, ctxStream(c)));
-    if (W.nw > 0) HIPCHK2(launchWellAssemble(ctxNC(c), ctxBulk(c), F, W, dt, ctxStream(c)));
+    if (F.fimn) {
+        // IsoT_FIMn::AssembleMat (src/OCPFluidMethod.cpp:2998-3006): bulks,
+        // wells, then AssembleRhsAccumulate(resAbs)
+        HIPCHK2(launchAssembleN(ctxNC(c), ctxBulk(c), F, dt, ctxStream(c)));
+        if (W.nw > 0) HIPCHK2(launchWellAssemble(ctxNC(c), ctxBulk(c), F, W, dt, ctxStream(c)));
+        HIPCHK2(launchRhsAccumulate(F, ctxStream(c)));
+    } else {
+        HIPCHK2(launchAssemble(ctxNC(c), ctxBulk(c), F, dt, ctxStream(c)));
+        if (W.nw > 0) HIPCHK2(launchWellAssemble(ctxNC(c), ctxBulk(c), F, W, dt, ctxStream(c)));
+    }
     ctxTimerEnd(c, T_ASM);
     return OCPG_OK;
 }
@@ -1321,7 +1344,8 @@ int ocpg_get_bsr(ocpg_ctx* c, int32_t* IA, int32_t* JA, double* val, double* rhs
     if (IA) HIPCHK2(hipMemcpy(IA, F.IA, (F.nrow + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
     if (JA) HIPCHK2(hipMemcpy(JA, F.JA, F.nnzb * sizeof(int32_t), hipMemcpyDeviceToHost));
     if (val) HIPCHK2(hipMemcpy(val, F.val, F.nnzb * bs * sizeof(double), hipMemcpyDeviceToHost));
-    if (rhs) HIPCHK2(hipMemcpy(rhs, F.res, F.nrow * F.ncol * sizeof(double), hipMemcpyDeviceToHost));
+    if (rhs)
+        HIPCHK2(hipMemcpy(rhs, F.fimn ? F.rhsN : F.res, F.nrow * F.ncol * sizeof(double), hipMemcpyDeviceToHost));
     return OCPG_OK;
 }
 
@@ -1340,7 +1364,9 @@ int ocpg_get_bsr_async(ocpg_ctx* c, int32_t* IA, int32_t* JA, double* val, doubl
     if (IA) HIPCHK2(hipMemcpyAsync(IA, F.IA, (F.nrow + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     if (JA) HIPCHK2(hipMemcpyAsync(JA, F.JA, F.nnzb * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     if (val) HIPCHK2(hipMemcpyAsync(val, F.val, F.nnzb * bs * sizeof(double), hipMemcpyDeviceToHost, s));
-    if (rhs) HIPCHK2(hipMemcpyAsync(rhs, F.res, F.nrow * F.ncol * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (rhs)
+        HIPCHK2(hipMemcpyAsync(rhs, F.fimn ? F.rhsN : F.res, F.nrow * F.ncol * sizeof(double), hipMemcpyDeviceToHost,
+                               s));
     return OCPG_OK;
 }
 

@@ -757,11 +757,21 @@ struct AsmShared {
     double sRec[2][R::LEN];             // [0] own cell, [1] current neighbour
     double sC[3][D::CS];                // per phase: column coefficients, [CS - 1] the pressure one
     double sD[3];                       // per phase: the x columns' diagonal term
-    double sG[3][D::ncol];              // per phase: C * dSec_own
+    double sG[3][D::ncol + 1];          // per phase: C * dSec_own ([ncol]: C * res_n, IsoT_FIMn)
 };
 
 // Row n of the bulk assembly, by one wavefront.
-template <int NC>
+//
+// FN: IsoT_FIMn::AssembleMatBulksNew (src/OCPFluidMethod.cpp:3198-3501).
+// The secondary columns are the phase moles n_ij (x_ij = n_ij / nj): where
+// the row's cell is upwind, every n_kj column of phase j gains
+// -x_ij xi transJ dP / nj and the diagonal term becomes xi transJ dP / nj,
+// both per-phase factors of the same C / d form.  The row also forms its
+// cell's right-hand-side terms (only where the cell has two hydrocarbon
+// phases): per connection -dFdXs res_n, i.e. -(sum_j x_ij (C[j] . res_n) +
+// sum_j d_j res_n[x(j, i)]), summed in the connection loop's order after
+// -resPc on the volume row, into F.rhsN.
+template <int NC, bool FN = false>
 __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F, double dt, int64_t n,
                                             AsmShared<NC>& sh)
 {
@@ -817,11 +827,32 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
     xb[1] = xb[0] + pvO[0];
     xb[2] = xb[1] + pvO[1];
 
-    // this lane's column of dSec_own for G (lanes (j, cc))
-    const int gcc = lane % ncol;
+    // this lane's column of dSec_own for G (lanes (j, cc)); with FN, lanes
+    // (j, ncol) take res_n instead
+    constexpr int GW = FN ? ncol + 1 : ncol;
+    const int gcc = lane % GW;
     double dS[D::maxRows];
+    if constexpr (FN) {
+        const int64_t ld = B.ld;
 #pragma unroll
-    for (int k = 0; k < D::maxRows; ++k) dS[k] = k < rowsOwn ? own[O::DSEC + k * ncol + gcc] : 0.0;
+        for (int k = 0; k < D::maxRows; ++k)
+            dS[k] = k < rowsOwn ? (gcc < ncol ? own[O::DSEC + k * ncol + gcc] : B.resN[(int64_t)k * ld + n]) : 0.0;
+    } else {
+#pragma unroll
+        for (int k = 0; k < D::maxRows; ++k) dS[k] = k < rowsOwn ? own[O::DSEC + k * ncol + gcc] : 0.0;
+    }
+    // IsoT_FIMn: the own cell's phase moles and whether it has two
+    // hydrocarbon phases (then its right-hand-side terms are formed)
+    double njO[3] = {1.0, 1.0, 1.0};
+    bool twoPh = false;
+    double rhsAcc = 0.0;
+    if constexpr (FN) {
+        const int64_t ld = B.ld;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) njO[j] = uniformD(B.nj[(int64_t)j * ld + n]);
+        twoPh = __builtin_amdgcn_readfirstlane(B.phaseNum[n]) > 2;
+        if (lane == 0) rhsAcc = 0.0 + -B.resPc[n];
+    }
 
     // diagonal starts as the accumulation block (src/OCPFluidMethod.cpp:1586-1597)
     double acc[D::perLane];
@@ -946,7 +977,11 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
                             }
                         }
                         cv = v;
-                        dv = upO ? sRec[0][R::XI + j] * transJ * dP : 0.0;
+                        if constexpr (FN)
+                            dv = upO ? sRec[0][R::XI + j] * transJ * dP / (j == 0 ? njO[0] : j == 1 ? njO[1] : njO[2])
+                                     : 0.0;
+                        else
+                            dv = upO ? sRec[0][R::XI + j] * transJ * dP : 0.0;
                     } else if (col < nsat) {
                         const int s1 = col == 0 ? sph[0] : col == 1 ? sph[1] : sph[2];
                         double v = sgn * A * sOwn[O::DPC + j * 3 + s1];
@@ -969,6 +1004,7 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
                                 const double rx = othEx ? rhox / 2 : rhox;
                                 cv = -A * rx * dGamma + transJ * sOwn[O::XIX + j * nc + kk] * dP -
                                      A * sOwn[O::MUXM + j * nc + kk] * dP;
+                                if constexpr (FN) cv -= A * dP / (j == 0 ? njO[0] : j == 1 ? njO[1] : njO[2]);
                             } else {
                                 cv = -A * (rhox / 2) * dGamma;
                             }
@@ -981,8 +1017,8 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
         }
         __syncthreads();
         // ---- G[j][cc] = [cc == 0] C[j][pressure] + sum_col C[j][col] dSec[col][cc] ----
-        if (lane < 3 * ncol) {
-            const int j = lane / ncol;
+        if (lane < 3 * GW) {
+            const int j = lane / GW;
             double g = 0.0;
             if ((liveM >> j) & 1u) {
                 // columns past the own cell's hold zero coefficients and
@@ -994,6 +1030,23 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
             sh.sG[j][gcc] = g;
         }
         __syncthreads();
+        // ---- IsoT_FIMn: this connection's -dFdXs res_n on rows 1..nc ----
+        if constexpr (FN) {
+            if (twoPh && lane >= 1 && lane < ncol) {
+                const int i = lane - 1;
+                const int64_t ld = B.ld;
+                double v = 0.0;
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    if (!((liveM >> j) & 1u)) continue;
+                    const double* rU = ((upOwnM >> j) & 1u) ? sRec[0] : sRec[1];
+                    v = fma(rU[R::XIJ + j * nc + i], sh.sG[j][ncol], v);
+                    if (((upOwnM >> j) & 1u) && i < pvO[j])
+                        v = fma(sh.sD[j], B.resN[(int64_t)(xb[j] + i) * ld + n], v);
+                }
+                rhsAcc += -v;
+            }
+        }
 
         // ---- block = dt (sum_j x_ij G[j] + sum_j d_j dSec[x(j, i)]), then scatter ----
         double* dst = F.val + mirror * bsize;
@@ -1033,9 +1086,11 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
         const int e = lane + t * kAsmBlock;
         if (e < bsize) diag[e] = acc[t];
     }
+    if constexpr (FN)
+        if (lane < ncol) F.rhsN[n * ncol + lane] = rhsAcc;
 }
 
-template <int NC>
+template <int NC, bool FN = false>
 __global__ void __launch_bounds__(kAsmBlock) __attribute__((amdgpu_waves_per_eu(NC <= 6 ? 5 : 4))) assembleKernel(DevBulk B, FimDevice F, double dt)
 {
     __shared__ AsmShared<NC> sh;
@@ -1043,7 +1098,17 @@ __global__ void __launch_bounds__(kAsmBlock) __attribute__((amdgpu_waves_per_eu(
     const int64_t chunk = gridDim.x / kXcds;
     const int64_t n = (int64_t)(blockIdx.x % kXcds) * chunk + blockIdx.x / kXcds;
     if (n >= B.nb || !cellOn(B, n)) return;
-    assembleRow<NC>(B, F, dt, n, sh);
+    assembleRow<NC, FN>(B, F, dt, n, sh);
+}
+
+// LinearSystem::AssembleRhsAccumulate (src/LinearSystem.cpp:70-76) of the
+// IsoT_FIMn right-hand side: rhsN = (bulk and perforation terms) + res; the
+// well rows hold no terms of their own.
+__global__ void __launch_bounds__(256) rhsAccumulateKernel(double* __restrict__ rhsN, const double* __restrict__ res,
+                                                           int64_t bulkLen, int64_t len)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < len) rhsN[k] = (k < bulkLen ? rhsN[k] : 0.0) + res[k];
 }
 
 // The rows in rows[0, *count) (the overlapped form's recomputation).
@@ -1098,6 +1163,25 @@ static hipError_t launchAssembleT(const DevBulk& B, FimDevice& F, double dt, hip
 {
     const unsigned grid = (unsigned)(((B.nb + kXcds - 1) / kXcds) * kXcds);
     hipLaunchKernelGGL(assembleKernel<NC>, dim3(grid), dim3(kAsmBlock), 0, s, B, F, dt);
+    return hipGetLastError();
+}
+
+template <int NC>
+static hipError_t launchAssembleNT(const DevBulk& B, FimDevice& F, double dt, hipStream_t s)
+{
+    if (!F.fimn || !F.rhsN || !B.nj || !B.resN || !B.resPc) return hipErrorInvalidValue;
+    const unsigned grid = (unsigned)(((B.nb + kXcds - 1) / kXcds) * kXcds);
+    hipLaunchKernelGGL((assembleKernel<NC, true>), dim3(grid), dim3(kAsmBlock), 0, s, B, F, dt);
+    return hipGetLastError();
+}
+
+hipError_t launchRhsAccumulate(FimDevice& F, hipStream_t s)
+{
+    if (!F.rhsN) return hipErrorInvalidValue;
+    const int64_t len = F.nrow * F.ncol;
+    if (len == 0) return hipSuccess;
+    hipLaunchKernelGGL(rhsAccumulateKernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, F.rhsN, F.res,
+                       F.nb * F.ncol, len);
     return hipGetLastError();
 }
 
@@ -1230,6 +1314,125 @@ __global__ void __launch_bounds__(kNewtonBlock) getSolutionKernel(DevBulk B, New
     fmBlockReduce(sh, bn, N.part, N.partIdx, 1);
 }
 
+// IsoT_FIMn::GetSolution (src/OCPFluidMethod.cpp:3847-3990), one lane per
+// cell in the reference's operation order: the secondary changes res_n +
+// dSec_dPri u, the saturation chop (the limit dSmaxlim, the critical
+// saturations scm, non-negativity), the phase moles nj and compositions
+// x_ij of each phase with saturation derivatives, P and Ni.  Saturations
+// are not updated (the next flash sets them); dSNRP keeps the predicted
+// changes.  The reference takes the count of n_ij rows of phase j from
+// bk.pVnumCom[j], cell 0's entry, for every cell (:3933-3936): kept.
+template <int NC>
+__global__ void __launch_bounds__(kNewtonBlock) getSolutionNKernel(DevBulk B, NewtonDevice N,
+                                                                   const double* __restrict__ u, double dSmaxlim,
+                                                                   double scm0, double scm1, double scm2)
+{
+    constexpr int nc = NC + 1;
+    constexpr int col = nc + 1;
+    constexpr int maxRows = 3 + 2 * NC;
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t ld = B.ld;
+    FirstMax bp{0.0, 0.0, INT64_MAX}, bn{0.0, 0.0, INT64_MAX};
+    if (n < B.nb) {
+#pragma unroll
+        for (int j = 0; j < kNP; ++j) {
+            N.dSNR[j * ld + n] = B.S[j * ld + n];
+            N.pexOld[j * ld + n] = B.phaseExist[j * ld + n];
+        }
+        const double dP = u[n * col];
+        B.P[n] += dP;
+        bp = FirstMax{fabs(dP), dP, n};
+        if (!(bp.key > 0)) bp = FirstMax{0.0, 0.0, INT64_MAX};
+        const int rows = min(max(B.bRowSizedSdP[n], 0), maxRows);
+        double dtmp[maxRows];
+#pragma unroll
+        for (int r = 0; r < maxRows; ++r) {
+            double y = 0;
+            if (r < rows) {
+                y = B.resN[(int64_t)r * ld + n];
+                y = 1.0 * y;
+#pragma unroll
+                for (int c = 0; c < col; ++c) y += 1.0 * B.dSec[(int64_t)(r * col + c) * ld + n] * u[n * col + c];
+            }
+            dtmp[r] = y;
+        }
+        auto at = [&](int r) {
+            double d = 0;
+#pragma unroll
+            for (int q = 0; q < maxRows; ++q)
+                if (q == r) d = dtmp[q];
+            return d;
+        };
+        const double scm[3] = {scm0, scm1, scm2};
+        double dSmax = 0;
+        int js = 0;
+#pragma unroll
+        for (int j = 0; j < kNP; ++j)
+            if (B.pSderExist[j * ld + n]) {
+                const double a = fabs(at(js));
+                dSmax = (a < dSmax) ? dSmax : a;  // std::max(fabs(dtmp[js]), dSmax)
+                js++;
+            }
+        double chop = 1;
+        if (dSmax > dSmaxlim) {
+            const double t = dSmaxlim / dSmax;
+            chop = (t < chop) ? t : chop;
+        }
+        js = 0;
+#pragma unroll
+        for (int j = 0; j < kNP; ++j)
+            if (B.pSderExist[j * ld + n]) {
+                const double S = B.S[j * ld + n];
+                const double d = at(js);
+                if (fabs(S - scm[j]) > 1E-8 && (S - scm[j]) / (chop * d) < 0) {
+                    const double t = -((S - scm[j]) / (chop * d));
+                    chop *= (t < 1.0) ? t : 1.0;  // std::min(1.0, t)
+                }
+                if (S + chop * d < 0) {
+                    const double t = -S / (chop * d);
+                    chop *= (t < 1.0) ? t : 1.0;
+                }
+                js++;
+            }
+        js = 0;
+        int jx = B.phaseNum[n];
+#pragma unroll
+        for (int j = 0; j < kNP; ++j) {
+            if (!B.pSderExist[j * ld + n]) continue;
+            N.dSNRP[j * ld + n] = chop * at(js);
+            js++;
+            const double nj = B.nj[j * ld + n];
+            const int pv = min(max(B.pVnumCom[j * ld + 0], 0), NC);
+            double t[nc];
+#pragma unroll
+            for (int i = 0; i < nc; ++i) {
+                t[i] = 0.0;
+                t[i] += nj * B.xij[(int64_t)(j * nc + i) * ld + n];
+            }
+#pragma unroll
+            for (int i = 0; i < NC; ++i)
+                if (i < pv) t[i] += chop * at(jx + i);
+            jx += pv;
+            double s1 = 0;
+#pragma unroll
+            for (int i = 0; i < nc; ++i) s1 += fabs(t[i]);
+            B.nj[j * ld + n] = s1;
+#pragma unroll
+            for (int i = 0; i < nc; ++i) B.xij[(int64_t)(j * nc + i) * ld + n] = t[i] / s1;
+        }
+        const double Nt = B.Nt[n];
+#pragma unroll
+        for (int i = 0; i < nc; ++i) {
+            const double dN = u[n * col + 1 + i] * chop;
+            if (bn.key < fabs(dN) / Nt) bn = FirstMax{fabs(dN) / Nt, dN / Nt, n * nc + i};
+            B.Ni[(int64_t)i * ld + n] += dN;
+        }
+    }
+    __shared__ FirstMax sh[kNewtonBlock];
+    fmBlockReduce(sh, bp, N.part, N.partIdx, 0);
+    fmBlockReduce(sh, bn, N.part, N.partIdx, 1);
+}
+
 // Well::SetBHP(BHP() + u) of the open wells (src/OCPFluidMethod.cpp:2846-2854).
 __global__ void wellBhpUpdateKernel(WellDevice W, const double* __restrict__ u, int64_t nb, int col)
 {
@@ -1266,6 +1469,18 @@ static hipError_t launchGetSolutionT(const DevBulk& B, NewtonDevice& N, const We
                                      const double* u, double dSmaxlim, hipStream_t s)
 {
     hipLaunchKernelGGL(getSolutionKernel<NC>, dim3(N.blocks), dim3(kNewtonBlock), 0, s, B, N, u, dSmaxlim);
+    if (W.nw > 0)
+        hipLaunchKernelGGL(wellBhpUpdateKernel, dim3((W.nw + 63) / 64), dim3(64), 0, s, W, u, nb, NC + 2);
+    return hipGetLastError();
+}
+
+template <int NC>
+static hipError_t launchGetSolutionNT(const DevBulk& B, NewtonDevice& N, const WellDevice& W, int64_t nb,
+                                      const double* u, double dSmaxlim, const double* scm, hipStream_t s)
+{
+    if (!B.nj || !B.resN) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(getSolutionNKernel<NC>, dim3(N.blocks), dim3(kNewtonBlock), 0, s, B, N, u, dSmaxlim, scm[0],
+                       scm[1], scm[2]);
     if (W.nw > 0)
         hipLaunchKernelGGL(wellBhpUpdateKernel, dim3((W.nw + 63) / 64), dim3(64), 0, s, W, u, nb, NC + 2);
     return hipGetLastError();
@@ -1424,6 +1639,16 @@ hipError_t launchGetSolution(int NC, const DevBulk& B, NewtonDevice& N, const We
     return hipErrorInvalidValue;
 }
 
+hipError_t launchGetSolutionN(int NC, const DevBulk& B, NewtonDevice& N, const WellDevice& W, int64_t nb,
+                              const double* u, double dSmaxlim, const double* scm, hipStream_t s)
+{
+#define X(N_) \
+    if (NC == N_) return launchGetSolutionNT<N_>(B, N, W, nb, u, dSmaxlim, scm, s);
+    OCPG_FIM_NC(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
 hipError_t launchCalRes(int NC, const DevBulk& B, FimDevice& F, const WellDevice& W, double dt, hipStream_t s,
                         bool norms)
 {
@@ -1539,6 +1764,15 @@ hipError_t launchPackRecordsList(int NC, const DevBulk& B, FimDevice& F, const C
     return hipErrorInvalidValue;
 }
 
+hipError_t launchAssembleN(int NC, const DevBulk& B, FimDevice& F, double dt, hipStream_t s)
+{
+#define X(N) \
+    if (NC == N) return launchAssembleNT<N>(B, F, dt, s);
+    OCPG_FIM_NC(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
 hipError_t launchAssemble(int NC, const DevBulk& B, FimDevice& F, double dt, hipStream_t s)
 {
     if (B.nb == 0) return hipSuccess;
@@ -1552,7 +1786,7 @@ hipError_t launchAssemble(int NC, const DevBulk& B, FimDevice& F, double dt, hip
 void fimFree(FimDevice& f)
 {
     void* ptrs[] = {f.bId, f.eId, f.area, f.IA, f.JA, f.slotConn, f.slotMirror, f.bulkEnd, f.upblock,
-                    f.upRho, f.upVel, f.res, f.val, f.normPart, f.normIdx, f.rec, f.own};
+                    f.upRho, f.upVel, f.res, f.val, f.normPart, f.normIdx, f.rec, f.own, f.rhsN};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     f = FimDevice{};

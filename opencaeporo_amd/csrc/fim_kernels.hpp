@@ -32,6 +32,11 @@ struct FimDevice {
     int64_t nrow = 0;               // rows: bulk cells, then open wells
     double* res = nullptr;          // nrow * ncol, cell-major (LinearSystem rhs layout), negated
     double* val = nullptr;          // nnzb * ncol * ncol, row-major blocks
+    // IsoT_FIMn (fimn set): the right-hand side AssembleMat builds, nrow *
+    // ncol: the bulk and perforation terms of the flash's correction, then
+    // + res (LinearSystem::AssembleRhsAccumulate)
+    int fimn = 0;
+    double* rhsN = nullptr;
     // per-cell records the assembly reads, packed (AoS) by launchPackRecords
     double* rec = nullptr;          // nb * recStride: a connection side's cell values
     double* own = nullptr;          // nb * ownStride: a row's own dSec_dPri, dPcj_dS, accumulation terms
@@ -53,6 +58,11 @@ hipError_t launchCalRes(int NC, const DevBulk& B, FimDevice& F, const WellDevice
                         bool norms = true);
 hipError_t launchResNorm(int NC, const DevBulk& B, FimDevice& F, hipStream_t s);
 hipError_t launchAssemble(int NC, const DevBulk& B, FimDevice& F, double dt, hipStream_t s);
+// IsoT_FIMn::AssembleMatBulksNew (F.fimn): the blocks, and the bulk terms of
+// the right-hand side into F.rhsN; then, after the wells' blocks,
+// launchRhsAccumulate adds res (rhsN = rhsN + res, row by row).
+hipError_t launchAssembleN(int NC, const DevBulk& B, FimDevice& F, double dt, hipStream_t s);
+hipError_t launchRhsAccumulate(FimDevice& F, hipStream_t s);
 // Doubles per cell of the packed records (FimDevice::rec, ::own), and their
 // packing from the SoA state (every cell, or the cells of two lists).
 int64_t recordDoubles(int NC);
@@ -81,6 +91,10 @@ struct NewtonDevice {
 };
 hipError_t launchGetSolution(int NC, const DevBulk& B, NewtonDevice& N, const WellDevice& W, int64_t nb,
                              const double* u, double dSmaxlim, hipStream_t s);
+// IsoT_FIMn::GetSolution (src/OCPFluidMethod.cpp:3847-3990); scm: the
+// critical saturation of each phase (Bulk::satcm).
+hipError_t launchGetSolutionN(int NC, const DevBulk& B, NewtonDevice& N, const WellDevice& W, int64_t nb,
+                              const double* u, double dSmaxlim, const double* scm, hipStream_t s);
 hipError_t launchNRChange(const DevBulk& B, NewtonDevice& N, hipStream_t s);
 // Bulk::CheckNi / CheckP into *flag (bit 1 / bit 0; *flag zeroed by the
 // caller) and Bulk::CalMaxChange's per-block maxima against the last-step

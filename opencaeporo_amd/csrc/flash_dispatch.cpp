@@ -17,10 +17,20 @@ bool flashSupportsNC(int nc)
 }
 
 hipError_t launchFlash(int NC, const DevEos* eos, const DevBulk& B, const FlashWork& W, DevFlashCounters* ctr,
-                       const double* zin, hipStream_t s, hipEvent_t* ev, const FlashSide* side)
+                       const double* zin, hipStream_t s, hipEvent_t* ev, const FlashSide* side, bool finish)
 {
 #define X(N) \
-    if (NC == N) return launchFlashT<N>(eos, B, W, ctr, zin, s, ev, side);
+    if (NC == N) return launchFlashT<N>(eos, B, W, ctr, zin, s, ev, side, finish);
+    OCPG_NC_LIST(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launchFlashFinishN(int NC, const DevEos* eos, const DevBulk& B, const FlashWork& W, int init,
+                              hipStream_t s)
+{
+#define X(N) \
+    if (NC == N) return launchFlashFinishNT<N>(eos, B, W, init, s);
     OCPG_NC_LIST(X)
 #undef X
     return hipErrorInvalidValue;

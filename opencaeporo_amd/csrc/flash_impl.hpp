@@ -1292,7 +1292,7 @@ __global__ void __launch_bounds__(kFlashBlock) flashFinishSideKernel(const DevEo
 
 template <int NC>
 hipError_t launchFlashT(const DevEos* eos, const DevBulk& B, const FlashWork& W, DevFlashCounters* ctr,
-                        const double* zin, hipStream_t s, hipEvent_t* ev, const FlashSide* side)
+                        const double* zin, hipStream_t s, hipEvent_t* ev, const FlashSide* side, bool finish)
 {
     if (B.nb == 0) return hipSuccess;
     const unsigned grid = (unsigned)((B.nb + kFlashBlock - 1) / kFlashBlock);
@@ -1326,6 +1326,7 @@ hipError_t launchFlashT(const DevEos* eos, const DevBulk& B, const FlashWork& W,
         hipLaunchKernelGGL(flashSplitNRKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, ctr, -1,
                            (int)LIST_NR, (int)LIST_NP1, (int)LIST_NP2);
         mark(3, s);
+        if (!finish) return hipGetLastError();
         mark(4, s);
         hipLaunchKernelGGL((flashFinishKernel<NC, 1>), dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, init,
                            (int)LIST_NP1);
@@ -1340,7 +1341,7 @@ hipError_t launchFlashT(const DevEos* eos, const DevBulk& B, const FlashWork& W,
     // serial order, when it is long (a cold pass puts every cell through the
     // stability test): a kernel pair gated on the list's length on the
     // device, exactly one of which works.
-    if (W.guess || zin) return hipErrorInvalidValue;
+    if (W.guess || zin || !finish) return hipErrorInvalidValue;
     if ((e = hipEventRecord(side->classified, s)) != hipSuccess) return e;
     // side stream B: the stability test and the split of the cells it finds unstable
     if ((e = hipStreamWaitEvent(side->sB, side->classified, 0)) != hipSuccess) return e;

@@ -29,8 +29,15 @@ struct FlashSide {
 // D's.  With side non-NULL the slow cells continue on the side streams (see
 // flash_impl.hpp): the caller makes s wait for side->doneB, doneC and doneD
 // and calls launchFlashSideFinish before anything reads those cells.
+// With finish false (serial form only) the finishing stage is left out:
+// the IsoT_FIMn flash runs its own (launchFlashFinishN) on the same lists.
 hipError_t launchFlash(int NC, const DevEos* eos, const DevBulk& B, const FlashWork& W, DevFlashCounters* ctr,
-                       const double* zin, hipStream_t s, hipEvent_t* ev = nullptr, const FlashSide* side = nullptr);
+                       const double* zin, hipStream_t s, hipEvent_t* ev = nullptr, const FlashSide* side = nullptr,
+                       bool finish = true);
+// IsoT_FIMn's finishing stage (FlashFIMn / InitFlashFIMn after the phase
+// equilibrium, flash_finish_n.hpp) for the cells the serial flash finished.
+hipError_t launchFlashFinishN(int NC, const DevEos* eos, const DevBulk& B, const FlashWork& W, int init,
+                              hipStream_t s);
 // After the join: the finishing stage of the side streams' cells, and
 // those cells.
 hipError_t launchFlashSideFinish(int NC, const DevEos* eos, const DevBulk& B, const FlashWork& W, hipStream_t s);
@@ -52,7 +59,9 @@ hipError_t launchPhaseDensity(int NC, const DevEos* eos, double P, double TF, co
 namespace ocpg {
 template <int NC>
 hipError_t launchFlashT(const DevEos* eos, const DevBulk& B, const FlashWork& W, DevFlashCounters* ctr,
-                        const double* zin, hipStream_t s, hipEvent_t* ev, const FlashSide* side);
+                        const double* zin, hipStream_t s, hipEvent_t* ev, const FlashSide* side, bool finish);
+template <int NC>
+hipError_t launchFlashFinishNT(const DevEos* eos, const DevBulk& B, const FlashWork& W, int init, hipStream_t s);
 template <int NC>
 hipError_t launchFlashSideFinishT(const DevEos* eos, const DevBulk& B, const FlashWork& W, hipStream_t s);
 

@@ -8,6 +8,6 @@
 
 namespace ocpg {
 template hipError_t launchFlashT<OCPG_NC>(const DevEos*, const DevBulk&, const FlashWork&, DevFlashCounters*,
-                                           const double*, hipStream_t, hipEvent_t*, const FlashSide*);
+                                           const double*, hipStream_t, hipEvent_t*, const FlashSide*, bool);
 template hipError_t launchFlashSideFinishT<OCPG_NC>(const DevEos*, const DevBulk&, const FlashWork&, hipStream_t);
 }  // namespace ocpg

@@ -24,7 +24,8 @@ namespace {
 
 // One SoA field: where its host array lives in ocpg_bulk, its element size
 // and how many entries a cell holds.
-enum Cnt { C1, CNC, CNCh, CNP, CNPNC, CNPNP, CLEN, C5 };
+// C1n, CNPn, CRESn: IsoT_FIMn fields, no entries in an IsoT_FIM context.
+enum Cnt { C1, CNC, CNCh, CNP, CNPNC, CNPNP, CLEN, C5, C1n, CNPn, CRESn };
 
 struct FieldDesc {
     size_t hostOff;   // offset of the pointer inside ocpg_bulk
@@ -59,6 +60,10 @@ const FieldDesc kFields[] = {
     {HB(pVnumCom), DB(pVnumCom), 4, CNP, true}, {HB(skipFlag), DB(skipFlag), 1, C1, true},
     {HB(skipMinEig), DB(skipMinEig), 8, C1, true}, {HB(skipP), DB(skipP), 8, C1, true},
     {HB(skipT), DB(skipT), 8, C1, true},        {HB(skipZi), DB(skipZi), 8, CNCh, true},
+    // IsoT_FIMn's additions (src/OCPFluidMethod.cpp:3999-4015: nj, res_n and
+    // resPc are part of its last-step copy)
+    {HB(nj), DB(nj), 8, CNPn, true},            {HB(res_n), DB(resN), 8, CRESn, true},
+    {HB(resPc), DB(resPc), 8, C1n, true},
     // not copied at the end of a time step
     {HB(T), DB(T), 8, C1, false},               {HB(v), DB(v), 8, C1, false},
     {HB(depth), DB(depth), 8, C1, false},       {HB(poroInit), DB(poroInit), 8, C1, false},
@@ -140,6 +145,10 @@ struct ocpg_ctx {
     // one Newton iteration captured as a graph (ocpg_capture_iteration)
     hipGraphExec_t iterGraph = nullptr;
     hipEvent_t iterDone = nullptr;
+    // IsoT_FIMn in place of IsoT_FIM (ocpg_set_method, before ocpg_set_cells)
+    // and the critical saturations its GetSolution reads (Bulk::satcm)
+    bool fimn = false;
+    double scm[3] = {0, 0, 0};
     // per-kernel timers (ocpg_timing_*)
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events[kTimerSlots];
@@ -170,6 +179,9 @@ static int64_t cellEntries(const ocpg_ctx* c, Cnt k)
         case CNPNP: return OCPG_NP * OCPG_NP;
         case CLEN: return (int64_t)(nc + 1) * (nc + 1) * OCPG_NP;
         case C5: return 5;
+        case C1n: return c->fimn ? 1 : 0;
+        case CNPn: return c->fimn ? OCPG_NP : 0;
+        case CRESn: return c->fimn ? (int64_t)OCPG_NP * (nc + 1) : 0;
     }
     return 0;
 }
@@ -426,6 +438,18 @@ int ocpg_create(ocpg_ctx** out, const ocpg_eos* eos, const ocpg_satfunc* sat, co
             std::memcpy(c->satHost.swof, sat->swof, sizeof(sat->swof));
             std::memcpy(c->satHost.sgof, sat->sgof, sizeof(sat->sgof));
             c->satHost.kroMax = sat->swof[0][2];  // FlowUnit_ODGW01: SWOF col 2, row 0
+            // Bulk::satcm of FlowUnit_ODGW01 (src/FlowUnit.cpp:196-202,
+            // OCPTable::GetRowZero src/OCPTable.cpp:35-44): oil 0; gas / water
+            // the saturation of the last SGOF / SWOF row before kr reaches 1e-8
+            auto rowZero = [](const double (*tab)[4], int rows) {
+                for (int r = 0; r < rows; r++)
+                    if (tab[r][1] >= 1E-8) return r - 1;
+                return rows - 1;
+            };
+            const int rg = rowZero(sat->sgof, sat->sgof_rows), rw = rowZero(sat->swof, sat->swof_rows);
+            c->scm[0] = 0;
+            c->scm[1] = rg >= 0 ? sat->sgof[rg][0] : 0.0;
+            c->scm[2] = rw >= 0 ? sat->swof[rw][0] : 0.0;
             if ((rc = hipStatus(hipMalloc(&c->dSat, sizeof(DevSat))))) break;
             if ((rc = hipStatus(hipMemcpy(c->dSat, &c->satHost, sizeof(DevSat), hipMemcpyHostToDevice)))) break;
         }
@@ -481,6 +505,32 @@ int ocpg_destroy(ocpg_ctx* c)
     destroySide(c);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+    return OCPG_OK;
+}
+
+// IsothermalSolver::SetupMethod's choice (src/IsothermalSolver.cpp:15-34):
+// IsoT_FIM (OCPG_METHOD_FIM, the default) or IsoT_FIMn (OCPG_METHOD_FIMN,
+// src/OCPFluidMethod.cpp:2962-4017).  Before ocpg_set_cells.
+int ocpg_set_method(ocpg_ctx* c, int method)
+{
+    if (!c || (method != OCPG_METHOD_FIM && method != OCPG_METHOD_FIMN)) return OCPG_EINVAL;
+    if (c->cur) return OCPG_ESTATE;
+    c->fimn = method == OCPG_METHOD_FIMN;
+    return OCPG_OK;
+}
+
+int ocpg_get_method(ocpg_ctx* c)
+{
+    if (!c) return OCPG_EINVAL;
+    return c->fimn ? OCPG_METHOD_FIMN : OCPG_METHOD_FIM;
+}
+
+// Bulk::satcm (src/Bulk.cpp:350-373): the critical saturation of each phase.
+int ocpg_critical_saturations(ocpg_ctx* c, double* scm)
+{
+    if (!c || !scm) return OCPG_EINVAL;
+    if (!c->dSat) return OCPG_ESTATE;
+    for (int j = 0; j < 3; j++) scm[j] = c->scm[j];
     return OCPG_OK;
 }
 
@@ -560,7 +610,7 @@ int ocpg_set_cells(ocpg_ctx* c, int64_t nb)
     c->B.nb = nb;
     c->B.ld = ld;
     for (int f = 0; f < kNumFields; f++) {
-        void* p = c->cur + c->fieldOff[f];
+        void* p = cellEntries(c, kFields[f].cnt) > 0 ? c->cur + c->fieldOff[f] : nullptr;
         std::memcpy(reinterpret_cast<char*>(&c->B) + kFields[f].devOff, &p, sizeof(void*));
     }
     // IsoT_FIM::CalRes reads the last-step moles as lNi.
@@ -571,7 +621,7 @@ int ocpg_set_cells(ocpg_ctx* c, int64_t nb)
     c->Blast.ld = ld;
     for (int f = 0; f < kNumFields; f++) {
         if (!kFields[f].lastStep) continue;
-        void* p = c->last + c->fieldOff[f];
+        void* p = cellEntries(c, kFields[f].cnt) > 0 ? c->last + c->fieldOff[f] : nullptr;
         std::memcpy(reinterpret_cast<char*>(&c->Blast) + kFields[f].devOff, &p, sizeof(void*));
     }
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -590,8 +640,8 @@ int ocpg_upload_bulk(ocpg_ctx* c, const ocpg_bulk* h)
         const FieldDesc& d = kFields[f];
         const char* src;
         std::memcpy(&src, reinterpret_cast<const char*>(h) + d.hostOff, sizeof(void*));
-        if (!src) continue;
         const int64_t cnt = cellEntries(c, d.cnt);
+        if (!src || cnt == 0) continue;
         tmp.assign((size_t)d.elem * cnt * ld, 0);
         for (int64_t n = 0; n < nb; n++)
             for (int64_t k = 0; k < cnt; k++)
@@ -619,8 +669,8 @@ int ocpg_download_bulk(ocpg_ctx* c, ocpg_bulk* h)
         const FieldDesc& d = kFields[f];
         char* dst;
         std::memcpy(&dst, reinterpret_cast<char*>(h) + d.hostOff, sizeof(void*));
-        if (!dst) continue;
         const int64_t cnt = cellEntries(c, d.cnt);
+        if (!dst || cnt == 0) continue;
         tmp.resize((size_t)d.elem * cnt * ld);
         HIPCHK(hipMemcpy(tmp.data(), c->cur + c->fieldOff[f], tmp.size(), hipMemcpyDeviceToHost));
         for (int64_t n = 0; n < nb; n++)
@@ -648,7 +698,13 @@ int ocpg_flash_fim(ocpg_ctx* c, ocpg_flash_stats* stats)
     for (int k = 0; k < 6 && stageEv; k++) stageEv = hipEventCreate(&ev[k]) == hipSuccess;
     timerBegin(c, T_FLASH);
     HIPCHK(launchFlash(c->NC, c->dEos, c->B, c->work, stats ? c->dCtr : nullptr, nullptr, c->stream,
-                       stageEv ? ev : nullptr));
+                       stageEv ? ev : nullptr, nullptr, !c->fimn));
+    if (c->fimn) {
+        // IsoT_FIMn::CalFlash: FlashFIMn's own finishing stage
+        if (stageEv) HIPCHK(hipEventRecord(ev[4], c->stream));
+        HIPCHK(launchFlashFinishN(c->NC, c->dEos, c->B, c->work, 0, c->stream));
+        if (stageEv) HIPCHK(hipEventRecord(ev[5], c->stream));
+    }
     timerEnd(c, T_FLASH);
     if (stageEv) {
         c->events[T_STAB].push_back({ev[0], ev[1]});
@@ -766,6 +822,17 @@ int ocpg_update_assemble(ocpg_ctx* c, double dt, ocpg_resnorm* norms, ocpg_flash
     FimDevice& F = c->fim;
     if (F.nb <= 0 || F.nb != c->B.nb) return OCPG_ESTATE;
     if (!c->dSat) return OCPG_ESTATE;
+    if (c->fimn) {
+        // IsoT_FIMn: the serial calls (no overlapped form)
+        timerBegin(c, T_UPDASM);
+        int r = ocpg_flash_fim(c, stats);
+        if (r == OCPG_OK) r = ocpg_calkrpc_rock(c);
+        if (r == OCPG_OK) r = ocpg_calres(c, dt, nullptr);
+        if (r == OCPG_OK) r = ocpg_assemble(c, dt);
+        timerEnd(c, T_UPDASM);
+        if (r == OCPG_OK && norms) r = fimReadNorms(c, norms);
+        return r;
+    }
     HIPCHK(hipSetDevice(c->device));
     int rc = ensureFixup(c);
     if (rc != OCPG_OK) return rc;
@@ -832,7 +899,8 @@ int ocpg_update_assemble(ocpg_ctx* c, double dt, ocpg_resnorm* norms, ocpg_flash
 
 // IsoT_FIM::InitFlash (src/OCPFluidMethod.cpp:1095-1106): InitFlashFIM from
 // P, T, Sw (S column 2), rockVp and the feed z (host, nb x NC row-major).
-// Ni becomes an output.
+// Ni becomes an output.  IsoT_FIMn::InitFlash (:3129-3150, InitFlashFIMn)
+// in an IsoT_FIMn context.
 int ocpg_init_flash_fim(ocpg_ctx* c, const double* zi)
 {
     if (!c || !c->cur) return OCPG_ESTATE;
@@ -846,7 +914,9 @@ int ocpg_init_flash_fim(ocpg_ctx* c, const double* zi)
     HIPCHK(hipMalloc(&dz, t.size() * sizeof(double)));
     hipError_t e = hipMemcpy(dz, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice);
     timerBegin(c, T_INIT);
-    if (e == hipSuccess) e = launchFlash(c->NC, c->dEos, c->B, c->work, nullptr, dz, c->stream);
+    if (e == hipSuccess) e = launchFlash(c->NC, c->dEos, c->B, c->work, nullptr, dz, c->stream, nullptr, nullptr,
+                                         !c->fimn);
+    if (e == hipSuccess && c->fimn) e = launchFlashFinishN(c->NC, c->dEos, c->B, c->work, 1, c->stream);
     timerEnd(c, T_INIT);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(dz);
@@ -1057,6 +1127,8 @@ int64_t ocpg_device_bytes(ocpg_ctx* c)
 // Accessors used by the FIM half of the C-ABI (fim_capi.cpp).
 namespace ocpg {
 DevBulk& ctxBulk(ocpg_ctx* c) { return c->B; }
+bool ctxFimn(ocpg_ctx* c) { return c->fimn; }
+const double* ctxScm(ocpg_ctx* c) { return c->scm; }
 const DevBulk& ctxLastBulk(ocpg_ctx* c) { return c->Blast; }
 bool ctxLastValid(ocpg_ctx* c) { return c->lastValid; }
 FimDevice& ctxFim(ocpg_ctx* c) { return c->fim; }
@@ -1234,7 +1306,7 @@ int ctxRangeCopy(ocpg_ctx* c, int64_t base, ocpg_bulk* h, bool toDevice, bool sa
         if (input) snapOff += align256(fieldBytes);
         char* hp;
         std::memcpy(&hp, reinterpret_cast<char*>(h) + d.hostOff, sizeof(void*));
-        if (hp) HIPCHK(move(devBase, hp, d.elem, cnt));
+        if (hp && cnt > 0) HIPCHK(move(devBase, hp, d.elem, cnt));
     }
     if (h->lNi) HIPCHK(move(reinterpret_cast<char*>(c->B.lNi), reinterpret_cast<char*>(h->lNi), 8, c->NC + 1));
     return OCPG_OK;

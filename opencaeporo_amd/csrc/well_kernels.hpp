@@ -299,7 +299,11 @@ __global__ void __launch_bounds__(kWellBlock) wellFluxKernel(const DevEos* __res
 
 // Per perforation: dQ/dXp and dQ/dXs of the perforated cell (own cell only;
 // a perforation's flow is always upwinded at its cell), condensed with the
-// cell's dSec_dPri.
+// cell's dSec_dPri.  With W.fimn, IsoT_FIMn's (src/OCPFluidMethod.cpp:
+// 3524-3845): a producer's n_ij columns gain -transIJ dP / nj and its
+// diagonal term becomes transj xi dP / nj, and the perforation's
+// right-hand-side term -dQdXsB res_n (cells with two hydrocarbon phases)
+// goes to W.perfRhs.
 template <int NC>
 __global__ void __launch_bounds__(kWellBlock) wellAssembleKernel(DevBulk B, FimDevice F, WellDevice W, double dt)
 {
@@ -393,10 +397,11 @@ __global__ void __launch_bounds__(kWellBlock) wellAssembleKernel(DevBulk B, FimD
                     tmp += transIJ * B.dPcj_dS[(j * 3 + j1S) * ld + n];
                     v += tmp;
                 } else if (jX == j) {
-                    const double tmp = dP * tj[j] * xij *
-                                       (B.xix[(j * nc + kX) * ld + n] - xi / mu * B.mux[(j * nc + kX) * ld + n]);
+                    double tmp = dP * tj[j] * xij *
+                                 (B.xix[(j * nc + kX) * ld + n] - xi / mu * B.mux[(j * nc + kX) * ld + n]);
+                    if (W.fimn) tmp -= transIJ * dP / B.nj[j * ld + n];
                     v += tmp;
-                    if (kX == i) v += tj[j] * xi * dP;
+                    if (kX == i) v += W.fimn ? tj[j] * xi * dP / B.nj[j * ld + n] : tj[j] * xi * dP;
                 }
             }
         }
@@ -424,6 +429,16 @@ __global__ void __launch_bounds__(kWellBlock) wellAssembleKernel(DevBulk B, FimD
         }
     }
     if (lane < nc) W.perfDQW[(int64_t)p * nc + lane] = sDQW[lane];
+    if (W.fimn && lane < ncol) {
+        // DaAxpby(ncol, ncolB, -1, dQdXsB, res_n, 1, tmpb): row 0 of dQdXsB is zero
+        double y = 0.0;
+        if (lane > 0 && B.phaseNum[n] > 2) {
+            y = 1.0 * y;
+            const double* rw = &sDF[(lane - 1) * S];
+            for (int kq = 0; kq < ncolB; ++kq) y += -1.0 * rw[1 + kq] * B.resN[(int64_t)kq * ld + n];
+        }
+        W.perfRhs[(int64_t)p * ncol + lane] = y;
+    }
     __syncthreads();
     // well -> cell block: sum_i weight * row i+1 of M (rate control), zero (BHP)
     const double weight = inj ? 1.0 : W.prodWeight[w];
@@ -457,6 +472,12 @@ __global__ void __launch_bounds__(kWellBlock) wellDiagKernel(FimDevice F, WellDe
             double v = d[e];
             for (int q = q0; q < q1; ++q) v += W.perfBlock[(int64_t)W.cellPerfIdx[q] * bsize + e];
             d[e] = v;
+        }
+        // IsoT_FIMn: the perforations' right-hand-side terms, in well order
+        if (W.fimn && lane < ncol) {
+            double v = F.rhsN[n * ncol + lane];
+            for (int q = q0; q < q1; ++q) v += W.perfRhs[(int64_t)W.cellPerfIdx[q] * ncol + lane];
+            F.rhsN[n * ncol + lane] = v;
         }
         return;
     }

@@ -45,6 +45,10 @@ struct WellDevice {
     // assembly scratch and slots
     double* perfBlock = nullptr;  // nperf*bsize: dt (dQdXpB + dQdXsB dSec) of each perforation
     double* perfDQW = nullptr;    // nperf*nc: column 0 of dQdXpW
+    // IsoT_FIMn (fimn set): each perforation's right-hand-side term
+    // -dQdXsB res_n (nperf*ncol; zero where the cell has one hydrocarbon phase)
+    int fimn = 0;
+    double* perfRhs = nullptr;
     int32_t* slotBW = nullptr;    // per perforation: slot of block (cell, well) in the cell's row
     int32_t* slotWB = nullptr;    // per perforation: slot of block (well, cell) in the well's row
     void* mem = nullptr;          // one allocation holding every array above

@@ -36,6 +36,9 @@ def load():
         "ocpg_create": ([P, P, P, P, C.c_int], C.c_int),
         "ocpg_destroy": ([P], C.c_int),
         "ocpg_set_cells": ([P, C.c_int64], C.c_int),
+        "ocpg_set_method": ([P, C.c_int], C.c_int),
+        "ocpg_get_method": ([P], C.c_int),
+        "ocpg_critical_saturations": ([P, P], C.c_int),
         "ocpg_upload_bulk": ([P, P], C.c_int),
         "ocpg_download_bulk": ([P, P], C.c_int),
         "ocpg_flash_fim": ([P, P], C.c_int),
@@ -150,6 +153,15 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def set_method(self, method):
+        """abi.METHOD_FIM or abi.METHOD_FIMN (before set_cells)."""
+        check(self.lib.ocpg_set_method(self.h, method), "ocpg_set_method")
+
+    def critical_saturations(self):
+        out = np.zeros(3)
+        check(self.lib.ocpg_critical_saturations(self.h, out.ctypes.data), "ocpg_critical_saturations")
+        return out
 
     def set_cells(self, nb):
         check(self.lib.ocpg_set_cells(self.h, nb), "ocpg_set_cells")

@@ -206,6 +206,31 @@ int orc_init_flash_fimn(const ocpg_eos* eos, ocpg_bulk* bk, const double* zi, in
     return OCPG_OK;
 }
 
+// The FlashFIMn of every cell, keeping each cell's CaldXsdXp03 system:
+// M (nb x NSYS^2, column-major, dim rows), B (nb x NSYS (NC + 3)), dim (nb;
+// 0 for one hydrocarbon phase).  Test helper: the conditioning of that solve.
+int orc_fimn_systems(const ocpg_eos* eos, ocpg_bulk* bk, double* M, double* B, int32_t* dim)
+{
+    if (!eos || !bk || !M || !B || !dim) return OCPG_EINVAL;
+    const int NC = eos->nc, nc = NC + 1, np = oracle::NPH;
+    constexpr int NS = oracle::Mixture::NSYS;
+    oracle::Mixture mix(*eos);
+    for (int64_t n = 0; n < bk->nb; n++) {
+        oracle::SkipState s;
+        s.flag = bk->skipFlag[n] != 0;
+        s.minEigen = bk->skipMinEig[n];
+        s.P = bk->skipP[n];
+        s.T = bk->skipT[n];
+        for (int i = 0; i < NC; i++) s.zi[i] = bk->skipZi[n * NC + i];
+        mix.FlashFIMn(bk->P[n], bk->T[n], &bk->Ni[n * nc], &bk->S[n * np], bk->phaseNum[n], &bk->xij[n * np * nc],
+                      s);
+        dim[n] = mix.sysDim;
+        for (int k = 0; k < NS * NS; k++) M[n * NS * NS + k] = oracle::todbl(mix.sysM[k]);
+        for (int k = 0; k < NS * (OCPG_MAXNC + 3); k++) B[n * NS * (OCPG_MAXNC + 3) + k] = oracle::todbl(mix.sysB[k]);
+    }
+    return OCPG_OK;
+}
+
 // IsoT_FIMn::AssembleMat: AssembleMatBulksNew, AssembleMatWellsNew (wells
 // may be NULL), then AssembleRhsAccumulate(resAbs): rhs ((nb + open wells)
 // * ncol) = the bulk and perforation terms + resAbs.

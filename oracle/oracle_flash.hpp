@@ -100,6 +100,12 @@ public:
     // and its volume-balance term.
     real resN[NPH * (MAXNC + 2)];
     real resPc;
+    // CaldXsdXp03's system as it was handed to dgesv (column-major, sysDim
+    // rows, NC + 3 right-hand sides; sysDim = 0 for one hydrocarbon phase):
+    // kept for the tests that measure its conditioning
+    static constexpr int NSYS = NPH + 2 * MAXNC;
+    real sysM[NSYS * NSYS], sysB[NSYS * (MAXNC + 3)];
+    int sysDim = 0;
     long iters[5];  // SSMSTA, NRSTA, SSMSP, NRSP, RR for the last call
 
 private:

@@ -129,6 +129,7 @@ void Mixture::CaldXsdXpAPI03()
     const real vf2 = vf * vf;
     const real vwp = vjp[numPhase - 1];
     const real vw = vj[numPhase - 1];
+    sysDim = 0;
     if (NP == 1) {
         const int j0 = phaseLabel[0];
         real* bId = dXsdXp;
@@ -302,6 +303,9 @@ void Mixture::CaldXsdXp03()
     for (int j = 0; j < NP; j++) myRes[j] = S[phaseLabel[j]] - vj[phaseLabel[j]] / vf;
     myRes[NP] = S[numPhase - 1] - vj[numPhase - 1] / vf;
 
+    sysDim = nrow;
+    for (int k = 0; k < nrow * nrow; k++) sysM[k] = JmatDerN[k];
+    for (int k = 0; k < nrow * (numCom + 2); k++) sysB[k] = rhsDerN[k];
     LUSolveColMajor(nrow, numCom + 1 + 1, JmatDerN, rhsDerN);
 }
 

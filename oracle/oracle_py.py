@@ -53,6 +53,7 @@ class Oracle:
         L.orc_time_flash_fim.argtypes = [P, P, C.c_int]
         L.orc_time_flash_fim.restype = C.c_double
         L.orc_phase_props.argtypes = [P, C.c_double, C.c_double, P, P]
+        L.orc_fimn_systems.argtypes = [P, P, P, P, P]
         L.orc_flash_fimn.argtypes = [P, P, C.c_int]
         L.orc_init_flash_fimn.argtypes = [P, P, P, C.c_int]
         L.orc_assemble_fimn.argtypes = [P, P, P, P, C.c_double, P, P, P, P, P, P, P]
@@ -97,6 +98,31 @@ class Oracle:
         x = np.ascontiguousarray(x, dtype=np.float64)
         out = np.zeros(3)
         assert self.lib.orc_phase_props(C.byref(eos), P, T, x.ctypes.data, out.ctypes.data) == 0
+        return out
+
+    def fimn_systems(self, eos, bulk):
+        """Each cell's CaldXsdXp03 system (FlashFIMn on a copy of bulk):
+        list of (M, B) with M dim x dim and B dim x (NC + 3), None for a cell
+        with one hydrocarbon phase."""
+        b = bulk.copy()
+        NS = 3 + 2 * abi.MAXNC
+        nb = b.nb
+        M = np.zeros(nb * NS * NS)
+        Bv = np.zeros(nb * NS * (abi.MAXNC + 3))
+        dim = np.zeros(nb, dtype=np.int32)
+        s = b.cstruct()
+        assert self.lib.orc_fimn_systems(C.byref(eos), C.byref(s), M.ctypes.data, Bv.ctypes.data,
+                                         dim.ctypes.data) == 0
+        out = []
+        nr = b.NC + 3
+        for n in range(nb):
+            d = int(dim[n])
+            if d == 0:
+                out.append(None)
+                continue
+            m = M[n * NS * NS:n * NS * NS + d * d].reshape(d, d).T  # column-major
+            rhs = Bv[n * NS * (abi.MAXNC + 3):n * NS * (abi.MAXNC + 3) + d * nr].reshape(nr, d).T
+            out.append((m.copy(), rhs.copy()))
         return out
 
     def flash_fimn(self, eos, bulk, nthreads=1):

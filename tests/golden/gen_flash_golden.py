@@ -150,6 +150,9 @@ def gen_fimn():
     gen_flash("flashn_spe5.npz", fluids.spe5_deck(), 256, (111, 112), 113, method="fimn")
     gen_flash("flashn_ten.npz", fluids.ten_deck(), 128, (311, 312), 313, method="fimn")
     gen_init("initflashn_spe5.npz", fluids.spe5_deck(), 128, (411, 412), method="fimn")
+    # capped iterations: unconverged splits, whose correction res_n is large
+    gen_flash("flashn_limits_bench6.npz", fluids.bench6_deck(), 256, (511, 512), 513, limits=(10, 2, 8, 2),
+              method="fimn")
 
 
 def main():

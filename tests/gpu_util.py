@@ -29,10 +29,13 @@ def cell_bounds(sens):
     return np.maximum(GPU_TOL, SENS_FACTOR * sens)
 
 
-def gpu_flash(ctx, bulk, init_z=None):
+def gpu_flash(ctx, bulk, init_z=None, method=None):
     """Upload ``bulk``, run the flash (or InitFlash with ``init_z``), download
-    into a copy.  Returns (bulk_out, stats)."""
+    into a copy.  Returns (bulk_out, stats).  method: abi.METHOD_FIMN for
+    the IsoT_FIMn flash (set before the cells)."""
     out = bulk.copy()
+    if method is not None:
+        ctx.set_method(method)
     ctx.set_cells(bulk.nb)
     ctx.upload(bulk)
     stats = None
@@ -44,11 +47,12 @@ def gpu_flash(ctx, bulk, init_z=None):
     return out, stats
 
 
-def snapshot_dict(b, prefix):
-    """Fixture-like dict of a bulk's state fields (for parity.compare_state)."""
+def snapshot_dict(b, prefix, fimn=False):
+    """Fixture-like dict of a bulk's state fields (for parity.compare_state);
+    fimn: also IsoT_FIMn's nj, res_n and resPc."""
     from tests import parity
     d = {}
-    for f in parity.STATE_FIELDS + ["nonconv"]:
+    for f in parity.STATE_FIELDS + ["nonconv"] + (parity.FIMN_FIELDS if fimn else []):
         a = getattr(b, f)
         if f == "dSec_dPri":
             L = abi.max_len_dsdp(b.nc)

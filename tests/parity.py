@@ -13,6 +13,7 @@ INT_FIELDS = ["phaseNum", "phaseExist", "pSderExist", "pVnumCom", "bRowSizedSdP"
 FLOAT_FIELDS = ["S", "xij", "rho", "xi", "mu", "rhoP", "xiP", "muP", "rhox", "xix", "mux", "Nt",
                 "vf", "vfP", "vfi", "dSec_dPri", "skipP", "skipT", "skipZi"]
 STATE_FIELDS = INT_FIELDS + FLOAT_FIELDS + ["skipMinEig", "iters"]
+FIMN_FIELDS = ["nj", "res_n", "resPc"]
 
 
 def load(name):
@@ -93,12 +94,16 @@ def scaled_err(a, ref):
 
 
 def worst_scaled_err(b, d, prefix):
-    """Per cell: the largest scaled error over the floating state fields."""
+    """Per cell: the largest scaled error over the floating state fields
+    (with IsoT_FIMn's when d holds them)."""
     worst = np.zeros(b.nb)
     for f in FLOAT_FIELDS:
         ref = d[prefix + f]
         got = field(b, f, used=ref.shape[1] if f == "dSec_dPri" else None)
         worst = np.maximum(worst, scaled_err(got, ref.reshape(b.nb, -1)))
+    if prefix + "res_n" in d:
+        for e in fimn_errs(b, d, prefix).values():
+            worst = np.maximum(worst, e)
     return worst
 
 
@@ -118,7 +123,7 @@ def rounding_sensitivity(flash, bulk, perturbations=SENS_PERTURBATIONS):
     from tests import gpu_util
     base = bulk.copy()
     flash(base)
-    ref = gpu_util.snapshot_dict(base, "s_")
+    ref = gpu_util.snapshot_dict(base, "s_", fimn=base.resPc is not None and bool(base.res_n.any()))
     sens = np.zeros(bulk.nb)
     for what, e in perturbations:
         p = bulk.copy()
@@ -161,6 +166,19 @@ def fimn_errs(b, d, prefix):
     dr = dr / np.maximum(np.abs(ref), np.where(rr < nS[:, None], 1.0, Nt))
     rpc = np.abs(b.resPc - d[prefix + "resPc"]) / np.maximum(np.abs(d[prefix + "resPc"]), vf)
     return {"nj": nj.max(axis=1), "res_n": dr.max(axis=1), "resPc": rpc}
+
+
+def cell_field_errs(b, d, prefix, c):
+    """Scaled error of each floating field in cell c (diagnostics)."""
+    out = {}
+    for f in FLOAT_FIELDS:
+        ref = d[prefix + f]
+        got = field(b, f, used=ref.shape[1] if f == "dSec_dPri" else None)
+        out[f] = scaled_err(got, ref.reshape(b.nb, -1))[c]
+    if prefix + "res_n" in d:
+        for f, e in fimn_errs(b, d, prefix).items():
+            out[f] = e[c]
+    return out
 
 
 def compare_state(b, d, prefix, tol, int_exact=True, skip_tol=1e-4, frac=1.0, loose=None, cell_tol=None):
@@ -207,8 +225,14 @@ def compare_state(b, d, prefix, tol, int_exact=True, skip_tol=1e-4, frac=1.0, lo
     if cell_tol is not None:
         over = np.flatnonzero(worst > cell_tol)
         report["cells_over_own_bound"] = [int(i) for i in over]
+        if len(over):
+            it_ref = d[prefix + "iters"].reshape(b.nb, 5)
+            detail = {int(c): dict(iters=field(b, "iters")[c].tolist(), ref_iters=it_ref[c].tolist(),
+                                   fields={f: float(v) for f, v in cell_field_errs(b, d, prefix, c).items()
+                                           if v > cell_tol[c]})
+                      for c in over[:5]}
         assert len(over) == 0, (f"{len(over)} cells beyond their conditioning bound, first {over[:5]}: "
-                                f"err {worst[over[:5]]} bound {cell_tol[over[:5]]}")
+                                f"err {worst[over[:5]]} bound {cell_tol[over[:5]]}; {detail}")
     ref = d[prefix + "skipMinEig"]
     got = b.skipMinEig
     err = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-30)

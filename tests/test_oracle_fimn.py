@@ -34,7 +34,7 @@ from tests import parity
 TOL = 1e-8
 FRAC = 0.97
 LOOSE = 1e-4
-CASES = ["flashn_spe5.npz", "flashn_ten.npz"]
+CASES = ["flashn_spe5.npz", "flashn_ten.npz", "flashn_limits_bench6.npz"]
 
 
 @pytest.fixture(scope="module")
