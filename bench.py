@@ -52,9 +52,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def setup(nx, ny, nz, seed=2024, irregular=False, model=None):
+def setup(nx, ny, nz, seed=2024, irregular=False, model=None, method=abi.METHOD_FIM):
     """model: a deck.Model (the workload read from an input deck) instead of
-    the synthetic generators."""
+    the synthetic generators.  method: abi.METHOD_FIMN for IsoT_FIMn's flash
+    and assembly (InitFlashFIMn for the initial state)."""
     t0 = time.time()
     if model is not None:
         eos = lib.eos_from_deck(model.eos_deck)
@@ -83,6 +84,8 @@ def setup(nx, ny, nz, seed=2024, irregular=False, model=None):
         b.depth[:] = grid.depth
         b.poroInit[:] = 0.3
     ctx = lib.Context(eos, sat, rock)
+    if method != abi.METHOD_FIM:
+        ctx.set_method(method)
     ctx.set_cells(nb)
     ctx.upload(b)
     ctx.set_conn(grid)
@@ -116,6 +119,32 @@ def setup(nx, ny, nz, seed=2024, irregular=False, model=None):
     ctx.save_flash_inputs()
     ctx.synchronize()
     return ctx, grid, nb, NC, wells
+
+
+def fimn_leg(nx, ny, nz, steps, warmup):
+    """IsoT_FIMn on the same field: the initial state from InitFlashFIMn,
+    then warm Newton iterations as the serial calls (FlashFIMn +
+    PassFlashValue, kr/Pc + rock, CalRes, IsoT_FIMn::AssembleMat with its
+    right-hand side), timed like the headline."""
+    ctx, grid, nb, NC, _ = setup(nx, ny, nz, method=abi.METHOD_FIMN)
+    for _ in range(warmup):
+        step(ctx, True)
+    ctx.synchronize()
+    ctx.timing(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step(ctx, True)
+    ctx.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    kern = {k: ctx.timer(k) for k in ("restore", "flash", "flash_stability", "flash_split", "flash_finish",
+                                      "krpc_rock", "calres", "pack", "assemble")}
+    ctx.timing(False)
+    per = {k: (v[0] / max(v[1], 1)) for k, v in kern.items()}
+    ctx.close()
+    log(f"[bench] IsoT_FIMn: {ms:.2f} ms per step; per-launch ms: " +
+        ", ".join(f"{k} {v:.3f}" for k, v in per.items()))
+    return {"method": "IsoT_FIMn", "ms_per_step": ms, "cell_updates_per_s": nb / (ms / 1e3), "steps": steps,
+            "kernel_ms": per}
 
 
 def learned_leg(ctx, nb, NC, reps=3):
@@ -303,6 +332,7 @@ def main():
                     help="requests of the serving-form leg (p50/p99 latency and capacity, continuous batching "
                          "vs one at a time, 256 realisations of 1e3..1e6 cells); 0 skips it")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-form leg")
+    ap.add_argument("--no-fimn", action="store_true", help="skip the IsoT_FIMn leg")
     ap.add_argument("--deck", default=None,
                     help="read the workload from an input deck (opencaeporo_amd/deck.py), e.g. "
                          "tests/golden/decks/spe5refine/spe5-70x70x30-2y.data (a measurement case, not the headline)")
@@ -506,6 +536,12 @@ def main():
         "cpu_baseline": cpu,
     }
     ctx.close()
+    if not side_case and not a.no_fimn:
+        # IsoT_FIMn (secondary variables S and n_ij) on the same field
+        try:
+            out["config"]["fimn"] = fimn_leg(nx, ny, nz, max(3, a.steps // 2), 1)
+        except Exception as e:  # informative only
+            log(f"[bench] IsoT_FIMn leg failed: {e}")
     if rank == 0 and a.serving_requests > 0 and not side_case and not a.serial:
         # the serving form beside the headline: 256 ensemble realisations of
         # 1e3..1e6 cells, one Newton iteration per request, admission by a

@@ -8,10 +8,11 @@ from opencaeporo_amd import abi, fluids, lib, workloads
 from oracle.oracle_py import Oracle
 
 
-def build(inj_factor=None, seed=20241204):
-    """Returns dict(eos, sat, rock, grid, bulk, wells, setup, NC, factor).
-    inj_factor: lbmol per Mscf of the injected gas (1000 XiPhase at
-    surface conditions); the oracle's when None."""
+def build(inj_factor=None, seed=20241204, method="fim"):
+    """Returns dict(eos, sat, rock, grid, bulk, wells, setup, NC, factor,
+    method).  inj_factor: lbmol per Mscf of the injected gas (1000 XiPhase
+    at surface conditions); the oracle's when None.  method "fimn": the
+    initial state from InitFlashFIMn (IsoT_FIMn::InitReservoir)."""
     c = workloads.config1(seed=seed)
     deck = fluids.bench6_deck()
     eos = lib.eos_from_deck(deck)
@@ -29,7 +30,7 @@ def build(inj_factor=None, seed=20241204):
     b.depth[:] = grid.depth
     b.poroInit[:] = 0.3
     orc.calrock(rock, b)
-    orc.init_flash_fim(eos, b, c["z"])
+    (orc.init_flash_fimn if method == "fimn" else orc.init_flash_fim)(eos, b, c["z"])
     orc.calkrpc(sat, b)
     b.lNi[:] = b.Ni
     if inj_factor is None:
@@ -42,11 +43,13 @@ def build(inj_factor=None, seed=20241204):
     wells = abi.Wells(NC + 1, specs)
     setup = abi.WellSetup(*c["setup"])
     return dict(eos=eos, sat=sat, rock=rock, grid=grid, bulk=b, wells=wells, setup=setup, NC=NC,
-                factor=inj_factor)
+                factor=inj_factor, method=method)
 
 
 def device_context(case):
     ctx = lib.Context(case["eos"], case["sat"], case["rock"])
+    if case.get("method") == "fimn":
+        ctx.set_method(abi.METHOD_FIMN)
     b = case["bulk"]
     ctx.set_cells(b.nb)
     ctx.upload(b)

@@ -7,8 +7,27 @@ from opencaeporo_amd import abi
 from oracle.oracle_py import Oracle, WellStepState
 
 
+def critical_saturations(sat):
+    """Bulk::satcm of FlowUnit_ODGW01 (src/FlowUnit.cpp:196-202): oil 0; gas
+    and water the saturation of the last SGOF / SWOF row before kr reaches
+    1e-8 (OCPTable::GetRowZero, src/OCPTable.cpp:35-44)."""
+    def row_zero(tab, rows):
+        for r in range(rows):
+            if tab[r][1] >= 1e-8:
+                return r - 1
+        return rows - 1
+    rg = row_zero(sat.sgof, sat.sgof_rows)
+    rw = row_zero(sat.swof, sat.swof_rows)
+    return np.array([0.0, sat.sgof[rg][0] if rg >= 0 else 0.0, sat.swof[rw][0] if rw >= 0 else 0.0])
+
+
 class OracleBackend:
-    def __init__(self, eos, sat, rock, grid, bulk, wells=None, setup=None):
+    """method "fimn": IsoT_FIMn's method set (FlashFIMn, its assembly and
+    right-hand side, its GetSolution)."""
+
+    def __init__(self, eos, sat, rock, grid, bulk, wells=None, setup=None, method="fim"):
+        self.fimn = method == "fimn"
+        self.scm = critical_saturations(sat)
         self.orc = Oracle()
         self.eos, self.sat, self.rock, self.grid = eos, sat, rock, grid
         self.b = bulk
@@ -43,6 +62,10 @@ class OracleBackend:
         return dict(V=n.maxRelRes_V, N=n.maxRelRes_N, well=n.maxWellRelRes_mol)
 
     def assemble(self, dt):
+        if self.fimn:
+            val, rhs = self.orc.assemble_fimn(self.eos, self.grid, self.b, dt, self.up, self.urho, self.IA, self.JA,
+                                              self.res, wells=self.w)
+            return self.IA, self.JA, val, rhs
         if self.w is not None:
             val = self.orc.assemble_wells(self.eos, self.w, self.grid, self.b, dt, self.up, self.urho, self.IA, self.JA)
         else:
@@ -50,7 +73,10 @@ class OracleBackend:
         return self.IA, self.JA, val, self.res.copy()
 
     def get_solution(self, u, dSmaxlim):
-        self.book, (dp, dn) = self.orc.get_solution(self.nc, self.b, u, dSmaxlim)
+        if self.fimn:
+            self.book, (dp, dn) = self.orc.get_solution_n(self.nc, self.b, u, dSmaxlim, self.scm)
+        else:
+            self.book, (dp, dn) = self.orc.get_solution(self.nc, self.b, u, dSmaxlim)
         if self.w is not None:
             nb = self.b.nb
             for w in range(self.w.nw):
@@ -67,7 +93,7 @@ class OracleBackend:
         return 0
 
     def update_property(self, dt):
-        self.orc.flash_fim(self.eos, self.b)
+        (self.orc.flash_fimn if self.fimn else self.orc.flash_fim)(self.eos, self.b)
         self.orc.calkrpc(self.sat, self.b)
         self.orc.calrock(self.rock, self.b)
         return self.calres(dt)

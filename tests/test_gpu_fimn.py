@@ -219,3 +219,86 @@ def test_fimn_method_is_set_before_cells_and_pool_refused():
     assert ctx.lib.ocpg_get_method(ctx.h) == abi.METHOD_FIMN
     assert ctx.lib.ocpg_pool_init(ctx.h, 100, 6) == abi.EUNSUP
     ctx.close()
+
+
+def test_fimn_config1_ten_steps_match_the_oracle():
+    """The 147-cell two-well case for 10 time steps with IsoT_FIMn under the
+    host Newton loop: at every Newton iteration the device's FlashFIMn
+    outputs against the oracle's from the same primary variables, and the
+    device's system (blocks and right-hand side) against the oracle's
+    assembly of the device's own state; then the same loop on the oracle
+    alone takes the same steps with the same Newton counts and rollback."""
+    from opencaeporo_amd import fim_loop
+    from tests import config1_case
+    from tests.oracle_backend import OracleBackend
+    from tests.test_gpu_config1 import _device_wells, residual_scale
+    gpu_util.say("[fimn config 1] 7x7x3, gas injector + oil producer, 10 FIM steps")
+    probe = config1_case.build(method="fimn")
+    ctx0 = config1_case.device_context(probe)
+    xi_dev = ctx0.phase_density(14.7, 60.0, probe["wells"].injZi[:probe["NC"] + 1], 1)[0]
+    ctx0.close()
+    case = config1_case.build(inj_factor=1000.0 * xi_dev, method="fimn")
+    ref_case = config1_case.build(inj_factor=1000.0 * xi_dev, method="fimn")
+    nc = case["NC"] + 1
+    ncol = nc + 1
+    ctx = config1_case.device_context(case)
+    dev = fim_loop.DeviceBackend(ctx, has_wells=True)
+    orc = Oracle()
+    stats = dict(iters=0, worst_state=0.0, worst_rhs=0.0, worst_block=0.0)
+    pre = {}
+
+    def observer(event, loop):
+        b = case["bulk"]
+        if event == "solution":
+            pre["b"] = b.copy()
+            ctx.download(pre["b"])
+        elif event == "update":
+            ob = pre["b"]
+            orc.flash_fimn(case["eos"], ob)
+            gb = b.copy()
+            ctx.download(gb)
+            assert np.array_equal(gb.phaseNum, ob.phaseNum), "phase counts differ"
+            assert np.array_equal(gb.phaseExist, ob.phaseExist), "phase labels differ"
+            for f in ("S", "xij", "rho", "xi", "mu", "vf", "dSec_dPri", "vfi"):
+                e = parity.scaled_err(getattr(gb, f), getattr(ob, f)).max()
+                stats["worst_state"] = max(stats["worst_state"], e)
+                assert e <= 1e-10, (f, e)
+        elif event == "system":
+            gb = b.copy()
+            ctx.download(gb)
+            w = _device_wells(ctx, case["wells"])
+            res, up, urho, _, _ = orc.calres_wells(case["eos"], w, case["grid"], gb, loop.dt)
+            IA, JA = orc.bsr_pattern_wells(case["grid"], w)
+            val_o, rhs_o = orc.assemble_fimn(case["eos"], case["grid"], gb, loop.dt, up, urho, IA, JA, res,
+                                             wells=w)
+            gIA, gJA, gval, grhs = loop.system
+            assert np.array_equal(gIA, IA) and np.array_equal(gJA, JA)
+            err = block_err(gval, val_o, ncol)
+            stats["worst_block"] = max(stats["worst_block"], float(err.max()))
+            assert err.max() <= 1e-10
+            e = (np.abs(grhs - rhs_o) / residual_scale(gb, w, ncol)).max()
+            stats["worst_rhs"] = max(stats["worst_rhs"], e)
+            assert e <= 1e-10
+            stats["iters"] += 1
+
+    loop = fim_loop.FimLoop(dev, observer=observer)
+    recs = loop.run(10)
+    gpu_util.say("  device: dt", [round(r.dt, 4) for r in recs], "NR", [r.nr for r in recs],
+                 "cuts", [r.cuts for r in recs])
+    gpu_util.say(f"  lockstep over {stats['iters']} Newton systems: state {stats['worst_state']:.2e}, "
+                 f"rhs {stats['worst_rhs']:.2e}, blocks {stats['worst_block']:.2e}")
+    ob = OracleBackend(ref_case["eos"], ref_case["sat"], ref_case["rock"], ref_case["grid"], ref_case["bulk"],
+                       ref_case["wells"], ref_case["setup"], method="fimn")
+    orecs = fim_loop.FimLoop(ob).run(10)
+    gpu_util.say("  oracle: dt", [round(r.dt, 4) for r in orecs], "NR", [r.nr for r in orecs],
+                 "cuts", [r.cuts for r in orecs])
+    assert [r.nr for r in recs] == [r.nr for r in orecs]
+    assert [r.cuts for r in recs] == [r.cuts for r in orecs]
+    np.testing.assert_allclose([r.dt for r in recs], [r.dt for r in orecs], rtol=1e-8)
+    fin = case["bulk"].copy()
+    ctx.download(fin)
+    ofin = ref_case["bulk"]
+    assert np.array_equal(fin.phaseNum, ofin.phaseNum)
+    np.testing.assert_allclose(fin.P, ofin.P, rtol=1e-8)
+    np.testing.assert_allclose(fin.Ni, ofin.Ni, rtol=1e-7, atol=1e-7 * np.abs(ofin.Ni).max())
+    ctx.close()

@@ -167,3 +167,21 @@ def test_fimn_get_solution_moves_phase_moles_by_the_update():
         np.testing.assert_allclose(new - old, uN[n], rtol=1e-8, atol=1e-12 * np.abs(old).max())
     assert np.array_equal(b.S, before.S)  # saturations wait for the next flash
     np.testing.assert_allclose(b.Ni.reshape(nb, nc)[:, :NC], before.Ni.reshape(nb, nc)[:, :NC] + uN, rtol=1e-14)
+
+
+def test_fimn_newton_loop_on_the_oracle_converges():
+    """The 147-cell two-well case for 10 time steps under the host Newton
+    loop with IsoT_FIMn's method set on the oracle: every step converges
+    within the reference's Newton limit, and the component moles stay
+    positive."""
+    from opencaeporo_amd import fim_loop
+    from tests import config1_case
+    from tests.oracle_backend import OracleBackend
+    case = config1_case.build(method="fimn")
+    ob = OracleBackend(case["eos"], case["sat"], case["rock"], case["grid"], case["bulk"], case["wells"],
+                       case["setup"], method="fimn")
+    recs = fim_loop.FimLoop(ob).run(10)
+    assert all(r.nr <= 10 for r in recs)
+    assert np.all(case["bulk"].Ni >= 0)
+    print("FIMn oracle loop: dt", [round(r.dt, 3) for r in recs], "NR", [r.nr for r in recs],
+          "cuts", [r.cuts for r in recs])
