@@ -341,8 +341,8 @@ int ocpg_assemble(ocpg_ctx* ctx, double dt);
  * formed, and then recomputes the rows that read the slow cells.  Identical
  * results to the four calls.  norms and stats may be NULL (then the call is
  * asynchronous).  Needs ocpg_set_conn and the kr/Pc tables.  In an
- * IsoT_FIMn context the four calls run one after the other (no side
- * streams).                                                                 */
+ * IsoT_FIMn context the same with IsoT_FIMn's flash, assembly and
+ * right-hand side.                                                          */
 int ocpg_update_assemble(ocpg_ctx* ctx, double dt, ocpg_resnorm* norms, ocpg_flash_stats* stats);
 /* IsoT_FIM::GetSolution (src/OCPFluidMethod.cpp:2735-2857) on the resident
  * state: the Newton update of P, Ni, S and x_ij with the saturation chop

@@ -1112,14 +1112,14 @@ __global__ void __launch_bounds__(256) rhsAccumulateKernel(double* __restrict__ 
 }
 
 // The rows in rows[0, *count) (the overlapped form's recomputation).
-template <int NC>
+template <int NC, bool FN = false>
 __global__ void __launch_bounds__(kAsmBlock) assembleListKernel(DevBulk B, FimDevice F, double dt,
                                                                 const int32_t* __restrict__ rows,
                                                                 const int32_t* __restrict__ count)
 {
     __shared__ AsmShared<NC> sh;
     const int64_t total = *count;
-    for (int64_t i = blockIdx.x; i < total; i += gridDim.x) assembleRow<NC>(B, F, dt, rows[i], sh);
+    for (int64_t i = blockIdx.x; i < total; i += gridDim.x) assembleRow<NC, FN>(B, F, dt, rows[i], sh);
 }
 
 template <int NC>
@@ -1154,7 +1154,11 @@ static hipError_t launchAssembleRowsT(const DevBulk& B, FimDevice& F, double dt,
                                       const int32_t* count, hipStream_t s)
 {
     const unsigned grid = (unsigned)std::min<int64_t>(B.nb, 4096);
-    hipLaunchKernelGGL(assembleListKernel<NC>, dim3(grid), dim3(kAsmBlock), 0, s, B, F, dt, rows, count);
+    if (F.fimn)
+        hipLaunchKernelGGL((assembleListKernel<NC, true>), dim3(grid), dim3(kAsmBlock), 0, s, B, F, dt, rows,
+                           count);
+    else
+        hipLaunchKernelGGL((assembleListKernel<NC>), dim3(grid), dim3(kAsmBlock), 0, s, B, F, dt, rows, count);
     return hipGetLastError();
 }
 

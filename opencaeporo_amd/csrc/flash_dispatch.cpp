@@ -37,6 +37,15 @@ hipError_t launchFlashFinishN(int NC, const DevEos* eos, const DevBulk& B, const
 }
 
 
+hipError_t launchFlashSideFinishN(int NC, const DevEos* eos, const DevBulk& B, const FlashWork& W, hipStream_t s)
+{
+#define X(N) \
+    if (NC == N) return launchFlashSideFinishNT<N>(eos, B, W, s);
+    OCPG_NC_LIST(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
 hipError_t launchFlashSideFinish(int NC, const DevEos* eos, const DevBulk& B, const FlashWork& W, hipStream_t s)
 {
 #define X(N) \

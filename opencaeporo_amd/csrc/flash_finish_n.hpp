@@ -655,6 +655,82 @@ __global__ void __launch_bounds__(kFlashBlock) flashFinishNKernel(const DevEos* 
     }
 }
 
+// The IsoT_FIMn finishing stage of the overlapped flash's side-stream cells
+// (after the join) that end with NPT hydrocarbon phases: the wavefront
+// chunks of lists B, C and D in turn, each cell as flashFinishNKernel
+// finishes it.  One instance per phase count keeps each kernel's register
+// use that of the main stream's finishing kernel.
+template <int NC, int NPT>
+__global__ void __launch_bounds__(kFlashBlock) flashFinishSideNKernel(const DevEos* __restrict__ eos, DevBulk B,
+                                                                      FlashWork W)
+{
+    __shared__ double sSq[FlashCell<NC>::NT * kFlashBlock];
+    const int lists[3] = {NPT == 1 ? LIST_NP1B : LIST_NP2B, NPT == 1 ? LIST_NP1C : LIST_NP2C,
+                          NPT == 1 ? LIST_NP1D : LIST_NP2D};
+    int cnt[3], chunks = 0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        cnt[q] = W.count[lists[q]];
+        chunks += (cnt[q] + 63) / 64;
+    }
+    const int64_t ld = B.ld;
+    constexpr int nc = NC + 1;
+    for (int ch = (int)((blockIdx.x * blockDim.x + threadIdx.x) / 64); ch < chunks;
+         ch += (int)(gridDim.x * blockDim.x / 64)) {
+        int r = ch, list = -1, idx = 0;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const int nch = (cnt[q] + 63) / 64;
+            if (list == -1) {
+                if (r < nch) {
+                    idx = r * 64 + (int)(threadIdx.x & 63);
+                    list = idx < cnt[q] ? lists[q] : -2;
+                } else {
+                    r -= nch;
+                }
+            }
+        }
+        if (list < 0) continue;
+        const int32_t n = W.list[list][idx];
+        FlashCell<NC> F(*eos, &sSq[threadIdx.x]);
+        loadFeed<NC>(F, B, W, n);
+        F.flagSkip = (W.flags[n] & FL_SKIP) != 0;
+        F.NP = NPT;
+#pragma unroll
+        for (int i = 0; i < nc; ++i) F.Ni[i] = B.Ni[i * ld + n];
+        F.Nh = W.Nh[n];
+        F.calAiBi();
+        if (NPT == 1) {
+            F.nu[0] = 1;
+#pragma unroll
+            for (int i = 0; i < NC; ++i) F.x[0][i] = F.zi[i];
+            F.calAjBj(F.Aj[0], F.Bj[0], F.x[0]);
+            F.Zj[0] = solEoS(F.Aj[0], F.Bj[0]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                F.nu[j] = W.nu[j * ld + n];
+#pragma unroll
+                for (int i = 0; i < NC; ++i) F.x[j][i] = W.x[(j * NC + i) * ld + n];
+            }
+            F.calFugPhiAll();
+        }
+#pragma unroll
+        for (int j = 0; j < NPT; ++j) F.nu[j] = F.Nh * F.nu[j];
+        FlashKernelN<NC>::template finish<NPT>(F, *eos, B, n, false, 0.0, 0.0);
+    }
+}
+
+template <int NC>
+hipError_t launchFlashSideFinishNT(const DevEos* eos, const DevBulk& B, const FlashWork& W, hipStream_t s)
+{
+    if (B.nb == 0) return hipSuccess;
+    const unsigned grid = std::min<unsigned>((unsigned)((B.nb + kFlashBlock - 1) / kFlashBlock), 256);
+    hipLaunchKernelGGL((flashFinishSideNKernel<NC, 1>), dim3(grid), dim3(kFlashBlock), 0, s, eos, B, W);
+    hipLaunchKernelGGL((flashFinishSideNKernel<NC, 2>), dim3(grid), dim3(kFlashBlock), 0, s, eos, B, W);
+    return hipGetLastError();
+}
+
 template <int NC>
 hipError_t launchFlashFinishNT(const DevEos* eos, const DevBulk& B, const FlashWork& W, int init, hipStream_t s)
 {

@@ -1241,12 +1241,14 @@ __global__ void __launch_bounds__(kFlashBlock) flashFinishSideKernel(const DevEo
         for (int q = 0; q < 3; ++q) {
             const int cnt = two ? c2[q] : c1[q];
             const int nch = (cnt + 63) / 64;
-            if (list < 0 && r < nch) {
-                list = two ? l2[q] : l1[q];
-                idx = r * 64 + (int)(threadIdx.x & 63);
-                if (idx >= cnt) list = -2;
-            } else if (list == -1) {
-                r -= nch;
+            if (list == -1) {
+                if (r < nch) {
+                    // the chunk is in this list; its lanes past the end idle
+                    idx = r * 64 + (int)(threadIdx.x & 63);
+                    list = idx < cnt ? (two ? l2[q] : l1[q]) : -2;
+                } else {
+                    r -= nch;
+                }
             }
         }
         if (list < 0) continue;
@@ -1341,7 +1343,7 @@ hipError_t launchFlashT(const DevEos* eos, const DevBulk& B, const FlashWork& W,
     // serial order, when it is long (a cold pass puts every cell through the
     // stability test): a kernel pair gated on the list's length on the
     // device, exactly one of which works.
-    if (W.guess || zin || !finish) return hipErrorInvalidValue;
+    if (W.guess || zin) return hipErrorInvalidValue;
     if ((e = hipEventRecord(side->classified, s)) != hipSuccess) return e;
     // side stream B: the stability test and the split of the cells it finds unstable
     if ((e = hipStreamWaitEvent(side->sB, side->classified, 0)) != hipSuccess) return e;
@@ -1389,6 +1391,7 @@ hipError_t launchFlashT(const DevEos* eos, const DevBulk& B, const FlashWork& W,
     // main stream: a long restart list, then finish the cells it holds
     hipLaunchKernelGGL(flashSplitTailKernel<NC>, dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, ctr,
                        (int)LIST_TNR, (int)TAIL_RESTART, (int)LIST_NP1, (int)LIST_NP2, -kSideMaxRestart);
+    if (!finish) return hipGetLastError();
     mark(4, s);
     hipLaunchKernelGGL((flashFinishKernel<NC, 1>), dim3(qgrid), dim3(kFlashBlock), 0, s, eos, B, W, 0,
                        (int)LIST_NP1);

@@ -29,8 +29,9 @@ struct FlashSide {
 // D's.  With side non-NULL the slow cells continue on the side streams (see
 // flash_impl.hpp): the caller makes s wait for side->doneB, doneC and doneD
 // and calls launchFlashSideFinish before anything reads those cells.
-// With finish false (serial form only) the finishing stage is left out:
-// the IsoT_FIMn flash runs its own (launchFlashFinishN) on the same lists.
+// With finish false the finishing stage of the main stream's cells is left
+// out: the IsoT_FIMn flash runs its own (launchFlashFinishN) on the same
+// lists, and after the join launchFlashSideFinishN for the side streams'.
 hipError_t launchFlash(int NC, const DevEos* eos, const DevBulk& B, const FlashWork& W, DevFlashCounters* ctr,
                        const double* zin, hipStream_t s, hipEvent_t* ev = nullptr, const FlashSide* side = nullptr,
                        bool finish = true);
@@ -38,6 +39,7 @@ hipError_t launchFlash(int NC, const DevEos* eos, const DevBulk& B, const FlashW
 // equilibrium, flash_finish_n.hpp) for the cells the serial flash finished.
 hipError_t launchFlashFinishN(int NC, const DevEos* eos, const DevBulk& B, const FlashWork& W, int init,
                               hipStream_t s);
+hipError_t launchFlashSideFinishN(int NC, const DevEos* eos, const DevBulk& B, const FlashWork& W, hipStream_t s);
 // After the join: the finishing stage of the side streams' cells, and
 // those cells.
 hipError_t launchFlashSideFinish(int NC, const DevEos* eos, const DevBulk& B, const FlashWork& W, hipStream_t s);
@@ -62,6 +64,8 @@ hipError_t launchFlashT(const DevEos* eos, const DevBulk& B, const FlashWork& W,
                         const double* zin, hipStream_t s, hipEvent_t* ev, const FlashSide* side, bool finish);
 template <int NC>
 hipError_t launchFlashFinishNT(const DevEos* eos, const DevBulk& B, const FlashWork& W, int init, hipStream_t s);
+template <int NC>
+hipError_t launchFlashSideFinishNT(const DevEos* eos, const DevBulk& B, const FlashWork& W, hipStream_t s);
 template <int NC>
 hipError_t launchFlashSideFinishT(const DevEos* eos, const DevBulk& B, const FlashWork& W, hipStream_t s);
 

@@ -8,4 +8,5 @@
 
 namespace ocpg {
 template hipError_t launchFlashFinishNT<OCPG_NC>(const DevEos*, const DevBulk&, const FlashWork&, int, hipStream_t);
+template hipError_t launchFlashSideFinishNT<OCPG_NC>(const DevEos*, const DevBulk&, const FlashWork&, hipStream_t);
 }  // namespace ocpg

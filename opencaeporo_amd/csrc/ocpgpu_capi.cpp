@@ -822,17 +822,6 @@ int ocpg_update_assemble(ocpg_ctx* c, double dt, ocpg_resnorm* norms, ocpg_flash
     FimDevice& F = c->fim;
     if (F.nb <= 0 || F.nb != c->B.nb) return OCPG_ESTATE;
     if (!c->dSat) return OCPG_ESTATE;
-    if (c->fimn) {
-        // IsoT_FIMn: the serial calls (no overlapped form)
-        timerBegin(c, T_UPDASM);
-        int r = ocpg_flash_fim(c, stats);
-        if (r == OCPG_OK) r = ocpg_calkrpc_rock(c);
-        if (r == OCPG_OK) r = ocpg_calres(c, dt, nullptr);
-        if (r == OCPG_OK) r = ocpg_assemble(c, dt);
-        timerEnd(c, T_UPDASM);
-        if (r == OCPG_OK && norms) r = fimReadNorms(c, norms);
-        return r;
-    }
     HIPCHK(hipSetDevice(c->device));
     int rc = ensureFixup(c);
     if (rc != OCPG_OK) return rc;
@@ -843,7 +832,13 @@ int ocpg_update_assemble(ocpg_ctx* c, double dt, ocpg_resnorm* norms, ocpg_flash
     timerBegin(c, T_UPDASM);
     timerBegin(c, T_FLASH);
     HIPCHK(launchFlash(c->NC, c->dEos, c->B, c->work, stats ? c->dCtr : nullptr, nullptr, c->stream,
-                       stageEv ? ev : nullptr, &c->side));
+                       stageEv ? ev : nullptr, &c->side, !c->fimn));
+    if (c->fimn) {
+        // IsoT_FIMn: FlashFIMn's finishing stage of the main stream's cells
+        if (stageEv) HIPCHK(hipEventRecord(ev[4], c->stream));
+        HIPCHK(launchFlashFinishN(c->NC, c->dEos, c->B, c->work, 0, c->stream));
+        if (stageEv) HIPCHK(hipEventRecord(ev[5], c->stream));
+    }
     timerEnd(c, T_FLASH);
     timerBegin(c, T_KRPC);
     HIPCHK(launchKrPcRock(c->B, c->dSat, c->rock, true, true, c->stream));
@@ -854,14 +849,20 @@ int ocpg_update_assemble(ocpg_ctx* c, double dt, ocpg_resnorm* norms, ocpg_flash
     HIPCHK(launchCalResPack(c->NC, c->B, F, W, dt, c->stream));
     timerEnd(c, T_RES);
     timerBegin(c, T_ASM);
-    HIPCHK(launchAssemble(c->NC, c->B, F, dt, c->stream));
+    if (c->fimn)
+        HIPCHK(launchAssembleN(c->NC, c->B, F, dt, c->stream));
+    else
+        HIPCHK(launchAssemble(c->NC, c->B, F, dt, c->stream));
     timerEnd(c, T_ASM);
     // join the side streams, finish their cells, recompute what read them
     HIPCHK(hipStreamWaitEvent(c->stream, c->side.doneB, 0));
     HIPCHK(hipStreamWaitEvent(c->stream, c->side.doneC, 0));
     HIPCHK(hipStreamWaitEvent(c->stream, c->side.doneD, 0));
     timerBegin(c, T_FIXUP);
-    HIPCHK(launchFlashSideFinish(c->NC, c->dEos, c->B, c->work, c->stream));
+    if (c->fimn)
+        HIPCHK(launchFlashSideFinishN(c->NC, c->dEos, c->B, c->work, c->stream));
+    else
+        HIPCHK(launchFlashSideFinish(c->NC, c->dEos, c->B, c->work, c->stream));
     const CellLists sideCells = flashSideCells(c->work);
     HIPCHK(launchKrPcRockList(c->B, c->dSat, c->rock, sideCells, c->stream));
     HIPCHK(launchFixupRows(F, W, sideCells, c->fix, c->stream));
@@ -870,6 +871,7 @@ int ocpg_update_assemble(ocpg_ctx* c, double dt, ocpg_resnorm* norms, ocpg_flash
     HIPCHK(launchPackRecordsList(c->NC, c->B, F, sideCells, c->stream));
     HIPCHK(launchAssembleRows(c->NC, c->B, F, dt, c->fix.asmRows, c->fix.counts + 1, c->stream));
     if (W.nw > 0) HIPCHK(launchWellAssemble(c->NC, c->B, F, W, dt, c->stream));
+    if (c->fimn) HIPCHK(launchRhsAccumulate(F, c->stream));
     HIPCHK(launchResNorm(c->NC, c->B, F, c->stream));
     timerEnd(c, T_FIXUP);
     timerEnd(c, T_UPDASM);

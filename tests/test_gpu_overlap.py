@@ -21,7 +21,7 @@ pytestmark = pytest.mark.gpu
 DT = 5.0
 
 
-def make_field(nx, ny, nz, seed, irregular=False):
+def make_field(nx, ny, nz, seed, irregular=False, method=abi.METHOD_FIM):
     deck = fluids.ten_deck() if irregular else fluids.bench6_deck()
     eos = lib.eos_from_deck(deck)
     sat = fluids.make_satfunc()
@@ -39,6 +39,8 @@ def make_field(nx, ny, nz, seed, irregular=False):
     b.depth[:] = grid.depth
     b.poroInit[:] = 0.3
     ctx = lib.Context(eos, sat, rock)
+    if method != abi.METHOD_FIM:
+        ctx.set_method(method)
     ctx.set_cells(nb)
     ctx.upload(b)
     ctx.set_conn(grid)
@@ -161,4 +163,23 @@ def test_overlapped_irregular_full_size():
     wells = workloads.irregular_wells(grid, P, NC, seed=2027)
     ctx.set_wells(wells)
     compare(ctx, grid, NC, wells=wells, label="irregular")
+    ctx.close()
+
+
+@pytest.mark.parametrize("irregular", [False, True])
+def test_overlapped_fimn(irregular):
+    """IsoT_FIMn's overlapped iteration (its finishing stage on the main
+    and the side streams' cells, its assembly and right-hand side, the rows
+    recomputed after the join) against its serial calls, byte for byte."""
+    gpu_util.say(f"[overlap] IsoT_FIMn warm pass, irregular {irregular}")
+    if irregular:
+        ctx, grid, NC = make_field(60, 220, 5, seed=514, irregular=True, method=abi.METHOD_FIMN)
+        P = warm_up(ctx, grid, NC, seed=515)
+        wells = workloads.irregular_wells(grid, P, NC, seed=516)
+        ctx.set_wells(wells)
+    else:
+        ctx, grid, NC = make_field(60, 220, 12, seed=512, method=abi.METHOD_FIMN)
+        warm_up(ctx, grid, NC, seed=513)
+        wells = None
+    compare(ctx, grid, NC, wells=wells, label="fimn")
     ctx.close()
