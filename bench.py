@@ -343,6 +343,8 @@ def main():
                          "vs one at a time, 256 realisations of 1e3..1e6 cells); 0 skips it")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-form leg")
     ap.add_argument("--no-fimn", action="store_true", help="skip the IsoT_FIMn leg")
+    ap.add_argument("--fimn-only", action="store_true",
+                    help="time only the IsoT_FIMn leg (--steps, --warmup) and print its record")
     ap.add_argument("--deck", default=None,
                     help="read the workload from an input deck (opencaeporo_amd/deck.py), e.g. "
                          "tests/golden/decks/spe5refine/spe5-70x70x30-2y.data (a measurement case, not the headline)")
@@ -354,6 +356,9 @@ def main():
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
     nx, ny, nz = 60, 220, a.nz
+    if a.fimn_only:
+        print(json.dumps(fimn_leg(nx, ny, nz, a.steps, a.warmup)), flush=True)
+        return
     model = None
     if a.deck:
         from opencaeporo_amd import deck

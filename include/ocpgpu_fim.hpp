@@ -55,11 +55,23 @@ struct BsrSystem {
 class IsoT_FIM_GPU {
 public:
     /* MixtureComp's deck data (ParamReservoir::comsParam), the saturation
-     * tables and the rock model; device = GPU ordinal. */
-    IsoT_FIM_GPU(const ocpg_eos_deck& deck, const ocpg_satfunc& sat, const ocpg_rock& rock, int device = 0)
+     * tables and the rock model; device = GPU ordinal.  method
+     * OCPG_METHOD_FIMN gives IsoT_FIMn's method set (src/OCPFluidMethod.cpp:
+     * 2962-4017) behind the same members: the bulk table then also binds
+     * nj, res_n and resPc. */
+    IsoT_FIM_GPU(const ocpg_eos_deck& deck, const ocpg_satfunc& sat, const ocpg_rock& rock, int device = 0,
+                 int method = OCPG_METHOD_FIM)
     {
         check(ocpg_eos_from_deck(&deck, &eos_), "ocpg_eos_from_deck");
         check(ocpg_create(&ctx_, &eos_, &sat, &rock, device), "ocpg_create");
+        if (method != OCPG_METHOD_FIM) {
+            const int rc = ocpg_set_method(ctx_, method);
+            if (rc != OCPG_OK) {
+                ocpg_destroy(ctx_);
+                ctx_ = nullptr;
+                check(rc, "ocpg_set_method");
+            }
+        }
     }
     IsoT_FIM_GPU(const IsoT_FIM_GPU&) = delete;
     IsoT_FIM_GPU& operator=(const IsoT_FIM_GPU&) = delete;
@@ -74,7 +86,8 @@ public:
     }
 
     /* IsoT_FIM::InitReservoir: InitRock (CalRock), InitFlash (InitFlashFIM
-     * from the feed zi, nb x NC row-major) and CalKrPc; the result becomes
+     * from the feed zi, nb x NC row-major; InitFlashFIMn for IsoT_FIMn,
+     * src/OCPFluidMethod.cpp:2975-2989) and CalKrPc; the result becomes
      * the last time step.  bk supplies P, T, S (water column), v, depth and
      * poroInit and receives the flash outputs. */
     void InitReservoir(ocpg_bulk& bk, const double* zi)

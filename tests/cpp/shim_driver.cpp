@@ -1,5 +1,6 @@
 // Test driver for the IsoT_FIM-shaped C++ shim (include/ocpgpu_fim.hpp): one
-// FIM Newton iteration the way IsothermalSolver sequences IsoT_FIM
+// FIM (method OCPG_METHOD_FIM) or FIMn (OCPG_METHOD_FIMN) Newton iteration
+// the way IsothermalSolver sequences IsoT_FIM
 // (src/IsothermalSolver.cpp:79-155), from host arrays in the reference's
 // layout.  Built by opencaeporo_amd/build.py into tests/cpp/libshim_driver.so
 // and called from tests/test_gpu_shim.py.
@@ -12,10 +13,11 @@
 extern "C" int shim_newton_iteration(const ocpg_eos_deck* deck, const ocpg_satfunc* sat, const ocpg_rock* rock,
                                      int64_t nb, const ocpg_conn* conn, ocpg_bulk* bk, const double* zi,
                                      const ocpg_wells* wells, const double* newP, const double* newNi, double dt,
-                                     int32_t* IA, int32_t* JA, double* val, double* rhs, ocpg_resnorm* norms)
+                                     int32_t* IA, int32_t* JA, double* val, double* rhs, ocpg_resnorm* norms,
+                                     int method)
 {
     try {
-        ocpg::IsoT_FIM_GPU fim(*deck, *sat, *rock, 0);
+        ocpg::IsoT_FIM_GPU fim(*deck, *sat, *rock, 0, method);
         fim.Setup(nb, *conn);
         fim.InitReservoir(*bk, zi);          // InitRock, InitFlash, CalKrPc, UpdateLastTimeStep
         if (wells) fim.Prepare(*wells);      // PrepareWell's per-step values

@@ -1,5 +1,6 @@
 """The IsoT_FIM-shaped C++ host shim (include/ocpgpu_fim.hpp), driven from
-C++ (tests/cpp/shim_driver.cpp) through one FIM Newton iteration with wells:
+C++ (tests/cpp/shim_driver.cpp) through one Newton iteration with wells, as
+IsoT_FIM and as IsoT_FIMn:
 its linear system must be the one the same C-ABI calls give, bit for bit,
 and match the oracle's within the parity bound."""
 import ctypes as C
@@ -16,13 +17,16 @@ pytestmark = pytest.mark.gpu
 DT = 5.0
 
 
-def test_shim_newton_iteration_matches_c_abi_and_oracle():
-    gpu_util.say("[shim] one Newton iteration through IsoT_FIM_GPU")
+@pytest.mark.parametrize("method", ["fim", "fimn"])
+def test_shim_newton_iteration_matches_c_abi_and_oracle(method):
+    gpu_util.say(f"[shim] one Newton iteration through IsoT_FIM_GPU, method {method}")
+    fimn = method == "fimn"
+    mcode = abi.METHOD_FIMN if fimn else abi.METHOD_FIM
     assert os.path.exists(build.SHIM_LIB), "build the library first (python -m opencaeporo_amd.build)"
     lib.load()  # libocpgpu.so first: the driver links against it
     drv = C.CDLL(build.SHIM_LIB)
     P = C.c_void_p
-    drv.shim_newton_iteration.argtypes = [P, P, P, C.c_int64, P, P, P, P, P, P, C.c_double, P, P, P, P, P]
+    drv.shim_newton_iteration.argtypes = [P, P, P, C.c_int64, P, P, P, P, P, P, C.c_double, P, P, P, P, P, C.c_int]
     case = fim_case.build_case(6, 5, 4, seed=21)
     nb, NC = case["bulk"].nb, case["NC"]
     nc, ncol = NC + 1, NC + 2
@@ -46,6 +50,7 @@ def test_shim_newton_iteration_matches_c_abi_and_oracle():
     # the update is applied to the initial flash's moles
     ref = b.copy()
     ctx = lib.Context(case["eos"], sat, rock)
+    ctx.set_method(mcode)
     ctx.set_cells(nb)
     ctx.upload(ref)
     ctx.set_conn(grid)
@@ -83,7 +88,7 @@ def test_shim_newton_iteration_matches_c_abi_and_oracle():
     rc = drv.shim_newton_iteration(C.byref(deck), C.byref(sat), C.byref(rock), nb, C.byref(conn), C.byref(s),
                                    zc.ctypes.data, C.byref(ws), newP.ctypes.data, newNi.ctypes.data, DT,
                                    IA.ctypes.data, JA.ctypes.data, val.ctypes.data, rhs.ctypes.data,
-                                   C.byref(norms))
+                                   C.byref(norms), mcode)
     assert rc == 0, rc
     np.testing.assert_array_equal(IA, IA_c)
     np.testing.assert_array_equal(JA, JA_c)
@@ -94,21 +99,25 @@ def test_shim_newton_iteration_matches_c_abi_and_oracle():
     orc = case["orc"]
     o = b.copy()
     orc.calrock(rock, o)
-    orc.init_flash_fim(case["eos"], o, z)
+    (orc.init_flash_fimn if fimn else orc.init_flash_fim)(case["eos"], o, z)
     orc.calkrpc(sat, o)
     o.lNi[:] = o.Ni
     o.P[:] = newP
     o.Ni[:] = newNi
-    orc.flash_fim(case["eos"], o)
+    (orc.flash_fimn if fimn else orc.flash_fim)(case["eos"], o)
     orc.calkrpc(sat, o)
     orc.calrock(rock, o)
     res_o, up_o, urho_o, _, _ = orc.calres_wells(case["eos"], wells, grid, o, DT)
     IA_o, JA_o = orc.bsr_pattern_wells(grid, wells)
-    val_o = orc.assemble_wells(case["eos"], wells, grid, o, DT, up_o, urho_o, IA_o, JA_o)
+    if fimn:
+        val_o, rhs_o = orc.assemble_fimn(case["eos"], grid, o, DT, up_o, urho_o, IA_o, JA_o, res_o, wells=wells)
+    else:
+        val_o = orc.assemble_wells(case["eos"], wells, grid, o, DT, up_o, urho_o, IA_o, JA_o)
+        rhs_o = res_o
     np.testing.assert_array_equal(IA, IA_o)
     np.testing.assert_array_equal(JA, JA_o)
-    scale = np.abs(res_o).reshape(-1, ncol).max(axis=1, keepdims=True)
-    err = (np.abs(rhs - res_o).reshape(-1, ncol) / np.where(scale > 0, scale, 1)).max()
+    scale = np.abs(rhs_o).reshape(-1, ncol).max(axis=1, keepdims=True)
+    err = (np.abs(rhs - rhs_o).reshape(-1, ncol) / np.where(scale > 0, scale, 1)).max()
     gpu_util.say("  rhs scaled err vs oracle", err)
     assert err < 1e-8
     from tests.test_gpu_fim import block_err

@@ -112,6 +112,30 @@ def test_two_well_config_and_state_update():
     ctx.close()
 
 
+def test_rate_producer_needs_phase_weights():
+    """A producer created at BHP control without prodPhaseWeight cannot be
+    switched to a rate control without them (its rate row would be zero:
+    WellOpt::prodPhaseWeight, src/Well.cpp:829-862); with weights supplied
+    at the switch it is accepted.  No kernel runs."""
+    case = fim_case.build_case(5, 4, 3, seed=7)
+    b = case["bulk"]
+    spec = [dict(type=abi.WELL_INJ, mode=abi.WELL_RATE, perf=[0, 20], ctrlRate=1000.0),
+            dict(type=abi.WELL_PROD, mode=abi.WELL_BHP, perf=[19, 39], prodPhaseWeight=[0.0, 0.0, 0.0])]
+    wells = fim_case.add_wells(case, spec=spec, seed=8)
+    ctx = lib.Context(case["eos"], case["sat"], case["rock"])
+    ctx.set_cells(b.nb)
+    ctx.upload(b)
+    ctx.set_conn(case["grid"])
+    ctx.set_wells(wells)
+    wells.mode[1] = abi.WELL_RATE
+    wells.ctrlRate[1] = 300.0
+    with pytest.raises(lib.OcpgError, match=f"code {abi.EINVAL}"):
+        ctx.set_well_state(wells)
+    wells.prodPhaseWeight[3:6] = [1.0, 0.0, 0.0]
+    ctx.set_well_state(wells)
+    ctx.close()
+
+
 def test_shut_well_and_removal():
     """A shut well contributes nothing and owns no row; nw = 0 restores the
     bulk-only system."""
