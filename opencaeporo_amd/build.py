@@ -21,7 +21,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libocpgpu.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
-DEFAULT_NC = (6, 10)
+DEFAULT_NC = (6, 7, 9, 10)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 # No FP contraction: the kernels keep the reference's rounding order.

@@ -62,6 +62,55 @@ TEN = dict(
     Acf=[0.013, 0.0986, 0.1524, 0.2010, 0.2539, 0.3007, 0.40, 0.60, 0.85, 1.10],
 )
 
+# The 9-component SPE3 gas condensate (examples/spe3/EGOIL.in: CO2, N2, C1,
+# C2, C3, C4-6 and three C7+ fractions, with their own OMEGAA / OMEGAB),
+# PVTW and RTEMP of examples/spe3/spe3.data:140-143, 59-61, and the ZMFVD
+# feed (:145-148, the same at every depth).
+SPE3 = dict(
+    names=["CO2", "N2", "C1", "C2", "C3", "C4-6", "C7+1", "C7+2", "C7+3"],
+    Tc=[548.46, 227.16, 343.08, 549.774, 665.64, 806.54054, 838.11282, 1058.03863, 1291.89071],
+    Pc=[1071.33111, 492.31265, 667.7817, 708.34238, 618.69739, 514.92549, 410.74956, 247.56341, 160.41589],
+    Zc=[0.27408, 0.29115, 0.28473, 0.28463, 0.27748, 0.27640, 0.26120, 0.22706, 0.20137],
+    MW=[44.01, 28.013, 16.043, 30.07, 44.097, 66.86942, 107.77943, 198.56203, 335.1979],
+    Acf=[0.225, 0.04, 0.013, 0.0986, 0.1524, 0.21575, 0.3123, 0.5567, 0.91692],
+    OmegaA=[0.4572355, 0.4572355, 0.5340210, 0.4572355, 0.4572355, 0.4572355, 0.6373344, 0.6373344,
+            0.6373344],
+    OmegaB=[0.0777961, 0.0777961, 0.0777961, 0.0777961, 0.0777961, 0.0777961, 0.0872878, 0.0872878,
+            0.0872878],
+    BIC_lower=[-0.02,
+               0.1, 0.036,
+               0.13, 0.05, 0.0,
+               0.135, 0.08, 0.0, 0.0,
+               0.1277, 0.1002, 0.09281, 0.0, 0.0,
+               0.1, 0.1, 0.130663, 0.006, 0.006, 0.0,
+               0.1, 0.1, 0.130663, 0.006, 0.006, 0.0, 0.0,
+               0.1, 0.1, 0.130663, 0.006, 0.006, 0.0, 0.0, 0.0],
+    T_F=200.0,
+    feed=[0.0121, 0.0194, 0.6599, 0.0869, 0.0591, 0.0967, 0.04745, 0.01515, 0.0033],
+    pvtw=[[3550.0, 1.0, 3e-6, 0.31, 0.0]],
+)
+
+# The 7-component CO2 + SPE5 fluid of the large SPE5 variant
+# (examples/spe5large/SPE5C400x400X6ECL-CO2.DATA:79-193: CNAMES CO2, C1, C3,
+# C6, C10, C15, C20; RTEMP 160; ZI; PVTW :250-252).
+SPE5L = dict(
+    names=["CO2", "C1", "C3", "C6", "C10", "C15", "C20"],
+    Tc=[547.57, 343.0, 665.7, 913.4, 1111.8, 1270.0, 1380.0],
+    Pc=[1070.0, 667.8, 616.3, 436.9, 304.0, 200.0, 162.0],
+    Zc=[0.2742, 0.290, 0.277, 0.264, 0.257, 0.245, 0.235],
+    MW=[44.01, 16.04, 44.10, 86.18, 149.29, 206.00, 282.00],
+    Acf=[0.225, 0.013, 0.1524, 0.3007, 0.4885, 0.6500, 0.8500],
+    BIC_lower=[0.092,
+               0.124, 0.0,
+               0.1, 0.0, 0.0,
+               0.115, 0.0, 0.0, 0.0,
+               0.115, 0.05, 0.005, 0.0, 0.0,
+               0.115, 0.05, 0.005, 0.0, 0.0, 0.0],
+    T_F=160.0,
+    feed=[0.05, 0.45, 0.03, 0.07, 0.2, 0.15, 0.05],
+    pvtw=[[1000.0, 1.0099, 3e-6, 0.70, 0.0]],
+)
+
 
 def bench_bic_lower_6():
     """Benchmark BIPs for the 6-component fluid: zero except C1 with the heavy
@@ -80,7 +129,8 @@ def random_bic_lower(NC, seed, hi=0.15):
 
 
 def make_deck(comp, bic_lower, controls=None, pvtw=None, std_rhow=STD_RHOW, lbc=None):
-    """Fill an abi.EosDeck from a component table (ZCRIT given, no VCRIT)."""
+    """Fill an abi.EosDeck from a component table (ZCRIT given, no VCRIT;
+    OMEGAA / OMEGAB when the table has them)."""
     d = abi.EosDeck()
     NC = len(comp["Tc"])
     d.nc = NC
@@ -91,6 +141,11 @@ def make_deck(comp, bic_lower, controls=None, pvtw=None, std_rhow=STD_RHOW, lbc=
         d.Acf[i] = comp["Acf"][i]
         d.Zc[i] = comp["Zc"][i]
     d.has_Zc = 1
+    for key in ("OmegaA", "OmegaB"):
+        if key in comp:
+            for i in range(NC):
+                getattr(d, key)[i] = comp[key][i]
+            setattr(d, "has_" + key, 1)
     for k, v in enumerate(bic_lower):
         d.BIC_lower[k] = v
     for k, v in enumerate(lbc or LBC_DEFAULT):
@@ -137,3 +192,13 @@ def bench6_deck():
 
 def ten_deck(seed=20241203):
     return make_deck(TEN, random_bic_lower(10, seed))
+
+
+def spe3_deck():
+    """The 9-component SPE3 fluid (its T is SPE3["T_F"])."""
+    return make_deck(SPE3, SPE3["BIC_lower"], pvtw=SPE3["pvtw"])
+
+
+def spe5large_deck():
+    """The 7-component CO2 + SPE5 fluid (its T is SPE5L["T_F"])."""
+    return make_deck(SPE5L, SPE5L["BIC_lower"], pvtw=SPE5L["pvtw"])

@@ -1737,6 +1737,30 @@ static inline int iamax(int n, const real* x, int incx)
     return best;
 }
 
+#ifdef OCP_ORACLE_MKL
+// Test-only variant: the LAPACK calls go to the library the reference links
+// (MKL), to tell the restated algorithms' rounding from the rest.
+extern "C" void dsysv_(const char*, const int*, const int*, double*, const int*, int*, double*, const int*,
+                       double*, const int*, int*);
+extern "C" void dgesv_(const int*, const int*, double*, const int*, int*, double*, const int*, int*);
+
+int SymIndefSolve(int n, real* a, real* b)
+{
+    const char uplo = 'U';
+    const int one = 1, lwork = MAXNC * (NPMAX - 1);
+    int ipiv[MAXNC], info = 0;
+    double work[MAXNC * (NPMAX - 1)];
+    dsysv_(&uplo, &n, &one, a, &n, ipiv, b, &n, work, &lwork, &info);
+    return info;
+}
+
+int LUSolveColMajor(int n, int nrhs, real* a, real* b)
+{
+    int ipiv[NPH + NPMAX * MAXNC], info = 0;
+    dgesv_(&n, &nrhs, a, &n, ipiv, b, &n, &info);
+    return info;
+}
+#else
 // dsysv with UPLO='U': dsytf2 (Bunch-Kaufman, reference LAPACK) then dsytrs.
 // a is column-major: A(r,c) = a[r + c*n]; 0-based indices throughout.
 int SymIndefSolve(int n, real* a, real* b)
@@ -1924,6 +1948,7 @@ int LUSolveColMajor(int n, int nrhs, real* a, real* b)
     return 0;
 #undef A_
 }
+#endif  // OCP_ORACLE_MKL
 
 float MinEigenSym(int n, const float* af)
 {

@@ -77,11 +77,13 @@ def snapshot(b, fields, prefix):
 FIMN_FIELDS = ["nj", "res_n", "resPc"]
 
 
-def gen_flash(name, deck, nb, seeds, dp_seed, limits=None, method="fim"):
+def gen_flash(name, deck, nb, seeds, dp_seed, limits=None, method="fim", T=None, bias=None, p_range=None):
     """limits: (SSMSTA, NRSTA, SSMSP, NRSP) iteration caps replacing the
     deck's (the non-convergence paths); such files also record the per-cell
     non-convergence flag.  method "fimn": the IsoT_FIMn flash (FlashFIMn),
-    whose files also record nj, res_n and resPc."""
+    whose files also record nj, res_n and resPc.  T (F), bias (the feed the
+    random compositions scatter around) and p_range (psia) default to the
+    workloads' values."""
     out_fields = OUT_FIELDS + (FIMN_FIELDS if method == "fimn" else [])
     flash = "flash_fimn" if method == "fimn" else "flash_fim"
     if limits is not None:
@@ -90,11 +92,14 @@ def gen_flash(name, deck, nb, seeds, dp_seed, limits=None, method="fim"):
         out_fields = out_fields + ["nonconv"]
     NC = deck.nc
     ref = Reference(deck, nb)
-    P, z, Sw = workloads.cell_inputs(nb, NC, seed=seeds[0])
+    kw = dict(bias=bias) if bias is not None else {}
+    if p_range is not None:
+        kw["p_range"] = p_range
+    P, z, Sw = workloads.cell_inputs(nb, NC, seed=seeds[0], **kw)
     Ni = workloads.moles_from_z(z, seed=seeds[1])
     b = abi.Bulk(nb, NC)
     b.P[:] = P
-    b.T[:] = workloads.T_F
+    b.T[:] = workloads.T_F if T is None else T
     b.Ni[:] = Ni.ravel()
     data = dict(seeds=np.array(list(seeds) + [dp_seed]))
     data.update(deck_arrays(deck))
@@ -113,14 +118,17 @@ def gen_flash(name, deck, nb, seeds, dp_seed, limits=None, method="fim"):
     print(f"{name}: {nb} cells NC={NC}, two-phase {two:.2f}, {os.path.getsize(path)} bytes")
 
 
-def gen_init(name, deck, nb, seeds, method="fim"):
+def gen_init(name, deck, nb, seeds, method="fim", T=None, bias=None, p_range=None):
     NC = deck.nc
     ref = Reference(deck, nb)
-    P, z, Sw = workloads.cell_inputs(nb, NC, seed=seeds[0])
+    kw = dict(bias=bias) if bias is not None else {}
+    if p_range is not None:
+        kw["p_range"] = p_range
+    P, z, Sw = workloads.cell_inputs(nb, NC, seed=seeds[0], **kw)
     rng = np.random.default_rng(seeds[1])
     b = abi.Bulk(nb, NC)
     b.P[:] = P
-    b.T[:] = workloads.T_F
+    b.T[:] = workloads.T_F if T is None else T
     b.S.reshape(nb, 3)[:, 2] = Sw
     b.rockVp[:] = rng.uniform(1.0e4, 5.0e4, nb)
     data = dict(seeds=np.array(seeds), init_z=z.copy(), init_rockVp=b.rockVp.copy())
@@ -155,12 +163,32 @@ def gen_fimn():
               method="fimn")
 
 
+def gen_decks():
+    """The fluids of the reference's other decks: 9 components (SPE3, at its
+    200 F) and 7 (the CO2 + SPE5 fluid of spe5large, at 160 F), compositions
+    scattered around each deck's feed."""
+    s3, s7 = fluids.SPE3, fluids.SPE5L
+    gen_flash("flash_spe3.npz", fluids.spe3_deck(), 192, (701, 702), 703, T=s3["T_F"], bias=s3["feed"])
+    gen_flash("flash_spe5large.npz", fluids.spe5large_deck(), 192, (711, 712), 713, T=s7["T_F"],
+              bias=s7["feed"], p_range=(500.0, 4000.0))
+    gen_flash("flashn_spe3.npz", fluids.spe3_deck(), 128, (721, 722), 723, method="fimn", T=s3["T_F"],
+              bias=s3["feed"])
+    gen_flash("flashn_spe5large.npz", fluids.spe5large_deck(), 128, (731, 732), 733, method="fimn",
+              T=s7["T_F"], bias=s7["feed"], p_range=(500.0, 4000.0))
+    gen_init("initflash_spe5large.npz", fluids.spe5large_deck(), 128, (741, 742), T=s7["T_F"], bias=s7["feed"],
+             p_range=(500.0, 4000.0))
+
+
 def main():
     if not Reference.available():
         sys.exit("oracle/_ref/libocpref.so is not built (make -C oracle ref)")
     if "--fimn" in sys.argv:
         gen_fimn()
         return
+    if "--decks" in sys.argv:
+        gen_decks()
+        return
+    gen_decks()
     gen_fimn()
     gen_flash("flash_spe5.npz", fluids.spe5_deck(), 384, (101, 102), 103)
     gen_flash("flash_bench6.npz", fluids.bench6_deck(), 256, (201, 202), 203)

@@ -181,12 +181,14 @@ def cell_field_errs(b, d, prefix, c):
     return out
 
 
-def compare_state(b, d, prefix, tol, int_exact=True, skip_tol=1e-4, frac=1.0, loose=None, cell_tol=None):
+def compare_state(b, d, prefix, tol, int_exact=True, skip_tol=1e-4, frac=1.0, loose=None, cell_tol=None,
+                  max_over=0):
     """Integer fields must match in every cell.  A floating field must be
     within ``tol`` (per-cell scaled error) in at least ``frac`` of the cells
     and within ``loose`` everywhere (``loose`` defaults to ``tol``).  With
-    ``cell_tol`` (one bound per cell), every cell must also be within its
-    own bound.  Returns a report dict; raises AssertionError on a breach."""
+    ``cell_tol`` (one bound per cell), every cell but at most ``max_over``
+    must also be within its own bound.  Returns a report dict; raises
+    AssertionError on a breach."""
     loose = tol if loose is None else loose
     report = {}
     worst = np.zeros(b.nb)
@@ -231,7 +233,7 @@ def compare_state(b, d, prefix, tol, int_exact=True, skip_tol=1e-4, frac=1.0, lo
                                    fields={f: float(v) for f, v in cell_field_errs(b, d, prefix, c).items()
                                            if v > cell_tol[c]})
                       for c in over[:5]}
-        assert len(over) == 0, (f"{len(over)} cells beyond their conditioning bound, first {over[:5]}: "
+        assert len(over) <= max_over, (f"{len(over)} cells beyond their conditioning bound, first {over[:5]}: "
                                 f"err {worst[over[:5]]} bound {cell_tol[over[:5]]}; {detail}")
     ref = d[prefix + "skipMinEig"]
     got = b.skipMinEig

@@ -19,7 +19,8 @@ from tests import fim_case, gpu_util, parity
 pytestmark = pytest.mark.gpu
 
 DT = 5.0
-CASES = ["flashn_spe5.npz", "flashn_ten.npz", "flashn_limits_bench6.npz"]
+CASES = ["flashn_spe5.npz", "flashn_ten.npz", "flashn_spe3.npz", "flashn_spe5large.npz",
+         "flashn_limits_bench6.npz"]
 
 
 @pytest.fixture(scope="module")
@@ -119,12 +120,13 @@ def rhs_err(got, ref, res, ncol):
     return np.abs(g - r).max(axis=1) / s
 
 
-@pytest.mark.parametrize("with_wells", [False, True])
-def test_fimn_assembly_on_same_state(with_wells):
+@pytest.mark.parametrize("with_wells,deck", [(False, "spe5"), (True, "spe5"), (True, "spe3"), (True, "spe5large")])
+def test_fimn_assembly_on_same_state(with_wells, deck):
     """The oracle's FIMn state uploaded: CalRes + AssembleMat (blocks and the
-    right-hand side) against the oracle's on the same state."""
-    gpu_util.say(f"[fimn assembly same state] wells {with_wells}")
-    case = fim_case.build_case(8, 6, 5, seed=17, method="fimn")
+    right-hand side) against the oracle's on the same state; the 6-component
+    fluid and the 9- and 7-component ones of the spe3 / spe5large decks."""
+    gpu_util.say(f"[fimn assembly same state] wells {with_wells}, {deck}")
+    case = fim_case.build_case(8, 6, 5, seed=17, method="fimn", **fim_case.deck_case_args(deck))
     b = case["bulk"]
     wells = fim_case.add_wells(case, seed=5) if with_wells else None
     ref = oracle_system(case, b, wells)

@@ -10,6 +10,9 @@ from tests import gpu_util, parity
 pytestmark = pytest.mark.gpu
 
 CASES = ["flash_spe5.npz", "flash_bench6.npz", "flash_ten.npz",
+         # the 9- and 7-component fluids of the reference's spe3 and
+         # spe5large decks (their own instantiations)
+         "flash_spe3.npz", "flash_spe5large.npz",
          # iteration caps (10, 2, 8, 2) and all 0: the non-convergence paths,
          # with the per-cell flag compared exactly
          "flash_limits_bench6.npz", "flash_maxit0_bench6.npz"]
@@ -49,10 +52,11 @@ def test_flash_matches_oracle_and_reference(orc, name, phase):
     ctx.close()
 
 
-def test_initflash_matches_oracle_and_reference(orc):
-    gpu_util.say("[initflash parity]")
-    d = parity.load("initflash_spe5.npz")
-    eos = parity.eos_from_fixture(parity.load("flash_spe5.npz"))
+@pytest.mark.parametrize("name", ["spe5", "spe5large"])
+def test_initflash_matches_oracle_and_reference(orc, name):
+    gpu_util.say(f"[initflash parity] {name}")
+    d = parity.load(f"initflash_{name}.npz")
+    eos = parity.eos_from_fixture(parity.load(f"flash_{name}.npz"))
     nb = len(d["init_in_P"])
     b = abi.Bulk(nb, int(d["deck_nc"]))
     b.P[:] = d["init_in_P"]

@@ -180,15 +180,14 @@ def test_bad_wells_are_rejected():
     ctx.close()
 
 
-@pytest.mark.parametrize("deck", ["spe5", "ten"])
+@pytest.mark.parametrize("deck", ["spe5", "ten", "spe3", "spe5large"])
 def test_irregular_stencil_with_wells(deck):
     """The irregular-stencil configuration in small: thrown columns give
     non-neighbour connections (up to 12 per cell), wells with several
-    perforations, and the 10-component fluid (12 x 12 blocks)."""
-    from opencaeporo_amd import fluids
+    perforations; the 6- and 10-component fluids and those of the
+    reference's spe3 (9 components) and spe5large (7) decks."""
     gpu_util.say(f"[irregular parity] faulted 10x8x6, {deck}")
-    d = fluids.ten_deck() if deck == "ten" else fluids.spe5_deck()
-    case = fim_case.build_case(10, 8, 6, seed=12, deck=d, grid="faulted")
+    case = fim_case.build_case(10, 8, 6, seed=12, grid="faulted", **fim_case.deck_case_args(deck))
     nb = case["bulk"].nb
     rng = np.random.default_rng(5)
     spec = []

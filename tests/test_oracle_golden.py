@@ -25,6 +25,9 @@ ILL_COND_FRAC = 0.97
 ILL_COND_LOOSE = 1e-4
 
 CASES = ["flash_spe5.npz", "flash_bench6.npz", "flash_ten.npz",
+         # the fluids of the reference's spe3 (9 components, per-component
+         # OMEGAA / OMEGAB, 200 F) and spe5large (7 components, 160 F) decks
+         "flash_spe3.npz", "flash_spe5large.npz",
          # iteration caps (10, 2, 8, 2) and all 0: the non-convergence paths,
          # with the per-cell flag compared exactly
          "flash_limits_bench6.npz", "flash_maxit0_bench6.npz"]
@@ -48,9 +51,10 @@ def test_flash_matches_reference(orc, name, phase):
     assert rep["iters_mismatch_cells"] <= 0.05 * b.nb, rep
 
 
-def test_initflash_matches_reference(orc):
-    d = parity.load("initflash_spe5.npz")
-    d_eos = parity.load("flash_spe5.npz")  # same deck, reference-derived EoS
+@pytest.mark.parametrize("name", ["spe5", "spe5large"])
+def test_initflash_matches_reference(orc, name):
+    d = parity.load(f"initflash_{name}.npz")
+    d_eos = parity.load(f"flash_{name}.npz")  # same deck, reference-derived EoS
     eos = parity.eos_from_fixture(d_eos)
     nb = len(d["init_in_P"])
     from opencaeporo_amd import abi
@@ -88,7 +92,20 @@ def test_reference_within_conditioning_bound(orc, name, phase):
     """The per-cell bound the GPU parity tests use, max(1e-10, 10 x rounding
     sensitivity), holds between the oracle and the reference's own flash (a
     different LAPACK, the same libm) in every cell, and the sensitivity
-    itself is tiny in most cells (the bound is not a loose catch-all)."""
+    itself is tiny in most cells (the bound is not a loose catch-all).
+
+    The bound measures how the oracle's own arithmetic responds to last-bit
+    input changes; it does not cover a different linear-algebra library.
+    In the CO2 fluid's warm pass two of 192 cells (same iteration counts)
+    differ from the reference by 2.7e-10 and 2.2e-9, in vfi and dSec_dPri
+    only (the derivative solves).  With the oracle's LAPACK calls routed to
+    the MKL the reference links (oracle/Makefile `mkl`, a test-only build)
+    the first becomes exact; the second stays at 4e-9, so it comes from
+    MKL's BLAS (FMA, its own summation order) in a near-singular cell, which
+    the oracle restates with the reference-BLAS loops.  Up to 1% of cells
+    (at least two) may therefore exceed their bound, each within the
+    oracle-vs-reference 1e-8; the GPU runs the oracle's algorithms and is
+    held to the bound in every cell."""
     from tests import gpu_util
     d = parity.load(name)
     eos = parity.eos_from_fixture(d)
@@ -97,5 +114,9 @@ def test_reference_within_conditioning_bound(orc, name, phase):
     assert np.quantile(sens, 0.9) < 1e-10 and np.median(sens) < 1e-12
     o = b.copy()
     orc.flash_fim(eos, o)
-    parity.compare_state(o, d, f"{phase}_out_", gpu_util.GPU_TOL, frac=gpu_util.GPU_FRAC,
-                         loose=gpu_util.GPU_LOOSE, cell_tol=gpu_util.cell_bounds(sens))
+    rep = parity.compare_state(o, d, f"{phase}_out_", gpu_util.GPU_TOL, frac=gpu_util.GPU_FRAC,
+                               loose=gpu_util.GPU_LOOSE, cell_tol=gpu_util.cell_bounds(sens),
+                               max_over=max(2, b.nb // 100))
+    for c in rep["cells_over_own_bound"]:
+        errs = parity.cell_field_errs(o, d, f"{phase}_out_", c)
+        assert max(errs.values()) <= ORACLE_VS_REF_TOL, (c, errs)
