@@ -241,3 +241,69 @@ def config1(seed=20241204, inj_mscf=20000.0, prod_stb=5000.0, max_bhp=10000.0, m
     depth = [float(grid.depth[inj_cells[0]]), float(grid.depth[prod_cells[0]])]
     setup = (depth, [160.0, 160.0], [1, 1], [abi.WELL_RATE, abi.WELL_RATE])
     return dict(grid=grid, P=P, z=z, Sw=Sw, wells=wells, setup=setup, inj_mscf=inj_mscf, nxny=nxny)
+
+
+def corner_point_arrays(nx, ny, nz, seed, dx=100.0, dy=100.0, dz=10.0, tops=8325.0, nfault=(2, 1),
+                        throw=(0.3, 1.7), tilt=0.05, jitter=0.1, scissor=0.0):
+    """COORD and ZCORN (deck order) of a synthetic corner-point reservoir:
+    pillars on a jittered dx x dy lattice leaning by up to ``tilt`` (lateral
+    per vertical), dipping horizons with layers of varying thickness around
+    dz, and nfault[0] faults between columns along i and nfault[1] along j,
+    each dropping the columns beyond it by throw x dz (uniform in the
+    range): whole fault blocks move down, so lateral faces across a fault
+    meet cells of other layers (non-neighbour connections).  scissor > 0
+    makes a fault's throw grow along its length (by up to +-scissor/2 of
+    itself from one end to the other), so the edges of faces across it are
+    no longer parallel and cross each other."""
+    rng = np.random.default_rng(seed)
+    NX, NY = nx + 1, ny + 1
+    X = np.arange(NX, dtype=np.float64)[None, :] * dx + np.zeros((NY, 1))
+    Y = np.arange(NY, dtype=np.float64)[:, None] * dy + np.zeros((1, NX))
+    inner = np.zeros((NY, NX), dtype=bool)
+    inner[1:-1, 1:-1] = True
+    X = X + np.where(inner, rng.uniform(-jitter, jitter, (NY, NX)) * dx, 0.0)
+    Y = Y + np.where(inner, rng.uniform(-jitter, jitter, (NY, NX)) * dy, 0.0)
+    ztop, zbot = tops - 200.0, tops + 4 * nz * dz + 400.0
+    lean_x = tilt * np.sin(np.pi * X / (NX * dx))
+    lean_y = 0.5 * tilt * np.cos(np.pi * Y / (NY * dy))
+    coord = np.empty((NY, NX, 6))
+    coord[..., 0], coord[..., 1], coord[..., 2] = X, Y, ztop
+    coord[..., 3] = X + lean_x * (zbot - ztop)
+    coord[..., 4] = Y + lean_y * (zbot - ztop)
+    coord[..., 5] = zbot
+
+    def horizons(x, y):
+        """Depths of the nz + 1 horizons at (x, y) before faulting."""
+        h = np.empty((nz + 1,) + np.shape(x))
+        h[0] = tops + 0.02 * x + 0.01 * y
+        for k in range(nz):
+            h[k + 1] = h[k] + dz * (1.0 + 0.25 * np.sin(0.7 * k + x / (3 * dx)) * np.cos(y / (4 * dy)))
+        return h
+
+    # faults: (axis, first column beyond, throw); a column's corner drops by
+    # the throws of the faults it lies beyond, at that corner's position
+    faults = []
+    for axis, count in ((0, nfault[0]), (1, nfault[1])):
+        n = nx if axis == 0 else ny
+        for _ in range(count):
+            if n < 2:
+                break
+            faults.append((axis, int(rng.integers(1, n)), rng.uniform(throw[0], throw[1]) * dz))
+    jj, ii = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    # ZCORN: per layer a top and a bottom sheet; per sheet (2 ny) rows of (2 nx)
+    z = np.empty((nz, 2, ny, 2, nx, 2))
+    for ys in (0, 1):
+        for xs in (0, 1):
+            px = X[ys:ys + ny, xs:xs + nx]
+            py = Y[ys:ys + ny, xs:xs + nx]
+            drop = np.zeros((ny, nx))
+            for axis, f, t in faults:
+                beyond = (ii >= f) if axis == 0 else (jj >= f)
+                along = (py / (ny * dy)) if axis == 0 else (px / (nx * dx))
+                drop += np.where(beyond, t * (1 + scissor * (along - 0.5)), 0.0)
+            # the corner lies on its leaning pillar: depth from the pillar's top position
+            h = horizons(px, py) + drop[None]
+            for k in range(nz):
+                z[k, 0, :, ys, :, xs] = h[k]
+                z[k, 1, :, ys, :, xs] = h[k + 1]
+    return coord.ravel(), z.ravel()

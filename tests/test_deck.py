@@ -16,6 +16,7 @@ from opencaeporo_amd import abi, deck, fluids
 HERE = os.path.dirname(os.path.abspath(__file__))
 SPE5 = os.path.join(HERE, "golden", "decks", "spe5", "spe5.data")
 REFINE = os.path.join(HERE, "golden", "decks", "spe5refine", "spe5-70x70x30-2y.data")
+CORNER = os.path.join(HERE, "golden", "decks", "corner", "corner-faulted.data")
 
 
 def test_tokens_and_defaults():
@@ -138,3 +139,44 @@ def test_unsupported_decks_are_rejected(tmp_path):
         deck.load(_write(tmp_path, base, eos=eos[:eos.index("NRSP")]))
     # the unchanged deck from the same place reads
     assert deck.load(_write(tmp_path, base)).grid.nb == 147
+
+
+def test_corner_point_deck():
+    """COORD / ZCORN decks: the cells and connections are the reference's
+    corner-point geometry (the corner_deck case of
+    tests/golden/corner_geometry.npz, written by OCP_COORD itself) with
+    BulkConn::CalAkd's harmonic transmissibilities, in BulkConn order;
+    Peaceman's well index takes the corner-point cell sizes."""
+    from opencaeporo_amd import cornergrid
+    from tests import parity
+    m = deck.load(CORNER)
+    g = m.grid
+    G = parity.load("corner_geometry.npz")
+    np.testing.assert_allclose(g.v, G["corner_deck_v"], rtol=1e-12)
+    np.testing.assert_allclose(g.depth, G["corner_deck_depth"], rtol=1e-15)
+    np.testing.assert_array_equal(m.bulk.depth, g.depth)
+    b, e, d = G["corner_deck_begin"], G["corner_deck_end"], G["corner_deck_dir"]
+    aB, aE = G["corner_deck_adB"], G["corner_deck_adE"]
+    kx, ky, kz = m.perm
+    k = lambda n: np.choose(d - 1, (kx[n], ky[n], kz[n]))
+    area = 1 / (1 / (k(b) * aB) + 1 / (k(e) * aE))  # NTG is 1 in this deck
+    order = np.lexsort((e, b))
+    assert np.array_equal(g.bId, b[order]) and np.array_equal(g.eId, e[order])
+    np.testing.assert_allclose(g.area, area[order], rtol=1e-14)
+    # faults: connections between cells that are not grid neighbours
+    nx, ny, _ = m.deck.dims
+    step = np.abs(g.eId - g.bId)
+    assert np.count_nonzero((step != 1) & (step != nx) & (step != nx * ny)) > 100
+    # the producer's first perforation (I = 11, J = 9, K = 3), vertical:
+    # Peaceman with the cell's corner-point sizes
+    w = m.deck.wells[0]
+    n = m.cell(11, 9, 3)
+    dx, dy, dz = (a[n] for a in m.dxyz)
+    cg = cornergrid.CornerGrid(*m.deck.dims, m.deck.coord, m.deck.zcorn)
+    assert (dx, dy, dz) == (cg.dx[n], cg.dy[n], cg.dz[n])
+    kk = kx[n] * ky[n]
+    ro = 0.28 * math.sqrt(dx * dx * math.sqrt(ky[n] / kx[n]) + dy * dy * math.sqrt(kx[n] / ky[n]))
+    ro /= (kx[n] / ky[n]) ** 0.25 + (ky[n] / kx[n]) ** 0.25
+    wi = deck.CONV2 * 2 * deck.PI * dz * math.sqrt(kk) / math.log(ro / (0.5 / 2))
+    assert w["name"] == "PROD1"
+    assert m.wells.perfWI[0] == pytest.approx(wi, rel=1e-12)

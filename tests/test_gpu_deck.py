@@ -2,10 +2,12 @@
 opencaeporo_amd/deck.py drive the device through the C-ABI, against the
 oracle on the same inputs.
 
-* SPE5 (147 cells, the first step's producer): the device's CalRock +
-  InitFlashFIM + CalKrPc from the deck's feed equal the oracle's; then, from
-  a Newton-like update of that state, CalRes, the assembly and the well
-  terms equal the oracle's on the same cell state (the test_gpu_wells bar).
+* SPE5 (147 cells, the first step's producer) and a faulted corner-point
+  deck (600 cells, non-neighbour connections across its faults, an injector
+  and a producer): the device's CalRock + InitFlashFIM + CalKrPc from the
+  deck's feed equal the oracle's; then, from a Newton-like update of that
+  state, CalRes, the assembly and the well terms equal the oracle's on the
+  same cell state (the test_gpu_wells bar).
 * The refined deck (70 x 70 x 30 = 147,000 cells, a producer perforated in
   ten layers): one Newton iteration through ocpg_update_assemble equals the
   serial calls bit for bit and, on the state it produced, the oracle's
@@ -24,6 +26,7 @@ pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 SPE5 = os.path.join(HERE, "golden", "decks", "spe5", "spe5.data")
 REFINE = os.path.join(HERE, "golden", "decks", "spe5refine", "spe5-70x70x30-2y.data")
+CORNER = os.path.join(HERE, "golden", "decks", "corner", "corner-faulted.data")
 DT = 5.0
 
 
@@ -40,9 +43,12 @@ def _context(m, eos):
     return ctx
 
 
-def test_spe5_deck_initial_state_and_system():
-    gpu_util.say("[deck] SPE5 7x7x3 from the deck: init, then CalRes + assembly with its well")
-    m = deck.load(SPE5)
+@pytest.mark.parametrize("path", [SPE5, CORNER], ids=["spe5", "corner"])
+def test_deck_initial_state_and_system(path):
+    """The SPE5 deck (7x7x3, a producer) and the faulted corner-point deck
+    (12x10x5, 179 non-neighbour connections, an injector and a producer)."""
+    gpu_util.say(f"[deck] {os.path.basename(path)}: init, then CalRes + assembly with its wells")
+    m = deck.load(path)
     eos = lib.eos_from_deck(m.eos_deck)
     orc = _oracle()
     o = m.bulk.copy()
