@@ -305,7 +305,8 @@ class CornerGrid:
 
     def _pair(self, half):
         """OCP_COORD's pairing: each half connection n -> m (m > n) with the
-        first half connection m -> n, both Ad_dd > 0."""
+        first half connection m -> n, both Ad_dd > 0; in the order of n,
+        then of n's half connections."""
         cell = np.concatenate([h[0] for h in half]).astype(np.int64)
         nbr = np.concatenate([h[1] for h in half]).astype(np.int64)
         dirn = np.concatenate([h[2] for h in half]).astype(np.int32)
@@ -313,29 +314,27 @@ class CornerGrid:
         seq = np.concatenate([h[4] for h in half]).astype(np.int64)
         # each cell's half connections in the order the reference adds them
         o = np.lexsort((seq, cell))
-        cell, nbr, dirn, ad = cell[o], nbr[o], dirn[o], ad[o]
-        first = {}
-        for q in range(len(cell)):
-            first.setdefault((int(cell[q]), int(nbr[q])), q)
-        b, e, dr, aB, aE = [], [], [], [], []
-        for q in range(len(cell)):
-            n, m = int(cell[q]), int(nbr[q])
-            if m < n:
-                continue
-            r = first.get((m, n))
-            if r is None or ad[q] <= 0 or ad[r] <= 0:
-                continue
-            b.append(n)
-            e.append(m)
-            dr.append(int(dirn[q]))
-            aB.append(ad[q])
-            aE.append(ad[r])
-        self.conn_begin = np.array(b, dtype=np.int32)
-        self.conn_end = np.array(e, dtype=np.int32)
-        self.conn_dir = np.array(dr, dtype=np.int32)
-        self.conn_adB = np.array(aB, dtype=np.float64)
-        self.conn_adE = np.array(aE, dtype=np.float64)
-        self.nconn = len(b)
+        cell, nbr, dirn, ad, seq = cell[o], nbr[o], dirn[o], ad[o], seq[o]
+        nb = np.int64(self.nb)
+        key = cell * nb + nbr
+        # the first half connection of each (cell, neighbour): lowest seq
+        by_key = np.lexsort((seq, key))
+        ukeys, first_pos = np.unique(key[by_key], return_index=True)
+        first = by_key[first_pos]
+        fwd = np.flatnonzero(nbr > cell)
+        want = nbr[fwd] * nb + cell[fwd]
+        pos = np.searchsorted(ukeys, want)
+        found = (pos < len(ukeys)) & (ukeys[np.minimum(pos, len(ukeys) - 1)] == want)
+        q = fwd[found]
+        r = first[pos[found]]
+        ok = (ad[q] > 0) & (ad[r] > 0)
+        q, r = q[ok], r[ok]
+        self.conn_begin = cell[q].astype(np.int32)
+        self.conn_end = nbr[q].astype(np.int32)
+        self.conn_dir = dirn[q].astype(np.int32)
+        self.conn_adB = ad[q].astype(np.float64)
+        self.conn_adE = ad[r].astype(np.float64)
+        self.nconn = len(q)
 
 
 class Connections:
