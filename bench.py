@@ -52,14 +52,21 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def setup(nx, ny, nz, seed=2024, irregular=False, model=None, method=abi.METHOD_FIM):
+def setup(nx, ny, nz, seed=2024, irregular=False, model=None, method=abi.METHOD_FIM, equilibrate=True):
     """model: a deck.Model (the workload read from an input deck) instead of
-    the synthetic generators.  method: abi.METHOD_FIMN for IsoT_FIMn's flash
+    the synthetic generators, started from its hydrostatic equilibration
+    (Bulk::InitPTSw, with the library's phase densities) when it has ZMFVD
+    and equilibrate is set.  method: abi.METHOD_FIMN for IsoT_FIMn's flash
     and assembly (InitFlashFIMn for the initial state)."""
     t0 = time.time()
     if model is not None:
         eos = lib.eos_from_deck(model.eos_deck)
         sat, rock, grid = model.sat, model.rock, model.grid
+        if equilibrate and "ZMFVD" in model.deck.tables:
+            c0 = lib.Context(eos)
+            model.equilibrate(lambda P, T, z, ph: c0.phase_density(P, T, z, ph)[1])
+            c0.close()
+            log(f"[bench] equilibrated: P in [{model.bulk.P.min():.1f}, {model.bulk.P.max():.1f}] psia")
     else:
         deck = fluids.ten_deck() if irregular else fluids.bench6_deck()
         eos = lib.eos_from_deck(deck)
@@ -343,6 +350,9 @@ def main():
                          "vs one at a time, 256 realisations of 1e3..1e6 cells); 0 skips it")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-form leg")
     ap.add_argument("--no-fimn", action="store_true", help="skip the IsoT_FIMn leg")
+    ap.add_argument("--no-equil", action="store_true",
+                    help="with --deck: start every cell at the datum pressure instead of the hydrostatic "
+                         "equilibration")
     ap.add_argument("--fimn-only", action="store_true",
                     help="time only the IsoT_FIMn leg (--steps, --warmup) and print its record")
     ap.add_argument("--deck", default=None,
@@ -365,7 +375,7 @@ def main():
         model = deck.load(a.deck)
         nx, ny, nz = model.deck.dims
     side_case = a.irregular or model is not None
-    ctx, grid, nb, NC, wells = setup(nx, ny, nz, irregular=a.irregular, model=model)
+    ctx, grid, nb, NC, wells = setup(nx, ny, nz, irregular=a.irregular, model=model, equilibrate=not a.no_equil)
     for _ in range(a.warmup):
         step(ctx, a.serial)
     ctx.synchronize()

@@ -28,21 +28,21 @@ The keyword semantics follow the reference's readers:
   pressure of the first perforated cell (include/Well.hpp:94).
 
 Not read: inactive cells (ACTNUM, or PORO x NTG below the reference's
-activity threshold): such decks raise DeckError.  The initial state is not the
-reference's hydrostatic equilibration (Bulk::InitPTSw, src/Bulk.cpp:443):
-every cell starts at EQUIL's datum pressure, with the composition of ZI or
-of ZMFVD at its depth (OCPTable::Eval_All0: linear, flat outside the
-table), the temperature of RTEMP and the water saturation of SWOF's first
-row (the connate saturation, which the equilibration gives above the water
-contact when the capillary pressure is zero); injection of a solvent
-stream (WELLSTRE) in the first step raises DeckError.
+activity threshold): such decks raise DeckError.  The loaded state starts
+every cell at EQUIL's datum pressure, with the composition of ZI or of
+ZMFVD at its depth (OCPTable::Eval_All0: linear, flat outside the table),
+the temperature of RTEMP and the water saturation of SWOF's first row;
+Model.equilibrate replaces it with the reference's hydrostatic
+equilibration (Bulk::InitPTSw, src/Bulk.cpp:443-1064; opencaeporo_amd/
+equil.py), with the phase densities from the library.  Injection of a
+solvent stream (WELLSTRE) in the first step raises DeckError.
 """
 import math
 import os
 
 import numpy as np
 
-from . import abi, cornergrid, fluids, workloads
+from . import abi, cornergrid, equil, fluids, workloads
 
 CONV2 = 1.12712E-3          # include/OCPConst.hpp:60
 CONV1 = 5.61458             # include/OCPConst.hpp:59
@@ -438,6 +438,30 @@ class Model:
         b.poroInit[:] = arr["PORO"]
         self.bulk = b
         self.wells = self._wells(d, b)
+
+    def equilibrate(self, density):
+        """Bulk::InitPTSw (opencaeporo_amd/equil.py): replaces the datum-
+        pressure start with the hydrostatic equilibrium of EQUIL: P, the
+        water saturation and the ZMFVD feed of every cell.  density(P, T,
+        z, phase) -> lb/ft3, e.g. from a library context:
+        ``lambda P, T, z, ph: ctx.phase_density(P, T, z, ph)[1]``.  The
+        wells' initial BHP follows their first perforation's new pressure."""
+        d = self.deck
+        if "ZMFVD" not in d.tables:
+            raise DeckError("the equilibration reads the composition from ZMFVD")
+        P, Sw, z = equil.init_ptsw(self.grid.depth, self.dxyz[2], d.equil, d.tables["ZMFVD"], d.rtemp,
+                                   d.tables["SWOF"], d.tables["SGOF"], density)
+        b = self.bulk
+        b.P[:] = P
+        S = b.S.reshape(b.nb, 3)
+        S[:, 0] = 0.0
+        S[:, 1] = 0.0
+        S[:, 2] = Sw
+        self.z = np.ascontiguousarray(z[:, :self.NC])
+        if self.wells is not None:
+            cells = self.wells.perfCell[self.wells.perfStart[:-1]]
+            self.wells.bhp[:] = P[cells]
+        return self
 
     def cell(self, I, J, K):
         nx, ny, _ = self.deck.dims

@@ -5,6 +5,12 @@
 
 #include "oracle_fim.hpp"
 
+namespace oracle {
+int InitPTSw(const ocpg_eos& eos, const ocpg_satfunc& sat, int64_t nb, const double* depth, const double* dz,
+             const double* equil, const double* zmfvd, int nzrows, double rtemp, int tabrow, double* P, double* Sw,
+             double* zout);
+}
+
 extern "C" {
 
 int orc_flash_fim(const ocpg_eos* eos, ocpg_bulk* bk, int nthreads)
@@ -79,6 +85,16 @@ int orc_phase_density(const ocpg_eos* eos, double P, double T, const double* zi,
     oracle::Mixture mix(*eos);
     mix.PhaseDensity(P, T, zi, phase == 2 ? 2 : 1, &out[0], &out[1]);
     return OCPG_OK;
+}
+
+// Bulk::InitPTSw of a compositional model (oracle_equil.cpp).
+int orc_init_ptsw(const ocpg_eos* eos, const ocpg_satfunc* sat, int64_t nb, const double* depth, const double* dz,
+                  const double* equil, const double* zmfvd, int nzrows, double rtemp, int tabrow, double* P,
+                  double* Sw, double* z)
+{
+    if (!eos || !sat || !depth || !dz || !equil || !zmfvd || nzrows < 1 || tabrow < 2 || !P || !Sw || !z)
+        return OCPG_EINVAL;
+    return oracle::InitPTSw(*eos, *sat, nb, depth, dz, equil, zmfvd, nzrows, rtemp, tabrow, P, Sw, z);
 }
 
 // xi, rho, mu of one hydrocarbon phase of composition x at P, T.

@@ -48,6 +48,7 @@ class Oracle:
         L.orc_prepare_wells.argtypes = [P, P, P, P, P, P, P, P, C.c_int]
         L.orc_check_wells.argtypes = [P, P, P, P, P, P]
         L.orc_phase_density.argtypes = [P, C.c_double, C.c_double, P, C.c_int, P]
+        L.orc_init_ptsw.argtypes = [P, P, C.c_int64, P, P, P, P, C.c_int, C.c_double, C.c_int, P, P, P]
         L.orc_flash_phases.argtypes = [P, C.c_double, C.c_double, P, P]
         L.orc_nr_change.argtypes = [C.c_int64, P, P, P, P, P]
         L.orc_time_flash_fim.argtypes = [P, P, C.c_int]
@@ -220,6 +221,21 @@ class Oracle:
 
 
     # ---- wells ----
+    def init_ptsw(self, eos, sat, depth, dz, equil, zmfvd, rtemp, tabrow=50):
+        """Bulk::InitPTSw (oracle_equil.cpp): (P, Sw, z) per cell.  equil:
+        (Dref, Pref, DOWC, PcOW, DGOC, PcGO); zmfvd: rows (depth, z...)."""
+        depth = np.ascontiguousarray(depth, dtype=np.float64)
+        dz = np.ascontiguousarray(dz, dtype=np.float64)
+        eq = np.ascontiguousarray(equil, dtype=np.float64)
+        zt = np.ascontiguousarray(zmfvd, dtype=np.float64)
+        nb = len(depth)
+        P, Sw, z = np.zeros(nb), np.zeros(nb), np.zeros((nb, eos.nc))
+        rc = self.lib.orc_init_ptsw(C.byref(eos), C.byref(sat), nb, depth.ctypes.data, dz.ctypes.data,
+                                    eq.ctypes.data, zt.ctypes.data, zt.shape[0], rtemp, tabrow, P.ctypes.data,
+                                    Sw.ctypes.data, z.ctypes.data)
+        assert rc == 0, rc
+        return P, Sw, z
+
     def phase_density(self, eos, P, T, zi, phase):
         """MixtureComp::XiPhase / RhoPhase: (xi, rho)."""
         zi = np.ascontiguousarray(zi, dtype=np.float64)

@@ -2,6 +2,8 @@
 opencaeporo_amd/deck.py drive the device through the C-ABI, against the
 oracle on the same inputs.
 
+* Model.equilibrate (Bulk::InitPTSw) with the library's phase densities
+  against the oracle's equilibration, and the flash from that state.
 * SPE5 (147 cells, the first step's producer) and a faulted corner-point
   deck (600 cells, non-neighbour connections across its faults, an injector
   and a producer): the device's CalRock + InitFlashFIM + CalKrPc from the
@@ -130,3 +132,42 @@ def test_refined_deck_newton_iteration():
     case = dict(eos=eos, sat=m.sat, rock=m.rock, grid=m.grid, bulk=got, NC=NC, orc=orc)
     ref = oracle_system(case, m.wells, got)
     compare(case, m.wells, dict(res=rhs, norms=norms, IA=IA, JA=JA, val=val, qp=qp, qw=qw, pw=pw), ref)
+
+
+@pytest.mark.parametrize("path", [SPE5, CORNER], ids=["spe5", "corner"])
+def test_equilibrated_deck(path):
+    """Model.equilibrate (Bulk::InitPTSw) with the library's phase
+    densities equals the oracle's equilibration on its own densities; the
+    device's InitFlashFIM + CalKrPc from that state equal the oracle's."""
+    gpu_util.say(f"[deck] {os.path.basename(path)}: hydrostatic equilibration with the device's densities")
+    m = deck.load(path)
+    eos = lib.eos_from_deck(m.eos_deck)
+    orc = _oracle()
+    ctx = lib.Context(eos, m.sat, m.rock)
+    m.equilibrate(lambda P, T, z, ph: ctx.phase_density(P, T, z, ph)[1])
+    d = m.deck
+    P, Sw, z = orc.init_ptsw(eos, m.sat, m.grid.depth, m.dxyz[2],
+                             [d.equil[k] for k in ("Dref", "Pref", "DOWC", "PcOW", "DGOC", "PcGO")],
+                             d.tables["ZMFVD"], d.rtemp)
+    np.testing.assert_allclose(m.bulk.P, P, rtol=1e-12, atol=0)
+    np.testing.assert_array_equal(m.bulk.S.reshape(-1, 3)[:, 2], Sw)
+    np.testing.assert_array_equal(m.z, z)
+    gpu_util.say("   P range", float(m.bulk.P.min()), float(m.bulk.P.max()))
+    o = m.bulk.copy()
+    orc.calrock(m.rock, o)
+    orc.init_flash_fim(eos, o, m.z)
+    orc.calkrpc(m.sat, o)
+    ctx.set_cells(m.grid.nb)
+    ctx.upload(m.bulk)
+    ctx.set_conn(m.grid)
+    ctx.calrock()
+    ctx.init_flash_fim(m.z)
+    ctx.calkrpc()
+    g = m.bulk.copy()
+    ctx.download(g)
+    # (the pressures differ in the last bits, through the two libms'
+    # densities: a trace-phase kr of 1e-18 may differ by its own size)
+    for f in ("Ni", "S", "xij", "rockVp", "kr", "Pc", "Pj"):
+        np.testing.assert_allclose(getattr(g, f), getattr(o, f), rtol=1e-12, atol=1e-14, err_msg=f)
+    assert np.array_equal(g.phaseNum, o.phaseNum)
+    ctx.close()
