@@ -156,7 +156,7 @@ class _Lines:
 
 
 _ARRAYS = ("DX", "DY", "DZ", "TOPS", "PORO", "NTG", "PERMX", "PERMY", "PERMZ")
-_VECTORS = ("TCRIT", "PCRIT", "ZCRIT", "VCRIT", "MW", "ACF", "BIC", "LBCCOEF", "ZI", "SSHIFT")
+_VECTORS = ("TCRIT", "PCRIT", "ZCRIT", "VCRIT", "MW", "ACF", "BIC", "LBCCOEF", "ZI", "SSHIFT", "OMEGAA", "OMEGAB")
 
 
 class Deck:
@@ -358,6 +358,12 @@ class Model:
                 raise DeckError(f"{kw} needs {NC} values")
         comp = dict(Tc=d.vec["TCRIT"], Pc=d.vec["PCRIT"], MW=d.vec["MW"], Acf=d.vec["ACF"],
                     Zc=d.vec.get("ZCRIT", [0.0] * NC))
+        # per-component EoS constants (MixtureComp's OmegaA / OmegaB)
+        for kw, key in (("OMEGAA", "OmegaA"), ("OMEGAB", "OmegaB")):
+            if kw in d.vec:
+                if len(d.vec[kw]) != NC:
+                    raise DeckError(f"{kw} needs {NC} values")
+                comp[key] = d.vec[kw]
         if "ZCRIT" not in d.vec:
             raise DeckError("ZCRIT is required (VCRIT-only fluids are not read)")
         bic = d.vec.get("BIC", [0.0] * (NC * (NC - 1) // 2))
@@ -463,6 +469,31 @@ class Model:
             self.wells.bhp[:] = P[cells]
         return self
 
+    def prepare_injectors(self, xi):
+        """Molar densities of the solvent injectors' stream (gas phase):
+        the surface-rate factor 1000 xi(14.7 psia, 60 F) that turns the
+        control rate (Mscf/day) into lbmol/day (MixtureComp::SetupWellOpt)
+        and the perforations' density at the well's BHP and the reservoir
+        temperature (Well::CalFlux's injector xi; dG is zero at the first
+        step).  xi(P, T, z, phase) -> lbmol/ft3."""
+        for s in self.solvent:
+            w = s["well"]
+            factor = 1000.0 * xi(14.7, 60.0, s["zi"], 1)
+            self.wells.ctrlRate[w] = s["rate"] * factor
+            p0, p1 = self.wells.perfStart[w], self.wells.perfStart[w + 1]
+            self.wells.perfXi[p0:p1] = xi(float(self.wells.bhp[w]), self.deck.rtemp, s["zi"], 1)
+        self.solvent_prepared = True
+        return self
+
+    def initialize(self, ctx):
+        """The deck's initial state with the library's densities (context
+        ctx): the hydrostatic equilibration when the deck has ZMFVD, then
+        the solvent injectors' densities at the new BHPs."""
+        if "ZMFVD" in self.deck.tables:
+            self.equilibrate(lambda P, T, z, ph: ctx.phase_density(P, T, z, ph)[1])
+        self.prepare_injectors(lambda P, T, z, ph: ctx.phase_density(P, T, z, ph)[0])
+        return self
+
     def cell(self, I, J, K):
         nx, ny, _ = self.deck.dims
         return (K - 1) * nx * ny + (J - 1) * nx + (I - 1)
@@ -489,6 +520,7 @@ class Model:
         return CONV2 * (2 * PI) * kh / (math.log(ro / (p["diameter"] / 2.0)) + p["skin"])
 
     def _wells(self, d, b):
+        self.solvent = []
         if not d.wells:
             return None
         nc = self.NC + 1
@@ -511,16 +543,28 @@ class Model:
                         WI=np.asarray(wi), mult=np.ones(len(cells)), dG=np.zeros(len(cells)))
             if inj:
                 fluid = opt["fluid"].upper()
-                if fluid not in ("WAT", "WATER"):
-                    raise DeckError(f"well {w['name']}: only water injection is read for the first step")
-                zi = np.zeros(nc)
-                zi[-1] = 1.0
-                spec["injZi"] = zi
-                # MixtureComp::XiPhase for water at the perforation pressure
-                # (the well's BHP: dG is zero at the first step)
-                Pw0, bw0, cbw = self.pvtw[0][0], self.pvtw[0][1], self.pvtw[0][2]
-                bw = bw0 * (1 - cbw * (bhp - Pw0))
-                spec["perfXi"] = np.full(len(cells), 1 / (CONV1 * bw))
+                if fluid in ("WAT", "WATER"):
+                    zi = np.zeros(nc)
+                    zi[-1] = 1.0
+                    spec["injZi"] = zi
+                    # MixtureComp::XiPhase for water at the perforation pressure
+                    # (the well's BHP: dG is zero at the first step)
+                    Pw0, bw0, cbw = self.pvtw[0][0], self.pvtw[0][1], self.pvtw[0][2]
+                    bw = bw0 * (1 - cbw * (bhp - Pw0))
+                    spec["perfXi"] = np.full(len(cells), 1 / (CONV1 * bw))
+                else:
+                    # a solvent stream (WELLSTRE) injected as gas
+                    # (MixtureComp::SetupWellOpt, src/MixtureComp.cpp:770-800):
+                    # its molar densities come from prepare_injectors
+                    stream = next((v for k, v in d.streams.items() if k.upper() == fluid), None)
+                    if stream is None:
+                        raise DeckError(f"well {w['name']}: injected fluid {opt['fluid']} is not a WELLSTRE stream")
+                    zi = np.zeros(nc)
+                    k = min(len(stream), nc)
+                    zi[:k] = stream[:k]
+                    spec["injZi"] = zi
+                    spec["perfXi"] = np.zeros(len(cells))
+                    self.solvent.append(dict(well=len(specs), zi=zi, rate=opt["rate"]))
             else:
                 m = opt["mode"].upper()
                 spec["prodPhaseWeight"] = {"ORAT": [1.0, 0.0, 0.0], "GRAT": [0.0, 1.0, 0.0],

@@ -17,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SPE5 = os.path.join(HERE, "golden", "decks", "spe5", "spe5.data")
 REFINE = os.path.join(HERE, "golden", "decks", "spe5refine", "spe5-70x70x30-2y.data")
 CORNER = os.path.join(HERE, "golden", "decks", "corner", "corner-faulted.data")
+SPE3 = os.path.join(HERE, "golden", "decks", "spe3", "spe3.data")
 
 
 def test_tokens_and_defaults():
@@ -180,3 +181,33 @@ def test_corner_point_deck():
     wi = deck.CONV2 * 2 * deck.PI * dz * math.sqrt(kk) / math.log(ro / (0.5 / 2))
     assert w["name"] == "PROD1"
     assert m.wells.perfWI[0] == pytest.approx(wi, rel=1e-12)
+
+
+def test_spe3_deck_fluid_and_solvent_injector():
+    """The reference's SPE3 deck: its 9-component fluid equals the SPE3
+    constants the 9-component flash fixtures were made with (per-component
+    OMEGAA / OMEGAB included), and its solvent injector takes the WELLSTRE
+    composition, with the surface-rate factor 1000 xi(14.7, 60 F) and the
+    perforations' density from prepare_injectors."""
+    m = deck.load(SPE3)
+    ref = fluids.spe3_deck()
+    e = m.eos_deck
+    assert e.nc == 9 and e.has_OmegaA and e.has_OmegaB
+    for f in ("Tc", "Pc", "MW", "Acf", "Zc", "OmegaA", "OmegaB"):
+        assert list(getattr(e, f)[:9]) == list(getattr(ref, f)[:9]), f
+    assert list(e.BIC_lower[:36]) == list(ref.BIC_lower[:36])
+    assert m.deck.rtemp == fluids.SPE3["T_F"]
+    w = m.wells
+    assert w.nw == 2 and w.type[0] == abi.WELL_INJ and w.type[1] == abi.WELL_PROD
+    assert list(w.prodPhaseWeight[3:6]) == [0.0, 1.0, 0.0]  # GRAT
+    np.testing.assert_allclose(w.injZi[:10], m.deck.streams["Solvent"][:10])
+    seen = []
+
+    def xi(P, T, z, phase):
+        seen.append((P, T, phase))
+        return 0.25 if P == 14.7 else 0.5
+
+    m.prepare_injectors(xi)
+    assert w.ctrlRate[0] == 4700.0 * 1000.0 * 0.25
+    assert np.all(w.perfXi[:w.perfStart[1]] == 0.5)
+    assert seen[0] == (14.7, 60.0, 1) and seen[1] == (float(w.bhp[0]), 200.0, 1)

@@ -29,6 +29,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SPE5 = os.path.join(HERE, "golden", "decks", "spe5", "spe5.data")
 REFINE = os.path.join(HERE, "golden", "decks", "spe5refine", "spe5-70x70x30-2y.data")
 CORNER = os.path.join(HERE, "golden", "decks", "corner", "corner-faulted.data")
+SPE3 = os.path.join(HERE, "golden", "decks", "spe3", "spe3.data")
 DT = 5.0
 
 
@@ -45,13 +46,23 @@ def _context(m, eos):
     return ctx
 
 
-@pytest.mark.parametrize("path", [SPE5, CORNER], ids=["spe5", "corner"])
-def test_deck_initial_state_and_system(path):
-    """The SPE5 deck (7x7x3, a producer) and the faulted corner-point deck
-    (12x10x5, 179 non-neighbour connections, an injector and a producer)."""
+@pytest.mark.parametrize("path,init", [(SPE5, False), (CORNER, False), (SPE3, True)], ids=["spe5", "corner", "spe3"])
+def test_deck_initial_state_and_system(path, init):
+    """The SPE5 deck (7x7x3, a producer), the faulted corner-point deck
+    (12x10x5, 179 non-neighbour connections, an injector and a producer) and
+    the reference's SPE3 deck (9x9x4, the 9-component gas condensate with
+    per-component OMEGAA / OMEGAB, a solvent injector and a gas-rate
+    producer), the last one equilibrated (Model.initialize) first."""
     gpu_util.say(f"[deck] {os.path.basename(path)}: init, then CalRes + assembly with its wells")
     m = deck.load(path)
     eos = lib.eos_from_deck(m.eos_deck)
+    if init:
+        c0 = lib.Context(eos)
+        m.initialize(c0)
+        c0.close()
+        for sv in m.solvent:
+            w = sv["well"]
+            assert m.wells.perfXi[m.wells.perfStart[w]:m.wells.perfStart[w + 1]].min() > 0
     orc = _oracle()
     o = m.bulk.copy()
     orc.calrock(m.rock, o)
