@@ -846,12 +846,20 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
     double njO[3] = {1.0, 1.0, 1.0};
     bool twoPh = false;
     double rhsAcc = 0.0;
+    // the right-hand-side lanes' n_ij corrections of the own cell
+    // (res_n rows x(j, i), i = lane - 1), read once for all connections
+    double resX[3] = {0.0, 0.0, 0.0};
     if constexpr (FN) {
         const int64_t ld = B.ld;
 #pragma unroll
         for (int j = 0; j < 3; ++j) njO[j] = uniformD(B.nj[(int64_t)j * ld + n]);
         twoPh = __builtin_amdgcn_readfirstlane(B.phaseNum[n]) > 2;
         if (lane == 0) rhsAcc = 0.0 + -B.resPc[n];
+        if (twoPh && lane >= 1 && lane < ncol) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                if (lane - 1 < pvO[j]) resX[j] = B.resN[(int64_t)(xb[j] + lane - 1) * ld + n];
+        }
     }
 
     // diagonal starts as the accumulation block (src/OCPFluidMethod.cpp:1586-1597)
@@ -1034,15 +1042,13 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
         if constexpr (FN) {
             if (twoPh && lane >= 1 && lane < ncol) {
                 const int i = lane - 1;
-                const int64_t ld = B.ld;
                 double v = 0.0;
 #pragma unroll
                 for (int j = 0; j < 3; ++j) {
                     if (!((liveM >> j) & 1u)) continue;
                     const double* rU = ((upOwnM >> j) & 1u) ? sRec[0] : sRec[1];
                     v = fma(rU[R::XIJ + j * nc + i], sh.sG[j][ncol], v);
-                    if (((upOwnM >> j) & 1u) && i < pvO[j])
-                        v = fma(sh.sD[j], B.resN[(int64_t)(xb[j] + i) * ld + n], v);
+                    if (((upOwnM >> j) & 1u) && i < pvO[j]) v = fma(sh.sD[j], resX[j], v);
                 }
                 rhsAcc += -v;
             }
