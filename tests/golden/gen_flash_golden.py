@@ -2,7 +2,7 @@
 
 Runs only where oracle/_ref/libocpref.so was compiled from the reference
 sources (``make -C oracle ref``); the committed .npz files are what the tests
-and the GPU box use.  Inputs are seeded; each file records the seeds, the
+use, on machines without the reference.  Inputs are seeded; each file records the seeds, the
 deck, the EoS description the reference constructor derived, and for each
 pass (cold: lastNP = 0 and empty skip memory; warm: the cold result with
 P + U[-5, 5] psia) every array PassFlashValue hands to the Bulk.
