@@ -969,6 +969,11 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
                     const double dP = rB[R::PJ + j] - rE[R::PJ + j] - (j == 0 ? upRho[0] : j == 1 ? upRho[1] : upRho[2]) * dGamma;
                     const double transJ = Akd * rU[R::LAM + j];
                     const double A = rU[R::XI + j] * transJ;
+                    // IsoT_FIMn: (xi transJ dP) / n_j of the upwind own cell,
+                    // shared by the pressure lane's d_j and the x columns
+                    double qN = 0.0;
+                    if constexpr (FN)
+                        if (upO) qN = A * dP / (j == 0 ? njO[0] : j == 1 ? njO[1] : njO[2]);
                     if (col == CS - 1) {
                         // pressure: sgn A, the density term when the other
                         // cell lacks the phase, half of it when both have it,
@@ -986,8 +991,7 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
                         }
                         cv = v;
                         if constexpr (FN)
-                            dv = upO ? sRec[0][R::XI + j] * transJ * dP / (j == 0 ? njO[0] : j == 1 ? njO[1] : njO[2])
-                                     : 0.0;
+                            dv = qN;
                         else
                             dv = upO ? sRec[0][R::XI + j] * transJ * dP : 0.0;
                     } else if (col < nsat) {
@@ -1012,7 +1016,7 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
                                 const double rx = othEx ? rhox / 2 : rhox;
                                 cv = -A * rx * dGamma + transJ * sOwn[O::XIX + j * nc + kk] * dP -
                                      A * sOwn[O::MUXM + j * nc + kk] * dP;
-                                if constexpr (FN) cv -= A * dP / (j == 0 ? njO[0] : j == 1 ? njO[1] : njO[2]);
+                                if constexpr (FN) cv -= qN;
                             } else {
                                 cv = -A * (rhox / 2) * dGamma;
                             }
