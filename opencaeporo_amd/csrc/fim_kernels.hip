@@ -1042,40 +1042,41 @@ __device__ __forceinline__ void assembleRow(const DevBulk& B, const FimDevice& F
             sh.sG[j][gcc] = g;
         }
         __syncthreads();
-        // ---- IsoT_FIMn: this connection's -dFdXs res_n on rows 1..nc ----
-        if constexpr (FN) {
-            if (twoPh && lane >= 1 && lane < ncol) {
-                const int i = lane - 1;
-                double v = 0.0;
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    if (!((liveM >> j) & 1u)) continue;
-                    const double* rU = ((upOwnM >> j) & 1u) ? sRec[0] : sRec[1];
-                    v = fma(rU[R::XIJ + j * nc + i], sh.sG[j][ncol], v);
-                    if (((upOwnM >> j) & 1u) && i < pvO[j]) v = fma(sh.sD[j], resX[j], v);
-                }
-                rhsAcc += -v;
-            }
-        }
-
         // ---- block = dt (sum_j x_ij G[j] + sum_j d_j dSec[x(j, i)]), then scatter ----
+        // (IsoT_FIMn: the block's row 0 is zero, and its lanes 1..nc form
+        // this connection's -dFdXs res_n term of right-hand-side row
+        // i = lane - 1 with the same sums over the res_n column of G and the
+        // own cell's res_n)
         double* dst = F.val + mirror * bsize;
 #pragma unroll
         for (int t = 0; t < D::perLane; ++t) {
             const int e = lane + t * kAsmBlock;
             if (e < bsize) {
                 const int r = e / ncol, cc = e % ncol;
+                bool rhsLane = false;
+                if constexpr (FN) rhsLane = t == 0 && r == 0 && twoPh && cc >= 1;
                 double v = 0;
-                if (r > 0) {
-                    const int i = r - 1;
+                if (r > 0 || rhsLane) {
+                    const int i = r > 0 ? r - 1 : cc - 1;
+                    const int gc = r > 0 ? cc : ncol;
 #pragma unroll
                     for (int j = 0; j < 3; ++j) {
                         if (!((liveM >> j) & 1u)) continue;
                         const double* rU = ((upOwnM >> j) & 1u) ? sRec[0] : sRec[1];
-                        v = fma(rU[R::XIJ + j * nc + i], sh.sG[j][cc], v);
-                        if (((upOwnM >> j) & 1u) && i < pvO[j])
-                            v = fma(sh.sD[j], sOwn[O::DSEC + (xb[j] + i) * ncol + cc], v);
+                        v = fma(rU[R::XIJ + j * nc + i], sh.sG[j][gc], v);
+                        if (((upOwnM >> j) & 1u) && i < pvO[j]) {
+                            double sec;
+                            if constexpr (FN)
+                                sec = rhsLane ? resX[j] : sOwn[O::DSEC + (xb[j] + i) * ncol + cc];
+                            else
+                                sec = sOwn[O::DSEC + (xb[j] + i) * ncol + cc];
+                            v = fma(sh.sD[j], sec, v);
+                        }
                     }
+                }
+                if (rhsLane) {
+                    rhsAcc += -v;
+                    v = 0;
                 }
                 v = dt * v;
                 if (ownB) {
