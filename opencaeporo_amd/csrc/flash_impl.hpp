@@ -594,6 +594,11 @@ constexpr int kSideMaxTail = 2048;
 constexpr int kSideMaxRestart = 1 << 30;  // restarts are the slowest cells: always beside
 constexpr unsigned kSideBlocks = 32;
 constexpr unsigned kCoopBlocks = 256;
+// The restart list can hold thousands of cells, each milliseconds long (a
+// faulted 10-component field): its kernel takes one wave on every SIMD
+// (256 CUs x 4), so that once the main stream is done the whole chip drains
+// it; a short list leaves the extra waves nothing to take.
+constexpr unsigned kRestartBlocks = 256 * 4 / (kFlashBlock / 64);
 
 // Append the active lanes' cells to list k: one atomic per wavefront.
 __device__ __forceinline__ void appendList(const FlashWork& W, int k, bool pred, int32_t cell)
@@ -1384,7 +1389,7 @@ hipError_t launchFlashT(const DevEos* eos, const DevBulk& B, const FlashWork& W,
     // side stream D: a short restart list, one wavefront per cell
     if ((e = hipStreamWaitEvent(side->sD, side->newton, 0)) != hipSuccess) return e;
     mark(8, side->sD);
-    hipLaunchKernelGGL(flashSplitTailCoopKernel<NC>, dim3(kCoopBlocks), dim3(kFlashBlock), 0, side->sD, eos, B,
+    hipLaunchKernelGGL(flashSplitTailCoopKernel<NC>, dim3(kRestartBlocks), dim3(kFlashBlock), 0, side->sD, eos, B,
                        W, ctr, (int)LIST_TNR, (int)LIST_NP1D, (int)LIST_NP2D, kSideMaxRestart);
     mark(9, side->sD);
     if ((e = hipEventRecord(side->doneD, side->sD)) != hipSuccess) return e;
